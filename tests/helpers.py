@@ -1,0 +1,52 @@
+"""Test-only input generators (numpy), independent of the engine's own generators."""
+import numpy as np
+
+
+def plummer_np(n, seed=1, a=1.0, rcut=0.999):
+    """Plummer positions (Aarseth, Henon & Wielen 1974 inverse CDF) and equal masses 1/n."""
+    rng = np.random.default_rng(seed)
+    X = rng.uniform(1e-6, rcut, n)
+    r = a / np.sqrt(X ** (-2.0 / 3.0) - 1.0)
+    ct = rng.uniform(-1.0, 1.0, n)
+    ph = rng.uniform(0.0, 2 * np.pi, n)
+    st = np.sqrt(1.0 - ct * ct)
+    pos = np.stack([r * st * np.cos(ph), r * st * np.sin(ph), r * ct], axis=1)
+    vel = rng.normal(0.0, 0.3, (n, 3))
+    mass = np.full(n, 1.0 / n)
+    return pos, vel, mass
+
+
+def interleave_bruteforce(x, y, z):
+    """Bit-by-bit Morton interleave of three 21-bit values, x most significant per triple."""
+    k = 1 << 63
+    for i in range(21):
+        k |= ((x >> i) & 1) << (3 * i + 2)
+        k |= ((y >> i) & 1) << (3 * i + 1)
+        k |= ((z >> i) & 1) << (3 * i)
+    return k
+
+
+def effective21(t):
+    """The 21-bit value the first magic-bits step keeps from any 64-bit t: bits 0-15 and
+    (bits 16-20 OR bits 48-52)."""
+    return (t & 0xFFFF) | ((((t >> 16) | (t >> 48)) & 0x1F) << 16)
+
+
+def canonical_node_keys(path_keys, leaf_cap):
+    """Node set from 21-level path keys by definition: the root, plus every level-L prefix
+    whose parent prefix holds more than leaf_cap bodies."""
+    pk = np.asarray(path_keys, dtype=np.uint64)
+    n = pk.size
+    if n == 0:
+        return set()
+    keys = {1}
+    for L in range(1, 22):
+        par = pk >> np.uint64(63 - 3 * (L - 1))
+        cur = pk >> np.uint64(63 - 3 * L)
+        pu, pc = np.unique(par, return_counts=True)
+        big = set(pu[pc > leaf_cap].tolist())
+        if not big:
+            break
+        sel = np.isin(par, np.array(sorted(big), dtype=np.uint64))
+        keys.update(np.unique(cur[sel]).tolist())
+    return keys
