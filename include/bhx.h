@@ -1,0 +1,194 @@
+/*
+ * bhx.h — C ABI of the MI355X Barnes-Hut engine (libbhx.so).
+ *
+ * Plain pointers and sizes only; every entry point returns a bhx_status.
+ * Host arrays of 3-vectors are interleaved xyz doubles (the memory layout
+ * of the reference's Vec3D[]), masses are doubles, keys are uint64.
+ *
+ * Each entry point replaces one function of the reference's per-step path
+ * (DavidRichardson02/Serial-Hashed-Octree-Barnes-Hut-N-Body-Simulator-0_3,
+ * src/); the replaced interface is cited at each declaration.  State lives
+ * in a context that owns all device buffers; bodies stay resident in HBM
+ * between calls, in structure-of-arrays FP64 form.
+ */
+#ifndef BHX_H
+#define BHX_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum bhx_status {
+    BHX_OK = 0,
+    BHX_E_NULL = 1,        /* a required pointer was NULL                         */
+    BHX_E_ARG = 2,         /* an argument is out of range                         */
+    BHX_E_DEPTH = 3,       /* cells at the 21-level cap hold more bodies than fit  */
+    BHX_E_HASH_FULL = 4,   /* node hash table could not take every node            */
+    BHX_E_NONFINITE = 5,   /* non-finite position or bounds                        */
+    BHX_E_HIP = 6,         /* a HIP runtime call failed                            */
+    BHX_E_NOMEM = 7,       /* device allocation failed                             */
+    BHX_E_STATE = 8,       /* call out of order (e.g. walk before build)           */
+    BHX_E_CAPACITY = 9,    /* more bodies than the context was created for         */
+    BHX_E_SORT = 10        /* radix-sort look-back did not complete                */
+} bhx_status;
+
+/* Node payload semantics. */
+typedef enum bhx_mode {
+    BHX_MODE_AS_RUN = 0,   /* as the reference executes: barycentre = first body in
+                              the cell (array order), quadrupole = 0               */
+    BHX_MODE_INTENDED = 1  /* true mass-weighted centre of mass and quadrupole     */
+} bhx_mode;
+
+typedef enum bhx_precision {
+    BHX_FP64 = 0,          /* FP64 walk with the reference's operation sequence     */
+    BHX_FP32 = 1           /* FP32 walk in group-relative coordinates               */
+} bhx_precision;
+
+typedef struct bhx_config {
+    double theta;          /* opening angle: node opens unless size^2/d^2 < theta^2 */
+    double eps;            /* Plummer softening                                     */
+    double dt;             /* leapfrog step                                         */
+    int32_t leaf_capacity; /* bodies per leaf before a cell splits (1..64); 1 = reference */
+    int32_t mode;          /* bhx_mode                                              */
+    int32_t precision;     /* bhx_precision                                         */
+    int32_t quadrupole;    /* 1: add the quadrupole term for cells of > 1 body      */
+    int32_t permute_acc;   /* 1: reorder accelerations with the bodies; 0: as run   */
+    int32_t compensated;   /* 1: compensated (Kahan) FP64 kick/drift accumulation   */
+    int32_t device;        /* HIP device ordinal                                    */
+    int32_t reserved;
+} bhx_config;
+
+typedef struct bhx_ctx bhx_ctx;
+
+/* Downloaded node record (HOTNode, HashedNode.hpp:60-95). */
+typedef struct bhx_node {
+    uint64_t key;          /* nodeKey: sentinel 1 then 3 bits per level             */
+    double   center[3];    /* nodeBounds.center                                     */
+    double   size;         /* nodeBounds.size                                       */
+    double   com[3];       /* baryCenter                                            */
+    double   mass;         /* mass                                                  */
+    double   quad[6];      /* quadrupoleMoment                                      */
+    int64_t  n;            /* N                                                     */
+    int64_t  first;        /* smallest array index among the node's bodies          */
+    int32_t  level;        /* depth (root 0)                                        */
+    uint32_t child_byte;   /* childByte                                             */
+    int64_t  skip;         /* pre-order index of the next node outside the subtree  */
+    int64_t  body_start;   /* first body of the node in tree order                  */
+} bhx_node;
+
+typedef struct bhx_timing {   /* milliseconds of the last bhx_step, per phase */
+    double bounds, keys, sort, kick_drift, treekeys, build, moments, walk, kick, total;
+} bhx_timing;
+
+/* ---- context ---------------------------------------------------------- */
+
+/* Defaults: theta 0.5, eps 0.025 (SOFTENING, HashedOctree.hpp:151), dt 1e-3
+ * (ofApp.h:62), leaf capacity 1, AS_RUN, FP64, no quadrupole, permute_acc 1. */
+int bhx_config_default(bhx_config *cfg);
+
+/* LinearHashedOctree() (HashedOctree.cpp:24-27) plus the caller-owned Body[] /
+ * Vec3D[] buffers of ofApp::setup (ofApp.cpp:83-84,136): one context holds up
+ * to `capacity` bodies on the configured device. */
+int bhx_create(const bhx_config *cfg, size_t capacity, bhx_ctx **out);
+int bhx_destroy(bhx_ctx *ctx);
+int bhx_set_config(bhx_ctx *ctx, const bhx_config *cfg);
+int bhx_get_config(const bhx_ctx *ctx, bhx_config *cfg);
+const char *bhx_last_error(const bhx_ctx *ctx);
+int bhx_sync(bhx_ctx *ctx);
+
+/* Body[] contents (Body.hpp:30-65): positions, velocities (xyz interleaved),
+ * masses; accelerations start at zero as `new Vec3D[N]` does (ofApp.cpp:136).
+ * vel and acc may be NULL (zero). */
+int bhx_set_bodies(bhx_ctx *ctx, const double *pos, const double *vel, const double *mass, size_t n);
+int bhx_set_accelerations(bhx_ctx *ctx, const double *acc);
+int bhx_get_bodies(bhx_ctx *ctx, double *pos, double *vel, double *mass, double *acc);
+int bhx_num_bodies(const bhx_ctx *ctx, size_t *n);
+
+/* ---- the per-step functions ---------------------------------------------- */
+
+/* OctantBounds(Body*, size_t) — OctantUtils.cpp:27-59: centre = sequential
+ * FP64 mean in array order, size = max(0, max |p| - centre). */
+int bhx_bounds(bhx_ctx *ctx, double center[3], double *size);
+
+/* computeMortonKey over all bodies — MortonKeys.hpp:137-181, ofApp.cpp:172-175.
+ * keys_out may be NULL (keys stay on the device). */
+int bhx_morton_keys(bhx_ctx *ctx, double size, uint64_t *keys_out);
+
+/* ThreePassRadixSortMortonKeys(keys, indexes, n) — MortonKeys.hpp:439-470:
+ * stable ascending sort of host arrays; idx is carried along (u64, as size_t). */
+int bhx_sort_keys(bhx_ctx *ctx, uint64_t *keys, uint64_t *idx, size_t n);
+
+/* RadixSortBodies(Body*, n) — Body.hpp:275-302 (+ ReorderBodiesByIndexes
+ * :257-273): sorts the resident bodies by the keys of the last
+ * bhx_morton_keys call.  perm_out (may be NULL) receives new[i] = old[perm[i]]. */
+int bhx_sort_bodies(bhx_ctx *ctx, uint64_t *perm_out);
+
+/* ReorderBodiesByIndexes(Body*, size_t*, n) — Body.hpp:257-273 with a caller
+ * permutation. */
+int bhx_reorder_bodies(bhx_ctx *ctx, const uint64_t *perm);
+
+/* ComputePositionAtHalfTimeStep — Body.hpp:306-313: v += (a dt) 0.5; x += v dt. */
+int bhx_kick_drift(bhx_ctx *ctx, double dt);
+
+/* ComputeVelocityAndPosition — Body.hpp:317-324: v += (a dt) 0.5. */
+int bhx_kick(bhx_ctx *ctx, double dt);
+
+/* buildLinearHashedOctreeInPlace(tree, bodies, n, bounds) — HashedOctree.hpp:
+ * 211-244 and HashedOctree.cpp:450-711: node set, payload and hash table for
+ * the resident bodies in the given root bounds.  n_nodes may be NULL. */
+int bhx_build(bhx_ctx *ctx, const double center[3], double size, size_t *n_nodes);
+
+/* Node records of the last build, in pre-order (depth first, children in
+ * octant order).  `cap` is the capacity of out in records. */
+int bhx_get_nodes(bhx_ctx *ctx, bhx_node *out, size_t cap, size_t *n_nodes);
+
+/* LinearHashedOctree::lookUpNode(key) — HashedOctree.cpp:111-115, batched
+ * through the device hash table: ids[i] = pre-order index or -1. */
+int bhx_lookup(bhx_ctx *ctx, const uint64_t *keys, size_t n, int64_t *ids);
+
+/* Bodies in the order the tree holds them: tree_perm[i] = array index of the
+ * i-th body in tree order. */
+int bhx_get_tree_order(bhx_ctx *ctx, uint64_t *tree_perm, uint64_t *tree_keys);
+
+/* Number of leaves at the depth cap that hold more than leaf_capacity bodies. */
+int bhx_unsplit_leaves(bhx_ctx *ctx, size_t *count);
+
+/* ComputeHOTOctreeForce — HashedOctree.hpp:262-272, TraverseHOTInteractionList
+ * :336-385, MAC (HashedOctree.cpp:872-885), ComputeHOTForceInteractionList
+ * HashedOctree.hpp:393-459: accelerations of all resident bodies from the last
+ * build (array order).  acc_out and counts (interaction-list length per body,
+ * the value TraverseHOTInteractionList returns) may be NULL. */
+int bhx_walk(bhx_ctx *ctx, double *acc_out, uint64_t *counts);
+
+/* One frame in the reference order (ofApp::update then ofApp::draw,
+ * ofApp.cpp:169-177,248-282), repeated nsteps times on the resident bodies. */
+int bhx_step(bhx_ctx *ctx, int nsteps);
+int bhx_last_timing(bhx_ctx *ctx, bhx_timing *t);
+/* 1: time each phase with events in bhx_step (adds synchronisation). */
+int bhx_set_profiling(bhx_ctx *ctx, int on);
+
+/* ---- diagnostics (no reference counterpart) ------------------------------ */
+
+/* FP64 direct sum over all bodies for the targets (array indices). */
+int bhx_direct(bhx_ctx *ctx, const int64_t *targets, size_t nt, double eps,
+               double *acc_out, double *pot_out);
+
+/* FP64 kinetic energy, linear and angular momentum of the resident bodies. */
+int bhx_diagnostics(bhx_ctx *ctx, double *ekin, double mom[3], double angmom[3]);
+
+/* ---- initial conditions (no reference counterpart) ----------------------- */
+
+/* Plummer sphere (Aarseth, Henon & Wielen 1974) with counter-based Philox
+ * random numbers: n equal masses summing to total_mass, scale radius a,
+ * isotropic velocities in virial equilibrium (G = 1), mass fraction cut
+ * 0.999, shifted to the centre-of-mass frame. */
+int bhx_ic_plummer(size_t n, uint64_t seed, double total_mass, double a,
+                   double *pos, double *vel, double *mass);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

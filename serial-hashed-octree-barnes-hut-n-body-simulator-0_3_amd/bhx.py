@@ -1,0 +1,277 @@
+"""Python binding of libbhx.so (the MI355X Barnes-Hut engine) over its C ABI.
+
+A thin ctypes layer: every method calls one entry point of include/bhx.h and
+raises BhxError on a non-zero status.  There is no CPU fallback: if the
+library has not been built, loading raises.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libbhx.so")
+
+AS_RUN, INTENDED = 0, 1
+FP64, FP32 = 0, 1
+
+STATUS = {
+    0: "BHX_OK", 1: "BHX_E_NULL", 2: "BHX_E_ARG", 3: "BHX_E_DEPTH", 4: "BHX_E_HASH_FULL",
+    5: "BHX_E_NONFINITE", 6: "BHX_E_HIP", 7: "BHX_E_NOMEM", 8: "BHX_E_STATE", 9: "BHX_E_CAPACITY",
+    10: "BHX_E_SORT",
+}
+
+NODE_DTYPE = np.dtype([
+    ("key", "<u8"), ("center", "<f8", (3,)), ("size", "<f8"), ("com", "<f8", (3,)), ("mass", "<f8"),
+    ("quad", "<f8", (6,)), ("n", "<i8"), ("first", "<i8"), ("level", "<i4"), ("child_byte", "<u4"),
+    ("skip", "<i8"), ("body_start", "<i8"),
+])
+assert NODE_DTYPE.itemsize == 160
+
+
+class BhxError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__("%s: %s" % (STATUS.get(code, code), msg))
+        self.code = code
+
+
+class Config(ctypes.Structure):
+    _fields_ = [("theta", ctypes.c_double), ("eps", ctypes.c_double), ("dt", ctypes.c_double),
+                ("leaf_capacity", ctypes.c_int32), ("mode", ctypes.c_int32), ("precision", ctypes.c_int32),
+                ("quadrupole", ctypes.c_int32), ("permute_acc", ctypes.c_int32), ("compensated", ctypes.c_int32),
+                ("device", ctypes.c_int32), ("reserved", ctypes.c_int32)]
+
+
+class Timing(ctypes.Structure):
+    _fields_ = [(k, ctypes.c_double) for k in
+                ("bounds", "keys", "sort", "kick_drift", "treekeys", "build", "moments", "walk", "kick", "total")]
+
+
+_lib = None
+
+
+def lib():
+    """Load libbhx.so; raises when it has not been built (no fallback)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise OSError("libbhx.so not built: run build.py (or __graft_entry__.build())")
+        L = ctypes.CDLL(LIB_PATH)
+        P, S, D, I = ctypes.c_void_p, ctypes.c_size_t, ctypes.c_double, ctypes.c_int
+        sigs = {
+            "bhx_config_default": [P], "bhx_create": [P, S, P], "bhx_destroy": [P], "bhx_set_config": [P, P],
+            "bhx_get_config": [P, P], "bhx_sync": [P], "bhx_set_bodies": [P, P, P, P, S],
+            "bhx_set_accelerations": [P, P], "bhx_get_bodies": [P, P, P, P, P], "bhx_num_bodies": [P, P],
+            "bhx_bounds": [P, P, P], "bhx_morton_keys": [P, D, P], "bhx_sort_keys": [P, P, P, S],
+            "bhx_sort_bodies": [P, P], "bhx_reorder_bodies": [P, P], "bhx_kick_drift": [P, D], "bhx_kick": [P, D],
+            "bhx_build": [P, P, D, P], "bhx_get_nodes": [P, P, S, P], "bhx_lookup": [P, P, S, P],
+            "bhx_get_tree_order": [P, P, P], "bhx_unsplit_leaves": [P, P], "bhx_walk": [P, P, P],
+            "bhx_step": [P, I], "bhx_last_timing": [P, P], "bhx_set_profiling": [P, I],
+            "bhx_direct": [P, P, S, D, P, P], "bhx_diagnostics": [P, P, P, P],
+            "bhx_ic_plummer": [S, ctypes.c_uint64, D, D, P, P, P],
+        }
+        for name, args in sigs.items():
+            f = getattr(L, name)
+            f.argtypes = args
+            f.restype = ctypes.c_int
+        L.bhx_last_error.argtypes = [P]
+        L.bhx_last_error.restype = ctypes.c_char_p
+        _lib = L
+    return _lib
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data_as(ctypes.c_void_p)
+
+
+def _f64(a, shape=None):
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    if shape is not None:
+        a = a.reshape(shape)
+    return a
+
+
+def plummer(n, seed=1, total_mass=1.0, a=1.0):
+    """Seeded Plummer sphere (host generator of the library): pos (n,3), vel (n,3), mass (n,)."""
+    pos = np.empty((n, 3))
+    vel = np.empty((n, 3))
+    mass = np.empty(n)
+    rc = lib().bhx_ic_plummer(n, seed, total_mass, a, _p(pos), _p(vel), _p(mass))
+    if rc:
+        raise BhxError(rc, "bhx_ic_plummer")
+    return pos, vel, mass
+
+
+def default_config(**kw):
+    cfg = Config()
+    lib().bhx_config_default(ctypes.byref(cfg))
+    for k, v in kw.items():
+        if not hasattr(cfg, k):
+            raise KeyError(k)
+        setattr(cfg, k, v)
+    return cfg
+
+
+class Engine:
+    """One bhx context: bodies resident on the GPU, the per-step functions as methods."""
+
+    def __init__(self, capacity, **cfg):
+        self.cfg = default_config(**cfg)
+        h = ctypes.c_void_p()
+        rc = lib().bhx_create(ctypes.byref(self.cfg), capacity, ctypes.byref(h))
+        if rc:
+            raise BhxError(rc, "bhx_create")
+        self.h = h
+        self.capacity = capacity
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().bhx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def _chk(self, rc, what):
+        if rc:
+            msg = lib().bhx_last_error(self.h) if self.h else b""
+            raise BhxError(rc, "%s (%s)" % (what, (msg or b"").decode()))
+
+    # --- configuration
+    def set_config(self, **kw):
+        for k, v in kw.items():
+            if not hasattr(self.cfg, k):
+                raise KeyError(k)
+            setattr(self.cfg, k, v)
+        self._chk(lib().bhx_set_config(self.h, ctypes.byref(self.cfg)), "bhx_set_config")
+
+    # --- bodies
+    @property
+    def n(self):
+        v = ctypes.c_size_t()
+        self._chk(lib().bhx_num_bodies(self.h, ctypes.byref(v)), "bhx_num_bodies")
+        return v.value
+
+    def set_bodies(self, pos, vel, mass):
+        pos = _f64(pos, (-1, 3))
+        mass = _f64(mass)
+        vel = None if vel is None else _f64(vel, (-1, 3))
+        self._chk(lib().bhx_set_bodies(self.h, _p(pos), _p(vel), _p(mass), pos.shape[0]), "bhx_set_bodies")
+
+    def set_accelerations(self, acc):
+        acc = _f64(acc, (-1, 3))
+        self._chk(lib().bhx_set_accelerations(self.h, _p(acc)), "bhx_set_accelerations")
+
+    def get_bodies(self):
+        n = self.n
+        pos, vel, acc, mass = np.empty((n, 3)), np.empty((n, 3)), np.empty((n, 3)), np.empty(n)
+        self._chk(lib().bhx_get_bodies(self.h, _p(pos), _p(vel), _p(mass), _p(acc)), "bhx_get_bodies")
+        return pos, vel, mass, acc
+
+    # --- the step functions
+    def bounds(self):
+        c = np.zeros(3)
+        s = np.zeros(1)
+        self._chk(lib().bhx_bounds(self.h, _p(c), _p(s)), "bhx_bounds")
+        return c, float(s[0])
+
+    def morton_keys(self, size):
+        keys = np.empty(self.n, np.uint64)
+        self._chk(lib().bhx_morton_keys(self.h, size, _p(keys)), "bhx_morton_keys")
+        return keys
+
+    def sort_keys(self, keys, idx=None):
+        keys = np.array(keys, dtype=np.uint64, copy=True)
+        idx = np.arange(keys.size, dtype=np.uint64) if idx is None else np.array(idx, np.uint64, copy=True)
+        self._chk(lib().bhx_sort_keys(self.h, _p(keys), _p(idx), keys.size), "bhx_sort_keys")
+        return keys, idx
+
+    def sort_bodies(self):
+        perm = np.empty(self.n, np.uint64)
+        self._chk(lib().bhx_sort_bodies(self.h, _p(perm)), "bhx_sort_bodies")
+        return perm
+
+    def reorder_bodies(self, perm):
+        perm = np.ascontiguousarray(perm, dtype=np.uint64)
+        self._chk(lib().bhx_reorder_bodies(self.h, _p(perm)), "bhx_reorder_bodies")
+
+    def kick_drift(self, dt):
+        self._chk(lib().bhx_kick_drift(self.h, dt), "bhx_kick_drift")
+
+    def kick(self, dt):
+        self._chk(lib().bhx_kick(self.h, dt), "bhx_kick")
+
+    def build(self, center, size):
+        c = _f64(center)
+        nn = ctypes.c_size_t()
+        self._chk(lib().bhx_build(self.h, _p(c), size, ctypes.byref(nn)), "bhx_build")
+        return nn.value
+
+    def nodes(self):
+        nn = ctypes.c_size_t()
+        self._chk(lib().bhx_get_nodes(self.h, None, 0, ctypes.byref(nn)), "bhx_get_nodes")
+        out = np.zeros(nn.value, NODE_DTYPE)
+        self._chk(lib().bhx_get_nodes(self.h, _p(out), nn.value, ctypes.byref(nn)), "bhx_get_nodes")
+        return out
+
+    def lookup(self, keys):
+        keys = np.ascontiguousarray(keys, dtype=np.uint64)
+        ids = np.empty(keys.size, np.int64)
+        self._chk(lib().bhx_lookup(self.h, _p(keys), keys.size, _p(ids)), "bhx_lookup")
+        return ids
+
+    def tree_order(self):
+        perm = np.empty(self.n, np.uint64)
+        keys = np.empty(self.n, np.uint64)
+        self._chk(lib().bhx_get_tree_order(self.h, _p(perm), _p(keys)), "bhx_get_tree_order")
+        return perm, keys
+
+    def unsplit_leaves(self):
+        v = ctypes.c_size_t()
+        self._chk(lib().bhx_unsplit_leaves(self.h, ctypes.byref(v)), "bhx_unsplit_leaves")
+        return v.value
+
+    def walk(self, counts=False):
+        n = self.n
+        acc = np.empty((n, 3))
+        cnt = np.empty(n, np.uint64) if counts else None
+        self._chk(lib().bhx_walk(self.h, _p(acc), _p(cnt)), "bhx_walk")
+        return (acc, cnt) if counts else acc
+
+    def step(self, nsteps=1):
+        self._chk(lib().bhx_step(self.h, nsteps), "bhx_step")
+
+    def sync(self):
+        self._chk(lib().bhx_sync(self.h), "bhx_sync")
+
+    def set_profiling(self, on):
+        self._chk(lib().bhx_set_profiling(self.h, int(on)), "bhx_set_profiling")
+
+    def last_timing(self):
+        t = Timing()
+        self._chk(lib().bhx_last_timing(self.h, ctypes.byref(t)), "bhx_last_timing")
+        return {k: getattr(t, k) for k, _ in Timing._fields_}
+
+    # --- diagnostics
+    def direct(self, targets, eps):
+        targets = np.ascontiguousarray(targets, dtype=np.int64)
+        acc = np.empty((targets.size, 3))
+        pot = np.empty(targets.size)
+        self._chk(lib().bhx_direct(self.h, _p(targets), targets.size, eps, _p(acc), _p(pot)), "bhx_direct")
+        return acc, pot
+
+    def diagnostics(self):
+        ek = np.zeros(1)
+        mom = np.zeros(3)
+        am = np.zeros(3)
+        self._chk(lib().bhx_diagnostics(self.h, _p(ek), _p(mom), _p(am)), "bhx_diagnostics")
+        return float(ek[0]), mom, am

@@ -1,0 +1,775 @@
+// bhx_api.hip — context management and the C ABI (include/bhx.h).
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "bhx_internal.hpp"
+
+using namespace bhx;
+
+namespace {
+
+constexpr int kBlock = 256;
+
+inline unsigned grid_for(size_t n) {
+    size_t g = (n + kBlock - 1) / kBlock;
+    if (g == 0) g = 1;
+    if (g > 65535u * 16u) g = 65535u * 16u;
+    return (unsigned)g;
+}
+
+__global__ void k_aos_to_soa(const double* __restrict__ aos, size_t n, double* __restrict__ x, double* __restrict__ y,
+                             double* __restrict__ z) {
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        x[i] = aos[3 * i];
+        y[i] = aos[3 * i + 1];
+        z[i] = aos[3 * i + 2];
+    }
+}
+
+__global__ void k_soa_to_aos(const double* __restrict__ x, const double* __restrict__ y, const double* __restrict__ z,
+                             size_t n, double* __restrict__ aos) {
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        aos[3 * i] = x[i];
+        aos[3 * i + 1] = y[i];
+        aos[3 * i + 2] = z[i];
+    }
+}
+
+__global__ void k_zero_d(double* __restrict__ p, size_t n) {
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) p[i] = 0.0;
+}
+
+__global__ void k_u32_to_u64(const uint32_t* __restrict__ a, size_t n, uint64_t* __restrict__ b) {
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) b[i] = a[i];
+}
+
+__global__ void k_u64_to_u32(const uint64_t* __restrict__ a, size_t n, uint32_t* __restrict__ b) {
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        b[i] = (uint32_t)a[i];
+}
+
+__global__ void k_copy_u64(const uint64_t* __restrict__ a, size_t n, uint64_t* __restrict__ b) {
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) b[i] = a[i];
+}
+
+int fail(bhx_ctx* c, int code, const char* what, hipError_t e = hipSuccess) {
+    if (c) {
+        c->err = what;
+        if (e != hipSuccess) {
+            c->err += ": ";
+            c->err += hipGetErrorString(e);
+        }
+    }
+    return code;
+}
+
+#define HIPCHK(expr)                                                 \
+    do {                                                             \
+        hipError_t e_ = (expr);                                      \
+        if (e_ != hipSuccess) return fail(c, BHX_E_HIP, #expr, e_); \
+    } while (0)
+
+template <typename T>
+hipError_t dalloc(T** p, size_t count) {
+    *p = nullptr;
+    if (count == 0) count = 1;
+    return hipMalloc((void**)p, count * sizeof(T));
+}
+
+void free_tree(bhx_ctx* c) {
+    DeviceTree& t = c->tree;
+    void* ptrs[] = {t.key, t.start, t.count, t.skip, t.parent, t.child_byte, t.level, t.first, t.center,
+                    t.com, t.mass, t.quad, t.quadf, t.walk, t.hkey, t.hval};
+    for (void* p : ptrs)
+        if (p) (void)hipFree(p);
+    t = DeviceTree{};
+}
+
+hipError_t alloc_tree(bhx_ctx* c, size_t cap) {
+    free_tree(c);
+    DeviceTree& t = c->tree;
+    hipError_t e;
+    if ((e = dalloc(&t.key, cap)) != hipSuccess) return e;
+    if ((e = dalloc(&t.start, cap)) != hipSuccess) return e;
+    if ((e = dalloc(&t.count, cap)) != hipSuccess) return e;
+    if ((e = dalloc(&t.skip, cap)) != hipSuccess) return e;
+    if ((e = dalloc(&t.parent, cap)) != hipSuccess) return e;
+    if ((e = dalloc(&t.child_byte, cap)) != hipSuccess) return e;
+    if ((e = dalloc(&t.level, cap)) != hipSuccess) return e;
+    if ((e = dalloc(&t.first, cap)) != hipSuccess) return e;
+    if ((e = dalloc(&t.center, 3 * cap)) != hipSuccess) return e;
+    if ((e = dalloc(&t.com, 3 * cap)) != hipSuccess) return e;
+    if ((e = dalloc(&t.mass, cap)) != hipSuccess) return e;
+    if ((e = dalloc(&t.quad, 6 * cap)) != hipSuccess) return e;
+    if ((e = dalloc(&t.quadf, 6 * cap)) != hipSuccess) return e;
+    if ((e = dalloc(&t.walk, cap)) != hipSuccess) return e;
+    size_t h = 1024;
+    while (h < 2 * cap) h <<= 1;
+    if ((e = dalloc(&t.hkey, h)) != hipSuccess) return e;
+    if ((e = dalloc(&t.hval, h)) != hipSuccess) return e;
+    t.cap = cap;
+    t.hcap = h;
+    return hipSuccess;
+}
+
+void free_all(bhx_ctx* c) {
+    free_tree(c);
+    double** dp[] = {&c->px, &c->py, &c->pz, &c->vx, &c->vy, &c->vz, &c->m, &c->ax, &c->ay, &c->az,
+                     &c->cpx, &c->cpy, &c->cpz, &c->cvx, &c->cvy, &c->cvz, &c->tx, &c->ty, &c->tz, &c->tm,
+                     &c->bounds, &c->stage};
+    for (double** p : dp)
+        if (*p) { (void)hipFree(*p); *p = nullptr; }
+    for (double*& p : c->alt)
+        if (p) { (void)hipFree(p); p = nullptr; }
+    void* vp[] = {c->keys, c->keys_alt, c->vals, c->vals_alt, c->mkeys, c->tkeys, c->tperm, c->nn, c->noff,
+                  c->nlo, c->scan_tmp, c->sort_hist, c->sort_status, c->sort_tile_ctr, c->wcount, c->ctr};
+    for (void* p : vp)
+        if (p) (void)hipFree(p);
+    if (c->ctr_host) (void)hipHostFree(c->ctr_host);
+    for (hipEvent_t& e : c->ev)
+        if (e) (void)hipEventDestroy(e);
+    if (c->ev_drift) (void)hipEventDestroy(c->ev_drift);
+    if (c->ev_bounds) (void)hipEventDestroy(c->ev_bounds);
+    if (c->s_main) (void)hipStreamDestroy(c->s_main);
+    if (c->s_side) (void)hipStreamDestroy(c->s_side);
+}
+
+int check_cfg(const bhx_config* cfg) {
+    if (!(cfg->theta >= 0.0) || !(cfg->eps >= 0.0)) return BHX_E_ARG;
+    if (cfg->leaf_capacity < 1 || cfg->leaf_capacity > kMaxLeafCap) return BHX_E_ARG;
+    if (cfg->mode != BHX_MODE_AS_RUN && cfg->mode != BHX_MODE_INTENDED) return BHX_E_ARG;
+    if (cfg->precision != BHX_FP64 && cfg->precision != BHX_FP32) return BHX_E_ARG;
+    return BHX_OK;
+}
+
+// tree keys -> sort -> tree-order copies -> node set, hash, moments
+int build_on_device(bhx_ctx* c, hipStream_t s) {
+    c->tree_valid = false;
+    if (c->n == 0) {
+        c->n_nodes = 0;
+        c->tree_valid = true;
+        return BHX_OK;
+    }
+    const double* b4 = c->bounds + 4 * c->bslot;
+    HIPCHK(launch_treekeys(c->px, c->py, c->pz, c->n, b4, c->tkeys, c->tperm, s));
+    HIPCHK(radix_sort_pairs(c->tkeys, c->tperm, c->keys_alt, c->vals_alt, c->n, c->sort_hist, c->sort_status,
+                            c->sort_tile_ctr, c->ctr, s));
+    HIPCHK(launch_tree_gather(c, s));
+    size_t total = 0;
+    for (int attempt = 0; attempt < 2; ++attempt) {
+        HIPCHK(build_tree(c, s, &total));
+        if (total <= c->tree.cap) break;
+        HIPCHK(hipStreamSynchronize(s));
+        HIPCHK(alloc_tree(c, total + total / 4 + 1024));
+    }
+    if (total > c->tree.cap) return fail(c, BHX_E_NOMEM, "node capacity");
+    c->n_nodes = total;
+    c->tree_valid = true;
+    return BHX_OK;
+}
+
+int read_counters(bhx_ctx* c, hipStream_t s) {
+    HIPCHK(hipMemcpyAsync(c->ctr_host, c->ctr, sizeof(Counters), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    if (c->ctr_host->sort_fail) return fail(c, BHX_E_SORT, "radix sort look-back did not complete");
+    if (c->ctr_host->hash_fail) return fail(c, BHX_E_HASH_FULL, "node hash table full");
+    return BHX_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int bhx_config_default(bhx_config* cfg) {
+    if (!cfg) return BHX_E_NULL;
+    std::memset(cfg, 0, sizeof *cfg);
+    cfg->theta = 0.5;
+    cfg->eps = 0.025;
+    cfg->dt = 1e-3;
+    cfg->leaf_capacity = 1;
+    cfg->mode = BHX_MODE_AS_RUN;
+    cfg->precision = BHX_FP64;
+    cfg->quadrupole = 0;
+    cfg->permute_acc = 1;
+    cfg->compensated = 0;
+    cfg->device = 0;
+    return BHX_OK;
+}
+
+int bhx_create(const bhx_config* cfg, size_t capacity, bhx_ctx** out) {
+    if (!cfg || !out) return BHX_E_NULL;
+    *out = nullptr;
+    if (check_cfg(cfg) != BHX_OK) return BHX_E_ARG;
+    if (capacity == 0 || capacity >= (1ull << 30)) return BHX_E_ARG;
+    bhx_ctx* c = new bhx_ctx();
+    c->cfg = *cfg;
+    c->device = cfg->device;
+    c->cap = capacity;
+    int rc = BHX_OK;
+    auto chk = [&](hipError_t e, const char* what) {
+        if (e != hipSuccess && rc == BHX_OK) rc = fail(c, e == hipErrorOutOfMemory ? BHX_E_NOMEM : BHX_E_HIP, what, e);
+    };
+    chk(hipSetDevice(c->device), "hipSetDevice");
+    if (rc == BHX_OK) {
+        chk(hipStreamCreateWithFlags(&c->s_main, hipStreamNonBlocking), "stream");
+        chk(hipStreamCreateWithFlags(&c->s_side, hipStreamNonBlocking), "stream");
+        chk(hipEventCreateWithFlags(&c->ev_drift, hipEventDisableTiming), "event");
+        chk(hipEventCreateWithFlags(&c->ev_bounds, hipEventDisableTiming), "event");
+        for (hipEvent_t& e : c->ev) chk(hipEventCreate(&e), "event");
+        const size_t n = capacity;
+        double** dp[] = {&c->px, &c->py, &c->pz, &c->vx, &c->vy, &c->vz, &c->m, &c->ax, &c->ay, &c->az,
+                         &c->cpx, &c->cpy, &c->cpz, &c->cvx, &c->cvy, &c->cvz, &c->tx, &c->ty, &c->tz, &c->tm};
+        for (double** p : dp) chk(dalloc(p, n), "alloc bodies");
+        for (double*& p : c->alt) chk(dalloc(&p, n), "alloc bodies");
+        chk(dalloc(&c->stage, 3 * n + 8 * 256), "alloc stage");
+        chk(dalloc(&c->bounds, 16), "alloc bounds");
+        chk(dalloc(&c->keys, n), "alloc keys");
+        chk(dalloc(&c->keys_alt, n), "alloc keys");
+        chk(dalloc(&c->vals, n), "alloc vals");
+        chk(dalloc(&c->vals_alt, n), "alloc vals");
+        chk(dalloc(&c->mkeys, n), "alloc keys");
+        chk(dalloc(&c->tkeys, n), "alloc keys");
+        chk(dalloc(&c->tperm, n), "alloc perm");
+        chk(dalloc(&c->nn, n + 1), "alloc build");
+        chk(dalloc(&c->noff, n + 1), "alloc build");
+        chk(dalloc(&c->nlo, n + 1), "alloc build");
+        chk(dalloc(&c->scan_tmp, n / 2048 + 16), "alloc scan");
+        chk(dalloc(&c->wcount, n), "alloc counts");
+        c->sort_tiles = sort_tiles_for(n);
+        chk(dalloc(&c->sort_hist, 8 * 256), "alloc sort");
+        chk(dalloc(&c->sort_tile_ctr, 8), "alloc sort");
+        chk(dalloc(&c->sort_status, 8 * 256 * (c->sort_tiles + 1)), "alloc sort");
+        chk(dalloc(&c->ctr, 1), "alloc counters");
+        chk(hipHostMalloc((void**)&c->ctr_host, sizeof(Counters), hipHostMallocDefault), "alloc pinned");
+        if (rc == BHX_OK) chk(alloc_tree(c, 2 * n + 1024), "alloc tree");
+        if (rc == BHX_OK) chk(hipMemsetAsync(c->ctr, 0, sizeof(Counters), c->s_main), "memset");
+        if (rc == BHX_OK) chk(hipStreamSynchronize(c->s_main), "sync");
+    }
+    if (rc != BHX_OK) {
+        free_all(c);
+        delete c;
+        return rc;
+    }
+    *out = c;
+    return BHX_OK;
+}
+
+int bhx_destroy(bhx_ctx* c) {
+    if (!c) return BHX_E_NULL;
+    (void)hipSetDevice(c->device);
+    if (c->s_main) (void)hipStreamSynchronize(c->s_main);
+    if (c->s_side) (void)hipStreamSynchronize(c->s_side);
+    free_all(c);
+    delete c;
+    return BHX_OK;
+}
+
+int bhx_set_config(bhx_ctx* c, const bhx_config* cfg) {
+    if (!c || !cfg) return BHX_E_NULL;
+    if (check_cfg(cfg) != BHX_OK) return fail(c, BHX_E_ARG, "invalid config");
+    if (cfg->device != c->device) return fail(c, BHX_E_ARG, "device cannot change");
+    const bool tree_shape = cfg->leaf_capacity != c->cfg.leaf_capacity || cfg->mode != c->cfg.mode;
+    c->cfg = *cfg;
+    if (tree_shape) c->tree_valid = false;
+    return BHX_OK;
+}
+
+int bhx_get_config(const bhx_ctx* c, bhx_config* cfg) {
+    if (!c || !cfg) return BHX_E_NULL;
+    *cfg = c->cfg;
+    return BHX_OK;
+}
+
+const char* bhx_last_error(const bhx_ctx* c) { return c ? c->err.c_str() : "null context"; }
+
+int bhx_sync(bhx_ctx* c) {
+    if (!c) return BHX_E_NULL;
+    HIPCHK(hipSetDevice(c->device));
+    HIPCHK(hipStreamSynchronize(c->s_side));
+    HIPCHK(hipStreamSynchronize(c->s_main));
+    return BHX_OK;
+}
+
+int bhx_set_bodies(bhx_ctx* c, const double* pos, const double* vel, const double* mass, size_t n) {
+    if (!c || !pos || !mass) return c ? fail(c, BHX_E_NULL, "bodies is a null pointer") : BHX_E_NULL;
+    if (n > c->cap) return fail(c, BHX_E_CAPACITY, "more bodies than the context capacity");
+    HIPCHK(hipSetDevice(c->device));
+    hipStream_t s = c->s_main;
+    HIPCHK(hipStreamSynchronize(c->s_side));
+    c->n = n;
+    if (n) {
+        HIPCHK(hipMemcpyAsync(c->stage, pos, 3 * n * sizeof(double), hipMemcpyHostToDevice, s));
+        hipLaunchKernelGGL(k_aos_to_soa, dim3(grid_for(n)), dim3(kBlock), 0, s, c->stage, n, c->px, c->py, c->pz);
+        if (vel) {
+            HIPCHK(hipMemcpyAsync(c->stage, vel, 3 * n * sizeof(double), hipMemcpyHostToDevice, s));
+            hipLaunchKernelGGL(k_aos_to_soa, dim3(grid_for(n)), dim3(kBlock), 0, s, c->stage, n, c->vx, c->vy, c->vz);
+        } else {
+            double* z3[] = {c->vx, c->vy, c->vz};
+            for (double* p : z3) hipLaunchKernelGGL(k_zero_d, dim3(grid_for(n)), dim3(kBlock), 0, s, p, n);
+        }
+        HIPCHK(hipMemcpyAsync(c->m, mass, n * sizeof(double), hipMemcpyHostToDevice, s));
+        double* zs[] = {c->ax, c->ay, c->az, c->cpx, c->cpy, c->cpz, c->cvx, c->cvy, c->cvz};
+        for (double* p : zs) hipLaunchKernelGGL(k_zero_d, dim3(grid_for(n)), dim3(kBlock), 0, s, p, n);
+        HIPCHK(hipGetLastError());
+    }
+    HIPCHK(hipStreamSynchronize(s));
+    c->bounds_valid = false;
+    c->tree_valid = false;
+    return BHX_OK;
+}
+
+int bhx_set_accelerations(bhx_ctx* c, const double* acc) {
+    if (!c || !acc) return c ? fail(c, BHX_E_NULL, "acc is a null pointer") : BHX_E_NULL;
+    HIPCHK(hipSetDevice(c->device));
+    hipStream_t s = c->s_main;
+    if (c->n) {
+        HIPCHK(hipMemcpyAsync(c->stage, acc, 3 * c->n * sizeof(double), hipMemcpyHostToDevice, s));
+        hipLaunchKernelGGL(k_aos_to_soa, dim3(grid_for(c->n)), dim3(kBlock), 0, s, c->stage, c->n, c->ax, c->ay, c->az);
+        HIPCHK(hipGetLastError());
+    }
+    HIPCHK(hipStreamSynchronize(s));
+    return BHX_OK;
+}
+
+int bhx_get_bodies(bhx_ctx* c, double* pos, double* vel, double* mass, double* acc) {
+    if (!c) return BHX_E_NULL;
+    HIPCHK(hipSetDevice(c->device));
+    hipStream_t s = c->s_main;
+    HIPCHK(hipStreamSynchronize(c->s_side));
+    const size_t n = c->n;
+    if (!n) return BHX_OK;
+    struct { double* dst; double *x, *y, *z; } outs[] = {{pos, c->px, c->py, c->pz}, {vel, c->vx, c->vy, c->vz},
+                                                         {acc, c->ax, c->ay, c->az}};
+    for (auto& o : outs) {
+        if (!o.dst) continue;
+        hipLaunchKernelGGL(k_soa_to_aos, dim3(grid_for(n)), dim3(kBlock), 0, s, o.x, o.y, o.z, n, c->stage);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipMemcpyAsync(o.dst, c->stage, 3 * n * sizeof(double), hipMemcpyDeviceToHost, s));
+        HIPCHK(hipStreamSynchronize(s));
+    }
+    if (mass) HIPCHK(hipMemcpyAsync(mass, c->m, n * sizeof(double), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    return BHX_OK;
+}
+
+int bhx_num_bodies(const bhx_ctx* c, size_t* n) {
+    if (!c || !n) return BHX_E_NULL;
+    *n = c->n;
+    return BHX_OK;
+}
+
+int bhx_bounds(bhx_ctx* c, double center[3], double* size) {
+    if (!c) return BHX_E_NULL;
+    HIPCHK(hipSetDevice(c->device));
+    hipStream_t s = c->s_main;
+    HIPCHK(hipStreamSynchronize(c->s_side));
+    double* b4 = c->bounds + 4 * c->bslot;
+    HIPCHK(launch_bounds_mean(c->px, c->py, c->pz, c->n, b4, s));
+    HIPCHK(launch_bounds_size(c->px, c->py, c->pz, c->n, b4, s));
+    double h[4];
+    HIPCHK(hipMemcpyAsync(h, b4, sizeof h, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    c->bounds_valid = true;
+    if (center) { center[0] = h[0]; center[1] = h[1]; center[2] = h[2]; }
+    if (size) *size = h[3];
+    return BHX_OK;
+}
+
+int bhx_morton_keys(bhx_ctx* c, double size, uint64_t* keys_out) {
+    if (!c) return BHX_E_NULL;
+    HIPCHK(hipSetDevice(c->device));
+    hipStream_t s = c->s_main;
+    double h[4] = {0.0, 0.0, 0.0, size};
+    double* b4 = c->bounds + 8;      // scratch slot
+    HIPCHK(hipMemcpyAsync(b4, h, sizeof h, hipMemcpyHostToDevice, s));
+    HIPCHK(launch_morton(c->px, c->py, c->pz, c->n, b4, c->mkeys, nullptr, s));
+    if (keys_out && c->n) HIPCHK(hipMemcpyAsync(keys_out, c->mkeys, c->n * sizeof(uint64_t), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    return BHX_OK;
+}
+
+int bhx_sort_keys(bhx_ctx* c, uint64_t* keys, uint64_t* idx, size_t n) {
+    if (!c || !keys || !idx) return c ? fail(c, BHX_E_NULL, "keys is a null pointer") : BHX_E_NULL;
+    if (n > c->cap) return fail(c, BHX_E_CAPACITY, "more keys than the context capacity");
+    if (n <= 1) return BHX_OK;
+    HIPCHK(hipSetDevice(c->device));
+    hipStream_t s = c->s_main;
+    HIPCHK(hipStreamSynchronize(c->s_side));
+    HIPCHK(hipMemcpyAsync(c->keys, keys, n * sizeof(uint64_t), hipMemcpyHostToDevice, s));
+    HIPCHK(launch_iota(c->vals, n, s));
+    HIPCHK(radix_sort_pairs(c->keys, c->vals, c->keys_alt, c->vals_alt, n, c->sort_hist, c->sort_status,
+                            c->sort_tile_ctr, c->ctr, s));
+    std::vector<uint32_t> perm(n);
+    HIPCHK(hipMemcpyAsync(keys, c->keys, n * sizeof(uint64_t), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipMemcpyAsync(perm.data(), c->vals, n * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+    int rc = read_counters(c, s);
+    if (rc != BHX_OK) return rc;
+    std::vector<uint64_t> old(idx, idx + n);
+    for (size_t i = 0; i < n; ++i) idx[i] = old[perm[i]];
+    return BHX_OK;
+}
+
+// ReorderBodiesByIndexes (Body.hpp:257-273) on the resident arrays: gather into
+// the ping-pong buffers, then swap.  Accelerations follow only when
+// permute_acc is set (the reference leaves them in place, ofApp.cpp:176).
+static int reorder_by(bhx_ctx* c, const uint32_t* perm_dev, hipStream_t s) {
+    double** all[16] = {&c->px, &c->py, &c->pz, &c->vx, &c->vy, &c->vz, &c->m, &c->cpx,
+                        &c->cpy, &c->cpz, &c->cvx, &c->cvy, &c->cvz, &c->ax, &c->ay, &c->az};
+    int slots[16];
+    int na = 0;
+    for (int k = 0; k < 7; ++k) slots[na++] = k;
+    if (c->cfg.compensated)
+        for (int k = 7; k < 13; ++k) slots[na++] = k;
+    if (c->cfg.permute_acc)
+        for (int k = 13; k < 16; ++k) slots[na++] = k;
+    double* src[16];
+    double* dst[16];
+    for (int k = 0; k < na; ++k) {
+        src[k] = *all[slots[k]];
+        dst[k] = c->alt[slots[k]];
+    }
+    HIPCHK(launch_gather(src, dst, na, perm_dev, c->n, s));
+    for (int k = 0; k < na; ++k) {
+        const int sl = slots[k];
+        double* t = *all[sl];
+        *all[sl] = c->alt[sl];
+        c->alt[sl] = t;
+    }
+    c->bounds_valid = false;
+    c->tree_valid = false;
+    return BHX_OK;
+}
+
+int bhx_sort_bodies(bhx_ctx* c, uint64_t* perm_out) {
+    if (!c) return BHX_E_NULL;
+    HIPCHK(hipSetDevice(c->device));
+    hipStream_t s = c->s_main;
+    HIPCHK(hipStreamSynchronize(c->s_side));
+    if (c->n == 0) return BHX_OK;
+    hipLaunchKernelGGL(k_copy_u64, dim3(grid_for(c->n)), dim3(kBlock), 0, s, c->mkeys, c->n, c->keys);
+    HIPCHK(launch_iota(c->vals, c->n, s));
+    HIPCHK(radix_sort_pairs(c->keys, c->vals, c->keys_alt, c->vals_alt, c->n, c->sort_hist, c->sort_status,
+                            c->sort_tile_ctr, c->ctr, s));
+    int rc = reorder_by(c, c->vals, s);
+    if (rc != BHX_OK) return rc;
+    if (perm_out) {
+        uint64_t* tmp = reinterpret_cast<uint64_t*>(c->stage);
+        hipLaunchKernelGGL(k_u32_to_u64, dim3(grid_for(c->n)), dim3(kBlock), 0, s, c->vals, c->n, tmp);
+        HIPCHK(hipMemcpyAsync(perm_out, tmp, c->n * sizeof(uint64_t), hipMemcpyDeviceToHost, s));
+    }
+    return read_counters(c, s);
+}
+
+int bhx_reorder_bodies(bhx_ctx* c, const uint64_t* perm) {
+    if (!c || !perm) return c ? fail(c, BHX_E_NULL, "indexes is a null pointer") : BHX_E_NULL;
+    HIPCHK(hipSetDevice(c->device));
+    hipStream_t s = c->s_main;
+    HIPCHK(hipStreamSynchronize(c->s_side));
+    if (c->n == 0) return BHX_OK;
+    for (size_t i = 0; i < c->n; ++i)
+        if (perm[i] >= c->n) return fail(c, BHX_E_ARG, "index out of range");
+    uint64_t* tmp = reinterpret_cast<uint64_t*>(c->stage);
+    HIPCHK(hipMemcpyAsync(tmp, perm, c->n * sizeof(uint64_t), hipMemcpyHostToDevice, s));
+    hipLaunchKernelGGL(k_u64_to_u32, dim3(grid_for(c->n)), dim3(kBlock), 0, s, tmp, c->n, c->vals_alt);
+    int rc = reorder_by(c, c->vals_alt, s);
+    if (rc != BHX_OK) return rc;
+    HIPCHK(hipStreamSynchronize(s));
+    return BHX_OK;
+}
+
+int bhx_kick_drift(bhx_ctx* c, double dt) {
+    if (!c) return BHX_E_NULL;
+    HIPCHK(hipSetDevice(c->device));
+    HIPCHK(hipStreamSynchronize(c->s_side));
+    if (c->n) HIPCHK(launch_kick_drift(c, dt, c->s_main));
+    HIPCHK(hipStreamSynchronize(c->s_main));
+    c->bounds_valid = false;
+    c->tree_valid = false;
+    return BHX_OK;
+}
+
+int bhx_kick(bhx_ctx* c, double dt) {
+    if (!c) return BHX_E_NULL;
+    HIPCHK(hipSetDevice(c->device));
+    if (c->n) HIPCHK(launch_kick(c, dt, c->s_main));
+    HIPCHK(hipStreamSynchronize(c->s_main));
+    return BHX_OK;
+}
+
+int bhx_build(bhx_ctx* c, const double center[3], double size, size_t* n_nodes) {
+    if (!c || !center) return BHX_E_NULL;
+    if (!std::isfinite(size) || !std::isfinite(center[0]) || !std::isfinite(center[1]) || !std::isfinite(center[2]))
+        return fail(c, BHX_E_NONFINITE, "non-finite bounds");
+    HIPCHK(hipSetDevice(c->device));
+    hipStream_t s = c->s_main;
+    HIPCHK(hipStreamSynchronize(c->s_side));
+    double h[4] = {center[0], center[1], center[2], size};
+    HIPCHK(hipMemcpyAsync(c->bounds + 4 * c->bslot, h, sizeof h, hipMemcpyHostToDevice, s));
+    c->bounds_valid = false;
+    int rc = build_on_device(c, s);
+    if (rc != BHX_OK) return rc;
+    rc = read_counters(c, s);
+    if (rc != BHX_OK) return rc;
+    std::memcpy(c->tree_center, center, sizeof c->tree_center);
+    c->tree_size = size;
+    if (n_nodes) *n_nodes = c->n_nodes;
+    return BHX_OK;
+}
+
+int bhx_get_nodes(bhx_ctx* c, bhx_node* out, size_t cap, size_t* n_nodes) {
+    if (!c) return BHX_E_NULL;
+    if (!c->tree_valid) return fail(c, BHX_E_STATE, "no tree built");
+    if (n_nodes) *n_nodes = c->n_nodes;
+    if (!out) return BHX_OK;
+    const size_t m = c->n_nodes < cap ? c->n_nodes : cap;
+    if (m == 0) return BHX_OK;
+    HIPCHK(hipSetDevice(c->device));
+    hipStream_t s = c->s_main;
+    const DeviceTree& t = c->tree;
+    std::vector<uint64_t> key(m);
+    std::vector<uint32_t> start(m), count(m), skip(m), cb(m), level(m), first(m);
+    std::vector<double> center(3 * m), com(3 * m), mass(m), quad(6 * m);
+    HIPCHK(hipMemcpyAsync(key.data(), t.key, m * 8, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipMemcpyAsync(start.data(), t.start, m * 4, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipMemcpyAsync(count.data(), t.count, m * 4, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipMemcpyAsync(skip.data(), t.skip, m * 4, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipMemcpyAsync(cb.data(), t.child_byte, m * 4, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipMemcpyAsync(level.data(), t.level, m * 4, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipMemcpyAsync(first.data(), t.first, m * 4, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipMemcpyAsync(center.data(), t.center, m * 24, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipMemcpyAsync(com.data(), t.com, m * 24, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipMemcpyAsync(mass.data(), t.mass, m * 8, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipMemcpyAsync(quad.data(), t.quad, m * 48, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    for (size_t i = 0; i < m; ++i) {
+        bhx_node& o = out[i];
+        o.key = key[i];
+        for (int a = 0; a < 3; ++a) { o.center[a] = center[3 * i + a]; o.com[a] = com[3 * i + a]; }
+        double sz = c->tree_size;
+        for (uint32_t l = 0; l < level[i]; ++l) sz = sz * 0.5;
+        o.size = sz;
+        o.mass = mass[i];
+        for (int q = 0; q < 6; ++q) o.quad[q] = quad[6 * i + q];
+        o.n = count[i];
+        o.first = first[i];
+        o.level = (int32_t)level[i];
+        o.child_byte = cb[i];
+        o.skip = skip[i];
+        o.body_start = start[i];
+    }
+    return BHX_OK;
+}
+
+int bhx_lookup(bhx_ctx* c, const uint64_t* keys, size_t n, int64_t* ids) {
+    if (!c || !keys || !ids) return c ? fail(c, BHX_E_NULL, "null argument") : BHX_E_NULL;
+    if (!c->tree_valid) return fail(c, BHX_E_STATE, "no tree built");
+    if (n == 0) return BHX_OK;
+    if (c->n_nodes == 0) {
+        for (size_t i = 0; i < n; ++i) ids[i] = -1;
+        return BHX_OK;
+    }
+    HIPCHK(hipSetDevice(c->device));
+    hipStream_t s = c->s_main;
+    uint64_t* dk = nullptr;
+    int64_t* di = nullptr;
+    HIPCHK(hipMalloc((void**)&dk, n * sizeof(uint64_t)));
+    hipError_t e = hipMalloc((void**)&di, n * sizeof(int64_t));
+    if (e == hipSuccess) e = hipMemcpyAsync(dk, keys, n * sizeof(uint64_t), hipMemcpyHostToDevice, s);
+    if (e == hipSuccess) e = launch_lookup(c->tree, dk, n, di, s);
+    if (e == hipSuccess) e = hipMemcpyAsync(ids, di, n * sizeof(int64_t), hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    (void)hipFree(dk);
+    if (di) (void)hipFree(di);
+    if (e != hipSuccess) return fail(c, BHX_E_HIP, "lookup", e);
+    return BHX_OK;
+}
+
+int bhx_get_tree_order(bhx_ctx* c, uint64_t* tree_perm, uint64_t* tree_keys) {
+    if (!c) return BHX_E_NULL;
+    if (!c->tree_valid) return fail(c, BHX_E_STATE, "no tree built");
+    HIPCHK(hipSetDevice(c->device));
+    hipStream_t s = c->s_main;
+    if (c->n == 0) return BHX_OK;
+    if (tree_perm) {
+        uint64_t* tmp = reinterpret_cast<uint64_t*>(c->stage);
+        hipLaunchKernelGGL(k_u32_to_u64, dim3(grid_for(c->n)), dim3(kBlock), 0, s, c->tperm, c->n, tmp);
+        HIPCHK(hipMemcpyAsync(tree_perm, tmp, c->n * 8, hipMemcpyDeviceToHost, s));
+        HIPCHK(hipStreamSynchronize(s));
+    }
+    if (tree_keys) HIPCHK(hipMemcpyAsync(tree_keys, c->tkeys, c->n * 8, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    return BHX_OK;
+}
+
+int bhx_unsplit_leaves(bhx_ctx* c, size_t* count) {
+    if (!c || !count) return BHX_E_NULL;
+    HIPCHK(hipSetDevice(c->device));
+    int rc = read_counters(c, c->s_main);
+    if (rc != BHX_OK) return rc;
+    *count = c->ctr_host->unsplit;
+    return BHX_OK;
+}
+
+int bhx_walk(bhx_ctx* c, double* acc_out, uint64_t* counts) {
+    if (!c) return BHX_E_NULL;
+    if (!c->tree_valid) return fail(c, BHX_E_STATE, "walk before build");
+    HIPCHK(hipSetDevice(c->device));
+    hipStream_t s = c->s_main;
+    HIPCHK(launch_walk(c, s));
+    if (acc_out && c->n) {
+        hipLaunchKernelGGL(k_soa_to_aos, dim3(grid_for(c->n)), dim3(kBlock), 0, s, c->ax, c->ay, c->az, c->n, c->stage);
+        HIPCHK(hipMemcpyAsync(acc_out, c->stage, 3 * c->n * sizeof(double), hipMemcpyDeviceToHost, s));
+        HIPCHK(hipStreamSynchronize(s));
+    }
+    if (counts && c->n) {
+        uint64_t* tmp = reinterpret_cast<uint64_t*>(c->stage);
+        hipLaunchKernelGGL(k_u32_to_u64, dim3(grid_for(c->n)), dim3(kBlock), 0, s, c->wcount, c->n, tmp);
+        HIPCHK(hipMemcpyAsync(counts, tmp, c->n * 8, hipMemcpyDeviceToHost, s));
+    }
+    HIPCHK(hipStreamSynchronize(s));
+    return BHX_OK;
+}
+
+int bhx_set_profiling(bhx_ctx* c, int on) {
+    if (!c) return BHX_E_NULL;
+    c->profiling = on != 0;
+    return BHX_OK;
+}
+
+int bhx_last_timing(bhx_ctx* c, bhx_timing* t) {
+    if (!c || !t) return BHX_E_NULL;
+    *t = c->timing;
+    return BHX_OK;
+}
+
+// ofApp::update() (bounds, keys, body sort on x_f — ofApp.cpp:169-177) then
+// ofApp::draw() (kick + drift to x_{f+1}, tree on x_{f+1} with the bounds of
+// x_f, walk, kick — ofApp.cpp:248-282).  The bounds of x_{f+1}, needed by the
+// next frame, are computed on a side stream as soon as the drift is done, in
+// parallel with this frame's tree build and walk.
+int bhx_step(bhx_ctx* c, int nsteps) {
+    if (!c) return BHX_E_NULL;
+    if (nsteps < 0) return fail(c, BHX_E_ARG, "negative step count");
+    HIPCHK(hipSetDevice(c->device));
+    hipStream_t s = c->s_main;
+    const double dt = c->cfg.dt;
+    const auto t0 = std::chrono::steady_clock::now();
+    for (int step = 0; step < nsteps && c->n; ++step) {
+        const int cur = c->bslot;
+        double* b_cur = c->bounds + 4 * cur;
+        double* b_next = c->bounds + 4 * (cur ^ 1);
+        const bool prof = c->profiling;
+        if (prof) HIPCHK(hipEventRecord(c->ev[0], s));
+        if (!c->bounds_valid) {
+            HIPCHK(launch_bounds_mean(c->px, c->py, c->pz, c->n, b_cur, s));
+            HIPCHK(launch_bounds_size(c->px, c->py, c->pz, c->n, b_cur, s));
+        } else {
+            HIPCHK(hipStreamWaitEvent(s, c->ev_bounds, 0));
+        }
+        if (prof) HIPCHK(hipEventRecord(c->ev[1], s));
+        HIPCHK(launch_morton(c->px, c->py, c->pz, c->n, b_cur, c->keys, c->vals, s));
+        if (prof) HIPCHK(hipEventRecord(c->ev[2], s));
+        HIPCHK(radix_sort_pairs(c->keys, c->vals, c->keys_alt, c->vals_alt, c->n, c->sort_hist, c->sort_status,
+                                c->sort_tile_ctr, c->ctr, s));
+        if (prof) HIPCHK(hipEventRecord(c->ev[3], s));
+        HIPCHK(launch_permute_kick_drift(c, c->vals, dt, s));
+        HIPCHK(hipEventRecord(c->ev_drift, s));
+        if (prof) HIPCHK(hipEventRecord(c->ev[4], s));
+        HIPCHK(hipStreamWaitEvent(c->s_side, c->ev_drift, 0));
+        HIPCHK(launch_bounds_mean(c->px, c->py, c->pz, c->n, b_next, c->s_side));
+        HIPCHK(launch_bounds_size(c->px, c->py, c->pz, c->n, b_next, c->s_side));
+        HIPCHK(hipEventRecord(c->ev_bounds, c->s_side));
+        // tree on x_{f+1} with the bounds of x_f
+        HIPCHK(launch_treekeys(c->px, c->py, c->pz, c->n, b_cur, c->tkeys, c->tperm, s));
+        HIPCHK(radix_sort_pairs(c->tkeys, c->tperm, c->keys_alt, c->vals_alt, c->n, c->sort_hist, c->sort_status,
+                                c->sort_tile_ctr, c->ctr, s));
+        HIPCHK(launch_tree_gather(c, s));
+        if (prof) HIPCHK(hipEventRecord(c->ev[5], s));
+        size_t total = 0;
+        for (int attempt = 0; attempt < 2; ++attempt) {
+            HIPCHK(build_tree(c, s, &total));
+            if (total <= c->tree.cap) break;
+            HIPCHK(hipStreamSynchronize(s));
+            HIPCHK(alloc_tree(c, total + total / 4 + 1024));
+        }
+        if (total > c->tree.cap) return fail(c, BHX_E_NOMEM, "node capacity");
+        c->n_nodes = total;
+        c->tree_valid = true;
+        if (prof) HIPCHK(hipEventRecord(c->ev[6], s));
+        HIPCHK(launch_walk(c, s));
+        if (prof) HIPCHK(hipEventRecord(c->ev[7], s));
+        HIPCHK(launch_kick(c, dt, s));
+        if (prof) HIPCHK(hipEventRecord(c->ev[8], s));
+        c->bslot = cur ^ 1;
+        c->bounds_valid = true;
+        if (prof) {
+            HIPCHK(hipEventSynchronize(c->ev[8]));
+            float ms[8];
+            for (int k = 0; k < 8; ++k) HIPCHK(hipEventElapsedTime(&ms[k], c->ev[k], c->ev[k + 1]));
+            c->timing.bounds = ms[0];
+            c->timing.keys = ms[1];
+            c->timing.sort = ms[2];
+            c->timing.kick_drift = ms[3];
+            c->timing.treekeys = ms[4];
+            c->timing.build = ms[5];
+            c->timing.moments = 0.0;
+            c->timing.walk = ms[6];
+            c->timing.kick = ms[7];
+        }
+    }
+    int rc = read_counters(c, s);
+    if (rc != BHX_OK) return rc;
+    HIPCHK(hipStreamSynchronize(c->s_side));
+    // keep the build bounds of the last frame for node downloads
+    double h[4];
+    HIPCHK(hipMemcpy(h, c->bounds + 4 * (c->bslot ^ 1), sizeof h, hipMemcpyDeviceToHost));
+    c->tree_center[0] = h[0]; c->tree_center[1] = h[1]; c->tree_center[2] = h[2];
+    c->tree_size = h[3];
+    const auto t1 = std::chrono::steady_clock::now();
+    c->timing.total = std::chrono::duration<double, std::milli>(t1 - t0).count();
+    return BHX_OK;
+}
+
+int bhx_direct(bhx_ctx* c, const int64_t* targets, size_t nt, double eps, double* acc_out, double* pot_out) {
+    if (!c || !targets || !acc_out) return c ? fail(c, BHX_E_NULL, "null argument") : BHX_E_NULL;
+    for (size_t k = 0; k < nt; ++k)
+        if (targets[k] < 0 || (size_t)targets[k] >= c->n) return fail(c, BHX_E_ARG, "target out of range");
+    if (nt == 0) return BHX_OK;
+    HIPCHK(hipSetDevice(c->device));
+    hipStream_t s = c->s_main;
+    int64_t* dt = nullptr;
+    double *da = nullptr, *dp = nullptr;
+    hipError_t e = hipMalloc((void**)&dt, nt * 8);
+    if (e == hipSuccess) e = hipMalloc((void**)&da, nt * 24);
+    if (e == hipSuccess) e = hipMalloc((void**)&dp, nt * 8);
+    if (e == hipSuccess) e = hipMemcpyAsync(dt, targets, nt * 8, hipMemcpyHostToDevice, s);
+    if (e == hipSuccess) e = launch_direct(c, dt, nt, eps, da, dp, s);
+    if (e == hipSuccess) e = hipMemcpyAsync(acc_out, da, nt * 24, hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess && pot_out) e = hipMemcpyAsync(pot_out, dp, nt * 8, hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    if (dt) (void)hipFree(dt);
+    if (da) (void)hipFree(da);
+    if (dp) (void)hipFree(dp);
+    if (e != hipSuccess) return fail(c, BHX_E_HIP, "direct sum", e);
+    return BHX_OK;
+}
+
+int bhx_diagnostics(bhx_ctx* c, double* ekin, double mom[3], double angmom[3]) {
+    if (!c) return BHX_E_NULL;
+    HIPCHK(hipSetDevice(c->device));
+    hipStream_t s = c->s_main;
+    HIPCHK(hipStreamSynchronize(c->s_side));
+    double* out = c->stage + 7 * 256;
+    HIPCHK(launch_diag(c, out, s));
+    double h[7];
+    HIPCHK(hipMemcpyAsync(h, out, sizeof h, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    if (ekin) *ekin = h[0];
+    if (mom) { mom[0] = h[1]; mom[1] = h[2]; mom[2] = h[3]; }
+    if (angmom) { angmom[0] = h[4]; angmom[1] = h[5]; angmom[2] = h[6]; }
+    return BHX_OK;
+}
+
+}  // extern "C"

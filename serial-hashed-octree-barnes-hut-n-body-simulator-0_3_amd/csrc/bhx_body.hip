@@ -1,0 +1,403 @@
+// bhx_body.hip — per-body kernels whose FP64 results must match the reference
+// bit for bit: bounds, Morton keys, tree path keys, gathers and the leapfrog.
+// Compiled with -ffp-contract=off: every operation rounds as written.
+#include "bhx_internal.hpp"
+
+namespace bhx {
+namespace {
+
+constexpr int kBlock = 256;
+constexpr int kMeanChunk = 1024;
+
+inline unsigned grid_for(size_t n, int per_block = kBlock, unsigned cap = 65535u * 16u) {
+    size_t g = (n + per_block - 1) / per_block;
+    if (g == 0) g = 1;
+    return (unsigned)(g < cap ? g : cap);
+}
+
+// ---------------------------------------------------------------- bounds
+
+// OctantBounds(Body*, N) centre — OctantUtils.cpp:31-38 with Vec3D::operator/=
+// (MathUtils.cpp:81-90): a left-to-right FP64 sum in array order, then a
+// multiply by fl(1/N).  The sum is one dependent chain per axis; waves 1..3
+// stream chunks of positions into LDS (double buffered) while lanes 0..2 of
+// wave 0 run the x, y and z chains over the staged chunk.
+__global__ __launch_bounds__(kBlock) void k_bounds_mean(const double* __restrict__ px,
+                                                        const double* __restrict__ py,
+                                                        const double* __restrict__ pz, size_t n,
+                                                        double* __restrict__ out4) {
+    __shared__ double buf[2][3][kMeanChunk];
+    const int tid = threadIdx.x;
+    const int wave = tid >> 6, lane = tid & 63;
+    const size_t nchunks = (n + kMeanChunk - 1) / kMeanChunk;
+
+    auto stage = [&](size_t ch, int b) {
+        if (wave == 0) return;
+        const size_t base = ch * kMeanChunk;
+        for (int k = tid - 64; k < 3 * kMeanChunk; k += kBlock - 64) {
+            const int a = k / kMeanChunk, j = k - a * kMeanChunk;
+            const size_t i = base + j;
+            const double* src = a == 0 ? px : (a == 1 ? py : pz);
+            buf[b][a][j] = i < n ? src[i] : 0.0;
+        }
+    };
+
+    double s = 0.0;
+    if (nchunks) stage(0, 0);
+    __syncthreads();
+    for (size_t ch = 0; ch < nchunks; ++ch) {
+        const int b = (int)(ch & 1);
+        if (ch + 1 < nchunks) stage(ch + 1, b ^ 1);
+        if (wave == 0 && lane < 3) {
+            const size_t base = ch * kMeanChunk;
+            const int cnt = (int)((n - base) < (size_t)kMeanChunk ? (n - base) : (size_t)kMeanChunk);
+            const double* v = buf[b][lane];
+            int j = 0;
+            for (; j + 16 <= cnt; j += 16) {
+                double t[16];
+#pragma unroll
+                for (int u = 0; u < 16; ++u) t[u] = v[j + u];
+#pragma unroll
+                for (int u = 0; u < 16; ++u) s = s + t[u];
+            }
+            for (; j < cnt; ++j) s = s + v[j];
+        }
+        __syncthreads();
+    }
+    if (wave == 0 && lane < 3) {
+        const double dn = (double)n;
+        if (dn != 0.0) {
+            const double inv = 1.0 / dn;
+            s = s * inv;
+        }
+        out4[lane] = s;
+    }
+    if (tid == 0) out4[3] = 0.0;
+}
+
+// OctantBounds size — OctantUtils.cpp:47-58: max over bodies and axes of
+// fabs(p) - c, floored at 0.  A max is exact in any order, so a parallel
+// reduction reproduces it; non-negative doubles order like their bit patterns.
+__global__ __launch_bounds__(kBlock) void k_bounds_size(const double* __restrict__ px,
+                                                        const double* __restrict__ py,
+                                                        const double* __restrict__ pz, size_t n,
+                                                        double* __restrict__ out4) {
+    const double cx = out4[0], cy = out4[1], cz = out4[2];
+    double s = 0.0;
+    for (size_t i = blockIdx.x * (size_t)kBlock + threadIdx.x; i < n; i += (size_t)gridDim.x * kBlock) {
+        double t;
+        t = fabs(px[i]) - cx; if (t > s) s = t;
+        t = fabs(py[i]) - cy; if (t > s) s = t;
+        t = fabs(pz[i]) - cz; if (t > s) s = t;
+    }
+    __shared__ double red[kBlock / 64];
+    for (int off = 32; off > 0; off >>= 1) {
+        const double o = __shfl_down(s, off, 64);
+        if (o > s) s = o;
+    }
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int w = 1; w < kBlock / 64; ++w)
+            if (red[w] > s) s = red[w];
+        atomicMax(reinterpret_cast<unsigned long long*>(out4 + 3),
+                  (unsigned long long)__double_as_longlong(s));
+    }
+}
+
+// ---------------------------------------------------------------- keys
+
+// (spatialKey)(xScaled) in MortonKeys.hpp:162-164 with x86-64 SSE2 semantics:
+// truncate to int64 (0x8000000000000000 when out of range or NaN), then
+// reinterpret as uint64.
+__device__ __forceinline__ uint64_t cvt_u64_x86(double u) {
+    if (!(u >= -9223372036854775808.0 && u < 9223372036854775808.0)) return 0x8000000000000000ull;
+    return (uint64_t)(int64_t)u;
+}
+
+// interleaveBits' per-axis spread (MortonKeys.hpp:100-116) with the magic-bit
+// masks of MortonKeys.hpp:51-58.
+__device__ __forceinline__ uint64_t spread21(uint64_t t) {
+    t = (t | (t << 32)) & 0x001f00000000ffffull;
+    t = (t | (t << 16)) & 0x001f0000ff0000ffull;
+    t = (t | (t << 8)) & 0x100f00f00f00f00full;
+    t = (t | (t << 4)) & 0x10c30c30c30c30c3ull;
+    t = (t | (t << 2)) & 0x1249249249249249ull;
+    return t;
+}
+
+// computeMortonKey — MortonKeys.hpp:137-181; the sort payload starts as the
+// identity (RadixSortBodies' indexes[i] = i, Body.hpp:285-289).
+__global__ __launch_bounds__(kBlock) void k_morton(const double* __restrict__ px,
+                                                   const double* __restrict__ py,
+                                                   const double* __restrict__ pz, size_t n,
+                                                   const double* __restrict__ b4,
+                                                   uint64_t* __restrict__ keys,
+                                                   uint32_t* __restrict__ vals) {
+    const double size = b4[3];
+    const double frac = 2097152.0 / (2.0 * size);
+    for (size_t i = blockIdx.x * (size_t)kBlock + threadIdx.x; i < n; i += (size_t)gridDim.x * kBlock) {
+        double xs = px[i] + size;
+        double ys = py[i] + size;
+        double zs = pz[i] + size;
+        xs = xs * frac;
+        ys = ys * frac;
+        zs = zs * frac;
+        const uint64_t X = spread21(cvt_u64_x86(xs));
+        const uint64_t Y = spread21(cvt_u64_x86(ys));
+        const uint64_t Z = spread21(cvt_u64_x86(zs));
+        keys[i] = (1ull << 63) | (X << 2) | (Y << 1) | Z;
+        if (vals) vals[i] = (uint32_t)i;
+    }
+}
+
+// The descent a body makes through insertBody (HashedOctree.cpp:474-505):
+// octant by strict > against the cell centre (DetermineOctant,
+// OctantUtils.hpp:64-70), child key parent<<3|o (GetChildKey :97-101), child
+// edge size*0.5 and centre c + (dir*0.5)*edge (HOTNode child constructor,
+// HashedNode.cpp:50-56).  Written as one 64-bit path key over 21 levels.
+__global__ __launch_bounds__(kBlock) void k_treekeys(const double* __restrict__ px,
+                                                     const double* __restrict__ py,
+                                                     const double* __restrict__ pz, size_t n,
+                                                     const double* __restrict__ b4,
+                                                     uint64_t* __restrict__ keys,
+                                                     uint32_t* __restrict__ vals) {
+    const double c0x = b4[0], c0y = b4[1], c0z = b4[2], s0 = b4[3];
+    for (size_t i = blockIdx.x * (size_t)kBlock + threadIdx.x; i < n; i += (size_t)gridDim.x * kBlock) {
+        const double x = px[i], y = py[i], z = pz[i];
+        double cx = c0x, cy = c0y, cz = c0z, s = s0;
+        uint64_t k = 1;
+#pragma unroll
+        for (int L = 1; L <= kMaxLevel; ++L) {
+            const unsigned ox = x > cx, oy = y > cy, oz = z > cz;
+            k = (k << 3) | (ox << 2) | (oy << 1) | oz;
+            s = s * 0.5;
+            cx = cx + ((ox ? 1.0 : -1.0) * 0.5) * s;
+            cy = cy + ((oy ? 1.0 : -1.0) * 0.5) * s;
+            cz = cz + ((oz ? 1.0 : -1.0) * 0.5) * s;
+        }
+        keys[i] = k;
+        vals[i] = (uint32_t)i;
+    }
+}
+
+__global__ void k_iota(uint32_t* __restrict__ v, size_t n) {
+    for (size_t i = blockIdx.x * (size_t)kBlock + threadIdx.x; i < n; i += (size_t)gridDim.x * kBlock)
+        v[i] = (uint32_t)i;
+}
+
+// ---------------------------------------------------------------- gathers / leapfrog
+
+struct GatherArgs {
+    const double* src[16];
+    double* dst[16];
+    int narr;
+};
+
+// ReorderBodiesByIndexes — Body.hpp:257-273: new[i] = old[perm[i]].
+__global__ __launch_bounds__(kBlock) void k_gather(GatherArgs g, const uint32_t* __restrict__ perm, size_t n) {
+    for (size_t i = blockIdx.x * (size_t)kBlock + threadIdx.x; i < n; i += (size_t)gridDim.x * kBlock) {
+        const uint32_t j = perm[i];
+        for (int a = 0; a < g.narr; ++a) g.dst[a][i] = g.src[a][j];
+    }
+}
+
+struct BodyArrays {
+    const double *px, *py, *pz, *vx, *vy, *vz, *m, *ax, *ay, *az;
+    const double *cpx, *cpy, *cpz, *cvx, *cvy, *cvz;
+    double *opx, *opy, *opz, *ovx, *ovy, *ovz, *om;
+    double *ocpx, *ocpy, *ocpz, *ocvx, *ocvy, *ocvz;
+};
+
+// Kick then drift — ComputePositionAtHalfTimeStep, Body.hpp:306-313:
+// v = v + (a*dt)*0.5; x = x + v*dt.  The compensated form keeps the rounding
+// error of each accumulation (Kahan) and feeds it into the next one.
+__device__ __forceinline__ void kick_drift_one(double& x, double& v, double a, double dt) {
+    v = v + (a * dt) * 0.5;
+    x = x + v * dt;
+}
+__device__ __forceinline__ void kahan_add(double& s, double& c, double inc) {
+    const double y = inc - c;
+    const double t = s + y;
+    c = (t - s) - y;
+    s = t;
+}
+
+// Fused RadixSortBodies gather (Body.hpp:296) + kick/drift (Body.hpp:306-313)
+// for the reference frame order: bodies are reordered by `perm` and moved in
+// one pass.  With permute_acc = 0 the acceleration slot i is not reordered
+// (the as-run behaviour, ofApp.cpp:176,248).
+template <bool kComp>
+__global__ __launch_bounds__(kBlock) void k_permute_kick_drift(BodyArrays b, const uint32_t* __restrict__ perm,
+                                                               size_t n, double dt, int permute_acc) {
+    for (size_t i = blockIdx.x * (size_t)kBlock + threadIdx.x; i < n; i += (size_t)gridDim.x * kBlock) {
+        const size_t j = perm ? perm[i] : i;
+        const size_t ja = permute_acc ? j : i;
+        double x = b.px[j], y = b.py[j], z = b.pz[j];
+        double vx = b.vx[j], vy = b.vy[j], vz = b.vz[j];
+        const double ax = b.ax[ja], ay = b.ay[ja], az = b.az[ja];
+        if (!kComp) {
+            kick_drift_one(x, vx, ax, dt);
+            kick_drift_one(y, vy, ay, dt);
+            kick_drift_one(z, vz, az, dt);
+        } else {
+            double cvx = b.cvx[j], cvy = b.cvy[j], cvz = b.cvz[j];
+            double cpx = b.cpx[j], cpy = b.cpy[j], cpz = b.cpz[j];
+            kahan_add(vx, cvx, (ax * dt) * 0.5);
+            kahan_add(vy, cvy, (ay * dt) * 0.5);
+            kahan_add(vz, cvz, (az * dt) * 0.5);
+            kahan_add(x, cpx, vx * dt);
+            kahan_add(y, cpy, vy * dt);
+            kahan_add(z, cpz, vz * dt);
+            b.ocvx[i] = cvx; b.ocvy[i] = cvy; b.ocvz[i] = cvz;
+            b.ocpx[i] = cpx; b.ocpy[i] = cpy; b.ocpz[i] = cpz;
+        }
+        b.opx[i] = x; b.opy[i] = y; b.opz[i] = z;
+        b.ovx[i] = vx; b.ovy[i] = vy; b.ovz[i] = vz;
+        b.om[i] = b.m[j];
+    }
+}
+
+// Second kick — ComputeVelocityAndPosition, Body.hpp:317-324: v = v + (a*dt)*0.5.
+template <bool kComp>
+__global__ __launch_bounds__(kBlock) void k_kick(double* __restrict__ vx, double* __restrict__ vy,
+                                                 double* __restrict__ vz, const double* __restrict__ ax,
+                                                 const double* __restrict__ ay, const double* __restrict__ az,
+                                                 double* __restrict__ cvx, double* __restrict__ cvy,
+                                                 double* __restrict__ cvz, size_t n, double dt) {
+    for (size_t i = blockIdx.x * (size_t)kBlock + threadIdx.x; i < n; i += (size_t)gridDim.x * kBlock) {
+        if (!kComp) {
+            vx[i] = vx[i] + (ax[i] * dt) * 0.5;
+            vy[i] = vy[i] + (ay[i] * dt) * 0.5;
+            vz[i] = vz[i] + (az[i] * dt) * 0.5;
+        } else {
+            double v0 = vx[i], v1 = vy[i], v2 = vz[i];
+            double c0 = cvx[i], c1 = cvy[i], c2 = cvz[i];
+            kahan_add(v0, c0, (ax[i] * dt) * 0.5);
+            kahan_add(v1, c1, (ay[i] * dt) * 0.5);
+            kahan_add(v2, c2, (az[i] * dt) * 0.5);
+            vx[i] = v0; vy[i] = v1; vz[i] = v2;
+            cvx[i] = c0; cvy[i] = c1; cvz[i] = c2;
+        }
+    }
+}
+
+// Tree-order copies of positions and masses for the walk.
+__global__ __launch_bounds__(kBlock) void k_tree_gather(const double* __restrict__ px, const double* __restrict__ py,
+                                                        const double* __restrict__ pz, const double* __restrict__ m,
+                                                        const uint32_t* __restrict__ perm, size_t n,
+                                                        double* __restrict__ tx, double* __restrict__ ty,
+                                                        double* __restrict__ tz, double* __restrict__ tm) {
+    for (size_t i = blockIdx.x * (size_t)kBlock + threadIdx.x; i < n; i += (size_t)gridDim.x * kBlock) {
+        const uint32_t j = perm[i];
+        tx[i] = px[j];
+        ty[i] = py[j];
+        tz[i] = pz[j];
+        tm[i] = m[j];
+    }
+}
+
+}  // namespace
+
+hipError_t launch_bounds_mean(const double* px, const double* py, const double* pz, size_t n, double* out4,
+                              hipStream_t s) {
+    hipLaunchKernelGGL(k_bounds_mean, dim3(1), dim3(kBlock), 0, s, px, py, pz, n, out4);
+    return hipGetLastError();
+}
+
+hipError_t launch_bounds_size(const double* px, const double* py, const double* pz, size_t n, double* out4,
+                              hipStream_t s) {
+    hipLaunchKernelGGL(k_bounds_size, dim3(grid_for(n, kBlock * 8, 2048)), dim3(kBlock), 0, s, px, py, pz, n, out4);
+    return hipGetLastError();
+}
+
+hipError_t launch_morton(const double* px, const double* py, const double* pz, size_t n, const double* b4,
+                         uint64_t* keys, uint32_t* vals, hipStream_t s) {
+    hipLaunchKernelGGL(k_morton, dim3(grid_for(n)), dim3(kBlock), 0, s, px, py, pz, n, b4, keys, vals);
+    return hipGetLastError();
+}
+
+hipError_t launch_treekeys(const double* px, const double* py, const double* pz, size_t n, const double* b4,
+                           uint64_t* keys, uint32_t* vals, hipStream_t s) {
+    hipLaunchKernelGGL(k_treekeys, dim3(grid_for(n)), dim3(kBlock), 0, s, px, py, pz, n, b4, keys, vals);
+    return hipGetLastError();
+}
+
+hipError_t launch_iota(uint32_t* vals, size_t n, hipStream_t s) {
+    hipLaunchKernelGGL(k_iota, dim3(grid_for(n)), dim3(kBlock), 0, s, vals, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_gather(double* const* src, double* const* dst, int narr, const uint32_t* perm, size_t n,
+                         hipStream_t s) {
+    GatherArgs g{};
+    g.narr = narr;
+    for (int a = 0; a < narr; ++a) {
+        g.src[a] = src[a];
+        g.dst[a] = dst[a];
+    }
+    hipLaunchKernelGGL(k_gather, dim3(grid_for(n)), dim3(kBlock), 0, s, g, perm, n);
+    return hipGetLastError();
+}
+
+static BodyArrays body_arrays(bhx_ctx* c) {
+    BodyArrays b{};
+    b.px = c->px; b.py = c->py; b.pz = c->pz;
+    b.vx = c->vx; b.vy = c->vy; b.vz = c->vz;
+    b.m = c->m; b.ax = c->ax; b.ay = c->ay; b.az = c->az;
+    b.cpx = c->cpx; b.cpy = c->cpy; b.cpz = c->cpz;
+    b.cvx = c->cvx; b.cvy = c->cvy; b.cvz = c->cvz;
+    b.opx = c->alt[0]; b.opy = c->alt[1]; b.opz = c->alt[2];
+    b.ovx = c->alt[3]; b.ovy = c->alt[4]; b.ovz = c->alt[5];
+    b.om = c->alt[6];
+    b.ocpx = c->alt[7]; b.ocpy = c->alt[8]; b.ocpz = c->alt[9];
+    b.ocvx = c->alt[10]; b.ocvy = c->alt[11]; b.ocvz = c->alt[12];
+    return b;
+}
+
+static void swap_alt(bhx_ctx* c, bool comp) {
+    double** cur[13] = {&c->px, &c->py, &c->pz, &c->vx, &c->vy, &c->vz, &c->m,
+                        &c->cpx, &c->cpy, &c->cpz, &c->cvx, &c->cvy, &c->cvz};
+    const int na = comp ? 13 : 7;
+    for (int a = 0; a < na; ++a) {
+        double* t = *cur[a];
+        *cur[a] = c->alt[a];
+        c->alt[a] = t;
+    }
+}
+
+hipError_t launch_permute_kick_drift(bhx_ctx* c, const uint32_t* perm, double dt, hipStream_t s) {
+    BodyArrays b = body_arrays(c);
+    const bool comp = c->cfg.compensated != 0;
+    if (comp)
+        hipLaunchKernelGGL(k_permute_kick_drift<true>, dim3(grid_for(c->n)), dim3(kBlock), 0, s, b, perm, c->n, dt,
+                           c->cfg.permute_acc);
+    else
+        hipLaunchKernelGGL(k_permute_kick_drift<false>, dim3(grid_for(c->n)), dim3(kBlock), 0, s, b, perm, c->n, dt,
+                           c->cfg.permute_acc);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) swap_alt(c, comp);
+    return e;
+}
+
+hipError_t launch_kick_drift(bhx_ctx* c, double dt, hipStream_t s) {
+    return launch_permute_kick_drift(c, nullptr, dt, s);
+}
+
+hipError_t launch_kick(bhx_ctx* c, double dt, hipStream_t s) {
+    if (c->cfg.compensated)
+        hipLaunchKernelGGL(k_kick<true>, dim3(grid_for(c->n)), dim3(kBlock), 0, s, c->vx, c->vy, c->vz, c->ax, c->ay,
+                           c->az, c->cvx, c->cvy, c->cvz, c->n, dt);
+    else
+        hipLaunchKernelGGL(k_kick<false>, dim3(grid_for(c->n)), dim3(kBlock), 0, s, c->vx, c->vy, c->vz, c->ax, c->ay,
+                           c->az, c->cvx, c->cvy, c->cvz, c->n, dt);
+    return hipGetLastError();
+}
+
+hipError_t launch_tree_gather(bhx_ctx* c, hipStream_t s) {
+    hipLaunchKernelGGL(k_tree_gather, dim3(grid_for(c->n)), dim3(kBlock), 0, s, c->px, c->py, c->pz, c->m, c->tperm,
+                       c->n, c->tx, c->ty, c->tz, c->tm);
+    return hipGetLastError();
+}
+
+}  // namespace bhx

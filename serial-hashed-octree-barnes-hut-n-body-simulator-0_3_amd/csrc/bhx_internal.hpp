@@ -1,0 +1,157 @@
+// bhx_internal.hpp — shared declarations of the engine's translation units.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+#include <string>
+
+#include "../../include/bhx.h"
+
+namespace bhx {
+
+constexpr int kMaxLevel = 21;   // 1 sentinel bit + 21 x 3 path bits = 64-bit key
+constexpr int kMaxLeafCap = 64;
+
+// Walk record of one node, 64 bytes so one wave-uniform node fetch is a
+// single scalar load burst.  Pre-order layout: the first child of an internal
+// node is the next record, `skip` is the first record outside its subtree.
+struct alignas(64) WalkNode {
+    double cx, cy, cz;      // centre of mass (barycentre)
+    double mass;
+    double size2;           // nodeBounds.size * nodeBounds.size (FP64, as the MAC forms it)
+    uint32_t skip;
+    uint32_t start;         // first body in tree order
+    uint32_t count;         // bodies in the cell
+    uint32_t level_leaf;    // level | (leaf << 8)
+    float size2f;           // size^2 rounded to FP32 (FP32 opening test)
+    float massf;
+};
+static_assert(sizeof(WalkNode) == 64, "walk node is one 64-byte record");
+
+struct DeviceTree {
+    size_t cap = 0;          // node capacity
+    uint64_t* key = nullptr;
+    uint32_t* start = nullptr;
+    uint32_t* count = nullptr;
+    uint32_t* skip = nullptr;
+    uint32_t* parent = nullptr;
+    uint32_t* child_byte = nullptr;
+    uint32_t* level = nullptr;
+    uint32_t* first = nullptr;       // min array index among members
+    double* center = nullptr;        // 3 per node
+    double* com = nullptr;           // 3 per node
+    double* mass = nullptr;
+    double* quad = nullptr;          // 6 per node
+    float* quadf = nullptr;          // 6 per node, FP32 copy for the walk
+    WalkNode* walk = nullptr;
+    // hash table
+    size_t hcap = 0;
+    uint64_t* hkey = nullptr;
+    uint32_t* hval = nullptr;
+};
+
+struct Counters {          // small device-side status block
+    uint32_t total_nodes;
+    uint32_t unsplit;
+    uint32_t hash_fail;
+    uint32_t sort_fail;
+    uint32_t nonfinite;
+    uint32_t pad[11];
+};
+
+}  // namespace bhx
+
+struct bhx_ctx {
+    bhx_config cfg;
+    int device = 0;
+    hipStream_t s_main = nullptr;
+    hipStream_t s_side = nullptr;
+    hipEvent_t ev_drift = nullptr;
+    hipEvent_t ev_bounds = nullptr;
+    hipEvent_t ev[12] = {};
+    bool profiling = false;
+    bhx_timing timing{};
+    std::string err;
+
+    size_t cap = 0;
+    size_t n = 0;
+    // bodies, array order, SoA FP64; *_alt are the ping-pong targets of a gather
+    double *px = nullptr, *py = nullptr, *pz = nullptr;
+    double *vx = nullptr, *vy = nullptr, *vz = nullptr;
+    double *m = nullptr;
+    double *ax = nullptr, *ay = nullptr, *az = nullptr;
+    double *cpx = nullptr, *cpy = nullptr, *cpz = nullptr;   // compensation terms
+    double *cvx = nullptr, *cvy = nullptr, *cvz = nullptr;
+    double *alt[16] = {};                     // gather targets: pos 0-2, vel 3-5, m 6, comp 7-12, acc 13-15
+    // sort buffers
+    uint64_t *keys = nullptr, *keys_alt = nullptr;
+    uint32_t *vals = nullptr, *vals_alt = nullptr;
+    uint64_t *mkeys = nullptr;                 // Morton keys of the last bhx_morton_keys
+    // tree order copies for the walk
+    double *tx = nullptr, *ty = nullptr, *tz = nullptr, *tm = nullptr;
+    uint64_t* tkeys = nullptr;                 // sorted tree path keys
+    uint32_t* tperm = nullptr;                 // tree order -> array index
+    // build scratch
+    uint32_t *nn = nullptr, *noff = nullptr, *nlo = nullptr;
+    uint32_t* scan_tmp = nullptr;
+    // sort scratch
+    uint32_t* sort_hist = nullptr;             // 8 x 256
+    uint32_t* sort_status = nullptr;           // 8 x tiles x 256
+    uint32_t* sort_tile_ctr = nullptr;         // 8
+    size_t sort_tiles = 0;
+    // walk scratch
+    uint32_t* wcount = nullptr;                // interaction counts per body (array order)
+    // bounds: two slots of (cx, cy, cz, size)
+    double* bounds = nullptr;
+    int bslot = 0;
+    bool bounds_valid = false;
+    // staging
+    double* stage = nullptr;                   // 3 * cap doubles
+    bhx::Counters* ctr = nullptr;
+    bhx::Counters* ctr_host = nullptr;         // pinned
+    bhx::DeviceTree tree;
+    size_t n_nodes = 0;
+    bool tree_valid = false;
+    double tree_center[3] = {0, 0, 0};
+    double tree_size = 0;
+};
+
+namespace bhx {
+
+// --- launchers implemented in the .hip files (all enqueue on `s`) ----------
+hipError_t launch_bounds_mean(const double* px, const double* py, const double* pz, size_t n,
+                              double* out4, hipStream_t s);
+hipError_t launch_bounds_size(const double* px, const double* py, const double* pz, size_t n,
+                              double* out4, hipStream_t s);
+hipError_t launch_morton(const double* px, const double* py, const double* pz, size_t n,
+                         const double* bounds4, uint64_t* keys, uint32_t* vals, hipStream_t s);
+hipError_t launch_treekeys(const double* px, const double* py, const double* pz, size_t n,
+                           const double* bounds4, uint64_t* keys, uint32_t* vals, hipStream_t s);
+hipError_t launch_iota(uint32_t* vals, size_t n, hipStream_t s);
+
+size_t sort_tiles_for(size_t n);
+hipError_t radix_sort_pairs(uint64_t* keys, uint32_t* vals, uint64_t* keys_alt, uint32_t* vals_alt,
+                            size_t n, uint32_t* hist, uint32_t* status, uint32_t* tile_ctr,
+                            Counters* ctr, hipStream_t s);
+
+hipError_t launch_gather(double* const* src, double* const* dst, int narr, const uint32_t* perm,
+                         size_t n, hipStream_t s);
+hipError_t launch_permute_kick_drift(bhx_ctx* c, const uint32_t* perm, double dt, hipStream_t s);
+hipError_t launch_kick_drift(bhx_ctx* c, double dt, hipStream_t s);
+hipError_t launch_kick(bhx_ctx* c, double dt, hipStream_t s);
+hipError_t launch_tree_gather(bhx_ctx* c, hipStream_t s);
+
+hipError_t build_tree(bhx_ctx* c, hipStream_t s, size_t* total_out);
+hipError_t launch_lookup(const DeviceTree& t, const uint64_t* keys, size_t n, int64_t* ids, hipStream_t s);
+
+hipError_t launch_walk(bhx_ctx* c, hipStream_t s);
+hipError_t launch_direct(bhx_ctx* c, const int64_t* targets, size_t nt, double eps, double* acc,
+                         double* pot, hipStream_t s);
+hipError_t launch_diag(bhx_ctx* c, double* out7, hipStream_t s);
+
+hipError_t scan_exclusive_u32(const uint32_t* in, uint32_t* out, size_t n, uint32_t* tmp,
+                              uint32_t* total_dev, hipStream_t s);
+
+}  // namespace bhx

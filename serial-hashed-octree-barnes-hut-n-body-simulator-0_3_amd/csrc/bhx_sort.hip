@@ -1,0 +1,239 @@
+// bhx_sort.hip — stable LSD radix sort of (uint64 key, uint32 value) pairs.
+//
+// Replaces ThreePassRadixSortMortonKeys / RadixSortPass (MortonKeys.hpp:
+// 373-470).  The reference runs nine stable byte passes at overlapping bit
+// offsets, which amounts to a stable full sort by the 64-bit key; the same
+// order comes out of eight stable passes over disjoint bytes.
+//
+// Each pass is a single "onesweep" kernel (Adinets & Merrill 2022): a
+// workgroup ranks a tile of 4096 keys in LDS, publishes its per-digit counts,
+// and finds its global offsets by decoupled look-back (Merrill & Garland
+// 2016) over the tiles before it.  Tile indices come from an atomic counter
+// in launch order, so every tile a workgroup waits on is already running or
+// done.  The spin on a predecessor's status is bounded; running out of spins
+// raises an error flag instead of hanging.
+#include "bhx_internal.hpp"
+
+namespace bhx {
+namespace {
+
+constexpr int kBlock = 256;
+constexpr int kItems = 16;
+constexpr int kTile = kBlock * kItems;        // 4096 keys per tile
+constexpr int kPasses = 8;
+constexpr int kBins = 256;
+constexpr uint32_t kFlagA = 1u << 30;         // tile aggregate available
+constexpr uint32_t kFlagP = 2u << 30;         // inclusive prefix available
+constexpr uint32_t kCountMask = (1u << 30) - 1;
+constexpr uint32_t kSpinLimit = 1u << 22;
+
+__device__ __forceinline__ void st_status(uint32_t* p, uint32_t v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ uint32_t ld_status(uint32_t* p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Exclusive scan of one value per thread over a 256-thread block.
+__device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t v, uint32_t* s_wsum, uint32_t* total) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    uint32_t x = v;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        const uint32_t y = __shfl_up(x, off, 64);
+        if (lane >= off) x += y;
+    }
+    if (lane == 63) s_wsum[wave] = x;
+    __syncthreads();
+    uint32_t wbase = 0, tot = 0;
+#pragma unroll
+    for (int w = 0; w < kBlock / 64; ++w) {
+        const uint32_t ws = s_wsum[w];
+        if (w < wave) wbase += ws;
+        tot += ws;
+    }
+    if (total) *total = tot;
+    __syncthreads();
+    return wbase + x - v;
+}
+
+__global__ void k_sort_init(uint32_t* __restrict__ hist, uint32_t* __restrict__ tile_ctr,
+                            uint32_t* __restrict__ status, size_t nstatus) {
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < nstatus; i += stride) status[i] = 0;
+    if (blockIdx.x == 0) {
+        for (int j = threadIdx.x; j < kPasses * kBins; j += blockDim.x) hist[j] = 0;
+        if (threadIdx.x < kPasses) tile_ctr[threadIdx.x] = 0;
+    }
+}
+
+// Digit histograms of all eight passes in one read of the keys.
+__global__ __launch_bounds__(kBlock) void k_sort_hist(const uint64_t* __restrict__ keys, size_t n,
+                                                      uint32_t* __restrict__ hist) {
+    __shared__ uint32_t h[kPasses * kBins];
+    for (int j = threadIdx.x; j < kPasses * kBins; j += kBlock) h[j] = 0;
+    __syncthreads();
+    for (size_t i = blockIdx.x * (size_t)kBlock + threadIdx.x; i < n; i += (size_t)gridDim.x * kBlock) {
+        const uint64_t k = keys[i];
+#pragma unroll
+        for (int p = 0; p < kPasses; ++p) atomicAdd(&h[p * kBins + ((k >> (8 * p)) & 0xFF)], 1u);
+    }
+    __syncthreads();
+    for (int j = threadIdx.x; j < kPasses * kBins; j += kBlock)
+        if (h[j]) atomicAdd(&hist[j], h[j]);
+}
+
+// Exclusive scan of each pass's histogram: start of every digit in the output.
+__global__ __launch_bounds__(kBlock) void k_sort_scan(uint32_t* __restrict__ hist) {
+    __shared__ uint32_t s_wsum[kBlock / 64];
+    for (int p = 0; p < kPasses; ++p) {
+        const uint32_t v = hist[p * kBins + threadIdx.x];
+        const uint32_t e = block_exclusive_scan(v, s_wsum, nullptr);
+        hist[p * kBins + threadIdx.x] = e;
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_sort_pass(const uint64_t* __restrict__ kin,
+                                                      const uint32_t* __restrict__ vin,
+                                                      uint64_t* __restrict__ kout,
+                                                      uint32_t* __restrict__ vout, size_t n, int pass,
+                                                      const uint32_t* __restrict__ goff,
+                                                      uint32_t* __restrict__ status,
+                                                      uint32_t* __restrict__ tile_ctr,
+                                                      Counters* __restrict__ ctr) {
+    __shared__ uint32_t s_tile;
+    __shared__ uint32_t s_wcnt[kBlock / 64][kBins];
+    __shared__ uint32_t s_tstart[kBins];
+    __shared__ uint32_t s_gbase[kBins];
+    __shared__ uint32_t s_wsum[kBlock / 64];
+    __shared__ uint64_t s_keys[kTile];
+    __shared__ uint32_t s_vals[kTile];
+
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    if (tid == 0) s_tile = atomicAdd(tile_ctr + pass, 1u);
+    for (int j = tid; j < (kBlock / 64) * kBins; j += kBlock) (&s_wcnt[0][0])[j] = 0;
+    __syncthreads();
+    const uint32_t tile = s_tile;
+    const size_t base = (size_t)tile * kTile;
+    const int shift = 8 * pass;
+
+    // Warp-striped load: item j of a lane sits at base + wave*1024 + j*64 + lane,
+    // so ranking items j = 0..15 in turn, lanes in order, follows index order.
+    uint64_t key[kItems];
+    uint32_t val[kItems];
+    uint32_t rank[kItems];
+#pragma unroll
+    for (int j = 0; j < kItems; ++j) {
+        const size_t idx = base + (size_t)wave * (64 * kItems) + (size_t)j * 64 + lane;
+        if (idx < n) {
+            key[j] = kin[idx];
+            val[j] = vin[idx];
+        } else {
+            key[j] = ~0ull;
+            val[j] = 0;
+        }
+    }
+    const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+#pragma unroll
+    for (int j = 0; j < kItems; ++j) {
+        const size_t idx = base + (size_t)wave * (64 * kItems) + (size_t)j * 64 + lane;
+        const bool valid = idx < n;
+        const uint32_t d = (uint32_t)(key[j] >> shift) & 0xFF;
+        uint64_t peers = __ballot(valid);
+#pragma unroll
+        for (int b = 0; b < 8; ++b) {
+            const bool bit = (d >> b) & 1;
+            const uint64_t bb = __ballot(bit);
+            peers &= bit ? bb : ~bb;
+        }
+        const uint32_t before = s_wcnt[wave][d];
+        rank[j] = before + (uint32_t)__popcll(peers & lt_mask);
+        if (valid && lane == __ffsll((unsigned long long)peers) - 1) s_wcnt[wave][d] = before + (uint32_t)__popcll(peers);
+    }
+    __syncthreads();
+
+    // Per digit (thread = digit): counts of the four waves, tile total.
+    const uint32_t c0 = s_wcnt[0][tid], c1 = s_wcnt[1][tid], c2 = s_wcnt[2][tid], c3 = s_wcnt[3][tid];
+    const uint32_t tot = c0 + c1 + c2 + c3;
+    uint32_t* st = status + (size_t)tile * kBins;
+    st_status(st + tid, (tile == 0 ? kFlagP : kFlagA) | tot);
+    const uint32_t tstart = block_exclusive_scan(tot, s_wsum, nullptr);
+
+    uint32_t excl = 0;
+    if (tile > 0) {
+        int64_t t = (int64_t)tile - 1;
+        uint32_t spins = 0;
+        for (;;) {
+            const uint32_t v = ld_status(status + (size_t)t * kBins + tid);
+            const uint32_t flag = v & ~kCountMask;
+            if (flag == 0) {
+                if (++spins > kSpinLimit) {
+                    ctr->sort_fail = 1;
+                    break;
+                }
+                __builtin_amdgcn_s_sleep(1);
+                continue;
+            }
+            excl += v & kCountMask;
+            if (flag == kFlagP || t == 0) break;
+            --t;
+        }
+        st_status(st + tid, kFlagP | (excl + tot));
+    }
+    s_tstart[tid] = tstart;
+    s_gbase[tid] = goff[pass * kBins + tid] + excl;
+    s_wcnt[0][tid] = 0;
+    s_wcnt[1][tid] = c0;
+    s_wcnt[2][tid] = c0 + c1;
+    s_wcnt[3][tid] = c0 + c1 + c2;
+    __syncthreads();
+
+    // Tile-local sorted order in LDS, then coalesced writes per digit run.
+#pragma unroll
+    for (int j = 0; j < kItems; ++j) {
+        const size_t idx = base + (size_t)wave * (64 * kItems) + (size_t)j * 64 + lane;
+        if (idx < n) {
+            const uint32_t d = (uint32_t)(key[j] >> shift) & 0xFF;
+            const uint32_t q = s_tstart[d] + s_wcnt[wave][d] + rank[j];
+            s_keys[q] = key[j];
+            s_vals[q] = val[j];
+        }
+    }
+    __syncthreads();
+    const uint32_t nvalid = (uint32_t)((n - base) < (size_t)kTile ? (n - base) : (size_t)kTile);
+    for (uint32_t q = tid; q < nvalid; q += kBlock) {
+        const uint64_t k = s_keys[q];
+        const uint32_t d = (uint32_t)(k >> shift) & 0xFF;
+        const uint32_t out = s_gbase[d] + (q - s_tstart[d]);
+        kout[out] = k;
+        vout[out] = s_vals[q];
+    }
+}
+
+}  // namespace
+
+size_t sort_tiles_for(size_t n) { return (n + kTile - 1) / kTile; }
+
+hipError_t radix_sort_pairs(uint64_t* keys, uint32_t* vals, uint64_t* keys_alt, uint32_t* vals_alt, size_t n,
+                            uint32_t* hist, uint32_t* status, uint32_t* tile_ctr, Counters* ctr, hipStream_t s) {
+    if (n <= 1) return hipSuccess;
+    const size_t tiles = sort_tiles_for(n);
+    const size_t nstatus = (size_t)kPasses * tiles * kBins;
+    unsigned ig = (unsigned)((nstatus + kBlock - 1) / kBlock);
+    if (ig > 4096) ig = 4096;
+    hipLaunchKernelGGL(k_sort_init, dim3(ig), dim3(kBlock), 0, s, hist, tile_ctr, status, nstatus);
+    unsigned hg = (unsigned)((n + kBlock * 16 - 1) / (kBlock * 16));
+    if (hg > 1024) hg = 1024;
+    if (hg == 0) hg = 1;
+    hipLaunchKernelGGL(k_sort_hist, dim3(hg), dim3(kBlock), 0, s, keys, n, hist);
+    hipLaunchKernelGGL(k_sort_scan, dim3(1), dim3(kBlock), 0, s, hist);
+    for (int p = 0; p < kPasses; ++p) {
+        const bool even = (p & 1) == 0;
+        hipLaunchKernelGGL(k_sort_pass, dim3((unsigned)tiles), dim3(kBlock), 0, s, even ? keys : keys_alt,
+                           even ? vals : vals_alt, even ? keys_alt : keys, even ? vals_alt : vals, n, p, hist,
+                           status + (size_t)p * tiles * kBins, tile_ctr, ctr);
+    }
+    return hipGetLastError();
+}
+
+}  // namespace bhx

@@ -1,0 +1,502 @@
+// bhx_tree.hip — hashed-octree construction on the GPU.
+//
+// Replaces buildLinearHashedOctreeInPlace and the insertion routines
+// (HashedOctree.hpp:211-244, HashedOctree.cpp:450-711) plus the node hash map
+// (LinearHashedOctree::nodes, insertHOTNode / lookUpNode, HashedOctree.cpp:
+// 111-115,152-205).
+//
+// The reference inserts bodies one at a time.  Its result does not depend on
+// the insertion order: a cell exists iff it is the root, or its parent holds
+// more than `leaf_capacity` bodies (1 in the reference) and it holds at least
+// one.  Given every body's 21-level path key sorted ascending, the cells of
+// level L are the runs of equal L-prefixes, so the whole node set follows
+// from adjacent key comparisons:
+//   lcp(i)  = levels shared by keys i-1 and i,
+//   D(i)    = deepest level at which the run containing body i holds more
+//             than B bodies = max over the windows [j, j+B] containing i of
+//             the levels shared by keys j and j+B,
+//   body i starts the cells of levels lcp(i)+1 .. min(D(i)+1, 21).
+// Numbering the cells by (first body, level) gives a depth-first pre-order in
+// which a cell's first child is the next record and its subtree is a
+// contiguous id range, closed by `skip` = first cell starting at the cell's
+// end.  Moments are then combined level by level from the leaves up.
+#include "bhx_internal.hpp"
+
+namespace bhx {
+namespace {
+
+constexpr int kBlock = 256;
+constexpr uint32_t kNone = 0xFFFFFFFFu;
+
+inline unsigned grid_for(size_t n, int per_block = kBlock, unsigned cap = 65535u * 16u) {
+    size_t g = (n + per_block - 1) / per_block;
+    if (g == 0) g = 1;
+    return (unsigned)(g < cap ? g : cap);
+}
+
+// Levels (below the root) shared by two path keys; 21 when equal.
+__device__ __forceinline__ int shared_levels(uint64_t a, uint64_t b) {
+    const uint64_t x = a ^ b;
+    if (x == 0) return kMaxLevel;
+    return (__clzll((long long)x) - 1) / 3;
+}
+
+__device__ __forceinline__ uint64_t prefix(uint64_t k, int L) { return k >> (63 - 3 * L); }
+
+// ---------------------------------------------------------------- scan (u32)
+
+constexpr int kScanItems = 8;
+constexpr int kScanTile = kBlock * kScanItems;
+
+__device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t* s_wsum, uint32_t* total) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    uint32_t x = v;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        const uint32_t y = __shfl_up(x, off, 64);
+        if (lane >= off) x += y;
+    }
+    if (lane == 63) s_wsum[wave] = x;
+    __syncthreads();
+    uint32_t wbase = 0, tot = 0;
+#pragma unroll
+    for (int w = 0; w < kBlock / 64; ++w) {
+        const uint32_t ws = s_wsum[w];
+        if (w < wave) wbase += ws;
+        tot += ws;
+    }
+    if (total) *total = tot;
+    __syncthreads();
+    return wbase + x - v;
+}
+
+__global__ __launch_bounds__(kBlock) void k_scan_reduce(const uint32_t* __restrict__ in, size_t n,
+                                                        uint32_t* __restrict__ part) {
+    __shared__ uint32_t s_wsum[kBlock / 64];
+    const size_t base = (size_t)blockIdx.x * kScanTile;
+    uint32_t sum = 0;
+    for (int j = 0; j < kScanItems; ++j) {
+        const size_t i = base + (size_t)j * kBlock + threadIdx.x;
+        if (i < n) sum += in[i];
+    }
+    uint32_t tot;
+    block_excl_scan(sum, s_wsum, &tot);
+    if (threadIdx.x == 0) part[blockIdx.x] = tot;
+}
+
+__global__ __launch_bounds__(kBlock) void k_scan_top(uint32_t* __restrict__ part, size_t nparts,
+                                                     uint32_t* __restrict__ total) {
+    __shared__ uint32_t s_wsum[kBlock / 64];
+    uint32_t carry = 0;
+    for (size_t b = 0; b < nparts; b += kBlock) {
+        const size_t i = b + threadIdx.x;
+        const uint32_t v = i < nparts ? part[i] : 0;
+        uint32_t tot;
+        const uint32_t e = block_excl_scan(v, s_wsum, &tot);
+        if (i < nparts) part[i] = carry + e;
+        carry += tot;
+    }
+    if (threadIdx.x == 0) *total = carry;
+}
+
+__global__ __launch_bounds__(kBlock) void k_scan_down(const uint32_t* __restrict__ in, size_t n,
+                                                      const uint32_t* __restrict__ part,
+                                                      uint32_t* __restrict__ out) {
+    __shared__ uint32_t s_wsum[kBlock / 64];
+    const size_t base = (size_t)blockIdx.x * kScanTile;
+    // blocked arrangement: thread t owns items [t*kScanItems, (t+1)*kScanItems)
+    uint32_t v[kScanItems];
+    uint32_t sum = 0;
+    for (int j = 0; j < kScanItems; ++j) {
+        const size_t i = base + (size_t)threadIdx.x * kScanItems + j;
+        v[j] = i < n ? in[i] : 0;
+        sum += v[j];
+    }
+    uint32_t run = part[blockIdx.x] + block_excl_scan(sum, s_wsum, nullptr);
+    for (int j = 0; j < kScanItems; ++j) {
+        const size_t i = base + (size_t)threadIdx.x * kScanItems + j;
+        if (i < n) out[i] = run;
+        run += v[j];
+    }
+}
+
+// ---------------------------------------------------------------- node counting
+
+// Per body: how many cells start at it (levels lo..hi) — the counting half of
+// the insertion loop, HashedOctree.cpp:474-505.
+__global__ __launch_bounds__(kBlock) void k_count(const uint64_t* __restrict__ keys, uint32_t n, int B,
+                                                  uint32_t* __restrict__ nn, uint32_t* __restrict__ nlo) {
+    __shared__ uint64_t s_key[kBlock + 2 * kMaxLeafCap];
+    __shared__ int8_t s_w[kBlock + kMaxLeafCap];
+    const int64_t b0 = (int64_t)blockIdx.x * kBlock;
+    // keys [b0 - B, b0 + kBlock + B)
+    for (int t = threadIdx.x; t < kBlock + 2 * B; t += kBlock) {
+        const int64_t g = b0 - B + t;
+        s_key[t] = (g >= 0 && g < (int64_t)n) ? keys[g] : 0;
+    }
+    __syncthreads();
+    // window levels W(j) for j in [b0 - B, b0 + kBlock): keys j and j+B
+    for (int t = threadIdx.x; t < kBlock + B; t += kBlock) {
+        const int64_t j = b0 - B + t;
+        int w = -1;
+        if (j >= 0 && j + B < (int64_t)n) w = shared_levels(s_key[t], s_key[t + B]);
+        s_w[t] = (int8_t)w;
+    }
+    __syncthreads();
+    const int64_t i = b0 + threadIdx.x;
+    if (i >= (int64_t)n) return;
+    const int li = threadIdx.x + B;   // local index of body i in s_key
+    const int lcp = (i == 0) ? -1 : shared_levels(s_key[li - 1], s_key[li]);
+    int D = -1;
+    for (int t = 0; t <= B; ++t) {     // windows starting at i-B .. i
+        const int w = s_w[threadIdx.x + t];
+        if (w > D) D = w;
+    }
+    const int lo = (lcp + 1 > 1) ? lcp + 1 : 1;
+    const int hi = (D + 1 < kMaxLevel) ? D + 1 : kMaxLevel;
+    nn[i] = hi >= lo ? (uint32_t)(hi - lo + 1) : 0u;
+    nlo[i] = (uint32_t)lo;
+}
+
+// End of the level-L run starting at i: first j > i with a different L-prefix.
+__device__ uint32_t run_end(const uint64_t* __restrict__ keys, uint32_t n, uint32_t i, int L) {
+    const uint64_t p = prefix(keys[i], L);
+    uint32_t lo = i, step = 1;          // keys[lo] has prefix p
+    uint32_t hi;
+    for (;;) {
+        const uint64_t j = (uint64_t)i + step;
+        if (j >= n) { hi = n; break; }
+        if (prefix(keys[j], L) != p) { hi = (uint32_t)j; break; }
+        lo = (uint32_t)j;
+        step <<= 1;
+    }
+    while (hi - lo > 1) {               // keys[lo] in run, keys[hi] (or n) out
+        const uint32_t mid = lo + (hi - lo) / 2;
+        if (prefix(keys[mid], L) == p) lo = mid; else hi = mid;
+    }
+    return hi;
+}
+
+// Start of the level-L run containing i: first j <= i with the same L-prefix.
+__device__ uint32_t run_start(const uint64_t* __restrict__ keys, uint32_t i, int L) {
+    const uint64_t p = prefix(keys[i], L);
+    uint32_t hi = i, step = 1;          // keys[hi] has prefix p
+    int64_t lo;
+    for (;;) {
+        const int64_t j = (int64_t)i - step;
+        if (j < 0) { lo = -1; break; }
+        if (prefix(keys[j], L) != p) { lo = j; break; }
+        hi = (uint32_t)j;
+        step <<= 1;
+    }
+    while ((int64_t)hi - lo > 1) {      // keys[lo] out (or -1), keys[hi] in
+        const int64_t mid = lo + ((int64_t)hi - lo) / 2;
+        if (prefix(keys[mid], L) == p) hi = (uint32_t)mid; else lo = mid;
+    }
+    return hi;
+}
+
+struct TreeArgs {
+    uint64_t* key;
+    uint32_t *start, *count, *skip, *parent, *child_byte, *level, *first;
+    double* center;
+    double* com;
+    double* mass;
+    double* quad;
+    float* quadf;
+    WalkNode* walk;
+};
+
+// Child cell geometry — HOTNode(parent, octant, ...) HashedNode.cpp:50-56.
+__device__ __forceinline__ void descend_center(uint64_t key, int L, const double* b4, double& cx, double& cy,
+                                               double& cz, double& s) {
+#pragma clang fp contract(off)
+    cx = b4[0]; cy = b4[1]; cz = b4[2]; s = b4[3];
+    for (int l = 1; l <= L; ++l) {
+        const unsigned o = (unsigned)(key >> (3 * (L - l))) & 7u;
+        s = s * 0.5;
+        cx = cx + (((o & 4) ? 1.0 : -1.0) * 0.5) * s;
+        cy = cy + (((o & 2) ? 1.0 : -1.0) * 0.5) * s;
+        cz = cz + (((o & 1) ? 1.0 : -1.0) * 0.5) * s;
+    }
+}
+
+// Writes every cell: key, body range, pre-order skip, parent, level, centre
+// (createChildNode / HOTNode child constructor, HashedOctree.cpp:697-711,
+// HashedNode.cpp:48-61).
+__global__ __launch_bounds__(kBlock) void k_emit(const uint64_t* __restrict__ keys, uint32_t n, int B,
+                                                 const uint32_t* __restrict__ nn, const uint32_t* __restrict__ nlo,
+                                                 const uint32_t* __restrict__ off, const uint32_t* __restrict__ total_p,
+                                                 const double* __restrict__ b4, TreeArgs t, Counters* __restrict__ ctr) {
+    const uint32_t total = *total_p + 1;   // + root
+    const size_t stride = (size_t)gridDim.x * kBlock;
+    for (size_t ii = blockIdx.x * (size_t)kBlock + threadIdx.x; ii < n; ii += stride) {
+        const uint32_t i = (uint32_t)ii;
+        if (i == 0) {
+            t.key[0] = 1;
+            t.start[0] = 0;
+            t.count[0] = n;
+            t.skip[0] = total;
+            t.parent[0] = kNone;
+            t.level[0] = 0;
+            t.center[0] = b4[0]; t.center[1] = b4[1]; t.center[2] = b4[2];
+        }
+        const uint32_t cnt = nn[i];
+        if (cnt == 0) continue;
+        const int lo = (int)nlo[i];
+        const uint32_t id0 = 1 + off[i];
+        for (int L = lo; L < lo + (int)cnt; ++L) {
+            const uint32_t id = id0 + (uint32_t)(L - lo);
+            const uint64_t k = prefix(keys[i], L);
+            const uint32_t end = run_end(keys, n, i, L);
+            uint32_t par;
+            if (L > lo) par = id - 1;
+            else if (L == 1) par = 0;
+            else {
+                const uint32_t ps = run_start(keys, i, L - 1);
+                par = 1 + off[ps] + (uint32_t)(L - 1 - (int)nlo[ps]);
+            }
+            t.key[id] = k;
+            t.start[id] = i;
+            t.count[id] = end - i;
+            t.skip[id] = end < n ? 1 + off[end] : total;
+            t.parent[id] = par;
+            t.level[id] = (uint32_t)L;
+            atomicOr(&t.child_byte[par], 1u << (unsigned)(k & 7));
+            double cx, cy, cz, s;
+            descend_center(k, L, b4, cx, cy, cz, s);
+            t.center[3 * (size_t)id] = cx;
+            t.center[3 * (size_t)id + 1] = cy;
+            t.center[3 * (size_t)id + 2] = cz;
+            if (L == kMaxLevel && (int)(end - i) > B) atomicAdd(&ctr->unsplit, 1u);
+        }
+    }
+}
+
+__global__ void k_fill_u32(uint32_t* __restrict__ p, size_t n, uint32_t v) {
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) p[i] = v;
+}
+__global__ void k_fill_u64(uint64_t* __restrict__ p, size_t n, uint64_t v) {
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) p[i] = v;
+}
+
+// ---------------------------------------------------------------- hash table
+
+__device__ __forceinline__ uint64_t mix64(uint64_t k) {
+    k ^= k >> 33;
+    k *= 0xff51afd7ed558ccdull;
+    k ^= k >> 33;
+    k *= 0xc4ceb9fe1a85ec53ull;
+    k ^= k >> 33;
+    return k;
+}
+
+// insertHOTNode — HashedOctree.cpp:152-205: open addressing with linear
+// probing in HBM; an empty slot holds key 0 (no valid key is 0: every key
+// carries the root sentinel bit).
+__global__ __launch_bounds__(kBlock) void k_hash_insert(const uint64_t* __restrict__ key, const uint32_t* __restrict__ total_p,
+                                                        uint64_t* __restrict__ hkey, uint32_t* __restrict__ hval,
+                                                        uint64_t mask, Counters* __restrict__ ctr) {
+    const uint32_t total = *total_p + 1;
+    for (size_t id = blockIdx.x * (size_t)kBlock + threadIdx.x; id < total; id += (size_t)gridDim.x * kBlock) {
+        const uint64_t k = key[id];
+        uint64_t h = mix64(k) & mask;
+        bool done = false;
+        for (uint64_t probe = 0; probe <= mask; ++probe) {
+            const unsigned long long prev = atomicCAS(reinterpret_cast<unsigned long long*>(hkey + h), 0ull,
+                                                      (unsigned long long)k);
+            if (prev == 0ull || prev == k) {
+                hval[h] = (uint32_t)id;
+                done = true;
+                break;
+            }
+            h = (h + 1) & mask;
+        }
+        if (!done) atomicAdd(&ctr->hash_fail, 1u);
+    }
+}
+
+// lookUpNode — HashedOctree.cpp:111-115.
+__global__ __launch_bounds__(kBlock) void k_hash_lookup(const uint64_t* __restrict__ hkey, const uint32_t* __restrict__ hval,
+                                                        uint64_t mask, const uint64_t* __restrict__ q, size_t n,
+                                                        int64_t* __restrict__ out) {
+    for (size_t i = blockIdx.x * (size_t)kBlock + threadIdx.x; i < n; i += (size_t)gridDim.x * kBlock) {
+        const uint64_t k = q[i];
+        int64_t r = -1;
+        if (k != 0) {
+            uint64_t h = mix64(k) & mask;
+            for (uint64_t probe = 0; probe <= mask; ++probe) {
+                const uint64_t s = hkey[h];
+                if (s == k) { r = (int64_t)hval[h]; break; }
+                if (s == 0) break;
+                h = (h + 1) & mask;
+            }
+        }
+        out[i] = r;
+    }
+}
+
+// ---------------------------------------------------------------- moments
+
+__device__ __forceinline__ void quad_add(double* q, double m, double dx, double dy, double dz) {
+    const double d2 = dx * dx + dy * dy + dz * dz;
+    q[0] += m * (3.0 * dx * dx - d2);
+    q[1] += m * (3.0 * dx * dy);
+    q[2] += m * (3.0 * dx * dz);
+    q[3] += m * (3.0 * dy * dy - d2);
+    q[4] += m * (3.0 * dy * dz);
+    q[5] += m * (3.0 * dz * dz - d2);
+}
+
+// Payload of the cells of one level, children (level L+1) already done.
+//  AS_RUN (what the reference holds after insertion, HashedOctree.cpp:480-504,
+//  536-559, 697-711; the moments pass never runs): N = count, barycentre =
+//  position of the first body in array order, mass = sum of member masses,
+//  quadrupole = 0.
+//  INTENDED (computeNodeBaryCenters' aim, HashedOctree.cpp:721-827, written
+//  from the physics): mass-weighted centre of mass and the parallel-axis
+//  quadrupole sum Q = sum_k [M_k (3 d d^T - |d|^2 I) + Q_k].
+__global__ __launch_bounds__(kBlock) void k_moments(int L, const uint32_t* __restrict__ total_p, int B, int intended,
+                                                    TreeArgs t, const double* __restrict__ b4,
+                                                    const double* __restrict__ px, const double* __restrict__ py,
+                                                    const double* __restrict__ pz, const double* __restrict__ tx,
+                                                    const double* __restrict__ ty, const double* __restrict__ tz,
+                                                    const double* __restrict__ tm, const uint32_t* __restrict__ tperm) {
+    const uint32_t total = *total_p + 1;
+    for (size_t pp = blockIdx.x * (size_t)kBlock + threadIdx.x; pp < total; pp += (size_t)gridDim.x * kBlock) {
+        const uint32_t p = (uint32_t)pp;
+        if ((int)t.level[p] != L) continue;
+        const uint32_t st = t.start[p], cnt = t.count[p], cb = t.child_byte[p];
+        const bool leaf = cb == 0;
+        double M = 0.0, sx = 0.0, sy = 0.0, sz = 0.0;
+        double q[6] = {0, 0, 0, 0, 0, 0};
+        uint32_t first = 0xFFFFFFFFu;
+        double cx, cy, cz;
+        if (leaf) {
+            for (uint32_t j = st; j < st + cnt; ++j) {
+                const double mj = tm[j];
+                M += mj;
+                sx += mj * tx[j];
+                sy += mj * ty[j];
+                sz += mj * tz[j];
+                const uint32_t a = tperm[j];
+                if (a < first) first = a;
+            }
+            if (!intended || cnt == 1) {
+                cx = px[first]; cy = py[first]; cz = pz[first];
+            } else {
+                cx = sx / M; cy = sy / M; cz = sz / M;
+            }
+            if (intended && cnt > 1)
+                for (uint32_t j = st; j < st + cnt; ++j) quad_add(q, tm[j], tx[j] - cx, ty[j] - cy, tz[j] - cz);
+        } else {
+            const uint32_t end = t.skip[p];
+            for (uint32_t c = p + 1; c < end; c = t.skip[c]) {
+                const double mc = t.mass[c];
+                M += mc;
+                sx += mc * t.com[3 * (size_t)c];
+                sy += mc * t.com[3 * (size_t)c + 1];
+                sz += mc * t.com[3 * (size_t)c + 2];
+                if (t.first[c] < first) first = t.first[c];
+            }
+            if (!intended) {
+                cx = px[first]; cy = py[first]; cz = pz[first];
+            } else {
+                cx = sx / M; cy = sy / M; cz = sz / M;
+                for (uint32_t c = p + 1; c < end; c = t.skip[c]) {
+                    const double* cc = t.com + 3 * (size_t)c;
+                    quad_add(q, t.mass[c], cc[0] - cx, cc[1] - cy, cc[2] - cz);
+                    for (int k = 0; k < 6; ++k) q[k] += t.quad[6 * (size_t)c + k];
+                }
+            }
+        }
+        t.mass[p] = M;
+        t.first[p] = first;
+        t.com[3 * (size_t)p] = cx;
+        t.com[3 * (size_t)p + 1] = cy;
+        t.com[3 * (size_t)p + 2] = cz;
+        for (int k = 0; k < 6; ++k) {
+            t.quad[6 * (size_t)p + k] = q[k];
+            t.quadf[6 * (size_t)p + k] = (float)q[k];
+        }
+        double s = b4[3];
+        for (int l = 0; l < L; ++l) s = s * 0.5;
+        WalkNode w;
+        w.cx = cx; w.cy = cy; w.cz = cz;
+        w.mass = M;
+        w.size2 = s * s;
+        w.skip = t.skip[p];
+        w.start = st;
+        w.count = cnt;
+        w.level_leaf = (uint32_t)L | (leaf ? 256u : 0u);
+        w.size2f = (float)(s * s);
+        w.massf = (float)M;
+        t.walk[p] = w;
+    }
+}
+
+TreeArgs tree_args(bhx_ctx* c) {
+    DeviceTree& d = c->tree;
+    TreeArgs t;
+    t.key = d.key; t.start = d.start; t.count = d.count; t.skip = d.skip; t.parent = d.parent;
+    t.child_byte = d.child_byte; t.level = d.level; t.first = d.first; t.center = d.center;
+    t.com = d.com; t.mass = d.mass; t.quad = d.quad; t.quadf = d.quadf; t.walk = d.walk;
+    return t;
+}
+
+}  // namespace
+
+hipError_t scan_exclusive_u32(const uint32_t* in, uint32_t* out, size_t n, uint32_t* tmp, uint32_t* total_dev,
+                              hipStream_t s) {
+    const size_t parts = (n + kScanTile - 1) / kScanTile;
+    if (parts == 0) {
+        hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, s, total_dev, (size_t)1, 0u);
+        return hipGetLastError();
+    }
+    hipLaunchKernelGGL(k_scan_reduce, dim3((unsigned)parts), dim3(kBlock), 0, s, in, n, tmp);
+    hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(kBlock), 0, s, tmp, parts, total_dev);
+    hipLaunchKernelGGL(k_scan_down, dim3((unsigned)parts), dim3(kBlock), 0, s, in, n, tmp, out);
+    return hipGetLastError();
+}
+
+// Node capacity check happens on the host after the count/scan; the caller
+// grows the tree arrays and calls again if needed.
+hipError_t build_tree(bhx_ctx* c, hipStream_t s, size_t* total_out) {
+    const uint32_t n = (uint32_t)c->n;
+    const int B = c->cfg.leaf_capacity;
+    const double* b4 = c->bounds + 4 * c->bslot;
+    uint32_t* total_dev = &c->ctr->total_nodes;
+    hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, s, &c->ctr->unsplit, (size_t)2, 0u);  // unsplit, hash_fail
+    hipLaunchKernelGGL(k_count, dim3(grid_for(n)), dim3(kBlock), 0, s, c->tkeys, n, B, c->nn, c->nlo);
+    hipError_t e = scan_exclusive_u32(c->nn, c->noff, n, c->scan_tmp, total_dev, s);
+    if (e != hipSuccess) return e;
+    // total node count decides capacity: read it back
+    e = hipMemcpyAsync(c->ctr_host, c->ctr, sizeof(Counters), hipMemcpyDeviceToHost, s);
+    if (e != hipSuccess) return e;
+    e = hipStreamSynchronize(s);
+    if (e != hipSuccess) return e;
+    const size_t total = (size_t)c->ctr_host->total_nodes + 1;
+    *total_out = total;
+    if (total > c->tree.cap) return hipSuccess;   // caller grows and retries
+    TreeArgs t = tree_args(c);
+    hipLaunchKernelGGL(k_fill_u32, dim3(grid_for(total)), dim3(kBlock), 0, s, c->tree.child_byte, total, 0u);
+    hipLaunchKernelGGL(k_emit, dim3(grid_for(n)), dim3(kBlock), 0, s, c->tkeys, n, B, c->nn, c->nlo, c->noff,
+                       total_dev, b4, t, c->ctr);
+    hipLaunchKernelGGL(k_fill_u64, dim3(grid_for(c->tree.hcap)), dim3(kBlock), 0, s, c->tree.hkey, c->tree.hcap,
+                       0ull);
+    hipLaunchKernelGGL(k_hash_insert, dim3(grid_for(total)), dim3(kBlock), 0, s, c->tree.key, total_dev,
+                       c->tree.hkey, c->tree.hval, (uint64_t)(c->tree.hcap - 1), c->ctr);
+    const int intended = c->cfg.mode == BHX_MODE_INTENDED;
+    for (int L = kMaxLevel; L >= 0; --L)
+        hipLaunchKernelGGL(k_moments, dim3(grid_for(total)), dim3(kBlock), 0, s, L, total_dev, B, intended, t, b4,
+                           c->px, c->py, c->pz, c->tx, c->ty, c->tz, c->tm, c->tperm);
+    return hipGetLastError();
+}
+
+hipError_t launch_lookup(const DeviceTree& t, const uint64_t* keys, size_t n, int64_t* ids, hipStream_t s) {
+    hipLaunchKernelGGL(k_hash_lookup, dim3(grid_for(n)), dim3(kBlock), 0, s, t.hkey, t.hval, (uint64_t)(t.hcap - 1),
+                       keys, n, ids);
+    return hipGetLastError();
+}
+
+}  // namespace bhx

@@ -1,0 +1,393 @@
+// bhx_walk.hip — the theta-criterion tree walk and gravity.
+//
+// Replaces ComputeHOTOctreeForce (HashedOctree.hpp:262-272),
+// TraverseHOTInteractionList (:336-385), LinearHashedOctree::MAC
+// (HashedOctree.cpp:872-885) and ComputeHOTForceInteractionList
+// (HashedOctree.hpp:393-459).
+//
+// One wave walks the tree for 64 consecutive bodies in tree order (spatially
+// close).  The traversal is wave-uniform and stackless over the pre-order
+// node array: every node is fetched once per wave through scalar loads, each
+// lane evaluates the reference's per-body opening test for it, and the wave
+// descends into a node when any lane opens it.  A per-lane bit per level
+// records which lanes opened the current ancestor at that level, so each
+// lane interacts with exactly the nodes of its own reference interaction
+// list (the same set, not the same order).
+#include "bhx_internal.hpp"
+
+namespace bhx {
+namespace {
+
+constexpr int kBlock = 256;
+constexpr int kWaves = kBlock / 64;
+
+typedef const __attribute__((address_space(4))) WalkNode* ConstNodes;
+typedef const __attribute__((address_space(4))) double* ConstD;
+typedef const __attribute__((address_space(4))) float* ConstF;
+
+struct WalkArgs {
+    const WalkNode* nodes;
+    const double* quad;      // 6 per node (FP64)
+    const float* quadf;      // 6 per node (FP32)
+    const double *tx, *ty, *tz, *tm;
+    const uint32_t* tperm;
+    uint32_t n;
+    uint32_t quad_on;
+    double theta2, eps2;
+    float theta2f, eps2f;
+    double *ax, *ay, *az;
+    uint32_t* counts;
+};
+
+// One term of ComputeHOTForceInteractionList (HashedOctree.hpp:411-454) in
+// the reference's FP64 operation order: D2 = |d|^2 + eps^2,
+// D1 = (1/sqrt(D2)) * (1/D2), a += (m*d)*D1, then the quadrupole correction
+// for cells holding more than one body.
+__device__ __forceinline__ void term64(double& ax, double& ay, double& az, double dx, double dy, double dz,
+                                       double d2, double m, double eps2, const double* q) {
+#pragma clang fp contract(off)
+    double D2 = d2;
+    D2 += eps2;
+    double D1 = 1.0 / sqrt(D2);
+    D2 = 1.0 / D2;
+    D1 = D1 * D2;
+    ax += m * dx * D1;
+    ay += m * dy * D1;
+    az += m * dz * D1;
+    if (q) {
+        double qx = q[0] * dx + q[1] * dy + q[2] * dz;
+        double qy = q[1] * dx + q[3] * dy + q[4] * dz;
+        double qz = q[2] * dx + q[4] * dy + q[5] * dz;
+        D1 *= D2;
+        ax -= qx * D1;
+        ay -= qy * D1;
+        az -= qz * D1;
+        qx = dx * qx + dy * qy + dz * qz;
+        qx *= 2.5;
+        D1 *= D2;
+        ax += qx * dx * D1;
+        ay += qx * dy * D1;
+        az += qx * dz * D1;
+    }
+}
+
+template <bool kQuad>
+__global__ __launch_bounds__(kBlock) void k_walk64(WalkArgs a) {
+#pragma clang fp contract(off)
+    const uint32_t group = blockIdx.x * kWaves + (threadIdx.x >> 6);
+    const uint32_t lane = threadIdx.x & 63;
+    if (group * 64 >= a.n) return;
+    const uint32_t i = group * 64 + lane;
+    const bool valid = i < a.n;
+    const uint32_t iv = valid ? i : group * 64;
+    const double x = a.tx[iv], y = a.ty[iv], z = a.tz[iv];
+    ConstNodes nodes = (ConstNodes)a.nodes;
+    ConstD tx = (ConstD)a.tx;
+    ConstD ty = (ConstD)a.ty;
+    ConstD tz = (ConstD)a.tz;
+    ConstD tm = (ConstD)a.tm;
+    double ax = 0.0, ay = 0.0, az = 0.0;
+    uint32_t cnt = 0;
+    uint32_t openbits = 1;                 // bit L: this lane opened the current level-L ancestor
+    const bool root_leaf = (nodes[0].level_leaf & 256u) != 0;
+    uint32_t p = root_leaf ? 0u : 1u;
+    const uint32_t end = nodes[0].skip;
+    while (p < end) {
+        p = __builtin_amdgcn_readfirstlane(p);
+        const double cx = nodes[p].cx, cy = nodes[p].cy, cz = nodes[p].cz;
+        const double m = nodes[p].mass, s2 = nodes[p].size2;
+        const uint32_t ll = nodes[p].level_leaf, nskip = nodes[p].skip;
+        if (nskip <= p) break;             // a subtree always ends after its root
+        const uint32_t L = ll & 255u;
+        const bool leaf = (ll & 256u) != 0;
+        const uint32_t ncount = nodes[p].count;
+        const bool active = valid && (L == 0 || ((openbits >> (L - 1)) & 1u));
+        const double dx = cx - x, dy = cy - y, dz = cz - z;
+        const double d2 = dx * dx + dy * dy + dz * dz;
+        if (leaf && ncount == 1) {
+            // a one-body leaf enters the list whether it passes (as a node) or
+            // is popped (as a leaf), unless it is the body itself
+            if (active && (cx != x || cy != y || cz != z)) {
+                term64(ax, ay, az, dx, dy, dz, d2, m, a.eps2, nullptr);
+                ++cnt;
+            }
+            p = nskip;
+            continue;
+        }
+        const bool pass = L != 0 && s2 / d2 < a.theta2;
+        const bool use = active && pass;
+        if (__ballot(use)) {
+            if (use) {
+                const double* q = nullptr;
+                if (kQuad && ncount > 1) q = a.quad + 6 * (size_t)p;
+                term64(ax, ay, az, dx, dy, dz, d2, m, a.eps2, q);
+                ++cnt;
+            }
+        }
+        const bool open = active && !pass;
+        const uint64_t om = __ballot(open);
+        if (leaf) {
+            if (om) {
+                const uint32_t st = nodes[p].start;
+                for (uint32_t j = st; j < st + ncount; ++j) {
+                    const double bx = tx[j], by = ty[j], bz = tz[j], bm = tm[j];
+                    if (open && (bx != x || by != y || bz != z)) {
+                        const double ex = bx - x, ey = by - y, ez = bz - z;
+                        term64(ax, ay, az, ex, ey, ez, ex * ex + ey * ey + ez * ez, bm, a.eps2, nullptr);
+                        ++cnt;
+                    }
+                }
+            }
+            p = nskip;
+        } else if (om) {
+            openbits = (openbits & ~(1u << L)) | ((open ? 1u : 0u) << L);
+            p = p + 1;
+        } else {
+            p = nskip;
+        }
+    }
+    if (valid) {
+        const uint32_t o = a.tperm[i];
+        a.ax[o] = ax;
+        a.ay[o] = ay;
+        a.az[o] = az;
+        if (a.counts) a.counts[o] = cnt;
+    }
+}
+
+// FP32 walk in group-relative coordinates: positions are taken relative to
+// the wave's first body (an FP64 subtraction, then a rounding to FP32), so
+// FP32 carries only the small separations.  Opening test size^2 < theta^2 d^2,
+// softened monopole with a hardware reciprocal square root, FP32 sums.
+template <bool kQuad>
+__global__ __launch_bounds__(kBlock) void k_walk32(WalkArgs a) {
+    const uint32_t group = blockIdx.x * kWaves + (threadIdx.x >> 6);
+    const uint32_t lane = threadIdx.x & 63;
+    if (group * 64 >= a.n) return;
+    const uint32_t i = group * 64 + lane;
+    const bool valid = i < a.n;
+    const uint32_t iv = valid ? i : group * 64;
+    ConstNodes nodes = (ConstNodes)a.nodes;
+    ConstD tx = (ConstD)a.tx;
+    ConstD ty = (ConstD)a.ty;
+    ConstD tz = (ConstD)a.tz;
+    ConstD tm = (ConstD)a.tm;
+    ConstF qf = (ConstF)a.quadf;
+    const uint32_t g0 = group * 64;
+    const double ox = tx[g0], oy = ty[g0], oz = tz[g0];
+    const double x = a.tx[iv], y = a.ty[iv], z = a.tz[iv];
+    const float xr = (float)(x - ox), yr = (float)(y - oy), zr = (float)(z - oz);
+    const float eps2 = a.eps2f, th2 = a.theta2f;
+    float ax = 0.f, ay = 0.f, az = 0.f;
+    uint32_t cnt = 0;
+    uint32_t openbits = 1;
+    const bool root_leaf = (nodes[0].level_leaf & 256u) != 0;
+    uint32_t p = root_leaf ? 0u : 1u;
+    const uint32_t end = nodes[0].skip;
+    while (p < end) {
+        p = __builtin_amdgcn_readfirstlane(p);
+        const double cx = nodes[p].cx, cy = nodes[p].cy, cz = nodes[p].cz;
+        const uint32_t ll = nodes[p].level_leaf, nskip = nodes[p].skip;
+        if (nskip <= p) break;             // a subtree always ends after its root
+        const uint32_t L = ll & 255u;
+        const bool leaf = (ll & 256u) != 0;
+        const uint32_t ncount = nodes[p].count;
+        const float mf = nodes[p].massf, s2 = nodes[p].size2f;
+        const bool active = valid && (L == 0 || ((openbits >> (L - 1)) & 1u));
+        const float dx = (float)(cx - ox) - xr, dy = (float)(cy - oy) - yr, dz = (float)(cz - oz) - zr;
+        const float d2 = dx * dx + dy * dy + dz * dz;
+        if (leaf && ncount == 1) {
+            if (active && (cx != x || cy != y || cz != z)) {
+                const float inv = __builtin_amdgcn_rsqf(d2 + eps2);
+                const float f = mf * (inv * inv * inv);
+                ax += f * dx; ay += f * dy; az += f * dz;
+                ++cnt;
+            }
+            p = nskip;
+            continue;
+        }
+        const bool pass = L != 0 && s2 < th2 * d2;
+        const bool use = active && pass;
+        if (__ballot(use)) {
+            if (use) {
+                const float inv = __builtin_amdgcn_rsqf(d2 + eps2);
+                const float inv2 = inv * inv;
+                float inv3 = inv * inv2;
+                const float f = mf * inv3;
+                ax += f * dx; ay += f * dy; az += f * dz;
+                if (kQuad && ncount > 1) {
+                    const float* q = (const float*)0;
+                    (void)q;
+                    const size_t qb = 6 * (size_t)p;
+                    const float q0 = qf[qb], q1 = qf[qb + 1], q2 = qf[qb + 2], q3 = qf[qb + 3], q4 = qf[qb + 4],
+                                q5 = qf[qb + 5];
+                    const float qx = q0 * dx + q1 * dy + q2 * dz;
+                    const float qy = q1 * dx + q3 * dy + q4 * dz;
+                    const float qz = q2 * dx + q4 * dy + q5 * dz;
+                    const float inv5 = inv3 * inv2;
+                    ax -= qx * inv5; ay -= qy * inv5; az -= qz * inv5;
+                    const float rqr = 2.5f * (dx * qx + dy * qy + dz * qz) * (inv5 * inv2);
+                    ax += rqr * dx; ay += rqr * dy; az += rqr * dz;
+                }
+                ++cnt;
+            }
+        }
+        const bool open = active && !pass;
+        const uint64_t om = __ballot(open);
+        if (leaf) {
+            if (om) {
+                const uint32_t st = nodes[p].start;
+                for (uint32_t j = st; j < st + ncount; ++j) {
+                    const double bx = tx[j], by = ty[j], bz = tz[j];
+                    const float bm = (float)tm[j];
+                    if (open && (bx != x || by != y || bz != z)) {
+                        const float ex = (float)(bx - ox) - xr, ey = (float)(by - oy) - yr, ez = (float)(bz - oz) - zr;
+                        const float inv = __builtin_amdgcn_rsqf(ex * ex + ey * ey + ez * ez + eps2);
+                        const float f = bm * (inv * inv * inv);
+                        ax += f * ex; ay += f * ey; az += f * ez;
+                        ++cnt;
+                    }
+                }
+            }
+            p = nskip;
+        } else if (om) {
+            openbits = (openbits & ~(1u << L)) | ((open ? 1u : 0u) << L);
+            p = p + 1;
+        } else {
+            p = nskip;
+        }
+    }
+    if (valid) {
+        const uint32_t o = a.tperm[i];
+        a.ax[o] = (double)ax;
+        a.ay[o] = (double)ay;
+        a.az[o] = (double)az;
+        if (a.counts) a.counts[o] = cnt;
+    }
+}
+
+// FP64 direct sum over all bodies (array order) for a list of targets; the
+// same softened monopole term, j != i by index.  pot = -sum m_j / sqrt(r^2+eps^2).
+__global__ __launch_bounds__(kBlock) void k_direct(const double* __restrict__ px, const double* __restrict__ py,
+                                                   const double* __restrict__ pz, const double* __restrict__ m,
+                                                   size_t n, const int64_t* __restrict__ targets, size_t nt,
+                                                   double eps2, double* __restrict__ acc, double* __restrict__ pot) {
+#pragma clang fp contract(off)
+    __shared__ double sx[kBlock], sy[kBlock], sz[kBlock], sm[kBlock];
+    const size_t k = blockIdx.x * (size_t)kBlock + threadIdx.x;
+    const bool valid = k < nt;
+    const int64_t ti = valid ? targets[k] : 0;
+    const double x = px[ti], y = py[ti], z = pz[ti];
+    double ax = 0, ay = 0, az = 0, ph = 0;
+    for (size_t b = 0; b < n; b += kBlock) {
+        const size_t j = b + threadIdx.x;
+        __syncthreads();
+        sx[threadIdx.x] = j < n ? px[j] : 0.0;
+        sy[threadIdx.x] = j < n ? py[j] : 0.0;
+        sz[threadIdx.x] = j < n ? pz[j] : 0.0;
+        sm[threadIdx.x] = j < n ? m[j] : 0.0;
+        __syncthreads();
+        const int lim = (int)((n - b) < (size_t)kBlock ? (n - b) : (size_t)kBlock);
+        for (int t = 0; t < lim; ++t) {
+            if ((int64_t)(b + t) == ti) continue;
+            const double dx = sx[t] - x, dy = sy[t] - y, dz = sz[t] - z;
+            const double d2 = dx * dx + dy * dy + dz * dz;
+            term64(ax, ay, az, dx, dy, dz, d2, sm[t], eps2, nullptr);
+            ph -= sm[t] / sqrt(d2 + eps2);
+        }
+    }
+    if (valid) {
+        acc[3 * k] = ax;
+        acc[3 * k + 1] = ay;
+        acc[3 * k + 2] = az;
+        if (pot) pot[k] = ph;
+    }
+}
+
+// FP64 kinetic energy, momentum and angular momentum, two-stage reduction in a
+// fixed order (deterministic).
+__global__ __launch_bounds__(kBlock) void k_diag_part(const double* __restrict__ px, const double* __restrict__ py,
+                                                      const double* __restrict__ pz, const double* __restrict__ vx,
+                                                      const double* __restrict__ vy, const double* __restrict__ vz,
+                                                      const double* __restrict__ m, size_t n, double* __restrict__ part) {
+    double acc[7] = {0, 0, 0, 0, 0, 0, 0};
+    for (size_t i = blockIdx.x * (size_t)kBlock + threadIdx.x; i < n; i += (size_t)gridDim.x * kBlock) {
+        const double mi = m[i], x = px[i], y = py[i], z = pz[i], u = vx[i], v = vy[i], w = vz[i];
+        acc[0] += 0.5 * mi * (u * u + v * v + w * w);
+        acc[1] += mi * u;
+        acc[2] += mi * v;
+        acc[3] += mi * w;
+        acc[4] += mi * (y * w - z * v);
+        acc[5] += mi * (z * u - x * w);
+        acc[6] += mi * (x * v - y * u);
+    }
+    __shared__ double red[7][kBlock];
+    for (int k = 0; k < 7; ++k) red[k][threadIdx.x] = acc[k];
+    __syncthreads();
+    for (int off = kBlock / 2; off > 0; off >>= 1) {
+        if ((int)threadIdx.x < off)
+            for (int k = 0; k < 7; ++k) red[k][threadIdx.x] += red[k][threadIdx.x + off];
+        __syncthreads();
+    }
+    if (threadIdx.x < 7) part[7 * blockIdx.x + threadIdx.x] = red[threadIdx.x][0];
+}
+
+__global__ void k_diag_final(const double* __restrict__ part, int nparts, double* __restrict__ out) {
+    if (threadIdx.x < 7) {
+        double s = 0.0;
+        for (int b = 0; b < nparts; ++b) s += part[7 * b + threadIdx.x];
+        out[threadIdx.x] = s;
+    }
+}
+
+}  // namespace
+
+hipError_t launch_walk(bhx_ctx* c, hipStream_t s) {
+    if (c->n == 0) return hipSuccess;
+    WalkArgs a;
+    a.nodes = c->tree.walk;
+    a.quad = c->tree.quad;
+    a.quadf = c->tree.quadf;
+    a.tx = c->tx; a.ty = c->ty; a.tz = c->tz; a.tm = c->tm;
+    a.tperm = c->tperm;
+    a.n = (uint32_t)c->n;
+    a.quad_on = (uint32_t)c->cfg.quadrupole;
+    a.theta2 = c->cfg.theta * c->cfg.theta;
+    a.eps2 = c->cfg.eps * c->cfg.eps;
+    a.theta2f = (float)a.theta2;
+    a.eps2f = (float)a.eps2;
+    a.ax = c->ax; a.ay = c->ay; a.az = c->az;
+    a.counts = c->wcount;
+    const unsigned groups = (unsigned)((c->n + 63) / 64);
+    const unsigned grid = (groups + kWaves - 1) / kWaves;
+    const bool quad = c->cfg.quadrupole != 0;
+    if (c->cfg.precision == BHX_FP32) {
+        if (quad) hipLaunchKernelGGL(k_walk32<true>, dim3(grid), dim3(kBlock), 0, s, a);
+        else hipLaunchKernelGGL(k_walk32<false>, dim3(grid), dim3(kBlock), 0, s, a);
+    } else {
+        if (quad) hipLaunchKernelGGL(k_walk64<true>, dim3(grid), dim3(kBlock), 0, s, a);
+        else hipLaunchKernelGGL(k_walk64<false>, dim3(grid), dim3(kBlock), 0, s, a);
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_direct(bhx_ctx* c, const int64_t* targets, size_t nt, double eps, double* acc, double* pot,
+                         hipStream_t s) {
+    const unsigned grid = (unsigned)((nt + kBlock - 1) / kBlock);
+    if (grid == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_direct, dim3(grid), dim3(kBlock), 0, s, c->px, c->py, c->pz, c->m, c->n, targets, nt,
+                       eps * eps, acc, pot);
+    return hipGetLastError();
+}
+
+hipError_t launch_diag(bhx_ctx* c, double* out7, hipStream_t s) {
+    // part buffer: reuse the staging area (needs 7 * parts doubles)
+    const int parts = 256;
+    double* part = c->stage;
+    hipLaunchKernelGGL(k_diag_part, dim3(parts), dim3(kBlock), 0, s, c->px, c->py, c->pz, c->vx, c->vy, c->vz, c->m,
+                       c->n, part);
+    hipLaunchKernelGGL(k_diag_final, dim3(1), dim3(64), 0, s, part, parts, out7);
+    return hipGetLastError();
+}
+
+}  // namespace bhx

@@ -50,3 +50,19 @@ def canonical_node_keys(path_keys, leaf_cap):
         sel = np.isin(par, np.array(sorted(big), dtype=np.uint64))
         keys.update(np.unique(cur[sel]).tolist())
     return keys
+
+
+def load_bhx():
+    """Import the engine's ctypes binding from the package directory."""
+    import importlib.util
+    import os
+    import sys
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    path = os.path.join(root, "serial-hashed-octree-barnes-hut-n-body-simulator-0_3_amd", "bhx.py")
+    if "bhx" in sys.modules:
+        return sys.modules["bhx"]
+    spec = importlib.util.spec_from_file_location("bhx", path)
+    mod = importlib.util.module_from_spec(spec)
+    sys.modules["bhx"] = mod
+    spec.loader.exec_module(mod)
+    return mod

@@ -1,0 +1,256 @@
+"""GPU parity: the HIP engine, called through its C ABI, against the CPU oracle
+on the same seeded inputs.
+
+Bit-exact: bounds, Morton keys, sort permutation, tree path keys, node set and
+as-run node payload (equal power-of-two masses make every mass sum exact).
+Tolerances (written at each check): accelerations, because the GPU sums the
+same interaction terms in a different order.
+"""
+import numpy as np
+import pytest
+
+from oracle import oracle as orc
+from tests.helpers import load_bhx
+
+pytestmark = pytest.mark.gpu
+
+N1 = 16384   # Plummer N of the CPU-runnable configuration
+
+
+@pytest.fixture(scope="module")
+def bhx():
+    return load_bhx()
+
+
+@pytest.fixture(scope="module")
+def plummer16k(bhx):
+    return bhx.plummer(N1, seed=1)
+
+
+def _engine(bhx, n, **cfg):
+    print("engine n=%d cfg=%s" % (n, cfg), flush=True)
+    return bhx.Engine(n, **cfg)
+
+
+def test_bounds_and_keys_bitexact(bhx, plummer16k):
+    pos, vel, mass = plummer16k
+    with _engine(bhx, N1) as e:
+        e.set_bodies(pos, vel, mass)
+        c, s = e.bounds()
+        oc, os_ = orc.bounds(pos)
+        assert np.array_equal(c, oc) and s == os_
+        keys = e.morton_keys(s)
+        assert np.array_equal(keys, orc.morton_keys(pos, s))
+
+
+def test_keys_out_of_domain_x86_semantics(bhx):
+    # bodies outside [-S, S] (negative and wrapped scaled coordinates)
+    pos = np.array([[-1.5, 0, 0], [1.0, 0, 0], [0.5, -0.5, 0], [-1, -1, -1], [3.7, -2.2, 9.0]], float)
+    with _engine(bhx, 8) as e:
+        e.set_bodies(pos, None, np.ones(5))
+        keys = e.morton_keys(1.0)
+    assert [int(k) for k in keys] == [orc.morton_key(p, 1.0) for p in pos]
+    assert int(keys[0]) == 0xF924000000000000 and int(keys[1]) == 0xB000000000000000
+
+
+@pytest.mark.parametrize("n", [2, 1000, 4096, 4097, 100003])
+def test_sort_keys_stable(bhx, n):
+    rng = np.random.default_rng(n)
+    base = rng.integers(0, 2 ** 63, 97, dtype=np.int64).astype(np.uint64) | np.uint64(1 << 63)
+    keys = base[rng.integers(0, 97, n)]
+    with _engine(bhx, n) as e:
+        sk, idx = e.sort_keys(keys)
+    ref = np.argsort(keys, kind="stable")
+    assert np.array_equal(idx.astype(np.int64), ref)
+    assert np.array_equal(sk, keys[ref])
+
+
+def test_sort_bodies_matches_reference_radix(bhx, plummer16k):
+    pos, vel, mass = plummer16k
+    with _engine(bhx, N1) as e:
+        e.set_bodies(pos, vel, mass)
+        _, s = e.bounds()
+        e.morton_keys(s)
+        perm = e.sort_bodies()
+        gp, gv, gm, _ = e.get_bodies()
+    keys = orc.morton_keys(pos, s)
+    _, idx = orc.radix_sort(keys)
+    assert np.array_equal(perm, idx)
+    assert np.array_equal(gp, pos[idx.astype(np.int64)]) and np.array_equal(gv, vel[idx.astype(np.int64)])
+
+
+def _compare_nodes(g, o, fields):
+    g = np.sort(g, order="key")
+    o = np.sort(o, order="key")
+    assert len(g) == len(o)
+    for f in fields:
+        assert np.array_equal(g[f], o[f]), f
+
+
+@pytest.mark.parametrize("n,seed", [(1, 3), (2, 4), (17, 5), (N1, 1)])
+def test_build_as_run_node_set_bitexact(bhx, n, seed):
+    pos, vel, mass = bhx.plummer(n, seed=seed)
+    c, s = orc.bounds(pos)
+    with _engine(bhx, n) as e:
+        e.set_bodies(pos, vel, mass)
+        nn = e.build(c, s)
+        g = e.nodes()
+        tperm, tkeys = e.tree_order()
+        unsplit = e.unsplit_leaves()
+    t = orc.build_insert(pos, mass, c, s)
+    o = t.nodes()
+    assert nn == len(o) and unsplit == t.unsplit
+    _compare_nodes(g, o, ["key", "center", "size", "n", "level", "child_byte", "first", "com", "mass"])
+    pk = orc.tree_path_keys(pos, c, s)
+    assert np.array_equal(tkeys, np.sort(pk))
+    assert np.array_equal(tperm.astype(np.int64), np.argsort(pk, kind="stable"))
+
+
+@pytest.mark.parametrize("cap", [4, 32])
+def test_build_bucket_node_set_bitexact(bhx, plummer16k, cap):
+    pos, vel, mass = plummer16k
+    c, s = orc.bounds(pos)
+    with _engine(bhx, N1, leaf_capacity=cap) as e:
+        e.set_bodies(pos, vel, mass)
+        e.build(c, s)
+        g = e.nodes()
+    o = orc.build_recursive(pos, mass, c, s, cap).nodes()
+    _compare_nodes(g, o, ["key", "center", "size", "n", "level", "child_byte", "first", "com", "mass"])
+
+
+def test_preorder_skip_structure(bhx, plummer16k):
+    pos, vel, mass = plummer16k
+    c, s = orc.bounds(pos)
+    with _engine(bhx, N1) as e:
+        e.set_bodies(pos, vel, mass)
+        e.build(c, s)
+        g = e.nodes()
+    n = len(g)
+    assert g["skip"][0] == n and g["level"][0] == 0
+    # subtree [p, skip) = exactly the nodes whose key has p's key as prefix
+    for p in range(0, n, max(1, n // 500)):
+        k, L, sk = int(g["key"][p]), int(g["level"][p]), int(g["skip"][p])
+        sub = g[p:sk]
+        sh = (sub["level"] - L) * 3
+        assert np.all((sub["key"] >> sh.astype(np.uint64)) == k)
+        if sk < n:
+            kk = int(g["key"][sk]) >> (3 * (int(g["level"][sk]) - L)) if g["level"][sk] >= L else None
+            assert kk != k
+
+
+def test_lookup_hash_table(bhx, plummer16k):
+    pos, vel, mass = plummer16k
+    c, s = orc.bounds(pos)
+    with _engine(bhx, N1) as e:
+        e.set_bodies(pos, vel, mass)
+        e.build(c, s)
+        g = e.nodes()
+        ids = e.lookup(g["key"])
+        assert np.array_equal(ids, np.arange(len(g)))
+        missing = np.array([0, 2, (int(g["key"][1]) << 3) | 7, 1 << 62], np.uint64)
+        present = set(int(k) for k in g["key"])
+        exp = [-1 if int(k) not in present else None for k in missing]
+        got = e.lookup(missing)
+        for a, b in zip(got, exp):
+            if b is not None:
+                assert a == b
+
+
+def test_walk_as_run_fp64_matches_reference_lists(bhx, plummer16k):
+    pos, vel, mass = plummer16k
+    c, s = orc.bounds(pos)
+    with _engine(bhx, N1, theta=0.5, eps=0.01) as e:
+        e.set_bodies(pos, vel, mass)
+        e.build(c, s)
+        acc, cnt = e.walk(counts=True)
+    t = orc.build_insert(pos, mass, c, s)
+    oacc, ocnt = t.walk(0.5, 0.01)
+    assert np.array_equal(cnt.astype(np.int64), ocnt)          # same interaction-list lengths
+    err = np.linalg.norm(acc - oacc, axis=1) / np.linalg.norm(oacc, axis=1)
+    assert err.max() < 1e-12                                    # same terms, other summation order
+
+
+def test_walk_theta_zero_is_direct_sum(bhx):
+    pos, vel, mass = bhx.plummer(3000, seed=9)
+    c, s = orc.bounds(pos)
+    with _engine(bhx, 3000, theta=0.0, eps=0.01) as e:
+        e.set_bodies(pos, vel, mass)
+        e.build(c, s)
+        acc, cnt = e.walk(counts=True)
+        dacc, _ = e.direct(np.arange(3000), 0.01)
+    ref, _ = orc.direct(pos, mass, 0.01)
+    assert np.all(cnt == 2999)
+    assert (np.linalg.norm(acc - ref, axis=1) / np.linalg.norm(ref, axis=1)).max() < 1e-12
+    assert (np.linalg.norm(dacc - ref, axis=1) / np.linalg.norm(ref, axis=1)).max() < 1e-12
+
+
+@pytest.mark.parametrize("prec", [0, 1])
+def test_walk_intended_quadrupole_accuracy(bhx, plummer16k, prec):
+    pos, vel, mass = plummer16k
+    c, s = orc.bounds(pos)
+    with _engine(bhx, N1, theta=0.5, eps=0.01, mode=1, quadrupole=1, leaf_capacity=32, precision=prec) as e:
+        e.set_bodies(pos, vel, mass)
+        e.build(c, s)
+        acc = e.walk()
+    t = orc.build_recursive(pos, mass, c, s, 32)
+    t.moments_intended()
+    oacc, _ = t.walk(0.5, 0.01, quad=True)
+    ref, _ = orc.direct(pos, mass, 0.01, targets=np.arange(0, N1, 8))
+    sub = acc[::8]
+    e_bh = np.linalg.norm(sub - ref, axis=1) / np.linalg.norm(ref, axis=1)
+    assert np.median(e_bh) < 1e-3                                # BH + precision error vs FP64 direct sum
+    e_or = np.linalg.norm(acc - oacc, axis=1) / np.linalg.norm(oacc, axis=1)
+    tol = 1e-11 if prec == 0 else 1e-4                          # FP64: summation order; FP32: rounding
+    assert np.median(e_or) < tol
+
+
+@pytest.mark.parametrize("permute_acc", [1, 0])
+def test_step_matches_reference_frames(bhx, permute_acc):
+    n = 4096
+    pos, vel, mass = bhx.plummer(n, seed=21)
+    dt, theta, eps = 1e-3, 0.5, 0.01
+    with _engine(bhx, n, theta=theta, eps=eps, dt=dt, permute_acc=permute_acc) as e:
+        e.set_bodies(pos, vel, mass)
+        e.step(3)
+        gp, gv, gm, ga = e.get_bodies()
+    p, v, m, a = pos, vel, mass, np.zeros_like(pos)
+    for _ in range(3):
+        p, v, m, a, keys, st = orc.frame(p, v, m, a, dt, theta, eps, permute_acc=bool(permute_acc))
+    assert np.array_equal(gm, m)
+    assert np.allclose(gp, p, rtol=0, atol=1e-13) and np.allclose(gv, v, rtol=0, atol=1e-12)
+    assert np.allclose(ga, a, rtol=1e-10, atol=1e-12)
+
+
+def test_kick_drift_kick_bitexact(bhx):
+    pos, vel, mass = bhx.plummer(1000, seed=2)
+    acc = np.random.default_rng(0).normal(size=(1000, 3))
+    with _engine(bhx, 1000) as e:
+        e.set_bodies(pos, vel, mass)
+        e.set_accelerations(acc)
+        e.kick_drift(1e-3)
+        e.kick(1e-3)
+        gp, gv, _, _ = e.get_bodies()
+    p1, v1 = orc.kick_drift(1e-3, pos, vel, acc)
+    v2 = orc.kick(1e-3, v1, acc)
+    assert np.array_equal(gp, p1) and np.array_equal(gv, v2)
+
+
+def test_empty_and_errors(bhx):
+    with _engine(bhx, 16) as e:
+        e.set_bodies(np.zeros((0, 3)), None, np.zeros(0))
+        assert e.build(np.zeros(3), 1.0) == 0
+        e.step(1)
+        with pytest.raises(bhx.BhxError):
+            e.set_bodies(np.zeros((17, 3)), None, np.ones(17))     # over capacity
+        with pytest.raises(bhx.BhxError):
+            e.build(np.array([np.nan, 0, 0]), 1.0)
+
+
+def test_diagnostics_momentum(bhx, plummer16k):
+    pos, vel, mass = plummer16k
+    with _engine(bhx, N1) as e:
+        e.set_bodies(pos, vel, mass)
+        ek, mom, am = e.diagnostics()
+    assert ek == pytest.approx(0.5 * (mass * (vel ** 2).sum(1)).sum(), rel=1e-12)
+    assert np.allclose(mom, (mass[:, None] * vel).sum(0), atol=1e-14)
+    assert np.allclose(am, (mass[:, None] * np.cross(pos, vel)).sum(0), atol=1e-13)
