@@ -1,0 +1,224 @@
+"""Barnes-Hut full-step throughput on one MI355X.
+
+Workload (the precision configuration of the headline metric): Plummer sphere,
+N = 1,048,576 equal masses summing to 1, G = 1, scale radius 1, seeded
+virial-equilibrium velocities; theta = 0.5, Plummer softening 0.01, leaf
+bucket 32, dt = 5e-4.  The walk runs in FP32 with group-relative coordinates
+and quadrupole moments; positions and velocities stay FP64 with compensated
+leapfrog accumulation.  A step is one full frame: bounds, Morton keys, body
+sort, kick+drift, tree keys, tree sort, node build, hash table, moments,
+walk, kick.
+
+Prints ONE JSON line.  value = body-updates/s = N * steps / wall time of the
+timed steps (bodies resident in HBM).  The same steps are then timed with the
+FP64 walk, the force error of both walks is measured against an FP64 direct
+sum on a seeded 4,096-body subsample, the walk kernel is timed with HIP events
+for the roofline, and the serial CPU oracle is timed on a bounded sample of
+the same workload on this host.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "body-updates/s at N=1M theta=0.5 (FP32 vs FP64 walk); fwd+adjoint/fwd time ratio"
+PEAK_FP32_TFLOPS = 157.3     # MI355X published vector FP32
+PEAK_FP64_TFLOPS = 78.6      # MI355X published vector FP64
+PEAK_HBM_GBS = 8000.0
+FLOPS_PER_INTERACTION = 20   # softened monopole term, the survey's counting convention
+
+
+def load_bhx():
+    import importlib.util
+    path = os.path.join(ROOT, "serial-hashed-octree-barnes-hut-n-body-simulator-0_3_amd", "bhx.py")
+    spec = importlib.util.spec_from_file_location("bhx", path)
+    mod = importlib.util.module_from_spec(spec)
+    sys.modules["bhx"] = mod
+    spec.loader.exec_module(mod)
+    return mod
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def timed_steps(e, steps, warmup):
+    e.step(warmup)
+    e.sync()
+    t0 = time.perf_counter()
+    e.step(steps)
+    e.sync()
+    return time.perf_counter() - t0
+
+
+def walk_kernel_ms(e, reps):
+    """Average walk-kernel time from HIP events recorded around the launch on the engine's stream."""
+    e.set_profiling(True)
+    ms = []
+    for _ in range(reps):
+        e.step(1)
+        ms.append(e.last_timing()["walk"])
+    e.set_profiling(False)
+    return float(np.mean(ms)), e.last_timing()
+
+
+def force_error(e, bhx, nsub, seed, eps):
+    n = e.n
+    rng = np.random.default_rng(seed)
+    tg = np.sort(rng.choice(n, size=min(nsub, n), replace=False))
+    ref, _ = e.direct(tg, eps)
+    acc = e.walk()[tg]
+    err = np.linalg.norm(acc - ref, axis=1) / np.linalg.norm(ref, axis=1)
+    return {"median": float(np.median(err)), "p90": float(np.percentile(err, 90)),
+            "p99": float(np.percentile(err, 99)), "subsample": int(tg.size)}
+
+
+def cpu_baseline(pos, vel, mass, theta, eps, dt, cap, nsub, seed):
+    """Serial oracle (oracle/liboracle.so, one core) on a bounded sample of the workload:
+    every O(N log N) phase over all N bodies, the walk for `nsub` seeded bodies,
+    extrapolated to N walk evaluations."""
+    from oracle import oracle as orc
+    n = pos.shape[0]
+    t = {}
+    t0 = time.perf_counter()
+    c, s = orc.bounds(pos)
+    t["bounds"] = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    keys = orc.morton_keys(pos, s)
+    t["keys"] = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    _, idx = orc.radix_sort(keys)
+    idx = idx.astype(np.int64)
+    p, v, m = pos[idx], vel[idx], mass[idx]
+    t["sort"] = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    p, v = orc.kick_drift(dt, p, v, np.zeros_like(p))
+    t["kick_drift"] = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    tree = orc.build_recursive(p, m, c, s, cap)
+    tree.moments_intended()
+    t["build"] = time.perf_counter() - t0
+    rng = np.random.default_rng(seed)
+    tg = np.sort(rng.choice(n, size=min(nsub, n), replace=False))
+    t0 = time.perf_counter()
+    acc, cnt = tree.walk(theta, eps, quad=True, targets=tg)
+    t_walk_sub = time.perf_counter() - t0
+    t["walk"] = t_walk_sub * n / tg.size
+    t0 = time.perf_counter()
+    orc.kick(dt, v, np.zeros_like(v))
+    t["kick"] = time.perf_counter() - t0
+    step = sum(t.values())
+    return {
+        "value": n / step, "unit": "body-updates/s", "cores": 1, "kind": "port",
+        "sample": ("serial oracle, FP64: bounds, keys, sort, kick/drift, bucket-32 build + moments and kick "
+                   "over all %d bodies; walk for %d seeded bodies (%.1f s) extrapolated x%.0f" %
+                   (n, tg.size, t_walk_sub, n / tg.size)),
+        "phase_s": {k: round(v_, 4) for k, v_ in t.items()},
+        "cpu": _cpu_model(),
+        "mean_interactions": float(np.mean(cnt)),
+    }
+
+
+def _cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--n", type=int, default=1 << 20)
+    ap.add_argument("--seed", type=int, default=1)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-walk-sample", type=int, default=1 << 19)
+    ap.add_argument("--no-fp64", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: this build runs the whole Barnes-Hut step on a single MI355X "
+              "(no multi-GPU path); use --gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    bhx = load_bhx()
+    theta, eps, dt, cap = 0.5, 0.01, 5e-4, 32
+    n = args.n
+    log("generating Plummer N=%d seed=%d" % (n, args.seed))
+    pos, vel, mass = bhx.plummer(n, seed=args.seed)
+    cfg = dict(theta=theta, eps=eps, dt=dt, leaf_capacity=cap, mode=bhx.INTENDED, precision=bhx.FP32,
+               quadrupole=1, compensated=1, permute_acc=1)
+    out = {}
+    with bhx.Engine(n, **cfg) as e:
+        e.set_bodies(pos, vel, mass)
+        log("fp32: warmup %d, timing %d steps" % (args.warmup, args.steps))
+        t32 = timed_steps(e, args.steps, args.warmup)
+        ms32 = 1e3 * t32 / args.steps
+        log("fp32 %.3f ms/step" % ms32)
+        walk_ms, phases = walk_kernel_ms(e, 3)
+        acc, cnt = e.walk(counts=True)
+        inter = float(cnt.astype(np.float64).sum())
+        err32 = force_error(e, bhx, 4096, 1234, eps)
+        ek, mom, _ = e.diagnostics()
+        out["fp32"] = dict(ms=ms32, walk_ms=walk_ms, phases=phases, inter=inter, err=err32)
+        if not args.no_fp64:
+            e.set_config(precision=bhx.FP64)
+            log("fp64: warmup %d, timing %d steps" % (args.warmup, args.steps))
+            t64 = timed_steps(e, args.steps, args.warmup)
+            walk64_ms, phases64 = walk_kernel_ms(e, 3)
+            _, cnt64 = e.walk(counts=True)
+            err64 = force_error(e, bhx, 4096, 1234, eps)
+            out["fp64"] = dict(ms=1e3 * t64 / args.steps, walk_ms=walk64_ms, phases=phases64,
+                               inter=float(cnt64.astype(np.float64).sum()), err=err64)
+    value = n * args.steps / t32
+    achieved = out["fp32"]["inter"] * FLOPS_PER_INTERACTION / (walk_ms * 1e-3) / 1e12
+    line = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "body-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms32,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic: seeded Plummer sphere (Philox4x32-10, seed %d) generated on the host" % args.seed,
+        "config": {"workload": "Plummer N=%d, theta=%g, eps=%g, leaf bucket %d, FP32 group-relative walk with "
+                               "quadrupoles, FP64 compensated KDK, dt=%g; full step = bounds+keys+sort+kick/drift"
+                               "+tree keys+sort+build+hash+moments+walk+kick" % (n, theta, eps, cap, dt),
+                   "n": n, "theta": theta, "eps": eps, "leaf_capacity": cap, "dt": dt},
+        "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
+                     "frac": achieved / PEAK_FP32_TFLOPS, "traffic": None,
+                     "kernel": "k_walk32<true>", "kernel_ms": walk_ms,
+                     "work": "%d interactions x %d flops" % (out["fp32"]["inter"], FLOPS_PER_INTERACTION)},
+        "force_error_fp32": out["fp32"]["err"],
+        "phases_ms_fp32": {k: round(v, 4) for k, v in out["fp32"]["phases"].items()},
+    }
+    if "fp64" in out:
+        f64 = out["fp64"]
+        ach64 = f64["inter"] * FLOPS_PER_INTERACTION / (f64["walk_ms"] * 1e-3) / 1e12
+        line["fp64_walk"] = {"value": n * args.steps / (f64["ms"] * 1e-3 * args.steps), "ms_per_step": f64["ms"],
+                             "walk_kernel_ms": f64["walk_ms"], "force_error": f64["err"],
+                             "achieved_tflops": ach64, "frac_fp64_peak": ach64 / PEAK_FP64_TFLOPS,
+                             "phases_ms": {k: round(v, 4) for k, v in f64["phases"].items()}}
+        line["fp32_over_fp64_walk"] = f64["walk_ms"] / walk_ms
+    if not args.no_cpu_baseline:
+        log("cpu baseline (serial oracle, bounded sample)")
+        line["cpu_baseline"] = cpu_baseline(pos, vel, mass, theta, eps, dt, cap, args.cpu_walk_sample, 99)
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()
