@@ -58,7 +58,10 @@ typedef struct bhx_config {
     int32_t permute_acc;   /* 1: reorder accelerations with the bodies; 0: as run   */
     int32_t compensated;   /* 1: compensated (Kahan) FP64 kick/drift accumulation   */
     int32_t device;        /* HIP device ordinal                                    */
-    int32_t reserved;
+    int32_t group_walk;    /* 0: per-body opening test, the reference's lists exactly;
+                              1: one conservative test per 64-body group (a cell is
+                              accepted only if every body of the group accepts it),
+                              shared interaction lists evaluated densely            */
 } bhx_config;
 
 typedef struct bhx_ctx bhx_ctx;
@@ -162,6 +165,12 @@ int bhx_unsplit_leaves(bhx_ctx *ctx, size_t *count);
  * build (array order).  acc_out and counts (interaction-list length per body,
  * the value TraverseHOTInteractionList returns) may be NULL. */
 int bhx_walk(bhx_ctx *ctx, double *acc_out, uint64_t *counts);
+
+/* TraverseHOTInteractionList's return value (HashedOctree.hpp:336-385) for
+ * every body (array order): the length of the reference's own per-body
+ * interaction list on the last tree, whichever walk the configuration
+ * selects.  The resident accelerations are left unchanged. */
+int bhx_interaction_counts(bhx_ctx *ctx, uint64_t *counts);
 
 /* One frame in the reference order (ofApp::update then ofApp::draw,
  * ofApp.cpp:169-177,248-282), repeated nsteps times on the resident bodies. */

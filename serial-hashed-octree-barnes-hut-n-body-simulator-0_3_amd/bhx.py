@@ -39,7 +39,7 @@ class Config(ctypes.Structure):
     _fields_ = [("theta", ctypes.c_double), ("eps", ctypes.c_double), ("dt", ctypes.c_double),
                 ("leaf_capacity", ctypes.c_int32), ("mode", ctypes.c_int32), ("precision", ctypes.c_int32),
                 ("quadrupole", ctypes.c_int32), ("permute_acc", ctypes.c_int32), ("compensated", ctypes.c_int32),
-                ("device", ctypes.c_int32), ("reserved", ctypes.c_int32)]
+                ("device", ctypes.c_int32), ("group_walk", ctypes.c_int32)]
 
 
 class Timing(ctypes.Structure):
@@ -66,6 +66,7 @@ def lib():
             "bhx_sort_bodies": [P, P], "bhx_reorder_bodies": [P, P], "bhx_kick_drift": [P, D], "bhx_kick": [P, D],
             "bhx_build": [P, P, D, P], "bhx_get_nodes": [P, P, S, P], "bhx_lookup": [P, P, S, P],
             "bhx_get_tree_order": [P, P, P], "bhx_unsplit_leaves": [P, P], "bhx_walk": [P, P, P],
+            "bhx_interaction_counts": [P, P],
             "bhx_step": [P, I], "bhx_last_timing": [P, P], "bhx_set_profiling": [P, I],
             "bhx_direct": [P, P, S, D, P, P], "bhx_diagnostics": [P, P, P, P],
             "bhx_ic_plummer": [S, ctypes.c_uint64, D, D, P, P, P],
@@ -246,6 +247,12 @@ class Engine:
         cnt = np.empty(n, np.uint64) if counts else None
         self._chk(lib().bhx_walk(self.h, _p(acc), _p(cnt)), "bhx_walk")
         return (acc, cnt) if counts else acc
+
+    def interaction_counts(self):
+        """Reference per-body interaction-list lengths on the last tree (accelerations untouched)."""
+        cnt = np.empty(self.n, np.uint64)
+        self._chk(lib().bhx_interaction_counts(self.h, _p(cnt)), "bhx_interaction_counts")
+        return cnt
 
     def step(self, nsteps=1):
         self._chk(lib().bhx_step(self.h, nsteps), "bhx_step")

@@ -632,6 +632,22 @@ int bhx_walk(bhx_ctx* c, double* acc_out, uint64_t* counts) {
     return BHX_OK;
 }
 
+int bhx_interaction_counts(bhx_ctx* c, uint64_t* counts) {
+    if (!c || !counts) return c ? fail(c, BHX_E_NULL, "counts is a null pointer") : BHX_E_NULL;
+    if (!c->tree_valid) return fail(c, BHX_E_STATE, "no tree built");
+    if (c->n == 0) return BHX_OK;
+    HIPCHK(hipSetDevice(c->device));
+    hipStream_t s = c->s_main;
+    // per-body FP64 walk into scratch (the acceleration ping-pong buffers)
+    HIPCHK(launch_walk_to(c, s, false, BHX_FP64, c->alt[13], c->alt[14], c->alt[15], c->wcount));
+    uint64_t* tmp = reinterpret_cast<uint64_t*>(c->stage);
+    hipLaunchKernelGGL(k_u32_to_u64, dim3(grid_for(c->n)), dim3(kBlock), 0, s, c->wcount, c->n, tmp);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(counts, tmp, c->n * 8, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    return BHX_OK;
+}
+
 int bhx_set_profiling(bhx_ctx* c, int on) {
     if (!c) return BHX_E_NULL;
     c->profiling = on != 0;

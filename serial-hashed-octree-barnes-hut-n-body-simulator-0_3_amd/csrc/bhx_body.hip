@@ -7,7 +7,7 @@ namespace bhx {
 namespace {
 
 constexpr int kBlock = 256;
-constexpr int kMeanChunk = 1024;
+constexpr int kMeanChunk = 2048;   // 2 x 3 x 2048 doubles = 96 KiB of the 160 KiB LDS
 
 inline unsigned grid_for(size_t n, int per_block = kBlock, unsigned cap = 65535u * 16u) {
     size_t g = (n + per_block - 1) / per_block;
@@ -30,37 +30,54 @@ __global__ __launch_bounds__(kBlock) void k_bounds_mean(const double* __restrict
     const int tid = threadIdx.x;
     const int wave = tid >> 6, lane = tid & 63;
     const size_t nchunks = (n + kMeanChunk - 1) / kMeanChunk;
-
-    auto stage = [&](size_t ch, int b) {
-        if (wave == 0) return;
-        const size_t base = ch * kMeanChunk;
-        for (int k = tid - 64; k < 3 * kMeanChunk; k += kBlock - 64) {
-            const int a = k / kMeanChunk, j = k - a * kMeanChunk;
-            const size_t i = base + j;
-            const double* src = a == 0 ? px : (a == 1 ? py : pz);
-            buf[b][a][j] = i < n ? src[i] : 0.0;
-        }
-    };
+    const double* const src[3] = {px, py, pz};
 
     double s = 0.0;
-    if (nchunks) stage(0, 0);
-    __syncthreads();
-    for (size_t ch = 0; ch < nchunks; ++ch) {
+    for (size_t ch = 0; ch <= nchunks; ++ch) {
         const int b = (int)(ch & 1);
-        if (ch + 1 < nchunks) stage(ch + 1, b ^ 1);
-        if (wave == 0 && lane < 3) {
-            const size_t base = ch * kMeanChunk;
-            const int cnt = (int)((n - base) < (size_t)kMeanChunk ? (n - base) : (size_t)kMeanChunk);
-            const double* v = buf[b][lane];
-            int j = 0;
-            for (; j + 16 <= cnt; j += 16) {
-                double t[16];
-#pragma unroll
-                for (int u = 0; u < 16; ++u) t[u] = v[j + u];
-#pragma unroll
-                for (int u = 0; u < 16; ++u) s = s + t[u];
+        if (wave != 0) {
+            // waves 1..3 stage chunk ch (axis = wave - 1) while wave 0 sums chunk ch-1
+            if (ch < nchunks) {
+                const int a = wave - 1;
+                const size_t base = ch * kMeanChunk;
+                const double* p = src[a] + base;
+                double* d = buf[b][a];
+                if (base + kMeanChunk <= n) {
+                    for (int j = 2 * lane; j < kMeanChunk; j += 128) {
+                        const double2 v = *reinterpret_cast<const double2*>(p + j);
+                        d[j] = v.x;
+                        d[j + 1] = v.y;
+                    }
+                } else {
+                    for (int j = lane; j < kMeanChunk; j += 64) d[j] = (base + j < n) ? p[j] : 0.0;
+                }
             }
-            for (; j < cnt; ++j) s = s + v[j];
+        } else if (ch > 0 && lane < 3) {
+            // the dependent chain: register double buffering keeps the LDS reads
+            // of the next 16 values in flight while the current 16 are added
+            const size_t base = (ch - 1) * kMeanChunk;
+            const int cnt = (int)((n - base) < (size_t)kMeanChunk ? (n - base) : (size_t)kMeanChunk);
+            const double* v = buf[b ^ 1][lane];
+            int j = 0;
+            if (cnt == kMeanChunk) {
+                double t0[16], t1[16];
+#pragma unroll
+                for (int u = 0; u < 16; ++u) t0[u] = v[u];
+                for (j = 0; j < kMeanChunk; j += 32) {
+#pragma unroll
+                    for (int u = 0; u < 16; ++u) t1[u] = v[j + 16 + u];
+#pragma unroll
+                    for (int u = 0; u < 16; ++u) s = s + t0[u];
+                    if (j + 32 < kMeanChunk) {
+#pragma unroll
+                        for (int u = 0; u < 16; ++u) t0[u] = v[j + 32 + u];
+                    }
+#pragma unroll
+                    for (int u = 0; u < 16; ++u) s = s + t1[u];
+                }
+            } else {
+                for (; j < cnt; ++j) s = s + v[j];
+            }
         }
         __syncthreads();
     }

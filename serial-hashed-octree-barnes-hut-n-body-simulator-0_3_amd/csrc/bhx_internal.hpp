@@ -147,6 +147,8 @@ hipError_t build_tree(bhx_ctx* c, hipStream_t s, size_t* total_out);
 hipError_t launch_lookup(const DeviceTree& t, const uint64_t* keys, size_t n, int64_t* ids, hipStream_t s);
 
 hipError_t launch_walk(bhx_ctx* c, hipStream_t s);
+hipError_t launch_walk_to(bhx_ctx* c, hipStream_t s, bool group, int precision, double* ox, double* oy, double* oz,
+                          uint32_t* counts);
 hipError_t launch_direct(bhx_ctx* c, const int64_t* targets, size_t nt, double eps, double* acc,
                          double* pot, hipStream_t s);
 hipError_t launch_diag(bhx_ctx* c, double* out7, hipStream_t s);

@@ -24,6 +24,28 @@ constexpr int kWaves = kBlock / 64;
 typedef const __attribute__((address_space(4))) WalkNode* ConstNodes;
 typedef const __attribute__((address_space(4))) double* ConstD;
 typedef const __attribute__((address_space(4))) float* ConstF;
+typedef const __attribute__((address_space(4))) unsigned long long* ConstU64;
+
+// One 64-byte node record as eight 8-byte scalar loads issued together (the
+// backend merges them into one burst), before the first field is used.
+__device__ __forceinline__ WalkNode load_node(ConstU64 raw, uint32_t p) {
+    const size_t b = 8 * (size_t)p;
+    const unsigned long long w0 = raw[b], w1 = raw[b + 1], w2 = raw[b + 2], w3 = raw[b + 3];
+    const unsigned long long w4 = raw[b + 4], w5 = raw[b + 5], w6 = raw[b + 6], w7 = raw[b + 7];
+    WalkNode nd;
+    nd.cx = __longlong_as_double((long long)w0);
+    nd.cy = __longlong_as_double((long long)w1);
+    nd.cz = __longlong_as_double((long long)w2);
+    nd.mass = __longlong_as_double((long long)w3);
+    nd.size2 = __longlong_as_double((long long)w4);
+    nd.skip = (uint32_t)w5;
+    nd.start = (uint32_t)(w5 >> 32);
+    nd.count = (uint32_t)w6;
+    nd.level_leaf = (uint32_t)(w6 >> 32);
+    nd.size2f = __uint_as_float((uint32_t)w7);
+    nd.massf = __uint_as_float((uint32_t)(w7 >> 32));
+    return nd;
+}
 
 struct WalkArgs {
     const WalkNode* nodes;
@@ -266,6 +288,232 @@ __global__ __launch_bounds__(kBlock) void k_walk32(WalkArgs a) {
     }
 }
 
+// ---------------------------------------------------------------- group walk
+//
+// One wave walks for a group of 64 consecutive tree-order bodies and builds
+// ONE interaction list that every lane evaluates.  Each lane applies the
+// reference opening test (size^2/d^2 < theta^2, d to the cell's centre of
+// mass) to every cell the wave visits; a cell enters the shared list only
+// when every lane accepts it, and is opened as soon as one lane opens it.
+// The shared list therefore refines each body's own reference list (a cell
+// is replaced by its children wherever any neighbour needs them), so no body
+// is approximated more coarsely than the reference would, and all 64 lanes
+// do useful work on every entry.  Accepted cells go to an LDS multipole list,
+// opened leaves to a queue of body ranges; full lists are evaluated densely.
+// Node records arrive as one 64-byte scalar burst and the bodies of queued
+// leaves in one batch of loads per flush.  FP32 works in offsets from the
+// group's first body (only the offsets are rounded); FP64 keeps absolute
+// coordinates, the reference's unfused opening test and its term
+// 1/sqrt(D2) * 1/D2.
+constexpr int kNodeCap = 128;
+constexpr int kBodyCap = 128;
+constexpr int kRangeCap = 64;
+
+template <typename T>
+__device__ __forceinline__ T rsqrt_t(T r2);
+template <>
+__device__ __forceinline__ float rsqrt_t<float>(float r2) { return __builtin_amdgcn_rsqf(r2); }
+template <>
+__device__ __forceinline__ double rsqrt_t<double>(double r2) { return 1.0 / sqrt(r2); }
+
+// 1/r2 from 1/sqrt(r2): squared in FP32, the reference's separate division in FP64
+template <typename T>
+__device__ __forceinline__ T inv_sq_t(T inv, T r2);
+template <>
+__device__ __forceinline__ float inv_sq_t<float>(float inv, float) { return inv * inv; }
+template <>
+__device__ __forceinline__ double inv_sq_t<double>(double, double r2) { return 1.0 / r2; }
+
+// The reference opening test size^2/d^2 < theta^2: exact FP64 form, and the
+// FP32 form size^2 < theta^2 d^2 (equal away from rounding at the boundary).
+__device__ __forceinline__ bool mac_t(double s2, double d2, double th2, float, float) {
+#pragma clang fp contract(off)
+    return s2 / d2 < th2;
+}
+__device__ __forceinline__ bool mac_t(float, float d2, float, float s2f, float th2f) { return s2f < th2f * d2; }
+
+// |d|^2 as the reference forms it: unfused in FP64, fused multiply-adds in FP32
+__device__ __forceinline__ double dist2(double dx, double dy, double dz) {
+#pragma clang fp contract(off)
+    return dx * dx + dy * dy + dz * dz;
+}
+__device__ __forceinline__ float dist2(float dx, float dy, float dz) { return dx * dx + dy * dy + dz * dz; }
+
+// kGuard: without softening a zero separation (the body itself) must add
+// nothing; with softening d = 0 already contributes 0.
+template <typename T, bool kQuad, bool kGuard>
+__device__ __forceinline__ void eval_nodes(const T (*nd)[4], const T (*q)[6], int nn, T xr, T yr, T zr, T eps2,
+                                           T& ax, T& ay, T& az) {
+    for (int k = 0; k < nn; ++k) {
+        const T dx = nd[k][0] - xr, dy = nd[k][1] - yr, dz = nd[k][2] - zr;
+        const T d2 = dist2(dx, dy, dz);
+        const T r2 = d2 + eps2;
+        T inv = rsqrt_t<T>(r2);
+        if (kGuard) inv = d2 > (T)0 ? inv : (T)0;
+        const T inv2 = inv_sq_t<T>(inv, r2);
+        const T inv3 = inv * inv2;
+        const T f = nd[k][3] * inv3;
+        ax += f * dx;
+        ay += f * dy;
+        az += f * dz;
+        if (kQuad) {
+            const T q0 = q[k][0], q1 = q[k][1], q2 = q[k][2], q3 = q[k][3], q4 = q[k][4], q5 = q[k][5];
+            const T qx = q0 * dx + q1 * dy + q2 * dz;
+            const T qy = q1 * dx + q3 * dy + q4 * dz;
+            const T qz = q2 * dx + q4 * dy + q5 * dz;
+            const T inv5 = inv3 * inv2;
+            const T rq = (T)2.5 * (dx * qx + dy * qy + dz * qz) * (inv5 * inv2);
+            ax += rq * dx - qx * inv5;
+            ay += rq * dy - qy * inv5;
+            az += rq * dz - qz * inv5;
+        }
+    }
+}
+
+template <typename T, bool kGuard>
+__device__ __forceinline__ void eval_bodies(const T (*bd)[4], int nb, T xr, T yr, T zr, T eps2, T& ax, T& ay,
+                                            T& az) {
+    for (int k = 0; k < nb; ++k) {
+        const T dx = bd[k][0] - xr, dy = bd[k][1] - yr, dz = bd[k][2] - zr;
+        const T d2 = dist2(dx, dy, dz);
+        const T r2 = d2 + eps2;
+        T inv = rsqrt_t<T>(r2);
+        if (kGuard) inv = d2 > (T)0 ? inv : (T)0;
+        const T inv3 = inv * inv_sq_t<T>(inv, r2);
+        const T f = bd[k][3] * inv3;
+        ax += f * dx;
+        ay += f * dy;
+        az += f * dz;
+    }
+}
+
+// Bodies of the queued leaf ranges into LDS with every load in flight at
+// once: lane l fills slots l and l+64, finding each slot's range by bisection
+// over the ranges' first slots.
+template <typename T>
+__device__ __forceinline__ void gather_bodies(T (*bd)[4], const uint32_t* rstart, const uint32_t* roff, int nr, int nq,
+                                              uint32_t lane, const WalkArgs& a, double ox, double oy, double oz) {
+#pragma unroll
+    for (int h = 0; h < kBodyCap / 64; ++h) {
+        const int e = (int)lane + 64 * h;
+        if (e < nq) {
+            int lo = 0, hi = nr - 1;
+            while (lo < hi) {
+                const int mid = (lo + hi + 1) >> 1;
+                if ((int)roff[mid] <= e) lo = mid; else hi = mid - 1;
+            }
+            const uint32_t j = rstart[lo] + (uint32_t)(e - (int)roff[lo]);
+            const double bx = a.tx[j], by = a.ty[j], bz = a.tz[j], bm = a.tm[j];
+            bd[e][0] = (T)(bx - ox);
+            bd[e][1] = (T)(by - oy);
+            bd[e][2] = (T)(bz - oz);
+            bd[e][3] = (T)bm;
+        }
+    }
+}
+
+template <typename T, bool kQuad, bool kGuard>
+__global__ __launch_bounds__(64) void k_walk_group(WalkArgs a) {
+    __shared__ T s_nd[kNodeCap][4];
+    __shared__ T s_q[kQuad ? kNodeCap : 1][6];
+    __shared__ T s_bd[kBodyCap][4];
+    __shared__ uint32_t s_rstart[kRangeCap], s_roff[kRangeCap];
+
+    const uint32_t g0 = blockIdx.x * 64u;
+    const uint32_t lane = threadIdx.x;
+    if (g0 >= a.n) return;
+    const uint32_t i = g0 + lane;
+    const bool valid = i < a.n;
+    const uint32_t iv = valid ? i : g0;
+    ConstU64 raw = (ConstU64)a.nodes;
+    ConstF qf = (ConstF)a.quadf;
+    ConstD qd = (ConstD)a.quad;
+    const bool rel = sizeof(T) == 4;
+    const double ox = rel ? a.tx[g0] : 0.0, oy = rel ? a.ty[g0] : 0.0, oz = rel ? a.tz[g0] : 0.0;
+    const T xr = (T)(a.tx[iv] - ox), yr = (T)(a.ty[iv] - oy), zr = (T)(a.tz[iv] - oz);
+    const T eps2 = (T)a.eps2;
+    const double th2d = a.theta2;
+    const float th2f = a.theta2f;
+    T ax = 0, ay = 0, az = 0;
+    uint32_t cnt = 0;
+    int nn = 0, nr = 0, nq = 0;            // queued cells, leaf ranges, bodies in those ranges
+    const WalkNode root = load_node(raw, 0);
+    const uint32_t end = root.skip;
+    uint32_t p = (root.level_leaf & 256u) ? 0u : 1u;
+    while (p < end) {
+        p = __builtin_amdgcn_readfirstlane(p);
+        const WalkNode nd = load_node(raw, p);
+        // a subtree always ends after its root; the clamp keeps the walk moving
+        // forward (and finite) whatever the record holds
+        const uint32_t nskip = nd.skip > p ? nd.skip : p + 1;
+        const uint32_t L = nd.level_leaf & 255u;
+        const bool leaf = (nd.level_leaf & 256u) != 0;
+        const T cx = (T)(nd.cx - ox), cy = (T)(nd.cy - oy), cz = (T)(nd.cz - oz);
+        const T dx = cx - xr, dy = cy - yr, dz = cz - zr;
+        const T d2 = dist2(dx, dy, dz);
+        const bool pass = (L != 0) & mac_t((T)nd.size2, d2, (T)th2d, nd.size2f, th2f);
+        const bool opened = __ballot(valid & !pass) != 0ull;
+        if (!opened) {
+            if (nn == kNodeCap) {
+                __syncthreads();
+                eval_nodes<T, kQuad, kGuard>(s_nd, s_q, nn, xr, yr, zr, eps2, ax, ay, az);
+                cnt += (uint32_t)nn;
+                nn = 0;
+                __syncthreads();
+            }
+            if (lane == 0) {
+                s_nd[nn][0] = cx;
+                s_nd[nn][1] = cy;
+                s_nd[nn][2] = cz;
+                s_nd[nn][3] = sizeof(T) == 4 ? (T)nd.massf : (T)nd.mass;
+            }
+            if (kQuad && lane < 6) {
+                const bool many = nd.count > 1;
+                s_q[nn][lane] = many ? (sizeof(T) == 4 ? (T)qf[6 * (size_t)p + lane] : (T)qd[6 * (size_t)p + lane])
+                                     : (T)0;
+            }
+            ++nn;
+            p = nskip;
+        } else if (leaf) {
+            for (uint32_t k = 0; k < nd.count; k += 64) {
+                const uint32_t m = (nd.count - k) < 64u ? (nd.count - k) : 64u;
+                if (nq + (int)m > kBodyCap || nr == kRangeCap) {
+                    __syncthreads();
+                    gather_bodies<T>(s_bd, s_rstart, s_roff, nr, nq, lane, a, ox, oy, oz);
+                    __syncthreads();
+                    eval_bodies<T, kGuard>(s_bd, nq, xr, yr, zr, eps2, ax, ay, az);
+                    cnt += (uint32_t)nq;
+                    nr = 0;
+                    nq = 0;
+                    __syncthreads();
+                }
+                if (lane == 0) {
+                    s_rstart[nr] = nd.start + k;
+                    s_roff[nr] = (uint32_t)nq;
+                }
+                ++nr;
+                nq += (int)m;
+            }
+            p = nskip;
+        } else {
+            p = p + 1;
+        }
+    }
+    __syncthreads();
+    eval_nodes<T, kQuad, kGuard>(s_nd, s_q, nn, xr, yr, zr, eps2, ax, ay, az);
+    gather_bodies<T>(s_bd, s_rstart, s_roff, nr, nq, lane, a, ox, oy, oz);
+    __syncthreads();
+    eval_bodies<T, kGuard>(s_bd, nq, xr, yr, zr, eps2, ax, ay, az);
+    cnt += (uint32_t)(nn + nq);
+    if (valid) {
+        const uint32_t o = a.tperm[i];
+        a.ax[o] = (double)ax;
+        a.ay[o] = (double)ay;
+        a.az[o] = (double)az;
+        if (a.counts) a.counts[o] = cnt;
+    }
+}
+
 // FP64 direct sum over all bodies (array order) for a list of targets; the
 // same softened monopole term, j != i by index.  pot = -sum m_j / sqrt(r^2+eps^2).
 __global__ __launch_bounds__(kBlock) void k_direct(const double* __restrict__ px, const double* __restrict__ py,
@@ -343,6 +591,11 @@ __global__ void k_diag_final(const double* __restrict__ part, int nparts, double
 }  // namespace
 
 hipError_t launch_walk(bhx_ctx* c, hipStream_t s) {
+    return launch_walk_to(c, s, c->cfg.group_walk != 0, c->cfg.precision, c->ax, c->ay, c->az, c->wcount);
+}
+
+hipError_t launch_walk_to(bhx_ctx* c, hipStream_t s, bool group, int precision, double* ox, double* oy, double* oz,
+                          uint32_t* counts) {
     if (c->n == 0) return hipSuccess;
     WalkArgs a;
     a.nodes = c->tree.walk;
@@ -356,12 +609,23 @@ hipError_t launch_walk(bhx_ctx* c, hipStream_t s) {
     a.eps2 = c->cfg.eps * c->cfg.eps;
     a.theta2f = (float)a.theta2;
     a.eps2f = (float)a.eps2;
-    a.ax = c->ax; a.ay = c->ay; a.az = c->az;
-    a.counts = c->wcount;
+    a.ax = ox; a.ay = oy; a.az = oz;
+    a.counts = counts;
     const unsigned groups = (unsigned)((c->n + 63) / 64);
     const unsigned grid = (groups + kWaves - 1) / kWaves;
     const bool quad = c->cfg.quadrupole != 0;
-    if (c->cfg.precision == BHX_FP32) {
+    if (group) {
+        const bool guard = !(a.eps2 > 0.0);
+#define BHX_GROUP(T, Q, G) hipLaunchKernelGGL((k_walk_group<T, Q, G>), dim3(groups), dim3(64), 0, s, a)
+        if (precision == BHX_FP32) {
+            if (quad) { if (guard) BHX_GROUP(float, true, true); else BHX_GROUP(float, true, false); }
+            else { if (guard) BHX_GROUP(float, false, true); else BHX_GROUP(float, false, false); }
+        } else {
+            if (quad) { if (guard) BHX_GROUP(double, true, true); else BHX_GROUP(double, true, false); }
+            else { if (guard) BHX_GROUP(double, false, true); else BHX_GROUP(double, false, false); }
+        }
+#undef BHX_GROUP
+    } else if (precision == BHX_FP32) {
         if (quad) hipLaunchKernelGGL(k_walk32<true>, dim3(grid), dim3(kBlock), 0, s, a);
         else hipLaunchKernelGGL(k_walk32<false>, dim3(grid), dim3(kBlock), 0, s, a);
     } else {

@@ -254,3 +254,65 @@ def test_diagnostics_momentum(bhx, plummer16k):
     assert ek == pytest.approx(0.5 * (mass * (vel ** 2).sum(1)).sum(), rel=1e-12)
     assert np.allclose(mom, (mass[:, None] * vel).sum(0), atol=1e-14)
     assert np.allclose(am, (mass[:, None] * np.cross(pos, vel)).sum(0), atol=1e-13)
+
+
+# ---------------------------------------------------------------- group walk (shared lists)
+
+@pytest.mark.parametrize("prec", [0, 1])
+def test_group_walk_theta_zero_is_direct_sum(bhx, prec):
+    pos, vel, mass = bhx.plummer(3000, seed=19)
+    c, s = orc.bounds(pos)
+    with _engine(bhx, 3000, theta=0.0, eps=0.01, group_walk=1, precision=prec, leaf_capacity=8, mode=1) as e:
+        e.set_bodies(pos, vel, mass)
+        e.build(c, s)
+        acc = e.walk()
+    ref, _ = orc.direct(pos, mass, 0.01)
+    err = np.linalg.norm(acc - ref, axis=1) / np.linalg.norm(ref, axis=1)
+    assert err.max() < (1e-12 if prec == 0 else 1e-4)      # FP64: summation order; FP32: rounding
+
+
+@pytest.mark.parametrize("prec", [0, 1])
+def test_group_walk_is_at_least_as_accurate(bhx, plummer16k, prec):
+    pos, vel, mass = plummer16k
+    c, s = orc.bounds(pos)
+    ref, _ = orc.direct(pos, mass, 0.01, targets=np.arange(0, N1, 8))
+    kw = dict(theta=0.5, eps=0.01, mode=1, quadrupole=1, leaf_capacity=32, precision=prec)
+    with _engine(bhx, N1, group_walk=1, **kw) as e:
+        e.set_bodies(pos, vel, mass)
+        e.build(c, s)
+        acc_g, cnt_g = e.walk(counts=True)
+        cnt_ref = e.interaction_counts()
+        acc_again = e.walk()
+    with _engine(bhx, N1, group_walk=0, **kw) as e:
+        e.set_bodies(pos, vel, mass)
+        e.build(c, s)
+        acc_b, cnt_b = e.walk(counts=True)
+    eg = np.linalg.norm(acc_g[::8] - ref, axis=1) / np.linalg.norm(ref, axis=1)
+    eb = np.linalg.norm(acc_b[::8] - ref, axis=1) / np.linalg.norm(ref, axis=1)
+    assert np.median(eg) < 1e-3
+    assert np.median(eg) <= 1.05 * np.median(eb)
+    assert np.array_equal(acc_g, acc_again)                    # deterministic, counts call left acc alone
+    t = orc.build_recursive(pos, mass, c, s, 32)
+    t.moments_intended()
+    _, ocnt = t.walk(0.5, 0.01, quad=True)
+    assert np.array_equal(cnt_ref.astype(np.int64), ocnt)     # reference list lengths
+    assert np.all(cnt_g.astype(np.int64) >= ocnt)              # shared lists refine every body's list
+
+
+def test_group_walk_step_energy(bhx):
+    n = 8192
+    pos, vel, mass = bhx.plummer(n, seed=23)
+    with _engine(bhx, n, theta=0.5, eps=0.01, dt=1e-3, mode=1, quadrupole=1, leaf_capacity=16,
+                 precision=1, group_walk=1, compensated=1) as e:
+        e.set_bodies(pos, vel, mass)
+        e.step(20)
+        p, v, m, a = e.get_bodies()
+    assert np.all(np.isfinite(p)) and np.all(np.isfinite(v))
+    assert np.array_equal(np.sort(m), np.sort(mass))
+    ek0 = 0.5 * (mass * (vel ** 2).sum(1)).sum()
+    ek1 = 0.5 * (m * (v ** 2).sum(1)).sum()
+    _, pot0 = orc.direct(pos, mass, 0.01)
+    _, pot1 = orc.direct(p, m, 0.01)
+    e0 = ek0 + 0.5 * (mass * pot0).sum()
+    e1 = ek1 + 0.5 * (m * pot1).sum()
+    assert abs(e1 - e0) / abs(e0) < 1e-4
