@@ -125,6 +125,43 @@ def cpu_baseline(pos, vel, mass, theta, eps, dt, cap, nsub, seed):
     }
 
 
+def copy_calibration(mib):
+    """One device-to-device hipMemcpy of `mib` MiB (the runtime's copy kernel), so that a
+    FETCH_SIZE / WRITE_SIZE counter pass can be scaled against a transfer of known size."""
+    import ctypes
+    hip = ctypes.CDLL("libamdhip64.so")
+    nbytes = mib << 20
+    src, dst = ctypes.c_void_p(), ctypes.c_void_p()
+    if hip.hipMalloc(ctypes.byref(src), ctypes.c_size_t(nbytes)) or hip.hipMalloc(ctypes.byref(dst), ctypes.c_size_t(nbytes)):
+        raise RuntimeError("hipMalloc failed")
+    hip.hipMemset(src, 1, ctypes.c_size_t(nbytes))
+    rc = hip.hipMemcpy(dst, src, ctypes.c_size_t(nbytes), 3)   # hipMemcpyDeviceToDevice
+    hip.hipDeviceSynchronize()
+    hip.hipFree(src)
+    hip.hipFree(dst)
+    if rc:
+        raise RuntimeError("hipMemcpy failed: %d" % rc)
+    log("copy calibration: %d MiB device-to-device" % mib)
+
+
+def read_traffic(n):
+    """HBM bytes per walk launch from the committed counter summary for this workload, if any."""
+    path = os.path.join(ROOT, "profiles", "walk_traffic_fp32_n%d.txt" % n)
+    if not os.path.exists(path):
+        return None
+    vals = {}
+    for line in open(path):
+        if ":" in line and not line.startswith("#"):
+            k, v = line.split(":", 1)
+            try:
+                vals[k.strip()] = float(v.split()[0])
+            except (ValueError, IndexError):
+                pass
+    if "hbm_bytes_per_launch" not in vals:
+        return None
+    return vals["hbm_bytes_per_launch"]
+
+
 def _cpu_model():
     try:
         for line in open("/proc/cpuinfo"):
@@ -145,6 +182,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-walk-sample", type=int, default=1 << 19)
     ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--calibrate-copy-mib", type=int, default=0,
+                    help="first copy this many MiB device-to-device (counter calibration runs)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build runs the whole Barnes-Hut step on a single MI355X "
@@ -152,6 +191,8 @@ def main():
         sys.exit(2)
 
     bhx = load_bhx()
+    if args.calibrate_copy_mib:
+        copy_calibration(args.calibrate_copy_mib)
     theta, eps, dt, cap = 0.5, 0.01, 5e-4, 32
     n = args.n
     log("generating Plummer N=%d seed=%d" % (n, args.seed))
@@ -203,7 +244,7 @@ def main():
                                "+tree keys+sort+build+hash+moments+walk+kick" % (n, theta, eps, cap, dt),
                    "n": n, "theta": theta, "eps": eps, "leaf_capacity": cap, "dt": dt},
         "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
-                     "frac": achieved / PEAK_FP32_TFLOPS, "traffic": None,
+                     "frac": achieved / PEAK_FP32_TFLOPS, "traffic": read_traffic(n),
                      "kernel": "k_walk_group<float, true, false>", "kernel_ms": walk_ms,
                      "work": "%d reference-list interactions x %d flops (group lists evaluate %d)" % (
                          out["fp32"]["inter_ref"], FLOPS_PER_INTERACTION, out["fp32"]["inter"])},

@@ -81,7 +81,7 @@ hipError_t dalloc(T** p, size_t count) {
 void free_tree(bhx_ctx* c) {
     DeviceTree& t = c->tree;
     void* ptrs[] = {t.key, t.start, t.count, t.skip, t.parent, t.child_byte, t.level, t.first, t.center,
-                    t.com, t.mass, t.quad, t.quadf, t.walk, t.hkey, t.hval};
+                    t.com, t.mass, t.quad, t.quadf, t.walk, t.walk32, t.hkey, t.hval};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     t = DeviceTree{};
@@ -105,6 +105,7 @@ hipError_t alloc_tree(bhx_ctx* c, size_t cap) {
     if ((e = dalloc(&t.quad, 6 * cap)) != hipSuccess) return e;
     if ((e = dalloc(&t.quadf, 6 * cap)) != hipSuccess) return e;
     if ((e = dalloc(&t.walk, cap)) != hipSuccess) return e;
+    if ((e = dalloc(&t.walk32, cap)) != hipSuccess) return e;
     size_t h = 1024;
     while (h < 2 * cap) h <<= 1;
     if ((e = dalloc(&t.hkey, h)) != hipSuccess) return e;
@@ -124,7 +125,7 @@ void free_all(bhx_ctx* c) {
     for (double*& p : c->alt)
         if (p) { (void)hipFree(p); p = nullptr; }
     void* vp[] = {c->keys, c->keys_alt, c->vals, c->vals_alt, c->mkeys, c->tkeys, c->tperm, c->nn, c->noff,
-                  c->nlo, c->scan_tmp, c->sort_hist, c->sort_status, c->sort_tile_ctr, c->wcount, c->ctr};
+                  c->nlo, c->scan_tmp, c->sort_hist, c->sort_status, c->sort_tile_ctr, c->wcount, c->ctr, c->tb4, c->tleaf};
     for (void* p : vp)
         if (p) (void)hipFree(p);
     if (c->ctr_host) (void)hipHostFree(c->ctr_host);
@@ -232,6 +233,8 @@ int bhx_create(const bhx_config* cfg, size_t capacity, bhx_ctx** out) {
         chk(dalloc(&c->mkeys, n), "alloc keys");
         chk(dalloc(&c->tkeys, n), "alloc keys");
         chk(dalloc(&c->tperm, n), "alloc perm");
+        chk(dalloc(&c->tb4, n), "alloc bodies");
+        chk(dalloc(&c->tleaf, n), "alloc bodies");
         chk(dalloc(&c->nn, n + 1), "alloc build");
         chk(dalloc(&c->noff, n + 1), "alloc build");
         chk(dalloc(&c->nlo, n + 1), "alloc build");

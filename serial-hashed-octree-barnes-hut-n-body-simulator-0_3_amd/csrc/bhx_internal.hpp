@@ -30,6 +30,18 @@ struct alignas(64) WalkNode {
 };
 static_assert(sizeof(WalkNode) == 64, "walk node is one 64-byte record");
 
+// FP32-walk record, also 64 bytes: the quadrupole travels with the node so an
+// accepted cell needs no second fetch; the cell edge follows from the level.
+struct alignas(64) WalkNode32 {
+    double cx, cy, cz;      // centre of mass (FP64: offsets are formed per group)
+    float mass;
+    float q[6];             // Qxx Qxy Qxz Qyy Qyz Qzz (0 for one-body cells)
+    uint32_t skip;
+    uint32_t start;         // first body in tree order
+    uint32_t meta;          // level (bits 0-4) | leaf (bit 5) | min(count, 2^26-1) << 6
+};
+static_assert(sizeof(WalkNode32) == 64, "FP32 walk node is one 64-byte record");
+
 struct DeviceTree {
     size_t cap = 0;          // node capacity
     uint64_t* key = nullptr;
@@ -46,6 +58,7 @@ struct DeviceTree {
     double* quad = nullptr;          // 6 per node
     float* quadf = nullptr;          // 6 per node, FP32 copy for the walk
     WalkNode* walk = nullptr;
+    WalkNode32* walk32 = nullptr;
     // hash table
     size_t hcap = 0;
     uint64_t* hkey = nullptr;
@@ -93,6 +106,8 @@ struct bhx_ctx {
     double *tx = nullptr, *ty = nullptr, *tz = nullptr, *tm = nullptr;
     uint64_t* tkeys = nullptr;                 // sorted tree path keys
     uint32_t* tperm = nullptr;                 // tree order -> array index
+    float4* tb4 = nullptr;                     // tree order: offset from the leaf's centre of mass, mass
+    uint32_t* tleaf = nullptr;                 // tree order: node id of the body's leaf
     // build scratch
     uint32_t *nn = nullptr, *noff = nullptr, *nlo = nullptr;
     uint32_t* scan_tmp = nullptr;
@@ -106,6 +121,7 @@ struct bhx_ctx {
     // bounds: two slots of (cx, cy, cz, size)
     double* bounds = nullptr;
     int bslot = 0;
+    int tree_slot = 0;                         // bounds slot the current tree was built with
     bool bounds_valid = false;
     // staging
     double* stage = nullptr;                   // 3 * cap doubles

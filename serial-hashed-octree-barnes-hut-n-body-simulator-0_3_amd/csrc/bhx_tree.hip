@@ -205,6 +205,9 @@ struct TreeArgs {
     double* quad;
     float* quadf;
     WalkNode* walk;
+    WalkNode32* walk32;
+    float4* tb4;
+    uint32_t* tleaf;
 };
 
 // Child cell geometry — HOTNode(parent, octant, ...) HashedNode.cpp:50-56.
@@ -432,6 +435,22 @@ __global__ __launch_bounds__(kBlock) void k_moments(int L, const uint32_t* __res
         w.size2f = (float)(s * s);
         w.massf = (float)M;
         t.walk[p] = w;
+        WalkNode32 w32;
+        w32.cx = cx; w32.cy = cy; w32.cz = cz;
+        w32.mass = (float)M;
+        for (int k = 0; k < 6; ++k) w32.q[k] = cnt > 1 ? (float)q[k] : 0.0f;
+        w32.skip = t.skip[p];
+        w32.start = st;
+        w32.meta = (uint32_t)L | (leaf ? 32u : 0u) | ((cnt < (1u << 26) ? cnt : (1u << 26) - 1) << 6);
+        t.walk32[p] = w32;
+        if (leaf) {
+            // compact FP32 copy of the leaf's bodies: offsets from the leaf's
+            // centre of mass (which the walk record carries), and the leaf id
+            for (uint32_t j = st; j < st + cnt; ++j) {
+                t.tb4[j] = make_float4((float)(tx[j] - cx), (float)(ty[j] - cy), (float)(tz[j] - cz), (float)tm[j]);
+                t.tleaf[j] = p;
+            }
+        }
     }
 }
 
@@ -441,6 +460,9 @@ TreeArgs tree_args(bhx_ctx* c) {
     t.key = d.key; t.start = d.start; t.count = d.count; t.skip = d.skip; t.parent = d.parent;
     t.child_byte = d.child_byte; t.level = d.level; t.first = d.first; t.center = d.center;
     t.com = d.com; t.mass = d.mass; t.quad = d.quad; t.quadf = d.quadf; t.walk = d.walk;
+    t.walk32 = d.walk32;
+    t.tb4 = c->tb4;
+    t.tleaf = c->tleaf;
     return t;
 }
 
@@ -465,6 +487,7 @@ hipError_t build_tree(bhx_ctx* c, hipStream_t s, size_t* total_out) {
     const uint32_t n = (uint32_t)c->n;
     const int B = c->cfg.leaf_capacity;
     const double* b4 = c->bounds + 4 * c->bslot;
+    c->tree_slot = c->bslot;
     uint32_t* total_dev = &c->ctr->total_nodes;
     hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, s, &c->ctr->unsplit, (size_t)2, 0u);  // unsplit, hash_fail
     hipLaunchKernelGGL(k_count, dim3(grid_for(n)), dim3(kBlock), 0, s, c->tkeys, n, B, c->nn, c->nlo);

@@ -49,6 +49,11 @@ __device__ __forceinline__ WalkNode load_node(ConstU64 raw, uint32_t p) {
 
 struct WalkArgs {
     const WalkNode* nodes;
+    const WalkNode32* nodes32;
+    const float4* tb4;       // tree order: offset from the leaf's centre of mass, mass
+    const uint32_t* tleaf;   // tree order: node id of the body's leaf
+    uint32_t groups;         // 64-body groups (grid is rounded up to a multiple of 8)
+    const double* b4;        // root bounds (cx, cy, cz, size) of the tree
     const double* quad;      // 6 per node (FP64)
     const float* quadf;      // 6 per node (FP32)
     const double *tx, *ty, *tz, *tm;
@@ -514,6 +519,201 @@ __global__ __launch_bounds__(64) void k_walk_group(WalkArgs a) {
     }
 }
 
+// FP32 group walk over the compact layout: 64-byte FP32 records carrying the
+// quadrupole (one scalar burst per visited cell, nothing more to fetch when a
+// cell is accepted), cell edges from the level, and opened leaves gathered as
+// 16-byte offsets from the leaf's centre of mass plus that centre's offset
+// from the group origin, which the opening test has just formed.  Every lane
+// builds its own position the same way (its leaf's centre offset + its own
+// offset), so a body meets itself at exactly d = 0.  Same shared-list rule
+// as k_walk_group.
+//
+// Work distribution: consecutive workgroups land on the 8 XCDs in turn, each
+// with its own L2.  Group g is given to block (g mod per) * 8 + g / per, so
+// XCD x walks the contiguous tree-order slice of groups [x*per, (x+1)*per):
+// its L2 holds that region's cells instead of the whole tree.
+typedef const __attribute__((address_space(4))) WalkNode32* ConstNodes32;
+
+__device__ __forceinline__ uint32_t xcd_group(uint32_t block, uint32_t groups) {
+    const uint32_t per = (groups + 7u) / 8u;
+    return (block % 8u) * per + block / 8u;
+}
+
+struct Node32 {
+    double cx, cy, cz;
+    float mass, q0, q1, q2, q3, q4, q5;
+    uint32_t skip, start, meta;
+};
+
+__device__ __forceinline__ Node32 load_node32(ConstU64 raw, uint32_t p) {
+    const size_t b = 8 * (size_t)p;
+    const unsigned long long w0 = raw[b], w1 = raw[b + 1], w2 = raw[b + 2], w3 = raw[b + 3];
+    const unsigned long long w4 = raw[b + 4], w5 = raw[b + 5], w6 = raw[b + 6], w7 = raw[b + 7];
+    Node32 nd;
+    nd.cx = __longlong_as_double((long long)w0);
+    nd.cy = __longlong_as_double((long long)w1);
+    nd.cz = __longlong_as_double((long long)w2);
+    nd.mass = __uint_as_float((uint32_t)w3);
+    nd.q0 = __uint_as_float((uint32_t)(w3 >> 32));
+    nd.q1 = __uint_as_float((uint32_t)w4);
+    nd.q2 = __uint_as_float((uint32_t)(w4 >> 32));
+    nd.q3 = __uint_as_float((uint32_t)w5);
+    nd.q4 = __uint_as_float((uint32_t)(w5 >> 32));
+    nd.q5 = __uint_as_float((uint32_t)w6);
+    nd.skip = (uint32_t)(w6 >> 32);
+    nd.start = (uint32_t)w7;
+    nd.meta = (uint32_t)(w7 >> 32);
+    return nd;
+}
+
+template <bool kQuad, bool kGuard>
+__global__ __launch_bounds__(64) void k_walk_group32(WalkArgs a) {
+    __shared__ float s_nd[kNodeCap][4];
+    __shared__ float s_q[kQuad ? kNodeCap : 1][6];
+    __shared__ float s_bd[kBodyCap][4];
+    __shared__ uint32_t s_rstart[kRangeCap], s_roff[kRangeCap];
+    __shared__ float s_rdel[kRangeCap][3];    // leaf's first body minus the group origin
+
+    const uint32_t grp = xcd_group(blockIdx.x, a.groups);
+    if (grp >= a.groups) return;
+    const uint32_t g0 = grp * 64u;
+    const uint32_t lane = threadIdx.x;
+    const uint32_t i = g0 + lane;
+    const bool valid = i < a.n;
+    const uint32_t iv = valid ? i : g0;
+    ConstU64 raw = (ConstU64)a.nodes32;
+    ConstD tx = (ConstD)a.tx;
+    ConstD ty = (ConstD)a.ty;
+    ConstD tz = (ConstD)a.tz;
+    const double ox = tx[g0], oy = ty[g0], oz = tz[g0];
+    // own position = (own leaf's centre - origin) + own offset: the same two
+    // roundings the body gets as a list entry
+    const uint32_t own = a.tleaf[iv];
+    const float4 ob = a.tb4[iv];
+    const float xr = (float)(a.nodes32[own].cx - ox) + ob.x;
+    const float yr = (float)(a.nodes32[own].cy - oy) + ob.y;
+    const float zr = (float)(a.nodes32[own].cz - oz) + ob.z;
+    const float eps2 = a.eps2f, th2 = a.theta2f;
+    const double s0 = a.b4[3];
+    const float s2root = (float)(s0 * s0);
+    float ax = 0.f, ay = 0.f, az = 0.f;
+    uint32_t cnt = 0;
+    int nn = 0, nr = 0, nq = 0;
+    const Node32 root = load_node32(raw, 0);
+    const uint32_t end = root.skip;
+    uint32_t p = (root.meta & 32u) ? 0u : 1u;
+    while (p < end) {
+        p = __builtin_amdgcn_readfirstlane(p);
+        const Node32 nd = load_node32(raw, p);
+        const uint32_t nskip = nd.skip > p ? nd.skip : p + 1;   // always forward
+        const uint32_t L = nd.meta & 31u;
+        const bool leaf = (nd.meta & 32u) != 0;
+        const float cx = (float)(nd.cx - ox), cy = (float)(nd.cy - oy), cz = (float)(nd.cz - oz);
+        const float dx = cx - xr, dy = cy - yr, dz = cz - zr;
+        const float d2 = dx * dx + dy * dy + dz * dz;
+        // edge^2 = size^2 / 4^L, exact power-of-two scaling of the rounded root value
+        const float s2 = ldexpf(s2root, -2 * (int)L);
+        const bool pass = (L != 0) & (s2 < th2 * d2);
+        const bool opened = __ballot(valid & !pass) != 0ull;
+        if (!opened) {
+            if (nn == kNodeCap) {
+                __syncthreads();
+                eval_nodes<float, kQuad, kGuard>(s_nd, s_q, nn, xr, yr, zr, eps2, ax, ay, az);
+                cnt += (uint32_t)nn;
+                nn = 0;
+                __syncthreads();
+            }
+            if (lane == 0) {
+                s_nd[nn][0] = cx;
+                s_nd[nn][1] = cy;
+                s_nd[nn][2] = cz;
+                s_nd[nn][3] = nd.mass;
+                if (kQuad) {
+                    s_q[nn][0] = nd.q0;
+                    s_q[nn][1] = nd.q1;
+                    s_q[nn][2] = nd.q2;
+                    s_q[nn][3] = nd.q3;
+                    s_q[nn][4] = nd.q4;
+                    s_q[nn][5] = nd.q5;
+                }
+            }
+            ++nn;
+            p = nskip;
+        } else if (leaf) {
+            const uint32_t count = nd.meta >> 6;
+            const float rx = cx, ry = cy, rz = cz;     // the leaf's centre relative to the group origin
+            for (uint32_t k = 0; k < count; k += 64) {
+                const uint32_t m = (count - k) < 64u ? (count - k) : 64u;
+                if (nq + (int)m > kBodyCap || nr == kRangeCap) {
+                    __syncthreads();
+#pragma unroll
+                    for (int h = 0; h < kBodyCap / 64; ++h) {
+                        const int e = (int)lane + 64 * h;
+                        if (e < nq) {
+                            int lo = 0, hi = nr - 1;
+                            while (lo < hi) {
+                                const int mid = (lo + hi + 1) >> 1;
+                                if ((int)s_roff[mid] <= e) lo = mid; else hi = mid - 1;
+                            }
+                            const float4 b = a.tb4[s_rstart[lo] + (uint32_t)(e - (int)s_roff[lo])];
+                            s_bd[e][0] = s_rdel[lo][0] + b.x;
+                            s_bd[e][1] = s_rdel[lo][1] + b.y;
+                            s_bd[e][2] = s_rdel[lo][2] + b.z;
+                            s_bd[e][3] = b.w;
+                        }
+                    }
+                    __syncthreads();
+                    eval_bodies<float, kGuard>(s_bd, nq, xr, yr, zr, eps2, ax, ay, az);
+                    cnt += (uint32_t)nq;
+                    nr = 0;
+                    nq = 0;
+                    __syncthreads();
+                }
+                if (lane == 0) {
+                    s_rstart[nr] = nd.start + k;
+                    s_roff[nr] = (uint32_t)nq;
+                    s_rdel[nr][0] = rx;
+                    s_rdel[nr][1] = ry;
+                    s_rdel[nr][2] = rz;
+                }
+                ++nr;
+                nq += (int)m;
+            }
+            p = nskip;
+        } else {
+            p = p + 1;
+        }
+    }
+    __syncthreads();
+    eval_nodes<float, kQuad, kGuard>(s_nd, s_q, nn, xr, yr, zr, eps2, ax, ay, az);
+#pragma unroll
+    for (int h = 0; h < kBodyCap / 64; ++h) {
+        const int e = (int)lane + 64 * h;
+        if (e < nq) {
+            int lo = 0, hi = nr - 1;
+            while (lo < hi) {
+                const int mid = (lo + hi + 1) >> 1;
+                if ((int)s_roff[mid] <= e) lo = mid; else hi = mid - 1;
+            }
+            const float4 b = a.tb4[s_rstart[lo] + (uint32_t)(e - (int)s_roff[lo])];
+            s_bd[e][0] = s_rdel[lo][0] + b.x;
+            s_bd[e][1] = s_rdel[lo][1] + b.y;
+            s_bd[e][2] = s_rdel[lo][2] + b.z;
+            s_bd[e][3] = b.w;
+        }
+    }
+    __syncthreads();
+    eval_bodies<float, kGuard>(s_bd, nq, xr, yr, zr, eps2, ax, ay, az);
+    cnt += (uint32_t)(nn + nq);
+    if (valid) {
+        const uint32_t o = a.tperm[i];
+        a.ax[o] = (double)ax;
+        a.ay[o] = (double)ay;
+        a.az[o] = (double)az;
+        if (a.counts) a.counts[o] = cnt;
+    }
+}
+
 // FP64 direct sum over all bodies (array order) for a list of targets; the
 // same softened monopole term, j != i by index.  pot = -sum m_j / sqrt(r^2+eps^2).
 __global__ __launch_bounds__(kBlock) void k_direct(const double* __restrict__ px, const double* __restrict__ py,
@@ -599,6 +799,11 @@ hipError_t launch_walk_to(bhx_ctx* c, hipStream_t s, bool group, int precision, 
     if (c->n == 0) return hipSuccess;
     WalkArgs a;
     a.nodes = c->tree.walk;
+    a.nodes32 = c->tree.walk32;
+    a.tb4 = c->tb4;
+    a.tleaf = c->tleaf;
+    a.groups = (uint32_t)((c->n + 63) / 64);
+    a.b4 = c->bounds + 4 * c->tree_slot;
     a.quad = c->tree.quad;
     a.quadf = c->tree.quadf;
     a.tx = c->tx; a.ty = c->ty; a.tz = c->tz; a.tm = c->tm;
@@ -618,8 +823,10 @@ hipError_t launch_walk_to(bhx_ctx* c, hipStream_t s, bool group, int precision, 
         const bool guard = !(a.eps2 > 0.0);
 #define BHX_GROUP(T, Q, G) hipLaunchKernelGGL((k_walk_group<T, Q, G>), dim3(groups), dim3(64), 0, s, a)
         if (precision == BHX_FP32) {
-            if (quad) { if (guard) BHX_GROUP(float, true, true); else BHX_GROUP(float, true, false); }
-            else { if (guard) BHX_GROUP(float, false, true); else BHX_GROUP(float, false, false); }
+#define BHX_GROUP32(Q, G) hipLaunchKernelGGL((k_walk_group32<Q, G>), dim3(8 * ((groups + 7) / 8)), dim3(64), 0, s, a)
+            if (quad) { if (guard) BHX_GROUP32(true, true); else BHX_GROUP32(true, false); }
+            else { if (guard) BHX_GROUP32(false, true); else BHX_GROUP32(false, false); }
+#undef BHX_GROUP32
         } else {
             if (quad) { if (guard) BHX_GROUP(double, true, true); else BHX_GROUP(double, true, false); }
             else { if (guard) BHX_GROUP(double, false, true); else BHX_GROUP(double, false, false); }
