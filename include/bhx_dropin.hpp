@@ -51,6 +51,12 @@ public:
 };
 static_assert(sizeof(Vec3D) == 3 * sizeof(double), "Vec3D is three packed doubles");
 
+// ResetVec3D (MathUtils.hpp:81-93): zero an array of vectors; null throws.
+static inline void ResetVec3D(Vec3D*& vec3d, int numVecs) {
+    if (!vec3d) throw std::invalid_argument("vec3d is a null pointer in 'ResetVec3D'.");
+    for (int i = 0; i < numVecs; ++i) vec3d[i].reset();
+}
+
 // Body (Body.hpp:30-65).
 class Body {
 public:

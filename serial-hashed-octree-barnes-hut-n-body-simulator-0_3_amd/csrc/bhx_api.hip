@@ -148,9 +148,11 @@ int check_cfg(const bhx_config* cfg) {
 // tree keys -> sort -> tree-order copies -> node set, hash, moments
 int build_on_device(bhx_ctx* c, hipStream_t s) {
     c->tree_valid = false;
+    c->tree_built = false;
     if (c->n == 0) {
         c->n_nodes = 0;
-        c->tree_valid = true;
+        c->tree_valid = c->tree_built = true;
+        c->tree_n = 0;
         return BHX_OK;
     }
     const double* b4 = c->bounds + 4 * c->bslot;
@@ -167,7 +169,8 @@ int build_on_device(bhx_ctx* c, hipStream_t s) {
     }
     if (total > c->tree.cap) return fail(c, BHX_E_NOMEM, "node capacity");
     c->n_nodes = total;
-    c->tree_valid = true;
+    c->tree_valid = c->tree_built = true;
+    c->tree_n = c->n;
     return BHX_OK;
 }
 
@@ -523,7 +526,7 @@ int bhx_build(bhx_ctx* c, const double center[3], double size, size_t* n_nodes) 
 
 int bhx_get_nodes(bhx_ctx* c, bhx_node* out, size_t cap, size_t* n_nodes) {
     if (!c) return BHX_E_NULL;
-    if (!c->tree_valid) return fail(c, BHX_E_STATE, "no tree built");
+    if (!c->tree_built) return fail(c, BHX_E_STATE, "no tree built");
     if (n_nodes) *n_nodes = c->n_nodes;
     if (!out) return BHX_OK;
     const size_t m = c->n_nodes < cap ? c->n_nodes : cap;
@@ -567,7 +570,7 @@ int bhx_get_nodes(bhx_ctx* c, bhx_node* out, size_t cap, size_t* n_nodes) {
 
 int bhx_lookup(bhx_ctx* c, const uint64_t* keys, size_t n, int64_t* ids) {
     if (!c || !keys || !ids) return c ? fail(c, BHX_E_NULL, "null argument") : BHX_E_NULL;
-    if (!c->tree_valid) return fail(c, BHX_E_STATE, "no tree built");
+    if (!c->tree_built) return fail(c, BHX_E_STATE, "no tree built");
     if (n == 0) return BHX_OK;
     if (c->n_nodes == 0) {
         for (size_t i = 0; i < n; ++i) ids[i] = -1;
@@ -591,17 +594,18 @@ int bhx_lookup(bhx_ctx* c, const uint64_t* keys, size_t n, int64_t* ids) {
 
 int bhx_get_tree_order(bhx_ctx* c, uint64_t* tree_perm, uint64_t* tree_keys) {
     if (!c) return BHX_E_NULL;
-    if (!c->tree_valid) return fail(c, BHX_E_STATE, "no tree built");
+    if (!c->tree_built) return fail(c, BHX_E_STATE, "no tree built");
     HIPCHK(hipSetDevice(c->device));
     hipStream_t s = c->s_main;
-    if (c->n == 0) return BHX_OK;
+    const size_t n = c->tree_n;
+    if (n == 0) return BHX_OK;
     if (tree_perm) {
         uint64_t* tmp = reinterpret_cast<uint64_t*>(c->stage);
-        hipLaunchKernelGGL(k_u32_to_u64, dim3(grid_for(c->n)), dim3(kBlock), 0, s, c->tperm, c->n, tmp);
-        HIPCHK(hipMemcpyAsync(tree_perm, tmp, c->n * 8, hipMemcpyDeviceToHost, s));
+        hipLaunchKernelGGL(k_u32_to_u64, dim3(grid_for(n)), dim3(kBlock), 0, s, c->tperm, n, tmp);
+        HIPCHK(hipMemcpyAsync(tree_perm, tmp, n * 8, hipMemcpyDeviceToHost, s));
         HIPCHK(hipStreamSynchronize(s));
     }
-    if (tree_keys) HIPCHK(hipMemcpyAsync(tree_keys, c->tkeys, c->n * 8, hipMemcpyDeviceToHost, s));
+    if (tree_keys) HIPCHK(hipMemcpyAsync(tree_keys, c->tkeys, n * 8, hipMemcpyDeviceToHost, s));
     HIPCHK(hipStreamSynchronize(s));
     return BHX_OK;
 }
@@ -701,6 +705,7 @@ int bhx_step(bhx_ctx* c, int nsteps) {
         HIPCHK(launch_bounds_size(c->px, c->py, c->pz, c->n, b_next, c->s_side));
         HIPCHK(hipEventRecord(c->ev_bounds, c->s_side));
         // tree on x_{f+1} with the bounds of x_f
+        c->tree_valid = c->tree_built = false;
         HIPCHK(launch_treekeys(c->px, c->py, c->pz, c->n, b_cur, c->tkeys, c->tperm, s));
         HIPCHK(radix_sort_pairs(c->tkeys, c->tperm, c->keys_alt, c->vals_alt, c->n, c->sort_hist, c->sort_status,
                                 c->sort_tile_ctr, c->ctr, s));
@@ -715,7 +720,8 @@ int bhx_step(bhx_ctx* c, int nsteps) {
         }
         if (total > c->tree.cap) return fail(c, BHX_E_NOMEM, "node capacity");
         c->n_nodes = total;
-        c->tree_valid = true;
+        c->tree_valid = c->tree_built = true;
+        c->tree_n = c->n;
         if (prof) HIPCHK(hipEventRecord(c->ev[6], s));
         HIPCHK(launch_walk(c, s));
         if (prof) HIPCHK(hipEventRecord(c->ev[7], s));

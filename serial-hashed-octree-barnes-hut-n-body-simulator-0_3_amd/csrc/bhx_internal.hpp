@@ -129,7 +129,9 @@ struct bhx_ctx {
     bhx::Counters* ctr_host = nullptr;         // pinned
     bhx::DeviceTree tree;
     size_t n_nodes = 0;
-    bool tree_valid = false;
+    bool tree_valid = false;   // tree built on the current bodies (walk allowed)
+    bool tree_built = false;   // node table of the last build still intact (downloads, lookups)
+    size_t tree_n = 0;         // bodies the last build covered
     double tree_center[3] = {0, 0, 0};
     double tree_size = 0;
 };

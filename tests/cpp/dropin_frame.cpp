@@ -45,11 +45,12 @@ int main(int argc, char** argv) {
         buildLinearHashedOctreeInPlace(tree, bodies, n, rootNodeBounds);
         HOTNode** walkList = new HOTNode*[n];
         HOTNode** interactList = new HOTNode*[n];
-        for (size_t i = 0; i < n; ++i) acc[i].reset();
+        ResetVec3D(acc, (int)n);
         ComputeHOTOctreeForce(tree, bodies, acc, n, walkList, interactList, theta);
         ComputeVelocityAndPosition(dt, bodies, n, acc);
         delete[] walkList;
         delete[] interactList;
+        if (f + 1 < frames) tree.deleteTree();
     }
     // tree checks: the root and one of its children resolve, a missing key does not
     const HOTNode* root = tree.lookUpNode(ROOT_KEY);
