@@ -179,9 +179,34 @@ int bhx_last_timing(bhx_ctx *ctx, bhx_timing *t);
 /* 1: time each phase with events in bhx_step (adds synchronisation). */
 int bhx_set_profiling(bhx_ctx *ctx, int on);
 
+/* ---- ensembles of independent systems (no reference counterpart) -------- */
+
+/* Splits the resident bodies into nsys independent systems: system k is the
+ * contiguous range [offsets[k], offsets[k+1]) (offsets[0] = 0,
+ * offsets[nsys] = n, every system non-empty, nsys <= 65536).  From then on
+ * bounds, keys, sorts, trees and walks are per system — each system steps
+ * exactly as it would alone in its own context — and bodies never leave
+ * their system's range.  bhx_set_bodies resets the context to one system. */
+int bhx_set_systems(bhx_ctx *ctx, const uint64_t *offsets, size_t nsys);
+int bhx_num_systems(const bhx_ctx *ctx, size_t *nsys);
+
+/* OctantBounds of every system (centers: 3 per system, sizes: 1 per system;
+ * either may be NULL). */
+int bhx_system_bounds(bhx_ctx *ctx, double *centers, double *sizes);
+
+/* bhx_build with one root box per system. */
+int bhx_build_systems(bhx_ctx *ctx, const double *centers, const double *sizes, size_t *n_nodes);
+
+/* Pre-order index of every system's root in the last tree. */
+int bhx_system_roots(bhx_ctx *ctx, uint64_t *roots);
+
+/* bhx_lookup in one system's tree (bhx_lookup is system 0). */
+int bhx_lookup_in(bhx_ctx *ctx, size_t system, const uint64_t *keys, size_t n, int64_t *ids);
+
 /* ---- diagnostics (no reference counterpart) ------------------------------ */
 
-/* FP64 direct sum over all bodies for the targets (array indices). */
+/* FP64 direct sum over the target's own system for each target (array
+ * indices). */
 int bhx_direct(bhx_ctx *ctx, const int64_t *targets, size_t nt, double eps,
                double *acc_out, double *pot_out);
 

@@ -70,6 +70,8 @@ def lib():
             "bhx_step": [P, I], "bhx_last_timing": [P, P], "bhx_set_profiling": [P, I],
             "bhx_direct": [P, P, S, D, P, P], "bhx_diagnostics": [P, P, P, P],
             "bhx_ic_plummer": [S, ctypes.c_uint64, D, D, P, P, P],
+            "bhx_set_systems": [P, P, S], "bhx_num_systems": [P, P], "bhx_system_bounds": [P, P, P],
+            "bhx_build_systems": [P, P, P, P], "bhx_system_roots": [P, P], "bhx_lookup_in": [P, S, P, S, P],
         }
         for name, args in sigs.items():
             f = getattr(L, name)
@@ -228,6 +230,42 @@ class Engine:
         keys = np.ascontiguousarray(keys, dtype=np.uint64)
         ids = np.empty(keys.size, np.int64)
         self._chk(lib().bhx_lookup(self.h, _p(keys), keys.size, _p(ids)), "bhx_lookup")
+        return ids
+
+    # --- ensembles of independent systems
+    def set_systems(self, offsets):
+        off = np.ascontiguousarray(offsets, dtype=np.uint64)
+        self._chk(lib().bhx_set_systems(self.h, _p(off), off.size - 1), "bhx_set_systems")
+
+    @property
+    def nsys(self):
+        v = ctypes.c_size_t()
+        self._chk(lib().bhx_num_systems(self.h, ctypes.byref(v)), "bhx_num_systems")
+        return v.value
+
+    def system_bounds(self):
+        k = self.nsys
+        c = np.empty((k, 3))
+        sz = np.empty(k)
+        self._chk(lib().bhx_system_bounds(self.h, _p(c), _p(sz)), "bhx_system_bounds")
+        return c, sz
+
+    def build_systems(self, centers, sizes):
+        c = _f64(centers)
+        sz = _f64(sizes)
+        nn = ctypes.c_size_t()
+        self._chk(lib().bhx_build_systems(self.h, _p(c), _p(sz), ctypes.byref(nn)), "bhx_build_systems")
+        return nn.value
+
+    def system_roots(self):
+        r = np.empty(self.nsys, np.uint64)
+        self._chk(lib().bhx_system_roots(self.h, _p(r)), "bhx_system_roots")
+        return r
+
+    def lookup_in(self, system, keys):
+        keys = np.ascontiguousarray(keys, dtype=np.uint64)
+        ids = np.empty(keys.size, np.int64)
+        self._chk(lib().bhx_lookup_in(self.h, system, _p(keys), keys.size, _p(ids)), "bhx_lookup_in")
         return ids
 
     def tree_order(self):

@@ -1,4 +1,5 @@
 // bhx_api.hip — context management and the C ABI (include/bhx.h).
+#include <algorithm>
 #include <chrono>
 #include <cmath>
 #include <cstring>
@@ -125,7 +126,8 @@ void free_all(bhx_ctx* c) {
     for (double*& p : c->alt)
         if (p) { (void)hipFree(p); p = nullptr; }
     void* vp[] = {c->keys, c->keys_alt, c->vals, c->vals_alt, c->mkeys, c->tkeys, c->tperm, c->nn, c->noff,
-                  c->nlo, c->scan_tmp, c->sort_hist, c->sort_status, c->sort_tile_ctr, c->wcount, c->ctr, c->tb4, c->tleaf};
+                  c->nlo, c->scan_tmp, c->sort_hist, c->sort_status, c->sort_tile_ctr, c->wcount, c->ctr, c->tb4, c->tleaf,
+                  c->sys_off, c->sysid, c->sys_root, c->gtab, c->skey, c->skey_alt, c->sidx, c->sidx_alt};
     for (void* p : vp)
         if (p) (void)hipFree(p);
     if (c->ctr_host) (void)hipHostFree(c->ctr_host);
@@ -145,6 +147,86 @@ int check_cfg(const bhx_config* cfg) {
     return BHX_OK;
 }
 
+// Per-system buffers for `nsys` systems (bounds slots, offsets, roots).
+hipError_t ensure_sys_cap(bhx_ctx* c, size_t nsys) {
+    if (nsys <= c->sys_cap && c->bounds) return hipSuccess;
+    size_t cap = c->sys_cap ? c->sys_cap : 1;
+    while (cap < nsys) cap *= 2;
+    double* nb = nullptr;
+    uint32_t *no = nullptr, *nr = nullptr;
+    hipError_t e = dalloc(&nb, 12 * cap);
+    if (e == hipSuccess) e = dalloc(&no, cap + 1);
+    if (e == hipSuccess) e = dalloc(&nr, cap);
+    if (e != hipSuccess) {
+        if (nb) (void)hipFree(nb);
+        if (no) (void)hipFree(no);
+        return e;
+    }
+    if (c->bounds) (void)hipFree(c->bounds);
+    if (c->sys_off) (void)hipFree(c->sys_off);
+    if (c->sys_root) (void)hipFree(c->sys_root);
+    c->bounds = nb;
+    c->sys_off = no;
+    c->sys_root = nr;
+    c->sys_cap = cap;
+    c->bounds_valid = false;
+    return hipSuccess;
+}
+
+// Splits the bodies into systems by `off` (nsys + 1 offsets, first 0, last n,
+// strictly increasing) and lays out the walk groups: runs of up to 64 bodies
+// that start at each system's first body.
+int set_layout(bhx_ctx* c, const std::vector<uint32_t>& off) {
+    const uint32_t nsys = (uint32_t)(off.size() - 1);
+    HIPCHK(ensure_sys_cap(c, nsys));
+    std::vector<uint4> groups;
+    uint32_t maxn = 0;
+    for (uint32_t k = 0; k < nsys; ++k) {
+        const uint32_t a = off[k], b = off[k + 1];
+        if (b - a > maxn) maxn = b - a;
+        for (uint32_t g = a; g < b; g += 64) groups.push_back(make_uint4(g, g + 64 < b ? g + 64 : b, k, 0u));
+    }
+    const size_t gcap = c->cap / 64 + 1 + nsys;
+    if (!c->gtab || c->gtab_cap < gcap) {
+        if (c->gtab) (void)hipFree(c->gtab);
+        c->gtab = nullptr;
+        c->gtab_cap = 0;
+        HIPCHK(dalloc(&c->gtab, gcap));
+        c->gtab_cap = gcap;
+    }
+    hipStream_t s = c->s_main;
+    HIPCHK(hipStreamSynchronize(c->s_side));
+    HIPCHK(hipMemcpyAsync(c->sys_off, off.data(), off.size() * sizeof(uint32_t), hipMemcpyHostToDevice, s));
+    if (!groups.empty())
+        HIPCHK(hipMemcpyAsync(c->gtab, groups.data(), groups.size() * sizeof(uint4), hipMemcpyHostToDevice, s));
+    if (nsys > 1) {
+        if (!c->sysid) {
+            HIPCHK(dalloc(&c->sysid, c->cap));
+            HIPCHK(dalloc(&c->skey, c->cap));
+            HIPCHK(dalloc(&c->skey_alt, c->cap));
+            HIPCHK(dalloc(&c->sidx, c->cap));
+            HIPCHK(dalloc(&c->sidx_alt, c->cap));
+        }
+        std::vector<uint32_t> id(off[nsys]);
+        for (uint32_t k = 0; k < nsys; ++k)
+            for (uint32_t i = off[k]; i < off[k + 1]; ++i) id[i] = k;
+        if (!id.empty())
+            HIPCHK(hipMemcpyAsync(c->sysid, id.data(), id.size() * sizeof(uint32_t), hipMemcpyHostToDevice, s));
+        HIPCHK(hipStreamSynchronize(s));
+    } else {
+        HIPCHK(hipStreamSynchronize(s));
+    }
+    if (off != c->sys_off_h) c->tree_built = false;
+    c->sys_off_h = off;
+    c->nsys = nsys;
+    c->max_sys_n = maxn;
+    c->ngroups = (uint32_t)groups.size();
+    c->bounds_valid = false;
+    c->tree_valid = false;
+    c->graph_valid = false;
+    return BHX_OK;
+}
+
 // tree keys -> sort -> tree-order copies -> node set, hash, moments
 int build_on_device(bhx_ctx* c, hipStream_t s) {
     c->tree_valid = false;
@@ -155,14 +237,13 @@ int build_on_device(bhx_ctx* c, hipStream_t s) {
         c->tree_n = 0;
         return BHX_OK;
     }
-    const double* b4 = c->bounds + 4 * c->bslot;
-    HIPCHK(launch_treekeys(c->px, c->py, c->pz, c->n, b4, c->tkeys, c->tperm, s));
-    HIPCHK(radix_sort_pairs(c->tkeys, c->tperm, c->keys_alt, c->vals_alt, c->n, c->sort_hist, c->sort_status,
-                            c->sort_tile_ctr, c->ctr, s));
+    const double* b4 = c->bounds_slot(c->bslot);
+    HIPCHK(launch_treekeys(c->px, c->py, c->pz, c->n, c->sys().id, b4, c->tkeys, c->tperm, s));
+    HIPCHK(sort_pairs(c, c->tkeys, c->tperm, s));
     HIPCHK(launch_tree_gather(c, s));
     size_t total = 0;
     for (int attempt = 0; attempt < 2; ++attempt) {
-        HIPCHK(build_tree(c, s, &total));
+        HIPCHK(build_tree(c, s, &total, true));
         if (total <= c->tree.cap) break;
         HIPCHK(hipStreamSynchronize(s));
         HIPCHK(alloc_tree(c, total + total / 4 + 1024));
@@ -228,7 +309,6 @@ int bhx_create(const bhx_config* cfg, size_t capacity, bhx_ctx** out) {
         for (double** p : dp) chk(dalloc(p, n), "alloc bodies");
         for (double*& p : c->alt) chk(dalloc(&p, n), "alloc bodies");
         chk(dalloc(&c->stage, 3 * n + 8 * 256), "alloc stage");
-        chk(dalloc(&c->bounds, 16), "alloc bounds");
         chk(dalloc(&c->keys, n), "alloc keys");
         chk(dalloc(&c->keys_alt, n), "alloc keys");
         chk(dalloc(&c->vals, n), "alloc vals");
@@ -250,6 +330,7 @@ int bhx_create(const bhx_config* cfg, size_t capacity, bhx_ctx** out) {
         chk(dalloc(&c->ctr, 1), "alloc counters");
         chk(hipHostMalloc((void**)&c->ctr_host, sizeof(Counters), hipHostMallocDefault), "alloc pinned");
         if (rc == BHX_OK) chk(alloc_tree(c, 2 * n + 1024), "alloc tree");
+        if (rc == BHX_OK) rc = set_layout(c, std::vector<uint32_t>{0u, 0u});
         if (rc == BHX_OK) chk(hipMemsetAsync(c->ctr, 0, sizeof(Counters), c->s_main), "memset");
         if (rc == BHX_OK) chk(hipStreamSynchronize(c->s_main), "sync");
     }
@@ -321,8 +402,58 @@ int bhx_set_bodies(bhx_ctx* c, const double* pos, const double* vel, const doubl
         HIPCHK(hipGetLastError());
     }
     HIPCHK(hipStreamSynchronize(s));
+    int rc = set_layout(c, std::vector<uint32_t>{0u, (uint32_t)n});
+    if (rc != BHX_OK) return rc;
     c->bounds_valid = false;
     c->tree_valid = false;
+    return BHX_OK;
+}
+
+int bhx_set_systems(bhx_ctx* c, const uint64_t* offsets, size_t nsys) {
+    if (!c || !offsets) return c ? fail(c, BHX_E_NULL, "offsets is a null pointer") : BHX_E_NULL;
+    if (nsys == 0 || nsys > kMaxSystems) return fail(c, BHX_E_ARG, "system count out of range");
+    if (offsets[0] != 0 || offsets[nsys] != c->n) return fail(c, BHX_E_ARG, "offsets must run from 0 to the body count");
+    std::vector<uint32_t> off(nsys + 1);
+    for (size_t k = 0; k <= nsys; ++k) {
+        if (k > 0 && offsets[k] <= offsets[k - 1]) return fail(c, BHX_E_ARG, "every system needs at least one body");
+        off[k] = (uint32_t)offsets[k];
+    }
+    HIPCHK(hipSetDevice(c->device));
+    return set_layout(c, off);
+}
+
+int bhx_num_systems(const bhx_ctx* c, size_t* nsys) {
+    if (!c || !nsys) return BHX_E_NULL;
+    *nsys = c->nsys;
+    return BHX_OK;
+}
+
+int bhx_system_bounds(bhx_ctx* c, double* centers, double* sizes) {
+    if (!c) return BHX_E_NULL;
+    HIPCHK(hipSetDevice(c->device));
+    hipStream_t s = c->s_main;
+    HIPCHK(hipStreamSynchronize(c->s_side));
+    double* b4 = c->bounds_slot(c->bslot);
+    if (c->n) HIPCHK(launch_bounds(c->px, c->py, c->pz, c->sys(), c->max_sys_n, b4, s));
+    std::vector<double> h(4 * (size_t)c->nsys);
+    HIPCHK(hipMemcpyAsync(h.data(), b4, h.size() * sizeof(double), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    c->bounds_valid = c->n > 0;
+    for (uint32_t k = 0; k < c->nsys; ++k) {
+        if (centers) for (int a = 0; a < 3; ++a) centers[3 * (size_t)k + a] = h[4 * (size_t)k + a];
+        if (sizes) sizes[k] = h[4 * (size_t)k + 3];
+    }
+    return BHX_OK;
+}
+
+int bhx_system_roots(bhx_ctx* c, uint64_t* roots) {
+    if (!c || !roots) return c ? fail(c, BHX_E_NULL, "roots is a null pointer") : BHX_E_NULL;
+    if (!c->tree_built) return fail(c, BHX_E_STATE, "no tree built");
+    HIPCHK(hipSetDevice(c->device));
+    std::vector<uint32_t> h(c->nsys);
+    HIPCHK(hipMemcpyAsync(h.data(), c->sys_root, h.size() * sizeof(uint32_t), hipMemcpyDeviceToHost, c->s_main));
+    HIPCHK(hipStreamSynchronize(c->s_main));
+    for (uint32_t k = 0; k < c->nsys; ++k) roots[k] = h[k];
     return BHX_OK;
 }
 
@@ -371,9 +502,9 @@ int bhx_bounds(bhx_ctx* c, double center[3], double* size) {
     HIPCHK(hipSetDevice(c->device));
     hipStream_t s = c->s_main;
     HIPCHK(hipStreamSynchronize(c->s_side));
-    double* b4 = c->bounds + 4 * c->bslot;
-    HIPCHK(launch_bounds_mean(c->px, c->py, c->pz, c->n, b4, s));
-    HIPCHK(launch_bounds_size(c->px, c->py, c->pz, c->n, b4, s));
+    if (c->nsys > 1) return fail(c, BHX_E_STATE, "several systems: use bhx_system_bounds");
+    double* b4 = c->bounds_slot(c->bslot);
+    HIPCHK(launch_bounds(c->px, c->py, c->pz, c->sys(), c->max_sys_n, b4, s));
     double h[4];
     HIPCHK(hipMemcpyAsync(h, b4, sizeof h, hipMemcpyDeviceToHost, s));
     HIPCHK(hipStreamSynchronize(s));
@@ -387,10 +518,11 @@ int bhx_morton_keys(bhx_ctx* c, double size, uint64_t* keys_out) {
     if (!c) return BHX_E_NULL;
     HIPCHK(hipSetDevice(c->device));
     hipStream_t s = c->s_main;
+    if (c->nsys > 1) return fail(c, BHX_E_STATE, "several systems: keys follow each system's bounds in bhx_step");
     double h[4] = {0.0, 0.0, 0.0, size};
-    double* b4 = c->bounds + 8;      // scratch slot
+    double* b4 = c->bounds_slot(2);      // scratch slot
     HIPCHK(hipMemcpyAsync(b4, h, sizeof h, hipMemcpyHostToDevice, s));
-    HIPCHK(launch_morton(c->px, c->py, c->pz, c->n, b4, c->mkeys, nullptr, s));
+    HIPCHK(launch_morton(c->px, c->py, c->pz, c->n, nullptr, b4, c->mkeys, nullptr, s));
     if (keys_out && c->n) HIPCHK(hipMemcpyAsync(keys_out, c->mkeys, c->n * sizeof(uint64_t), hipMemcpyDeviceToHost, s));
     HIPCHK(hipStreamSynchronize(s));
     return BHX_OK;
@@ -456,8 +588,7 @@ int bhx_sort_bodies(bhx_ctx* c, uint64_t* perm_out) {
     if (c->n == 0) return BHX_OK;
     hipLaunchKernelGGL(k_copy_u64, dim3(grid_for(c->n)), dim3(kBlock), 0, s, c->mkeys, c->n, c->keys);
     HIPCHK(launch_iota(c->vals, c->n, s));
-    HIPCHK(radix_sort_pairs(c->keys, c->vals, c->keys_alt, c->vals_alt, c->n, c->sort_hist, c->sort_status,
-                            c->sort_tile_ctr, c->ctr, s));
+    HIPCHK(sort_pairs(c, c->keys, c->vals, s));
     int rc = reorder_by(c, c->vals, s);
     if (rc != BHX_OK) return rc;
     if (perm_out) {
@@ -476,6 +607,11 @@ int bhx_reorder_bodies(bhx_ctx* c, const uint64_t* perm) {
     if (c->n == 0) return BHX_OK;
     for (size_t i = 0; i < c->n; ++i)
         if (perm[i] >= c->n) return fail(c, BHX_E_ARG, "index out of range");
+    if (c->nsys > 1)   // bodies stay inside their system's range
+        for (uint32_t k = 0; k < c->nsys; ++k)
+            for (size_t i = c->sys_off_h[k]; i < c->sys_off_h[k + 1]; ++i)
+                if (perm[i] < c->sys_off_h[k] || perm[i] >= c->sys_off_h[k + 1])
+                    return fail(c, BHX_E_ARG, "permutation moves a body to another system");
     uint64_t* tmp = reinterpret_cast<uint64_t*>(c->stage);
     HIPCHK(hipMemcpyAsync(tmp, perm, c->n * sizeof(uint64_t), hipMemcpyHostToDevice, s));
     hipLaunchKernelGGL(k_u64_to_u32, dim3(grid_for(c->n)), dim3(kBlock), 0, s, tmp, c->n, c->vals_alt);
@@ -511,15 +647,38 @@ int bhx_build(bhx_ctx* c, const double center[3], double size, size_t* n_nodes) 
     HIPCHK(hipSetDevice(c->device));
     hipStream_t s = c->s_main;
     HIPCHK(hipStreamSynchronize(c->s_side));
+    if (c->nsys > 1) return fail(c, BHX_E_STATE, "several systems: use bhx_build_systems");
     double h[4] = {center[0], center[1], center[2], size};
-    HIPCHK(hipMemcpyAsync(c->bounds + 4 * c->bslot, h, sizeof h, hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemcpyAsync(c->bounds_slot(c->bslot), h, sizeof h, hipMemcpyHostToDevice, s));
     c->bounds_valid = false;
     int rc = build_on_device(c, s);
     if (rc != BHX_OK) return rc;
     rc = read_counters(c, s);
     if (rc != BHX_OK) return rc;
-    std::memcpy(c->tree_center, center, sizeof c->tree_center);
-    c->tree_size = size;
+    c->tree_bounds_h.assign(h, h + 4);
+    if (n_nodes) *n_nodes = c->n_nodes;
+    return BHX_OK;
+}
+
+int bhx_build_systems(bhx_ctx* c, const double* centers, const double* sizes, size_t* n_nodes) {
+    if (!c || !centers || !sizes) return c ? fail(c, BHX_E_NULL, "bounds is a null pointer") : BHX_E_NULL;
+    std::vector<double> h(4 * (size_t)c->nsys);
+    for (uint32_t k = 0; k < c->nsys; ++k) {
+        for (int a = 0; a < 3; ++a) h[4 * (size_t)k + a] = centers[3 * (size_t)k + a];
+        h[4 * (size_t)k + 3] = sizes[k];
+    }
+    for (double v : h)
+        if (!std::isfinite(v)) return fail(c, BHX_E_NONFINITE, "non-finite bounds");
+    HIPCHK(hipSetDevice(c->device));
+    hipStream_t s = c->s_main;
+    HIPCHK(hipStreamSynchronize(c->s_side));
+    HIPCHK(hipMemcpyAsync(c->bounds_slot(c->bslot), h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice, s));
+    c->bounds_valid = false;
+    int rc = build_on_device(c, s);
+    if (rc != BHX_OK) return rc;
+    rc = read_counters(c, s);
+    if (rc != BHX_OK) return rc;
+    c->tree_bounds_h = h;
     if (n_nodes) *n_nodes = c->n_nodes;
     return BHX_OK;
 }
@@ -553,7 +712,10 @@ int bhx_get_nodes(bhx_ctx* c, bhx_node* out, size_t cap, size_t* n_nodes) {
         bhx_node& o = out[i];
         o.key = key[i];
         for (int a = 0; a < 3; ++a) { o.center[a] = center[3 * i + a]; o.com[a] = com[3 * i + a]; }
-        double sz = c->tree_size;
+        // the node's system: last offset <= its first body
+        const size_t k = (size_t)(std::upper_bound(c->sys_off_h.begin(), c->sys_off_h.end() - 1, start[i]) -
+                                  c->sys_off_h.begin()) - 1;
+        double sz = 4 * k + 3 < c->tree_bounds_h.size() ? c->tree_bounds_h[4 * k + 3] : 0.0;
         for (uint32_t l = 0; l < level[i]; ++l) sz = sz * 0.5;
         o.size = sz;
         o.mass = mass[i];
@@ -568,9 +730,12 @@ int bhx_get_nodes(bhx_ctx* c, bhx_node* out, size_t cap, size_t* n_nodes) {
     return BHX_OK;
 }
 
-int bhx_lookup(bhx_ctx* c, const uint64_t* keys, size_t n, int64_t* ids) {
+int bhx_lookup(bhx_ctx* c, const uint64_t* keys, size_t n, int64_t* ids) { return bhx_lookup_in(c, 0, keys, n, ids); }
+
+int bhx_lookup_in(bhx_ctx* c, size_t system, const uint64_t* keys, size_t n, int64_t* ids) {
     if (!c || !keys || !ids) return c ? fail(c, BHX_E_NULL, "null argument") : BHX_E_NULL;
     if (!c->tree_built) return fail(c, BHX_E_STATE, "no tree built");
+    if (system >= c->nsys) return fail(c, BHX_E_ARG, "system out of range");
     if (n == 0) return BHX_OK;
     if (c->n_nodes == 0) {
         for (size_t i = 0; i < n; ++i) ids[i] = -1;
@@ -583,7 +748,7 @@ int bhx_lookup(bhx_ctx* c, const uint64_t* keys, size_t n, int64_t* ids) {
     HIPCHK(hipMalloc((void**)&dk, n * sizeof(uint64_t)));
     hipError_t e = hipMalloc((void**)&di, n * sizeof(int64_t));
     if (e == hipSuccess) e = hipMemcpyAsync(dk, keys, n * sizeof(uint64_t), hipMemcpyHostToDevice, s);
-    if (e == hipSuccess) e = launch_lookup(c->tree, dk, n, di, s);
+    if (e == hipSuccess) e = launch_lookup(c, (uint32_t)system, dk, n, di, s);
     if (e == hipSuccess) e = hipMemcpyAsync(ids, di, n * sizeof(int64_t), hipMemcpyDeviceToHost, s);
     if (e == hipSuccess) e = hipStreamSynchronize(s);
     (void)hipFree(dk);
@@ -681,39 +846,36 @@ int bhx_step(bhx_ctx* c, int nsteps) {
     const auto t0 = std::chrono::steady_clock::now();
     for (int step = 0; step < nsteps && c->n; ++step) {
         const int cur = c->bslot;
-        double* b_cur = c->bounds + 4 * cur;
-        double* b_next = c->bounds + 4 * (cur ^ 1);
+        double* b_cur = c->bounds_slot(cur);
+        double* b_next = c->bounds_slot(cur ^ 1);
+        const SysView sv = c->sys();
         const bool prof = c->profiling;
         if (prof) HIPCHK(hipEventRecord(c->ev[0], s));
         if (!c->bounds_valid) {
-            HIPCHK(launch_bounds_mean(c->px, c->py, c->pz, c->n, b_cur, s));
-            HIPCHK(launch_bounds_size(c->px, c->py, c->pz, c->n, b_cur, s));
+            HIPCHK(launch_bounds(c->px, c->py, c->pz, sv, c->max_sys_n, b_cur, s));
         } else {
             HIPCHK(hipStreamWaitEvent(s, c->ev_bounds, 0));
         }
         if (prof) HIPCHK(hipEventRecord(c->ev[1], s));
-        HIPCHK(launch_morton(c->px, c->py, c->pz, c->n, b_cur, c->keys, c->vals, s));
+        HIPCHK(launch_morton(c->px, c->py, c->pz, c->n, sv.id, b_cur, c->keys, c->vals, s));
         if (prof) HIPCHK(hipEventRecord(c->ev[2], s));
-        HIPCHK(radix_sort_pairs(c->keys, c->vals, c->keys_alt, c->vals_alt, c->n, c->sort_hist, c->sort_status,
-                                c->sort_tile_ctr, c->ctr, s));
+        HIPCHK(sort_pairs(c, c->keys, c->vals, s));
         if (prof) HIPCHK(hipEventRecord(c->ev[3], s));
         HIPCHK(launch_permute_kick_drift(c, c->vals, dt, s));
         HIPCHK(hipEventRecord(c->ev_drift, s));
         if (prof) HIPCHK(hipEventRecord(c->ev[4], s));
         HIPCHK(hipStreamWaitEvent(c->s_side, c->ev_drift, 0));
-        HIPCHK(launch_bounds_mean(c->px, c->py, c->pz, c->n, b_next, c->s_side));
-        HIPCHK(launch_bounds_size(c->px, c->py, c->pz, c->n, b_next, c->s_side));
+        HIPCHK(launch_bounds(c->px, c->py, c->pz, sv, c->max_sys_n, b_next, c->s_side));
         HIPCHK(hipEventRecord(c->ev_bounds, c->s_side));
         // tree on x_{f+1} with the bounds of x_f
         c->tree_valid = c->tree_built = false;
-        HIPCHK(launch_treekeys(c->px, c->py, c->pz, c->n, b_cur, c->tkeys, c->tperm, s));
-        HIPCHK(radix_sort_pairs(c->tkeys, c->tperm, c->keys_alt, c->vals_alt, c->n, c->sort_hist, c->sort_status,
-                                c->sort_tile_ctr, c->ctr, s));
+        HIPCHK(launch_treekeys(c->px, c->py, c->pz, c->n, sv.id, b_cur, c->tkeys, c->tperm, s));
+        HIPCHK(sort_pairs(c, c->tkeys, c->tperm, s));
         HIPCHK(launch_tree_gather(c, s));
         if (prof) HIPCHK(hipEventRecord(c->ev[5], s));
         size_t total = 0;
         for (int attempt = 0; attempt < 2; ++attempt) {
-            HIPCHK(build_tree(c, s, &total));
+            HIPCHK(build_tree(c, s, &total, true));
             if (total <= c->tree.cap) break;
             HIPCHK(hipStreamSynchronize(s));
             HIPCHK(alloc_tree(c, total + total / 4 + 1024));
@@ -748,10 +910,11 @@ int bhx_step(bhx_ctx* c, int nsteps) {
     if (rc != BHX_OK) return rc;
     HIPCHK(hipStreamSynchronize(c->s_side));
     // keep the build bounds of the last frame for node downloads
-    double h[4];
-    HIPCHK(hipMemcpy(h, c->bounds + 4 * (c->bslot ^ 1), sizeof h, hipMemcpyDeviceToHost));
-    c->tree_center[0] = h[0]; c->tree_center[1] = h[1]; c->tree_center[2] = h[2];
-    c->tree_size = h[3];
+    if (c->tree_built) {
+        c->tree_bounds_h.resize(4 * (size_t)c->nsys);
+        HIPCHK(hipMemcpy(c->tree_bounds_h.data(), c->bounds_slot(c->tree_slot), c->tree_bounds_h.size() * sizeof(double),
+                         hipMemcpyDeviceToHost));
+    }
     const auto t1 = std::chrono::steady_clock::now();
     c->timing.total = std::chrono::duration<double, std::milli>(t1 - t0).count();
     return BHX_OK;

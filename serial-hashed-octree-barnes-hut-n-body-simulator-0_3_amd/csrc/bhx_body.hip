@@ -21,16 +21,22 @@ inline unsigned grid_for(size_t n, int per_block = kBlock, unsigned cap = 65535u
 // (MathUtils.cpp:81-90): a left-to-right FP64 sum in array order, then a
 // multiply by fl(1/N).  The sum is one dependent chain per axis; waves 1..3
 // stream chunks of positions into LDS (double buffered) while lanes 0..2 of
-// wave 0 run the x, y and z chains over the staged chunk.
+// wave 0 run the x, y and z chains over the staged chunk.  One workgroup per
+// system; the chains of different systems run side by side.
 __global__ __launch_bounds__(kBlock) void k_bounds_mean(const double* __restrict__ px,
                                                         const double* __restrict__ py,
-                                                        const double* __restrict__ pz, size_t n,
-                                                        double* __restrict__ out4) {
+                                                        const double* __restrict__ pz, SysView sv,
+                                                        double* __restrict__ bounds4) {
     __shared__ double buf[2][3][kMeanChunk];
     const int tid = threadIdx.x;
     const int wave = tid >> 6, lane = tid & 63;
+    const uint32_t sys = blockIdx.x;
+    const size_t first = sv.off[sys];
+    const size_t n = sv.off[sys + 1] - first;
+    double* out4 = bounds4 + 4 * (size_t)sys;
     const size_t nchunks = (n + kMeanChunk - 1) / kMeanChunk;
-    const double* const src[3] = {px, py, pz};
+    const double* const src[3] = {px + first, py + first, pz + first};
+    const bool pairs = (first & 1) == 0;   // 16-byte aligned double pairs
 
     double s = 0.0;
     for (size_t ch = 0; ch <= nchunks; ++ch) {
@@ -42,7 +48,7 @@ __global__ __launch_bounds__(kBlock) void k_bounds_mean(const double* __restrict
                 const size_t base = ch * kMeanChunk;
                 const double* p = src[a] + base;
                 double* d = buf[b][a];
-                if (base + kMeanChunk <= n) {
+                if (pairs && base + kMeanChunk <= n) {
                     for (int j = 2 * lane; j < kMeanChunk; j += 128) {
                         const double2 v = *reinterpret_cast<const double2*>(p + j);
                         d[j] = v.x;
@@ -95,13 +101,18 @@ __global__ __launch_bounds__(kBlock) void k_bounds_mean(const double* __restrict
 // OctantBounds size — OctantUtils.cpp:47-58: max over bodies and axes of
 // fabs(p) - c, floored at 0.  A max is exact in any order, so a parallel
 // reduction reproduces it; non-negative doubles order like their bit patterns.
+// blockIdx.y is the system, blockIdx.x strides over its bodies.
 __global__ __launch_bounds__(kBlock) void k_bounds_size(const double* __restrict__ px,
                                                         const double* __restrict__ py,
-                                                        const double* __restrict__ pz, size_t n,
-                                                        double* __restrict__ out4) {
+                                                        const double* __restrict__ pz, SysView sv,
+                                                        double* __restrict__ bounds4) {
+    const uint32_t sys = blockIdx.y;
+    const size_t first = sv.off[sys], end = sv.off[sys + 1];
+    double* out4 = bounds4 + 4 * (size_t)sys;
+    if (first + (size_t)blockIdx.x * kBlock >= end) return;
     const double cx = out4[0], cy = out4[1], cz = out4[2];
     double s = 0.0;
-    for (size_t i = blockIdx.x * (size_t)kBlock + threadIdx.x; i < n; i += (size_t)gridDim.x * kBlock) {
+    for (size_t i = first + blockIdx.x * (size_t)kBlock + threadIdx.x; i < end; i += (size_t)gridDim.x * kBlock) {
         double t;
         t = fabs(px[i]) - cx; if (t > s) s = t;
         t = fabs(py[i]) - cy; if (t > s) s = t;
@@ -148,12 +159,13 @@ __device__ __forceinline__ uint64_t spread21(uint64_t t) {
 __global__ __launch_bounds__(kBlock) void k_morton(const double* __restrict__ px,
                                                    const double* __restrict__ py,
                                                    const double* __restrict__ pz, size_t n,
-                                                   const double* __restrict__ b4,
+                                                   const uint32_t* __restrict__ sysid,
+                                                   const double* __restrict__ bounds4,
                                                    uint64_t* __restrict__ keys,
                                                    uint32_t* __restrict__ vals) {
-    const double size = b4[3];
-    const double frac = 2097152.0 / (2.0 * size);
     for (size_t i = blockIdx.x * (size_t)kBlock + threadIdx.x; i < n; i += (size_t)gridDim.x * kBlock) {
+        const double size = bounds4[4 * (size_t)(sysid ? sysid[i] : 0u) + 3];
+        const double frac = 2097152.0 / (2.0 * size);
         double xs = px[i] + size;
         double ys = py[i] + size;
         double zs = pz[i] + size;
@@ -176,11 +188,13 @@ __global__ __launch_bounds__(kBlock) void k_morton(const double* __restrict__ px
 __global__ __launch_bounds__(kBlock) void k_treekeys(const double* __restrict__ px,
                                                      const double* __restrict__ py,
                                                      const double* __restrict__ pz, size_t n,
-                                                     const double* __restrict__ b4,
+                                                     const uint32_t* __restrict__ sysid,
+                                                     const double* __restrict__ bounds4,
                                                      uint64_t* __restrict__ keys,
                                                      uint32_t* __restrict__ vals) {
-    const double c0x = b4[0], c0y = b4[1], c0z = b4[2], s0 = b4[3];
     for (size_t i = blockIdx.x * (size_t)kBlock + threadIdx.x; i < n; i += (size_t)gridDim.x * kBlock) {
+        const double* b4 = bounds4 + 4 * (size_t)(sysid ? sysid[i] : 0u);
+        const double c0x = b4[0], c0y = b4[1], c0z = b4[2], s0 = b4[3];
         const double x = px[i], y = py[i], z = pz[i];
         double cx = c0x, cy = c0y, cz = c0z, s = s0;
         uint64_t k = 1;
@@ -316,27 +330,24 @@ __global__ __launch_bounds__(kBlock) void k_tree_gather(const double* __restrict
 
 }  // namespace
 
-hipError_t launch_bounds_mean(const double* px, const double* py, const double* pz, size_t n, double* out4,
-                              hipStream_t s) {
-    hipLaunchKernelGGL(k_bounds_mean, dim3(1), dim3(kBlock), 0, s, px, py, pz, n, out4);
+hipError_t launch_bounds(const double* px, const double* py, const double* pz, const SysView& sv,
+                         uint32_t max_sys_n, double* bounds4, hipStream_t s) {
+    if (sv.nsys == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_bounds_mean, dim3(sv.nsys), dim3(kBlock), 0, s, px, py, pz, sv, bounds4);
+    hipLaunchKernelGGL(k_bounds_size, dim3(grid_for(max_sys_n, kBlock * 8, 2048), sv.nsys), dim3(kBlock), 0, s, px, py,
+                       pz, sv, bounds4);
     return hipGetLastError();
 }
 
-hipError_t launch_bounds_size(const double* px, const double* py, const double* pz, size_t n, double* out4,
-                              hipStream_t s) {
-    hipLaunchKernelGGL(k_bounds_size, dim3(grid_for(n, kBlock * 8, 2048)), dim3(kBlock), 0, s, px, py, pz, n, out4);
+hipError_t launch_morton(const double* px, const double* py, const double* pz, size_t n, const uint32_t* sysid,
+                         const double* b4, uint64_t* keys, uint32_t* vals, hipStream_t s) {
+    hipLaunchKernelGGL(k_morton, dim3(grid_for(n)), dim3(kBlock), 0, s, px, py, pz, n, sysid, b4, keys, vals);
     return hipGetLastError();
 }
 
-hipError_t launch_morton(const double* px, const double* py, const double* pz, size_t n, const double* b4,
-                         uint64_t* keys, uint32_t* vals, hipStream_t s) {
-    hipLaunchKernelGGL(k_morton, dim3(grid_for(n)), dim3(kBlock), 0, s, px, py, pz, n, b4, keys, vals);
-    return hipGetLastError();
-}
-
-hipError_t launch_treekeys(const double* px, const double* py, const double* pz, size_t n, const double* b4,
-                           uint64_t* keys, uint32_t* vals, hipStream_t s) {
-    hipLaunchKernelGGL(k_treekeys, dim3(grid_for(n)), dim3(kBlock), 0, s, px, py, pz, n, b4, keys, vals);
+hipError_t launch_treekeys(const double* px, const double* py, const double* pz, size_t n, const uint32_t* sysid,
+                           const double* b4, uint64_t* keys, uint32_t* vals, hipStream_t s) {
+    hipLaunchKernelGGL(k_treekeys, dim3(grid_for(n)), dim3(kBlock), 0, s, px, py, pz, n, sysid, b4, keys, vals);
     return hipGetLastError();
 }
 

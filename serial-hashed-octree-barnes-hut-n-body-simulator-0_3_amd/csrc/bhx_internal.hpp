@@ -6,6 +6,7 @@
 #include <cstddef>
 #include <cstdint>
 #include <string>
+#include <vector>
 
 #include "../../include/bhx.h"
 
@@ -13,6 +14,7 @@ namespace bhx {
 
 constexpr int kMaxLevel = 21;   // 1 sentinel bit + 21 x 3 path bits = 64-bit key
 constexpr int kMaxLeafCap = 64;
+constexpr size_t kMaxSystems = 65536;   // system ids sort in two byte passes
 
 // Walk record of one node, 64 bytes so one wave-uniform node fetch is a
 // single scalar load burst.  Pre-order layout: the first child of an internal
@@ -71,8 +73,23 @@ struct Counters {          // small device-side status block
     uint32_t hash_fail;
     uint32_t sort_fail;
     uint32_t nonfinite;
-    uint32_t pad[11];
+    uint32_t node_overflow;   // the last build asked for more nodes than the tree holds
+    uint32_t max_nodes;       // largest node count a build asked for (sticky)
+    uint32_t overflow_seen;   // some build overflowed (sticky until the host clears it)
+    uint32_t nodes_kept;      // min(total_nodes, capacity): the nodes the last build wrote
+    uint32_t pad[7];
 };
+
+// Independent systems: contiguous body ranges that are bounded, sorted, built
+// and walked each on its own (one system is the plain case).  Bodies never
+// leave their system's range, so the system of a body position is fixed.
+struct SysView {
+    const uint32_t* off;      // nsys + 1 body offsets
+    const uint32_t* id;       // system of each body position; null when nsys == 1
+    uint32_t nsys;
+};
+
+__device__ __forceinline__ uint32_t sys_of(const SysView& v, uint32_t i) { return v.id ? v.id[i] : 0u; }
 
 }  // namespace bhx
 
@@ -119,6 +136,7 @@ struct bhx_ctx {
     // walk scratch
     uint32_t* wcount = nullptr;                // interaction counts per body (array order)
     // bounds: two slots of (cx, cy, cz, size)
+    // (nsys x 4 doubles per slot; slots 0 and 1 alternate between frames, slot 2 is scratch)
     double* bounds = nullptr;
     int bslot = 0;
     int tree_slot = 0;                         // bounds slot the current tree was built with
@@ -132,27 +150,48 @@ struct bhx_ctx {
     bool tree_valid = false;   // tree built on the current bodies (walk allowed)
     bool tree_built = false;   // node table of the last build still intact (downloads, lookups)
     size_t tree_n = 0;         // bodies the last build covered
-    double tree_center[3] = {0, 0, 0};
-    double tree_size = 0;
+    // independent systems
+    uint32_t nsys = 1;
+    uint32_t max_sys_n = 0;                    // bodies in the largest system
+    size_t sys_cap = 0;                        // systems the per-system buffers hold
+    std::vector<uint32_t> sys_off_h;           // nsys + 1 body offsets (host copy)
+    uint32_t* sys_off = nullptr;               // device copy
+    uint32_t* sysid = nullptr;                 // system of each body position (null for one system)
+    uint32_t* sys_root = nullptr;              // root node of each system in the current tree
+    uint4* gtab = nullptr;                     // walk groups: {first body, end body, system, 0}
+    size_t gtab_cap = 0;
+    uint32_t ngroups = 0;
+    std::vector<double> tree_bounds_h;         // (centre, size) per system the last tree was built with
+    // captured step graph (bhx_step_graph)
+    hipGraphExec_t graph_exec = nullptr;
+    bool graph_valid = false;
+    int graph_steps = 0;                       // steps one replay advances
+    // segmented sort scratch (several systems only)
+    uint64_t *skey = nullptr, *skey_alt = nullptr;
+    uint32_t *sidx = nullptr, *sidx_alt = nullptr;
+
+    bhx::SysView sys() const { return bhx::SysView{sys_off, nsys > 1 ? sysid : nullptr, nsys}; }
+    double* bounds_slot(int k) const { return bounds + 4 * sys_cap * (size_t)k; }
 };
 
 namespace bhx {
 
 // --- launchers implemented in the .hip files (all enqueue on `s`) ----------
-hipError_t launch_bounds_mean(const double* px, const double* py, const double* pz, size_t n,
-                              double* out4, hipStream_t s);
-hipError_t launch_bounds_size(const double* px, const double* py, const double* pz, size_t n,
-                              double* out4, hipStream_t s);
-hipError_t launch_morton(const double* px, const double* py, const double* pz, size_t n,
+// bounds4 holds 4 doubles (centre, size) per system
+hipError_t launch_bounds(const double* px, const double* py, const double* pz, const SysView& sv,
+                         uint32_t max_sys_n, double* bounds4, hipStream_t s);
+hipError_t launch_morton(const double* px, const double* py, const double* pz, size_t n, const uint32_t* sysid,
                          const double* bounds4, uint64_t* keys, uint32_t* vals, hipStream_t s);
-hipError_t launch_treekeys(const double* px, const double* py, const double* pz, size_t n,
+hipError_t launch_treekeys(const double* px, const double* py, const double* pz, size_t n, const uint32_t* sysid,
                            const double* bounds4, uint64_t* keys, uint32_t* vals, hipStream_t s);
 hipError_t launch_iota(uint32_t* vals, size_t n, hipStream_t s);
 
 size_t sort_tiles_for(size_t n);
 hipError_t radix_sort_pairs(uint64_t* keys, uint32_t* vals, uint64_t* keys_alt, uint32_t* vals_alt,
                             size_t n, uint32_t* hist, uint32_t* status, uint32_t* tile_ctr,
-                            Counters* ctr, hipStream_t s);
+                            Counters* ctr, hipStream_t s, int passes = 8);
+// stable sort of (key, value) pairs within each system's body range
+hipError_t sort_pairs(bhx_ctx* c, uint64_t* keys, uint32_t* vals, hipStream_t s);
 
 hipError_t launch_gather(double* const* src, double* const* dst, int narr, const uint32_t* perm,
                          size_t n, hipStream_t s);
@@ -161,8 +200,8 @@ hipError_t launch_kick_drift(bhx_ctx* c, double dt, hipStream_t s);
 hipError_t launch_kick(bhx_ctx* c, double dt, hipStream_t s);
 hipError_t launch_tree_gather(bhx_ctx* c, hipStream_t s);
 
-hipError_t build_tree(bhx_ctx* c, hipStream_t s, size_t* total_out);
-hipError_t launch_lookup(const DeviceTree& t, const uint64_t* keys, size_t n, int64_t* ids, hipStream_t s);
+hipError_t build_tree(bhx_ctx* c, hipStream_t s, size_t* total_out, bool sync);
+hipError_t launch_lookup(bhx_ctx* c, uint32_t sys, const uint64_t* keys, size_t n, int64_t* ids, hipStream_t s);
 
 hipError_t launch_walk(bhx_ctx* c, hipStream_t s);
 hipError_t launch_walk_to(bhx_ctx* c, hipStream_t s, bool group, int precision, double* ox, double* oy, double* oz,

@@ -215,10 +215,12 @@ __global__ __launch_bounds__(kBlock) void k_sort_pass(const uint64_t* __restrict
 size_t sort_tiles_for(size_t n) { return (n + kTile - 1) / kTile; }
 
 hipError_t radix_sort_pairs(uint64_t* keys, uint32_t* vals, uint64_t* keys_alt, uint32_t* vals_alt, size_t n,
-                            uint32_t* hist, uint32_t* status, uint32_t* tile_ctr, Counters* ctr, hipStream_t s) {
+                            uint32_t* hist, uint32_t* status, uint32_t* tile_ctr, Counters* ctr, hipStream_t s,
+                            int passes) {
     if (n <= 1) return hipSuccess;
+    if (passes < 2 || passes > kPasses || (passes & 1)) return hipErrorInvalidValue;   // result lands in `keys`
     const size_t tiles = sort_tiles_for(n);
-    const size_t nstatus = (size_t)kPasses * tiles * kBins;
+    const size_t nstatus = (size_t)passes * tiles * kBins;
     unsigned ig = (unsigned)((nstatus + kBlock - 1) / kBlock);
     if (ig > 4096) ig = 4096;
     hipLaunchKernelGGL(k_sort_init, dim3(ig), dim3(kBlock), 0, s, hist, tile_ctr, status, nstatus);
@@ -227,12 +229,72 @@ hipError_t radix_sort_pairs(uint64_t* keys, uint32_t* vals, uint64_t* keys_alt, 
     if (hg == 0) hg = 1;
     hipLaunchKernelGGL(k_sort_hist, dim3(hg), dim3(kBlock), 0, s, keys, n, hist);
     hipLaunchKernelGGL(k_sort_scan, dim3(1), dim3(kBlock), 0, s, hist);
-    for (int p = 0; p < kPasses; ++p) {
+    for (int p = 0; p < passes; ++p) {
         const bool even = (p & 1) == 0;
         hipLaunchKernelGGL(k_sort_pass, dim3((unsigned)tiles), dim3(kBlock), 0, s, even ? keys : keys_alt,
                            even ? vals : vals_alt, even ? keys_alt : keys, even ? vals_alt : vals, n, p, hist,
                            status + (size_t)p * tiles * kBins, tile_ctr, ctr);
     }
+    return hipGetLastError();
+}
+
+namespace {
+
+// Second-stage keys of the segmented sort: the system of each sorted element
+// (values are body positions) and the element's place in the key order.
+__global__ __launch_bounds__(kBlock) void k_sys_keys(const uint32_t* __restrict__ vals,
+                                                     const uint32_t* __restrict__ sysid, size_t n,
+                                                     uint64_t* __restrict__ skey, uint32_t* __restrict__ sidx) {
+    for (size_t i = blockIdx.x * (size_t)kBlock + threadIdx.x; i < n; i += (size_t)gridDim.x * kBlock) {
+        skey[i] = sysid[vals[i]];
+        sidx[i] = (uint32_t)i;
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_sys_gather(const uint64_t* __restrict__ keys,
+                                                       const uint32_t* __restrict__ vals,
+                                                       const uint32_t* __restrict__ sidx, size_t n,
+                                                       uint64_t* __restrict__ kout, uint32_t* __restrict__ vout) {
+    for (size_t i = blockIdx.x * (size_t)kBlock + threadIdx.x; i < n; i += (size_t)gridDim.x * kBlock) {
+        const uint32_t j = sidx[i];
+        kout[i] = keys[j];
+        vout[i] = vals[j];
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_copy_pairs(const uint64_t* __restrict__ kin,
+                                                       const uint32_t* __restrict__ vin, size_t n,
+                                                       uint64_t* __restrict__ kout, uint32_t* __restrict__ vout) {
+    for (size_t i = blockIdx.x * (size_t)kBlock + threadIdx.x; i < n; i += (size_t)gridDim.x * kBlock) {
+        kout[i] = kin[i];
+        vout[i] = vin[i];
+    }
+}
+
+unsigned grid_n(size_t n) {
+    size_t g = (n + kBlock - 1) / kBlock;
+    if (g == 0) g = 1;
+    if (g > 65535u * 16u) g = 65535u * 16u;
+    return (unsigned)g;
+}
+
+}  // namespace
+
+// Several systems: a stable sort by key, then a stable sort by system (LSD
+// order), gives every system's range sorted by key with ties in their prior
+// order — the single-system order, system by system.
+hipError_t sort_pairs(bhx_ctx* c, uint64_t* keys, uint32_t* vals, hipStream_t s) {
+    const size_t n = c->n;
+    hipError_t e = radix_sort_pairs(keys, vals, c->keys_alt, c->vals_alt, n, c->sort_hist, c->sort_status,
+                                    c->sort_tile_ctr, c->ctr, s);
+    if (e != hipSuccess || c->nsys <= 1 || n <= 1) return e;
+    hipLaunchKernelGGL(k_sys_keys, dim3(grid_n(n)), dim3(kBlock), 0, s, vals, c->sysid, n, c->skey, c->sidx);
+    e = radix_sort_pairs(c->skey, c->sidx, c->skey_alt, c->sidx_alt, n, c->sort_hist, c->sort_status,
+                         c->sort_tile_ctr, c->ctr, s, 2);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_sys_gather, dim3(grid_n(n)), dim3(kBlock), 0, s, keys, vals, c->sidx, n, c->keys_alt,
+                       c->vals_alt);
+    hipLaunchKernelGGL(k_copy_pairs, dim3(grid_n(n)), dim3(kBlock), 0, s, c->keys_alt, c->vals_alt, n, keys, vals);
     return hipGetLastError();
 }
 

@@ -122,31 +122,37 @@ __global__ __launch_bounds__(kBlock) void k_scan_down(const uint32_t* __restrict
 
 // ---------------------------------------------------------------- node counting
 
-// Per body: how many cells start at it (levels lo..hi) — the counting half of
-// the insertion loop, HashedOctree.cpp:474-505.
-__global__ __launch_bounds__(kBlock) void k_count(const uint64_t* __restrict__ keys, uint32_t n, int B,
+// Per body: how many nodes start at it — the cells of levels lo..hi (the
+// counting half of the insertion loop, HashedOctree.cpp:474-505) plus the
+// root when the body opens its system.  Windows and shared prefixes never
+// reach across a system boundary.
+__global__ __launch_bounds__(kBlock) void k_count(const uint64_t* __restrict__ keys, uint32_t n, int B, SysView sv,
                                                   uint32_t* __restrict__ nn, uint32_t* __restrict__ nlo) {
     __shared__ uint64_t s_key[kBlock + 2 * kMaxLeafCap];
+    __shared__ uint32_t s_sys[kBlock + 2 * kMaxLeafCap];
     __shared__ int8_t s_w[kBlock + kMaxLeafCap];
     const int64_t b0 = (int64_t)blockIdx.x * kBlock;
     // keys [b0 - B, b0 + kBlock + B)
     for (int t = threadIdx.x; t < kBlock + 2 * B; t += kBlock) {
         const int64_t g = b0 - B + t;
-        s_key[t] = (g >= 0 && g < (int64_t)n) ? keys[g] : 0;
+        const bool in = g >= 0 && g < (int64_t)n;
+        s_key[t] = in ? keys[g] : 0;
+        s_sys[t] = in ? sys_of(sv, (uint32_t)g) : 0xFFFFFFFFu;
     }
     __syncthreads();
     // window levels W(j) for j in [b0 - B, b0 + kBlock): keys j and j+B
     for (int t = threadIdx.x; t < kBlock + B; t += kBlock) {
         const int64_t j = b0 - B + t;
         int w = -1;
-        if (j >= 0 && j + B < (int64_t)n) w = shared_levels(s_key[t], s_key[t + B]);
+        if (j >= 0 && j + B < (int64_t)n && s_sys[t] == s_sys[t + B]) w = shared_levels(s_key[t], s_key[t + B]);
         s_w[t] = (int8_t)w;
     }
     __syncthreads();
     const int64_t i = b0 + threadIdx.x;
     if (i >= (int64_t)n) return;
     const int li = threadIdx.x + B;   // local index of body i in s_key
-    const int lcp = (i == 0) ? -1 : shared_levels(s_key[li - 1], s_key[li]);
+    const bool first = i == 0 || s_sys[li - 1] != s_sys[li];
+    const int lcp = first ? -1 : shared_levels(s_key[li - 1], s_key[li]);
     int D = -1;
     for (int t = 0; t <= B; ++t) {     // windows starting at i-B .. i
         const int w = s_w[threadIdx.x + t];
@@ -154,42 +160,42 @@ __global__ __launch_bounds__(kBlock) void k_count(const uint64_t* __restrict__ k
     }
     const int lo = (lcp + 1 > 1) ? lcp + 1 : 1;
     const int hi = (D + 1 < kMaxLevel) ? D + 1 : kMaxLevel;
-    nn[i] = hi >= lo ? (uint32_t)(hi - lo + 1) : 0u;
+    nn[i] = (hi >= lo ? (uint32_t)(hi - lo + 1) : 0u) + (first ? 1u : 0u);
     nlo[i] = (uint32_t)lo;
 }
 
-// End of the level-L run starting at i: first j > i with a different L-prefix.
-__device__ uint32_t run_end(const uint64_t* __restrict__ keys, uint32_t n, uint32_t i, int L) {
+// End of the level-L run starting at i: first j > i (j <= lim) with a different L-prefix.
+__device__ uint32_t run_end(const uint64_t* __restrict__ keys, uint32_t lim, uint32_t i, int L) {
     const uint64_t p = prefix(keys[i], L);
     uint32_t lo = i, step = 1;          // keys[lo] has prefix p
     uint32_t hi;
     for (;;) {
         const uint64_t j = (uint64_t)i + step;
-        if (j >= n) { hi = n; break; }
+        if (j >= lim) { hi = lim; break; }
         if (prefix(keys[j], L) != p) { hi = (uint32_t)j; break; }
         lo = (uint32_t)j;
         step <<= 1;
     }
-    while (hi - lo > 1) {               // keys[lo] in run, keys[hi] (or n) out
+    while (hi - lo > 1) {               // keys[lo] in run, keys[hi] (or lim) out
         const uint32_t mid = lo + (hi - lo) / 2;
         if (prefix(keys[mid], L) == p) lo = mid; else hi = mid;
     }
     return hi;
 }
 
-// Start of the level-L run containing i: first j <= i with the same L-prefix.
-__device__ uint32_t run_start(const uint64_t* __restrict__ keys, uint32_t i, int L) {
+// Start of the level-L run containing i: first j in [base, i] with the same L-prefix.
+__device__ uint32_t run_start(const uint64_t* __restrict__ keys, uint32_t base, uint32_t i, int L) {
     const uint64_t p = prefix(keys[i], L);
     uint32_t hi = i, step = 1;          // keys[hi] has prefix p
     int64_t lo;
     for (;;) {
         const int64_t j = (int64_t)i - step;
-        if (j < 0) { lo = -1; break; }
+        if (j < (int64_t)base) { lo = (int64_t)base - 1; break; }
         if (prefix(keys[j], L) != p) { lo = j; break; }
         hi = (uint32_t)j;
         step <<= 1;
     }
-    while ((int64_t)hi - lo > 1) {      // keys[lo] out (or -1), keys[hi] in
+    while ((int64_t)hi - lo > 1) {      // keys[lo] out (or base-1), keys[hi] in
         const int64_t mid = lo + ((int64_t)hi - lo) / 2;
         if (prefix(keys[mid], L) == p) hi = (uint32_t)mid; else lo = mid;
     }
@@ -224,45 +230,58 @@ __device__ __forceinline__ void descend_center(uint64_t key, int L, const double
     }
 }
 
-// Writes every cell: key, body range, pre-order skip, parent, level, centre
+// Writes every node: key, body range, pre-order skip, parent, level, centre
 // (createChildNode / HOTNode child constructor, HashedOctree.cpp:697-711,
-// HashedNode.cpp:48-61).
-__global__ __launch_bounds__(kBlock) void k_emit(const uint64_t* __restrict__ keys, uint32_t n, int B,
+// HashedNode.cpp:48-61).  Node ids: the nodes starting at body i are
+// off[i] + (root first, then levels lo, lo+1, ...); a subtree ends where the
+// nodes of its end body begin.
+__global__ __launch_bounds__(kBlock) void k_emit(const uint64_t* __restrict__ keys, uint32_t n, int B, SysView sv,
                                                  const uint32_t* __restrict__ nn, const uint32_t* __restrict__ nlo,
-                                                 const uint32_t* __restrict__ off, const uint32_t* __restrict__ total_p,
-                                                 const double* __restrict__ b4, TreeArgs t, Counters* __restrict__ ctr) {
-    const uint32_t total = *total_p + 1;   // + root
+                                                 const uint32_t* __restrict__ off, const double* __restrict__ bounds4,
+                                                 uint32_t cap, TreeArgs t, uint32_t* __restrict__ sys_root,
+                                                 Counters* __restrict__ ctr) {
+    const uint32_t total = ctr->total_nodes;
     const size_t stride = (size_t)gridDim.x * kBlock;
     for (size_t ii = blockIdx.x * (size_t)kBlock + threadIdx.x; ii < n; ii += stride) {
         const uint32_t i = (uint32_t)ii;
-        if (i == 0) {
-            t.key[0] = 1;
-            t.start[0] = 0;
-            t.count[0] = n;
-            t.skip[0] = total;
-            t.parent[0] = kNone;
-            t.level[0] = 0;
-            t.center[0] = b4[0]; t.center[1] = b4[1]; t.center[2] = b4[2];
+        const uint32_t sys = sys_of(sv, i);
+        const uint32_t base = sv.off[sys], lim = sv.off[sys + 1];
+        const double* b4 = bounds4 + 4 * (size_t)sys;
+        const bool first = i == base;
+        uint32_t id = off[i];
+        if (first) {
+            sys_root[sys] = id;
+            if (id < cap) {
+                t.key[id] = 1;
+                t.start[id] = i;
+                t.count[id] = lim - i;
+                t.skip[id] = lim < n ? off[lim] : total;
+                t.parent[id] = kNone;
+                t.level[id] = 0;
+                t.center[3 * (size_t)id] = b4[0];
+                t.center[3 * (size_t)id + 1] = b4[1];
+                t.center[3 * (size_t)id + 2] = b4[2];
+            }
+            ++id;
         }
-        const uint32_t cnt = nn[i];
+        const uint32_t cnt = nn[i] - (first ? 1u : 0u);
         if (cnt == 0) continue;
         const int lo = (int)nlo[i];
-        const uint32_t id0 = 1 + off[i];
-        for (int L = lo; L < lo + (int)cnt; ++L) {
-            const uint32_t id = id0 + (uint32_t)(L - lo);
+        for (int L = lo; L < lo + (int)cnt; ++L, ++id) {
+            if (id >= cap) break;
             const uint64_t k = prefix(keys[i], L);
-            const uint32_t end = run_end(keys, n, i, L);
+            const uint32_t end = run_end(keys, lim, i, L);
             uint32_t par;
             if (L > lo) par = id - 1;
-            else if (L == 1) par = 0;
+            else if (L == 1) par = off[base];
             else {
-                const uint32_t ps = run_start(keys, i, L - 1);
-                par = 1 + off[ps] + (uint32_t)(L - 1 - (int)nlo[ps]);
+                const uint32_t ps = run_start(keys, base, i, L - 1);
+                par = off[ps] + (ps == base ? 1u : 0u) + (uint32_t)(L - 1 - (int)nlo[ps]);
             }
             t.key[id] = k;
             t.start[id] = i;
             t.count[id] = end - i;
-            t.skip[id] = end < n ? 1 + off[end] : total;
+            t.skip[id] = end < n ? off[end] : total;
             t.parent[id] = par;
             t.level[id] = (uint32_t)L;
             atomicOr(&t.child_byte[par], 1u << (unsigned)(k & 7));
@@ -274,6 +293,18 @@ __global__ __launch_bounds__(kBlock) void k_emit(const uint64_t* __restrict__ ke
             if (L == kMaxLevel && (int)(end - i) > B) atomicAdd(&ctr->unsplit, 1u);
         }
     }
+}
+
+// Total node count against the capacity: the clamped count bounds every
+// later kernel of the build, the overflow flags tell the host to grow.
+__global__ void k_check_cap(uint32_t cap, Counters* __restrict__ ctr) {
+    if (threadIdx.x != 0) return;
+    const uint32_t total = ctr->total_nodes;
+    const bool over = total > cap;
+    ctr->node_overflow = over ? 1u : 0u;
+    if (over) ctr->overflow_seen = 1u;
+    if (total > ctr->max_nodes) ctr->max_nodes = total;
+    ctr->nodes_kept = over ? cap : total;
 }
 
 __global__ void k_fill_u32(uint32_t* __restrict__ p, size_t n, uint32_t v) {
@@ -294,21 +325,28 @@ __device__ __forceinline__ uint64_t mix64(uint64_t k) {
     return k;
 }
 
+// Home slot of (system, key); system 0 hashes the key alone.
+__device__ __forceinline__ uint64_t hash_home(uint64_t k, uint32_t sys) {
+    return mix64(k ^ ((uint64_t)sys * 0x9e3779b97f4a7c15ull));
+}
+
 // insertHOTNode — HashedOctree.cpp:152-205: open addressing with linear
 // probing in HBM; an empty slot holds key 0 (no valid key is 0: every key
-// carries the root sentinel bit).
-__global__ __launch_bounds__(kBlock) void k_hash_insert(const uint64_t* __restrict__ key, const uint32_t* __restrict__ total_p,
+// carries the root sentinel bit).  Every node is unique, so an insert takes
+// the first empty slot of its probe sequence.
+__global__ __launch_bounds__(kBlock) void k_hash_insert(const uint64_t* __restrict__ key,
+                                                        const uint32_t* __restrict__ start, SysView sv,
                                                         uint64_t* __restrict__ hkey, uint32_t* __restrict__ hval,
                                                         uint64_t mask, Counters* __restrict__ ctr) {
-    const uint32_t total = *total_p + 1;
+    const uint32_t total = ctr->nodes_kept;
     for (size_t id = blockIdx.x * (size_t)kBlock + threadIdx.x; id < total; id += (size_t)gridDim.x * kBlock) {
         const uint64_t k = key[id];
-        uint64_t h = mix64(k) & mask;
+        uint64_t h = hash_home(k, sys_of(sv, start[id])) & mask;
         bool done = false;
         for (uint64_t probe = 0; probe <= mask; ++probe) {
             const unsigned long long prev = atomicCAS(reinterpret_cast<unsigned long long*>(hkey + h), 0ull,
                                                       (unsigned long long)k);
-            if (prev == 0ull || prev == k) {
+            if (prev == 0ull) {
                 hval[h] = (uint32_t)id;
                 done = true;
                 break;
@@ -319,19 +357,20 @@ __global__ __launch_bounds__(kBlock) void k_hash_insert(const uint64_t* __restri
     }
 }
 
-// lookUpNode — HashedOctree.cpp:111-115.
+// lookUpNode — HashedOctree.cpp:111-115, in system `sys`.
 __global__ __launch_bounds__(kBlock) void k_hash_lookup(const uint64_t* __restrict__ hkey, const uint32_t* __restrict__ hval,
+                                                        const uint32_t* __restrict__ start, SysView sv, uint32_t sys,
                                                         uint64_t mask, const uint64_t* __restrict__ q, size_t n,
                                                         int64_t* __restrict__ out) {
     for (size_t i = blockIdx.x * (size_t)kBlock + threadIdx.x; i < n; i += (size_t)gridDim.x * kBlock) {
         const uint64_t k = q[i];
         int64_t r = -1;
         if (k != 0) {
-            uint64_t h = mix64(k) & mask;
+            uint64_t h = hash_home(k, sys) & mask;
             for (uint64_t probe = 0; probe <= mask; ++probe) {
                 const uint64_t s = hkey[h];
-                if (s == k) { r = (int64_t)hval[h]; break; }
                 if (s == 0) break;
+                if (s == k && sys_of(sv, start[hval[h]]) == sys) { r = (int64_t)hval[h]; break; }
                 h = (h + 1) & mask;
             }
         }
@@ -359,13 +398,14 @@ __device__ __forceinline__ void quad_add(double* q, double m, double dx, double 
 //  INTENDED (computeNodeBaryCenters' aim, HashedOctree.cpp:721-827, written
 //  from the physics): mass-weighted centre of mass and the parallel-axis
 //  quadrupole sum Q = sum_k [M_k (3 d d^T - |d|^2 I) + Q_k].
-__global__ __launch_bounds__(kBlock) void k_moments(int L, const uint32_t* __restrict__ total_p, int B, int intended,
-                                                    TreeArgs t, const double* __restrict__ b4,
+__global__ __launch_bounds__(kBlock) void k_moments(int L, const Counters* __restrict__ ctr, SysView sv, int intended,
+                                                    TreeArgs t, const double* __restrict__ bounds4,
                                                     const double* __restrict__ px, const double* __restrict__ py,
                                                     const double* __restrict__ pz, const double* __restrict__ tx,
                                                     const double* __restrict__ ty, const double* __restrict__ tz,
                                                     const double* __restrict__ tm, const uint32_t* __restrict__ tperm) {
-    const uint32_t total = *total_p + 1;
+    if (ctr->node_overflow) return;   // the node table is incomplete: nothing to combine
+    const uint32_t total = ctr->total_nodes;
     for (size_t pp = blockIdx.x * (size_t)kBlock + threadIdx.x; pp < total; pp += (size_t)gridDim.x * kBlock) {
         const uint32_t p = (uint32_t)pp;
         if ((int)t.level[p] != L) continue;
@@ -422,7 +462,7 @@ __global__ __launch_bounds__(kBlock) void k_moments(int L, const uint32_t* __res
             t.quad[6 * (size_t)p + k] = q[k];
             t.quadf[6 * (size_t)p + k] = (float)q[k];
         }
-        double s = b4[3];
+        double s = bounds4[4 * (size_t)sys_of(sv, st) + 3];
         for (int l = 0; l < L; ++l) s = s * 0.5;
         WalkNode w;
         w.cx = cx; w.cy = cy; w.cz = cz;
@@ -481,44 +521,57 @@ hipError_t scan_exclusive_u32(const uint32_t* in, uint32_t* out, size_t n, uint3
     return hipGetLastError();
 }
 
-// Node capacity check happens on the host after the count/scan; the caller
-// grows the tree arrays and calls again if needed.
-hipError_t build_tree(bhx_ctx* c, hipStream_t s, size_t* total_out) {
+// Builds the tree of the current bodies (sorted tree keys in c->tkeys) with
+// the bounds in slot c->bslot.  With `sync` the node count is read back
+// first and *total_out reports it (the caller grows the arrays and calls
+// again when it exceeds the capacity); without, nothing waits on the host —
+// grids cover the capacity, the kernels stop at the count the device holds,
+// and an overflow is left in the counters for the host to find later.
+hipError_t build_tree(bhx_ctx* c, hipStream_t s, size_t* total_out, bool sync) {
     const uint32_t n = (uint32_t)c->n;
     const int B = c->cfg.leaf_capacity;
-    const double* b4 = c->bounds + 4 * c->bslot;
+    const double* b4 = c->bounds_slot(c->bslot);
+    const SysView sv = c->sys();
+    const uint32_t cap = (uint32_t)c->tree.cap;
     c->tree_slot = c->bslot;
     uint32_t* total_dev = &c->ctr->total_nodes;
     hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, s, &c->ctr->unsplit, (size_t)2, 0u);  // unsplit, hash_fail
-    hipLaunchKernelGGL(k_count, dim3(grid_for(n)), dim3(kBlock), 0, s, c->tkeys, n, B, c->nn, c->nlo);
+    hipLaunchKernelGGL(k_count, dim3(grid_for(n)), dim3(kBlock), 0, s, c->tkeys, n, B, sv, c->nn, c->nlo);
     hipError_t e = scan_exclusive_u32(c->nn, c->noff, n, c->scan_tmp, total_dev, s);
     if (e != hipSuccess) return e;
-    // total node count decides capacity: read it back
-    e = hipMemcpyAsync(c->ctr_host, c->ctr, sizeof(Counters), hipMemcpyDeviceToHost, s);
-    if (e != hipSuccess) return e;
-    e = hipStreamSynchronize(s);
-    if (e != hipSuccess) return e;
-    const size_t total = (size_t)c->ctr_host->total_nodes + 1;
-    *total_out = total;
-    if (total > c->tree.cap) return hipSuccess;   // caller grows and retries
+    hipLaunchKernelGGL(k_check_cap, dim3(1), dim3(64), 0, s, cap, c->ctr);
+    size_t span = cap;
+    if (sync) {
+        e = hipMemcpyAsync(c->ctr_host, c->ctr, sizeof(Counters), hipMemcpyDeviceToHost, s);
+        if (e != hipSuccess) return e;
+        e = hipStreamSynchronize(s);
+        if (e != hipSuccess) return e;
+        const size_t total = c->ctr_host->total_nodes;
+        *total_out = total;
+        if (total > cap) return hipSuccess;   // caller grows and retries
+        span = total;
+    } else {
+        *total_out = 0;
+    }
     TreeArgs t = tree_args(c);
-    hipLaunchKernelGGL(k_fill_u32, dim3(grid_for(total)), dim3(kBlock), 0, s, c->tree.child_byte, total, 0u);
-    hipLaunchKernelGGL(k_emit, dim3(grid_for(n)), dim3(kBlock), 0, s, c->tkeys, n, B, c->nn, c->nlo, c->noff,
-                       total_dev, b4, t, c->ctr);
+    hipLaunchKernelGGL(k_fill_u32, dim3(grid_for(span)), dim3(kBlock), 0, s, c->tree.child_byte, span, 0u);
+    hipLaunchKernelGGL(k_emit, dim3(grid_for(n)), dim3(kBlock), 0, s, c->tkeys, n, B, sv, c->nn, c->nlo, c->noff, b4,
+                       cap, t, c->sys_root, c->ctr);
     hipLaunchKernelGGL(k_fill_u64, dim3(grid_for(c->tree.hcap)), dim3(kBlock), 0, s, c->tree.hkey, c->tree.hcap,
                        0ull);
-    hipLaunchKernelGGL(k_hash_insert, dim3(grid_for(total)), dim3(kBlock), 0, s, c->tree.key, total_dev,
+    hipLaunchKernelGGL(k_hash_insert, dim3(grid_for(span)), dim3(kBlock), 0, s, c->tree.key, c->tree.start, sv,
                        c->tree.hkey, c->tree.hval, (uint64_t)(c->tree.hcap - 1), c->ctr);
     const int intended = c->cfg.mode == BHX_MODE_INTENDED;
     for (int L = kMaxLevel; L >= 0; --L)
-        hipLaunchKernelGGL(k_moments, dim3(grid_for(total)), dim3(kBlock), 0, s, L, total_dev, B, intended, t, b4,
+        hipLaunchKernelGGL(k_moments, dim3(grid_for(span)), dim3(kBlock), 0, s, L, c->ctr, sv, intended, t, b4,
                            c->px, c->py, c->pz, c->tx, c->ty, c->tz, c->tm, c->tperm);
     return hipGetLastError();
 }
 
-hipError_t launch_lookup(const DeviceTree& t, const uint64_t* keys, size_t n, int64_t* ids, hipStream_t s) {
-    hipLaunchKernelGGL(k_hash_lookup, dim3(grid_for(n)), dim3(kBlock), 0, s, t.hkey, t.hval, (uint64_t)(t.hcap - 1),
-                       keys, n, ids);
+hipError_t launch_lookup(bhx_ctx* c, uint32_t sys, const uint64_t* keys, size_t n, int64_t* ids, hipStream_t s) {
+    const DeviceTree& t = c->tree;
+    hipLaunchKernelGGL(k_hash_lookup, dim3(grid_for(n)), dim3(kBlock), 0, s, t.hkey, t.hval, t.start, c->sys(), sys,
+                       (uint64_t)(t.hcap - 1), keys, n, ids);
     return hipGetLastError();
 }
 

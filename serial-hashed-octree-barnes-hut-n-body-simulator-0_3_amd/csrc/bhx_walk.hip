@@ -52,8 +52,11 @@ struct WalkArgs {
     const WalkNode32* nodes32;
     const float4* tb4;       // tree order: offset from the leaf's centre of mass, mass
     const uint32_t* tleaf;   // tree order: node id of the body's leaf
-    uint32_t groups;         // 64-body groups (grid is rounded up to a multiple of 8)
-    const double* b4;        // root bounds (cx, cy, cz, size) of the tree
+    uint32_t groups;         // walk groups (grid is rounded up to a multiple of 8)
+    const uint4* gtab;       // per group: first body, end body (tree order), system
+    const uint32_t* sys_root;  // root node of each system
+    const uint32_t* abort_flag;  // non-zero: the node table is incomplete, walk nothing
+    const double* b4;        // root bounds (cx, cy, cz, size) per system
     const double* quad;      // 6 per node (FP64)
     const float* quadf;      // 6 per node (FP32)
     const double *tx, *ty, *tz, *tm;
@@ -103,10 +106,12 @@ __global__ __launch_bounds__(kBlock) void k_walk64(WalkArgs a) {
 #pragma clang fp contract(off)
     const uint32_t group = blockIdx.x * kWaves + (threadIdx.x >> 6);
     const uint32_t lane = threadIdx.x & 63;
-    if (group * 64 >= a.n) return;
-    const uint32_t i = group * 64 + lane;
-    const bool valid = i < a.n;
-    const uint32_t iv = valid ? i : group * 64;
+    if (group >= a.groups || *a.abort_flag) return;
+    const uint4 gr = a.gtab[group];
+    const uint32_t i = gr.x + lane;
+    const bool valid = i < gr.y;
+    const uint32_t iv = valid ? i : gr.x;
+    const uint32_t root = a.sys_root[gr.z];
     const double x = a.tx[iv], y = a.ty[iv], z = a.tz[iv];
     ConstNodes nodes = (ConstNodes)a.nodes;
     ConstD tx = (ConstD)a.tx;
@@ -116,9 +121,9 @@ __global__ __launch_bounds__(kBlock) void k_walk64(WalkArgs a) {
     double ax = 0.0, ay = 0.0, az = 0.0;
     uint32_t cnt = 0;
     uint32_t openbits = 1;                 // bit L: this lane opened the current level-L ancestor
-    const bool root_leaf = (nodes[0].level_leaf & 256u) != 0;
-    uint32_t p = root_leaf ? 0u : 1u;
-    const uint32_t end = nodes[0].skip;
+    const bool root_leaf = (nodes[root].level_leaf & 256u) != 0;
+    uint32_t p = root_leaf ? root : root + 1;
+    const uint32_t end = nodes[root].skip;
     while (p < end) {
         p = __builtin_amdgcn_readfirstlane(p);
         const double cx = nodes[p].cx, cy = nodes[p].cy, cz = nodes[p].cz;
@@ -190,17 +195,19 @@ template <bool kQuad>
 __global__ __launch_bounds__(kBlock) void k_walk32(WalkArgs a) {
     const uint32_t group = blockIdx.x * kWaves + (threadIdx.x >> 6);
     const uint32_t lane = threadIdx.x & 63;
-    if (group * 64 >= a.n) return;
-    const uint32_t i = group * 64 + lane;
-    const bool valid = i < a.n;
-    const uint32_t iv = valid ? i : group * 64;
+    if (group >= a.groups || *a.abort_flag) return;
+    const uint4 gr = a.gtab[group];
+    const uint32_t i = gr.x + lane;
+    const bool valid = i < gr.y;
+    const uint32_t iv = valid ? i : gr.x;
+    const uint32_t root = a.sys_root[gr.z];
     ConstNodes nodes = (ConstNodes)a.nodes;
     ConstD tx = (ConstD)a.tx;
     ConstD ty = (ConstD)a.ty;
     ConstD tz = (ConstD)a.tz;
     ConstD tm = (ConstD)a.tm;
     ConstF qf = (ConstF)a.quadf;
-    const uint32_t g0 = group * 64;
+    const uint32_t g0 = gr.x;
     const double ox = tx[g0], oy = ty[g0], oz = tz[g0];
     const double x = a.tx[iv], y = a.ty[iv], z = a.tz[iv];
     const float xr = (float)(x - ox), yr = (float)(y - oy), zr = (float)(z - oz);
@@ -208,9 +215,9 @@ __global__ __launch_bounds__(kBlock) void k_walk32(WalkArgs a) {
     float ax = 0.f, ay = 0.f, az = 0.f;
     uint32_t cnt = 0;
     uint32_t openbits = 1;
-    const bool root_leaf = (nodes[0].level_leaf & 256u) != 0;
-    uint32_t p = root_leaf ? 0u : 1u;
-    const uint32_t end = nodes[0].skip;
+    const bool root_leaf = (nodes[root].level_leaf & 256u) != 0;
+    uint32_t p = root_leaf ? root : root + 1;
+    const uint32_t end = nodes[root].skip;
     while (p < end) {
         p = __builtin_amdgcn_readfirstlane(p);
         const double cx = nodes[p].cx, cy = nodes[p].cy, cz = nodes[p].cz;
@@ -424,12 +431,14 @@ __global__ __launch_bounds__(64) void k_walk_group(WalkArgs a) {
     __shared__ T s_bd[kBodyCap][4];
     __shared__ uint32_t s_rstart[kRangeCap], s_roff[kRangeCap];
 
-    const uint32_t g0 = blockIdx.x * 64u;
+    if (blockIdx.x >= a.groups || *a.abort_flag) return;
+    const uint4 gr = a.gtab[blockIdx.x];
+    const uint32_t g0 = gr.x;
     const uint32_t lane = threadIdx.x;
-    if (g0 >= a.n) return;
     const uint32_t i = g0 + lane;
-    const bool valid = i < a.n;
+    const bool valid = i < gr.y;
     const uint32_t iv = valid ? i : g0;
+    const uint32_t rid = a.sys_root[gr.z];
     ConstU64 raw = (ConstU64)a.nodes;
     ConstF qf = (ConstF)a.quadf;
     ConstD qd = (ConstD)a.quad;
@@ -442,9 +451,9 @@ __global__ __launch_bounds__(64) void k_walk_group(WalkArgs a) {
     T ax = 0, ay = 0, az = 0;
     uint32_t cnt = 0;
     int nn = 0, nr = 0, nq = 0;            // queued cells, leaf ranges, bodies in those ranges
-    const WalkNode root = load_node(raw, 0);
+    const WalkNode root = load_node(raw, rid);
     const uint32_t end = root.skip;
-    uint32_t p = (root.level_leaf & 256u) ? 0u : 1u;
+    uint32_t p = (root.level_leaf & 256u) ? rid : rid + 1;
     while (p < end) {
         p = __builtin_amdgcn_readfirstlane(p);
         const WalkNode nd = load_node(raw, p);
@@ -575,12 +584,14 @@ __global__ __launch_bounds__(64) void k_walk_group32(WalkArgs a) {
     __shared__ float s_rdel[kRangeCap][3];    // leaf's first body minus the group origin
 
     const uint32_t grp = xcd_group(blockIdx.x, a.groups);
-    if (grp >= a.groups) return;
-    const uint32_t g0 = grp * 64u;
+    if (grp >= a.groups || *a.abort_flag) return;
+    const uint4 gr = a.gtab[grp];
+    const uint32_t g0 = gr.x;
     const uint32_t lane = threadIdx.x;
     const uint32_t i = g0 + lane;
-    const bool valid = i < a.n;
+    const bool valid = i < gr.y;
     const uint32_t iv = valid ? i : g0;
+    const uint32_t rid = a.sys_root[gr.z];
     ConstU64 raw = (ConstU64)a.nodes32;
     ConstD tx = (ConstD)a.tx;
     ConstD ty = (ConstD)a.ty;
@@ -594,14 +605,14 @@ __global__ __launch_bounds__(64) void k_walk_group32(WalkArgs a) {
     const float yr = (float)(a.nodes32[own].cy - oy) + ob.y;
     const float zr = (float)(a.nodes32[own].cz - oz) + ob.z;
     const float eps2 = a.eps2f, th2 = a.theta2f;
-    const double s0 = a.b4[3];
+    const double s0 = a.b4[4 * (size_t)gr.z + 3];
     const float s2root = (float)(s0 * s0);
     float ax = 0.f, ay = 0.f, az = 0.f;
     uint32_t cnt = 0;
     int nn = 0, nr = 0, nq = 0;
-    const Node32 root = load_node32(raw, 0);
+    const Node32 root = load_node32(raw, rid);
     const uint32_t end = root.skip;
-    uint32_t p = (root.meta & 32u) ? 0u : 1u;
+    uint32_t p = (root.meta & 32u) ? rid : rid + 1;
     while (p < end) {
         p = __builtin_amdgcn_readfirstlane(p);
         const Node32 nd = load_node32(raw, p);
@@ -718,26 +729,39 @@ __global__ __launch_bounds__(64) void k_walk_group32(WalkArgs a) {
 // same softened monopole term, j != i by index.  pot = -sum m_j / sqrt(r^2+eps^2).
 __global__ __launch_bounds__(kBlock) void k_direct(const double* __restrict__ px, const double* __restrict__ py,
                                                    const double* __restrict__ pz, const double* __restrict__ m,
-                                                   size_t n, const int64_t* __restrict__ targets, size_t nt,
+                                                   SysView sv, const int64_t* __restrict__ targets, size_t nt,
                                                    double eps2, double* __restrict__ acc, double* __restrict__ pot) {
 #pragma clang fp contract(off)
     __shared__ double sx[kBlock], sy[kBlock], sz[kBlock], sm[kBlock];
+    __shared__ unsigned long long s_lo, s_hi;
     const size_t k = blockIdx.x * (size_t)kBlock + threadIdx.x;
     const bool valid = k < nt;
     const int64_t ti = valid ? targets[k] : 0;
+    const uint32_t sys = sys_of(sv, (uint32_t)ti);
+    // sources: the target's own system; the block streams the union of its targets' ranges
+    const size_t lo = sv.off[sys], hi = sv.off[sys + 1];
+    if (threadIdx.x == 0) { s_lo = ~0ull; s_hi = 0ull; }
+    __syncthreads();
+    if (valid) {
+        atomicMin(&s_lo, (unsigned long long)lo);
+        atomicMax(&s_hi, (unsigned long long)hi);
+    }
+    __syncthreads();
+    const size_t blo = s_lo, bhi = s_hi;
     const double x = px[ti], y = py[ti], z = pz[ti];
     double ax = 0, ay = 0, az = 0, ph = 0;
-    for (size_t b = 0; b < n; b += kBlock) {
+    for (size_t b = blo; b < bhi; b += kBlock) {
         const size_t j = b + threadIdx.x;
         __syncthreads();
-        sx[threadIdx.x] = j < n ? px[j] : 0.0;
-        sy[threadIdx.x] = j < n ? py[j] : 0.0;
-        sz[threadIdx.x] = j < n ? pz[j] : 0.0;
-        sm[threadIdx.x] = j < n ? m[j] : 0.0;
+        sx[threadIdx.x] = j < bhi ? px[j] : 0.0;
+        sy[threadIdx.x] = j < bhi ? py[j] : 0.0;
+        sz[threadIdx.x] = j < bhi ? pz[j] : 0.0;
+        sm[threadIdx.x] = j < bhi ? m[j] : 0.0;
         __syncthreads();
-        const int lim = (int)((n - b) < (size_t)kBlock ? (n - b) : (size_t)kBlock);
+        const int lim = (int)((bhi - b) < (size_t)kBlock ? (bhi - b) : (size_t)kBlock);
         for (int t = 0; t < lim; ++t) {
-            if ((int64_t)(b + t) == ti) continue;
+            const size_t jj = b + t;
+            if ((int64_t)jj == ti || jj < lo || jj >= hi) continue;
             const double dx = sx[t] - x, dy = sy[t] - y, dz = sz[t] - z;
             const double d2 = dx * dx + dy * dy + dz * dz;
             term64(ax, ay, az, dx, dy, dz, d2, sm[t], eps2, nullptr);
@@ -802,8 +826,11 @@ hipError_t launch_walk_to(bhx_ctx* c, hipStream_t s, bool group, int precision, 
     a.nodes32 = c->tree.walk32;
     a.tb4 = c->tb4;
     a.tleaf = c->tleaf;
-    a.groups = (uint32_t)((c->n + 63) / 64);
-    a.b4 = c->bounds + 4 * c->tree_slot;
+    a.groups = c->ngroups;
+    a.gtab = c->gtab;
+    a.sys_root = c->sys_root;
+    a.abort_flag = &c->ctr->node_overflow;
+    a.b4 = c->bounds_slot(c->tree_slot);
     a.quad = c->tree.quad;
     a.quadf = c->tree.quadf;
     a.tx = c->tx; a.ty = c->ty; a.tz = c->tz; a.tm = c->tm;
@@ -816,7 +843,7 @@ hipError_t launch_walk_to(bhx_ctx* c, hipStream_t s, bool group, int precision, 
     a.eps2f = (float)a.eps2;
     a.ax = ox; a.ay = oy; a.az = oz;
     a.counts = counts;
-    const unsigned groups = (unsigned)((c->n + 63) / 64);
+    const unsigned groups = c->ngroups;
     const unsigned grid = (groups + kWaves - 1) / kWaves;
     const bool quad = c->cfg.quadrupole != 0;
     if (group) {
@@ -846,7 +873,7 @@ hipError_t launch_direct(bhx_ctx* c, const int64_t* targets, size_t nt, double e
                          hipStream_t s) {
     const unsigned grid = (unsigned)((nt + kBlock - 1) / kBlock);
     if (grid == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_direct, dim3(grid), dim3(kBlock), 0, s, c->px, c->py, c->pz, c->m, c->n, targets, nt,
+    hipLaunchKernelGGL(k_direct, dim3(grid), dim3(kBlock), 0, s, c->px, c->py, c->pz, c->m, c->sys(), targets, nt,
                        eps * eps, acc, pot);
     return hipGetLastError();
 }
