@@ -175,6 +175,13 @@ int bhx_interaction_counts(bhx_ctx *ctx, uint64_t *counts);
 /* One frame in the reference order (ofApp::update then ofApp::draw,
  * ofApp.cpp:169-177,248-282), repeated nsteps times on the resident bodies. */
 int bhx_step(bhx_ctx *ctx, int nsteps);
+
+/* bhx_step with every frame replayed from a captured HIP graph: no host wait
+ * between frames (the node count is checked once at the end; when a frame
+ * needed more nodes than the tree holds, the call restarts from a device copy
+ * of its starting state with larger arrays).  Results are identical to
+ * bhx_step.  (No reference counterpart.) */
+int bhx_step_graph(bhx_ctx *ctx, int nsteps);
 int bhx_last_timing(bhx_ctx *ctx, bhx_timing *t);
 /* 1: time each phase with events in bhx_step (adds synchronisation). */
 int bhx_set_profiling(bhx_ctx *ctx, int on);

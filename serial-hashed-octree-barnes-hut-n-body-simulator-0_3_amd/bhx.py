@@ -71,7 +71,7 @@ def lib():
             "bhx_direct": [P, P, S, D, P, P], "bhx_diagnostics": [P, P, P, P],
             "bhx_ic_plummer": [S, ctypes.c_uint64, D, D, P, P, P],
             "bhx_set_systems": [P, P, S], "bhx_num_systems": [P, P], "bhx_system_bounds": [P, P, P],
-            "bhx_build_systems": [P, P, P, P], "bhx_system_roots": [P, P], "bhx_lookup_in": [P, S, P, S, P],
+            "bhx_build_systems": [P, P, P, P], "bhx_step_graph": [P, I], "bhx_system_roots": [P, P], "bhx_lookup_in": [P, S, P, S, P],
         }
         for name, args in sigs.items():
             f = getattr(L, name)
@@ -294,6 +294,9 @@ class Engine:
 
     def step(self, nsteps=1):
         self._chk(lib().bhx_step(self.h, nsteps), "bhx_step")
+
+    def step_graph(self, nsteps=1):
+        self._chk(lib().bhx_step_graph(self.h, nsteps), "bhx_step_graph")
 
     def sync(self):
         self._chk(lib().bhx_sync(self.h), "bhx_sync")

@@ -79,6 +79,14 @@ hipError_t dalloc(T** p, size_t count) {
     return hipMalloc((void**)p, count * sizeof(T));
 }
 
+void drop_graphs(bhx_ctx* c) {
+    for (auto& g : c->graphs) {
+        if (g.exec) (void)hipGraphExecDestroy(g.exec);
+        g = bhx_ctx::StepGraph{};
+    }
+    c->graph_valid = false;
+}
+
 void free_tree(bhx_ctx* c) {
     DeviceTree& t = c->tree;
     void* ptrs[] = {t.key, t.start, t.count, t.skip, t.parent, t.child_byte, t.level, t.first, t.center,
@@ -89,6 +97,7 @@ void free_tree(bhx_ctx* c) {
 }
 
 hipError_t alloc_tree(bhx_ctx* c, size_t cap) {
+    drop_graphs(c);   // captured launches hold the old node arrays
     free_tree(c);
     DeviceTree& t = c->tree;
     hipError_t e;
@@ -117,7 +126,9 @@ hipError_t alloc_tree(bhx_ctx* c, size_t cap) {
 }
 
 void free_all(bhx_ctx* c) {
+    drop_graphs(c);
     free_tree(c);
+    if (c->snap) { (void)hipFree(c->snap); c->snap = nullptr; }
     double** dp[] = {&c->px, &c->py, &c->pz, &c->vx, &c->vy, &c->vz, &c->m, &c->ax, &c->ay, &c->az,
                      &c->cpx, &c->cpy, &c->cpz, &c->cvx, &c->cvy, &c->cvz, &c->tx, &c->ty, &c->tz, &c->tm,
                      &c->bounds, &c->stage};
@@ -217,6 +228,7 @@ int set_layout(bhx_ctx* c, const std::vector<uint32_t>& off) {
         HIPCHK(hipStreamSynchronize(s));
     }
     if (off != c->sys_off_h) c->tree_built = false;
+    drop_graphs(c);
     c->sys_off_h = off;
     c->nsys = nsys;
     c->max_sys_n = maxn;
@@ -260,6 +272,140 @@ int read_counters(bhx_ctx* c, hipStream_t s) {
     HIPCHK(hipStreamSynchronize(s));
     if (c->ctr_host->sort_fail) return fail(c, BHX_E_SORT, "radix sort look-back did not complete");
     if (c->ctr_host->hash_fail) return fail(c, BHX_E_HASH_FULL, "node hash table full");
+    return BHX_OK;
+}
+
+__global__ void k_fill_u32_api(uint32_t* __restrict__ p, size_t n, uint32_t v) {
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) p[i] = v;
+}
+
+// One frame in the reference order with no host wait: the tree is built into
+// the arrays it has (an overflow is flagged on the device, the walk then does
+// nothing), and the side branch computing the next bounds is joined at the
+// end so one frame is one closed unit of work.
+int enqueue_step_async(bhx_ctx* c, hipStream_t s) {
+    const int cur = c->bslot;
+    double* b_cur = c->bounds_slot(cur);
+    double* b_next = c->bounds_slot(cur ^ 1);
+    const SysView sv = c->sys();
+    const double dt = c->cfg.dt;
+    HIPCHK(launch_morton(c->px, c->py, c->pz, c->n, sv.id, b_cur, c->keys, c->vals, s));
+    HIPCHK(sort_pairs(c, c->keys, c->vals, s));
+    HIPCHK(launch_permute_kick_drift(c, c->vals, dt, s));
+    HIPCHK(hipEventRecord(c->ev_drift, s));
+    HIPCHK(hipStreamWaitEvent(c->s_side, c->ev_drift, 0));
+    HIPCHK(launch_bounds(c->px, c->py, c->pz, sv, c->max_sys_n, b_next, c->s_side));
+    HIPCHK(hipEventRecord(c->ev_bounds, c->s_side));
+    HIPCHK(launch_treekeys(c->px, c->py, c->pz, c->n, sv.id, b_cur, c->tkeys, c->tperm, s));
+    HIPCHK(sort_pairs(c, c->tkeys, c->tperm, s));
+    HIPCHK(launch_tree_gather(c, s));
+    size_t total = 0;
+    HIPCHK(build_tree(c, s, &total, false));
+    HIPCHK(launch_walk(c, s));
+    HIPCHK(launch_kick(c, dt, s));
+    HIPCHK(hipStreamWaitEvent(s, c->ev_bounds, 0));
+    c->bslot = cur ^ 1;
+    return BHX_OK;
+}
+
+void buffer_set(const bhx_ctx* c, const double** out) {
+    const double* a[16] = {c->px, c->py, c->pz, c->vx, c->vy, c->vz, c->m, c->cpx, c->cpy, c->cpz,
+                           c->cvx, c->cvy, c->cvz, c->ax, c->ay, c->az};
+    for (int k = 0; k < 16; ++k) {
+        out[k] = a[k];
+        out[16 + k] = c->alt[k];
+    }
+}
+
+// Captures one frame starting from the current buffer parity into `g`.
+int capture_step(bhx_ctx* c, bhx_ctx::StepGraph& g) {
+    buffer_set(c, g.bufs);
+    g.bslot = c->bslot;
+    HIPCHK(hipStreamBeginCapture(c->s_main, hipStreamCaptureModeRelaxed));
+    const int rc = enqueue_step_async(c, c->s_main);
+    hipGraph_t graph = nullptr;
+    const hipError_t e = hipStreamEndCapture(c->s_main, &graph);
+    if (rc != BHX_OK || e != hipSuccess) {
+        if (graph) (void)hipGraphDestroy(graph);
+        return rc != BHX_OK ? rc : fail(c, BHX_E_HIP, "hipStreamEndCapture", e);
+    }
+    const hipError_t ei = hipGraphInstantiate(&g.exec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    if (ei != hipSuccess) {
+        g.exec = nullptr;
+        return fail(c, BHX_E_HIP, "hipGraphInstantiate", ei);
+    }
+    return BHX_OK;
+}
+
+// The 16 state arrays of the bodies (positions, velocities, masses,
+// compensation terms, accelerations) and their gather twins.
+double** state_ptrs(bhx_ctx* c, double** out) {
+    double** a[16] = {&c->px, &c->py, &c->pz, &c->vx, &c->vy, &c->vz, &c->m, &c->cpx, &c->cpy, &c->cpz,
+                      &c->cvx, &c->cvy, &c->cvz, &c->ax, &c->ay, &c->az};
+    for (int k = 0; k < 16; ++k) out[k] = *a[k];
+    return out;
+}
+
+int take_snapshot(bhx_ctx* c, hipStream_t s) {
+    const size_t len = 16 * c->cap + 12 * c->sys_cap;
+    if (!c->snap || c->snap_len < len) {
+        if (c->snap) (void)hipFree(c->snap);
+        c->snap = nullptr;
+        c->snap_len = 0;
+        HIPCHK(dalloc(&c->snap, len));
+        c->snap_len = len;
+    }
+    double* cur[16];
+    state_ptrs(c, cur);
+    for (int k = 0; k < 16; ++k)
+        HIPCHK(hipMemcpyAsync(c->snap + k * c->cap, cur[k], c->n * sizeof(double), hipMemcpyDeviceToDevice, s));
+    HIPCHK(hipMemcpyAsync(c->snap + 16 * c->cap, c->bounds, 12 * c->sys_cap * sizeof(double), hipMemcpyDeviceToDevice,
+                          s));
+    for (int k = 0; k < 13; ++k) {
+        c->snap_ptr[k] = cur[k];
+        c->snap_ptr[13 + k] = c->alt[k];
+    }
+    c->snap_bslot = c->bslot;
+    return BHX_OK;
+}
+
+int restore_snapshot(bhx_ctx* c, hipStream_t s) {
+    double** a[13] = {&c->px, &c->py, &c->pz, &c->vx, &c->vy, &c->vz, &c->m, &c->cpx, &c->cpy, &c->cpz,
+                      &c->cvx, &c->cvy, &c->cvz};
+    for (int k = 0; k < 13; ++k) {
+        *a[k] = c->snap_ptr[k];
+        c->alt[k] = c->snap_ptr[13 + k];
+    }
+    c->bslot = c->snap_bslot;
+    double* cur[16];
+    state_ptrs(c, cur);
+    for (int k = 0; k < 16; ++k)
+        HIPCHK(hipMemcpyAsync(cur[k], c->snap + k * c->cap, c->n * sizeof(double), hipMemcpyDeviceToDevice, s));
+    HIPCHK(hipMemcpyAsync(c->bounds, c->snap + 16 * c->cap, 12 * c->sys_cap * sizeof(double), hipMemcpyDeviceToDevice,
+                          s));
+    return BHX_OK;
+}
+
+// Graph of the frame that starts from the current parity (capturing both
+// parities when none matches).
+int step_graph_for(bhx_ctx* c, hipGraphExec_t* out) {
+    const double* cur[32];
+    buffer_set(c, cur);
+    for (auto& g : c->graphs)
+        if (g.exec && g.bslot == c->bslot && std::equal(cur, cur + 32, g.bufs)) {
+            *out = g.exec;
+            return BHX_OK;
+        }
+    drop_graphs(c);
+    int rc = capture_step(c, c->graphs[0]);           // leaves the host state one frame on
+    if (rc == BHX_OK) rc = capture_step(c, c->graphs[1]);   // and back
+    if (rc != BHX_OK) {
+        drop_graphs(c);
+        return rc;
+    }
+    c->graph_valid = true;
+    *out = c->graphs[0].exec;
     return BHX_OK;
 }
 
@@ -360,6 +506,7 @@ int bhx_set_config(bhx_ctx* c, const bhx_config* cfg) {
     const bool tree_shape = cfg->leaf_capacity != c->cfg.leaf_capacity || cfg->mode != c->cfg.mode;
     c->cfg = *cfg;
     if (tree_shape) c->tree_valid = false;
+    drop_graphs(c);   // captured launches hold the old parameters
     return BHX_OK;
 }
 
@@ -915,6 +1062,61 @@ int bhx_step(bhx_ctx* c, int nsteps) {
         HIPCHK(hipMemcpy(c->tree_bounds_h.data(), c->bounds_slot(c->tree_slot), c->tree_bounds_h.size() * sizeof(double),
                          hipMemcpyDeviceToHost));
     }
+    const auto t1 = std::chrono::steady_clock::now();
+    c->timing.total = std::chrono::duration<double, std::milli>(t1 - t0).count();
+    return BHX_OK;
+}
+
+int bhx_step_graph(bhx_ctx* c, int nsteps) {
+    if (!c) return BHX_E_NULL;
+    if (nsteps < 0) return fail(c, BHX_E_ARG, "negative step count");
+    if (nsteps == 0 || c->n == 0) return BHX_OK;
+    HIPCHK(hipSetDevice(c->device));
+    hipStream_t s = c->s_main;
+    const auto t0 = std::chrono::steady_clock::now();
+    HIPCHK(hipStreamSynchronize(c->s_side));
+    if (!c->bounds_valid) {
+        HIPCHK(launch_bounds(c->px, c->py, c->pz, c->sys(), c->max_sys_n, c->bounds_slot(c->bslot), s));
+        c->bounds_valid = true;
+    }
+    int rc = take_snapshot(c, s);
+    if (rc != BHX_OK) return rc;
+    bool done = false;
+    for (int attempt = 0; attempt < 3 && !done; ++attempt) {
+        hipLaunchKernelGGL(k_fill_u32_api, dim3(1), dim3(64), 0, s, &c->ctr->overflow_seen, (size_t)1, 0u);
+        HIPCHK(hipGetLastError());
+        c->tree_valid = c->tree_built = false;
+        for (int k = 0; k < nsteps; ++k) {
+            hipGraphExec_t g = nullptr;
+            rc = step_graph_for(c, &g);
+            if (rc != BHX_OK) return rc;
+            HIPCHK(hipGraphLaunch(g, s));
+            swap_body_buffers(c);      // the frame's permute/kick/drift exchanged them
+            c->tree_slot = c->bslot;
+            c->bslot ^= 1;
+        }
+        rc = read_counters(c, s);
+        if (rc != BHX_OK) return rc;
+        if (!c->ctr_host->overflow_seen) {
+            done = true;
+            break;
+        }
+        // some frame needed more nodes than the tree holds: back to the start, grow, run again
+        const size_t need = c->ctr_host->max_nodes;
+        rc = restore_snapshot(c, s);
+        if (rc != BHX_OK) return rc;
+        HIPCHK(hipStreamSynchronize(s));
+        HIPCHK(alloc_tree(c, need + need / 4 + 1024));
+    }
+    if (!done) return fail(c, BHX_E_NOMEM, "node capacity");
+    HIPCHK(hipStreamSynchronize(c->s_side));
+    c->n_nodes = c->ctr_host->total_nodes;
+    c->tree_valid = c->tree_built = true;
+    c->tree_n = c->n;
+    c->bounds_valid = true;
+    c->tree_bounds_h.resize(4 * (size_t)c->nsys);
+    HIPCHK(hipMemcpy(c->tree_bounds_h.data(), c->bounds_slot(c->tree_slot), c->tree_bounds_h.size() * sizeof(double),
+                     hipMemcpyDeviceToHost));
     const auto t1 = std::chrono::steady_clock::now();
     c->timing.total = std::chrono::duration<double, std::milli>(t1 - t0).count();
     return BHX_OK;

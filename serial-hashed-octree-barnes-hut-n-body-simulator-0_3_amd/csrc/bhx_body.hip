@@ -408,6 +408,8 @@ hipError_t launch_permute_kick_drift(bhx_ctx* c, const uint32_t* perm, double dt
     return e;
 }
 
+void swap_body_buffers(bhx_ctx* c) { swap_alt(c, c->cfg.compensated != 0); }
+
 hipError_t launch_kick_drift(bhx_ctx* c, double dt, hipStream_t s) {
     return launch_permute_kick_drift(c, nullptr, dt, s);
 }

@@ -162,10 +162,19 @@ struct bhx_ctx {
     size_t gtab_cap = 0;
     uint32_t ngroups = 0;
     std::vector<double> tree_bounds_h;         // (centre, size) per system the last tree was built with
-    // captured step graph (bhx_step_graph)
-    hipGraphExec_t graph_exec = nullptr;
+    // captured one-step graphs (bhx_step_graph): one per parity of the body
+    // ping-pong buffers / bounds slot, each valid for the state it starts from
+    struct StepGraph {
+        hipGraphExec_t exec = nullptr;
+        const double* bufs[32] = {};           // body arrays and gather targets at capture
+        int bslot = -1;
+    } graphs[2];
     bool graph_valid = false;
-    int graph_steps = 0;                       // steps one replay advances
+    // device snapshot of the state a graph run starts from (restored on node overflow)
+    double* snap = nullptr;
+    size_t snap_len = 0;
+    double* snap_ptr[26] = {};
+    int snap_bslot = 0;
     // segmented sort scratch (several systems only)
     uint64_t *skey = nullptr, *skey_alt = nullptr;
     uint32_t *sidx = nullptr, *sidx_alt = nullptr;
@@ -199,6 +208,8 @@ hipError_t launch_permute_kick_drift(bhx_ctx* c, const uint32_t* perm, double dt
 hipError_t launch_kick_drift(bhx_ctx* c, double dt, hipStream_t s);
 hipError_t launch_kick(bhx_ctx* c, double dt, hipStream_t s);
 hipError_t launch_tree_gather(bhx_ctx* c, hipStream_t s);
+// exchange the body arrays with their gather targets, as a permute/kick/drift does
+void swap_body_buffers(bhx_ctx* c);
 
 hipError_t build_tree(bhx_ctx* c, hipStream_t s, size_t* total_out, bool sync);
 hipError_t launch_lookup(bhx_ctx* c, uint32_t sys, const uint64_t* keys, size_t n, int64_t* ids, hipStream_t s);
