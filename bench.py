@@ -79,6 +79,87 @@ def force_error(e, bhx, nsub, seed, eps):
             "p99": float(np.percentile(err, 99)), "subsample": int(tg.size)}
 
 
+def total_energy(e, eps):
+    """Kinetic + softened potential energy, the potential by FP64 direct sum over all bodies."""
+    ek, _, _ = e.diagnostics()
+    _, _, mass, _ = e.get_bodies()
+    _, pot = e.direct(np.arange(e.n), eps)
+    return ek + 0.5 * float(np.dot(mass, pot))
+
+
+def energy_drift(bhx, n=16384, steps=100, seed=1):
+    """Relative total-energy change over `steps` frames of an N=16384 Plummer sphere
+    (theta 0.5, eps 0.01, dt 1e-3): the reference's own configuration (leaf capacity 1,
+    as-run payload, FP64 per-body walk) and the fast one (bucket 32, FP32 group walk with
+    quadrupoles, compensated FP64 leapfrog)."""
+    pos, vel, mass = bhx.plummer(n, seed=seed)
+    out = {"n": n, "steps": steps, "dt": 1e-3, "theta": 0.5, "eps": 0.01}
+    cfgs = {"reference_fp64": dict(theta=0.5, eps=0.01, dt=1e-3),
+            "fast_fp32": dict(theta=0.5, eps=0.01, dt=1e-3, leaf_capacity=32, mode=bhx.INTENDED,
+                              precision=bhx.FP32, quadrupole=1, compensated=1, group_walk=1)}
+    for name, cfg in cfgs.items():
+        with bhx.Engine(n, **cfg) as e:
+            e.set_bodies(pos, vel, mass)
+            e0 = total_energy(e, 0.01)
+            e.step_graph(steps)
+            e1 = total_energy(e, 0.01)
+        out[name] = {"e0": e0, "e1": e1, "rel_drift": (e1 - e0) / abs(e0)}
+    return out
+
+
+def ensemble_bench(bhx, nsys, steps, warmup, seed=1):
+    """Batched ensemble (all systems in one context, one graph launch per frame) against the
+    same systems stepped one after another (one system at a time in one context, eager frames).
+    Both start from the same state and run the same frames; final states must agree bitwise."""
+    t0 = time.perf_counter()
+    pos, vel, mass, off = bhx.ensemble(nsys, seed=seed)
+    n = pos.shape[0]
+    sizes = np.diff(off).astype(np.int64)
+    log("ensemble: %d systems, %d bodies (%.1f s to generate)" % (nsys, n, time.perf_counter() - t0))
+    cfg = dict(theta=0.5, eps=0.01, dt=5e-4, leaf_capacity=32, mode=bhx.INTENDED, precision=bhx.FP32,
+               quadrupole=1, compensated=1, group_walk=1)
+    res = {"systems": nsys, "bodies": n, "sizes": {"min": int(sizes.min()), "max": int(sizes.max())},
+           "theta": 0.5, "eps": 0.01, "dt": 5e-4, "warmup": warmup, "steps": steps}
+    with bhx.Engine(n, **cfg) as e:
+        e.set_bodies(pos, vel, mass)
+        e.set_systems(off)
+        chosen, table = e.autotune(caps=(16, 32, 64), steps=2, max_err=1e-3)
+        res["autotune"] = {"chosen_leaf_capacity": chosen,
+                           "candidates": [{"leaf_capacity": c, "ms_per_step": ms, "median_err": er}
+                                          for c, ms, er in table]}
+        cfg["leaf_capacity"] = chosen
+        log("ensemble autotune: leaf capacity %d %s" % (chosen, table))
+        e.step_graph(warmup)
+        e.sync()
+        t0 = time.perf_counter()
+        e.step_graph(steps)
+        e.sync()
+        tb = time.perf_counter() - t0
+        P, V, M, A = e.get_bodies()
+    log("ensemble batched: %.2f ms/step" % (1e3 * tb / steps))
+    tl = 0.0
+    same = True
+    with bhx.Engine(int(sizes.max()), **cfg) as s:
+        for k in range(nsys):
+            a, b = int(off[k]), int(off[k + 1])
+            s.set_bodies(pos[a:b], vel[a:b], mass[a:b])
+            s.step(warmup)
+            s.sync()
+            t0 = time.perf_counter()
+            s.step(steps)
+            s.sync()
+            tl += time.perf_counter() - t0
+            p1, v1, m1, a1 = s.get_bodies()
+            same = same and np.array_equal(P[a:b], p1) and np.array_equal(V[a:b], v1) and np.array_equal(A[a:b], a1)
+    log("ensemble looped: %.2f ms/step, bitwise equal: %s" % (1e3 * tl / steps, same))
+    res.update({"batched_ms_per_step": 1e3 * tb / steps, "looped_ms_per_step": 1e3 * tl / steps,
+                "batched_body_updates_per_s": n * steps / tb, "looped_body_updates_per_s": n * steps / tl,
+                "batched_over_looped": tl / tb, "bitwise_equal_to_looped": bool(same),
+                "looped": "each system alone in one context, bhx_step per system (eager frames)",
+                "batched": "all systems in one context (bhx_set_systems), bhx_step_graph"})
+    return res
+
+
 def cpu_baseline(pos, vel, mass, theta, eps, dt, cap, nsub, seed):
     """Serial oracle (oracle/liboracle.so, one core) on a bounded sample of the workload:
     every O(N log N) phase over all N bodies, the walk for `nsub` seeded bodies,
@@ -182,6 +263,9 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-walk-sample", type=int, default=1 << 19)
     ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--no-ensemble", action="store_true")
+    ap.add_argument("--ensemble-systems", type=int, default=512)
+    ap.add_argument("--no-drift", action="store_true")
     ap.add_argument("--calibrate-copy-mib", type=int, default=0,
                     help="first copy this many MiB device-to-device (counter calibration runs)")
     args = ap.parse_args()
@@ -259,6 +343,11 @@ def main():
                              "achieved_tflops": ach64, "frac_fp64_peak": ach64 / PEAK_FP64_TFLOPS,
                              "phases_ms": {k: round(v, 4) for k, v in f64["phases"].items()}}
         line["fp32_over_fp64_walk"] = f64["walk_ms"] / walk_ms
+    if not args.no_ensemble:
+        line["ensemble"] = ensemble_bench(bhx, args.ensemble_systems, 10, 2)
+    if not args.no_drift:
+        log("energy drift (N=16384, 100 steps)")
+        line["energy_drift"] = energy_drift(bhx)
     if not args.no_cpu_baseline:
         log("cpu baseline (serial oracle, bounded sample)")
         line["cpu_baseline"] = cpu_baseline(pos, vel, mass, theta, eps, dt, cap, args.cpu_walk_sample, 99)

@@ -210,6 +210,24 @@ int bhx_system_roots(bhx_ctx *ctx, uint64_t *roots);
 /* bhx_lookup in one system's tree (bhx_lookup is system 0). */
 int bhx_lookup_in(bhx_ctx *ctx, size_t system, const uint64_t *keys, size_t n, int64_t *ids);
 
+/* One candidate of bhx_autotune. */
+typedef struct bhx_tune_entry {
+    int32_t leaf_capacity;
+    int32_t pad;
+    double ms_per_step;     /* wall time per bhx_step_graph frame */
+    double median_err;      /* median |a - a*| / |a*| on the sample, a* = FP64 direct sum */
+} bhx_tune_entry;
+
+/* Picks the leaf capacity (bodies per leaf bucket, which sets how much of
+ * each walk group's list is direct body terms): from the current state,
+ * for each of the ncaps candidates runs `steps` graph frames, times them and
+ * measures the median force error of the last frame on `nsample` seeded
+ * bodies; keeps the fastest candidate whose median error is <= max_err (the
+ * configured capacity if none is).  The bodies are left in their starting
+ * state; out (may be NULL) receives one entry per candidate. */
+int bhx_autotune(bhx_ctx *ctx, const int32_t *caps, size_t ncaps, int steps, double max_err, size_t nsample,
+                 uint64_t seed, bhx_tune_entry *out, int32_t *chosen);
+
 /* ---- diagnostics (no reference counterpart) ------------------------------ */
 
 /* FP64 direct sum over the target's own system for each target (array
@@ -228,6 +246,17 @@ int bhx_diagnostics(bhx_ctx *ctx, double *ekin, double mom[3], double angmom[3])
  * 0.999, shifted to the centre-of-mass frame. */
 int bhx_ic_plummer(size_t n, uint64_t seed, double total_mass, double a,
                    double *pos, double *vel, double *mass);
+
+/* Ensemble of Plummer systems: N_k = round(exp(U[ln nmin, ln nmax])), total
+ * mass and scale radius 1 + jitter * U[-1, 1] (masses, radii may be NULL). */
+int bhx_ic_ensemble_sizes(size_t nsys, uint64_t seed, uint64_t nmin, uint64_t nmax, double jitter,
+                          uint64_t *sizes, double *masses, double *radii);
+
+/* Bodies of that ensemble, system after system (system k from
+ * bhx_ic_plummer(N_k, seed + 1 + k, M_k, a_k)), on up to `threads` host
+ * threads (0: all). */
+int bhx_ic_ensemble(size_t nsys, uint64_t seed, const uint64_t *sizes, const double *masses,
+                    const double *radii, int threads, double *pos, double *vel, double *mass);
 
 #ifdef __cplusplus
 }

@@ -27,6 +27,7 @@ NODE_DTYPE = np.dtype([
     ("skip", "<i8"), ("body_start", "<i8"),
 ])
 assert NODE_DTYPE.itemsize == 160
+TUNE_DTYPE = np.dtype([("leaf_capacity", "<i4"), ("pad", "<i4"), ("ms_per_step", "<f8"), ("median_err", "<f8")])
 
 
 class BhxError(RuntimeError):
@@ -70,8 +71,11 @@ def lib():
             "bhx_step": [P, I], "bhx_last_timing": [P, P], "bhx_set_profiling": [P, I],
             "bhx_direct": [P, P, S, D, P, P], "bhx_diagnostics": [P, P, P, P],
             "bhx_ic_plummer": [S, ctypes.c_uint64, D, D, P, P, P],
+            "bhx_ic_ensemble_sizes": [S, ctypes.c_uint64, ctypes.c_uint64, ctypes.c_uint64, D, P, P, P],
+            "bhx_ic_ensemble": [S, ctypes.c_uint64, P, P, P, I, P, P, P],
             "bhx_set_systems": [P, P, S], "bhx_num_systems": [P, P], "bhx_system_bounds": [P, P, P],
-            "bhx_build_systems": [P, P, P, P], "bhx_step_graph": [P, I], "bhx_system_roots": [P, P], "bhx_lookup_in": [P, S, P, S, P],
+            "bhx_build_systems": [P, P, P, P], "bhx_step_graph": [P, I],
+            "bhx_autotune": [P, P, S, I, D, S, ctypes.c_uint64, P, P], "bhx_system_roots": [P, P], "bhx_lookup_in": [P, S, P, S, P],
         }
         for name, args in sigs.items():
             f = getattr(L, name)
@@ -103,6 +107,24 @@ def plummer(n, seed=1, total_mass=1.0, a=1.0):
     if rc:
         raise BhxError(rc, "bhx_ic_plummer")
     return pos, vel, mass
+
+
+def ensemble(nsys, seed=1, nmin=1024, nmax=65536, jitter=0.2, threads=0):
+    """Seeded ensemble of Plummer systems (host generator of the library).
+    Returns pos (N,3), vel (N,3), mass (N,), offsets (nsys+1,) with N = sum of the sizes."""
+    sizes = np.empty(nsys, np.uint64)
+    ms = np.empty(nsys)
+    rs = np.empty(nsys)
+    rc = lib().bhx_ic_ensemble_sizes(nsys, seed, nmin, nmax, jitter, _p(sizes), _p(ms), _p(rs))
+    if rc:
+        raise BhxError(rc, "bhx_ic_ensemble_sizes")
+    n = int(sizes.sum())
+    pos, vel, mass = np.empty((n, 3)), np.empty((n, 3)), np.empty(n)
+    rc = lib().bhx_ic_ensemble(nsys, seed, _p(sizes), _p(ms), _p(rs), threads, _p(pos), _p(vel), _p(mass))
+    if rc:
+        raise BhxError(rc, "bhx_ic_ensemble")
+    off = np.concatenate([[0], np.cumsum(sizes)]).astype(np.uint64)
+    return pos, vel, mass, off
 
 
 def default_config(**kw):
@@ -297,6 +319,17 @@ class Engine:
 
     def step_graph(self, nsteps=1):
         self._chk(lib().bhx_step_graph(self.h, nsteps), "bhx_step_graph")
+
+    def autotune(self, caps=(8, 16, 24, 32, 48, 64), steps=3, max_err=1e-3, nsample=4096, seed=7):
+        """Leaf capacity with the fastest graph frames at median force error <= max_err (set on
+        the context; bodies left as they were).  Returns (chosen, [(cap, ms_per_step, median_err)])."""
+        caps = np.ascontiguousarray(caps, dtype=np.int32)
+        out = np.zeros(caps.size, TUNE_DTYPE)
+        ch = ctypes.c_int32()
+        self._chk(lib().bhx_autotune(self.h, _p(caps), caps.size, steps, max_err, nsample, seed, _p(out),
+                                     ctypes.byref(ch)), "bhx_autotune")
+        self.cfg.leaf_capacity = ch.value
+        return ch.value, [(int(r["leaf_capacity"]), float(r["ms_per_step"]), float(r["median_err"])) for r in out]
 
     def sync(self):
         self._chk(lib().bhx_sync(self.h), "bhx_sync")

@@ -27,7 +27,7 @@ UNITS = {
     "bhx_walk.hip": [],
     "bhx_api.hip": [],
 }
-HOST_UNITS = {"bhx_ic.cpp": ["-O2", "-fPIC", "-std=c++17", "-ffp-contract=off"]}
+HOST_UNITS = {"bhx_ic.cpp": ["-O2", "-fPIC", "-std=c++17", "-ffp-contract=off", "-pthread"]}
 
 
 def _run(cmd):
@@ -52,7 +52,7 @@ def build(verbose=False):
                 sys.stderr.write(r.stdout + r.stderr)
     objs = [os.path.join(OUT_DIR, s.rsplit(".", 1)[0] + ".o") for s in list(UNITS) + list(HOST_UNITS)]
     tmp = LIB + ".tmp"
-    _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp, *objs])
+    _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-pthread", "-o", tmp, *objs])
     os.replace(tmp, LIB)
     return LIB
 

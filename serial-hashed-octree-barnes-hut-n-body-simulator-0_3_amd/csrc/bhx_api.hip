@@ -128,7 +128,8 @@ hipError_t alloc_tree(bhx_ctx* c, size_t cap) {
 void free_all(bhx_ctx* c) {
     drop_graphs(c);
     free_tree(c);
-    if (c->snap) { (void)hipFree(c->snap); c->snap = nullptr; }
+    for (auto& sn : c->snaps)
+        if (sn.buf) { (void)hipFree(sn.buf); sn.buf = nullptr; }
     double** dp[] = {&c->px, &c->py, &c->pz, &c->vx, &c->vy, &c->vz, &c->m, &c->ax, &c->ay, &c->az,
                      &c->cpx, &c->cpy, &c->cpz, &c->cvx, &c->cvy, &c->cvz, &c->tx, &c->ty, &c->tz, &c->tm,
                      &c->bounds, &c->stage};
@@ -275,6 +276,16 @@ int read_counters(bhx_ctx* c, hipStream_t s) {
     return BHX_OK;
 }
 
+__global__ void k_pick3(const double* __restrict__ x, const double* __restrict__ y, const double* __restrict__ z,
+                        const int64_t* __restrict__ idx, size_t n, double* __restrict__ out) {
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        const int64_t j = idx[i];
+        out[3 * i] = x[j];
+        out[3 * i + 1] = y[j];
+        out[3 * i + 2] = z[j];
+    }
+}
+
 __global__ void k_fill_u32_api(uint32_t* __restrict__ p, size_t n, uint32_t v) {
     for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) p[i] = v;
 }
@@ -347,43 +358,46 @@ double** state_ptrs(bhx_ctx* c, double** out) {
     return out;
 }
 
-int take_snapshot(bhx_ctx* c, hipStream_t s) {
+int take_snapshot(bhx_ctx* c, bhx_ctx::Snapshot& sn, hipStream_t s) {
     const size_t len = 16 * c->cap + 12 * c->sys_cap;
-    if (!c->snap || c->snap_len < len) {
-        if (c->snap) (void)hipFree(c->snap);
-        c->snap = nullptr;
-        c->snap_len = 0;
-        HIPCHK(dalloc(&c->snap, len));
-        c->snap_len = len;
+    if (!sn.buf || sn.len < len) {
+        if (sn.buf) (void)hipFree(sn.buf);
+        sn.buf = nullptr;
+        sn.len = 0;
+        HIPCHK(dalloc(&sn.buf, len));
+        sn.len = len;
     }
     double* cur[16];
     state_ptrs(c, cur);
     for (int k = 0; k < 16; ++k)
-        HIPCHK(hipMemcpyAsync(c->snap + k * c->cap, cur[k], c->n * sizeof(double), hipMemcpyDeviceToDevice, s));
-    HIPCHK(hipMemcpyAsync(c->snap + 16 * c->cap, c->bounds, 12 * c->sys_cap * sizeof(double), hipMemcpyDeviceToDevice,
+        HIPCHK(hipMemcpyAsync(sn.buf + k * c->cap, cur[k], c->n * sizeof(double), hipMemcpyDeviceToDevice, s));
+    HIPCHK(hipMemcpyAsync(sn.buf + 16 * c->cap, c->bounds, 12 * c->sys_cap * sizeof(double), hipMemcpyDeviceToDevice,
                           s));
     for (int k = 0; k < 13; ++k) {
-        c->snap_ptr[k] = cur[k];
-        c->snap_ptr[13 + k] = c->alt[k];
+        sn.ptr[k] = cur[k];
+        sn.ptr[13 + k] = c->alt[k];
     }
-    c->snap_bslot = c->bslot;
+    sn.bslot = c->bslot;
+    sn.bounds_valid = c->bounds_valid;
     return BHX_OK;
 }
 
-int restore_snapshot(bhx_ctx* c, hipStream_t s) {
+int restore_snapshot(bhx_ctx* c, const bhx_ctx::Snapshot& sn, hipStream_t s) {
     double** a[13] = {&c->px, &c->py, &c->pz, &c->vx, &c->vy, &c->vz, &c->m, &c->cpx, &c->cpy, &c->cpz,
                       &c->cvx, &c->cvy, &c->cvz};
     for (int k = 0; k < 13; ++k) {
-        *a[k] = c->snap_ptr[k];
-        c->alt[k] = c->snap_ptr[13 + k];
+        *a[k] = sn.ptr[k];
+        c->alt[k] = sn.ptr[13 + k];
     }
-    c->bslot = c->snap_bslot;
+    c->bslot = sn.bslot;
+    c->bounds_valid = sn.bounds_valid;
     double* cur[16];
     state_ptrs(c, cur);
     for (int k = 0; k < 16; ++k)
-        HIPCHK(hipMemcpyAsync(cur[k], c->snap + k * c->cap, c->n * sizeof(double), hipMemcpyDeviceToDevice, s));
-    HIPCHK(hipMemcpyAsync(c->bounds, c->snap + 16 * c->cap, 12 * c->sys_cap * sizeof(double), hipMemcpyDeviceToDevice,
+        HIPCHK(hipMemcpyAsync(cur[k], sn.buf + k * c->cap, c->n * sizeof(double), hipMemcpyDeviceToDevice, s));
+    HIPCHK(hipMemcpyAsync(c->bounds, sn.buf + 16 * c->cap, 12 * c->sys_cap * sizeof(double), hipMemcpyDeviceToDevice,
                           s));
+    c->tree_valid = c->tree_built = false;
     return BHX_OK;
 }
 
@@ -1079,7 +1093,7 @@ int bhx_step_graph(bhx_ctx* c, int nsteps) {
         HIPCHK(launch_bounds(c->px, c->py, c->pz, c->sys(), c->max_sys_n, c->bounds_slot(c->bslot), s));
         c->bounds_valid = true;
     }
-    int rc = take_snapshot(c, s);
+    int rc = take_snapshot(c, c->snaps[0], s);
     if (rc != BHX_OK) return rc;
     bool done = false;
     for (int attempt = 0; attempt < 3 && !done; ++attempt) {
@@ -1103,7 +1117,7 @@ int bhx_step_graph(bhx_ctx* c, int nsteps) {
         }
         // some frame needed more nodes than the tree holds: back to the start, grow, run again
         const size_t need = c->ctr_host->max_nodes;
-        rc = restore_snapshot(c, s);
+        rc = restore_snapshot(c, c->snaps[0], s);
         if (rc != BHX_OK) return rc;
         HIPCHK(hipStreamSynchronize(s));
         HIPCHK(alloc_tree(c, need + need / 4 + 1024));
@@ -1119,6 +1133,110 @@ int bhx_step_graph(bhx_ctx* c, int nsteps) {
                      hipMemcpyDeviceToHost));
     const auto t1 = std::chrono::steady_clock::now();
     c->timing.total = std::chrono::duration<double, std::milli>(t1 - t0).count();
+    return BHX_OK;
+}
+
+int bhx_autotune(bhx_ctx* c, const int32_t* caps, size_t ncaps, int steps, double max_err, size_t nsample,
+                 uint64_t seed, bhx_tune_entry* out, int32_t* chosen) {
+    if (!c || !caps) return c ? fail(c, BHX_E_NULL, "caps is a null pointer") : BHX_E_NULL;
+    if (ncaps == 0 || steps < 1 || nsample == 0) return fail(c, BHX_E_ARG, "empty tuning set");
+    for (size_t k = 0; k < ncaps; ++k)
+        if (caps[k] < 1 || caps[k] > kMaxLeafCap) return fail(c, BHX_E_ARG, "leaf capacity out of range");
+    if (c->n == 0) return fail(c, BHX_E_STATE, "no bodies");
+    HIPCHK(hipSetDevice(c->device));
+    hipStream_t s = c->s_main;
+    HIPCHK(hipStreamSynchronize(c->s_side));
+    if (!c->bounds_valid) {
+        HIPCHK(launch_bounds(c->px, c->py, c->pz, c->sys(), c->max_sys_n, c->bounds_slot(c->bslot), s));
+        c->bounds_valid = true;
+    }
+    int rc = take_snapshot(c, c->snaps[1], s);
+    if (rc != BHX_OK) return rc;
+    // seeded sample of body positions (sorted: the direct sum streams each target's system once per block)
+    const size_t ns = nsample < c->n ? nsample : c->n;
+    std::vector<int64_t> tg(ns);
+    {
+        uint64_t st = seed * 0x9e3779b97f4a7c15ull + 1;
+        std::vector<char> used(c->n, 0);
+        for (size_t i = 0; i < ns;) {
+            st += 0x9e3779b97f4a7c15ull;
+            uint64_t z = st;
+            z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+            z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+            z ^= z >> 31;
+            const size_t j = (size_t)(z % c->n);
+            if (!used[j]) { used[j] = 1; tg[i++] = (int64_t)j; }
+        }
+        std::sort(tg.begin(), tg.end());
+    }
+    int64_t* dt = nullptr;
+    double *dref = nullptr, *dwalk = nullptr;
+    HIPCHK(dalloc(&dt, ns));
+    HIPCHK(dalloc(&dref, 3 * ns));
+    HIPCHK(dalloc(&dwalk, 3 * ns));
+    HIPCHK(hipMemcpyAsync(dt, tg.data(), ns * sizeof(int64_t), hipMemcpyHostToDevice, s));
+    const bhx_config base = c->cfg;
+    int32_t best = base.leaf_capacity;
+    double best_ms = 0.0;
+    bool found = false;
+    std::vector<double> ref(3 * ns), walk(3 * ns), err(ns);
+    for (size_t k = 0; k < ncaps && rc == BHX_OK; ++k) {
+        bhx_config cfg = base;
+        cfg.leaf_capacity = caps[k];
+        rc = restore_snapshot(c, c->snaps[1], s);
+        if (rc != BHX_OK) break;
+        c->cfg = cfg;
+        drop_graphs(c);
+        rc = bhx_step_graph(c, 1);                     // capture + first frame
+        if (rc != BHX_OK) break;
+        const auto t0 = std::chrono::steady_clock::now();
+        rc = bhx_step_graph(c, steps);
+        const auto t1 = std::chrono::steady_clock::now();
+        if (rc != BHX_OK) break;
+        const double ms = std::chrono::duration<double, std::milli>(t1 - t0).count() / steps;
+        // the last frame's walk is at the current positions: compare with the FP64 direct sum
+        hipError_t e = launch_direct(c, dt, ns, c->cfg.eps, dref, nullptr, s);
+        if (e == hipSuccess) {
+            hipLaunchKernelGGL(k_pick3, dim3(grid_for(ns)), dim3(kBlock), 0, s, c->ax, c->ay, c->az, dt, ns, dwalk);
+            e = hipGetLastError();
+        }
+        if (e == hipSuccess) e = hipMemcpyAsync(ref.data(), dref, 3 * ns * sizeof(double), hipMemcpyDeviceToHost, s);
+        if (e == hipSuccess) e = hipMemcpyAsync(walk.data(), dwalk, 3 * ns * sizeof(double), hipMemcpyDeviceToHost, s);
+        if (e == hipSuccess) e = hipStreamSynchronize(s);
+        if (e != hipSuccess) { rc = fail(c, BHX_E_HIP, "autotune error sample", e); break; }
+        for (size_t i = 0; i < ns; ++i) {
+            const double dx = walk[3 * i] - ref[3 * i], dy = walk[3 * i + 1] - ref[3 * i + 1],
+                         dz = walk[3 * i + 2] - ref[3 * i + 2];
+            const double r = std::sqrt(ref[3 * i] * ref[3 * i] + ref[3 * i + 1] * ref[3 * i + 1] +
+                                       ref[3 * i + 2] * ref[3 * i + 2]);
+            err[i] = r > 0.0 ? std::sqrt(dx * dx + dy * dy + dz * dz) / r : 0.0;
+        }
+        std::nth_element(err.begin(), err.begin() + ns / 2, err.end());
+        const double med = err[ns / 2];
+        if (out) {
+            out[k].leaf_capacity = caps[k];
+            out[k].pad = 0;
+            out[k].ms_per_step = ms;
+            out[k].median_err = med;
+        }
+        if (med <= max_err && (!found || ms < best_ms)) {
+            found = true;
+            best = caps[k];
+            best_ms = ms;
+        }
+    }
+    (void)hipFree(dt);
+    (void)hipFree(dref);
+    (void)hipFree(dwalk);
+    // back to the starting state, with the chosen capacity
+    c->cfg = base;
+    c->cfg.leaf_capacity = best;
+    drop_graphs(c);
+    const int rc2 = restore_snapshot(c, c->snaps[1], s);
+    HIPCHK(hipStreamSynchronize(s));
+    if (rc != BHX_OK) return rc;
+    if (rc2 != BHX_OK) return rc2;
+    if (chosen) *chosen = best;
     return BHX_OK;
 }
 

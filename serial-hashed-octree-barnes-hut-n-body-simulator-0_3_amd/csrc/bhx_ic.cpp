@@ -7,9 +7,12 @@
 // Plummer sampling follows Aarseth, Henon & Wielen (1974): radius from the
 // inverse cumulative mass, isotropic directions, speed by von Neumann
 // rejection from g(q) = q^2 (1 - q^2)^{7/2} times the local escape speed.
+#include <atomic>
 #include <cmath>
 #include <cstddef>
 #include <cstdint>
+#include <thread>
+#include <vector>
 
 #include "../../include/bhx.h"
 
@@ -123,4 +126,56 @@ extern "C" int bhx_ic_plummer(size_t n, uint64_t seed, double total_mass, double
                 vel[3 * i + k] -= cm[3 + k] / msum;
             }
     return BHX_OK;
+}
+
+// Sizes, masses and scale radii of an ensemble of Plummer systems: system k
+// has N_k = round(exp(U[ln nmin, ln nmax])) bodies (log-uniform), total mass
+// and scale radius 1 + jitter * U[-1, 1], each draw from Philox with counter
+// (k, draw, "ENSM").
+extern "C" int bhx_ic_ensemble_sizes(size_t nsys, uint64_t seed, uint64_t nmin, uint64_t nmax, double jitter,
+                                     uint64_t* sizes, double* masses, double* radii) {
+    if (!sizes) return BHX_E_NULL;
+    if (nmin == 0 || nmax < nmin || !(jitter >= 0.0) || !(jitter < 1.0)) return BHX_E_ARG;
+    const double l0 = std::log((double)nmin), l1 = std::log((double)nmax);
+    for (size_t k = 0; k < nsys; ++k) {
+        uint32_t r[4];
+        philox4x32_10(r, seed, (uint32_t)k, 0u, 0x454e534du /* "ENSM" */, 0u);
+        const double u = u01(r[0], r[1]);
+        uint32_t r2[4];
+        philox4x32_10(r2, seed, (uint32_t)k, 1u, 0x454e534du, 0u);
+        double nk = std::nearbyint(std::exp(l0 + u * (l1 - l0)));
+        if (nk < (double)nmin) nk = (double)nmin;
+        if (nk > (double)nmax) nk = (double)nmax;
+        sizes[k] = (uint64_t)nk;
+        if (masses) masses[k] = 1.0 + jitter * (2.0 * u01(r[2], r[3]) - 1.0);
+        if (radii) radii[k] = 1.0 + jitter * (2.0 * u01(r2[0], r2[1]) - 1.0);
+    }
+    return BHX_OK;
+}
+
+// The bodies of that ensemble, system after system: system k is
+// bhx_ic_plummer(N_k, seed + 1 + k, M_k, a_k) at offset sum_{j<k} N_j.
+// Systems are generated on up to `threads` host threads (0: all cores).
+extern "C" int bhx_ic_ensemble(size_t nsys, uint64_t seed, const uint64_t* sizes, const double* masses,
+                               const double* radii, int threads, double* pos, double* vel, double* mass) {
+    if (!sizes || !masses || !radii || !pos || !vel || !mass) return BHX_E_NULL;
+    std::vector<size_t> off(nsys + 1, 0);
+    for (size_t k = 0; k < nsys; ++k) off[k + 1] = off[k] + (size_t)sizes[k];
+    unsigned nt = threads > 0 ? (unsigned)threads : std::thread::hardware_concurrency();
+    if (nt == 0) nt = 1;
+    if (nt > nsys) nt = (unsigned)(nsys ? nsys : 1);
+    std::atomic<size_t> next{0};
+    std::atomic<int> status{BHX_OK};
+    auto work = [&]() {
+        for (size_t k; (k = next.fetch_add(1)) < nsys;) {
+            const int rc = bhx_ic_plummer((size_t)sizes[k], seed + 1 + k, masses[k], radii[k], pos + 3 * off[k],
+                                          vel + 3 * off[k], mass + off[k]);
+            if (rc != BHX_OK) status = rc;
+        }
+    };
+    std::vector<std::thread> pool;
+    for (unsigned t = 1; t < nt; ++t) pool.emplace_back(work);
+    work();
+    for (auto& t : pool) t.join();
+    return status.load();
 }

@@ -170,11 +170,15 @@ struct bhx_ctx {
         int bslot = -1;
     } graphs[2];
     bool graph_valid = false;
-    // device snapshot of the state a graph run starts from (restored on node overflow)
-    double* snap = nullptr;
-    size_t snap_len = 0;
-    double* snap_ptr[26] = {};
-    int snap_bslot = 0;
+    // device snapshots of the body state: [0] where a graph run starts
+    // (restored on node overflow), [1] where the autotuner starts
+    struct Snapshot {
+        double* buf = nullptr;
+        size_t len = 0;
+        double* ptr[26] = {};
+        int bslot = 0;
+        bool bounds_valid = false;
+    } snaps[2];
     // segmented sort scratch (several systems only)
     uint64_t *skey = nullptr, *skey_alt = nullptr;
     uint32_t *sidx = nullptr, *sidx_alt = nullptr;

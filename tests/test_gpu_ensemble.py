@@ -154,3 +154,33 @@ def test_ensemble_argument_errors(bhx):
         e.set_bodies(pos, vel, mass)        # resets to one system
         assert e.nsys == 1
         e.bounds()
+
+
+def test_autotune_leaves_state_and_sets_capacity(bhx):
+    parts, pos, vel, mass, off = _systems(bhx, [4000, 2500, 300], seed0=120)
+    cfg = CONFIGS["intended_b32_fp32_group"]
+    with bhx.Engine(pos.shape[0], **cfg) as e:
+        e.set_bodies(pos, vel, mass)
+        e.set_systems(off)
+        e.step(1)
+        before = e.get_bodies()
+        chosen, table = e.autotune(caps=(8, 32), steps=2, max_err=1e-2, nsample=512)
+        assert chosen in (8, 32) and len(table) == 2
+        for cap, ms, err in table:
+            assert ms > 0 and 0 < err < 1e-2
+        after = e.get_bodies()
+        assert all(np.array_equal(x, y) for x, y in zip(before, after)), "state restored"
+        # nothing is allowed: the configured capacity stays
+        c2, _ = e.autotune(caps=(8, 32), steps=1, max_err=0.0, nsample=256)
+        assert c2 == chosen
+        e.step(2)
+        P = e.get_bodies()
+    with bhx.Engine(pos.shape[0], **dict(cfg, leaf_capacity=chosen)) as r:
+        r.set_bodies(pos, vel, mass)
+        r.set_systems(off)
+        r.set_config(leaf_capacity=32)
+        r.step(1)
+        r.set_config(leaf_capacity=chosen)
+        r.step(2)
+        R = r.get_bodies()
+    assert all(np.array_equal(x, y) for x, y in zip(P, R))
