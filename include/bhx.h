@@ -228,6 +228,36 @@ typedef struct bhx_tune_entry {
 int bhx_autotune(bhx_ctx *ctx, const int32_t *caps, size_t ncaps, int steps, double max_err, size_t nsample,
                  uint64_t seed, bhx_tune_entry *out, int32_t *chosen);
 
+/* ---- differentiation (no reference counterpart) -------------------------- */
+
+/* Runs nsteps frames from the resident state (as bhx_step_graph) and records
+ * them for bhx_adjoint: a device checkpoint of the state every
+ * checkpoint_every frames.  init_acc != 0 first sets the accelerations to the
+ * walk of the starting positions (a_0 = A(x_0)); otherwise the resident
+ * accelerations are the frames' a_0.  Needs group_walk = 1, the intended
+ * payload, quadrupole = 0 and permute_acc = 1. */
+int bhx_record(bhx_ctx *ctx, int nsteps, int checkpoint_every, int init_acc);
+
+/* State after the recorded frames, in the body order at the start of the
+ * recording (any pointer may be NULL). */
+int bhx_recorded_bodies(bhx_ctx *ctx, double *pos, double *vel, double *mass, double *acc);
+
+/* Reverse pass of the recorded frames: from dL/dx_T, dL/dv_T, dL/da_T (each
+ * 3 per body in the starting order, NULL = 0) to dL/dx_0, dL/dv_0, dL/da_0
+ * (3 per body) and dL/dm (1 per body), all in the starting order; NULL
+ * outputs are skipped.  Every walk is differentiated with the tree, the
+ * interaction lists and the body orders of its recorded frame (frozen
+ * decisions); the frames of each checkpoint segment are re-run, bitwise as
+ * recorded.  dL/da_0 is 0 when bhx_record computed a_0 (its adjoint flows
+ * into dL/dx_0 and dL/dm).  The bodies are left at the end of the recording.
+ * Entry sums use FP64 atomics, so results may differ in the last bits
+ * between runs. */
+int bhx_adjoint(bhx_ctx *ctx, const double *gx, const double *gv, const double *ga,
+                double *gx0, double *gv0, double *gm, double *ga0);
+
+/* Wall time of the last bhx_record and bhx_adjoint calls. */
+int bhx_grad_timing(bhx_ctx *ctx, double *record_ms, double *adjoint_ms);
+
 /* ---- diagnostics (no reference counterpart) ------------------------------ */
 
 /* FP64 direct sum over the target's own system for each target (array

@@ -21,6 +21,8 @@
  *   orc_walk              theta = 0 reduces to orc_direct; two-body closed form
  *   orc_direct            two-body closed form; Newton's third law (sum m a = 0)
  *   orc_kick_drift/kick   closed form on constant acceleration
+ *   orc_traj_direct_adjoint  central finite differences of orc_traj_direct (no tree,
+ *                         no decisions: differentiable everywhere); two-body closed form
  */
 #define _POSIX_C_SOURCE 199309L
 #include "bhx_oracle.h"
@@ -770,4 +772,108 @@ void orc_frame(double *pos, double *vel, double *mass, double *acc, uint64_t *ke
     s.t_bounds = t1 - t0; s.t_keys = t2 - t1; s.t_sort = t3 - t2; s.t_kick_drift = t4 - t3;
     s.t_build = t5 - t4; s.t_walk = t6 - t5; s.t_kick = t7 - t6;
     if (st) *st = s;
+}
+
+/* ------------------------------------------------------------------------- */
+/* trajectory gradient (no reference counterpart: the differentiable form)   */
+/* ------------------------------------------------------------------------- */
+
+/* Softened direct-sum accelerations a_i = sum_{j != i} m_j d (|d|^2+eps^2)^-3/2,
+ * d = x_j - x_i, in plain FP64 (the order of orc_direct). */
+static void orc_acc_all(const double *pos, const double *mass, size_t n, double eps, double *acc)
+{
+    orc_direct(pos, mass, n, NULL, n, eps, acc, NULL);
+}
+
+/* Vector-Jacobian product of orc_acc_all: for weights w (3 per body),
+ * gx += (da/dx)^T w and gm += (da/dm)^T w, pair by pair:
+ *   J = m_j (r^-3 I - 3 r^-5 d d^T),  gx_i -= J w_i,  gx_j += J w_i,
+ *   gm_j += r^-3 (d . w_i). */
+static void orc_acc_vjp(const double *pos, const double *mass, size_t n, double eps, const double *w,
+                        double *gx, double *gm)
+{
+    for (size_t i = 0; i < n; ++i) {
+        const double *wi = w + 3 * i;
+        for (size_t j = 0; j < n; ++j) {
+            if (j == i) continue;
+            double d[3] = {pos[3 * j] - pos[3 * i], pos[3 * j + 1] - pos[3 * i + 1], pos[3 * j + 2] - pos[3 * i + 2]};
+            double r2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2] + eps * eps;
+            double inv = 1.0 / sqrt(r2);
+            double inv3 = inv * inv * inv;
+            double inv5 = inv3 / r2;
+            double dw = d[0] * wi[0] + d[1] * wi[1] + d[2] * wi[2];
+            for (int a = 0; a < 3; ++a) {
+                double jw = mass[j] * (inv3 * wi[a] - 3.0 * inv5 * dw * d[a]);
+                gx[3 * i + a] -= jw;
+                gx[3 * j + a] += jw;
+            }
+            gm[j] += inv3 * dw;
+        }
+    }
+}
+
+/* Leapfrog (kick-drift-kick) with direct-sum forces for nsteps frames:
+ *   h = v + a dt/2; x += h dt; a = A(x); v = h + a dt/2.
+ * a_0 is A(x_0) when init_acc, else the given acc (NULL = 0).  pos, vel, acc
+ * are updated in place (acc must hold 3n doubles). */
+void orc_traj_direct(double *pos, double *vel, const double *mass, double *acc, size_t n, double eps,
+                     double dt, int nsteps, int init_acc)
+{
+    if (init_acc) orc_acc_all(pos, mass, n, eps, acc);
+    for (int f = 0; f < nsteps; ++f) {
+        for (size_t k = 0; k < 3 * n; ++k) {
+            vel[k] = vel[k] + acc[k] * (0.5 * dt);
+            pos[k] = pos[k] + vel[k] * dt;
+        }
+        orc_acc_all(pos, mass, n, eps, acc);
+        for (size_t k = 0; k < 3 * n; ++k) vel[k] = vel[k] + acc[k] * (0.5 * dt);
+    }
+}
+
+/* Reverse pass of orc_traj_direct from (pos, vel, mass, acc) at frame 0:
+ * (gx, gv, ga) = dL/d(x_T, v_T, a_T) in, (gx0, gv0, ga0, gm) out.  States are
+ * recomputed forward and kept for every frame (small n only). */
+void orc_traj_direct_adjoint(const double *pos0, const double *vel0, const double *mass, const double *acc0,
+                             size_t n, double eps, double dt, int nsteps, int init_acc,
+                             const double *gx, const double *gv, const double *ga,
+                             double *gx0, double *gv0, double *ga0, double *gm)
+{
+    const size_t m3 = 3 * n;
+    double *xs = malloc(sizeof(double) * m3 * (size_t)(nsteps + 1));
+    double *v = malloc(sizeof(double) * m3), *a = malloc(sizeof(double) * m3);
+    double *X = malloc(sizeof(double) * m3), *V = malloc(sizeof(double) * m3), *A = malloc(sizeof(double) * m3);
+    double *w = malloc(sizeof(double) * m3);
+    memcpy(xs, pos0, sizeof(double) * m3);
+    memcpy(v, vel0, sizeof(double) * m3);
+    if (acc0) memcpy(a, acc0, sizeof(double) * m3); else memset(a, 0, sizeof(double) * m3);
+    if (init_acc) orc_acc_all(xs, mass, n, eps, a);
+    for (int f = 0; f < nsteps; ++f) {
+        double *x = xs + m3 * (size_t)(f + 1);
+        memcpy(x, xs + m3 * (size_t)f, sizeof(double) * m3);
+        orc_traj_direct(x, v, mass, a, n, eps, dt, 1, 0);
+    }
+    for (size_t k = 0; k < m3; ++k) {
+        X[k] = gx ? gx[k] : 0.0;
+        V[k] = gv ? gv[k] : 0.0;
+        A[k] = ga ? ga[k] : 0.0;
+    }
+    memset(gm, 0, sizeof(double) * n);
+    for (int f = nsteps - 1; f >= 0; --f) {
+        const double *y = xs + m3 * (size_t)(f + 1);
+        for (size_t k = 0; k < m3; ++k) w[k] = A[k] + V[k] * (0.5 * dt);
+        orc_acc_vjp(y, mass, n, eps, w, X, gm);          /* X += (dA/dy)^T w */
+        for (size_t k = 0; k < m3; ++k) {
+            double h = V[k] + X[k] * dt;
+            V[k] = h;
+            A[k] = h * (0.5 * dt);
+        }
+    }
+    if (init_acc) {
+        orc_acc_vjp(xs, mass, n, eps, A, X, gm);
+        memset(A, 0, sizeof(double) * m3);
+    }
+    memcpy(gx0, X, sizeof(double) * m3);
+    memcpy(gv0, V, sizeof(double) * m3);
+    if (ga0) memcpy(ga0, A, sizeof(double) * m3);
+    free(xs); free(v); free(a); free(X); free(V); free(A); free(w);
 }

@@ -86,6 +86,14 @@ void orc_frame(double *pos, double *vel, double *mass, double *acc, uint64_t *ke
                size_t n, double dt, double theta, double eps, int leaf_cap, int mode,
                int use_quad, int permute_acc, orc_frame_stats *st);
 
+/* ---- trajectory gradient with direct-sum forces (differentiable form) ---- */
+void orc_traj_direct(double *pos, double *vel, const double *mass, double *acc, size_t n, double eps,
+                     double dt, int nsteps, int init_acc);
+void orc_traj_direct_adjoint(const double *pos0, const double *vel0, const double *mass, const double *acc0,
+                             size_t n, double eps, double dt, int nsteps, int init_acc,
+                             const double *gx, const double *gv, const double *ga,
+                             double *gx0, double *gv0, double *ga0, double *gm);
+
 #ifdef __cplusplus
 }
 #endif

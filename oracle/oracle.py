@@ -72,6 +72,8 @@ def lib():
         L.orc_walk_subset.argtypes = [P, P, P, P, S, D, D, I, P, P]
         L.orc_direct.argtypes = [P, P, S, P, S, D, P, P]
         L.orc_frame.argtypes = [P, P, P, P, P, S, D, D, D, I, I, I, I, ctypes.POINTER(FrameStats)]
+        L.orc_traj_direct.argtypes = [P, P, P, P, S, D, D, I, I]
+        L.orc_traj_direct_adjoint.argtypes = [P, P, P, P, S, D, D, I, I, P, P, P, P, P, P, P]
         _lib = L
     return _lib
 
@@ -231,3 +233,28 @@ def frame(pos, vel, mass, acc, dt, theta, eps, leaf_cap=1, mode=AS_RUN, quad=Fal
     lib().orc_frame(_p(pos), _p(vel), _p(mass), _p(acc), _p(keys), pos.shape[0], dt, theta, eps,
                     leaf_cap, mode, int(quad), int(permute_acc), ctypes.byref(st))
     return pos, vel, mass, acc, keys, st
+
+
+def traj_direct(pos, vel, mass, eps, dt, nsteps, init_acc=True, acc=None):
+    """Leapfrog with FP64 direct-sum forces; returns (pos, vel, acc) after nsteps frames."""
+    pos = np.array(pos, np.float64, copy=True)
+    vel = np.array(vel, np.float64, copy=True)
+    mass = _c(mass, np.float64)
+    acc = np.zeros_like(pos) if acc is None else np.array(acc, np.float64, copy=True)
+    lib().orc_traj_direct(_p(pos), _p(vel), _p(mass), _p(acc), pos.shape[0], eps, dt, nsteps, int(init_acc))
+    return pos, vel, acc
+
+
+def traj_direct_adjoint(pos, vel, mass, eps, dt, nsteps, gx=None, gv=None, ga=None, init_acc=True, acc=None):
+    """dL/d(x0, v0, a0, m) of traj_direct for dL/d(x_T, v_T, a_T) = (gx, gv, ga)."""
+    pos = _c(pos, np.float64)
+    vel = _c(vel, np.float64)
+    mass = _c(mass, np.float64)
+    n = pos.shape[0]
+    a0 = None if acc is None else _c(acc, np.float64)
+    g = [None if v is None else _c(v, np.float64) for v in (gx, gv, ga)]
+    gx0, gv0, ga0, gm = np.zeros((n, 3)), np.zeros((n, 3)), np.zeros((n, 3)), np.zeros(n)
+    lib().orc_traj_direct_adjoint(_p(pos), _p(vel), _p(mass), None if a0 is None else _p(a0), n, eps, dt, nsteps,
+                                  int(init_acc), *[None if v is None else _p(v) for v in g],
+                                  _p(gx0), _p(gv0), _p(ga0), _p(gm))
+    return gx0, gv0, ga0, gm

@@ -75,7 +75,9 @@ def lib():
             "bhx_ic_ensemble": [S, ctypes.c_uint64, P, P, P, I, P, P, P],
             "bhx_set_systems": [P, P, S], "bhx_num_systems": [P, P], "bhx_system_bounds": [P, P, P],
             "bhx_build_systems": [P, P, P, P], "bhx_step_graph": [P, I],
-            "bhx_autotune": [P, P, S, I, D, S, ctypes.c_uint64, P, P], "bhx_system_roots": [P, P], "bhx_lookup_in": [P, S, P, S, P],
+            "bhx_autotune": [P, P, S, I, D, S, ctypes.c_uint64, P, P],
+            "bhx_record": [P, I, I, I], "bhx_recorded_bodies": [P, P, P, P, P],
+            "bhx_adjoint": [P, P, P, P, P, P, P, P], "bhx_grad_timing": [P, P, P], "bhx_system_roots": [P, P], "bhx_lookup_in": [P, S, P, S, P],
         }
         for name, args in sigs.items():
             f = getattr(L, name)
@@ -330,6 +332,31 @@ class Engine:
                                      ctypes.byref(ch)), "bhx_autotune")
         self.cfg.leaf_capacity = ch.value
         return ch.value, [(int(r["leaf_capacity"]), float(r["ms_per_step"]), float(r["median_err"])) for r in out]
+
+    # --- differentiation
+    def record(self, nsteps, checkpoint_every=100, init_acc=True):
+        """Run and record nsteps frames for adjoint(); see bhx_record."""
+        self._chk(lib().bhx_record(self.h, nsteps, checkpoint_every, int(init_acc)), "bhx_record")
+
+    def recorded_bodies(self):
+        """(pos, vel, mass, acc) after the recorded frames, in the order the recording started with."""
+        n = self.n
+        pos, vel, acc, mass = np.empty((n, 3)), np.empty((n, 3)), np.empty((n, 3)), np.empty(n)
+        self._chk(lib().bhx_recorded_bodies(self.h, _p(pos), _p(vel), _p(mass), _p(acc)), "bhx_recorded_bodies")
+        return pos, vel, mass, acc
+
+    def adjoint(self, gx=None, gv=None, ga=None):
+        """dL/d(x0, v0, a0, m) from dL/d(x_T, v_T, a_T), all in the starting body order."""
+        n = self.n
+        g = [None if v is None else _f64(v, (n, 3)) for v in (gx, gv, ga)]
+        gx0, gv0, ga0, gm = np.empty((n, 3)), np.empty((n, 3)), np.empty((n, 3)), np.empty(n)
+        self._chk(lib().bhx_adjoint(self.h, *[_p(v) for v in g], _p(gx0), _p(gv0), _p(gm), _p(ga0)), "bhx_adjoint")
+        return gx0, gv0, ga0, gm
+
+    def grad_timing(self):
+        r, a = ctypes.c_double(), ctypes.c_double()
+        self._chk(lib().bhx_grad_timing(self.h, ctypes.byref(r), ctypes.byref(a)), "bhx_grad_timing")
+        return r.value, a.value
 
     def sync(self):
         self._chk(lib().bhx_sync(self.h), "bhx_sync")

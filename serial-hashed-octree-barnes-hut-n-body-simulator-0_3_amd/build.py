@@ -26,6 +26,7 @@ UNITS = {
     "bhx_tree.hip": [],
     "bhx_walk.hip": [],
     "bhx_api.hip": [],
+    "bhx_grad.hip": [],
 }
 HOST_UNITS = {"bhx_ic.cpp": ["-O2", "-fPIC", "-std=c++17", "-ffp-contract=off", "-pthread"]}
 

@@ -55,29 +55,9 @@ __global__ void k_copy_u64(const uint64_t* __restrict__ a, size_t n, uint64_t* _
     for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) b[i] = a[i];
 }
 
-int fail(bhx_ctx* c, int code, const char* what, hipError_t e = hipSuccess) {
-    if (c) {
-        c->err = what;
-        if (e != hipSuccess) {
-            c->err += ": ";
-            c->err += hipGetErrorString(e);
-        }
-    }
-    return code;
-}
+}  // namespace
 
-#define HIPCHK(expr)                                                 \
-    do {                                                             \
-        hipError_t e_ = (expr);                                      \
-        if (e_ != hipSuccess) return fail(c, BHX_E_HIP, #expr, e_); \
-    } while (0)
-
-template <typename T>
-hipError_t dalloc(T** p, size_t count) {
-    *p = nullptr;
-    if (count == 0) count = 1;
-    return hipMalloc((void**)p, count * sizeof(T));
-}
+namespace bhx {
 
 void drop_graphs(bhx_ctx* c) {
     for (auto& g : c->graphs) {
@@ -126,6 +106,7 @@ hipError_t alloc_tree(bhx_ctx* c, size_t cap) {
 }
 
 void free_all(bhx_ctx* c) {
+    free_grad(c);
     drop_graphs(c);
     free_tree(c);
     for (auto& sn : c->snaps)
@@ -401,6 +382,19 @@ int restore_snapshot(bhx_ctx* c, const bhx_ctx::Snapshot& sn, hipStream_t s) {
     return BHX_OK;
 }
 
+// One frame from its captured graph; the host mirrors the buffer exchange
+// and bounds-slot flip the frame performs on the device.
+int graph_frame(bhx_ctx* c, hipStream_t s) {
+    hipGraphExec_t g = nullptr;
+    const int rc = step_graph_for(c, &g);
+    if (rc != BHX_OK) return rc;
+    HIPCHK(hipGraphLaunch(g, s));
+    swap_body_buffers(c);
+    c->tree_slot = c->bslot;
+    c->bslot ^= 1;
+    return BHX_OK;
+}
+
 // Graph of the frame that starts from the current parity (capturing both
 // parities when none matches).
 int step_graph_for(bhx_ctx* c, hipGraphExec_t* out) {
@@ -423,7 +417,7 @@ int step_graph_for(bhx_ctx* c, hipGraphExec_t* out) {
     return BHX_OK;
 }
 
-}  // namespace
+}  // namespace bhx
 
 extern "C" {
 
@@ -1101,13 +1095,8 @@ int bhx_step_graph(bhx_ctx* c, int nsteps) {
         HIPCHK(hipGetLastError());
         c->tree_valid = c->tree_built = false;
         for (int k = 0; k < nsteps; ++k) {
-            hipGraphExec_t g = nullptr;
-            rc = step_graph_for(c, &g);
+            rc = graph_frame(c, s);
             if (rc != BHX_OK) return rc;
-            HIPCHK(hipGraphLaunch(g, s));
-            swap_body_buffers(c);      // the frame's permute/kick/drift exchanged them
-            c->tree_slot = c->bslot;
-            c->bslot ^= 1;
         }
         rc = read_counters(c, s);
         if (rc != BHX_OK) return rc;

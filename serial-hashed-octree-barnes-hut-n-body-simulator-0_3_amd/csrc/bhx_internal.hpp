@@ -170,6 +170,7 @@ struct bhx_ctx {
         int bslot = -1;
     } graphs[2];
     bool graph_valid = false;
+    struct GradRecord* grad = nullptr;         // recorded trajectory for bhx_adjoint
     // device snapshots of the body state: [0] where a graph run starts
     // (restored on node overflow), [1] where the autotuner starts
     struct Snapshot {
@@ -188,6 +189,40 @@ struct bhx_ctx {
 };
 
 namespace bhx {
+
+// Records `what` (and the HIP error) as the context's last error; returns code.
+inline int fail(bhx_ctx* c, int code, const char* what, hipError_t e = hipSuccess) {
+    if (c) {
+        c->err = what;
+        if (e != hipSuccess) {
+            c->err += ": ";
+            c->err += hipGetErrorString(e);
+        }
+    }
+    return code;
+}
+
+#define HIPCHK(expr)                                                 \
+    do {                                                             \
+        hipError_t e_ = (expr);                                      \
+        if (e_ != hipSuccess) return fail(c, BHX_E_HIP, #expr, e_); \
+    } while (0)
+
+template <typename T>
+hipError_t dalloc(T** p, size_t count) {
+    *p = nullptr;
+    if (count == 0) count = 1;
+    return hipMalloc((void**)p, count * sizeof(T));
+}
+
+// --- context services (bhx_api.hip) ------------------------------------------
+void drop_graphs(bhx_ctx* c);
+hipError_t alloc_tree(bhx_ctx* c, size_t cap);
+int build_on_device(bhx_ctx* c, hipStream_t s);       // tree of the resident bodies, bounds slot c->bslot
+int read_counters(bhx_ctx* c, hipStream_t s);         // waits; reports sort / hash failures
+int step_graph_for(bhx_ctx* c, hipGraphExec_t* out);  // one-frame graph for the current buffer parity
+int graph_frame(bhx_ctx* c, hipStream_t s);           // enqueue one frame from its graph
+void free_grad(bhx_ctx* c);                           // recorded trajectory (bhx_grad.hip)
 
 // --- launchers implemented in the .hip files (all enqueue on `s`) ----------
 // bounds4 holds 4 doubles (centre, size) per system
@@ -223,6 +258,24 @@ hipError_t launch_walk_to(bhx_ctx* c, hipStream_t s, bool group, int precision, 
                           uint32_t* counts);
 hipError_t launch_direct(bhx_ctx* c, const int64_t* targets, size_t nt, double eps, double* acc,
                          double* pot, hipStream_t s);
+
+// The arrays a walk reads: the live tree or a stored copy of one.
+struct WalkTree {
+    const WalkNode* walk;
+    const WalkNode32* walk32;
+    const float4* tb4;
+    const uint32_t* tleaf;
+    const double *tx, *ty, *tz, *tm;
+    const uint32_t* tperm;
+    const uint32_t* sys_root;
+    const double* b4;
+};
+// Adjoint of the group walk on tree t with frozen lists: per body weights w
+// (array order) -> dL/dx of the own position (gself, 3 per body, array
+// order) and per-entry (dL/dX, dL/dM) accumulated into gnode (4 per node)
+// and gbody (4 per body, tree order), which must start at zero.
+hipError_t launch_walk_adjoint(bhx_ctx* c, const WalkTree& t, int precision, const double* wx, const double* wy,
+                               const double* wz, double* gself, double* gnode, double* gbody, hipStream_t s);
 hipError_t launch_diag(bhx_ctx* c, double* out7, hipStream_t s);
 
 hipError_t scan_exclusive_u32(const uint32_t* in, uint32_t* out, size_t n, uint32_t* tmp,

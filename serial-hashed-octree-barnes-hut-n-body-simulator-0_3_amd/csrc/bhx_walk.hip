@@ -67,6 +67,13 @@ struct WalkArgs {
     float theta2f, eps2f;
     double *ax, *ay, *az;
     uint32_t* counts;
+    // adjoint walk: weights w = dL/da per body (array order) in, dL/dx of each
+    // body's own position out (array order, 3 per body), and per list entry
+    // (dL/dX, dL/dM) summed over the targets: 4 per node, 4 per body (tree order)
+    const double *wx, *wy, *wz;
+    double* gself;
+    double* gnode;
+    double* gbody;
 };
 
 // One term of ComputeHOTForceInteractionList (HashedOctree.hpp:411-454) in
@@ -399,12 +406,73 @@ __device__ __forceinline__ void eval_bodies(const T (*bd)[4], int nb, T xr, T yr
     }
 }
 
+// ---- adjoint of the list evaluation (frozen lists).  For an entry of mass
+// M at X and a target at x with weight w = dL/da, d = X - x, r^2 = |d|^2 +
+// eps^2, the term M d r^-3 has Jacobian M (r^-3 I - 3 r^-5 d d^T) in d, so
+//   dL/dx (target)  -= M (r^-3 w - 3 r^-5 (d.w) d)
+//   dL/dX (entry)   += M (r^-3 w - 3 r^-5 (d.w) d)
+//   dL/dM (entry)   += r^-3 (d.w)
+// A zero separation (the body itself) has no term.  The target side runs
+// lane = target over the entries, the entry side lane = entry over the 64
+// targets of the group, so neither needs a cross-lane reduction; entry sums
+// go to memory with FP64 atomics.
+template <typename T>
+__device__ __forceinline__ void adj_self(const T (*ent)[4], int ne, T xr, T yr, T zr, T wx, T wy, T wz, T eps2,
+                                         T& gx, T& gy, T& gz) {
+    for (int k = 0; k < ne; ++k) {
+        const T dx = ent[k][0] - xr, dy = ent[k][1] - yr, dz = ent[k][2] - zr;
+        const T d2 = dist2(dx, dy, dz);
+        T inv = rsqrt_t<T>(d2 + eps2);
+        inv = d2 > (T)0 ? inv : (T)0;
+        const T inv2 = inv * inv;
+        const T inv3 = inv * inv2;
+        const T dw = dx * wx + dy * wy + dz * wz;
+        const T c3 = ent[k][3] * inv3;
+        const T c5 = (T)3 * c3 * inv2 * dw;
+        gx -= c3 * wx - c5 * dx;
+        gy -= c3 * wy - c5 * dy;
+        gz -= c3 * wz - c5 * dz;
+    }
+}
+
+template <typename T>
+__device__ __forceinline__ void adj_scatter(const T (*ent)[4], const uint32_t* ids, int ne, const T (*tg)[6], T eps2,
+                                            uint32_t lane, double* __restrict__ g4) {
+    for (int e = (int)lane; e < ne; e += 64) {
+        const T X = ent[e][0], Y = ent[e][1], Z = ent[e][2];
+        T gx = 0, gy = 0, gz = 0, h = 0;
+        for (int t = 0; t < 64; ++t) {
+            const T dx = X - tg[t][0], dy = Y - tg[t][1], dz = Z - tg[t][2];
+            const T wx = tg[t][3], wy = tg[t][4], wz = tg[t][5];
+            const T d2 = dist2(dx, dy, dz);
+            T inv = rsqrt_t<T>(d2 + eps2);
+            inv = d2 > (T)0 ? inv : (T)0;
+            const T inv2 = inv * inv;
+            const T inv3 = inv * inv2;
+            const T dw = dx * wx + dy * wy + dz * wz;
+            const T c5 = (T)3 * inv3 * inv2 * dw;
+            gx += inv3 * wx - c5 * dx;
+            gy += inv3 * wy - c5 * dy;
+            gz += inv3 * wz - c5 * dz;
+            h += inv3 * dw;
+        }
+        const T M = ent[e][3];
+        double* g = g4 + 4 * (size_t)ids[e];
+        unsafeAtomicAdd(g, (double)(M * gx));
+        unsafeAtomicAdd(g + 1, (double)(M * gy));
+        unsafeAtomicAdd(g + 2, (double)(M * gz));
+        unsafeAtomicAdd(g + 3, (double)h);
+    }
+}
+
 // Bodies of the queued leaf ranges into LDS with every load in flight at
 // once: lane l fills slots l and l+64, finding each slot's range by bisection
-// over the ranges' first slots.
+// over the ranges' first slots.  `bid` (adjoint walk) receives each slot's
+// tree-order body index.
 template <typename T>
 __device__ __forceinline__ void gather_bodies(T (*bd)[4], const uint32_t* rstart, const uint32_t* roff, int nr, int nq,
-                                              uint32_t lane, const WalkArgs& a, double ox, double oy, double oz) {
+                                              uint32_t lane, const WalkArgs& a, double ox, double oy, double oz,
+                                              uint32_t* bid = nullptr) {
 #pragma unroll
     for (int h = 0; h < kBodyCap / 64; ++h) {
         const int e = (int)lane + 64 * h;
@@ -420,16 +488,20 @@ __device__ __forceinline__ void gather_bodies(T (*bd)[4], const uint32_t* rstart
             bd[e][1] = (T)(by - oy);
             bd[e][2] = (T)(bz - oz);
             bd[e][3] = (T)bm;
+            if (bid) bid[e] = j;
         }
     }
 }
 
-template <typename T, bool kQuad, bool kGuard>
+template <typename T, bool kQuad, bool kGuard, bool kAdj = false>
 __global__ __launch_bounds__(64) void k_walk_group(WalkArgs a) {
+    static_assert(!(kAdj && kQuad), "the adjoint walk is monopole only");
     __shared__ T s_nd[kNodeCap][4];
     __shared__ T s_q[kQuad ? kNodeCap : 1][6];
     __shared__ T s_bd[kBodyCap][4];
     __shared__ uint32_t s_rstart[kRangeCap], s_roff[kRangeCap];
+    __shared__ uint32_t s_nid[kAdj ? kNodeCap : 1], s_bid[kAdj ? kBodyCap : 1];
+    __shared__ T s_tg[kAdj ? 64 : 1][6];
 
     if (blockIdx.x >= a.groups || *a.abort_flag) return;
     const uint4 gr = a.gtab[blockIdx.x];
@@ -449,6 +521,17 @@ __global__ __launch_bounds__(64) void k_walk_group(WalkArgs a) {
     const double th2d = a.theta2;
     const float th2f = a.theta2f;
     T ax = 0, ay = 0, az = 0;
+    T wxr = 0, wyr = 0, wzr = 0;
+    if constexpr (kAdj) {
+        if (valid) {
+            const uint32_t o = a.tperm[i];
+            wxr = (T)a.wx[o];
+            wyr = (T)a.wy[o];
+            wzr = (T)a.wz[o];
+        }
+        s_tg[lane][0] = xr; s_tg[lane][1] = yr; s_tg[lane][2] = zr;
+        s_tg[lane][3] = wxr; s_tg[lane][4] = wyr; s_tg[lane][5] = wzr;
+    }
     uint32_t cnt = 0;
     int nn = 0, nr = 0, nq = 0;            // queued cells, leaf ranges, bodies in those ranges
     const WalkNode root = load_node(raw, rid);
@@ -470,7 +553,12 @@ __global__ __launch_bounds__(64) void k_walk_group(WalkArgs a) {
         if (!opened) {
             if (nn == kNodeCap) {
                 __syncthreads();
-                eval_nodes<T, kQuad, kGuard>(s_nd, s_q, nn, xr, yr, zr, eps2, ax, ay, az);
+                if constexpr (kAdj) {
+                    adj_self<T>(s_nd, nn, xr, yr, zr, wxr, wyr, wzr, eps2, ax, ay, az);
+                    adj_scatter<T>(s_nd, s_nid, nn, s_tg, eps2, lane, a.gnode);
+                } else {
+                    eval_nodes<T, kQuad, kGuard>(s_nd, s_q, nn, xr, yr, zr, eps2, ax, ay, az);
+                }
                 cnt += (uint32_t)nn;
                 nn = 0;
                 __syncthreads();
@@ -480,6 +568,7 @@ __global__ __launch_bounds__(64) void k_walk_group(WalkArgs a) {
                 s_nd[nn][1] = cy;
                 s_nd[nn][2] = cz;
                 s_nd[nn][3] = sizeof(T) == 4 ? (T)nd.massf : (T)nd.mass;
+                if constexpr (kAdj) s_nid[nn] = p;
             }
             if (kQuad && lane < 6) {
                 const bool many = nd.count > 1;
@@ -493,9 +582,14 @@ __global__ __launch_bounds__(64) void k_walk_group(WalkArgs a) {
                 const uint32_t m = (nd.count - k) < 64u ? (nd.count - k) : 64u;
                 if (nq + (int)m > kBodyCap || nr == kRangeCap) {
                     __syncthreads();
-                    gather_bodies<T>(s_bd, s_rstart, s_roff, nr, nq, lane, a, ox, oy, oz);
+                    gather_bodies<T>(s_bd, s_rstart, s_roff, nr, nq, lane, a, ox, oy, oz, kAdj ? s_bid : nullptr);
                     __syncthreads();
-                    eval_bodies<T, kGuard>(s_bd, nq, xr, yr, zr, eps2, ax, ay, az);
+                    if constexpr (kAdj) {
+                        adj_self<T>(s_bd, nq, xr, yr, zr, wxr, wyr, wzr, eps2, ax, ay, az);
+                        adj_scatter<T>(s_bd, s_bid, nq, s_tg, eps2, lane, a.gbody);
+                    } else {
+                        eval_bodies<T, kGuard>(s_bd, nq, xr, yr, zr, eps2, ax, ay, az);
+                    }
                     cnt += (uint32_t)nq;
                     nr = 0;
                     nq = 0;
@@ -514,16 +608,32 @@ __global__ __launch_bounds__(64) void k_walk_group(WalkArgs a) {
         }
     }
     __syncthreads();
-    eval_nodes<T, kQuad, kGuard>(s_nd, s_q, nn, xr, yr, zr, eps2, ax, ay, az);
-    gather_bodies<T>(s_bd, s_rstart, s_roff, nr, nq, lane, a, ox, oy, oz);
+    if constexpr (kAdj) {
+        adj_self<T>(s_nd, nn, xr, yr, zr, wxr, wyr, wzr, eps2, ax, ay, az);
+        adj_scatter<T>(s_nd, s_nid, nn, s_tg, eps2, lane, a.gnode);
+    } else {
+        eval_nodes<T, kQuad, kGuard>(s_nd, s_q, nn, xr, yr, zr, eps2, ax, ay, az);
+    }
+    gather_bodies<T>(s_bd, s_rstart, s_roff, nr, nq, lane, a, ox, oy, oz, kAdj ? s_bid : nullptr);
     __syncthreads();
-    eval_bodies<T, kGuard>(s_bd, nq, xr, yr, zr, eps2, ax, ay, az);
+    if constexpr (kAdj) {
+        adj_self<T>(s_bd, nq, xr, yr, zr, wxr, wyr, wzr, eps2, ax, ay, az);
+        adj_scatter<T>(s_bd, s_bid, nq, s_tg, eps2, lane, a.gbody);
+    } else {
+        eval_bodies<T, kGuard>(s_bd, nq, xr, yr, zr, eps2, ax, ay, az);
+    }
     cnt += (uint32_t)(nn + nq);
     if (valid) {
         const uint32_t o = a.tperm[i];
-        a.ax[o] = (double)ax;
-        a.ay[o] = (double)ay;
-        a.az[o] = (double)az;
+        if constexpr (kAdj) {
+            a.gself[3 * (size_t)o] = (double)ax;
+            a.gself[3 * (size_t)o + 1] = (double)ay;
+            a.gself[3 * (size_t)o + 2] = (double)az;
+        } else {
+            a.ax[o] = (double)ax;
+            a.ay[o] = (double)ay;
+            a.az[o] = (double)az;
+        }
         if (a.counts) a.counts[o] = cnt;
     }
 }
@@ -575,13 +685,16 @@ __device__ __forceinline__ Node32 load_node32(ConstU64 raw, uint32_t p) {
     return nd;
 }
 
-template <bool kQuad, bool kGuard>
+template <bool kQuad, bool kGuard, bool kAdj = false>
 __global__ __launch_bounds__(64) void k_walk_group32(WalkArgs a) {
+    static_assert(!(kAdj && kQuad), "the adjoint walk is monopole only");
     __shared__ float s_nd[kNodeCap][4];
     __shared__ float s_q[kQuad ? kNodeCap : 1][6];
     __shared__ float s_bd[kBodyCap][4];
     __shared__ uint32_t s_rstart[kRangeCap], s_roff[kRangeCap];
     __shared__ float s_rdel[kRangeCap][3];    // leaf's first body minus the group origin
+    __shared__ uint32_t s_nid[kAdj ? kNodeCap : 1], s_bid[kAdj ? kBodyCap : 1];
+    __shared__ float s_tg[kAdj ? 64 : 1][6];
 
     const uint32_t grp = xcd_group(blockIdx.x, a.groups);
     if (grp >= a.groups || *a.abort_flag) return;
@@ -608,6 +721,17 @@ __global__ __launch_bounds__(64) void k_walk_group32(WalkArgs a) {
     const double s0 = a.b4[4 * (size_t)gr.z + 3];
     const float s2root = (float)(s0 * s0);
     float ax = 0.f, ay = 0.f, az = 0.f;
+    float wxr = 0.f, wyr = 0.f, wzr = 0.f;
+    if constexpr (kAdj) {
+        if (valid) {
+            const uint32_t o = a.tperm[i];
+            wxr = (float)a.wx[o];
+            wyr = (float)a.wy[o];
+            wzr = (float)a.wz[o];
+        }
+        s_tg[lane][0] = xr; s_tg[lane][1] = yr; s_tg[lane][2] = zr;
+        s_tg[lane][3] = wxr; s_tg[lane][4] = wyr; s_tg[lane][5] = wzr;
+    }
     uint32_t cnt = 0;
     int nn = 0, nr = 0, nq = 0;
     const Node32 root = load_node32(raw, rid);
@@ -629,7 +753,12 @@ __global__ __launch_bounds__(64) void k_walk_group32(WalkArgs a) {
         if (!opened) {
             if (nn == kNodeCap) {
                 __syncthreads();
-                eval_nodes<float, kQuad, kGuard>(s_nd, s_q, nn, xr, yr, zr, eps2, ax, ay, az);
+                if constexpr (kAdj) {
+                    adj_self<float>(s_nd, nn, xr, yr, zr, wxr, wyr, wzr, eps2, ax, ay, az);
+                    adj_scatter<float>(s_nd, s_nid, nn, s_tg, eps2, lane, a.gnode);
+                } else {
+                    eval_nodes<float, kQuad, kGuard>(s_nd, s_q, nn, xr, yr, zr, eps2, ax, ay, az);
+                }
                 cnt += (uint32_t)nn;
                 nn = 0;
                 __syncthreads();
@@ -639,6 +768,7 @@ __global__ __launch_bounds__(64) void k_walk_group32(WalkArgs a) {
                 s_nd[nn][1] = cy;
                 s_nd[nn][2] = cz;
                 s_nd[nn][3] = nd.mass;
+                if constexpr (kAdj) s_nid[nn] = p;
                 if (kQuad) {
                     s_q[nn][0] = nd.q0;
                     s_q[nn][1] = nd.q1;
@@ -666,15 +796,22 @@ __global__ __launch_bounds__(64) void k_walk_group32(WalkArgs a) {
                                 const int mid = (lo + hi + 1) >> 1;
                                 if ((int)s_roff[mid] <= e) lo = mid; else hi = mid - 1;
                             }
-                            const float4 b = a.tb4[s_rstart[lo] + (uint32_t)(e - (int)s_roff[lo])];
+                            const uint32_t j = s_rstart[lo] + (uint32_t)(e - (int)s_roff[lo]);
+                            const float4 b = a.tb4[j];
                             s_bd[e][0] = s_rdel[lo][0] + b.x;
                             s_bd[e][1] = s_rdel[lo][1] + b.y;
                             s_bd[e][2] = s_rdel[lo][2] + b.z;
                             s_bd[e][3] = b.w;
+                            if constexpr (kAdj) s_bid[e] = j;
                         }
                     }
                     __syncthreads();
-                    eval_bodies<float, kGuard>(s_bd, nq, xr, yr, zr, eps2, ax, ay, az);
+                    if constexpr (kAdj) {
+                        adj_self<float>(s_bd, nq, xr, yr, zr, wxr, wyr, wzr, eps2, ax, ay, az);
+                        adj_scatter<float>(s_bd, s_bid, nq, s_tg, eps2, lane, a.gbody);
+                    } else {
+                        eval_bodies<float, kGuard>(s_bd, nq, xr, yr, zr, eps2, ax, ay, az);
+                    }
                     cnt += (uint32_t)nq;
                     nr = 0;
                     nq = 0;
@@ -696,7 +833,12 @@ __global__ __launch_bounds__(64) void k_walk_group32(WalkArgs a) {
         }
     }
     __syncthreads();
-    eval_nodes<float, kQuad, kGuard>(s_nd, s_q, nn, xr, yr, zr, eps2, ax, ay, az);
+    if constexpr (kAdj) {
+        adj_self<float>(s_nd, nn, xr, yr, zr, wxr, wyr, wzr, eps2, ax, ay, az);
+        adj_scatter<float>(s_nd, s_nid, nn, s_tg, eps2, lane, a.gnode);
+    } else {
+        eval_nodes<float, kQuad, kGuard>(s_nd, s_q, nn, xr, yr, zr, eps2, ax, ay, az);
+    }
 #pragma unroll
     for (int h = 0; h < kBodyCap / 64; ++h) {
         const int e = (int)lane + 64 * h;
@@ -706,21 +848,34 @@ __global__ __launch_bounds__(64) void k_walk_group32(WalkArgs a) {
                 const int mid = (lo + hi + 1) >> 1;
                 if ((int)s_roff[mid] <= e) lo = mid; else hi = mid - 1;
             }
-            const float4 b = a.tb4[s_rstart[lo] + (uint32_t)(e - (int)s_roff[lo])];
+            const uint32_t j = s_rstart[lo] + (uint32_t)(e - (int)s_roff[lo]);
+            const float4 b = a.tb4[j];
             s_bd[e][0] = s_rdel[lo][0] + b.x;
             s_bd[e][1] = s_rdel[lo][1] + b.y;
             s_bd[e][2] = s_rdel[lo][2] + b.z;
             s_bd[e][3] = b.w;
+            if constexpr (kAdj) s_bid[e] = j;
         }
     }
     __syncthreads();
-    eval_bodies<float, kGuard>(s_bd, nq, xr, yr, zr, eps2, ax, ay, az);
+    if constexpr (kAdj) {
+        adj_self<float>(s_bd, nq, xr, yr, zr, wxr, wyr, wzr, eps2, ax, ay, az);
+        adj_scatter<float>(s_bd, s_bid, nq, s_tg, eps2, lane, a.gbody);
+    } else {
+        eval_bodies<float, kGuard>(s_bd, nq, xr, yr, zr, eps2, ax, ay, az);
+    }
     cnt += (uint32_t)(nn + nq);
     if (valid) {
         const uint32_t o = a.tperm[i];
-        a.ax[o] = (double)ax;
-        a.ay[o] = (double)ay;
-        a.az[o] = (double)az;
+        if constexpr (kAdj) {
+            a.gself[3 * (size_t)o] = (double)ax;
+            a.gself[3 * (size_t)o + 1] = (double)ay;
+            a.gself[3 * (size_t)o + 2] = (double)az;
+        } else {
+            a.ax[o] = (double)ax;
+            a.ay[o] = (double)ay;
+            a.az[o] = (double)az;
+        }
         if (a.counts) a.counts[o] = cnt;
     }
 }
@@ -821,7 +976,7 @@ hipError_t launch_walk(bhx_ctx* c, hipStream_t s) {
 hipError_t launch_walk_to(bhx_ctx* c, hipStream_t s, bool group, int precision, double* ox, double* oy, double* oz,
                           uint32_t* counts) {
     if (c->n == 0) return hipSuccess;
-    WalkArgs a;
+    WalkArgs a{};
     a.nodes = c->tree.walk;
     a.nodes32 = c->tree.walk32;
     a.tb4 = c->tb4;
@@ -865,6 +1020,42 @@ hipError_t launch_walk_to(bhx_ctx* c, hipStream_t s, bool group, int precision, 
     } else {
         if (quad) hipLaunchKernelGGL(k_walk64<true>, dim3(grid), dim3(kBlock), 0, s, a);
         else hipLaunchKernelGGL(k_walk64<false>, dim3(grid), dim3(kBlock), 0, s, a);
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_walk_adjoint(bhx_ctx* c, const WalkTree& t, int precision, const double* wx, const double* wy,
+                               const double* wz, double* gself, double* gnode, double* gbody, hipStream_t s) {
+    if (c->n == 0 || c->ngroups == 0) return hipSuccess;
+    WalkArgs a{};
+    a.nodes = t.walk;
+    a.nodes32 = t.walk32;
+    a.tb4 = t.tb4;
+    a.tleaf = t.tleaf;
+    a.groups = c->ngroups;
+    a.gtab = c->gtab;
+    a.sys_root = t.sys_root;
+    a.abort_flag = &c->ctr->pad[0];   // never written: stored trees are complete
+    a.b4 = t.b4;
+    a.tx = t.tx; a.ty = t.ty; a.tz = t.tz; a.tm = t.tm;
+    a.tperm = t.tperm;
+    a.n = (uint32_t)c->n;
+    a.theta2 = c->cfg.theta * c->cfg.theta;
+    a.eps2 = c->cfg.eps * c->cfg.eps;
+    a.theta2f = (float)a.theta2;
+    a.eps2f = (float)a.eps2;
+    a.wx = wx; a.wy = wy; a.wz = wz;
+    a.gself = gself;
+    a.gnode = gnode;
+    a.gbody = gbody;
+    const unsigned groups = c->ngroups;
+    const bool guard = !(a.eps2 > 0.0);
+    if (precision == BHX_FP32) {
+        if (guard) hipLaunchKernelGGL((k_walk_group32<false, true, true>), dim3(8 * ((groups + 7) / 8)), dim3(64), 0, s, a);
+        else hipLaunchKernelGGL((k_walk_group32<false, false, true>), dim3(8 * ((groups + 7) / 8)), dim3(64), 0, s, a);
+    } else {
+        if (guard) hipLaunchKernelGGL((k_walk_group<double, false, true, true>), dim3(groups), dim3(64), 0, s, a);
+        else hipLaunchKernelGGL((k_walk_group<double, false, false, true>), dim3(groups), dim3(64), 0, s, a);
     }
     return hipGetLastError();
 }

@@ -325,3 +325,64 @@ def test_frame_order_matches_composition():
     assert np.array_equal(keys, sk)
     assert np.array_equal(p, pp) and np.array_equal(v, vv) and np.array_equal(a, aa)
     assert st.nodes == len(t)
+
+
+# ---------------------------------------------------------------- trajectory gradient
+
+
+def _traj_loss(p, v, m, gx, gv, eps, dt, T, init_acc=True, acc=None, ga=None):
+    P, V, A = orc.traj_direct(p, v, m, eps, dt, T, init_acc=init_acc, acc=acc)
+    out = (gx * P).sum() + (gv * V).sum()
+    if ga is not None:
+        out += (ga * A).sum()
+    return out
+
+
+@pytest.mark.parametrize("init_acc", [True, False])
+def test_traj_adjoint_matches_central_differences(init_acc):
+    """The direct-sum trajectory has no decisions, so central differences pin its adjoint."""
+    rng = np.random.default_rng(3)
+    n, eps, dt, T, h = 7, 0.05, 1e-2, 12, 1e-6
+    p = rng.normal(size=(n, 3))
+    v = 0.3 * rng.normal(size=(n, 3))
+    m = rng.uniform(0.5, 1.5, n) / n
+    a0 = None if init_acc else 0.1 * rng.normal(size=(n, 3))
+    gx, gv, ga = rng.normal(size=(n, 3)), rng.normal(size=(n, 3)), rng.normal(size=(n, 3))
+    G = orc.traj_direct_adjoint(p, v, m, eps, dt, T, gx, gv, ga, init_acc=init_acc, acc=a0)
+    L = lambda p_, v_, m_, a_: _traj_loss(p_, v_, m_, gx, gv, eps, dt, T, init_acc, a_, ga)
+    for (i, a) in [(0, 0), (3, 2), (6, 1)]:
+        for arg in range(2 if init_acc else 3):
+            base = [p, v, a0][arg]
+            hp, hm = base.copy(), base.copy()
+            hp[i, a] += h
+            hm[i, a] -= h
+            args_p = [p, v, a0]
+            args_m = [p, v, a0]
+            args_p[arg], args_m[arg] = hp, hm
+            fd = (L(args_p[0], args_p[1], m, args_p[2]) - L(args_m[0], args_m[1], m, args_m[2])) / (2 * h)
+            got = G[arg][i, a]
+            assert abs(fd - got) <= 1e-6 * abs(fd) + 1e-9, (arg, i, a, fd, got)
+    for i in (0, 5):
+        hh = 1e-7
+        mp, mm = m.copy(), m.copy()
+        mp[i] += hh
+        mm[i] -= hh
+        fd = (L(p, v, mp, a0) - L(p, v, mm, a0)) / (2 * hh)
+        assert abs(fd - G[3][i]) <= 1e-5 * abs(fd) + 1e-9
+    if init_acc:
+        assert np.all(G[2] == 0.0)
+
+
+def test_traj_adjoint_two_body_one_frame_closed_form():
+    """One frame of two bodies: dL/dm for L = x_T[0, 0] from the closed form of the kick-drift-kick."""
+    eps, dt = 0.1, 0.05
+    p = np.array([[0.0, 0.0, 0.0], [1.0, 0.0, 0.0]])
+    v = np.zeros((2, 3))
+    m = np.array([0.6, 0.4])
+    gx = np.zeros((2, 3))
+    gx[0, 0] = 1.0
+    G = orc.traj_direct_adjoint(p, v, m, eps, dt, 1, gx, np.zeros((2, 3)), init_acc=True)
+    # x_T[0,0] = x0 + (a0 dt/2) dt with a0_x = m1 r^-3 (r^2 = 1 + eps^2): d/dm1 = dt^2/2 r^-3
+    r3 = (1.0 + eps * eps) ** 1.5
+    assert G[3][1] == pytest.approx(0.5 * dt * dt / r3, rel=1e-12)
+    assert G[3][0] == pytest.approx(0.0, abs=1e-15)
