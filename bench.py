@@ -160,6 +160,50 @@ def ensemble_bench(bhx, nsys, steps, warmup, seed=1):
     return res
 
 
+def adjoint_bench(bhx, n=65536, steps=1000, every=100, seed=5):
+    """Forward + reverse pass over a recorded trajectory against the forward alone.
+    Loss: mean |x_T - x_T*|^2, the target trajectory from initial conditions perturbed by
+    1e-3 (positions, velocities) and 1% (masses).  FP64 state, FP32 walk, monopole."""
+    pos, vel, mass = bhx.plummer(n, seed=seed)
+    rng = np.random.default_rng(seed + 1)
+    pos_t = pos + 1e-3 * rng.normal(size=pos.shape)
+    vel_t = vel + 1e-3 * rng.normal(size=vel.shape)
+    mass_t = mass * (1.0 + 0.01 * rng.normal(size=mass.shape))
+    cfg = dict(theta=0.5, eps=0.01, dt=1e-3, leaf_capacity=32, mode=bhx.INTENDED, precision=bhx.FP32,
+               quadrupole=0, compensated=1, group_walk=1)
+    res = {"n": n, "steps": steps, "checkpoint_every": every, "theta": 0.5, "eps": 0.01, "dt": 1e-3,
+           "walk": "FP32 group walk, monopole", "loss": "mean |x_T - x_T*|^2"}
+    with bhx.Engine(n, **cfg) as e:
+        e.set_bodies(pos_t, vel_t, mass_t)
+        e.record(steps, every)
+        target = e.recorded_bodies()[0]
+        e.set_bodies(pos, vel, mass)
+        e.step_graph(5)                      # capture and warm the frame graphs
+        e.set_bodies(pos, vel, mass)
+        e.sync()
+        t0 = time.perf_counter()
+        e.step_graph(steps)
+        e.sync()
+        t_fwd = time.perf_counter() - t0
+        e.set_bodies(pos, vel, mass)
+        t0 = time.perf_counter()
+        e.record(steps, every)
+        xT = e.recorded_bodies()[0]
+        t_rec = time.perf_counter() - t0
+        loss = float(np.mean(np.sum((xT - target) ** 2, axis=1)))
+        gx = 2.0 * (xT - target) / n
+        t0 = time.perf_counter()
+        gx0, gv0, _, gm = e.adjoint(gx)
+        t_adj = time.perf_counter() - t0
+    log("adjoint: forward %.3f s, recorded forward %.3f s, reverse %.3f s" % (t_fwd, t_rec, t_adj))
+    res.update({"forward_s": t_fwd, "recorded_forward_s": t_rec, "adjoint_s": t_adj, "loss_value": loss,
+                "grad_norm_x0": float(np.linalg.norm(gx0)), "grad_norm_m": float(np.linalg.norm(gm)),
+                "fwd_adjoint_over_fwd": (t_rec + t_adj) / t_fwd,
+                "forward": "bhx_step_graph, no recording",
+                "fwd_adjoint": "bhx_record (checkpoints) + bhx_adjoint (segments re-run, adjoint walks)"})
+    return res
+
+
 def cpu_baseline(pos, vel, mass, theta, eps, dt, cap, nsub, seed):
     """Serial oracle (oracle/liboracle.so, one core) on a bounded sample of the workload:
     every O(N log N) phase over all N bodies, the walk for `nsub` seeded bodies,
@@ -266,6 +310,7 @@ def main():
     ap.add_argument("--no-ensemble", action="store_true")
     ap.add_argument("--ensemble-systems", type=int, default=512)
     ap.add_argument("--no-drift", action="store_true")
+    ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--calibrate-copy-mib", type=int, default=0,
                     help="first copy this many MiB device-to-device (counter calibration runs)")
     args = ap.parse_args()
@@ -345,6 +390,9 @@ def main():
         line["fp32_over_fp64_walk"] = f64["walk_ms"] / walk_ms
     if not args.no_ensemble:
         line["ensemble"] = ensemble_bench(bhx, args.ensemble_systems, 10, 2)
+    if not args.no_adjoint:
+        line["adjoint"] = adjoint_bench(bhx)
+        line["fwd_adjoint_over_fwd"] = line["adjoint"]["fwd_adjoint_over_fwd"]
     if not args.no_drift:
         log("energy drift (N=16384, 100 steps)")
         line["energy_drift"] = energy_drift(bhx)

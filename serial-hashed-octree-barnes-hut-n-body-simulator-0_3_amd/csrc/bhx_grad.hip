@@ -195,8 +195,12 @@ struct GradRecord {
     uint32_t* ident_tmp = nullptr;
     void* init_store = nullptr;                // the initial walk's tree
     GradSlot init_slot;
-    void* seg_store = nullptr;                 // `every` frame slots, reused per segment
+    void* seg_store = nullptr;                 // two banks of `every` frame slots
     std::vector<GradSlot> slots;
+    // the reverse pass re-runs segment k-1 on the main stream while the
+    // adjoint of segment k runs on its own stream (default priority)
+    hipStream_t s_adj = nullptr;
+    hipEvent_t ev_stored[2] = {}, ev_consumed[2] = {}, ev_ready = nullptr;
     double* work = nullptr;                    // adjoint vectors
     size_t work_nodes = 0;
     double record_ms = 0.0, adjoint_ms = 0.0;
@@ -209,9 +213,13 @@ void free_grad(bhx_ctx* c) {
     if (!g) return;
     for (double* p : g->ckpt)
         if (p) (void)hipFree(p);
+    if (g->s_adj) (void)hipStreamSynchronize(g->s_adj);
     void* ptrs[] = {g->final_state, g->ident, g->ident_tmp, g->init_store, g->seg_store, g->work};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
+    for (hipEvent_t e : {g->ev_stored[0], g->ev_stored[1], g->ev_consumed[0], g->ev_consumed[1], g->ev_ready})
+        if (e) (void)hipEventDestroy(e);
+    if (g->s_adj) (void)hipStreamDestroy(g->s_adj);
     delete g;
     c->grad = nullptr;
 }
@@ -521,15 +529,24 @@ int bhx_adjoint(bhx_ctx* c, const double* gx, const double* gv, const double* ga
     if (!g->seg_store) {
         const size_t one = slot_bytes(n, g->node_cap, c->nsys);
         void* st = nullptr;
-        if (hipMalloc(&st, one * (size_t)every) != hipSuccess) return fail(c, BHX_E_NOMEM, "store allocation");
+        if (hipMalloc(&st, 2 * one * (size_t)every) != hipSuccess) return fail(c, BHX_E_NOMEM, "store allocation");
         g->seg_store = st;
         char* p = (char*)st;
-        g->slots.resize(every);
-        for (int k = 0; k < every; ++k) g->slots[k] = carve_slot(p, n, g->node_cap, c->nsys, g->precision);
+        g->slots.resize(2 * (size_t)every);
+        for (auto& sl : g->slots) sl = carve_slot(p, n, g->node_cap, c->nsys, g->precision);
+    }
+    if (!g->s_adj) {
+        HIPCHK(hipStreamCreateWithFlags(&g->s_adj, hipStreamNonBlocking));
+        for (int b = 0; b < 2; ++b) {
+            HIPCHK(hipEventCreateWithFlags(&g->ev_stored[b], hipEventDisableTiming));
+            HIPCHK(hipEventCreateWithFlags(&g->ev_consumed[b], hipEventDisableTiming));
+        }
+        HIPCHK(hipEventCreateWithFlags(&g->ev_ready, hipEventDisableTiming));
     }
     if (!g->work) {
         if (dalloc(&g->work, work_len(n, g->work_nodes)) != hipSuccess) return fail(c, BHX_E_NOMEM, "adjoint allocation");
     }
+    hipStream_t sa = g->s_adj;
     Work k = carve_work(g->work, n, g->work_nodes);
     hipLaunchKernelGGL(k_zero_flags, dim3(1), dim3(64), 0, s, c->ctr);
     // incoming adjoints (starting order) into the slot order of the last frame; dL/dm starts at 0
@@ -548,39 +565,59 @@ int bhx_adjoint(bhx_ctx* c, const double* gx, const double* gv, const double* ga
         }
         hipLaunchKernelGGL(k_fill_d, dim3(grid_of(n)), dim3(kBlock), 0, s, k.M[0], n, 0.0);
         HIPCHK(hipGetLastError());
+        HIPCHK(hipEventRecord(g->ev_ready, s));
+        HIPCHK(hipStreamWaitEvent(sa, g->ev_ready, 0));
     }
     int cur = 0;
     const int nseg = (int)g->ckpt.size();
     int rc = BHX_OK;
-    for (int seg = nseg - 1; seg >= 0; --seg) {
-        const int f0 = seg * every;
-        const int f1 = std::min(g->nsteps, f0 + every);
-        // the segment's frames again, bitwise as recorded, keeping what the adjoint reads
-        rc = load_state(c, g->ckpt[seg], s);
-        if (rc != BHX_OK) return rc;
+    bool bank_read[2] = {false, false};
+    // the frames of segment `seg` again, bitwise as recorded, into store bank `bank`
+    auto rerun = [&](int seg) -> int {
+        const int bank = seg & 1;
+        const int f0 = seg * every, f1 = std::min(g->nsteps, f0 + every);
+        if (bank_read[bank]) HIPCHK(hipStreamWaitEvent(s, g->ev_consumed[bank], 0));
+        int r = load_state(c, g->ckpt[seg], s);
+        if (r != BHX_OK) return r;
         for (int f = f0; f < f1; ++f) {
-            rc = graph_frame(c, s);
-            if (rc != BHX_OK) return rc;
-            rc = store_frame(c, g->slots[f - f0], g->node_cap, g->precision, true, s);
+            r = graph_frame(c, s);
+            if (r != BHX_OK) return r;
+            r = store_frame(c, g->slots[(size_t)bank * every + (f - f0)], g->node_cap, g->precision, true, s);
+            if (r != BHX_OK) return r;
+        }
+        HIPCHK(hipEventRecord(g->ev_stored[bank], s));
+        return BHX_OK;
+    };
+    rc = rerun(nseg - 1);
+    if (rc != BHX_OK) return rc;
+    for (int seg = nseg - 1; seg >= 0; --seg) {
+        const int bank = seg & 1;
+        const int f0 = seg * every, f1 = std::min(g->nsteps, f0 + every);
+        if (seg > 0) {   // queued ahead: runs while this segment's adjoint walks do
+            rc = rerun(seg - 1);
             if (rc != BHX_OK) return rc;
         }
+        HIPCHK(hipStreamWaitEvent(sa, g->ev_stored[bank], 0));
         for (int f = f1 - 1; f >= f0; --f) {
-            const GradSlot& d = g->slots[f - f0];
-            rc = adjoint_walk(c, g, k, d, k.A[cur], k.V[cur], 0.5 * g->dt, s);
+            const GradSlot& d = g->slots[(size_t)bank * every + (f - f0)];
+            rc = adjoint_walk(c, g, k, d, k.A[cur], k.V[cur], 0.5 * g->dt, sa);
             if (rc != BHX_OK) return rc;
-            hipLaunchKernelGGL(k_adj_kdk, dim3(grid_of(n)), dim3(kBlock), 0, s, k.X[cur], k.V[cur], k.M[cur], k.gself,
-                               k.push, d.perm, n, g->dt, k.X[cur ^ 1], k.V[cur ^ 1], k.A[cur ^ 1], k.M[cur ^ 1]);
+            hipLaunchKernelGGL(k_adj_kdk, dim3(grid_of(n)), dim3(kBlock), 0, sa, k.X[cur], k.V[cur], k.M[cur],
+                               k.gself, k.push, d.perm, n, g->dt, k.X[cur ^ 1], k.V[cur ^ 1], k.A[cur ^ 1],
+                               k.M[cur ^ 1]);
             HIPCHK(hipGetLastError());
             cur ^= 1;
         }
+        HIPCHK(hipEventRecord(g->ev_consumed[bank], sa));
+        bank_read[bank] = true;
     }
     if (g->init_acc) {
         // a_0 = A(x_0, m): its adjoint weights are dL/da_0
-        rc = adjoint_walk(c, g, k, g->init_slot, k.A[cur], k.A[cur], 0.0, s);
+        rc = adjoint_walk(c, g, k, g->init_slot, k.A[cur], k.A[cur], 0.0, sa);
         if (rc != BHX_OK) return rc;
-        hipLaunchKernelGGL(k_adj_init, dim3(grid_of(n)), dim3(kBlock), 0, s, k.X[cur], k.M[cur], k.gself, k.push, n);
+        hipLaunchKernelGGL(k_adj_init, dim3(grid_of(n)), dim3(kBlock), 0, sa, k.X[cur], k.M[cur], k.gself, k.push, n);
         HIPCHK(hipGetLastError());
-        hipLaunchKernelGGL(k_fill_d, dim3(grid_of(3 * n)), dim3(kBlock), 0, s, k.A[cur], 3 * n, 0.0);
+        hipLaunchKernelGGL(k_fill_d, dim3(grid_of(3 * n)), dim3(kBlock), 0, sa, k.A[cur], 3 * n, 0.0);
         HIPCHK(hipGetLastError());
     }
     // the starting slot order is the caller's order
@@ -588,7 +625,8 @@ int bhx_adjoint(bhx_ctx* c, const double* gx, const double* gv, const double* ga
     const double* src[4] = {k.X[cur], k.V[cur], k.A[cur], k.M[cur]};
     const size_t len[4] = {3 * n, 3 * n, 3 * n, n};
     for (int q = 0; q < 4; ++q)
-        if (res[q]) HIPCHK(hipMemcpyAsync(res[q], src[q], len[q] * sizeof(double), hipMemcpyDeviceToHost, s));
+        if (res[q]) HIPCHK(hipMemcpyAsync(res[q], src[q], len[q] * sizeof(double), hipMemcpyDeviceToHost, sa));
+    HIPCHK(hipStreamSynchronize(sa));
     // leave the context at the end of the trajectory
     rc = load_state(c, g->final_state, s);
     if (rc != BHX_OK) return rc;

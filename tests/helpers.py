@@ -66,3 +66,20 @@ def load_bhx():
     sys.modules["bhx"] = mod
     spec.loader.exec_module(mod)
     return mod
+
+
+def load_bhx_torch():
+    """Import the PyTorch autograd wrapper from the package directory."""
+    import importlib.util
+    import os
+    import sys
+    load_bhx()
+    if "bhx_torch" in sys.modules:
+        return sys.modules["bhx_torch"]
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    path = os.path.join(root, "serial-hashed-octree-barnes-hut-n-body-simulator-0_3_amd", "bhx_torch.py")
+    spec = importlib.util.spec_from_file_location("bhx_torch", path)
+    mod = importlib.util.module_from_spec(spec)
+    sys.modules["bhx_torch"] = mod
+    spec.loader.exec_module(mod)
+    return mod

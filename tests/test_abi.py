@@ -59,3 +59,9 @@ def test_plummer_generator_profile_and_virial():
     _, pot = orc.direct(pos, mass, 0.0)
     ep = 0.5 * (mass * pot).sum()
     assert abs(2 * ek / -ep - 1.0) < 0.08      # virial equilibrium 2K = -W
+
+
+def test_torch_wrapper_imports():
+    from tests.helpers import load_bhx_torch
+    bt = load_bhx_torch()
+    assert callable(bt.simulate) and callable(bt.make_engine)
