@@ -70,7 +70,7 @@ void drop_graphs(bhx_ctx* c) {
 void free_tree(bhx_ctx* c) {
     DeviceTree& t = c->tree;
     void* ptrs[] = {t.key, t.start, t.count, t.skip, t.parent, t.child_byte, t.level, t.first, t.center,
-                    t.com, t.mass, t.quad, t.quadf, t.walk, t.walk32, t.hkey, t.hval};
+                    t.com, t.mass, t.quad, t.quadf, t.walk, t.walk32, t.by_level, t.lvl, t.hkey, t.hval};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     t = DeviceTree{};
@@ -96,6 +96,8 @@ hipError_t alloc_tree(bhx_ctx* c, size_t cap) {
     if ((e = dalloc(&t.quadf, 6 * cap)) != hipSuccess) return e;
     if ((e = dalloc(&t.walk, cap)) != hipSuccess) return e;
     if ((e = dalloc(&t.walk32, cap)) != hipSuccess) return e;
+    if ((e = dalloc(&t.by_level, cap)) != hipSuccess) return e;
+    if ((e = dalloc(&t.lvl, 3 * (kMaxLevel + 1) + 1)) != hipSuccess) return e;
     size_t h = 1024;
     while (h < 2 * cap) h <<= 1;
     if ((e = dalloc(&t.hkey, h)) != hipSuccess) return e;
@@ -286,7 +288,7 @@ int enqueue_step_async(bhx_ctx* c, hipStream_t s) {
     HIPCHK(launch_permute_kick_drift(c, c->vals, dt, s));
     HIPCHK(hipEventRecord(c->ev_drift, s));
     HIPCHK(hipStreamWaitEvent(c->s_side, c->ev_drift, 0));
-    HIPCHK(launch_bounds(c->px, c->py, c->pz, sv, c->max_sys_n, b_next, c->s_side));
+    HIPCHK(launch_bounds(c, b_next, c->s_side));
     HIPCHK(hipEventRecord(c->ev_bounds, c->s_side));
     HIPCHK(launch_treekeys(c->px, c->py, c->pz, c->n, sv.id, b_cur, c->tkeys, c->tperm, s));
     HIPCHK(sort_pairs(c, c->tkeys, c->tperm, s));
@@ -589,7 +591,7 @@ int bhx_system_bounds(bhx_ctx* c, double* centers, double* sizes) {
     hipStream_t s = c->s_main;
     HIPCHK(hipStreamSynchronize(c->s_side));
     double* b4 = c->bounds_slot(c->bslot);
-    if (c->n) HIPCHK(launch_bounds(c->px, c->py, c->pz, c->sys(), c->max_sys_n, b4, s));
+    if (c->n) HIPCHK(launch_bounds(c, b4, s));
     std::vector<double> h(4 * (size_t)c->nsys);
     HIPCHK(hipMemcpyAsync(h.data(), b4, h.size() * sizeof(double), hipMemcpyDeviceToHost, s));
     HIPCHK(hipStreamSynchronize(s));
@@ -659,7 +661,7 @@ int bhx_bounds(bhx_ctx* c, double center[3], double* size) {
     HIPCHK(hipStreamSynchronize(c->s_side));
     if (c->nsys > 1) return fail(c, BHX_E_STATE, "several systems: use bhx_system_bounds");
     double* b4 = c->bounds_slot(c->bslot);
-    HIPCHK(launch_bounds(c->px, c->py, c->pz, c->sys(), c->max_sys_n, b4, s));
+    HIPCHK(launch_bounds(c, b4, s));
     double h[4];
     HIPCHK(hipMemcpyAsync(h, b4, sizeof h, hipMemcpyDeviceToHost, s));
     HIPCHK(hipStreamSynchronize(s));
@@ -1007,7 +1009,7 @@ int bhx_step(bhx_ctx* c, int nsteps) {
         const bool prof = c->profiling;
         if (prof) HIPCHK(hipEventRecord(c->ev[0], s));
         if (!c->bounds_valid) {
-            HIPCHK(launch_bounds(c->px, c->py, c->pz, sv, c->max_sys_n, b_cur, s));
+            HIPCHK(launch_bounds(c, b_cur, s));
         } else {
             HIPCHK(hipStreamWaitEvent(s, c->ev_bounds, 0));
         }
@@ -1020,7 +1022,7 @@ int bhx_step(bhx_ctx* c, int nsteps) {
         HIPCHK(hipEventRecord(c->ev_drift, s));
         if (prof) HIPCHK(hipEventRecord(c->ev[4], s));
         HIPCHK(hipStreamWaitEvent(c->s_side, c->ev_drift, 0));
-        HIPCHK(launch_bounds(c->px, c->py, c->pz, sv, c->max_sys_n, b_next, c->s_side));
+        HIPCHK(launch_bounds(c, b_next, c->s_side));
         HIPCHK(hipEventRecord(c->ev_bounds, c->s_side));
         // tree on x_{f+1} with the bounds of x_f
         c->tree_valid = c->tree_built = false;
@@ -1084,7 +1086,7 @@ int bhx_step_graph(bhx_ctx* c, int nsteps) {
     const auto t0 = std::chrono::steady_clock::now();
     HIPCHK(hipStreamSynchronize(c->s_side));
     if (!c->bounds_valid) {
-        HIPCHK(launch_bounds(c->px, c->py, c->pz, c->sys(), c->max_sys_n, c->bounds_slot(c->bslot), s));
+        HIPCHK(launch_bounds(c, c->bounds_slot(c->bslot), s));
         c->bounds_valid = true;
     }
     int rc = take_snapshot(c, c->snaps[0], s);
@@ -1136,7 +1138,7 @@ int bhx_autotune(bhx_ctx* c, const int32_t* caps, size_t ncaps, int steps, doubl
     hipStream_t s = c->s_main;
     HIPCHK(hipStreamSynchronize(c->s_side));
     if (!c->bounds_valid) {
-        HIPCHK(launch_bounds(c->px, c->py, c->pz, c->sys(), c->max_sys_n, c->bounds_slot(c->bslot), s));
+        HIPCHK(launch_bounds(c, c->bounds_slot(c->bslot), s));
         c->bounds_valid = true;
     }
     int rc = take_snapshot(c, c->snaps[1], s);

@@ -330,12 +330,12 @@ __global__ __launch_bounds__(kBlock) void k_tree_gather(const double* __restrict
 
 }  // namespace
 
-hipError_t launch_bounds(const double* px, const double* py, const double* pz, const SysView& sv,
-                         uint32_t max_sys_n, double* bounds4, hipStream_t s) {
-    if (sv.nsys == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_bounds_mean, dim3(sv.nsys), dim3(kBlock), 0, s, px, py, pz, sv, bounds4);
-    hipLaunchKernelGGL(k_bounds_size, dim3(grid_for(max_sys_n, kBlock * 8, 2048), sv.nsys), dim3(kBlock), 0, s, px, py,
-                       pz, sv, bounds4);
+hipError_t launch_bounds(bhx_ctx* c, double* bounds4, hipStream_t s) {
+    const SysView sv = c->sys();
+    if (sv.nsys == 0 || c->n == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_bounds_mean, dim3(sv.nsys), dim3(kBlock), 0, s, c->px, c->py, c->pz, sv, bounds4);
+    hipLaunchKernelGGL(k_bounds_size, dim3(grid_for(c->max_sys_n, kBlock * 8, 2048), sv.nsys), dim3(kBlock), 0, s,
+                       c->px, c->py, c->pz, sv, bounds4);
     return hipGetLastError();
 }
 

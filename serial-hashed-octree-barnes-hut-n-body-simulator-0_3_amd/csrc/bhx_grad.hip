@@ -429,7 +429,7 @@ int bhx_record(bhx_ctx* c, int nsteps, int checkpoint_every, int init_acc) {
         return bail(fail(c, BHX_E_NOMEM, "record allocation"));
     if (g->init_acc) {
         // a_0 = A(x_0): the tree of the starting positions in their own bounds
-        HIPCHK(launch_bounds(c->px, c->py, c->pz, c->sys(), c->max_sys_n, c->bounds_slot(c->bslot), s));
+        HIPCHK(launch_bounds(c, c->bounds_slot(c->bslot), s));
         c->bounds_valid = true;
         rc = build_on_device(c, s);
         if (rc != BHX_OK) return bail(rc);
@@ -446,7 +446,7 @@ int bhx_record(bhx_ctx* c, int nsteps, int checkpoint_every, int init_acc) {
         g->work_nodes = nodes;
     }
     if (!c->bounds_valid) {
-        HIPCHK(launch_bounds(c->px, c->py, c->pz, c->sys(), c->max_sys_n, c->bounds_slot(c->bslot), s));
+        HIPCHK(launch_bounds(c, c->bounds_slot(c->bslot), s));
         c->bounds_valid = true;
     }
     bool done = false;

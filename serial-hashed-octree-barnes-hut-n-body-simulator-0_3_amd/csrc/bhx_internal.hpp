@@ -61,6 +61,8 @@ struct DeviceTree {
     float* quadf = nullptr;          // 6 per node, FP32 copy for the walk
     WalkNode* walk = nullptr;
     WalkNode32* walk32 = nullptr;
+    uint32_t* by_level = nullptr;    // moments: nodes left for the second phase
+    uint32_t* lvl = nullptr;         // moments: their count
     // hash table
     size_t hcap = 0;
     uint64_t* hkey = nullptr;
@@ -90,6 +92,7 @@ struct SysView {
 };
 
 __device__ __forceinline__ uint32_t sys_of(const SysView& v, uint32_t i) { return v.id ? v.id[i] : 0u; }
+
 
 }  // namespace bhx
 
@@ -225,9 +228,9 @@ int graph_frame(bhx_ctx* c, hipStream_t s);           // enqueue one frame from 
 void free_grad(bhx_ctx* c);                           // recorded trajectory (bhx_grad.hip)
 
 // --- launchers implemented in the .hip files (all enqueue on `s`) ----------
-// bounds4 holds 4 doubles (centre, size) per system
-hipError_t launch_bounds(const double* px, const double* py, const double* pz, const SysView& sv,
-                         uint32_t max_sys_n, double* bounds4, hipStream_t s);
+// OctantBounds of the resident bodies of every system into bounds4 (centre,
+// size: 4 doubles per system)
+hipError_t launch_bounds(bhx_ctx* c, double* bounds4, hipStream_t s);
 hipError_t launch_morton(const double* px, const double* py, const double* pz, size_t n, const uint32_t* sysid,
                          const double* bounds4, uint64_t* keys, uint32_t* vals, hipStream_t s);
 hipError_t launch_treekeys(const double* px, const double* py, const double* pz, size_t n, const uint32_t* sysid,

@@ -398,17 +398,13 @@ __device__ __forceinline__ void quad_add(double* q, double m, double dx, double 
 //  INTENDED (computeNodeBaryCenters' aim, HashedOctree.cpp:721-827, written
 //  from the physics): mass-weighted centre of mass and the parallel-axis
 //  quadrupole sum Q = sum_k [M_k (3 d d^T - |d|^2 I) + Q_k].
-__global__ __launch_bounds__(kBlock) void k_moments(int L, const Counters* __restrict__ ctr, SysView sv, int intended,
-                                                    TreeArgs t, const double* __restrict__ bounds4,
-                                                    const double* __restrict__ px, const double* __restrict__ py,
-                                                    const double* __restrict__ pz, const double* __restrict__ tx,
-                                                    const double* __restrict__ ty, const double* __restrict__ tz,
-                                                    const double* __restrict__ tm, const uint32_t* __restrict__ tperm) {
-    if (ctr->node_overflow) return;   // the node table is incomplete: nothing to combine
-    const uint32_t total = ctr->total_nodes;
-    for (size_t pp = blockIdx.x * (size_t)kBlock + threadIdx.x; pp < total; pp += (size_t)gridDim.x * kBlock) {
-        const uint32_t p = (uint32_t)pp;
-        if ((int)t.level[p] != L) continue;
+__device__ void moments_of(uint32_t p, int intended, const TreeArgs& t, const double* __restrict__ bounds4,
+                           const SysView& sv, const double* __restrict__ px, const double* __restrict__ py,
+                           const double* __restrict__ pz, const double* __restrict__ tx,
+                           const double* __restrict__ ty, const double* __restrict__ tz,
+                           const double* __restrict__ tm, const uint32_t* __restrict__ tperm) {
+    {
+        const int L = (int)t.level[p];
         const uint32_t st = t.start[p], cnt = t.count[p], cb = t.child_byte[p];
         const bool leaf = cb == 0;
         double M = 0.0, sx = 0.0, sy = 0.0, sz = 0.0;
@@ -494,6 +490,68 @@ __global__ __launch_bounds__(kBlock) void k_moments(int L, const Counters* __res
     }
 }
 
+// Moments in two launches.  Phase 1: workgroup b owns the pre-order slice
+// of nodes [256 b, 256 b + 256); a node whose whole subtree [p, skip) lies in
+// the slice is combined there, deepest level first, with a workgroup barrier
+// between levels (children and parent share the compute unit, so no
+// device-wide ordering is needed).  Nodes whose subtree leaves the slice
+// (the slice-straddling ancestors, at most 21 per slice boundary) are listed
+// for phase 2, where one workgroup combines them level by level.  Every node
+// is combined once, from its final children in pre-order.
+__global__ __launch_bounds__(kBlock) void k_moments_local(const Counters* __restrict__ ctr, SysView sv, int intended,
+                                                          TreeArgs t, const double* __restrict__ bounds4,
+                                                          const double* __restrict__ px, const double* __restrict__ py,
+                                                          const double* __restrict__ pz, const double* __restrict__ tx,
+                                                          const double* __restrict__ ty, const double* __restrict__ tz,
+                                                          const double* __restrict__ tm,
+                                                          const uint32_t* __restrict__ tperm,
+                                                          uint32_t* __restrict__ rest, uint32_t* __restrict__ nrest) {
+    __shared__ uint32_t s_levels;
+    if (ctr->node_overflow) return;   // the node table is incomplete: nothing to combine
+    const uint32_t total = ctr->total_nodes;
+    const uint32_t b0 = blockIdx.x * kBlock;
+    if (b0 >= total) return;
+    const uint32_t b1 = b0 + kBlock < total ? b0 + kBlock : total;
+    const uint32_t p = b0 + threadIdx.x;
+    const bool valid = p < b1;
+    if (threadIdx.x == 0) s_levels = 0;
+    __syncthreads();
+    uint32_t L = 0;
+    bool local = false;
+    if (valid) {
+        L = t.level[p];
+        local = t.skip[p] <= b1;
+        if (local) atomicOr(&s_levels, 1u << L);
+        else rest[atomicAdd(nrest, 1u)] = p;
+    }
+    __syncthreads();
+    const uint32_t levels = s_levels;
+    for (int l = kMaxLevel; l >= 0; --l) {
+        if (!((levels >> l) & 1u)) continue;   // block-uniform
+        if (local && (int)L == l) moments_of(p, intended, t, bounds4, sv, px, py, pz, tx, ty, tz, tm, tperm);
+        __syncthreads();
+    }
+}
+
+__global__ __launch_bounds__(1024) void k_moments_rest(const Counters* __restrict__ ctr, SysView sv, int intended,
+                                                       TreeArgs t, const double* __restrict__ bounds4,
+                                                       const double* __restrict__ px, const double* __restrict__ py,
+                                                       const double* __restrict__ pz, const double* __restrict__ tx,
+                                                       const double* __restrict__ ty, const double* __restrict__ tz,
+                                                       const double* __restrict__ tm, const uint32_t* __restrict__ tperm,
+                                                       const uint32_t* __restrict__ rest,
+                                                       const uint32_t* __restrict__ nrest) {
+    if (ctr->node_overflow) return;
+    const uint32_t m = *nrest;
+    for (int l = kMaxLevel; l >= 0; --l) {
+        for (uint32_t i = threadIdx.x; i < m; i += blockDim.x) {
+            const uint32_t p = rest[i];
+            if ((int)t.level[p] == l) moments_of(p, intended, t, bounds4, sv, px, py, pz, tx, ty, tz, tm, tperm);
+        }
+        __syncthreads();
+    }
+}
+
 TreeArgs tree_args(bhx_ctx* c) {
     DeviceTree& d = c->tree;
     TreeArgs t;
@@ -562,9 +620,13 @@ hipError_t build_tree(bhx_ctx* c, hipStream_t s, size_t* total_out, bool sync) {
     hipLaunchKernelGGL(k_hash_insert, dim3(grid_for(span)), dim3(kBlock), 0, s, c->tree.key, c->tree.start, sv,
                        c->tree.hkey, c->tree.hval, (uint64_t)(c->tree.hcap - 1), c->ctr);
     const int intended = c->cfg.mode == BHX_MODE_INTENDED;
-    for (int L = kMaxLevel; L >= 0; --L)
-        hipLaunchKernelGGL(k_moments, dim3(grid_for(span)), dim3(kBlock), 0, s, L, c->ctr, sv, intended, t, b4,
-                           c->px, c->py, c->pz, c->tx, c->ty, c->tz, c->tm, c->tperm);
+    uint32_t* nrest = c->tree.lvl;
+    hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, s, nrest, (size_t)1, 0u);
+    hipLaunchKernelGGL(k_moments_local, dim3(grid_for(span, kBlock, 0xFFFFFFFFu)), dim3(kBlock), 0, s, c->ctr, sv,
+                       intended, t, b4, c->px, c->py, c->pz, c->tx, c->ty, c->tz, c->tm, c->tperm, c->tree.by_level,
+                       nrest);
+    hipLaunchKernelGGL(k_moments_rest, dim3(1), dim3(1024), 0, s, c->ctr, sv, intended, t, b4, c->px, c->py, c->pz,
+                       c->tx, c->ty, c->tz, c->tm, c->tperm, c->tree.by_level, nrest);
     return hipGetLastError();
 }
 
