@@ -7,7 +7,12 @@ namespace bhx {
 namespace {
 
 constexpr int kBlock = 256;
-constexpr int kMeanChunk = 2048;   // 2 x 3 x 2048 doubles = 96 KiB of the 160 KiB LDS
+// 2 x 3 x (3328 + 16) doubles = 156.75 KiB: the chain's workgroup takes
+// (nearly) all of its compute unit's 160 KiB LDS, so no other kernel's waves
+// share that CU and delay the dependent adds.  Each row is padded by 16
+// doubles so that the chain's read-ahead never needs a bounds test.
+constexpr int kMeanChunk = 3328;
+constexpr int kMeanRow = kMeanChunk + 16;
 
 inline unsigned grid_for(size_t n, int per_block = kBlock, unsigned cap = 65535u * 16u) {
     size_t g = (n + per_block - 1) / per_block;
@@ -27,7 +32,7 @@ __global__ __launch_bounds__(kBlock) void k_bounds_mean(const double* __restrict
                                                         const double* __restrict__ py,
                                                         const double* __restrict__ pz, SysView sv,
                                                         double* __restrict__ bounds4) {
-    __shared__ double buf[2][3][kMeanChunk];
+    __shared__ double buf[2][3][kMeanRow];
     const int tid = threadIdx.x;
     const int wave = tid >> 6, lane = tid & 63;
     const uint32_t sys = blockIdx.x;
@@ -66,20 +71,25 @@ __global__ __launch_bounds__(kBlock) void k_bounds_mean(const double* __restrict
             const double* v = buf[b ^ 1][lane];
             int j = 0;
             if (cnt == kMeanChunk) {
+                // the last read-ahead lands in the row's padding and is unused;
+                // scheduling barriers keep each read-ahead ahead of the adds
+                // that precede its use (the scheduler would otherwise sink it)
                 double t0[16], t1[16];
 #pragma unroll
                 for (int u = 0; u < 16; ++u) t0[u] = v[u];
                 for (j = 0; j < kMeanChunk; j += 32) {
 #pragma unroll
                     for (int u = 0; u < 16; ++u) t1[u] = v[j + 16 + u];
+                    __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
                     for (int u = 0; u < 16; ++u) s = s + t0[u];
-                    if (j + 32 < kMeanChunk) {
+                    __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
-                        for (int u = 0; u < 16; ++u) t0[u] = v[j + 32 + u];
-                    }
+                    for (int u = 0; u < 16; ++u) t0[u] = v[j + 32 + u];
+                    __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
                     for (int u = 0; u < 16; ++u) s = s + t1[u];
+                    __builtin_amdgcn_sched_barrier(0);
                 }
             } else {
                 for (; j < cnt; ++j) s = s + v[j];

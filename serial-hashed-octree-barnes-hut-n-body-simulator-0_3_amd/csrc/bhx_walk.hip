@@ -685,15 +685,85 @@ __device__ __forceinline__ Node32 load_node32(ConstU64 raw, uint32_t p) {
     return nd;
 }
 
+// FP32 list terms.  Without the zero-separation guard r^2 is formed as
+// ((eps^2 + dx^2) + dy^2) + dz^2 in fused multiply-adds, and the quadrupole
+// folds into the monopole's coefficient:
+//   a += d (M r^-3 + 2.5 (d.Q.d) r^-7) - (Q.d) r^-5
+// (32 vector instructions per cell, 13 per body).  Bodies are entries with
+// Q = 0 and go through the kQuad = false form.
+template <bool kQuad, bool kGuard>
+__device__ __forceinline__ void eval_list32(const float (*nd)[4], const float (*q)[6], int nn, float xr, float yr,
+                                            float zr, float eps2, float& ax, float& ay, float& az) {
+    for (int k = 0; k < nn; ++k) {
+        const float dx = nd[k][0] - xr, dy = nd[k][1] - yr, dz = nd[k][2] - zr;
+        float inv;
+        if (kGuard) {
+            const float d2 = dx * dx + dy * dy + dz * dz;
+            inv = d2 > 0.f ? __builtin_amdgcn_rsqf(d2 + eps2) : 0.f;
+        } else {
+            inv = __builtin_amdgcn_rsqf(fmaf(dz, dz, fmaf(dy, dy, fmaf(dx, dx, eps2))));
+        }
+        const float inv2 = inv * inv;
+        const float inv3 = inv * inv2;
+        float c = nd[k][3] * inv3;
+        if (kQuad) {
+            const float q0 = q[k][0], q1 = q[k][1], q2 = q[k][2], q3 = q[k][3], q4 = q[k][4], q5 = q[k][5];
+            const float qx = fmaf(q2, dz, fmaf(q1, dy, q0 * dx));
+            const float qy = fmaf(q4, dz, fmaf(q3, dy, q1 * dx));
+            const float qz = fmaf(q5, dz, fmaf(q4, dy, q2 * dx));
+            const float dqd = fmaf(dz, qz, fmaf(dy, qy, dx * qx));
+            const float inv5 = inv3 * inv2;
+            c = fmaf(2.5f * dqd, inv5 * inv2, c);
+            ax = fmaf(-inv5, qx, ax);
+            ay = fmaf(-inv5, qy, ay);
+            az = fmaf(-inv5, qz, az);
+        }
+        ax = fmaf(c, dx, ax);
+        ay = fmaf(c, dy, ay);
+        az = fmaf(c, dz, az);
+    }
+}
+
+// List capacities of the FP32 walk, sized so that one workgroup (one wave)
+// uses under 5 KiB of LDS and eight waves fit on every SIMD.
+constexpr int kCells32 = 64;
+constexpr int kBodies32 = 96;
+constexpr int kRanges32 = 32;
+
+// Accepted cells of the FP32 walk are queued as node ids only; a flush
+// gathers them lane-parallel (lane j fetches entry j's record) into the
+// group-relative FP32 entries the evaluation reads.
+template <bool kQuad>
+__device__ __forceinline__ void gather_cells32(float (*nd)[4], float (*q)[6], const uint32_t* nid, int nn,
+                                               uint32_t lane, const WalkNode32* __restrict__ nodes, double ox,
+                                               double oy, double oz) {
+#pragma unroll
+    for (int h = 0; h < (kCells32 + 63) / 64; ++h) {
+        const int e = (int)lane + 64 * h;
+        if (e < nn) {
+            const WalkNode32& c = nodes[nid[e]];
+            nd[e][0] = (float)(c.cx - ox);
+            nd[e][1] = (float)(c.cy - oy);
+            nd[e][2] = (float)(c.cz - oz);
+            nd[e][3] = c.mass;
+            if (kQuad) {
+#pragma unroll
+                for (int u = 0; u < 6; ++u) q[e][u] = c.q[u];
+            }
+        }
+    }
+}
+
 template <bool kQuad, bool kGuard, bool kAdj = false>
 __global__ __launch_bounds__(64) void k_walk_group32(WalkArgs a) {
     static_assert(!(kAdj && kQuad), "the adjoint walk is monopole only");
-    __shared__ float s_nd[kNodeCap][4];
-    __shared__ float s_q[kQuad ? kNodeCap : 1][6];
-    __shared__ float s_bd[kBodyCap][4];
-    __shared__ uint32_t s_rstart[kRangeCap], s_roff[kRangeCap];
-    __shared__ float s_rdel[kRangeCap][3];    // leaf's first body minus the group origin
-    __shared__ uint32_t s_nid[kAdj ? kNodeCap : 1], s_bid[kAdj ? kBodyCap : 1];
+    __shared__ float s_nd[kCells32][4];
+    __shared__ float s_q[kQuad ? kCells32 : 1][6];
+    __shared__ uint32_t s_nid[kCells32];
+    __shared__ float s_bd[kBodies32][4];
+    __shared__ uint32_t s_rstart[kRanges32], s_roff[kRanges32];
+    __shared__ float s_rdel[kRanges32][3];    // leaf's centre minus the group origin
+    __shared__ uint32_t s_bid[kAdj ? kBodies32 : 1];
     __shared__ float s_tg[kAdj ? 64 : 1][6];
 
     const uint32_t grp = xcd_group(blockIdx.x, a.groups);
@@ -703,13 +773,18 @@ __global__ __launch_bounds__(64) void k_walk_group32(WalkArgs a) {
     const uint32_t lane = threadIdx.x;
     const uint32_t i = g0 + lane;
     const bool valid = i < gr.y;
+    const uint64_t vmask = __builtin_amdgcn_ballot_w64(valid);
     const uint32_t iv = valid ? i : g0;
     const uint32_t rid = a.sys_root[gr.z];
     ConstU64 raw = (ConstU64)a.nodes32;
     ConstD tx = (ConstD)a.tx;
     ConstD ty = (ConstD)a.ty;
     ConstD tz = (ConstD)a.tz;
-    const double ox = tx[g0], oy = ty[g0], oz = tz[g0];
+    double ox = tx[g0], oy = ty[g0], oz = tz[g0];
+    // keep the origin in vector registers: the per-cell v_add_f64 then reads
+    // the cell centre as its one scalar operand instead of copying the origin
+    // out of scalar registers on every visit
+    asm volatile("" : "+v"(ox), "+v"(oy), "+v"(oz));
     // own position = (own leaf's centre - origin) + own offset: the same two
     // roundings the body gets as a list entry
     const uint32_t own = a.tleaf[iv];
@@ -717,9 +792,11 @@ __global__ __launch_bounds__(64) void k_walk_group32(WalkArgs a) {
     const float xr = (float)(a.nodes32[own].cx - ox) + ob.x;
     const float yr = (float)(a.nodes32[own].cy - oy) + ob.y;
     const float zr = (float)(a.nodes32[own].cz - oz) + ob.z;
-    const float eps2 = a.eps2f, th2 = a.theta2f;
+    const float eps2 = a.eps2f;
+    // opening test edge^2 < theta^2 d^2 as (edge^2 / theta^2) < d^2, with
+    // edge^2 / theta^2 = (root value) * 4^-L (theta = 0: infinite, never accepts)
     const double s0 = a.b4[4 * (size_t)gr.z + 3];
-    const float s2root = (float)(s0 * s0);
+    const float s2th = (float)(s0 * s0) / a.theta2f;
     float ax = 0.f, ay = 0.f, az = 0.f;
     float wxr = 0.f, wyr = 0.f, wzr = 0.f;
     if constexpr (kAdj) {
@@ -732,138 +809,133 @@ __global__ __launch_bounds__(64) void k_walk_group32(WalkArgs a) {
         s_tg[lane][0] = xr; s_tg[lane][1] = yr; s_tg[lane][2] = zr;
         s_tg[lane][3] = wxr; s_tg[lane][4] = wyr; s_tg[lane][5] = wzr;
     }
+    auto flush_cells = [&](int nn) {
+        __syncthreads();
+        gather_cells32<kQuad>(s_nd, s_q, s_nid, nn, lane, a.nodes32, ox, oy, oz);
+        __syncthreads();
+        if constexpr (kAdj) {
+            adj_self<float>(s_nd, nn, xr, yr, zr, wxr, wyr, wzr, eps2, ax, ay, az);
+            adj_scatter<float>(s_nd, s_nid, nn, s_tg, eps2, lane, a.gnode);
+        } else {
+            eval_list32<kQuad, kGuard>(s_nd, s_q, nn, xr, yr, zr, eps2, ax, ay, az);
+        }
+        __syncthreads();
+    };
+    auto flush_bodies = [&](int nr, int nq) {
+        __syncthreads();
+#pragma unroll
+        for (int h = 0; h < (kBodies32 + 63) / 64; ++h) {
+            const int e = (int)lane + 64 * h;
+            if (e < nq) {
+                int lo = 0, hi = nr - 1;
+                while (lo < hi) {
+                    const int mid = (lo + hi + 1) >> 1;
+                    if ((int)s_roff[mid] <= e) lo = mid; else hi = mid - 1;
+                }
+                const uint32_t j = s_rstart[lo] + (uint32_t)(e - (int)s_roff[lo]);
+                const float4 b = a.tb4[j];
+                s_bd[e][0] = s_rdel[lo][0] + b.x;
+                s_bd[e][1] = s_rdel[lo][1] + b.y;
+                s_bd[e][2] = s_rdel[lo][2] + b.z;
+                s_bd[e][3] = b.w;
+                if constexpr (kAdj) s_bid[e] = j;
+            }
+        }
+        __syncthreads();
+        if constexpr (kAdj) {
+            adj_self<float>(s_bd, nq, xr, yr, zr, wxr, wyr, wzr, eps2, ax, ay, az);
+            adj_scatter<float>(s_bd, s_bid, nq, s_tg, eps2, lane, a.gbody);
+        } else {
+            eval_list32<false, kGuard>(s_bd, s_q, nq, xr, yr, zr, eps2, ax, ay, az);
+        }
+        __syncthreads();
+    };
+    // The traversal fills the lists without touching the accumulators; it
+    // stops when a list needs evaluating (cells full, bodies or ranges unable
+    // to take another leaf chunk of up to 64) and the outer loop evaluates it.
+    // A leaf of more than 64 bodies is queued one chunk per visit (kres = the
+    // next chunk).  Only the root has level 0, and the root is never accepted:
+    // a leaf root queues its bodies before the loop.
     uint32_t cnt = 0;
     int nn = 0, nr = 0, nq = 0;
     const Node32 root = load_node32(raw, rid);
     const uint32_t end = root.skip;
-    uint32_t p = (root.meta & 32u) ? rid : rid + 1;
-    while (p < end) {
-        p = __builtin_amdgcn_readfirstlane(p);
-        const Node32 nd = load_node32(raw, p);
-        const uint32_t nskip = nd.skip > p ? nd.skip : p + 1;   // always forward
-        const uint32_t L = nd.meta & 31u;
-        const bool leaf = (nd.meta & 32u) != 0;
-        const float cx = (float)(nd.cx - ox), cy = (float)(nd.cy - oy), cz = (float)(nd.cz - oz);
-        const float dx = cx - xr, dy = cy - yr, dz = cz - zr;
-        const float d2 = dx * dx + dy * dy + dz * dz;
-        // edge^2 = size^2 / 4^L, exact power-of-two scaling of the rounded root value
-        const float s2 = ldexpf(s2root, -2 * (int)L);
-        const bool pass = (L != 0) & (s2 < th2 * d2);
-        const bool opened = __ballot(valid & !pass) != 0ull;
-        if (!opened) {
-            if (nn == kNodeCap) {
-                __syncthreads();
-                if constexpr (kAdj) {
-                    adj_self<float>(s_nd, nn, xr, yr, zr, wxr, wyr, wzr, eps2, ax, ay, az);
-                    adj_scatter<float>(s_nd, s_nid, nn, s_tg, eps2, lane, a.gnode);
-                } else {
-                    eval_nodes<float, kQuad, kGuard>(s_nd, s_q, nn, xr, yr, zr, eps2, ax, ay, az);
-                }
-                cnt += (uint32_t)nn;
-                nn = 0;
-                __syncthreads();
+    uint32_t p = rid + 1;
+    uint32_t kres = 0;
+    if (root.meta & 32u) {
+        const uint32_t count = root.meta >> 6;
+        const float cx = (float)(root.cx - ox), cy = (float)(root.cy - oy), cz = (float)(root.cz - oz);
+        for (uint32_t k = 0; k < count; k += 64) {
+            if (nq > kBodies32 - 64 || nr == kRanges32) {
+                flush_bodies(nr, nq);
+                cnt += (uint32_t)nq;
+                nr = 0;
+                nq = 0;
             }
             if (lane == 0) {
-                s_nd[nn][0] = cx;
-                s_nd[nn][1] = cy;
-                s_nd[nn][2] = cz;
-                s_nd[nn][3] = nd.mass;
-                if constexpr (kAdj) s_nid[nn] = p;
-                if (kQuad) {
-                    s_q[nn][0] = nd.q0;
-                    s_q[nn][1] = nd.q1;
-                    s_q[nn][2] = nd.q2;
-                    s_q[nn][3] = nd.q3;
-                    s_q[nn][4] = nd.q4;
-                    s_q[nn][5] = nd.q5;
-                }
+                s_rstart[nr] = root.start + k;
+                s_roff[nr] = (uint32_t)nq;
+                s_rdel[nr][0] = cx;
+                s_rdel[nr][1] = cy;
+                s_rdel[nr][2] = cz;
             }
-            ++nn;
-            p = nskip;
-        } else if (leaf) {
-            const uint32_t count = nd.meta >> 6;
-            const float rx = cx, ry = cy, rz = cz;     // the leaf's centre relative to the group origin
-            for (uint32_t k = 0; k < count; k += 64) {
-                const uint32_t m = (count - k) < 64u ? (count - k) : 64u;
-                if (nq + (int)m > kBodyCap || nr == kRangeCap) {
-                    __syncthreads();
-#pragma unroll
-                    for (int h = 0; h < kBodyCap / 64; ++h) {
-                        const int e = (int)lane + 64 * h;
-                        if (e < nq) {
-                            int lo = 0, hi = nr - 1;
-                            while (lo < hi) {
-                                const int mid = (lo + hi + 1) >> 1;
-                                if ((int)s_roff[mid] <= e) lo = mid; else hi = mid - 1;
-                            }
-                            const uint32_t j = s_rstart[lo] + (uint32_t)(e - (int)s_roff[lo]);
-                            const float4 b = a.tb4[j];
-                            s_bd[e][0] = s_rdel[lo][0] + b.x;
-                            s_bd[e][1] = s_rdel[lo][1] + b.y;
-                            s_bd[e][2] = s_rdel[lo][2] + b.z;
-                            s_bd[e][3] = b.w;
-                            if constexpr (kAdj) s_bid[e] = j;
-                        }
-                    }
-                    __syncthreads();
-                    if constexpr (kAdj) {
-                        adj_self<float>(s_bd, nq, xr, yr, zr, wxr, wyr, wzr, eps2, ax, ay, az);
-                        adj_scatter<float>(s_bd, s_bid, nq, s_tg, eps2, lane, a.gbody);
-                    } else {
-                        eval_bodies<float, kGuard>(s_bd, nq, xr, yr, zr, eps2, ax, ay, az);
-                    }
-                    cnt += (uint32_t)nq;
-                    nr = 0;
-                    nq = 0;
-                    __syncthreads();
-                }
+            ++nr;
+            nq += (int)((count - k) < 64u ? (count - k) : 64u);
+        }
+        p = end;
+    }
+    while (p < end) {
+        for (;;) {
+            p = __builtin_amdgcn_readfirstlane(p);
+            const Node32 nd = load_node32(raw, p);
+            const uint32_t nskip = nd.skip > p ? nd.skip : p + 1;   // always forward
+            const int L = (int)(nd.meta & 31u);
+            const float cx = (float)(nd.cx - ox), cy = (float)(nd.cy - oy), cz = (float)(nd.cz - oz);
+            const float dx = cx - xr, dy = cy - yr, dz = cz - zr;
+            const float d2 = dx * dx + dy * dy + dz * dz;
+            const float s2L = ldexpf(s2th, -2 * L);
+            const bool opened = (__builtin_amdgcn_ballot_w64(!(s2L < d2)) & vmask) != 0ull;
+            if (!opened) {
+                if (lane == 0) s_nid[nn] = p;
+                ++nn;
+                p = nskip;
+                if (nn == kCells32) break;
+            } else if (nd.meta & 32u) {
+                const uint32_t count = nd.meta >> 6;
                 if (lane == 0) {
-                    s_rstart[nr] = nd.start + k;
+                    s_rstart[nr] = nd.start + kres;
                     s_roff[nr] = (uint32_t)nq;
-                    s_rdel[nr][0] = rx;
-                    s_rdel[nr][1] = ry;
-                    s_rdel[nr][2] = rz;
+                    s_rdel[nr][0] = cx;
+                    s_rdel[nr][1] = cy;
+                    s_rdel[nr][2] = cz;
                 }
                 ++nr;
-                nq += (int)m;
+                nq += (int)((count - kres) < 64u ? (count - kres) : 64u);
+                kres += 64u;
+                if (kres >= count) {
+                    kres = 0;
+                    p = nskip;
+                }
+                if (nq > kBodies32 - 64 || nr == kRanges32) break;
+            } else {
+                p = p + 1;
             }
-            p = nskip;
-        } else {
-            p = p + 1;
+            if (p >= end) break;
+        }
+        if (nn == kCells32) {
+            flush_cells(nn);
+            cnt += (uint32_t)nn;
+            nn = 0;
+        }
+        if (nq > kBodies32 - 64 || nr == kRanges32) {
+            flush_bodies(nr, nq);
+            cnt += (uint32_t)nq;
+            nr = 0;
+            nq = 0;
         }
     }
-    __syncthreads();
-    if constexpr (kAdj) {
-        adj_self<float>(s_nd, nn, xr, yr, zr, wxr, wyr, wzr, eps2, ax, ay, az);
-        adj_scatter<float>(s_nd, s_nid, nn, s_tg, eps2, lane, a.gnode);
-    } else {
-        eval_nodes<float, kQuad, kGuard>(s_nd, s_q, nn, xr, yr, zr, eps2, ax, ay, az);
-    }
-#pragma unroll
-    for (int h = 0; h < kBodyCap / 64; ++h) {
-        const int e = (int)lane + 64 * h;
-        if (e < nq) {
-            int lo = 0, hi = nr - 1;
-            while (lo < hi) {
-                const int mid = (lo + hi + 1) >> 1;
-                if ((int)s_roff[mid] <= e) lo = mid; else hi = mid - 1;
-            }
-            const uint32_t j = s_rstart[lo] + (uint32_t)(e - (int)s_roff[lo]);
-            const float4 b = a.tb4[j];
-            s_bd[e][0] = s_rdel[lo][0] + b.x;
-            s_bd[e][1] = s_rdel[lo][1] + b.y;
-            s_bd[e][2] = s_rdel[lo][2] + b.z;
-            s_bd[e][3] = b.w;
-            if constexpr (kAdj) s_bid[e] = j;
-        }
-    }
-    __syncthreads();
-    if constexpr (kAdj) {
-        adj_self<float>(s_bd, nq, xr, yr, zr, wxr, wyr, wzr, eps2, ax, ay, az);
-        adj_scatter<float>(s_bd, s_bid, nq, s_tg, eps2, lane, a.gbody);
-    } else {
-        eval_bodies<float, kGuard>(s_bd, nq, xr, yr, zr, eps2, ax, ay, az);
-    }
+    flush_cells(nn);
+    flush_bodies(nr, nq);
     cnt += (uint32_t)(nn + nq);
     if (valid) {
         const uint32_t o = a.tperm[i];
