@@ -32,7 +32,8 @@ typedef enum bhx_status {
     BHX_E_NOMEM = 7,       /* device allocation failed                             */
     BHX_E_STATE = 8,       /* call out of order (e.g. walk before build)           */
     BHX_E_CAPACITY = 9,    /* more bodies than the context was created for         */
-    BHX_E_SORT = 10        /* radix-sort look-back did not complete                */
+    BHX_E_SORT = 10,       /* radix-sort look-back did not complete                */
+    BHX_E_IO = 11          /* a file could not be opened, read, written or parsed   */
 } bhx_status;
 
 /* Node payload semantics. */
@@ -62,6 +63,14 @@ typedef struct bhx_config {
                               1: one conservative test per 64-body group (a cell is
                               accepted only if every body of the group accepts it),
                               shared interaction lists evaluated densely            */
+    int32_t geometry;      /* cell geometry of the tree.  0: the reference's — the
+                              root edge is the bounds size (a half-extent), child
+                              edge = edge/2, child centre = centre +- edge/4
+                              (HashedNode.cpp:50-56), so outer bodies can lie
+                              outside every cell.  1: enclosing cube — the same
+                              recurrence from a root edge of twice the size, so
+                              every cell contains its bodies and the opening test
+                              uses the true edge.  Morton keys are unchanged.       */
 } bhx_config;
 
 typedef struct bhx_ctx bhx_ctx;
@@ -109,6 +118,14 @@ int bhx_set_bodies(bhx_ctx *ctx, const double *pos, const double *vel, const dou
 int bhx_set_accelerations(bhx_ctx *ctx, const double *acc);
 int bhx_get_bodies(bhx_ctx *ctx, double *pos, double *vel, double *mass, double *acc);
 int bhx_num_bodies(const bhx_ctx *ctx, size_t *n);
+
+/* Compensation terms of the compensated leapfrog (config `compensated`): the
+ * rounding error carried per body for position and velocity (3 per body,
+ * xyz interleaved); part of the state a frame continues from.  NULL skips
+ * (get) or zeroes (set).  bhx_set_bodies zeroes them.  (No reference
+ * counterpart: the reference's leapfrog is uncompensated.) */
+int bhx_get_compensation(bhx_ctx *ctx, double *pos_c, double *vel_c);
+int bhx_set_compensation(bhx_ctx *ctx, const double *pos_c, const double *vel_c);
 
 /* ---- the per-step functions ---------------------------------------------- */
 
@@ -204,6 +221,9 @@ int bhx_system_bounds(bhx_ctx *ctx, double *centers, double *sizes);
 /* bhx_build with one root box per system. */
 int bhx_build_systems(bhx_ctx *ctx, const double *centers, const double *sizes, size_t *n_nodes);
 
+/* The current system offsets (nsys + 1 values, offsets[0] = 0). */
+int bhx_get_systems(const bhx_ctx *ctx, uint64_t *offsets);
+
 /* Pre-order index of every system's root in the last tree. */
 int bhx_system_roots(bhx_ctx *ctx, uint64_t *roots);
 
@@ -268,6 +288,15 @@ int bhx_direct(bhx_ctx *ctx, const int64_t *targets, size_t nt, double eps,
 /* FP64 kinetic energy, linear and angular momentum of the resident bodies. */
 int bhx_diagnostics(bhx_ctx *ctx, double *ekin, double mom[3], double angmom[3]);
 
+/* Energies of the resident bodies: kinetic K = sum m v^2 / 2 (FP64),
+ * potential W = sum m phi / 2 summed in FP64 over phi from the group walk of
+ * the configured precision on a tree built for the current positions
+ * (softened; cell quadrupoles when configured; a body at zero separation —
+ * the body itself — adds nothing), and the virial ratio 2K/|W|.  pot_out
+ * (n doubles, array order; may be NULL) receives phi.  The accelerations are
+ * left as they are; the last tree is replaced. */
+int bhx_energy(bhx_ctx *ctx, double *ekin, double *epot, double *virial, double *pot_out);
+
 /* ---- initial conditions (no reference counterpart) ----------------------- */
 
 /* Plummer sphere (Aarseth, Henon & Wielen 1974) with counter-based Philox
@@ -287,6 +316,97 @@ int bhx_ic_ensemble_sizes(size_t nsys, uint64_t seed, uint64_t nmin, uint64_t nm
  * threads (0: all). */
 int bhx_ic_ensemble(size_t nsys, uint64_t seed, const uint64_t *sizes, const double *masses,
                     const double *radii, int threads, double *pos, double *vel, double *mass);
+
+/* Two disk galaxies on a parabolic orbit (G = 1).  Each galaxy: an
+ * exponential disk (surface density ~ exp(-R/R_d), vertical sech^2(z/z_d),
+ * cut at disk_cut R_d) holding disk_fraction of its mass and bodies, and a
+ * Hernquist halo (scale a, cut at halo_cut a) holding the rest.  Halo
+ * velocities are drawn from the isotropic distribution function of the
+ * untruncated Hernquist halo (Hernquist 1990, eq. 17) with the halo's mass;
+ * disk velocities are the circular speed of halo + exponential disk (Freeman
+ * 1970) plus Gaussian dispersion disk_sigma * v_c per component.  The
+ * secondary has mass total / (1 + mass_ratio) and lengths scaled by
+ * (M_2 / M_1)^(1/3).  Bodies are equal-mass, so the body counts split as the
+ * masses.  The galaxies start `separation` apart on the parabolic orbit with
+ * the given pericentre, in the x-y plane, each shifted to its own centre of
+ * mass first; disk k is inclined by incl_k about the x axis.  The pair is
+ * not in exact equilibrium (each component's velocities ignore the other's
+ * truncation). */
+typedef struct bhx_galaxy_params {
+    double total_mass;     /* both galaxies                              (1)   */
+    double mass_ratio;     /* primary : secondary                        (3)   */
+    double disk_fraction;  /* of each galaxy's mass and bodies           (0.2) */
+    double disk_scale;     /* R_d of the primary                         (1)   */
+    double disk_height;    /* z_d of the primary                         (0.1) */
+    double disk_cut;       /* disk truncation radius / R_d               (10)  */
+    double halo_scale;     /* Hernquist a of the primary                 (3)   */
+    double halo_cut;       /* halo truncation radius / a                 (10)  */
+    double disk_sigma;     /* disk velocity dispersion / v_c             (0.1) */
+    double separation;     /* initial distance of the centres            (30)  */
+    double pericentre;     /* pericentre of the parabolic orbit          (5)   */
+    double incl1, incl2;   /* disk inclinations to the orbital plane  (0, pi/3) */
+} bhx_galaxy_params;
+
+int bhx_galaxy_defaults(bhx_galaxy_params *p);
+
+/* n bodies: the primary's first (halo then disk), then the secondary's;
+ * counter-based draws (body i depends only on seed and i), on up to
+ * `threads` host threads (0: all).  Centre-of-mass frame. */
+int bhx_ic_galaxy_pair(size_t n, uint64_t seed, const bhx_galaxy_params *p, int threads,
+                       double *pos, double *vel, double *mass);
+
+/* ---- snapshots (no reference counterpart) --------------------------------
+ * File layout, little-endian: a 256-byte header — magic "BHXSNAP1", uint32
+ * header size (256), uint32 version (1), uint64 n, uint64 nsys, double time,
+ * uint64 step, uint32 flags (bit 0: accelerations stored, bit 1:
+ * compensation terms stored), uint32 0, then the configuration (theta, eps,
+ * dt as doubles; leaf_capacity, mode, precision, quadrupole, permute_acc,
+ * compensated, device, group_walk, geometry as int32), zero padding — followed by
+ * pos[3n], vel[3n], mass[n] (doubles, xyz interleaved), acc[3n] if stored,
+ * the position then velocity compensation terms [6n] if stored, and
+ * offsets[nsys + 1] (uint64). */
+typedef struct bhx_snapshot_info {
+    uint64_t n;            /* bodies                                            */
+    uint64_t nsys;         /* systems (1: one system)                           */
+    double time;           /* simulation time, as given to the writer           */
+    uint64_t step;         /* frame counter, as given to the writer             */
+    int32_t has_acc;       /* accelerations stored                              */
+    int32_t has_comp;      /* compensation terms stored                         */
+    bhx_config cfg;        /* configuration stored with the bodies              */
+} bhx_snapshot_info;
+
+/* Writes host arrays (acc, comp NULL: not stored; comp = 6n doubles, the
+ * position terms then the velocity terms; offsets NULL: one system, the
+ * info's nsys is then taken as 1). */
+int bhx_snapshot_write(const char *path, const bhx_snapshot_info *info, const double *pos, const double *vel,
+                       const double *mass, const double *acc, const double *comp, const uint64_t *offsets);
+int bhx_snapshot_read_info(const char *path, bhx_snapshot_info *info);
+/* Arrays sized from the header; any pointer may be NULL (skipped); acc and
+ * comp are left untouched when the file holds none. */
+int bhx_snapshot_read(const char *path, double *pos, double *vel, double *mass, double *acc, double *comp,
+                      uint64_t *offsets);
+
+/* The context's frame state — bodies, accelerations, compensation terms and
+ * systems — with its configuration to a file; and back: the state is
+ * restored (a frame stepped after bhx_load equals the frame the saving
+ * context would have stepped, bit for bit, under the same configuration),
+ * the context's configuration is left as it is (the stored one is in the
+ * header).  time / step may be NULL on load. */
+int bhx_save(bhx_ctx *ctx, const char *path, double time, uint64_t step);
+int bhx_load(bhx_ctx *ctx, const char *path, double *time, uint64_t *step);
+
+/* ---- visualisation export (VisualizeBodies, Body.hpp:326-336, and
+ * visualizeTree / visualizeNode, HashedOctree.cpp:939-974, as offline files)
+ * Legacy VTK, binary (big-endian as the format requires).  Bodies: a point
+ * per body with point data "mass" and "speed".  Nodes: an axis-aligned box
+ * per node, centred on the node centre with edge = node size (the box the
+ * reference draws with ofDrawBox(center, size)), cell data "level", "count"
+ * and "mass"; only levels min_level..max_level. */
+int bhx_vtk_write_bodies(const char *path, const double *pos, const double *vel, const double *mass, size_t n);
+int bhx_vtk_write_nodes(const char *path, const bhx_node *nodes, size_t count, int min_level, int max_level);
+/* Both files from the context: its bodies, and the nodes of its last build
+ * (either path may be NULL). */
+int bhx_export_vtk(bhx_ctx *ctx, const char *bodies_path, const char *nodes_path, int min_level, int max_level);
 
 #ifdef __cplusplus
 }

@@ -18,7 +18,7 @@ FP64, FP32 = 0, 1
 STATUS = {
     0: "BHX_OK", 1: "BHX_E_NULL", 2: "BHX_E_ARG", 3: "BHX_E_DEPTH", 4: "BHX_E_HASH_FULL",
     5: "BHX_E_NONFINITE", 6: "BHX_E_HIP", 7: "BHX_E_NOMEM", 8: "BHX_E_STATE", 9: "BHX_E_CAPACITY",
-    10: "BHX_E_SORT",
+    10: "BHX_E_SORT", 11: "BHX_E_IO",
 }
 
 NODE_DTYPE = np.dtype([
@@ -40,7 +40,7 @@ class Config(ctypes.Structure):
     _fields_ = [("theta", ctypes.c_double), ("eps", ctypes.c_double), ("dt", ctypes.c_double),
                 ("leaf_capacity", ctypes.c_int32), ("mode", ctypes.c_int32), ("precision", ctypes.c_int32),
                 ("quadrupole", ctypes.c_int32), ("permute_acc", ctypes.c_int32), ("compensated", ctypes.c_int32),
-                ("device", ctypes.c_int32), ("group_walk", ctypes.c_int32)]
+                ("device", ctypes.c_int32), ("group_walk", ctypes.c_int32), ("geometry", ctypes.c_int32)]
 
 
 class Timing(ctypes.Structure):
@@ -78,6 +78,16 @@ def lib():
             "bhx_autotune": [P, P, S, I, D, S, ctypes.c_uint64, P, P],
             "bhx_record": [P, I, I, I], "bhx_recorded_bodies": [P, P, P, P, P],
             "bhx_adjoint": [P, P, P, P, P, P, P, P], "bhx_grad_timing": [P, P, P], "bhx_system_roots": [P, P], "bhx_lookup_in": [P, S, P, S, P],
+            "bhx_get_systems": [P, P], "bhx_energy": [P, P, P, P, P],
+            "bhx_galaxy_defaults": [P], "bhx_ic_galaxy_pair": [S, ctypes.c_uint64, P, I, P, P, P],
+            "bhx_snapshot_write": [ctypes.c_char_p, P, P, P, P, P, P, P],
+            "bhx_snapshot_read_info": [ctypes.c_char_p, P],
+            "bhx_snapshot_read": [ctypes.c_char_p, P, P, P, P, P, P],
+            "bhx_get_compensation": [P, P, P], "bhx_set_compensation": [P, P, P],
+            "bhx_save": [P, ctypes.c_char_p, D, ctypes.c_uint64], "bhx_load": [P, ctypes.c_char_p, P, P],
+            "bhx_vtk_write_bodies": [ctypes.c_char_p, P, P, P, S],
+            "bhx_vtk_write_nodes": [ctypes.c_char_p, P, S, I, I],
+            "bhx_export_vtk": [P, ctypes.c_char_p, ctypes.c_char_p, I, I],
         }
         for name, args in sigs.items():
             f = getattr(L, name)
@@ -127,6 +137,97 @@ def ensemble(nsys, seed=1, nmin=1024, nmax=65536, jitter=0.2, threads=0):
         raise BhxError(rc, "bhx_ic_ensemble")
     off = np.concatenate([[0], np.cumsum(sizes)]).astype(np.uint64)
     return pos, vel, mass, off
+
+
+class GalaxyParams(ctypes.Structure):
+    _fields_ = [(k, ctypes.c_double) for k in
+                ("total_mass", "mass_ratio", "disk_fraction", "disk_scale", "disk_height", "disk_cut",
+                 "halo_scale", "halo_cut", "disk_sigma", "separation", "pericentre", "incl1", "incl2")]
+
+
+def galaxy_params(**kw):
+    """bhx_galaxy_params with the library's defaults, then the given fields."""
+    p = GalaxyParams()
+    lib().bhx_galaxy_defaults(ctypes.byref(p))
+    for k, v in kw.items():
+        if not hasattr(p, k):
+            raise KeyError(k)
+        setattr(p, k, v)
+    return p
+
+
+def galaxy_pair(n, seed=1, threads=0, **params):
+    """Two disk + Hernquist-halo galaxies on a parabolic orbit (host generator of the
+    library): pos (n,3), vel (n,3), mass (n,); the primary's bodies first (halo, disk)."""
+    p = galaxy_params(**params)
+    pos, vel, mass = np.empty((n, 3)), np.empty((n, 3)), np.empty(n)
+    rc = lib().bhx_ic_galaxy_pair(n, seed, ctypes.byref(p), threads, _p(pos), _p(vel), _p(mass))
+    if rc:
+        raise BhxError(rc, "bhx_ic_galaxy_pair")
+    return pos, vel, mass
+
+
+class SnapshotInfo(ctypes.Structure):
+    _fields_ = [("n", ctypes.c_uint64), ("nsys", ctypes.c_uint64), ("time", ctypes.c_double),
+                ("step", ctypes.c_uint64), ("has_acc", ctypes.c_int32), ("has_comp", ctypes.c_int32),
+                ("cfg", Config)]
+
+
+def _path(path):
+    return os.fsencode(path)
+
+
+def write_snapshot(path, pos, vel, mass, acc=None, offsets=None, time=0.0, step=0, config=None, comp=None):
+    """Write host arrays as a snapshot file (see bhx_snapshot_write); comp = (6n,)
+    or (2, n, 3): position then velocity compensation terms."""
+    pos, vel, mass = _f64(pos), _f64(vel), _f64(mass)
+    acc = None if acc is None else _f64(acc)
+    comp = None if comp is None else _f64(comp).reshape(-1)
+    off = None if offsets is None else np.ascontiguousarray(offsets, dtype=np.uint64)
+    info = SnapshotInfo()
+    info.n = mass.size
+    info.nsys = 1 if off is None else off.size - 1
+    info.time, info.step = time, step
+    info.cfg = config if config is not None else default_config()
+    rc = lib().bhx_snapshot_write(_path(path), ctypes.byref(info), _p(pos), _p(vel), _p(mass), _p(acc), _p(comp),
+                                  _p(off))
+    if rc:
+        raise BhxError(rc, "bhx_snapshot_write")
+
+
+def read_snapshot(path):
+    """Contents of a snapshot file: dict with pos, vel, mass, acc and comp (None if
+    absent; comp shaped (2, n, 3)), offsets, time, step and config (a Config)."""
+    info = SnapshotInfo()
+    rc = lib().bhx_snapshot_read_info(_path(path), ctypes.byref(info))
+    if rc:
+        raise BhxError(rc, "bhx_snapshot_read_info")
+    n = int(info.n)
+    pos, vel, mass = np.empty((n, 3)), np.empty((n, 3)), np.empty(n)
+    acc = np.empty((n, 3)) if info.has_acc else None
+    comp = np.empty((2, n, 3)) if info.has_comp else None
+    off = np.empty(int(info.nsys) + 1, np.uint64)
+    rc = lib().bhx_snapshot_read(_path(path), _p(pos), _p(vel), _p(mass), _p(acc), _p(comp), _p(off))
+    if rc:
+        raise BhxError(rc, "bhx_snapshot_read")
+    return dict(pos=pos, vel=vel, mass=mass, acc=acc, comp=comp, offsets=off, time=info.time,
+                step=int(info.step), config=info.cfg)
+
+
+def write_vtk_bodies(path, pos, vel=None, mass=None):
+    pos = _f64(pos)
+    vel = None if vel is None else _f64(vel)
+    mass = None if mass is None else _f64(mass)
+    rc = lib().bhx_vtk_write_bodies(_path(path), _p(pos), _p(vel), _p(mass), pos.shape[0])
+    if rc:
+        raise BhxError(rc, "bhx_vtk_write_bodies")
+
+
+def write_vtk_nodes(path, nodes, min_level=0, max_level=21):
+    nodes = np.ascontiguousarray(nodes, dtype=NODE_DTYPE)
+    rc = lib().bhx_vtk_write_nodes(_path(path), _p(nodes), nodes.size, min_level, max_level)
+    if rc:
+        raise BhxError(rc, "bhx_vtk_write_nodes")
 
 
 def default_config(**kw):
@@ -369,6 +470,37 @@ class Engine:
         self._chk(lib().bhx_last_timing(self.h, ctypes.byref(t)), "bhx_last_timing")
         return {k: getattr(t, k) for k, _ in Timing._fields_}
 
+    def get_compensation(self):
+        pc, vc = np.empty((self.n, 3)), np.empty((self.n, 3))
+        self._chk(lib().bhx_get_compensation(self.h, _p(pc), _p(vc)), "bhx_get_compensation")
+        return pc, vc
+
+    def set_compensation(self, pos_c=None, vel_c=None):
+        pc = None if pos_c is None else _f64(pos_c, (self.n, 3))
+        vc = None if vel_c is None else _f64(vel_c, (self.n, 3))
+        self._chk(lib().bhx_set_compensation(self.h, _p(pc), _p(vc)), "bhx_set_compensation")
+
+    def get_systems(self):
+        off = np.empty(self.nsys + 1, np.uint64)
+        self._chk(lib().bhx_get_systems(self.h, _p(off)), "bhx_get_systems")
+        return off
+
+    # --- snapshots and visualisation export
+    def save(self, path, time=0.0, step=0):
+        self._chk(lib().bhx_save(self.h, _path(path), time, step), "bhx_save")
+
+    def load(self, path):
+        """Restores bodies, accelerations and systems; returns (time, step)."""
+        t = ctypes.c_double()
+        st = ctypes.c_uint64()
+        self._chk(lib().bhx_load(self.h, _path(path), ctypes.byref(t), ctypes.byref(st)), "bhx_load")
+        return t.value, st.value
+
+    def export_vtk(self, bodies_path=None, nodes_path=None, min_level=0, max_level=21):
+        self._chk(lib().bhx_export_vtk(self.h, None if bodies_path is None else _path(bodies_path),
+                                       None if nodes_path is None else _path(nodes_path), min_level, max_level),
+                  "bhx_export_vtk")
+
     # --- diagnostics
     def direct(self, targets, eps):
         targets = np.ascontiguousarray(targets, dtype=np.int64)
@@ -376,6 +508,14 @@ class Engine:
         pot = np.empty(targets.size)
         self._chk(lib().bhx_direct(self.h, _p(targets), targets.size, eps, _p(acc), _p(pot)), "bhx_direct")
         return acc, pot
+
+    def energy(self, potentials=False):
+        """(K, W, 2K/|W|) with W from the tree walk; with potentials=True also phi per body."""
+        k, w, v = ctypes.c_double(), ctypes.c_double(), ctypes.c_double()
+        phi = np.empty(self.n) if potentials else None
+        self._chk(lib().bhx_energy(self.h, ctypes.byref(k), ctypes.byref(w), ctypes.byref(v), _p(phi)),
+                  "bhx_energy")
+        return (k.value, w.value, v.value, phi) if potentials else (k.value, w.value, v.value)
 
     def diagnostics(self):
         ek = np.zeros(1)

@@ -28,7 +28,8 @@ UNITS = {
     "bhx_api.hip": [],
     "bhx_grad.hip": [],
 }
-HOST_UNITS = {"bhx_ic.cpp": ["-O2", "-fPIC", "-std=c++17", "-ffp-contract=off", "-pthread"]}
+HOST_UNITS = {"bhx_ic.cpp": ["-O2", "-fPIC", "-std=c++17", "-ffp-contract=off", "-pthread"],
+              "bhx_io.cpp": ["-O2", "-fPIC", "-std=c++17", "-Wall"]}
 
 
 def _run(cmd):

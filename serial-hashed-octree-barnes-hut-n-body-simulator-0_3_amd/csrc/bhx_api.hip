@@ -139,6 +139,7 @@ int check_cfg(const bhx_config* cfg) {
     if (cfg->leaf_capacity < 1 || cfg->leaf_capacity > kMaxLeafCap) return BHX_E_ARG;
     if (cfg->mode != BHX_MODE_AS_RUN && cfg->mode != BHX_MODE_INTENDED) return BHX_E_ARG;
     if (cfg->precision != BHX_FP64 && cfg->precision != BHX_FP32) return BHX_E_ARG;
+    if (cfg->geometry != 0 && cfg->geometry != 1) return BHX_E_ARG;
     return BHX_OK;
 }
 
@@ -234,7 +235,7 @@ int build_on_device(bhx_ctx* c, hipStream_t s) {
         return BHX_OK;
     }
     const double* b4 = c->bounds_slot(c->bslot);
-    HIPCHK(launch_treekeys(c->px, c->py, c->pz, c->n, c->sys().id, b4, c->tkeys, c->tperm, s));
+    HIPCHK(launch_treekeys(c->px, c->py, c->pz, c->n, c->sys().id, b4, c->cfg_tscale(), c->tkeys, c->tperm, s));
     HIPCHK(sort_pairs(c, c->tkeys, c->tperm, s));
     HIPCHK(launch_tree_gather(c, s));
     size_t total = 0;
@@ -290,7 +291,7 @@ int enqueue_step_async(bhx_ctx* c, hipStream_t s) {
     HIPCHK(hipStreamWaitEvent(c->s_side, c->ev_drift, 0));
     HIPCHK(launch_bounds(c, b_next, c->s_side));
     HIPCHK(hipEventRecord(c->ev_bounds, c->s_side));
-    HIPCHK(launch_treekeys(c->px, c->py, c->pz, c->n, sv.id, b_cur, c->tkeys, c->tperm, s));
+    HIPCHK(launch_treekeys(c->px, c->py, c->pz, c->n, sv.id, b_cur, c->cfg_tscale(), c->tkeys, c->tperm, s));
     HIPCHK(sort_pairs(c, c->tkeys, c->tperm, s));
     HIPCHK(launch_tree_gather(c, s));
     size_t total = 0;
@@ -436,6 +437,7 @@ int bhx_config_default(bhx_config* cfg) {
     cfg->permute_acc = 1;
     cfg->compensated = 0;
     cfg->device = 0;
+    cfg->geometry = 0;
     return BHX_OK;
 }
 
@@ -513,7 +515,8 @@ int bhx_set_config(bhx_ctx* c, const bhx_config* cfg) {
     if (!c || !cfg) return BHX_E_NULL;
     if (check_cfg(cfg) != BHX_OK) return fail(c, BHX_E_ARG, "invalid config");
     if (cfg->device != c->device) return fail(c, BHX_E_ARG, "device cannot change");
-    const bool tree_shape = cfg->leaf_capacity != c->cfg.leaf_capacity || cfg->mode != c->cfg.mode;
+    const bool tree_shape = cfg->leaf_capacity != c->cfg.leaf_capacity || cfg->mode != c->cfg.mode ||
+                            cfg->geometry != c->cfg.geometry;
     c->cfg = *cfg;
     if (tree_shape) c->tree_valid = false;
     drop_graphs(c);   // captured launches hold the old parameters
@@ -585,6 +588,17 @@ int bhx_num_systems(const bhx_ctx* c, size_t* nsys) {
     return BHX_OK;
 }
 
+int bhx_get_systems(const bhx_ctx* c, uint64_t* offsets) {
+    if (!c || !offsets) return BHX_E_NULL;
+    if (c->sys_off_h.size() != (size_t)c->nsys + 1) {
+        offsets[0] = 0;
+        offsets[1] = c->n;
+        return BHX_OK;
+    }
+    for (size_t k = 0; k <= (size_t)c->nsys; ++k) offsets[k] = c->sys_off_h[k];
+    return BHX_OK;
+}
+
 int bhx_system_bounds(bhx_ctx* c, double* centers, double* sizes) {
     if (!c) return BHX_E_NULL;
     HIPCHK(hipSetDevice(c->device));
@@ -645,6 +659,48 @@ int bhx_get_bodies(bhx_ctx* c, double* pos, double* vel, double* mass, double* a
     }
     if (mass) HIPCHK(hipMemcpyAsync(mass, c->m, n * sizeof(double), hipMemcpyDeviceToHost, s));
     HIPCHK(hipStreamSynchronize(s));
+    return BHX_OK;
+}
+
+int bhx_get_compensation(bhx_ctx* c, double* pos_c, double* vel_c) {
+    if (!c) return BHX_E_NULL;
+    HIPCHK(hipSetDevice(c->device));
+    hipStream_t s = c->s_main;
+    HIPCHK(hipStreamSynchronize(c->s_side));
+    const size_t n = c->n;
+    if (!n) return BHX_OK;
+    struct { double* dst; double *x, *y, *z; } outs[] = {{pos_c, c->cpx, c->cpy, c->cpz},
+                                                         {vel_c, c->cvx, c->cvy, c->cvz}};
+    for (auto& o : outs) {
+        if (!o.dst) continue;
+        hipLaunchKernelGGL(k_soa_to_aos, dim3(grid_for(n)), dim3(kBlock), 0, s, o.x, o.y, o.z, n, c->stage);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipMemcpyAsync(o.dst, c->stage, 3 * n * sizeof(double), hipMemcpyDeviceToHost, s));
+        HIPCHK(hipStreamSynchronize(s));
+    }
+    return BHX_OK;
+}
+
+int bhx_set_compensation(bhx_ctx* c, const double* pos_c, const double* vel_c) {
+    if (!c) return BHX_E_NULL;
+    HIPCHK(hipSetDevice(c->device));
+    hipStream_t s = c->s_main;
+    HIPCHK(hipStreamSynchronize(c->s_side));
+    const size_t n = c->n;
+    if (!n) return BHX_OK;
+    struct { const double* src; double *x, *y, *z; } ins[] = {{pos_c, c->cpx, c->cpy, c->cpz},
+                                                              {vel_c, c->cvx, c->cvy, c->cvz}};
+    for (auto& o : ins) {
+        if (o.src) {
+            HIPCHK(hipMemcpyAsync(c->stage, o.src, 3 * n * sizeof(double), hipMemcpyHostToDevice, s));
+            hipLaunchKernelGGL(k_aos_to_soa, dim3(grid_for(n)), dim3(kBlock), 0, s, c->stage, n, o.x, o.y, o.z);
+        } else {
+            double* z3[] = {o.x, o.y, o.z};
+            for (double* p : z3) hipLaunchKernelGGL(k_zero_d, dim3(grid_for(n)), dim3(kBlock), 0, s, p, n);
+        }
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipStreamSynchronize(s));
+    }
     return BHX_OK;
 }
 
@@ -872,7 +928,7 @@ int bhx_get_nodes(bhx_ctx* c, bhx_node* out, size_t cap, size_t* n_nodes) {
         // the node's system: last offset <= its first body
         const size_t k = (size_t)(std::upper_bound(c->sys_off_h.begin(), c->sys_off_h.end() - 1, start[i]) -
                                   c->sys_off_h.begin()) - 1;
-        double sz = 4 * k + 3 < c->tree_bounds_h.size() ? c->tree_bounds_h[4 * k + 3] : 0.0;
+        double sz = 4 * k + 3 < c->tree_bounds_h.size() ? c->tree_bounds_h[4 * k + 3] * c->tree_tscale : 0.0;
         for (uint32_t l = 0; l < level[i]; ++l) sz = sz * 0.5;
         o.size = sz;
         o.mass = mass[i];
@@ -1026,7 +1082,7 @@ int bhx_step(bhx_ctx* c, int nsteps) {
         HIPCHK(hipEventRecord(c->ev_bounds, c->s_side));
         // tree on x_{f+1} with the bounds of x_f
         c->tree_valid = c->tree_built = false;
-        HIPCHK(launch_treekeys(c->px, c->py, c->pz, c->n, sv.id, b_cur, c->tkeys, c->tperm, s));
+        HIPCHK(launch_treekeys(c->px, c->py, c->pz, c->n, sv.id, b_cur, c->cfg_tscale(), c->tkeys, c->tperm, s));
         HIPCHK(sort_pairs(c, c->tkeys, c->tperm, s));
         HIPCHK(launch_tree_gather(c, s));
         if (prof) HIPCHK(hipEventRecord(c->ev[5], s));
@@ -1252,6 +1308,40 @@ int bhx_direct(bhx_ctx* c, const int64_t* targets, size_t nt, double eps, double
     if (da) (void)hipFree(da);
     if (dp) (void)hipFree(dp);
     if (e != hipSuccess) return fail(c, BHX_E_HIP, "direct sum", e);
+    return BHX_OK;
+}
+
+int bhx_energy(bhx_ctx* c, double* ekin, double* epot, double* virial, double* pot_out) {
+    if (!c) return BHX_E_NULL;
+    double K = 0.0;
+    int rc = bhx_diagnostics(c, &K, nullptr, nullptr);
+    if (rc != BHX_OK) return rc;
+    double W = 0.0;
+    const size_t n = c->n;
+    if (n) {
+        hipStream_t s = c->s_main;
+        // the tree of the current positions in their own bounds
+        HIPCHK(launch_bounds(c, c->bounds_slot(c->bslot), s));
+        c->bounds_valid = true;
+        rc = build_on_device(c, s);
+        if (rc != BHX_OK) return rc;
+        rc = read_counters(c, s);
+        if (rc != BHX_OK) return rc;
+        double* tmp = nullptr;
+        if (dalloc(&tmp, 4 * n) != hipSuccess) return fail(c, BHX_E_NOMEM, "potential scratch");
+        std::vector<double> phi(n), m(n);
+        hipError_t e = launch_walk_potential(c, s, tmp, tmp + n, tmp + 2 * n, tmp + 3 * n);
+        if (e == hipSuccess) e = hipMemcpyAsync(phi.data(), tmp + 3 * n, n * 8, hipMemcpyDeviceToHost, s);
+        if (e == hipSuccess) e = hipMemcpyAsync(m.data(), c->m, n * 8, hipMemcpyDeviceToHost, s);
+        if (e == hipSuccess) e = hipStreamSynchronize(s);
+        (void)hipFree(tmp);
+        HIPCHK(e);
+        for (size_t i = 0; i < n; ++i) W += 0.5 * m[i] * phi[i];
+        if (pot_out) std::copy(phi.begin(), phi.end(), pot_out);
+    }
+    if (ekin) *ekin = K;
+    if (epot) *epot = W;
+    if (virial) *virial = W != 0.0 ? 2.0 * K / std::fabs(W) : 0.0;
     return BHX_OK;
 }
 

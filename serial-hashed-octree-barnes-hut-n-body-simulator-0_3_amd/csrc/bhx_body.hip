@@ -199,12 +199,14 @@ __global__ __launch_bounds__(kBlock) void k_treekeys(const double* __restrict__ 
                                                      const double* __restrict__ py,
                                                      const double* __restrict__ pz, size_t n,
                                                      const uint32_t* __restrict__ sysid,
-                                                     const double* __restrict__ bounds4,
+                                                     const double* __restrict__ bounds4, double tscale,
                                                      uint64_t* __restrict__ keys,
                                                      uint32_t* __restrict__ vals) {
     for (size_t i = blockIdx.x * (size_t)kBlock + threadIdx.x; i < n; i += (size_t)gridDim.x * kBlock) {
         const double* b4 = bounds4 + 4 * (size_t)(sysid ? sysid[i] : 0u);
-        const double c0x = b4[0], c0y = b4[1], c0z = b4[2], s0 = b4[3];
+        // root edge: the bounds size (reference geometry) or twice it (enclosing
+        // cube); the factor is a power of two, so both recurrences are exact
+        const double c0x = b4[0], c0y = b4[1], c0z = b4[2], s0 = b4[3] * tscale;
         const double x = px[i], y = py[i], z = pz[i];
         double cx = c0x, cy = c0y, cz = c0z, s = s0;
         uint64_t k = 1;
@@ -356,8 +358,9 @@ hipError_t launch_morton(const double* px, const double* py, const double* pz, s
 }
 
 hipError_t launch_treekeys(const double* px, const double* py, const double* pz, size_t n, const uint32_t* sysid,
-                           const double* b4, uint64_t* keys, uint32_t* vals, hipStream_t s) {
-    hipLaunchKernelGGL(k_treekeys, dim3(grid_for(n)), dim3(kBlock), 0, s, px, py, pz, n, sysid, b4, keys, vals);
+                           const double* b4, double tscale, uint64_t* keys, uint32_t* vals, hipStream_t s) {
+    hipLaunchKernelGGL(k_treekeys, dim3(grid_for(n)), dim3(kBlock), 0, s, px, py, pz, n, sysid, b4, tscale, keys,
+                       vals);
     return hipGetLastError();
 }
 

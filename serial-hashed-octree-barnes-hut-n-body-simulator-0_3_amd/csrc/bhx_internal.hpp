@@ -143,6 +143,7 @@ struct bhx_ctx {
     double* bounds = nullptr;
     int bslot = 0;
     int tree_slot = 0;                         // bounds slot the current tree was built with
+    double tree_tscale = 1.0;                  // root edge / bounds size of the current tree
     bool bounds_valid = false;
     // staging
     double* stage = nullptr;                   // 3 * cap doubles
@@ -188,6 +189,8 @@ struct bhx_ctx {
     uint32_t *sidx = nullptr, *sidx_alt = nullptr;
 
     bhx::SysView sys() const { return bhx::SysView{sys_off, nsys > 1 ? sysid : nullptr, nsys}; }
+    // root edge / bounds size of the configured cell geometry (exact: 1 or 2)
+    double cfg_tscale() const { return cfg.geometry ? 2.0 : 1.0; }
     double* bounds_slot(int k) const { return bounds + 4 * sys_cap * (size_t)k; }
 };
 
@@ -233,8 +236,9 @@ void free_grad(bhx_ctx* c);                           // recorded trajectory (bh
 hipError_t launch_bounds(bhx_ctx* c, double* bounds4, hipStream_t s);
 hipError_t launch_morton(const double* px, const double* py, const double* pz, size_t n, const uint32_t* sysid,
                          const double* bounds4, uint64_t* keys, uint32_t* vals, hipStream_t s);
+// tree path keys; the root edge is tscale * (bounds size)
 hipError_t launch_treekeys(const double* px, const double* py, const double* pz, size_t n, const uint32_t* sysid,
-                           const double* bounds4, uint64_t* keys, uint32_t* vals, hipStream_t s);
+                           const double* bounds4, double tscale, uint64_t* keys, uint32_t* vals, hipStream_t s);
 hipError_t launch_iota(uint32_t* vals, size_t n, hipStream_t s);
 
 size_t sort_tiles_for(size_t n);
@@ -277,6 +281,9 @@ struct WalkTree {
 // (array order) -> dL/dx of the own position (gself, 3 per body, array
 // order) and per-entry (dL/dX, dL/dM) accumulated into gnode (4 per node)
 // and gbody (4 per body, tree order), which must start at zero.
+// group walk of the current tree that also writes each body's potential
+// (array order); accelerations go to ox/oy/oz
+hipError_t launch_walk_potential(bhx_ctx* c, hipStream_t s, double* ox, double* oy, double* oz, double* pot);
 hipError_t launch_walk_adjoint(bhx_ctx* c, const WalkTree& t, int precision, const double* wx, const double* wy,
                                const double* wz, double* gself, double* gnode, double* gbody, hipStream_t s);
 hipError_t launch_diag(bhx_ctx* c, double* out7, hipStream_t s);

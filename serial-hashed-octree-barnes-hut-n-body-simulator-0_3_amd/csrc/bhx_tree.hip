@@ -214,13 +214,14 @@ struct TreeArgs {
     WalkNode32* walk32;
     float4* tb4;
     uint32_t* tleaf;
+    double tscale;   // root edge / bounds size (1: reference geometry, 2: enclosing cube)
 };
 
 // Child cell geometry — HOTNode(parent, octant, ...) HashedNode.cpp:50-56.
-__device__ __forceinline__ void descend_center(uint64_t key, int L, const double* b4, double& cx, double& cy,
-                                               double& cz, double& s) {
+__device__ __forceinline__ void descend_center(uint64_t key, int L, const double* b4, double tscale, double& cx,
+                                               double& cy, double& cz, double& s) {
 #pragma clang fp contract(off)
-    cx = b4[0]; cy = b4[1]; cz = b4[2]; s = b4[3];
+    cx = b4[0]; cy = b4[1]; cz = b4[2]; s = b4[3] * tscale;
     for (int l = 1; l <= L; ++l) {
         const unsigned o = (unsigned)(key >> (3 * (L - l))) & 7u;
         s = s * 0.5;
@@ -286,7 +287,7 @@ __global__ __launch_bounds__(kBlock) void k_emit(const uint64_t* __restrict__ ke
             t.level[id] = (uint32_t)L;
             atomicOr(&t.child_byte[par], 1u << (unsigned)(k & 7));
             double cx, cy, cz, s;
-            descend_center(k, L, b4, cx, cy, cz, s);
+            descend_center(k, L, b4, t.tscale, cx, cy, cz, s);
             t.center[3 * (size_t)id] = cx;
             t.center[3 * (size_t)id + 1] = cy;
             t.center[3 * (size_t)id + 2] = cz;
@@ -458,7 +459,7 @@ __device__ void moments_of(uint32_t p, int intended, const TreeArgs& t, const do
             t.quad[6 * (size_t)p + k] = q[k];
             t.quadf[6 * (size_t)p + k] = (float)q[k];
         }
-        double s = bounds4[4 * (size_t)sys_of(sv, st) + 3];
+        double s = bounds4[4 * (size_t)sys_of(sv, st) + 3] * t.tscale;
         for (int l = 0; l < L; ++l) s = s * 0.5;
         WalkNode w;
         w.cx = cx; w.cy = cy; w.cz = cz;
@@ -561,6 +562,7 @@ TreeArgs tree_args(bhx_ctx* c) {
     t.walk32 = d.walk32;
     t.tb4 = c->tb4;
     t.tleaf = c->tleaf;
+    t.tscale = c->cfg_tscale();
     return t;
 }
 
@@ -592,6 +594,7 @@ hipError_t build_tree(bhx_ctx* c, hipStream_t s, size_t* total_out, bool sync) {
     const SysView sv = c->sys();
     const uint32_t cap = (uint32_t)c->tree.cap;
     c->tree_slot = c->bslot;
+    c->tree_tscale = c->cfg_tscale();
     uint32_t* total_dev = &c->ctr->total_nodes;
     hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, s, &c->ctr->unsplit, (size_t)2, 0u);  // unsplit, hash_fail
     hipLaunchKernelGGL(k_count, dim3(grid_for(n)), dim3(kBlock), 0, s, c->tkeys, n, B, sv, c->nn, c->nlo);

@@ -65,7 +65,9 @@ struct WalkArgs {
     uint32_t quad_on;
     double theta2, eps2;
     float theta2f, eps2f;
+    double tscale;           // root edge / bounds size of the walked tree
     double *ax, *ay, *az;
+    double* pot;             // potential per body (array order), potential walks only
     uint32_t* counts;
     // adjoint walk: weights w = dL/da per body (array order) in, dL/dx of each
     // body's own position out (array order, 3 per body), and per list entry
@@ -360,9 +362,12 @@ __device__ __forceinline__ float dist2(float dx, float dy, float dz) { return dx
 
 // kGuard: without softening a zero separation (the body itself) must add
 // nothing; with softening d = 0 already contributes 0.
-template <typename T, bool kQuad, bool kGuard>
+// kPot: also the softened potential phi -= M/r + (d.Q.d) / (2 r^5) (a = -grad
+// phi for the force above); an entry at zero separation (the body itself)
+// adds no potential.
+template <typename T, bool kQuad, bool kGuard, bool kPot = false>
 __device__ __forceinline__ void eval_nodes(const T (*nd)[4], const T (*q)[6], int nn, T xr, T yr, T zr, T eps2,
-                                           T& ax, T& ay, T& az) {
+                                           T& ax, T& ay, T& az, T& phi) {
     for (int k = 0; k < nn; ++k) {
         const T dx = nd[k][0] - xr, dy = nd[k][1] - yr, dz = nd[k][2] - zr;
         const T d2 = dist2(dx, dy, dz);
@@ -375,23 +380,27 @@ __device__ __forceinline__ void eval_nodes(const T (*nd)[4], const T (*q)[6], in
         ax += f * dx;
         ay += f * dy;
         az += f * dz;
+        T pt = kPot ? nd[k][3] * inv : (T)0;
         if (kQuad) {
             const T q0 = q[k][0], q1 = q[k][1], q2 = q[k][2], q3 = q[k][3], q4 = q[k][4], q5 = q[k][5];
             const T qx = q0 * dx + q1 * dy + q2 * dz;
             const T qy = q1 * dx + q3 * dy + q4 * dz;
             const T qz = q2 * dx + q4 * dy + q5 * dz;
             const T inv5 = inv3 * inv2;
-            const T rq = (T)2.5 * (dx * qx + dy * qy + dz * qz) * (inv5 * inv2);
+            const T dqd = dx * qx + dy * qy + dz * qz;
+            const T rq = (T)2.5 * dqd * (inv5 * inv2);
             ax += rq * dx - qx * inv5;
             ay += rq * dy - qy * inv5;
             az += rq * dz - qz * inv5;
+            if (kPot) pt += (T)0.5 * dqd * inv5;
         }
+        if (kPot) phi -= d2 > (T)0 ? pt : (T)0;
     }
 }
 
-template <typename T, bool kGuard>
+template <typename T, bool kGuard, bool kPot = false>
 __device__ __forceinline__ void eval_bodies(const T (*bd)[4], int nb, T xr, T yr, T zr, T eps2, T& ax, T& ay,
-                                            T& az) {
+                                            T& az, T& phi) {
     for (int k = 0; k < nb; ++k) {
         const T dx = bd[k][0] - xr, dy = bd[k][1] - yr, dz = bd[k][2] - zr;
         const T d2 = dist2(dx, dy, dz);
@@ -403,6 +412,7 @@ __device__ __forceinline__ void eval_bodies(const T (*bd)[4], int nb, T xr, T yr
         ax += f * dx;
         ay += f * dy;
         az += f * dz;
+        if (kPot) phi -= d2 > (T)0 ? bd[k][3] * inv : (T)0;
     }
 }
 
@@ -493,9 +503,10 @@ __device__ __forceinline__ void gather_bodies(T (*bd)[4], const uint32_t* rstart
     }
 }
 
-template <typename T, bool kQuad, bool kGuard, bool kAdj = false>
+template <typename T, bool kQuad, bool kGuard, bool kAdj = false, bool kPot = false>
 __global__ __launch_bounds__(64) void k_walk_group(WalkArgs a) {
     static_assert(!(kAdj && kQuad), "the adjoint walk is monopole only");
+    static_assert(!(kAdj && kPot), "the adjoint walk has no potential");
     __shared__ T s_nd[kNodeCap][4];
     __shared__ T s_q[kQuad ? kNodeCap : 1][6];
     __shared__ T s_bd[kBodyCap][4];
@@ -520,7 +531,7 @@ __global__ __launch_bounds__(64) void k_walk_group(WalkArgs a) {
     const T eps2 = (T)a.eps2;
     const double th2d = a.theta2;
     const float th2f = a.theta2f;
-    T ax = 0, ay = 0, az = 0;
+    T ax = 0, ay = 0, az = 0, phi = 0;
     T wxr = 0, wyr = 0, wzr = 0;
     if constexpr (kAdj) {
         if (valid) {
@@ -557,7 +568,7 @@ __global__ __launch_bounds__(64) void k_walk_group(WalkArgs a) {
                     adj_self<T>(s_nd, nn, xr, yr, zr, wxr, wyr, wzr, eps2, ax, ay, az);
                     adj_scatter<T>(s_nd, s_nid, nn, s_tg, eps2, lane, a.gnode);
                 } else {
-                    eval_nodes<T, kQuad, kGuard>(s_nd, s_q, nn, xr, yr, zr, eps2, ax, ay, az);
+                    eval_nodes<T, kQuad, kGuard, kPot>(s_nd, s_q, nn, xr, yr, zr, eps2, ax, ay, az, phi);
                 }
                 cnt += (uint32_t)nn;
                 nn = 0;
@@ -588,7 +599,7 @@ __global__ __launch_bounds__(64) void k_walk_group(WalkArgs a) {
                         adj_self<T>(s_bd, nq, xr, yr, zr, wxr, wyr, wzr, eps2, ax, ay, az);
                         adj_scatter<T>(s_bd, s_bid, nq, s_tg, eps2, lane, a.gbody);
                     } else {
-                        eval_bodies<T, kGuard>(s_bd, nq, xr, yr, zr, eps2, ax, ay, az);
+                        eval_bodies<T, kGuard, kPot>(s_bd, nq, xr, yr, zr, eps2, ax, ay, az, phi);
                     }
                     cnt += (uint32_t)nq;
                     nr = 0;
@@ -612,7 +623,7 @@ __global__ __launch_bounds__(64) void k_walk_group(WalkArgs a) {
         adj_self<T>(s_nd, nn, xr, yr, zr, wxr, wyr, wzr, eps2, ax, ay, az);
         adj_scatter<T>(s_nd, s_nid, nn, s_tg, eps2, lane, a.gnode);
     } else {
-        eval_nodes<T, kQuad, kGuard>(s_nd, s_q, nn, xr, yr, zr, eps2, ax, ay, az);
+        eval_nodes<T, kQuad, kGuard, kPot>(s_nd, s_q, nn, xr, yr, zr, eps2, ax, ay, az, phi);
     }
     gather_bodies<T>(s_bd, s_rstart, s_roff, nr, nq, lane, a, ox, oy, oz, kAdj ? s_bid : nullptr);
     __syncthreads();
@@ -620,7 +631,7 @@ __global__ __launch_bounds__(64) void k_walk_group(WalkArgs a) {
         adj_self<T>(s_bd, nq, xr, yr, zr, wxr, wyr, wzr, eps2, ax, ay, az);
         adj_scatter<T>(s_bd, s_bid, nq, s_tg, eps2, lane, a.gbody);
     } else {
-        eval_bodies<T, kGuard>(s_bd, nq, xr, yr, zr, eps2, ax, ay, az);
+        eval_bodies<T, kGuard, kPot>(s_bd, nq, xr, yr, zr, eps2, ax, ay, az, phi);
     }
     cnt += (uint32_t)(nn + nq);
     if (valid) {
@@ -633,6 +644,7 @@ __global__ __launch_bounds__(64) void k_walk_group(WalkArgs a) {
             a.ax[o] = (double)ax;
             a.ay[o] = (double)ay;
             a.az[o] = (double)az;
+            if constexpr (kPot) a.pot[o] = (double)phi;
         }
         if (a.counts) a.counts[o] = cnt;
     }
@@ -691,9 +703,9 @@ __device__ __forceinline__ Node32 load_node32(ConstU64 raw, uint32_t p) {
 //   a += d (M r^-3 + 2.5 (d.Q.d) r^-7) - (Q.d) r^-5
 // (32 vector instructions per cell, 13 per body).  Bodies are entries with
 // Q = 0 and go through the kQuad = false form.
-template <bool kQuad, bool kGuard>
+template <bool kQuad, bool kGuard, bool kPot = false>
 __device__ __forceinline__ void eval_list32(const float (*nd)[4], const float (*q)[6], int nn, float xr, float yr,
-                                            float zr, float eps2, float& ax, float& ay, float& az) {
+                                            float zr, float eps2, float& ax, float& ay, float& az, float& phi) {
     for (int k = 0; k < nn; ++k) {
         const float dx = nd[k][0] - xr, dy = nd[k][1] - yr, dz = nd[k][2] - zr;
         float inv;
@@ -706,6 +718,7 @@ __device__ __forceinline__ void eval_list32(const float (*nd)[4], const float (*
         const float inv2 = inv * inv;
         const float inv3 = inv * inv2;
         float c = nd[k][3] * inv3;
+        float pt = kPot ? nd[k][3] * inv : 0.f;
         if (kQuad) {
             const float q0 = q[k][0], q1 = q[k][1], q2 = q[k][2], q3 = q[k][3], q4 = q[k][4], q5 = q[k][5];
             const float qx = fmaf(q2, dz, fmaf(q1, dy, q0 * dx));
@@ -717,10 +730,12 @@ __device__ __forceinline__ void eval_list32(const float (*nd)[4], const float (*
             ax = fmaf(-inv5, qx, ax);
             ay = fmaf(-inv5, qy, ay);
             az = fmaf(-inv5, qz, az);
+            if (kPot) pt = fmaf(0.5f * dqd, inv5, pt);
         }
         ax = fmaf(c, dx, ax);
         ay = fmaf(c, dy, ay);
         az = fmaf(c, dz, az);
+        if (kPot) phi -= (dx * dx + dy * dy + dz * dz) > 0.f ? pt : 0.f;
     }
 }
 
@@ -754,9 +769,10 @@ __device__ __forceinline__ void gather_cells32(float (*nd)[4], float (*q)[6], co
     }
 }
 
-template <bool kQuad, bool kGuard, bool kAdj = false>
+template <bool kQuad, bool kGuard, bool kAdj = false, bool kPot = false>
 __global__ __launch_bounds__(64) void k_walk_group32(WalkArgs a) {
     static_assert(!(kAdj && kQuad), "the adjoint walk is monopole only");
+    static_assert(!(kAdj && kPot), "the adjoint walk has no potential");
     __shared__ float s_nd[kCells32][4];
     __shared__ float s_q[kQuad ? kCells32 : 1][6];
     __shared__ uint32_t s_nid[kCells32];
@@ -795,9 +811,9 @@ __global__ __launch_bounds__(64) void k_walk_group32(WalkArgs a) {
     const float eps2 = a.eps2f;
     // opening test edge^2 < theta^2 d^2 as (edge^2 / theta^2) < d^2, with
     // edge^2 / theta^2 = (root value) * 4^-L (theta = 0: infinite, never accepts)
-    const double s0 = a.b4[4 * (size_t)gr.z + 3];
+    const double s0 = a.b4[4 * (size_t)gr.z + 3] * a.tscale;
     const float s2th = (float)(s0 * s0) / a.theta2f;
-    float ax = 0.f, ay = 0.f, az = 0.f;
+    float ax = 0.f, ay = 0.f, az = 0.f, phi = 0.f;
     float wxr = 0.f, wyr = 0.f, wzr = 0.f;
     if constexpr (kAdj) {
         if (valid) {
@@ -817,7 +833,7 @@ __global__ __launch_bounds__(64) void k_walk_group32(WalkArgs a) {
             adj_self<float>(s_nd, nn, xr, yr, zr, wxr, wyr, wzr, eps2, ax, ay, az);
             adj_scatter<float>(s_nd, s_nid, nn, s_tg, eps2, lane, a.gnode);
         } else {
-            eval_list32<kQuad, kGuard>(s_nd, s_q, nn, xr, yr, zr, eps2, ax, ay, az);
+            eval_list32<kQuad, kGuard, kPot>(s_nd, s_q, nn, xr, yr, zr, eps2, ax, ay, az, phi);
         }
         __syncthreads();
     };
@@ -846,7 +862,7 @@ __global__ __launch_bounds__(64) void k_walk_group32(WalkArgs a) {
             adj_self<float>(s_bd, nq, xr, yr, zr, wxr, wyr, wzr, eps2, ax, ay, az);
             adj_scatter<float>(s_bd, s_bid, nq, s_tg, eps2, lane, a.gbody);
         } else {
-            eval_list32<false, kGuard>(s_bd, s_q, nq, xr, yr, zr, eps2, ax, ay, az);
+            eval_list32<false, kGuard, kPot>(s_bd, s_q, nq, xr, yr, zr, eps2, ax, ay, az, phi);
         }
         __syncthreads();
     };
@@ -947,6 +963,7 @@ __global__ __launch_bounds__(64) void k_walk_group32(WalkArgs a) {
             a.ax[o] = (double)ax;
             a.ay[o] = (double)ay;
             a.az[o] = (double)az;
+            if constexpr (kPot) a.pot[o] = (double)phi;
         }
         if (a.counts) a.counts[o] = cnt;
     }
@@ -1068,6 +1085,7 @@ hipError_t launch_walk_to(bhx_ctx* c, hipStream_t s, bool group, int precision, 
     a.eps2 = c->cfg.eps * c->cfg.eps;
     a.theta2f = (float)a.theta2;
     a.eps2f = (float)a.eps2;
+    a.tscale = c->tree_tscale;
     a.ax = ox; a.ay = oy; a.az = oz;
     a.counts = counts;
     const unsigned groups = c->ngroups;
@@ -1096,6 +1114,48 @@ hipError_t launch_walk_to(bhx_ctx* c, hipStream_t s, bool group, int precision, 
     return hipGetLastError();
 }
 
+hipError_t launch_walk_potential(bhx_ctx* c, hipStream_t s, double* ox, double* oy, double* oz, double* pot) {
+    if (c->n == 0) return hipSuccess;
+    WalkArgs a{};
+    a.nodes = c->tree.walk;
+    a.nodes32 = c->tree.walk32;
+    a.tb4 = c->tb4;
+    a.tleaf = c->tleaf;
+    a.groups = c->ngroups;
+    a.gtab = c->gtab;
+    a.sys_root = c->sys_root;
+    a.abort_flag = &c->ctr->node_overflow;
+    a.b4 = c->bounds_slot(c->tree_slot);
+    a.quad = c->tree.quad;
+    a.quadf = c->tree.quadf;
+    a.tx = c->tx; a.ty = c->ty; a.tz = c->tz; a.tm = c->tm;
+    a.tperm = c->tperm;
+    a.n = (uint32_t)c->n;
+    a.quad_on = (uint32_t)c->cfg.quadrupole;
+    a.theta2 = c->cfg.theta * c->cfg.theta;
+    a.eps2 = c->cfg.eps * c->cfg.eps;
+    a.theta2f = (float)a.theta2;
+    a.eps2f = (float)a.eps2;
+    a.tscale = c->tree_tscale;
+    a.ax = ox; a.ay = oy; a.az = oz;
+    a.pot = pot;
+    const unsigned groups = c->ngroups;
+    const bool quad = c->cfg.quadrupole != 0, guard = !(a.eps2 > 0.0);
+    if (c->cfg.precision == BHX_FP32) {
+#define BHX_POT32(Q, G) \
+    hipLaunchKernelGGL((k_walk_group32<Q, G, false, true>), dim3(8 * ((groups + 7) / 8)), dim3(64), 0, s, a)
+        if (quad) { if (guard) BHX_POT32(true, true); else BHX_POT32(true, false); }
+        else { if (guard) BHX_POT32(false, true); else BHX_POT32(false, false); }
+#undef BHX_POT32
+    } else {
+#define BHX_POT64(Q, G) hipLaunchKernelGGL((k_walk_group<double, Q, G, false, true>), dim3(groups), dim3(64), 0, s, a)
+        if (quad) { if (guard) BHX_POT64(true, true); else BHX_POT64(true, false); }
+        else { if (guard) BHX_POT64(false, true); else BHX_POT64(false, false); }
+#undef BHX_POT64
+    }
+    return hipGetLastError();
+}
+
 hipError_t launch_walk_adjoint(bhx_ctx* c, const WalkTree& t, int precision, const double* wx, const double* wy,
                                const double* wz, double* gself, double* gnode, double* gbody, hipStream_t s) {
     if (c->n == 0 || c->ngroups == 0) return hipSuccess;
@@ -1116,6 +1176,7 @@ hipError_t launch_walk_adjoint(bhx_ctx* c, const WalkTree& t, int precision, con
     a.eps2 = c->cfg.eps * c->cfg.eps;
     a.theta2f = (float)a.theta2;
     a.eps2f = (float)a.eps2;
+    a.tscale = c->tree_tscale;
     a.wx = wx; a.wy = wy; a.wz = wz;
     a.gself = gself;
     a.gnode = gnode;

@@ -118,6 +118,40 @@ def test_build_bucket_node_set_bitexact(bhx, plummer16k, cap):
     _compare_nodes(g, o, ["key", "center", "size", "n", "level", "child_byte", "first", "com", "mass"])
 
 
+@pytest.mark.parametrize("cap", [1, 32])
+def test_build_enclosing_geometry(bhx, plummer16k, cap):
+    """geometry = 1 is the reference recurrence from a root edge of twice the bounds
+    size: the node set equals the oracle's tree built with size 2S, and every body
+    lies inside its leaf's cube (centre +- edge/2)."""
+    pos, vel, mass = plummer16k
+    c, s = orc.bounds(pos)
+    with _engine(bhx, N1, leaf_capacity=cap, geometry=1) as e:
+        e.set_bodies(pos, vel, mass)
+        e.build(c, s)
+        g = e.nodes()
+        tperm, _ = e.tree_order()
+    t = orc.build_insert(pos, mass, c, 2 * s) if cap == 1 else orc.build_recursive(pos, mass, c, 2 * s, cap)
+    _compare_nodes(g, t.nodes(), ["key", "center", "size", "n", "level", "child_byte", "first", "com", "mass"])
+    assert g["size"][0] == 2 * s
+    leaves = g[g["child_byte"] == 0]
+    tp = tperm.astype(np.int64)
+    for lf in leaves[:: max(1, leaves.size // 2000)]:
+        b = pos[tp[lf["body_start"]:lf["body_start"] + lf["n"]]]
+        # (centres are rounded FP64 sums: allow their last bits)
+        assert np.all(np.abs(b - lf["center"]) <= 0.5 * lf["size"] * (1 + 1e-12))
+    # the reference geometry does not enclose its outer bodies (its root edge is S)
+    with _engine(bhx, N1, leaf_capacity=cap) as e:
+        e.set_bodies(pos, vel, mass)
+        e.build(c, s)
+        g0 = e.nodes()
+        tp0 = e.tree_order()[0].astype(np.int64)
+    outside = 0
+    for lf in g0[g0["child_byte"] == 0]:
+        b = pos[tp0[lf["body_start"]:lf["body_start"] + lf["n"]]]
+        outside += int(np.any(np.abs(b - lf["center"]) > 0.5 * lf["size"]))
+    assert outside > 0
+
+
 def test_preorder_skip_structure(bhx, plummer16k):
     pos, vel, mass = plummer16k
     c, s = orc.bounds(pos)
