@@ -107,6 +107,33 @@ def energy_drift(bhx, n=16384, steps=100, seed=1):
     return out
 
 
+def galaxy_bench(bhx, n=4194304, steps=20, warmup=3, seed=2026):
+    """Two-galaxy workload: a 3:1 pair of exponential-disk + Hernquist-halo galaxies on a
+    parabolic orbit (separation 30, pericentre 5), FP32 group walk with quadrupoles,
+    theta 0.6, eps 0.02, bucket 32, dt 1e-3, enclosing-cube cells (the secondary lies
+    beyond half the bounds size).  Energy from the tree potential before and after."""
+    log("galaxy pair N=%d: generating" % n)
+    pos, vel, mass = bhx.galaxy_pair(n, seed=seed, threads=16)
+    cfg = dict(theta=0.6, eps=0.02, dt=1e-3, leaf_capacity=32, mode=bhx.INTENDED, precision=bhx.FP32,
+               quadrupole=1, compensated=1, permute_acc=1, group_walk=1, geometry=1)
+    res = {"n": n, "steps": steps, "warmup": warmup, "theta": 0.6, "eps": 0.02, "dt": 1e-3, "leaf_capacity": 32,
+           "geometry": "enclosing cube", "data": "synthetic: bhx_ic_galaxy_pair seed %d" % seed}
+    with bhx.Engine(n, **cfg) as e:
+        e.set_bodies(pos, vel, mass)
+        k0, w0, vir0 = e.energy()
+        t = timed_steps(e, steps, warmup)
+        k1, w1, vir1 = e.energy()
+        err = force_error(e, bhx, 1024, 7, 0.02)
+        _, phases = walk_kernel_ms(e, 2)
+    ms = 1e3 * t / steps
+    res.update({"ms_per_step": ms, "body_updates_per_s": n / (ms * 1e-3), "force_error": err,
+                "phases_ms": {k: round(v, 4) for k, v in phases.items()},
+                "energy": {"e0": k0 + w0, "e1": k1 + w1, "rel_change": (k1 + w1 - k0 - w0) / abs(k0 + w0),
+                           "virial0": vir0, "frames_between": steps + warmup}})
+    log("galaxy pair %.3f ms/step, median force error %.2e" % (ms, err["median"]))
+    return res
+
+
 def ensemble_bench(bhx, nsys, steps, warmup, seed=1):
     """Batched ensemble (all systems in one context, one graph launch per frame) against the
     same systems stepped one after another (one system at a time in one context, eager frames).
@@ -311,6 +338,7 @@ def main():
     ap.add_argument("--ensemble-systems", type=int, default=512)
     ap.add_argument("--no-drift", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--no-galaxy", action="store_true")
     ap.add_argument("--calibrate-copy-mib", type=int, default=0,
                     help="first copy this many MiB device-to-device (counter calibration runs)")
     args = ap.parse_args()
@@ -390,6 +418,8 @@ def main():
         line["fp32_over_fp64_walk"] = f64["walk_ms"] / walk_ms
     if not args.no_ensemble:
         line["ensemble"] = ensemble_bench(bhx, args.ensemble_systems, 10, 2)
+    if not args.no_galaxy:
+        line["galaxy_pair"] = galaxy_bench(bhx)
     if not args.no_adjoint:
         line["adjoint"] = adjoint_bench(bhx)
         line["fwd_adjoint_over_fwd"] = line["adjoint"]["fwd_adjoint_over_fwd"]

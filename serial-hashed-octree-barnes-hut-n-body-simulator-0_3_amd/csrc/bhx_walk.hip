@@ -740,10 +740,13 @@ __device__ __forceinline__ void eval_list32(const float (*nd)[4], const float (*
 }
 
 // List capacities of the FP32 walk, sized so that one workgroup (one wave)
-// uses under 5 KiB of LDS and eight waves fit on every SIMD.
+// uses under 5 KiB of LDS and eight waves fit on every SIMD.  The adjoint
+// walk scatters with one lane per list entry and keeps 128 bodies, so that
+// its body flushes fill both halves of the scatter.
 constexpr int kCells32 = 64;
-constexpr int kBodies32 = 96;
 constexpr int kRanges32 = 32;
+template <bool kAdj>
+constexpr int bodies32() { return kAdj ? 128 : 96; }
 
 // Accepted cells of the FP32 walk are queued as node ids only; a flush
 // gathers them lane-parallel (lane j fetches entry j's record) into the
@@ -773,6 +776,7 @@ template <bool kQuad, bool kGuard, bool kAdj = false, bool kPot = false>
 __global__ __launch_bounds__(64) void k_walk_group32(WalkArgs a) {
     static_assert(!(kAdj && kQuad), "the adjoint walk is monopole only");
     static_assert(!(kAdj && kPot), "the adjoint walk has no potential");
+    constexpr int kBodies32 = bodies32<kAdj>();
     __shared__ float s_nd[kCells32][4];
     __shared__ float s_q[kQuad ? kCells32 : 1][6];
     __shared__ uint32_t s_nid[kCells32];
@@ -867,11 +871,12 @@ __global__ __launch_bounds__(64) void k_walk_group32(WalkArgs a) {
         __syncthreads();
     };
     // The traversal fills the lists without touching the accumulators; it
-    // stops when a list needs evaluating (cells full, bodies or ranges unable
-    // to take another leaf chunk of up to 64) and the outer loop evaluates it.
-    // A leaf of more than 64 bodies is queued one chunk per visit (kres = the
-    // next chunk).  Only the root has level 0, and the root is never accepted:
-    // a leaf root queues its bodies before the loop.
+    // stops when a list needs evaluating — the cell list is full, or the next
+    // leaf chunk (up to 64 bodies) does not fit the body or range list; that
+    // leaf is visited again after the flush — and the outer loop evaluates
+    // it.  A leaf of more than 64 bodies is queued one chunk per visit (kres =
+    // the next chunk).  Only the root has level 0, and the root is never
+    // accepted: a leaf root queues its bodies before the loop.
     uint32_t cnt = 0;
     int nn = 0, nr = 0, nq = 0;
     const Node32 root = load_node32(raw, rid);
@@ -882,7 +887,8 @@ __global__ __launch_bounds__(64) void k_walk_group32(WalkArgs a) {
         const uint32_t count = root.meta >> 6;
         const float cx = (float)(root.cx - ox), cy = (float)(root.cy - oy), cz = (float)(root.cz - oz);
         for (uint32_t k = 0; k < count; k += 64) {
-            if (nq > kBodies32 - 64 || nr == kRanges32) {
+            const int m = (int)((count - k) < 64u ? (count - k) : 64u);
+            if (nq + m > kBodies32 || nr == kRanges32) {
                 flush_bodies(nr, nq);
                 cnt += (uint32_t)nq;
                 nr = 0;
@@ -896,11 +902,12 @@ __global__ __launch_bounds__(64) void k_walk_group32(WalkArgs a) {
                 s_rdel[nr][2] = cz;
             }
             ++nr;
-            nq += (int)((count - k) < 64u ? (count - k) : 64u);
+            nq += m;
         }
         p = end;
     }
     while (p < end) {
+        bool bodies_full = false;
         for (;;) {
             p = __builtin_amdgcn_readfirstlane(p);
             const Node32 nd = load_node32(raw, p);
@@ -918,21 +925,25 @@ __global__ __launch_bounds__(64) void k_walk_group32(WalkArgs a) {
                 if (nn == kCells32) break;
             } else if (nd.meta & 32u) {
                 const uint32_t count = nd.meta >> 6;
+                const int m = (int)((count - kres) < 64u ? (count - kres) : 64u);
+                if (nq + m > kBodies32 || nr == kRanges32) {
+                    bodies_full = true;
+                    break;
+                }
                 if (lane == 0) {
                     s_rstart[nr] = nd.start + kres;
                     s_roff[nr] = (uint32_t)nq;
-                    s_rdel[nr][0] = cx;
+                    s_rdel[nr][0] = cx;     // the leaf's centre relative to the group origin
                     s_rdel[nr][1] = cy;
                     s_rdel[nr][2] = cz;
                 }
                 ++nr;
-                nq += (int)((count - kres) < 64u ? (count - kres) : 64u);
+                nq += m;
                 kres += 64u;
                 if (kres >= count) {
                     kres = 0;
                     p = nskip;
                 }
-                if (nq > kBodies32 - 64 || nr == kRanges32) break;
             } else {
                 p = p + 1;
             }
@@ -943,7 +954,7 @@ __global__ __launch_bounds__(64) void k_walk_group32(WalkArgs a) {
             cnt += (uint32_t)nn;
             nn = 0;
         }
-        if (nq > kBodies32 - 64 || nr == kRanges32) {
+        if (bodies_full) {
             flush_bodies(nr, nq);
             cnt += (uint32_t)nq;
             nr = 0;
