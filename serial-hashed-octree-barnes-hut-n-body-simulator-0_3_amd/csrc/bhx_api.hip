@@ -64,6 +64,7 @@ void drop_graphs(bhx_ctx* c) {
         if (g.exec) (void)hipGraphExecDestroy(g.exec);
         g = bhx_ctx::StepGraph{};
     }
+    c->graph_pair_next = 0;
     c->graph_valid = false;
 }
 
@@ -132,6 +133,7 @@ void free_all(bhx_ctx* c) {
     if (c->ev_bounds) (void)hipEventDestroy(c->ev_bounds);
     if (c->s_main) (void)hipStreamDestroy(c->s_main);
     if (c->s_side) (void)hipStreamDestroy(c->s_side);
+    if (c->s_adj) (void)hipStreamDestroy(c->s_adj);
 }
 
 int check_cfg(const bhx_config* cfg) {
@@ -278,7 +280,7 @@ __global__ void k_fill_u32_api(uint32_t* __restrict__ p, size_t n, uint32_t v) {
 // the arrays it has (an overflow is flagged on the device, the walk then does
 // nothing), and the side branch computing the next bounds is joined at the
 // end so one frame is one closed unit of work.
-int enqueue_step_async(bhx_ctx* c, hipStream_t s) {
+int enqueue_step_async(bhx_ctx* c, hipStream_t s, bool linear) {
     const int cur = c->bslot;
     double* b_cur = c->bounds_slot(cur);
     double* b_next = c->bounds_slot(cur ^ 1);
@@ -287,10 +289,14 @@ int enqueue_step_async(bhx_ctx* c, hipStream_t s) {
     HIPCHK(launch_morton(c->px, c->py, c->pz, c->n, sv.id, b_cur, c->keys, c->vals, s));
     HIPCHK(sort_pairs(c, c->keys, c->vals, s));
     HIPCHK(launch_permute_kick_drift(c, c->vals, dt, s));
-    HIPCHK(hipEventRecord(c->ev_drift, s));
-    HIPCHK(hipStreamWaitEvent(c->s_side, c->ev_drift, 0));
-    HIPCHK(launch_bounds(c, b_next, c->s_side));
-    HIPCHK(hipEventRecord(c->ev_bounds, c->s_side));
+    if (linear) {
+        HIPCHK(launch_bounds(c, b_next, s));
+    } else {
+        HIPCHK(hipEventRecord(c->ev_drift, s));
+        HIPCHK(hipStreamWaitEvent(c->s_side, c->ev_drift, 0));
+        HIPCHK(launch_bounds(c, b_next, c->s_side));
+        HIPCHK(hipEventRecord(c->ev_bounds, c->s_side));
+    }
     HIPCHK(launch_treekeys(c->px, c->py, c->pz, c->n, sv.id, b_cur, c->cfg_tscale(), c->tkeys, c->tperm, s));
     HIPCHK(sort_pairs(c, c->tkeys, c->tperm, s));
     HIPCHK(launch_tree_gather(c, s));
@@ -298,7 +304,7 @@ int enqueue_step_async(bhx_ctx* c, hipStream_t s) {
     HIPCHK(build_tree(c, s, &total, false));
     HIPCHK(launch_walk(c, s));
     HIPCHK(launch_kick(c, dt, s));
-    HIPCHK(hipStreamWaitEvent(s, c->ev_bounds, 0));
+    if (!linear) HIPCHK(hipStreamWaitEvent(s, c->ev_bounds, 0));
     c->bslot = cur ^ 1;
     return BHX_OK;
 }
@@ -313,11 +319,12 @@ void buffer_set(const bhx_ctx* c, const double** out) {
 }
 
 // Captures one frame starting from the current buffer parity into `g`.
-int capture_step(bhx_ctx* c, bhx_ctx::StepGraph& g) {
+int capture_step(bhx_ctx* c, bhx_ctx::StepGraph& g, bool linear) {
     buffer_set(c, g.bufs);
     g.bslot = c->bslot;
+    g.linear = linear;
     HIPCHK(hipStreamBeginCapture(c->s_main, hipStreamCaptureModeRelaxed));
-    const int rc = enqueue_step_async(c, c->s_main);
+    const int rc = enqueue_step_async(c, c->s_main, linear);
     hipGraph_t graph = nullptr;
     const hipError_t e = hipStreamEndCapture(c->s_main, &graph);
     if (rc != BHX_OK || e != hipSuccess) {
@@ -387,9 +394,9 @@ int restore_snapshot(bhx_ctx* c, const bhx_ctx::Snapshot& sn, hipStream_t s) {
 
 // One frame from its captured graph; the host mirrors the buffer exchange
 // and bounds-slot flip the frame performs on the device.
-int graph_frame(bhx_ctx* c, hipStream_t s) {
+int graph_frame(bhx_ctx* c, hipStream_t s, bool linear) {
     hipGraphExec_t g = nullptr;
-    const int rc = step_graph_for(c, &g);
+    const int rc = step_graph_for(c, &g, linear);
     if (rc != BHX_OK) return rc;
     HIPCHK(hipGraphLaunch(g, s));
     swap_body_buffers(c);
@@ -400,23 +407,34 @@ int graph_frame(bhx_ctx* c, hipStream_t s) {
 
 // Graph of the frame that starts from the current parity (capturing both
 // parities when none matches).
-int step_graph_for(bhx_ctx* c, hipGraphExec_t* out) {
+int step_graph_for(bhx_ctx* c, hipGraphExec_t* out, bool linear) {
     const double* cur[32];
     buffer_set(c, cur);
-    for (auto& g : c->graphs)
-        if (g.exec && g.bslot == c->bslot && std::equal(cur, cur + 32, g.bufs)) {
+    for (int k = 0; k < 4; ++k) {
+        const auto& g = c->graphs[k];
+        if (g.exec && g.linear == linear && g.bslot == c->bslot && std::equal(cur, cur + 32, g.bufs)) {
+            c->graph_pair_next = (k >> 1) ^ 1;   // keep this pair, replace the other
             *out = g.exec;
             return BHX_OK;
         }
-    drop_graphs(c);
-    int rc = capture_step(c, c->graphs[0]);           // leaves the host state one frame on
-    if (rc == BHX_OK) rc = capture_step(c, c->graphs[1]);   // and back
+    }
+    const int pair = c->graph_pair_next;
+    auto drop_pair = [&]() {
+        for (int k = 2 * pair; k < 2 * pair + 2; ++k) {
+            if (c->graphs[k].exec) (void)hipGraphExecDestroy(c->graphs[k].exec);
+            c->graphs[k] = bhx_ctx::StepGraph{};
+        }
+    };
+    drop_pair();
+    int rc = capture_step(c, c->graphs[2 * pair], linear);               // leaves the host state one frame on
+    if (rc == BHX_OK) rc = capture_step(c, c->graphs[2 * pair + 1], linear);   // and back
     if (rc != BHX_OK) {
-        drop_graphs(c);
+        drop_pair();
         return rc;
     }
+    c->graph_pair_next = pair ^ 1;
     c->graph_valid = true;
-    *out = c->graphs[0].exec;
+    *out = c->graphs[2 * pair].exec;
     return BHX_OK;
 }
 
@@ -458,6 +476,7 @@ int bhx_create(const bhx_config* cfg, size_t capacity, bhx_ctx** out) {
     if (rc == BHX_OK) {
         chk(hipStreamCreateWithFlags(&c->s_main, hipStreamNonBlocking), "stream");
         chk(hipStreamCreateWithFlags(&c->s_side, hipStreamNonBlocking), "stream");
+        chk(hipStreamCreateWithFlags(&c->s_adj, hipStreamNonBlocking), "stream");
         chk(hipEventCreateWithFlags(&c->ev_drift, hipEventDisableTiming), "event");
         chk(hipEventCreateWithFlags(&c->ev_bounds, hipEventDisableTiming), "event");
         for (hipEvent_t& e : c->ev) chk(hipEventCreate(&e), "event");

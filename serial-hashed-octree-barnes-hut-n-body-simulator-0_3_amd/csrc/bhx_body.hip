@@ -7,11 +7,12 @@ namespace bhx {
 namespace {
 
 constexpr int kBlock = 256;
-// 2 x 3 x (3328 + 16) doubles = 156.75 KiB: the chain's workgroup takes
-// (nearly) all of its compute unit's 160 KiB LDS, so no other kernel's waves
-// share that CU and delay the dependent adds.  Each row is padded by 16
-// doubles so that the chain's read-ahead never needs a bounds test.
-constexpr int kMeanChunk = 3328;
+// 2 x 3 x (1024 + 16) doubles = 48.75 KiB of LDS: small enough that the
+// workgroup finds room on a compute unit that other kernels' waves still
+// occupy (a whole-CU allocation would wait for a CU to drain and stall the
+// stream behind it).  Each row is padded by 16 doubles so that the chain's
+// read-ahead never needs a bounds test.
+constexpr int kMeanChunk = 1024;
 constexpr int kMeanRow = kMeanChunk + 16;
 
 inline unsigned grid_for(size_t n, int per_block = kBlock, unsigned cap = 65535u * 16u) {
@@ -64,6 +65,9 @@ __global__ __launch_bounds__(kBlock) void k_bounds_mean(const double* __restrict
                 }
             }
         } else if (ch > 0 && lane < 3) {
+            // the chain's wave issues ahead of other waves on its SIMD (walk waves
+            // of the same frame run beside it)
+            if (ch == 1) __builtin_amdgcn_s_setprio(3);
             // the dependent chain: register double buffering keeps the LDS reads
             // of the next 16 values in flight while the current 16 are added
             const size_t base = (ch - 1) * kMeanChunk;

@@ -198,8 +198,8 @@ struct GradRecord {
     void* seg_store = nullptr;                 // two banks of `every` frame slots
     std::vector<GradSlot> slots;
     // the reverse pass re-runs segment k-1 on the main stream while the
-    // adjoint of segment k runs on its own stream (default priority)
-    hipStream_t s_adj = nullptr;
+    // adjoint of segment k runs on the context's adjoint stream
+    bool events = false;
     hipEvent_t ev_stored[2] = {}, ev_consumed[2] = {}, ev_ready = nullptr;
     double* work = nullptr;                    // adjoint vectors
     size_t work_nodes = 0;
@@ -213,13 +213,12 @@ void free_grad(bhx_ctx* c) {
     if (!g) return;
     for (double* p : g->ckpt)
         if (p) (void)hipFree(p);
-    if (g->s_adj) (void)hipStreamSynchronize(g->s_adj);
+    if (c->s_adj) (void)hipStreamSynchronize(c->s_adj);
     void* ptrs[] = {g->final_state, g->ident, g->ident_tmp, g->init_store, g->seg_store, g->work};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     for (hipEvent_t e : {g->ev_stored[0], g->ev_stored[1], g->ev_consumed[0], g->ev_consumed[1], g->ev_ready})
         if (e) (void)hipEventDestroy(e);
-    if (g->s_adj) (void)hipStreamDestroy(g->s_adj);
     delete g;
     c->grad = nullptr;
 }
@@ -535,8 +534,8 @@ int bhx_adjoint(bhx_ctx* c, const double* gx, const double* gv, const double* ga
         g->slots.resize(2 * (size_t)every);
         for (auto& sl : g->slots) sl = carve_slot(p, n, g->node_cap, c->nsys, g->precision);
     }
-    if (!g->s_adj) {
-        HIPCHK(hipStreamCreateWithFlags(&g->s_adj, hipStreamNonBlocking));
+    if (!g->events) {
+        g->events = true;
         for (int b = 0; b < 2; ++b) {
             HIPCHK(hipEventCreateWithFlags(&g->ev_stored[b], hipEventDisableTiming));
             HIPCHK(hipEventCreateWithFlags(&g->ev_consumed[b], hipEventDisableTiming));
@@ -546,7 +545,7 @@ int bhx_adjoint(bhx_ctx* c, const double* gx, const double* gv, const double* ga
     if (!g->work) {
         if (dalloc(&g->work, work_len(n, g->work_nodes)) != hipSuccess) return fail(c, BHX_E_NOMEM, "adjoint allocation");
     }
-    hipStream_t sa = g->s_adj;
+    hipStream_t sa = c->s_adj;
     Work k = carve_work(g->work, n, g->work_nodes);
     hipLaunchKernelGGL(k_zero_flags, dim3(1), dim3(64), 0, s, c->ctr);
     // incoming adjoints (starting order) into the slot order of the last frame; dL/dm starts at 0
@@ -580,7 +579,8 @@ int bhx_adjoint(bhx_ctx* c, const double* gx, const double* gv, const double* ga
         int r = load_state(c, g->ckpt[seg], s);
         if (r != BHX_OK) return r;
         for (int f = f0; f < f1; ++f) {
-            r = graph_frame(c, s);
+            // linear frames: the adjoint stream runs beside them
+            r = graph_frame(c, s, true);
             if (r != BHX_OK) return r;
             r = store_frame(c, g->slots[(size_t)bank * every + (f - f0)], g->node_cap, g->precision, true, s);
             if (r != BHX_OK) return r;

@@ -101,6 +101,11 @@ struct bhx_ctx {
     int device = 0;
     hipStream_t s_main = nullptr;
     hipStream_t s_side = nullptr;
+    // the reverse pass's adjoint walks (bhx_adjoint).  Created with the other
+    // two so that the three take distinct hardware queues of the process's
+    // pool: a stream created later can share the main stream's queue, and its
+    // kernels then serialise with the main stream's
+    hipStream_t s_adj = nullptr;
     hipEvent_t ev_drift = nullptr;
     hipEvent_t ev_bounds = nullptr;
     hipEvent_t ev[12] = {};
@@ -168,11 +173,18 @@ struct bhx_ctx {
     std::vector<double> tree_bounds_h;         // (centre, size) per system the last tree was built with
     // captured one-step graphs (bhx_step_graph): one per parity of the body
     // ping-pong buffers / bounds slot, each valid for the state it starts from
+    // Frame graphs, captured in pairs (the frame from the current buffer
+    // parity and bounds slot, and the frame after it).  Two pairs are kept:
+    // state restores (checkpoints, bhx_set_bodies) can leave the buffers and
+    // the bounds slot in the other pairing, and a single pair would then be
+    // captured again at every switch.
     struct StepGraph {
         hipGraphExec_t exec = nullptr;
         const double* bufs[32] = {};           // body arrays and gather targets at capture
         int bslot = -1;
-    } graphs[2];
+        bool linear = false;                   // next bounds computed in line (no side branch)
+    } graphs[4];
+    int graph_pair_next = 0;                   // pair replaced by the next capture
     bool graph_valid = false;
     struct GradRecord* grad = nullptr;         // recorded trajectory for bhx_adjoint
     // device snapshots of the body state: [0] where a graph run starts
@@ -226,8 +238,13 @@ void drop_graphs(bhx_ctx* c);
 hipError_t alloc_tree(bhx_ctx* c, size_t cap);
 int build_on_device(bhx_ctx* c, hipStream_t s);       // tree of the resident bodies, bounds slot c->bslot
 int read_counters(bhx_ctx* c, hipStream_t s);         // waits; reports sort / hash failures
-int step_graph_for(bhx_ctx* c, hipGraphExec_t* out);  // one-frame graph for the current buffer parity
-int graph_frame(bhx_ctx* c, hipStream_t s);           // enqueue one frame from its graph
+// One-frame graph for the current buffer parity.  linear: the next frame's
+// bounds are computed in line on the frame's stream instead of on a side
+// branch — for callers that run other streams beside the frames (a graph's
+// parallel branch executes on a runtime-chosen stream that may share a
+// hardware queue with them and wait behind their kernels).
+int step_graph_for(bhx_ctx* c, hipGraphExec_t* out, bool linear = false);
+int graph_frame(bhx_ctx* c, hipStream_t s, bool linear = false);   // enqueue one frame from its graph
 void free_grad(bhx_ctx* c);                           // recorded trajectory (bhx_grad.hip)
 
 // --- launchers implemented in the .hip files (all enqueue on `s`) ----------
