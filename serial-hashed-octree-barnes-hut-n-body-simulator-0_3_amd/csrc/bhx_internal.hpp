@@ -61,6 +61,7 @@ struct DeviceTree {
     float* quadf = nullptr;          // 6 per node, FP32 copy for the walk
     WalkNode* walk = nullptr;
     WalkNode32* walk32 = nullptr;
+    uint32_t* child = nullptr;       // 8 per node: child id by octant (valid where child_byte has the bit)
     uint32_t* by_level = nullptr;    // moments: nodes left for the second phase
     uint32_t* lvl = nullptr;         // moments: their count
     // hash table

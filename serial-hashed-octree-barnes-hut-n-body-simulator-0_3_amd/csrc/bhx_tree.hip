@@ -205,6 +205,7 @@ __device__ uint32_t run_start(const uint64_t* __restrict__ keys, uint32_t base, 
 struct TreeArgs {
     uint64_t* key;
     uint32_t *start, *count, *skip, *parent, *child_byte, *level, *first;
+    uint32_t* child;   // 8 per node, by octant
     double* center;
     double* com;
     double* mass;
@@ -286,6 +287,7 @@ __global__ __launch_bounds__(kBlock) void k_emit(const uint64_t* __restrict__ ke
             t.parent[id] = par;
             t.level[id] = (uint32_t)L;
             atomicOr(&t.child_byte[par], 1u << (unsigned)(k & 7));
+            t.child[8 * (size_t)par + (unsigned)(k & 7)] = id;   // one writer per slot
             double cx, cy, cz, s;
             descend_center(k, L, b4, t.tscale, cx, cy, cz, s);
             t.center[3 * (size_t)id] = cx;
@@ -413,25 +415,58 @@ __device__ void moments_of(uint32_t p, int intended, const TreeArgs& t, const do
         uint32_t first = 0xFFFFFFFFu;
         double cx, cy, cz;
         if (leaf) {
-            for (uint32_t j = st; j < st + cnt; ++j) {
-                const double mj = tm[j];
-                M += mj;
-                sx += mj * tx[j];
-                sy += mj * ty[j];
-                sz += mj * tz[j];
-                const uint32_t a = tperm[j];
-                if (a < first) first = a;
+            // bodies in batches of 8 with every load of a batch in flight before
+            // the (sequential, in body order) sums use them
+            const uint32_t e = st + cnt;
+            for (uint32_t j0 = st; j0 < e; j0 += 8) {
+                double mj[8], xj[8], yj[8], zj[8];
+                uint32_t aj[8];
+#pragma unroll
+                for (int k = 0; k < 8; ++k) {
+                    const uint32_t j = j0 + k < e ? j0 + k : j0;
+                    mj[k] = tm[j]; xj[k] = tx[j]; yj[k] = ty[j]; zj[k] = tz[j]; aj[k] = tperm[j];
+                }
+#pragma unroll
+                for (int k = 0; k < 8; ++k) {
+                    if (j0 + k >= e) break;
+                    M += mj[k];
+                    sx += mj[k] * xj[k];
+                    sy += mj[k] * yj[k];
+                    sz += mj[k] * zj[k];
+                    if (aj[k] < first) first = aj[k];
+                }
             }
             if (!intended || cnt == 1) {
                 cx = px[first]; cy = py[first]; cz = pz[first];
             } else {
                 cx = sx / M; cy = sy / M; cz = sz / M;
             }
-            if (intended && cnt > 1)
-                for (uint32_t j = st; j < st + cnt; ++j) quad_add(q, tm[j], tx[j] - cx, ty[j] - cy, tz[j] - cz);
+            if (intended && cnt > 1) {
+                for (uint32_t j0 = st; j0 < e; j0 += 8) {
+                    double mj[8], xj[8], yj[8], zj[8];
+#pragma unroll
+                    for (int k = 0; k < 8; ++k) {
+                        const uint32_t j = j0 + k < e ? j0 + k : j0;
+                        mj[k] = tm[j]; xj[k] = tx[j]; yj[k] = ty[j]; zj[k] = tz[j];
+                    }
+#pragma unroll
+                    for (int k = 0; k < 8; ++k) {
+                        if (j0 + k >= e) break;
+                        quad_add(q, mj[k], xj[k] - cx, yj[k] - cy, zj[k] - cz);
+                    }
+                }
+            }
         } else {
-            const uint32_t end = t.skip[p];
-            for (uint32_t c = p + 1; c < end; c = t.skip[c]) {
+            // children in octant order = pre-order: the sums keep their order; the
+            // child ids come from the octant table, so the child records are
+            // fetched independently rather than along the pre-order skip chain
+            const uint4 ca = *reinterpret_cast<const uint4*>(t.child + 8 * (size_t)p);
+            const uint4 cb4 = *reinterpret_cast<const uint4*>(t.child + 8 * (size_t)p + 4);
+            const uint32_t ids[8] = {ca.x, ca.y, ca.z, ca.w, cb4.x, cb4.y, cb4.z, cb4.w};
+#pragma unroll
+            for (int o = 0; o < 8; ++o) {
+                if (!((cb >> o) & 1u)) continue;
+                const uint32_t c = ids[o];
                 const double mc = t.mass[c];
                 M += mc;
                 sx += mc * t.com[3 * (size_t)c];
@@ -443,7 +478,10 @@ __device__ void moments_of(uint32_t p, int intended, const TreeArgs& t, const do
                 cx = px[first]; cy = py[first]; cz = pz[first];
             } else {
                 cx = sx / M; cy = sy / M; cz = sz / M;
-                for (uint32_t c = p + 1; c < end; c = t.skip[c]) {
+#pragma unroll
+                for (int o = 0; o < 8; ++o) {
+                    if (!((cb >> o) & 1u)) continue;
+                    const uint32_t c = ids[o];
                     const double* cc = t.com + 3 * (size_t)c;
                     quad_add(q, t.mass[c], cc[0] - cx, cc[1] - cy, cc[2] - cz);
                     for (int k = 0; k < 6; ++k) q[k] += t.quad[6 * (size_t)c + k];
@@ -483,9 +521,21 @@ __device__ void moments_of(uint32_t p, int intended, const TreeArgs& t, const do
         if (leaf) {
             // compact FP32 copy of the leaf's bodies: offsets from the leaf's
             // centre of mass (which the walk record carries), and the leaf id
-            for (uint32_t j = st; j < st + cnt; ++j) {
-                t.tb4[j] = make_float4((float)(tx[j] - cx), (float)(ty[j] - cy), (float)(tz[j] - cz), (float)tm[j]);
-                t.tleaf[j] = p;
+            const uint32_t e = st + cnt;
+            for (uint32_t j0 = st; j0 < e; j0 += 8) {
+                double mj[8], xj[8], yj[8], zj[8];
+#pragma unroll
+                for (int k = 0; k < 8; ++k) {
+                    const uint32_t j = j0 + k < e ? j0 + k : j0;
+                    mj[k] = tm[j]; xj[k] = tx[j]; yj[k] = ty[j]; zj[k] = tz[j];
+                }
+#pragma unroll
+                for (int k = 0; k < 8; ++k) {
+                    if (j0 + k >= e) break;
+                    t.tb4[j0 + k] = make_float4((float)(xj[k] - cx), (float)(yj[k] - cy), (float)(zj[k] - cz),
+                                                (float)mj[k]);
+                    t.tleaf[j0 + k] = p;
+                }
             }
         }
     }
@@ -496,9 +546,10 @@ __device__ void moments_of(uint32_t p, int intended, const TreeArgs& t, const do
 // the slice is combined there, deepest level first, with a workgroup barrier
 // between levels (children and parent share the compute unit, so no
 // device-wide ordering is needed).  Nodes whose subtree leaves the slice
-// (the slice-straddling ancestors, at most 21 per slice boundary) are listed
-// for phase 2, where one workgroup combines them level by level.  Every node
-// is combined once, from its final children in pre-order.
+// are ancestors of the first node after the slice, so a slice has at most
+// one per level: they are listed by level (rest[L * slices + i], count
+// nrest[L]) for phase 2, where one workgroup combines them level by level.
+// Every node is combined once, from its final children in pre-order.
 __global__ __launch_bounds__(kBlock) void k_moments_local(const Counters* __restrict__ ctr, SysView sv, int intended,
                                                           TreeArgs t, const double* __restrict__ bounds4,
                                                           const double* __restrict__ px, const double* __restrict__ py,
@@ -523,7 +574,7 @@ __global__ __launch_bounds__(kBlock) void k_moments_local(const Counters* __rest
         L = t.level[p];
         local = t.skip[p] <= b1;
         if (local) atomicOr(&s_levels, 1u << L);
-        else rest[atomicAdd(nrest, 1u)] = p;
+        else rest[L * (size_t)gridDim.x + atomicAdd(&nrest[L], 1u)] = p;
     }
     __syncthreads();
     const uint32_t levels = s_levels;
@@ -541,14 +592,13 @@ __global__ __launch_bounds__(1024) void k_moments_rest(const Counters* __restric
                                                        const double* __restrict__ ty, const double* __restrict__ tz,
                                                        const double* __restrict__ tm, const uint32_t* __restrict__ tperm,
                                                        const uint32_t* __restrict__ rest,
-                                                       const uint32_t* __restrict__ nrest) {
+                                                       const uint32_t* __restrict__ nrest, uint32_t slices) {
     if (ctr->node_overflow) return;
-    const uint32_t m = *nrest;
     for (int l = kMaxLevel; l >= 0; --l) {
-        for (uint32_t i = threadIdx.x; i < m; i += blockDim.x) {
-            const uint32_t p = rest[i];
-            if ((int)t.level[p] == l) moments_of(p, intended, t, bounds4, sv, px, py, pz, tx, ty, tz, tm, tperm);
-        }
+        const uint32_t m = nrest[l];
+        if (m == 0) continue;   // block-uniform
+        for (uint32_t i = threadIdx.x; i < m; i += blockDim.x)
+            moments_of(rest[l * (size_t)slices + i], intended, t, bounds4, sv, px, py, pz, tx, ty, tz, tm, tperm);
         __syncthreads();
     }
 }
@@ -558,6 +608,7 @@ TreeArgs tree_args(bhx_ctx* c) {
     TreeArgs t;
     t.key = d.key; t.start = d.start; t.count = d.count; t.skip = d.skip; t.parent = d.parent;
     t.child_byte = d.child_byte; t.level = d.level; t.first = d.first; t.center = d.center;
+    t.child = d.child;
     t.com = d.com; t.mass = d.mass; t.quad = d.quad; t.quadf = d.quadf; t.walk = d.walk;
     t.walk32 = d.walk32;
     t.tb4 = c->tb4;
@@ -623,13 +674,16 @@ hipError_t build_tree(bhx_ctx* c, hipStream_t s, size_t* total_out, bool sync) {
     hipLaunchKernelGGL(k_hash_insert, dim3(grid_for(span)), dim3(kBlock), 0, s, c->tree.key, c->tree.start, sv,
                        c->tree.hkey, c->tree.hval, (uint64_t)(c->tree.hcap - 1), c->ctr);
     const int intended = c->cfg.mode == BHX_MODE_INTENDED;
+    // per-level lists of slice-straddling nodes: (kMaxLevel + 1) x slices
+    // entries, within the node capacity (slices = capacity / 256, rounded up)
     uint32_t* nrest = c->tree.lvl;
-    hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, s, nrest, (size_t)1, 0u);
-    hipLaunchKernelGGL(k_moments_local, dim3(grid_for(span, kBlock, 0xFFFFFFFFu)), dim3(kBlock), 0, s, c->ctr, sv,
-                       intended, t, b4, c->px, c->py, c->pz, c->tx, c->ty, c->tz, c->tm, c->tperm, c->tree.by_level,
-                       nrest);
+    const unsigned slices = grid_for(span, kBlock, 0xFFFFFFFFu);
+    if ((size_t)(kMaxLevel + 1) * slices > c->tree.cap) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, s, nrest, (size_t)(kMaxLevel + 1), 0u);
+    hipLaunchKernelGGL(k_moments_local, dim3(slices), dim3(kBlock), 0, s, c->ctr, sv, intended, t, b4, c->px, c->py,
+                       c->pz, c->tx, c->ty, c->tz, c->tm, c->tperm, c->tree.by_level, nrest);
     hipLaunchKernelGGL(k_moments_rest, dim3(1), dim3(1024), 0, s, c->ctr, sv, intended, t, b4, c->px, c->py, c->pz,
-                       c->tx, c->ty, c->tz, c->tm, c->tperm, c->tree.by_level, nrest);
+                       c->tx, c->ty, c->tz, c->tm, c->tperm, c->tree.by_level, nrest, (uint32_t)slices);
     return hipGetLastError();
 }
 
