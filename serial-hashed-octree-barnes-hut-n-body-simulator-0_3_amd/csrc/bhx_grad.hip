@@ -333,6 +333,7 @@ WalkTree tree_of(const GradSlot& d) {
     WalkTree t;
     t.walk = d.walk;
     t.walk32 = d.walk32;
+    t.com = d.com;
     t.tb4 = d.tb4;
     t.tleaf = d.tleaf;
     t.tx = d.tx; t.ty = d.ty; t.tz = d.tz; t.tm = d.tm;

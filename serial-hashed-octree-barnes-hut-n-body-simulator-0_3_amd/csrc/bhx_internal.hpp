@@ -34,13 +34,19 @@ static_assert(sizeof(WalkNode) == 64, "walk node is one 64-byte record");
 
 // FP32-walk record, also 64 bytes: the quadrupole travels with the node so an
 // accepted cell needs no second fetch; the cell edge follows from the level.
+// The first 32 bytes are what a visit reads (one scalar load): the centre of
+// mass relative to the system's root centre in FP32 (for the opening test
+// only; evaluated entries take their group-relative FP32 centre from the
+// FP64 centres), the structure, the mass and the cell edge squared.
 struct alignas(64) WalkNode32 {
-    double cx, cy, cz;      // centre of mass (FP64: offsets are formed per group)
-    float mass;
-    float q[6];             // Qxx Qxy Qxz Qyy Qyz Qzz (0 for one-body cells)
+    float cx, cy, cz;       // centre of mass - root centre (FP32)
+    uint32_t meta;          // level (bits 0-4) | leaf (bit 5) | min(count, 2^26-1) << 6
     uint32_t skip;
     uint32_t start;         // first body in tree order
-    uint32_t meta;          // level (bits 0-4) | leaf (bit 5) | min(count, 2^26-1) << 6
+    float mass;
+    float size2;            // cell edge squared
+    float q[6];             // Qxx Qxy Qxz Qyy Qyz Qzz (0 for one-body cells)
+    uint32_t pad[2];
 };
 static_assert(sizeof(WalkNode32) == 64, "FP32 walk node is one 64-byte record");
 
@@ -288,6 +294,7 @@ hipError_t launch_direct(bhx_ctx* c, const int64_t* targets, size_t nt, double e
 struct WalkTree {
     const WalkNode* walk;
     const WalkNode32* walk32;
+    const double* com;      // FP64 centres of mass, 3 per node
     const float4* tb4;
     const uint32_t* tleaf;
     const double *tx, *ty, *tz, *tm;

@@ -497,7 +497,8 @@ __device__ void moments_of(uint32_t p, int intended, const TreeArgs& t, const do
             t.quad[6 * (size_t)p + k] = q[k];
             t.quadf[6 * (size_t)p + k] = (float)q[k];
         }
-        double s = bounds4[4 * (size_t)sys_of(sv, st) + 3] * t.tscale;
+        const double* b4n = bounds4 + 4 * (size_t)sys_of(sv, st);
+        double s = b4n[3] * t.tscale;
         for (int l = 0; l < L; ++l) s = s * 0.5;
         WalkNode w;
         w.cx = cx; w.cy = cy; w.cz = cz;
@@ -511,12 +512,16 @@ __device__ void moments_of(uint32_t p, int intended, const TreeArgs& t, const do
         w.massf = (float)M;
         t.walk[p] = w;
         WalkNode32 w32;
-        w32.cx = cx; w32.cy = cy; w32.cz = cz;
-        w32.mass = (float)M;
-        for (int k = 0; k < 6; ++k) w32.q[k] = cnt > 1 ? (float)q[k] : 0.0f;
+        w32.cx = (float)(cx - b4n[0]);
+        w32.cy = (float)(cy - b4n[1]);
+        w32.cz = (float)(cz - b4n[2]);
+        w32.meta = (uint32_t)L | (leaf ? 32u : 0u) | ((cnt < (1u << 26) ? cnt : (1u << 26) - 1) << 6);
         w32.skip = t.skip[p];
         w32.start = st;
-        w32.meta = (uint32_t)L | (leaf ? 32u : 0u) | ((cnt < (1u << 26) ? cnt : (1u << 26) - 1) << 6);
+        w32.mass = (float)M;
+        w32.size2 = (float)(s * s);
+        for (int k = 0; k < 6; ++k) w32.q[k] = cnt > 1 ? (float)q[k] : 0.0f;
+        w32.pad[0] = w32.pad[1] = 0u;
         t.walk32[p] = w32;
         if (leaf) {
             // compact FP32 copy of the leaf's bodies: offsets from the leaf's

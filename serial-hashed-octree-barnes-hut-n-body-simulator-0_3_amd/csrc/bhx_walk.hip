@@ -50,6 +50,7 @@ __device__ __forceinline__ WalkNode load_node(ConstU64 raw, uint32_t p) {
 struct WalkArgs {
     const WalkNode* nodes;
     const WalkNode32* nodes32;
+    const double* com;       // FP64 centres of mass, 3 per node (the FP32 walk's entries)
     const float4* tb4;       // tree order: offset from the leaf's centre of mass, mass
     const uint32_t* tleaf;   // tree order: node id of the body's leaf
     uint32_t groups;         // walk groups (grid is rounded up to a multiple of 8)
@@ -650,14 +651,16 @@ __global__ __launch_bounds__(64) void k_walk_group(WalkArgs a) {
     }
 }
 
-// FP32 group walk over the compact layout: 64-byte FP32 records carrying the
-// quadrupole (one scalar burst per visited cell, nothing more to fetch when a
-// cell is accepted), cell edges from the level, and opened leaves gathered as
-// 16-byte offsets from the leaf's centre of mass plus that centre's offset
-// from the group origin, which the opening test has just formed.  Every lane
-// builds its own position the same way (its leaf's centre offset + its own
-// offset), so a body meets itself at exactly d = 0.  Same shared-list rule
-// as k_walk_group.
+// FP32 group walk over the compact layout: 64-byte FP32 records whose first
+// 32 bytes (one scalar burst per visited cell) hold what the opening test
+// needs — the centre relative to the system's root centre in FP32, the edge
+// squared, the structure — and whose rest holds the mass and quadrupole.  An
+// entry's group-relative FP32 centre is formed from the FP64 centres when its
+// list is evaluated; opened leaves are gathered as 16-byte offsets from the
+// leaf's centre of mass plus that centre's offset from the group origin.
+// Every lane builds its own position the same way (its leaf's centre offset +
+// its own offset), so a body meets itself at exactly d = 0.  Same
+// shared-list rule as k_walk_group.
 //
 // Work distribution: consecutive workgroups land on the 8 XCDs in turn, each
 // with its own L2.  Group g is given to block (g mod per) * 8 + g / per, so
@@ -670,39 +673,26 @@ __device__ __forceinline__ uint32_t xcd_group(uint32_t block, uint32_t groups) {
     return (block % 8u) * per + block / 8u;
 }
 
+// What a visit reads: the first 32 bytes of the record, one scalar burst.
 struct Node32 {
-    double cx, cy, cz;
-    float mass, q0, q1, q2, q3, q4, q5;
-    uint32_t skip, start, meta;
+    float cx, cy, cz, size2;   // centre - root centre; edge squared
+    uint32_t meta, skip, start;
 };
 
 __device__ __forceinline__ Node32 load_node32(ConstU64 raw, uint32_t p) {
     const size_t b = 8 * (size_t)p;
     const unsigned long long w0 = raw[b], w1 = raw[b + 1], w2 = raw[b + 2], w3 = raw[b + 3];
-    const unsigned long long w4 = raw[b + 4], w5 = raw[b + 5], w6 = raw[b + 6], w7 = raw[b + 7];
     Node32 nd;
-    nd.cx = __longlong_as_double((long long)w0);
-    nd.cy = __longlong_as_double((long long)w1);
-    nd.cz = __longlong_as_double((long long)w2);
-    nd.mass = __uint_as_float((uint32_t)w3);
-    nd.q0 = __uint_as_float((uint32_t)(w3 >> 32));
-    nd.q1 = __uint_as_float((uint32_t)w4);
-    nd.q2 = __uint_as_float((uint32_t)(w4 >> 32));
-    nd.q3 = __uint_as_float((uint32_t)w5);
-    nd.q4 = __uint_as_float((uint32_t)(w5 >> 32));
-    nd.q5 = __uint_as_float((uint32_t)w6);
-    nd.skip = (uint32_t)(w6 >> 32);
-    nd.start = (uint32_t)w7;
-    nd.meta = (uint32_t)(w7 >> 32);
+    nd.cx = __uint_as_float((uint32_t)w0);
+    nd.cy = __uint_as_float((uint32_t)(w0 >> 32));
+    nd.cz = __uint_as_float((uint32_t)w1);
+    nd.meta = (uint32_t)(w1 >> 32);
+    nd.skip = (uint32_t)w2;
+    nd.start = (uint32_t)(w2 >> 32);
+    nd.size2 = __uint_as_float((uint32_t)(w3 >> 32));
     return nd;
 }
 
-// FP32 list terms.  Without the zero-separation guard r^2 is formed as
-// ((eps^2 + dx^2) + dy^2) + dz^2 in fused multiply-adds, and the quadrupole
-// folds into the monopole's coefficient:
-//   a += d (M r^-3 + 2.5 (d.Q.d) r^-7) - (Q.d) r^-5
-// (32 vector instructions per cell, 13 per body).  Bodies are entries with
-// Q = 0 and go through the kQuad = false form.
 template <bool kQuad, bool kGuard, bool kPot = false>
 __device__ __forceinline__ void eval_list32(const float (*nd)[4], const float (*q)[6], int nn, float xr, float yr,
                                             float zr, float eps2, float& ax, float& ay, float& az, float& phi) {
@@ -753,16 +743,17 @@ constexpr int bodies32() { return kAdj ? 128 : 96; }
 // group-relative FP32 entries the evaluation reads.
 template <bool kQuad>
 __device__ __forceinline__ void gather_cells32(float (*nd)[4], float (*q)[6], const uint32_t* nid, int nn,
-                                               uint32_t lane, const WalkNode32* __restrict__ nodes, double ox,
-                                               double oy, double oz) {
+                                               uint32_t lane, const WalkNode32* __restrict__ nodes,
+                                               const double* __restrict__ com, double ox, double oy, double oz) {
 #pragma unroll
     for (int h = 0; h < (kCells32 + 63) / 64; ++h) {
         const int e = (int)lane + 64 * h;
         if (e < nn) {
-            const WalkNode32& c = nodes[nid[e]];
-            nd[e][0] = (float)(c.cx - ox);
-            nd[e][1] = (float)(c.cy - oy);
-            nd[e][2] = (float)(c.cz - oz);
+            const uint32_t id = nid[e];
+            const WalkNode32& c = nodes[id];
+            nd[e][0] = (float)(com[3 * (size_t)id] - ox);
+            nd[e][1] = (float)(com[3 * (size_t)id + 1] - oy);
+            nd[e][2] = (float)(com[3 * (size_t)id + 2] - oz);
             nd[e][3] = c.mass;
             if (kQuad) {
 #pragma unroll
@@ -783,6 +774,7 @@ __global__ __launch_bounds__(64) void k_walk_group32(WalkArgs a) {
     __shared__ float s_bd[kBodies32][4];
     __shared__ uint32_t s_rstart[kRanges32], s_roff[kRanges32];
     __shared__ float s_rdel[kRanges32][3];    // leaf's centre minus the group origin
+    __shared__ uint32_t s_rleaf[kRanges32];   // leaf node of each range
     __shared__ uint32_t s_bid[kAdj ? kBodies32 : 1];
     __shared__ float s_tg[kAdj ? 64 : 1][6];
 
@@ -809,14 +801,16 @@ __global__ __launch_bounds__(64) void k_walk_group32(WalkArgs a) {
     // roundings the body gets as a list entry
     const uint32_t own = a.tleaf[iv];
     const float4 ob = a.tb4[iv];
-    const float xr = (float)(a.nodes32[own].cx - ox) + ob.x;
-    const float yr = (float)(a.nodes32[own].cy - oy) + ob.y;
-    const float zr = (float)(a.nodes32[own].cz - oz) + ob.z;
+    const float xr = (float)(a.com[3 * (size_t)own] - ox) + ob.x;
+    const float yr = (float)(a.com[3 * (size_t)own + 1] - oy) + ob.y;
+    const float zr = (float)(a.com[3 * (size_t)own + 2] - oz) + ob.z;
     const float eps2 = a.eps2f;
-    // opening test edge^2 < theta^2 d^2 as (edge^2 / theta^2) < d^2, with
-    // edge^2 / theta^2 = (root value) * 4^-L (theta = 0: infinite, never accepts)
-    const double s0 = a.b4[4 * (size_t)gr.z + 3] * a.tscale;
-    const float s2th = (float)(s0 * s0) / a.theta2f;
+    // the opening test edge^2 < theta^2 d^2 in FP32 coordinates relative to the
+    // system's root centre, the frame of the records' centres (the decision
+    // only; theta = 0 opens every cell)
+    const double* b4s = a.b4 + 4 * (size_t)gr.z;
+    const float xs = (float)(tx[iv] - b4s[0]), ys = (float)(ty[iv] - b4s[1]), zs = (float)(tz[iv] - b4s[2]);
+    const float th2 = a.theta2f;
     float ax = 0.f, ay = 0.f, az = 0.f, phi = 0.f;
     float wxr = 0.f, wyr = 0.f, wzr = 0.f;
     if constexpr (kAdj) {
@@ -831,7 +825,7 @@ __global__ __launch_bounds__(64) void k_walk_group32(WalkArgs a) {
     }
     auto flush_cells = [&](int nn) {
         __syncthreads();
-        gather_cells32<kQuad>(s_nd, s_q, s_nid, nn, lane, a.nodes32, ox, oy, oz);
+        gather_cells32<kQuad>(s_nd, s_q, s_nid, nn, lane, a.nodes32, a.com, ox, oy, oz);
         __syncthreads();
         if constexpr (kAdj) {
             adj_self<float>(s_nd, nn, xr, yr, zr, wxr, wyr, wzr, eps2, ax, ay, az);
@@ -842,6 +836,14 @@ __global__ __launch_bounds__(64) void k_walk_group32(WalkArgs a) {
         __syncthreads();
     };
     auto flush_bodies = [&](int nr, int nq) {
+        __syncthreads();
+        // each range's leaf centre relative to the group origin, from its FP64 centre
+        if ((int)lane < nr) {
+            const size_t l3 = 3 * (size_t)s_rleaf[lane];
+            s_rdel[lane][0] = (float)(a.com[l3] - ox);
+            s_rdel[lane][1] = (float)(a.com[l3 + 1] - oy);
+            s_rdel[lane][2] = (float)(a.com[l3 + 2] - oz);
+        }
         __syncthreads();
 #pragma unroll
         for (int h = 0; h < (kBodies32 + 63) / 64; ++h) {
@@ -885,7 +887,6 @@ __global__ __launch_bounds__(64) void k_walk_group32(WalkArgs a) {
     uint32_t kres = 0;
     if (root.meta & 32u) {
         const uint32_t count = root.meta >> 6;
-        const float cx = (float)(root.cx - ox), cy = (float)(root.cy - oy), cz = (float)(root.cz - oz);
         for (uint32_t k = 0; k < count; k += 64) {
             const int m = (int)((count - k) < 64u ? (count - k) : 64u);
             if (nq + m > kBodies32 || nr == kRanges32) {
@@ -897,9 +898,7 @@ __global__ __launch_bounds__(64) void k_walk_group32(WalkArgs a) {
             if (lane == 0) {
                 s_rstart[nr] = root.start + k;
                 s_roff[nr] = (uint32_t)nq;
-                s_rdel[nr][0] = cx;
-                s_rdel[nr][1] = cy;
-                s_rdel[nr][2] = cz;
+                s_rleaf[nr] = rid;
             }
             ++nr;
             nq += m;
@@ -912,12 +911,9 @@ __global__ __launch_bounds__(64) void k_walk_group32(WalkArgs a) {
             p = __builtin_amdgcn_readfirstlane(p);
             const Node32 nd = load_node32(raw, p);
             const uint32_t nskip = nd.skip > p ? nd.skip : p + 1;   // always forward
-            const int L = (int)(nd.meta & 31u);
-            const float cx = (float)(nd.cx - ox), cy = (float)(nd.cy - oy), cz = (float)(nd.cz - oz);
-            const float dx = cx - xr, dy = cy - yr, dz = cz - zr;
+            const float dx = nd.cx - xs, dy = nd.cy - ys, dz = nd.cz - zs;
             const float d2 = dx * dx + dy * dy + dz * dz;
-            const float s2L = ldexpf(s2th, -2 * L);
-            const bool opened = (__builtin_amdgcn_ballot_w64(!(s2L < d2)) & vmask) != 0ull;
+            const bool opened = (__builtin_amdgcn_ballot_w64(!(nd.size2 < th2 * d2)) & vmask) != 0ull;
             if (!opened) {
                 if (lane == 0) s_nid[nn] = p;
                 ++nn;
@@ -933,9 +929,7 @@ __global__ __launch_bounds__(64) void k_walk_group32(WalkArgs a) {
                 if (lane == 0) {
                     s_rstart[nr] = nd.start + kres;
                     s_roff[nr] = (uint32_t)nq;
-                    s_rdel[nr][0] = cx;     // the leaf's centre relative to the group origin
-                    s_rdel[nr][1] = cy;
-                    s_rdel[nr][2] = cz;
+                    s_rleaf[nr] = p;
                 }
                 ++nr;
                 nq += m;
@@ -1079,6 +1073,7 @@ hipError_t launch_walk_to(bhx_ctx* c, hipStream_t s, bool group, int precision, 
     WalkArgs a{};
     a.nodes = c->tree.walk;
     a.nodes32 = c->tree.walk32;
+    a.com = c->tree.com;
     a.tb4 = c->tb4;
     a.tleaf = c->tleaf;
     a.groups = c->ngroups;
@@ -1130,6 +1125,7 @@ hipError_t launch_walk_potential(bhx_ctx* c, hipStream_t s, double* ox, double* 
     WalkArgs a{};
     a.nodes = c->tree.walk;
     a.nodes32 = c->tree.walk32;
+    a.com = c->tree.com;
     a.tb4 = c->tb4;
     a.tleaf = c->tleaf;
     a.groups = c->ngroups;
@@ -1173,6 +1169,7 @@ hipError_t launch_walk_adjoint(bhx_ctx* c, const WalkTree& t, int precision, con
     WalkArgs a{};
     a.nodes = t.walk;
     a.nodes32 = t.walk32;
+    a.com = t.com;
     a.tb4 = t.tb4;
     a.tleaf = t.tleaf;
     a.groups = c->ngroups;
