@@ -295,7 +295,7 @@ int enqueue_step_async(bhx_ctx* c, hipStream_t s, bool linear) {
     } else {
         HIPCHK(hipEventRecord(c->ev_drift, s));
         HIPCHK(hipStreamWaitEvent(c->s_side, c->ev_drift, 0));
-        HIPCHK(launch_bounds(c, b_next, c->s_side));
+        HIPCHK(launch_bounds(c, b_next, c->s_side, true));
         HIPCHK(hipEventRecord(c->ev_bounds, c->s_side));
     }
     HIPCHK(launch_treekeys(c->px, c->py, c->pz, c->n, sv.id, b_cur, c->cfg_tscale(), c->tkeys, c->tperm, s));
@@ -1098,7 +1098,7 @@ int bhx_step(bhx_ctx* c, int nsteps) {
         HIPCHK(hipEventRecord(c->ev_drift, s));
         if (prof) HIPCHK(hipEventRecord(c->ev[4], s));
         HIPCHK(hipStreamWaitEvent(c->s_side, c->ev_drift, 0));
-        HIPCHK(launch_bounds(c, b_next, c->s_side));
+        HIPCHK(launch_bounds(c, b_next, c->s_side, true));
         HIPCHK(hipEventRecord(c->ev_bounds, c->s_side));
         // tree on x_{f+1} with the bounds of x_f
         c->tree_valid = c->tree_built = false;

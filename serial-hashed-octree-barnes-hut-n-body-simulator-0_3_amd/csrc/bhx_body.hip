@@ -7,13 +7,16 @@ namespace bhx {
 namespace {
 
 constexpr int kBlock = 256;
-// 2 x 3 x (1024 + 16) doubles = 48.75 KiB of LDS: small enough that the
-// workgroup finds room on a compute unit that other kernels' waves still
-// occupy (a whole-CU allocation would wait for a CU to drain and stall the
-// stream behind it).  Each row is padded by 16 doubles so that the chain's
-// read-ahead never needs a bounds test.
-constexpr int kMeanChunk = 1024;
-constexpr int kMeanRow = kMeanChunk + 16;
+// Chunk sizes of the sequential mean (doubles per axis and buffer; each row
+// is padded by 16 doubles so that the chain's read-ahead never needs a bounds
+// test).  Exclusive: 2 x 3 x (3328 + 16) doubles = 156.75 KiB of LDS, so the
+// workgroup holds its compute unit alone and no other kernel's waves slow the
+// dependent adds; it can only start on a drained compute unit, so it is used
+// where nothing else keeps the GPU full (the frame's own side branch of a
+// single system).  Shared: 2 x 3 x (1024 + 16) doubles = 48.75 KiB, which
+// finds room beside other waves.
+constexpr int kMeanChunkExclusive = 3328;
+constexpr int kMeanChunkShared = 1024;
 
 inline unsigned grid_for(size_t n, int per_block = kBlock, unsigned cap = 65535u * 16u) {
     size_t g = (n + per_block - 1) / per_block;
@@ -29,10 +32,12 @@ inline unsigned grid_for(size_t n, int per_block = kBlock, unsigned cap = 65535u
 // stream chunks of positions into LDS (double buffered) while lanes 0..2 of
 // wave 0 run the x, y and z chains over the staged chunk.  One workgroup per
 // system; the chains of different systems run side by side.
+template <int kMeanChunk>
 __global__ __launch_bounds__(kBlock) void k_bounds_mean(const double* __restrict__ px,
                                                         const double* __restrict__ py,
                                                         const double* __restrict__ pz, SysView sv,
                                                         double* __restrict__ bounds4) {
+    constexpr int kMeanRow = kMeanChunk + 16;
     __shared__ double buf[2][3][kMeanRow];
     const int tid = threadIdx.x;
     const int wave = tid >> 6, lane = tid & 63;
@@ -346,10 +351,15 @@ __global__ __launch_bounds__(kBlock) void k_tree_gather(const double* __restrict
 
 }  // namespace
 
-hipError_t launch_bounds(bhx_ctx* c, double* bounds4, hipStream_t s) {
+hipError_t launch_bounds(bhx_ctx* c, double* bounds4, hipStream_t s, bool exclusive) {
     const SysView sv = c->sys();
     if (sv.nsys == 0 || c->n == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_bounds_mean, dim3(sv.nsys), dim3(kBlock), 0, s, c->px, c->py, c->pz, sv, bounds4);
+    if (exclusive && sv.nsys == 1)
+        hipLaunchKernelGGL(k_bounds_mean<kMeanChunkExclusive>, dim3(1), dim3(kBlock), 0, s, c->px, c->py, c->pz, sv,
+                           bounds4);
+    else
+        hipLaunchKernelGGL(k_bounds_mean<kMeanChunkShared>, dim3(sv.nsys), dim3(kBlock), 0, s, c->px, c->py, c->pz,
+                           sv, bounds4);
     hipLaunchKernelGGL(k_bounds_size, dim3(grid_for(c->max_sys_n, kBlock * 8, 2048), sv.nsys), dim3(kBlock), 0, s,
                        c->px, c->py, c->pz, sv, bounds4);
     return hipGetLastError();

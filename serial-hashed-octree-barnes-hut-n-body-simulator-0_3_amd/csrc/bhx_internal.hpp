@@ -257,7 +257,10 @@ void free_grad(bhx_ctx* c);                           // recorded trajectory (bh
 // --- launchers implemented in the .hip files (all enqueue on `s`) ----------
 // OctantBounds of the resident bodies of every system into bounds4 (centre,
 // size: 4 doubles per system)
-hipError_t launch_bounds(bhx_ctx* c, double* bounds4, hipStream_t s);
+// exclusive: the next frame's bounds on the frame's own side branch, where
+// nothing else keeps the GPU busy (one system only: the chain's workgroup then
+// takes a whole compute unit)
+hipError_t launch_bounds(bhx_ctx* c, double* bounds4, hipStream_t s, bool exclusive = false);
 hipError_t launch_morton(const double* px, const double* py, const double* pz, size_t n, const uint32_t* sysid,
                          const double* bounds4, uint64_t* keys, uint32_t* vals, hipStream_t s);
 // tree path keys; the root edge is tscale * (bounds size)
