@@ -401,11 +401,42 @@ __device__ __forceinline__ void quad_add(double* q, double m, double dx, double 
 //  INTENDED (computeNodeBaryCenters' aim, HashedOctree.cpp:721-827, written
 //  from the physics): mass-weighted centre of mass and the parallel-axis
 //  quadrupole sum Q = sum_k [M_k (3 d d^T - |d|^2 I) + Q_k].
-__device__ void moments_of(uint32_t p, int intended, const TreeArgs& t, const double* __restrict__ bounds4,
-                           const SysView& sv, const double* __restrict__ px, const double* __restrict__ py,
-                           const double* __restrict__ pz, const double* __restrict__ tx,
-                           const double* __restrict__ ty, const double* __restrict__ tz,
-                           const double* __restrict__ tm, const uint32_t* __restrict__ tperm) {
+//  Children's payloads come through `Child`: the node arrays in global memory,
+//  or the slice's LDS copy when the node's whole subtree lies in the slice.
+//  The node's own payload is written to the node arrays and returned.
+struct Payload {
+    double M, cx, cy, cz;
+    double q[6];
+    uint32_t first;
+};
+
+struct ChildFromGlobal {
+    const TreeArgs& t;
+    __device__ double mass(uint32_t c) const { return t.mass[c]; }
+    __device__ double com(uint32_t c, int k) const { return t.com[3 * (size_t)c + k]; }
+    __device__ uint32_t first(uint32_t c) const { return t.first[c]; }
+    __device__ double quad(uint32_t c, int k) const { return t.quad[6 * (size_t)c + k]; }
+};
+
+struct ChildFromSlice {
+    const double* m;
+    const double (*c)[3];
+    const uint32_t* f;
+    const double (*q)[6];
+    uint32_t b0;
+    __device__ double mass(uint32_t i) const { return m[i - b0]; }
+    __device__ double com(uint32_t i, int k) const { return c[i - b0][k]; }
+    __device__ uint32_t first(uint32_t i) const { return f[i - b0]; }
+    __device__ double quad(uint32_t i, int k) const { return q[i - b0][k]; }
+};
+
+template <class Child>
+__device__ Payload moments_of(uint32_t p, int intended, const TreeArgs& t, const double* __restrict__ bounds4,
+                              const SysView& sv, const double* __restrict__ px, const double* __restrict__ py,
+                              const double* __restrict__ pz, const double* __restrict__ tx,
+                              const double* __restrict__ ty, const double* __restrict__ tz,
+                              const double* __restrict__ tm, const uint32_t* __restrict__ tperm,
+                              const Child& ch) {
     {
         const int L = (int)t.level[p];
         const uint32_t st = t.start[p], cnt = t.count[p], cb = t.child_byte[p];
@@ -467,12 +498,13 @@ __device__ void moments_of(uint32_t p, int intended, const TreeArgs& t, const do
             for (int o = 0; o < 8; ++o) {
                 if (!((cb >> o) & 1u)) continue;
                 const uint32_t c = ids[o];
-                const double mc = t.mass[c];
+                const double mc = ch.mass(c);
                 M += mc;
-                sx += mc * t.com[3 * (size_t)c];
-                sy += mc * t.com[3 * (size_t)c + 1];
-                sz += mc * t.com[3 * (size_t)c + 2];
-                if (t.first[c] < first) first = t.first[c];
+                sx += mc * ch.com(c, 0);
+                sy += mc * ch.com(c, 1);
+                sz += mc * ch.com(c, 2);
+                const uint32_t fc = ch.first(c);
+                if (fc < first) first = fc;
             }
             if (!intended) {
                 cx = px[first]; cy = py[first]; cz = pz[first];
@@ -482,9 +514,8 @@ __device__ void moments_of(uint32_t p, int intended, const TreeArgs& t, const do
                 for (int o = 0; o < 8; ++o) {
                     if (!((cb >> o) & 1u)) continue;
                     const uint32_t c = ids[o];
-                    const double* cc = t.com + 3 * (size_t)c;
-                    quad_add(q, t.mass[c], cc[0] - cx, cc[1] - cy, cc[2] - cz);
-                    for (int k = 0; k < 6; ++k) q[k] += t.quad[6 * (size_t)c + k];
+                    quad_add(q, ch.mass(c), ch.com(c, 0) - cx, ch.com(c, 1) - cy, ch.com(c, 2) - cz);
+                    for (int k = 0; k < 6; ++k) q[k] += ch.quad(c, k);
                 }
             }
         }
@@ -543,6 +574,12 @@ __device__ void moments_of(uint32_t p, int intended, const TreeArgs& t, const do
                 }
             }
         }
+        Payload out;
+        out.M = M;
+        out.cx = cx; out.cy = cy; out.cz = cz;
+        for (int k = 0; k < 6; ++k) out.q[k] = q[k];
+        out.first = first;
+        return out;
     }
 }
 
@@ -564,6 +601,8 @@ __global__ __launch_bounds__(kBlock) void k_moments_local(const Counters* __rest
                                                           const uint32_t* __restrict__ tperm,
                                                           uint32_t* __restrict__ rest, uint32_t* __restrict__ nrest) {
     __shared__ uint32_t s_levels;
+    __shared__ double s_m[kBlock], s_c[kBlock][3], s_q[kBlock][6];
+    __shared__ uint32_t s_f[kBlock];
     if (ctr->node_overflow) return;   // the node table is incomplete: nothing to combine
     const uint32_t total = ctr->total_nodes;
     const uint32_t b0 = blockIdx.x * kBlock;
@@ -583,9 +622,17 @@ __global__ __launch_bounds__(kBlock) void k_moments_local(const Counters* __rest
     }
     __syncthreads();
     const uint32_t levels = s_levels;
+    const ChildFromSlice slice{s_m, s_c, s_f, s_q, b0};
     for (int l = kMaxLevel; l >= 0; --l) {
         if (!((levels >> l) & 1u)) continue;   // block-uniform
-        if (local && (int)L == l) moments_of(p, intended, t, bounds4, sv, px, py, pz, tx, ty, tz, tm, tperm);
+        if (local && (int)L == l) {
+            const Payload r = moments_of(p, intended, t, bounds4, sv, px, py, pz, tx, ty, tz, tm, tperm, slice);
+            const uint32_t i = threadIdx.x;
+            s_m[i] = r.M;
+            s_c[i][0] = r.cx; s_c[i][1] = r.cy; s_c[i][2] = r.cz;
+            for (int k = 0; k < 6; ++k) s_q[i][k] = r.q[k];
+            s_f[i] = r.first;
+        }
         __syncthreads();
     }
 }
@@ -603,7 +650,8 @@ __global__ __launch_bounds__(1024) void k_moments_rest(const Counters* __restric
         const uint32_t m = nrest[l];
         if (m == 0) continue;   // block-uniform
         for (uint32_t i = threadIdx.x; i < m; i += blockDim.x)
-            moments_of(rest[l * (size_t)slices + i], intended, t, bounds4, sv, px, py, pz, tx, ty, tz, tm, tperm);
+            (void)moments_of(rest[l * (size_t)slices + i], intended, t, bounds4, sv, px, py, pz, tx, ty, tz, tm, tperm,
+                             ChildFromGlobal{t});
         __syncthreads();
     }
 }
