@@ -269,6 +269,7 @@ int bhx_load(bhx_ctx* ctx, const char* path, double* time, uint64_t* step) {
 
 int bhx_vtk_write_bodies(const char* path, const double* pos, const double* vel, const double* mass, size_t n) {
     if (!path || (n && !pos)) return BHX_E_NULL;
+    if (n > (size_t)INT32_MAX / 2) return BHX_E_ARG;   // the format's int32 vertex indices
     FileCloser fc;
     fc.f = std::fopen(path, "wb");
     if (!fc.f) return BHX_E_IO;
@@ -307,6 +308,7 @@ int bhx_vtk_write_nodes(const char* path, const bhx_node* nodes, size_t count, i
     for (size_t i = 0; i < count; ++i)
         if (nodes[i].level >= min_level && nodes[i].level <= max_level) sel.push_back(&nodes[i]);
     const size_t m = sel.size();
+    if (m > (size_t)INT32_MAX / 9) return BHX_E_ARG;   // the format's int32 point indices and cell list
     FileCloser fc;
     fc.f = std::fopen(path, "wb");
     if (!fc.f) return BHX_E_IO;
