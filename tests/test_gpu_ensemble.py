@@ -65,6 +65,26 @@ def test_ensemble_steps_equal_looped_systems_bitwise(bhx, name):
         assert np.array_equal(A[a:b], a1), "accelerations of system %d" % k
 
 
+@pytest.mark.parametrize("prec", [0, 1])
+def test_repeated_walks_cover_every_group(bhx, prec):
+    """Every walk group of a ragged ensemble is walked on every call, with the same
+    result: the accelerations are overwritten with NaN between walks."""
+    parts, pos, vel, mass, off = _systems(bhx, SIZES + [20000], seed0=90)
+    cfg = dict(theta=0.5, eps=0.01, leaf_capacity=16, mode=1, precision=prec, quadrupole=1, group_walk=1)
+    print("repeated walks prec=%d n=%d" % (prec, pos.shape[0]), flush=True)
+    with bhx.Engine(pos.shape[0], **cfg) as e:
+        e.set_bodies(pos, vel, mass)
+        e.set_systems(off)
+        C, S = e.system_bounds()
+        e.build_systems(C, S)
+        ref = e.walk()
+        assert np.all(np.isfinite(ref))
+        nan = np.full_like(ref, np.nan)
+        for _ in range(3):
+            e.set_accelerations(nan)
+            assert np.array_equal(e.walk(), ref)
+
+
 def test_ensemble_matches_oracle_frames(bhx):
     """As-run frames (leaf capacity 1, FP64 per-body walk) of every system against the oracle."""
     sizes = [300, 1, 77, 512]
