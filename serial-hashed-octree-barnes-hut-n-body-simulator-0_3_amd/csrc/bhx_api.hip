@@ -71,7 +71,7 @@ void drop_graphs(bhx_ctx* c) {
 void free_tree(bhx_ctx* c) {
     DeviceTree& t = c->tree;
     void* ptrs[] = {t.key, t.start, t.count, t.skip, t.parent, t.child_byte, t.level, t.first, t.center,
-                    t.com, t.mass, t.quad, t.quadf, t.walk, t.walk32, t.child, t.by_level, t.lvl, t.hkey, t.hval};
+                    t.com, t.mass, t.quad, t.quadf, t.walk, t.walk32, t.pay32, t.child, t.by_level, t.lvl, t.hkey, t.hval};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     t = DeviceTree{};
@@ -97,6 +97,7 @@ hipError_t alloc_tree(bhx_ctx* c, size_t cap) {
     if ((e = dalloc(&t.quadf, 6 * cap)) != hipSuccess) return e;
     if ((e = dalloc(&t.walk, cap)) != hipSuccess) return e;
     if ((e = dalloc(&t.walk32, cap)) != hipSuccess) return e;
+    if ((e = dalloc(&t.pay32, cap)) != hipSuccess) return e;
     if ((e = dalloc(&t.child, 8 * cap)) != hipSuccess) return e;
     if ((e = dalloc(&t.by_level, cap)) != hipSuccess) return e;
     if ((e = dalloc(&t.lvl, 3 * (kMaxLevel + 1) + 1)) != hipSuccess) return e;

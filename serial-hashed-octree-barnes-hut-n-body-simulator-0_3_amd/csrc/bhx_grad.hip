@@ -176,7 +176,8 @@ struct GradSlot {
     float4* tb4 = nullptr;
     uint32_t* tleaf = nullptr;
     WalkNode* walk = nullptr;
-    WalkNode32* walk32 = nullptr;
+    WalkVisit32* walk32 = nullptr;
+    WalkPay32* pay32 = nullptr;
     uint32_t* parent = nullptr;
     double* com = nullptr;
     double* mass = nullptr;
@@ -251,8 +252,12 @@ GradSlot carve_slot(char*& p, size_t n, size_t nodes, size_t nsys, int precision
     s.tb4 = (float4*)take(16 * n);
     s.tleaf = (uint32_t*)take(4 * n);
     char* w = take(64 * nodes);
-    if (precision == BHX_FP32) s.walk32 = (WalkNode32*)w;
-    else s.walk = (WalkNode*)w;
+    if (precision == BHX_FP32) {
+        s.walk32 = (WalkVisit32*)w;
+        s.pay32 = (WalkPay32*)(w + 32 * nodes);
+    } else {
+        s.walk = (WalkNode*)w;
+    }
     s.parent = (uint32_t*)take(4 * nodes);
     s.com = (double*)take(24 * nodes);
     s.mass = (double*)take(8 * nodes);
@@ -318,8 +323,12 @@ int store_frame(bhx_ctx* c, const GradSlot& d, size_t nodes, int precision, bool
     add_job(J, c->tperm, d.tperm, 4 * n);
     add_job(J, c->tb4, d.tb4, 16 * n);
     add_job(J, c->tleaf, d.tleaf, 4 * n);
-    if (precision == BHX_FP32) add_job(J, c->tree.walk32, d.walk32, 64 * nodes);
-    else add_job(J, c->tree.walk, d.walk, 64 * nodes);
+    if (precision == BHX_FP32) {
+        add_job(J, c->tree.walk32, d.walk32, 32 * nodes);
+        add_job(J, c->tree.pay32, d.pay32, 32 * nodes);
+    } else {
+        add_job(J, c->tree.walk, d.walk, 64 * nodes);
+    }
     add_job(J, c->tree.parent, d.parent, 4 * nodes);
     add_job(J, c->tree.com, d.com, 24 * nodes);
     add_job(J, c->tree.mass, d.mass, 8 * nodes);
@@ -333,6 +342,7 @@ WalkTree tree_of(const GradSlot& d) {
     WalkTree t;
     t.walk = d.walk;
     t.walk32 = d.walk32;
+    t.pay32 = d.pay32;
     t.com = d.com;
     t.tb4 = d.tb4;
     t.tleaf = d.tleaf;

@@ -32,23 +32,29 @@ struct alignas(64) WalkNode {
 };
 static_assert(sizeof(WalkNode) == 64, "walk node is one 64-byte record");
 
-// FP32-walk record, also 64 bytes: the quadrupole travels with the node so an
-// accepted cell needs no second fetch; the cell edge follows from the level.
-// The first 32 bytes are what a visit reads (one scalar load): the centre of
-// mass relative to the system's root centre in FP32 (for the opening test
-// only; evaluated entries take their group-relative FP32 centre from the
-// FP64 centres), the structure, the mass and the cell edge squared.
-struct alignas(64) WalkNode32 {
+// FP32-walk records, two arrays indexed by node.  A visit reads one
+// WalkVisit32 (one 32-byte scalar load; the pre-order traversal moves
+// through consecutive records, two per 64-byte line): the centre of mass
+// relative to the system's root centre in FP32 (for the opening test only;
+// evaluated entries take their group-relative FP32 centre from the FP64
+// centres), the structure and the cell edge squared.  The payload an
+// evaluated entry needs, mass and quadrupole, sits in WalkPay32, gathered
+// lane-parallel when a list is flushed.
+struct alignas(32) WalkVisit32 {
     float cx, cy, cz;       // centre of mass - root centre (FP32)
     uint32_t meta;          // level (bits 0-4) | leaf (bit 5) | min(count, 2^26-1) << 6
     uint32_t skip;
     uint32_t start;         // first body in tree order
-    float mass;
+    uint32_t pad;
     float size2;            // cell edge squared
-    float q[6];             // Qxx Qxy Qxz Qyy Qyz Qzz (0 for one-body cells)
-    uint32_t pad[2];
 };
-static_assert(sizeof(WalkNode32) == 64, "FP32 walk node is one 64-byte record");
+static_assert(sizeof(WalkVisit32) == 32, "FP32 visit record is 32 bytes");
+struct alignas(32) WalkPay32 {
+    float mass;
+    float q[6];             // Qxx Qxy Qxz Qyy Qyz Qzz (0 for one-body cells)
+    uint32_t pad;
+};
+static_assert(sizeof(WalkPay32) == 32, "FP32 payload record is 32 bytes");
 
 struct DeviceTree {
     size_t cap = 0;          // node capacity
@@ -66,7 +72,8 @@ struct DeviceTree {
     double* quad = nullptr;          // 6 per node
     float* quadf = nullptr;          // 6 per node, FP32 copy for the walk
     WalkNode* walk = nullptr;
-    WalkNode32* walk32 = nullptr;
+    WalkVisit32* walk32 = nullptr;
+    WalkPay32* pay32 = nullptr;
     uint32_t* child = nullptr;       // 8 per node: child id by octant (valid where child_byte has the bit)
     uint32_t* by_level = nullptr;    // moments: nodes left for the second phase
     uint32_t* lvl = nullptr;         // moments: their count
@@ -296,7 +303,8 @@ hipError_t launch_direct(bhx_ctx* c, const int64_t* targets, size_t nt, double e
 // The arrays a walk reads: the live tree or a stored copy of one.
 struct WalkTree {
     const WalkNode* walk;
-    const WalkNode32* walk32;
+    const WalkVisit32* walk32;
+    const WalkPay32* pay32;
     const double* com;      // FP64 centres of mass, 3 per node
     const float4* tb4;
     const uint32_t* tleaf;

@@ -212,7 +212,8 @@ struct TreeArgs {
     double* quad;
     float* quadf;
     WalkNode* walk;
-    WalkNode32* walk32;
+    WalkVisit32* walk32;
+    WalkPay32* pay32;
     float4* tb4;
     uint32_t* tleaf;
     double tscale;   // root edge / bounds size (1: reference geometry, 2: enclosing cube)
@@ -542,18 +543,21 @@ __device__ Payload moments_of(uint32_t p, int intended, const TreeArgs& t, const
         w.size2f = (float)(s * s);
         w.massf = (float)M;
         t.walk[p] = w;
-        WalkNode32 w32;
-        w32.cx = (float)(cx - b4n[0]);
-        w32.cy = (float)(cy - b4n[1]);
-        w32.cz = (float)(cz - b4n[2]);
-        w32.meta = (uint32_t)L | (leaf ? 32u : 0u) | ((cnt < (1u << 26) ? cnt : (1u << 26) - 1) << 6);
-        w32.skip = t.skip[p];
-        w32.start = st;
-        w32.mass = (float)M;
-        w32.size2 = (float)(s * s);
-        for (int k = 0; k < 6; ++k) w32.q[k] = cnt > 1 ? (float)q[k] : 0.0f;
-        w32.pad[0] = w32.pad[1] = 0u;
-        t.walk32[p] = w32;
+        WalkVisit32 v32;
+        v32.cx = (float)(cx - b4n[0]);
+        v32.cy = (float)(cy - b4n[1]);
+        v32.cz = (float)(cz - b4n[2]);
+        v32.meta = (uint32_t)L | (leaf ? 32u : 0u) | ((cnt < (1u << 26) ? cnt : (1u << 26) - 1) << 6);
+        v32.skip = t.skip[p];
+        v32.start = st;
+        v32.pad = 0u;
+        v32.size2 = (float)(s * s);
+        t.walk32[p] = v32;
+        WalkPay32 p32;
+        p32.mass = (float)M;
+        for (int k = 0; k < 6; ++k) p32.q[k] = cnt > 1 ? (float)q[k] : 0.0f;
+        p32.pad = 0u;
+        t.pay32[p] = p32;
         if (leaf) {
             // compact FP32 copy of the leaf's bodies: offsets from the leaf's
             // centre of mass (which the walk record carries), and the leaf id
@@ -664,6 +668,7 @@ TreeArgs tree_args(bhx_ctx* c) {
     t.child = d.child;
     t.com = d.com; t.mass = d.mass; t.quad = d.quad; t.quadf = d.quadf; t.walk = d.walk;
     t.walk32 = d.walk32;
+    t.pay32 = d.pay32;
     t.tb4 = c->tb4;
     t.tleaf = c->tleaf;
     t.tscale = c->cfg_tscale();

@@ -49,7 +49,8 @@ __device__ __forceinline__ WalkNode load_node(ConstU64 raw, uint32_t p) {
 
 struct WalkArgs {
     const WalkNode* nodes;
-    const WalkNode32* nodes32;
+    const WalkVisit32* nodes32;   // FP32 visit records
+    const WalkPay32* pay32;       // FP32 payloads (mass, quadrupole)
     const double* com;       // FP64 centres of mass, 3 per node (the FP32 walk's entries)
     const float4* tb4;       // tree order: offset from the leaf's centre of mass, mass
     const uint32_t* tleaf;   // tree order: node id of the body's leaf
@@ -651,10 +652,10 @@ __global__ __launch_bounds__(64) void k_walk_group(WalkArgs a) {
     }
 }
 
-// FP32 group walk over the compact layout: 64-byte FP32 records whose first
-// 32 bytes (one scalar burst per visited cell) hold what the opening test
-// needs — the centre relative to the system's root centre in FP32, the edge
-// squared, the structure — and whose rest holds the mass and quadrupole.  An
+// FP32 group walk over the compact layout: 32-byte visit records (one scalar
+// burst per visited cell) hold what the opening test needs — the centre
+// relative to the system's root centre in FP32, the edge squared, the
+// structure — and 32-byte payloads the mass and quadrupole.  An
 // entry's group-relative FP32 centre is formed from the FP64 centres when its
 // list is evaluated; opened leaves are gathered as 16-byte offsets from the
 // leaf's centre of mass plus that centre's offset from the group origin.
@@ -666,21 +667,20 @@ __global__ __launch_bounds__(64) void k_walk_group(WalkArgs a) {
 // with its own L2.  Group g is given to block (g mod per) * 8 + g / per, so
 // XCD x walks the contiguous tree-order slice of groups [x*per, (x+1)*per):
 // its L2 holds that region's cells instead of the whole tree.
-typedef const __attribute__((address_space(4))) WalkNode32* ConstNodes32;
 
 __device__ __forceinline__ uint32_t xcd_group(uint32_t block, uint32_t groups) {
     const uint32_t per = (groups + 7u) / 8u;
     return (block % 8u) * per + block / 8u;
 }
 
-// What a visit reads: the first 32 bytes of the record, one scalar burst.
+// What a visit reads: one 32-byte visit record, one scalar burst.
 struct Node32 {
     float cx, cy, cz, size2;   // centre - root centre; edge squared
     uint32_t meta, skip, start;
 };
 
 __device__ __forceinline__ Node32 load_node32(ConstU64 raw, uint32_t p) {
-    const size_t b = 8 * (size_t)p;
+    const size_t b = 4 * (size_t)p;   // one 32-byte WalkVisit32
     const unsigned long long w0 = raw[b], w1 = raw[b + 1], w2 = raw[b + 2], w3 = raw[b + 3];
     Node32 nd;
     nd.cx = __uint_as_float((uint32_t)w0);
@@ -743,14 +743,14 @@ constexpr int bodies32() { return kAdj ? 128 : 96; }
 // group-relative FP32 entries the evaluation reads.
 template <bool kQuad>
 __device__ __forceinline__ void gather_cells32(float (*nd)[4], float (*q)[6], const uint32_t* nid, int nn,
-                                               uint32_t lane, const WalkNode32* __restrict__ nodes,
+                                               uint32_t lane, const WalkPay32* __restrict__ pay,
                                                const double* __restrict__ com, double ox, double oy, double oz) {
 #pragma unroll
     for (int h = 0; h < (kCells32 + 63) / 64; ++h) {
         const int e = (int)lane + 64 * h;
         if (e < nn) {
             const uint32_t id = nid[e];
-            const WalkNode32& c = nodes[id];
+            const WalkPay32& c = pay[id];
             nd[e][0] = (float)(com[3 * (size_t)id] - ox);
             nd[e][1] = (float)(com[3 * (size_t)id + 1] - oy);
             nd[e][2] = (float)(com[3 * (size_t)id + 2] - oz);
@@ -825,7 +825,7 @@ __global__ __launch_bounds__(64) void k_walk_group32(WalkArgs a) {
     }
     auto flush_cells = [&](int nn) {
         __syncthreads();
-        gather_cells32<kQuad>(s_nd, s_q, s_nid, nn, lane, a.nodes32, a.com, ox, oy, oz);
+        gather_cells32<kQuad>(s_nd, s_q, s_nid, nn, lane, a.pay32, a.com, ox, oy, oz);
         __syncthreads();
         if constexpr (kAdj) {
             adj_self<float>(s_nd, nn, xr, yr, zr, wxr, wyr, wzr, eps2, ax, ay, az);
@@ -1073,6 +1073,7 @@ hipError_t launch_walk_to(bhx_ctx* c, hipStream_t s, bool group, int precision, 
     WalkArgs a{};
     a.nodes = c->tree.walk;
     a.nodes32 = c->tree.walk32;
+    a.pay32 = c->tree.pay32;
     a.com = c->tree.com;
     a.tb4 = c->tb4;
     a.tleaf = c->tleaf;
@@ -1125,6 +1126,7 @@ hipError_t launch_walk_potential(bhx_ctx* c, hipStream_t s, double* ox, double* 
     WalkArgs a{};
     a.nodes = c->tree.walk;
     a.nodes32 = c->tree.walk32;
+    a.pay32 = c->tree.pay32;
     a.com = c->tree.com;
     a.tb4 = c->tb4;
     a.tleaf = c->tleaf;
@@ -1169,6 +1171,7 @@ hipError_t launch_walk_adjoint(bhx_ctx* c, const WalkTree& t, int precision, con
     WalkArgs a{};
     a.nodes = t.walk;
     a.nodes32 = t.walk32;
+    a.pay32 = t.pay32;
     a.com = t.com;
     a.tb4 = t.tb4;
     a.tleaf = t.tleaf;
