@@ -698,16 +698,15 @@ __device__ __forceinline__ void eval_list32(const float (*nd)[4], const float (*
                                             float zr, float eps2, float& ax, float& ay, float& az, float& phi) {
     for (int k = 0; k < nn; ++k) {
         const float dx = nd[k][0] - xr, dy = nd[k][1] - yr, dz = nd[k][2] - zr;
-        float inv;
+        float inv, D2 = 0.f;
         if (kGuard) {
             const float d2 = dx * dx + dy * dy + dz * dz;
             inv = d2 > 0.f ? __builtin_amdgcn_rsqf(d2 + eps2) : 0.f;
         } else {
-            inv = __builtin_amdgcn_rsqf(fmaf(dz, dz, fmaf(dy, dy, fmaf(dx, dx, eps2))));
+            D2 = fmaf(dz, dz, fmaf(dy, dy, fmaf(dx, dx, eps2)));
+            inv = __builtin_amdgcn_rsqf(D2);
         }
         const float inv2 = inv * inv;
-        const float inv3 = inv * inv2;
-        float c = nd[k][3] * inv3;
         float pt = kPot ? nd[k][3] * inv : 0.f;
         if (kQuad) {
             const float q0 = q[k][0], q1 = q[k][1], q2 = q[k][2], q3 = q[k][3], q4 = q[k][4], q5 = q[k][5];
@@ -715,16 +714,33 @@ __device__ __forceinline__ void eval_list32(const float (*nd)[4], const float (*
             const float qy = fmaf(q4, dz, fmaf(q3, dy, q1 * dx));
             const float qz = fmaf(q5, dz, fmaf(q4, dy, q2 * dx));
             const float dqd = fmaf(dz, qz, fmaf(dy, qy, dx * qx));
-            const float inv5 = inv3 * inv2;
-            c = fmaf(2.5f * dqd, inv5 * inv2, c);
-            ax = fmaf(-inv5, qx, ax);
-            ay = fmaf(-inv5, qy, ay);
-            az = fmaf(-inv5, qz, az);
-            if (kPot) pt = fmaf(0.5f * dqd, inv5, pt);
+            if (!kGuard) {
+                // softened: a += r^-5 (d (M r^2 + 2.5 (d.Q.d) r^-2) - Q.d) with
+                // r^2 = |d|^2 + eps^2, the monopole and both quadrupole terms
+                // over one r^-5 factor (one multiply fewer per cell)
+                const float inv5 = inv * (inv2 * inv2);
+                const float cc = fmaf(2.5f * dqd, inv2, nd[k][3] * D2);
+                ax = fmaf(inv5, fmaf(dx, cc, -qx), ax);
+                ay = fmaf(inv5, fmaf(dy, cc, -qy), ay);
+                az = fmaf(inv5, fmaf(dz, cc, -qz), az);
+                if (kPot) pt = fmaf(0.5f * dqd, inv5, pt);
+            } else {
+                const float inv3 = inv * inv2, inv5 = inv3 * inv2;
+                const float c = fmaf(2.5f * dqd, inv5 * inv2, nd[k][3] * inv3);
+                ax = fmaf(-inv5, qx, ax);
+                ay = fmaf(-inv5, qy, ay);
+                az = fmaf(-inv5, qz, az);
+                ax = fmaf(c, dx, ax);
+                ay = fmaf(c, dy, ay);
+                az = fmaf(c, dz, az);
+                if (kPot) pt = fmaf(0.5f * dqd, inv5, pt);
+            }
+        } else {
+            const float c = nd[k][3] * (inv * inv2);
+            ax = fmaf(c, dx, ax);
+            ay = fmaf(c, dy, ay);
+            az = fmaf(c, dz, az);
         }
-        ax = fmaf(c, dx, ax);
-        ay = fmaf(c, dy, ay);
-        az = fmaf(c, dz, az);
         if (kPot) phi -= (dx * dx + dy * dy + dz * dz) > 0.f ? pt : 0.f;
     }
 }
