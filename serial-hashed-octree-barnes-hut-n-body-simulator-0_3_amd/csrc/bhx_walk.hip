@@ -754,18 +754,18 @@ constexpr int kRanges32 = 32;
 template <bool kAdj>
 constexpr int bodies32() { return kAdj ? 128 : 96; }
 
-// Accepted cells of the FP32 walk are queued as node ids only; a flush
-// gathers them lane-parallel (lane j fetches entry j's record) into the
+// Accepted cells of the FP32 walk are queued as node ids only, one per lane
+// of a vector register (lane j holds entry j: kCells32 is the wave width); a
+// flush gathers them lane-parallel (lane j fetches entry j's record) into the
 // group-relative FP32 entries the evaluation reads.
+static_assert(kCells32 == 64, "the FP32 cell list is one id per lane");
 template <bool kQuad>
-__device__ __forceinline__ void gather_cells32(float (*nd)[4], float (*q)[6], const uint32_t* nid, int nn,
+__device__ __forceinline__ void gather_cells32(float (*nd)[4], float (*q)[6], uint32_t id, int nn,
                                                uint32_t lane, const WalkPay32* __restrict__ pay,
                                                const double* __restrict__ com, double ox, double oy, double oz) {
-#pragma unroll
-    for (int h = 0; h < (kCells32 + 63) / 64; ++h) {
-        const int e = (int)lane + 64 * h;
+    {
+        const int e = (int)lane;
         if (e < nn) {
-            const uint32_t id = nid[e];
             const WalkPay32& c = pay[id];
             nd[e][0] = (float)(com[3 * (size_t)id] - ox);
             nd[e][1] = (float)(com[3 * (size_t)id + 1] - oy);
@@ -839,9 +839,13 @@ __global__ __launch_bounds__(64) void k_walk_group32(WalkArgs a) {
         s_tg[lane][0] = xr; s_tg[lane][1] = yr; s_tg[lane][2] = zr;
         s_tg[lane][3] = wxr; s_tg[lane][4] = wyr; s_tg[lane][5] = wzr;
     }
+    uint32_t cell_ids = 0;   // lane j: node id of queued cell j
     auto flush_cells = [&](int nn) {
+        if constexpr (kAdj) {
+            if ((int)lane < nn) s_nid[lane] = cell_ids;
+        }
         __syncthreads();
-        gather_cells32<kQuad>(s_nd, s_q, s_nid, nn, lane, a.pay32, a.com, ox, oy, oz);
+        gather_cells32<kQuad>(s_nd, s_q, cell_ids, nn, lane, a.pay32, a.com, ox, oy, oz);
         __syncthreads();
         if constexpr (kAdj) {
             adj_self<float>(s_nd, nn, xr, yr, zr, wxr, wyr, wzr, eps2, ax, ay, az);
@@ -931,11 +935,17 @@ __global__ __launch_bounds__(64) void k_walk_group32(WalkArgs a) {
             const float d2 = dx * dx + dy * dy + dz * dz;
             const bool opened = (__builtin_amdgcn_ballot_w64(!(nd.size2 < th2 * d2)) & vmask) != 0ull;
             if (!opened) {
-                if (lane == 0) s_nid[nn] = p;
+                cell_ids = lane == (uint32_t)nn ? p : cell_ids;
                 ++nn;
                 p = nskip;
-                if (nn == kCells32) break;
-            } else if (nd.meta & 32u) {
+                if (nn == kCells32 || p >= end) break;
+                continue;
+            }
+            if (!(nd.meta & 32u)) {
+                p = p + 1;   // first child: inside the subtree, before end
+                continue;
+            }
+            {
                 const uint32_t count = nd.meta >> 6;
                 const int m = (int)((count - kres) < 64u ? (count - kres) : 64u);
                 if (nq + m > kBodies32 || nr == kRanges32) {
@@ -954,8 +964,6 @@ __global__ __launch_bounds__(64) void k_walk_group32(WalkArgs a) {
                     kres = 0;
                     p = nskip;
                 }
-            } else {
-                p = p + 1;
             }
             if (p >= end) break;
         }
