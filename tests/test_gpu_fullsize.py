@@ -1,0 +1,116 @@
+"""The workloads at their full sizes, checked through properties that do not
+depend on size: the 100-frame CPU-runnable configuration against the oracle's
+frames (16,384 bodies), the precision configuration (1,048,576 bodies, 50
+steps) through conservation, tree invariants and force accuracy, and the
+512-system ensemble (about 7.8 million bodies) against its systems stepped one
+at a time."""
+import numpy as np
+import pytest
+
+from oracle import oracle as orc
+from tests.helpers import load_bhx
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def bhx():
+    return load_bhx()
+
+
+def test_plummer_16k_100_frames_match_oracle(bhx):
+    """Leaf capacity 1, as-run payload, FP64 per-body walk, dt 1e-3, theta 0.5,
+    eps 0.01: all 100 frames of the CPU-runnable configuration."""
+    n, frames = 16384, 100
+    dt, theta, eps = 1e-3, 0.5, 0.01
+    pos, vel, mass = bhx.plummer(n, seed=1)
+    print("16k x 100 frames", flush=True)
+    with bhx.Engine(n, theta=theta, eps=eps, dt=dt) as e:
+        e.set_bodies(pos, vel, mass)
+        e.step(frames)
+        gp, gv, gm, ga = e.get_bodies()
+    p, v, m, a = pos, vel, mass, np.zeros_like(pos)
+    for _ in range(frames):
+        p, v, m, a, _, _ = orc.frame(p, v, m, a, dt, theta, eps, permute_acc=True)
+    # match bodies by position (the walks sum the same terms in another order,
+    # so a body at a key boundary could in principle take the other side);
+    # measured: 4.4e-16 in positions and velocities, 1.7e-14 in accelerations
+    gi, oi = np.argsort(gp[:, 0], kind="stable"), np.argsort(p[:, 0], kind="stable")
+    dp = np.abs(gp[gi] - p[oi]).max()
+    dv = np.abs(gv[gi] - v[oi]).max()
+    da = (np.linalg.norm(ga[gi] - a[oi], axis=1) / np.linalg.norm(a[oi], axis=1)).max()
+    print("max |dx| %.3e |dv| %.3e rel |da| %.3e" % (dp, dv, da), flush=True)
+    assert np.array_equal(np.sort(gm), np.sort(m))
+    assert dp < 1e-13 and dv < 1e-13 and da < 1e-11
+
+
+def test_plummer_1m_fp32_50_steps(bhx):
+    n, steps = 1 << 20, 50
+    cfg = dict(theta=0.5, eps=0.01, dt=5e-4, leaf_capacity=32, mode=1, precision=1, quadrupole=1, compensated=1,
+               permute_acc=1, group_walk=1)
+    pos, vel, mass = bhx.plummer(n, seed=1)
+    p0 = (mass[:, None] * vel).sum(0)
+    print("1M x %d steps" % steps, flush=True)
+    with bhx.Engine(n, **cfg) as e:
+        e.set_bodies(pos, vel, mass)
+        K0, W0, _ = e.energy()
+        e.step_graph(steps)
+        p, v, m, a = e.get_bodies()
+        assert np.all(np.isfinite(p)) and np.all(np.isfinite(v)) and np.all(np.isfinite(a))
+        assert np.array_equal(np.sort(m), np.sort(mass))
+        # linear momentum moves by the force error only (tree forces are not pairwise antisymmetric)
+        scale = (m * np.linalg.norm(v, axis=1)).sum()
+        assert np.abs((m[:, None] * v).sum(0) - p0).max() < 1e-5 * scale
+        K1, W1, _ = e.energy()
+        rel = ((K1 + W1) - (K0 + W0)) / abs(K0 + W0)
+        print("energy change over %d steps %.3e" % (steps, rel), flush=True)
+        assert abs(rel) < 1e-5
+        # the tree of the final bodies
+        c, s = e.bounds()
+        e.build(c, s)
+        nodes = e.nodes()
+        assert nodes["n"][0] == n and nodes[nodes["level"] == 1]["n"].sum() == n
+        tp, tk = e.tree_order()
+        assert np.all(tk[1:] >= tk[:-1])
+        # FP32 walk against the FP64 direct sum on a seeded sample, and against
+        # the FP64 walk on the same tree
+        acc32 = e.walk()
+        tg = np.sort(np.random.default_rng(5).choice(n, 1024, replace=False))
+        dacc, _ = e.direct(tg, cfg["eps"])
+        err = np.linalg.norm(acc32[tg] - dacc, axis=1) / np.linalg.norm(dacc, axis=1)
+        print("force error median %.3e p99 %.3e" % (np.median(err), np.quantile(err, 0.99)), flush=True)
+        assert np.median(err) < 1e-3 and np.quantile(err, 0.99) < 5e-3
+        e.set_config(precision=0)
+        acc64 = e.walk()
+        d = np.linalg.norm(acc32 - acc64, axis=1) / np.linalg.norm(acc64, axis=1)
+        print("FP32 vs FP64 walk median %.3e max %.3e" % (np.median(d), d.max()), flush=True)
+        assert np.median(d) < 1e-5 and d.max() < 1e-3   # measured 8.2e-7, 1.1e-4
+
+
+def test_ensemble_512_systems_equal_single_systems(bhx):
+    """The 512-system ensemble (sizes log-uniform in [1024, 65536], masses and
+    radii jittered): graph frames of the whole ensemble against three of its
+    systems stepped alone, bitwise."""
+    nsys, frames = 512, 2
+    pos, vel, mass, off = bhx.ensemble(nsys, seed=1, threads=16)
+    n = pos.shape[0]
+    sizes = np.diff(off).astype(np.int64)
+    cfg = dict(theta=0.5, eps=0.01, dt=5e-4, leaf_capacity=32, mode=1, precision=1, quadrupole=1, compensated=1,
+               group_walk=1)
+    print("ensemble %d systems %d bodies" % (nsys, n), flush=True)
+    with bhx.Engine(n, **cfg) as e:
+        e.set_bodies(pos, vel, mass)
+        e.set_systems(off)
+        e.step_graph(frames)
+        P, V, M, A = e.get_bodies()
+    assert np.all(np.isfinite(P)) and np.all(np.isfinite(A))
+    for k in (int(np.argmin(sizes)), int(np.argmax(sizes)), nsys // 2):
+        lo, hi = int(off[k]), int(off[k + 1])
+        print("system %d (%d bodies)" % (k, hi - lo), flush=True)
+        assert np.array_equal(np.sort(M[lo:hi]), np.sort(mass[lo:hi]))
+        with bhx.Engine(hi - lo, **cfg) as s:
+            s.set_bodies(pos[lo:hi], vel[lo:hi], mass[lo:hi])
+            s.step(frames)
+            p1, v1, m1, a1 = s.get_bodies()
+        assert np.array_equal(P[lo:hi], p1) and np.array_equal(V[lo:hi], v1)
+        assert np.array_equal(M[lo:hi], m1) and np.array_equal(A[lo:hi], a1)
