@@ -786,7 +786,7 @@ __global__ __launch_bounds__(64) void k_walk_group32(WalkArgs a) {
     constexpr int kBodies32 = bodies32<kAdj>();
     __shared__ float s_nd[kCells32][4];
     __shared__ float s_q[kQuad ? kCells32 : 1][6];
-    __shared__ uint32_t s_nid[kCells32];
+    __shared__ uint32_t s_nid[kAdj ? kCells32 : 1];   // the adjoint's scatter reads the ids from LDS
     __shared__ float s_bd[kBodies32][4];
     __shared__ uint32_t s_rstart[kRanges32], s_roff[kRanges32];
     __shared__ float s_rdel[kRanges32][3];    // leaf's centre minus the group origin
