@@ -375,8 +375,11 @@ __device__ __forceinline__ void eval_nodes(const T (*nd)[4], const T (*q)[6], in
         const T d2 = dist2(dx, dy, dz);
         const T r2 = d2 + eps2;
         T inv = rsqrt_t<T>(r2);
-        if (kGuard) inv = d2 > (T)0 ? inv : (T)0;
-        const T inv2 = inv_sq_t<T>(inv, r2);
+        T inv2 = inv_sq_t<T>(inv, r2);
+        if (kGuard) {   // both factors: in FP64 1/r2 comes from r2, which is 0 here
+            inv = d2 > (T)0 ? inv : (T)0;
+            inv2 = d2 > (T)0 ? inv2 : (T)0;
+        }
         const T inv3 = inv * inv2;
         const T f = nd[k][3] * inv3;
         ax += f * dx;
@@ -408,8 +411,12 @@ __device__ __forceinline__ void eval_bodies(const T (*bd)[4], int nb, T xr, T yr
         const T d2 = dist2(dx, dy, dz);
         const T r2 = d2 + eps2;
         T inv = rsqrt_t<T>(r2);
-        if (kGuard) inv = d2 > (T)0 ? inv : (T)0;
-        const T inv3 = inv * inv_sq_t<T>(inv, r2);
+        T inv2 = inv_sq_t<T>(inv, r2);
+        if (kGuard) {   // both factors: in FP64 1/r2 comes from r2, which is 0 here
+            inv = d2 > (T)0 ? inv : (T)0;
+            inv2 = d2 > (T)0 ? inv2 : (T)0;
+        }
+        const T inv3 = inv * inv2;
         const T f = bd[k][3] * inv3;
         ax += f * dx;
         ay += f * dy;

@@ -305,6 +305,31 @@ def test_group_walk_theta_zero_is_direct_sum(bhx, prec):
     assert err.max() < (1e-12 if prec == 0 else 1e-4)      # FP64: summation order; FP32: rounding
 
 
+@pytest.mark.parametrize("quad", [0, 1])
+@pytest.mark.parametrize("prec", [0, 1])
+def test_group_walk_unsoftened(bhx, prec, quad):
+    """eps = 0 takes the walks' guarded form (a body meets itself at d = 0 and adds
+    nothing): theta = 0 equals the unsoftened direct sum, and theta = 0.5 on the same
+    tree stays as accurate as the softened walk."""
+    n = 3000
+    pos, vel, mass = bhx.plummer(n, seed=29)
+    c, s = orc.bounds(pos)
+    with _engine(bhx, n, theta=0.0, eps=0.0, group_walk=1, precision=prec, leaf_capacity=8, mode=1,
+                 quadrupole=quad) as e:
+        e.set_bodies(pos, vel, mass)
+        e.build(c, s)
+        acc0 = e.walk()
+        e.set_config(theta=0.5)
+        acc5 = e.walk()
+    ref, _ = orc.direct(pos, mass, 0.0)
+    e0 = np.linalg.norm(acc0 - ref, axis=1) / np.linalg.norm(ref, axis=1)
+    e5 = np.linalg.norm(acc5 - ref, axis=1) / np.linalg.norm(ref, axis=1)
+    print("eps 0: theta 0 max %.3e, theta 0.5 median %.3e" % (e0.max(), np.median(e5)), flush=True)
+    assert np.all(np.isfinite(acc0)) and np.all(np.isfinite(acc5))
+    assert e0.max() < (1e-12 if prec == 0 else 1e-4)      # FP64: summation order; FP32: rounding
+    assert np.median(e5) < 1e-3
+
+
 @pytest.mark.parametrize("prec", [0, 1])
 def test_group_walk_is_at_least_as_accurate(bhx, plummer16k, prec):
     pos, vel, mass = plummer16k
