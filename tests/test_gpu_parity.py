@@ -305,16 +305,17 @@ def test_group_walk_theta_zero_is_direct_sum(bhx, prec):
     assert err.max() < (1e-12 if prec == 0 else 1e-4)      # FP64: summation order; FP32: rounding
 
 
+@pytest.mark.parametrize("group", [1, 0])
 @pytest.mark.parametrize("quad", [0, 1])
 @pytest.mark.parametrize("prec", [0, 1])
-def test_group_walk_unsoftened(bhx, prec, quad):
+def test_walk_unsoftened(bhx, prec, quad, group):
     """eps = 0 takes the walks' guarded form (a body meets itself at d = 0 and adds
     nothing): theta = 0 equals the unsoftened direct sum, and theta = 0.5 on the same
-    tree stays as accurate as the softened walk."""
+    tree stays as accurate as the softened walk (group and per-body walks)."""
     n = 3000
     pos, vel, mass = bhx.plummer(n, seed=29)
     c, s = orc.bounds(pos)
-    with _engine(bhx, n, theta=0.0, eps=0.0, group_walk=1, precision=prec, leaf_capacity=8, mode=1,
+    with _engine(bhx, n, theta=0.0, eps=0.0, group_walk=group, precision=prec, leaf_capacity=8, mode=1,
                  quadrupole=quad) as e:
         e.set_bodies(pos, vel, mass)
         e.build(c, s)
@@ -327,7 +328,9 @@ def test_group_walk_unsoftened(bhx, prec, quad):
     print("eps 0: theta 0 max %.3e, theta 0.5 median %.3e" % (e0.max(), np.median(e5)), flush=True)
     assert np.all(np.isfinite(acc0)) and np.all(np.isfinite(acc5))
     assert e0.max() < (1e-12 if prec == 0 else 1e-4)      # FP64: summation order; FP32: rounding
-    assert np.median(e5) < 1e-3
+    # the per-body walk is the reference's own criterion, coarser than the shared
+    # group lists (measured at theta 0.5, monopole: 2.0e-3 against 6.3e-4)
+    assert np.median(e5) < (1e-3 if group else 5e-3)
 
 
 @pytest.mark.parametrize("prec", [0, 1])
