@@ -125,7 +125,7 @@ void free_all(bhx_ctx* c) {
         if (p) { (void)hipFree(p); p = nullptr; }
     void* vp[] = {c->keys, c->keys_alt, c->vals, c->vals_alt, c->mkeys, c->tkeys, c->tperm, c->nn, c->noff,
                   c->nlo, c->scan_tmp, c->sort_hist, c->sort_status, c->sort_tile_ctr, c->wcount, c->ctr, c->tb4, c->tleaf,
-                  c->sys_off, c->sysid, c->sys_root, c->gtab, c->skey, c->skey_alt, c->sidx, c->sidx_alt};
+                  c->sys_off, c->sysid, c->sys_root, c->sys_scale, c->gtab, c->skey, c->skey_alt, c->sidx, c->sidx_alt};
     for (void* p : vp)
         if (p) (void)hipFree(p);
     if (c->ctr_host) (void)hipHostFree(c->ctr_host);
@@ -152,22 +152,26 @@ hipError_t ensure_sys_cap(bhx_ctx* c, size_t nsys) {
     if (nsys <= c->sys_cap && c->bounds) return hipSuccess;
     size_t cap = c->sys_cap ? c->sys_cap : 1;
     while (cap < nsys) cap *= 2;
-    double* nb = nullptr;
+    double *nb = nullptr, *ns = nullptr;
     uint32_t *no = nullptr, *nr = nullptr;
     hipError_t e = dalloc(&nb, 12 * cap);
     if (e == hipSuccess) e = dalloc(&no, cap + 1);
     if (e == hipSuccess) e = dalloc(&nr, cap);
+    if (e == hipSuccess) e = dalloc(&ns, 2 * cap);
     if (e != hipSuccess) {
         if (nb) (void)hipFree(nb);
         if (no) (void)hipFree(no);
+        if (nr) (void)hipFree(nr);
         return e;
     }
     if (c->bounds) (void)hipFree(c->bounds);
     if (c->sys_off) (void)hipFree(c->sys_off);
     if (c->sys_root) (void)hipFree(c->sys_root);
+    if (c->sys_scale) (void)hipFree(c->sys_scale);
     c->bounds = nb;
     c->sys_off = no;
     c->sys_root = nr;
+    c->sys_scale = ns;
     c->sys_cap = cap;
     c->bounds_valid = false;
     return hipSuccess;

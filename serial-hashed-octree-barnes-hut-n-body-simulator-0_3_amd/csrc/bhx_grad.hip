@@ -183,6 +183,7 @@ struct GradSlot {
     double* mass = nullptr;
     uint32_t* sys_root = nullptr;
     double* b4 = nullptr;
+    double* scale32 = nullptr;
 };
 
 struct GradRecord {
@@ -233,7 +234,7 @@ size_t align_up(size_t v) { return (v + 255) & ~(size_t)255; }
 size_t slot_bytes(size_t n, size_t nodes, size_t nsys) {
     return align_up(4 * n) + 4 * align_up(8 * n) + align_up(4 * n) + align_up(16 * n) + align_up(4 * n) +
            align_up(64 * nodes) + align_up(4 * nodes) + align_up(24 * nodes) + align_up(8 * nodes) +
-           align_up(4 * nsys) + align_up(32 * nsys);
+           align_up(4 * nsys) + align_up(32 * nsys) + align_up(16 * nsys);
 }
 
 GradSlot carve_slot(char*& p, size_t n, size_t nodes, size_t nsys, int precision) {
@@ -263,6 +264,7 @@ GradSlot carve_slot(char*& p, size_t n, size_t nodes, size_t nsys, int precision
     s.mass = (double*)take(8 * nodes);
     s.sys_root = (uint32_t*)take(4 * nsys);
     s.b4 = (double*)take(32 * nsys);
+    s.scale32 = (double*)take(16 * nsys);
     return s;
 }
 
@@ -334,6 +336,7 @@ int store_frame(bhx_ctx* c, const GradSlot& d, size_t nodes, int precision, bool
     add_job(J, c->tree.mass, d.mass, 8 * nodes);
     add_job(J, c->sys_root, d.sys_root, 4 * c->nsys);
     add_job(J, c->bounds_slot(c->tree_slot), d.b4, 32 * c->nsys);
+    add_job(J, c->sys_scale, d.scale32, 16 * c->nsys);
     HIPCHK(run_jobs(J, 16 * (n > nodes ? n : nodes), s));
     return BHX_OK;
 }
@@ -343,6 +346,7 @@ WalkTree tree_of(const GradSlot& d) {
     t.walk = d.walk;
     t.walk32 = d.walk32;
     t.pay32 = d.pay32;
+    t.scale32 = d.scale32;
     t.com = d.com;
     t.tb4 = d.tb4;
     t.tleaf = d.tleaf;

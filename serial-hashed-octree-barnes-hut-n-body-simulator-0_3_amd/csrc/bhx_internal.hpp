@@ -107,6 +107,19 @@ struct SysView {
 
 __device__ __forceinline__ uint32_t sys_of(const SysView& v, uint32_t i) { return v.id ? v.id[i] : 0u; }
 
+// The FP32 walks work in scaled units: a system's lengths times lam (the power
+// of two nearest below the inverse root edge) and masses times mu (the same
+// for its first body's mass).  Power-of-two scalings are exact, so wherever
+// the unscaled FP32 values stay in range every rounding and every opening
+// decision is unchanged; in any units r^-5, r^-7 and the mass products then
+// stay in range.  Results go back to physical units in FP64.
+__device__ __forceinline__ double pow2_inverse_near(double v) {
+    if (!(v > 0.0) || !(v < 1.0e300)) return 1.0;
+    int e = ilogb(v);
+    e = e < -1000 ? -1000 : (e > 1000 ? 1000 : e);
+    return ldexp(1.0, -e);
+}
+
 
 }  // namespace bhx
 
@@ -181,6 +194,7 @@ struct bhx_ctx {
     uint32_t* sys_off = nullptr;               // device copy
     uint32_t* sysid = nullptr;                 // system of each body position (null for one system)
     uint32_t* sys_root = nullptr;              // root node of each system in the current tree
+    double* sys_scale = nullptr;               // per system: FP32 walk length and mass factors
     uint4* gtab = nullptr;                     // walk groups: {first body, end body, system, 0}
     size_t gtab_cap = 0;
     uint32_t ngroups = 0;
@@ -305,6 +319,7 @@ struct WalkTree {
     const WalkNode* walk;
     const WalkVisit32* walk32;
     const WalkPay32* pay32;
+    const double* scale32;  // per system: FP32 length and mass factors
     const double* com;      // FP64 centres of mass, 3 per node
     const float4* tb4;
     const uint32_t* tleaf;

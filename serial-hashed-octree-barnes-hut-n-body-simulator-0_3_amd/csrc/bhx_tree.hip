@@ -216,6 +216,7 @@ struct TreeArgs {
     WalkPay32* pay32;
     float4* tb4;
     uint32_t* tleaf;
+    const double* scale32;   // per system: FP32 walk length and mass factors
     double tscale;   // root edge / bounds size (1: reference geometry, 2: enclosing cube)
 };
 
@@ -525,11 +526,15 @@ __device__ Payload moments_of(uint32_t p, int intended, const TreeArgs& t, const
         t.com[3 * (size_t)p] = cx;
         t.com[3 * (size_t)p + 1] = cy;
         t.com[3 * (size_t)p + 2] = cz;
+        // the FP32 copies in the system's scaled units (exact powers of two)
+        const uint32_t sysn = sys_of(sv, st);
+        const double lam = t.scale32[2 * (size_t)sysn], mu = t.scale32[2 * (size_t)sysn + 1];
+        const double lam2 = lam * lam, qs = mu * lam2;
         for (int k = 0; k < 6; ++k) {
             t.quad[6 * (size_t)p + k] = q[k];
-            t.quadf[6 * (size_t)p + k] = (float)q[k];
+            t.quadf[6 * (size_t)p + k] = (float)(q[k] * qs);
         }
-        const double* b4n = bounds4 + 4 * (size_t)sys_of(sv, st);
+        const double* b4n = bounds4 + 4 * (size_t)sysn;
         double s = b4n[3] * t.tscale;
         for (int l = 0; l < L; ++l) s = s * 0.5;
         WalkNode w;
@@ -540,22 +545,22 @@ __device__ Payload moments_of(uint32_t p, int intended, const TreeArgs& t, const
         w.start = st;
         w.count = cnt;
         w.level_leaf = (uint32_t)L | (leaf ? 256u : 0u);
-        w.size2f = (float)(s * s);
-        w.massf = (float)M;
+        w.size2f = (float)(s * s * lam2);
+        w.massf = (float)(M * mu);
         t.walk[p] = w;
         WalkVisit32 v32;
-        v32.cx = (float)(cx - b4n[0]);
-        v32.cy = (float)(cy - b4n[1]);
-        v32.cz = (float)(cz - b4n[2]);
+        v32.cx = (float)((cx - b4n[0]) * lam);
+        v32.cy = (float)((cy - b4n[1]) * lam);
+        v32.cz = (float)((cz - b4n[2]) * lam);
         v32.meta = (uint32_t)L | (leaf ? 32u : 0u) | ((cnt < (1u << 26) ? cnt : (1u << 26) - 1) << 6);
         v32.skip = t.skip[p];
         v32.start = st;
         v32.pad = 0u;
-        v32.size2 = (float)(s * s);
+        v32.size2 = (float)(s * s * lam2);
         t.walk32[p] = v32;
         WalkPay32 p32;
-        p32.mass = (float)M;
-        for (int k = 0; k < 6; ++k) p32.q[k] = cnt > 1 ? (float)q[k] : 0.0f;
+        p32.mass = (float)(M * mu);
+        for (int k = 0; k < 6; ++k) p32.q[k] = cnt > 1 ? (float)(q[k] * qs) : 0.0f;
         p32.pad = 0u;
         t.pay32[p] = p32;
         if (leaf) {
@@ -572,8 +577,8 @@ __device__ Payload moments_of(uint32_t p, int intended, const TreeArgs& t, const
 #pragma unroll
                 for (int k = 0; k < 8; ++k) {
                     if (j0 + k >= e) break;
-                    t.tb4[j0 + k] = make_float4((float)(xj[k] - cx), (float)(yj[k] - cy), (float)(zj[k] - cz),
-                                                (float)mj[k]);
+                    t.tb4[j0 + k] = make_float4((float)((xj[k] - cx) * lam), (float)((yj[k] - cy) * lam),
+                                                (float)((zj[k] - cz) * lam), (float)(mj[k] * mu));
                     t.tleaf[j0 + k] = p;
                 }
             }
@@ -660,6 +665,17 @@ __global__ __launch_bounds__(1024) void k_moments_rest(const Counters* __restric
     }
 }
 
+// FP32 walk factors of each system: lengths from the tree's root edge, masses
+// from the system's first body (tree order), both powers of two.
+__global__ void k_scale32(const double* __restrict__ bounds4, double tscale, SysView sv,
+                          const double* __restrict__ tm, double* __restrict__ scale) {
+    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= sv.nsys) return;
+    const uint32_t first = sv.off[k];
+    scale[2 * (size_t)k] = pow2_inverse_near(bounds4[4 * (size_t)k + 3] * tscale);
+    scale[2 * (size_t)k + 1] = sv.off[k + 1] > first ? pow2_inverse_near(fabs(tm[first])) : 1.0;
+}
+
 TreeArgs tree_args(bhx_ctx* c) {
     DeviceTree& d = c->tree;
     TreeArgs t;
@@ -669,6 +685,7 @@ TreeArgs tree_args(bhx_ctx* c) {
     t.com = d.com; t.mass = d.mass; t.quad = d.quad; t.quadf = d.quadf; t.walk = d.walk;
     t.walk32 = d.walk32;
     t.pay32 = d.pay32;
+    t.scale32 = c->sys_scale;
     t.tb4 = c->tb4;
     t.tleaf = c->tleaf;
     t.tscale = c->cfg_tscale();
@@ -738,6 +755,8 @@ hipError_t build_tree(bhx_ctx* c, hipStream_t s, size_t* total_out, bool sync) {
     const unsigned slices = grid_for(span, kBlock, 0xFFFFFFFFu);
     if ((size_t)(kMaxLevel + 1) * slices > c->tree.cap) return hipErrorInvalidValue;
     hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, s, nrest, (size_t)(kMaxLevel + 1), 0u);
+    hipLaunchKernelGGL(k_scale32, dim3((sv.nsys + kBlock - 1) / kBlock), dim3(kBlock), 0, s, b4, c->tree_tscale, sv,
+                       c->tm, c->sys_scale);
     hipLaunchKernelGGL(k_moments_local, dim3(slices), dim3(kBlock), 0, s, c->ctr, sv, intended, t, b4, c->px, c->py,
                        c->pz, c->tx, c->ty, c->tz, c->tm, c->tperm, c->tree.by_level, nrest);
     hipLaunchKernelGGL(k_moments_rest, dim3(1), dim3(1024), 0, s, c->ctr, sv, intended, t, b4, c->px, c->py, c->pz,

@@ -51,6 +51,7 @@ struct WalkArgs {
     const WalkNode* nodes;
     const WalkVisit32* nodes32;   // FP32 visit records
     const WalkPay32* pay32;       // FP32 payloads (mass, quadrupole)
+    const double* scale32;        // per system: FP32 length and mass factors (lam, mu)
     const double* com;       // FP64 centres of mass, 3 per node (the FP32 walk's entries)
     const float4* tb4;       // tree order: offset from the leaf's centre of mass, mass
     const uint32_t* tleaf;   // tree order: node id of the body's leaf
@@ -221,8 +222,9 @@ __global__ __launch_bounds__(kBlock) void k_walk32(WalkArgs a) {
     const uint32_t g0 = gr.x;
     const double ox = tx[g0], oy = ty[g0], oz = tz[g0];
     const double x = a.tx[iv], y = a.ty[iv], z = a.tz[iv];
-    const float xr = (float)(x - ox), yr = (float)(y - oy), zr = (float)(z - oz);
-    const float eps2 = a.eps2f, th2 = a.theta2f;
+    const double lam = a.scale32[2 * (size_t)gr.z], mu = a.scale32[2 * (size_t)gr.z + 1];
+    const float xr = (float)((x - ox) * lam), yr = (float)((y - oy) * lam), zr = (float)((z - oz) * lam);
+    const float eps2 = (float)(a.eps2 * lam * lam), th2 = a.theta2f;
     float ax = 0.f, ay = 0.f, az = 0.f;
     uint32_t cnt = 0;
     uint32_t openbits = 1;
@@ -239,7 +241,8 @@ __global__ __launch_bounds__(kBlock) void k_walk32(WalkArgs a) {
         const uint32_t ncount = nodes[p].count;
         const float mf = nodes[p].massf, s2 = nodes[p].size2f;
         const bool active = valid && (L == 0 || ((openbits >> (L - 1)) & 1u));
-        const float dx = (float)(cx - ox) - xr, dy = (float)(cy - oy) - yr, dz = (float)(cz - oz) - zr;
+        const float dx = (float)((cx - ox) * lam) - xr, dy = (float)((cy - oy) * lam) - yr,
+                    dz = (float)((cz - oz) * lam) - zr;
         const float d2 = dx * dx + dy * dy + dz * dz;
         if (leaf && ncount == 1) {
             if (active && (cx != x || cy != y || cz != z)) {
@@ -284,9 +287,10 @@ __global__ __launch_bounds__(kBlock) void k_walk32(WalkArgs a) {
                 const uint32_t st = nodes[p].start;
                 for (uint32_t j = st; j < st + ncount; ++j) {
                     const double bx = tx[j], by = ty[j], bz = tz[j];
-                    const float bm = (float)tm[j];
+                    const float bm = (float)(tm[j] * mu);
                     if (open && (bx != x || by != y || bz != z)) {
-                        const float ex = (float)(bx - ox) - xr, ey = (float)(by - oy) - yr, ez = (float)(bz - oz) - zr;
+                        const float ex = (float)((bx - ox) * lam) - xr, ey = (float)((by - oy) * lam) - yr,
+                                    ez = (float)((bz - oz) * lam) - zr;
                         const float inv = __builtin_amdgcn_rsqf(ex * ex + ey * ey + ez * ez + eps2);
                         const float f = bm * (inv * inv * inv);
                         ax += f * ex; ay += f * ey; az += f * ez;
@@ -304,9 +308,10 @@ __global__ __launch_bounds__(kBlock) void k_walk32(WalkArgs a) {
     }
     if (valid) {
         const uint32_t o = a.tperm[i];
-        a.ax[o] = (double)ax;
-        a.ay[o] = (double)ay;
-        a.az[o] = (double)az;
+        const double back = lam * lam / mu;   // scaled units to physical (exact)
+        a.ax[o] = (double)ax * back;
+        a.ay[o] = (double)ay * back;
+        a.az[o] = (double)az * back;
         if (a.counts) a.counts[o] = cnt;
     }
 }
@@ -456,7 +461,7 @@ __device__ __forceinline__ void adj_self(const T (*ent)[4], int ne, T xr, T yr, 
 
 template <typename T>
 __device__ __forceinline__ void adj_scatter(const T (*ent)[4], const uint32_t* ids, int ne, const T (*tg)[6], T eps2,
-                                            uint32_t lane, double* __restrict__ g4) {
+                                            uint32_t lane, double* __restrict__ g4, double sx = 1.0, double sm = 1.0) {
     for (int e = (int)lane; e < ne; e += 64) {
         const T X = ent[e][0], Y = ent[e][1], Z = ent[e][2];
         T gx = 0, gy = 0, gz = 0, h = 0;
@@ -477,10 +482,10 @@ __device__ __forceinline__ void adj_scatter(const T (*ent)[4], const uint32_t* i
         }
         const T M = ent[e][3];
         double* g = g4 + 4 * (size_t)ids[e];
-        unsafeAtomicAdd(g, (double)(M * gx));
-        unsafeAtomicAdd(g + 1, (double)(M * gy));
-        unsafeAtomicAdd(g + 2, (double)(M * gz));
-        unsafeAtomicAdd(g + 3, (double)h);
+        unsafeAtomicAdd(g, (double)(M * gx) * sx);
+        unsafeAtomicAdd(g + 1, (double)(M * gy) * sx);
+        unsafeAtomicAdd(g + 2, (double)(M * gz) * sx);
+        unsafeAtomicAdd(g + 3, (double)h * sm);
     }
 }
 
@@ -769,14 +774,15 @@ static_assert(kCells32 == 64, "the FP32 cell list is one id per lane");
 template <bool kQuad>
 __device__ __forceinline__ void gather_cells32(float (*nd)[4], float (*q)[6], uint32_t id, int nn,
                                                uint32_t lane, const WalkPay32* __restrict__ pay,
-                                               const double* __restrict__ com, double ox, double oy, double oz) {
+                                               const double* __restrict__ com, double ox, double oy, double oz,
+                                               double lam) {
     {
         const int e = (int)lane;
         if (e < nn) {
             const WalkPay32& c = pay[id];
-            nd[e][0] = (float)(com[3 * (size_t)id] - ox);
-            nd[e][1] = (float)(com[3 * (size_t)id + 1] - oy);
-            nd[e][2] = (float)(com[3 * (size_t)id + 2] - oz);
+            nd[e][0] = (float)((com[3 * (size_t)id] - ox) * lam);
+            nd[e][1] = (float)((com[3 * (size_t)id + 1] - oy) * lam);
+            nd[e][2] = (float)((com[3 * (size_t)id + 2] - oz) * lam);
             nd[e][3] = c.mass;
             if (kQuad) {
 #pragma unroll
@@ -824,15 +830,18 @@ __global__ __launch_bounds__(64) void k_walk_group32(WalkArgs a) {
     // roundings the body gets as a list entry
     const uint32_t own = a.tleaf[iv];
     const float4 ob = a.tb4[iv];
-    const float xr = (float)(a.com[3 * (size_t)own] - ox) + ob.x;
-    const float yr = (float)(a.com[3 * (size_t)own + 1] - oy) + ob.y;
-    const float zr = (float)(a.com[3 * (size_t)own + 2] - oz) + ob.z;
-    const float eps2 = a.eps2f;
+    // FP32 quantities are in the system's scaled units (exact powers of two)
+    const double lam = a.scale32[2 * (size_t)gr.z], mu = a.scale32[2 * (size_t)gr.z + 1];
+    const float xr = (float)((a.com[3 * (size_t)own] - ox) * lam) + ob.x;
+    const float yr = (float)((a.com[3 * (size_t)own + 1] - oy) * lam) + ob.y;
+    const float zr = (float)((a.com[3 * (size_t)own + 2] - oz) * lam) + ob.z;
+    const float eps2 = (float)(a.eps2 * lam * lam);
     // the opening test edge^2 < theta^2 d^2 in FP32 coordinates relative to the
     // system's root centre, the frame of the records' centres (the decision
     // only; theta = 0 opens every cell)
     const double* b4s = a.b4 + 4 * (size_t)gr.z;
-    const float xs = (float)(tx[iv] - b4s[0]), ys = (float)(ty[iv] - b4s[1]), zs = (float)(tz[iv] - b4s[2]);
+    const float xs = (float)((tx[iv] - b4s[0]) * lam), ys = (float)((ty[iv] - b4s[1]) * lam),
+                zs = (float)((tz[iv] - b4s[2]) * lam);
     const float th2 = a.theta2f;
     float ax = 0.f, ay = 0.f, az = 0.f, phi = 0.f;
     float wxr = 0.f, wyr = 0.f, wzr = 0.f;
@@ -852,11 +861,11 @@ __global__ __launch_bounds__(64) void k_walk_group32(WalkArgs a) {
             if ((int)lane < nn) s_nid[lane] = cell_ids;
         }
         __syncthreads();
-        gather_cells32<kQuad>(s_nd, s_q, cell_ids, nn, lane, a.pay32, a.com, ox, oy, oz);
+        gather_cells32<kQuad>(s_nd, s_q, cell_ids, nn, lane, a.pay32, a.com, ox, oy, oz, lam);
         __syncthreads();
         if constexpr (kAdj) {
             adj_self<float>(s_nd, nn, xr, yr, zr, wxr, wyr, wzr, eps2, ax, ay, az);
-            adj_scatter<float>(s_nd, s_nid, nn, s_tg, eps2, lane, a.gnode);
+            adj_scatter<float>(s_nd, s_nid, nn, s_tg, eps2, lane, a.gnode, lam * lam * lam / mu, lam * lam);
         } else {
             eval_list32<kQuad, kGuard, kPot>(s_nd, s_q, nn, xr, yr, zr, eps2, ax, ay, az, phi);
         }
@@ -867,9 +876,9 @@ __global__ __launch_bounds__(64) void k_walk_group32(WalkArgs a) {
         // each range's leaf centre relative to the group origin, from its FP64 centre
         if ((int)lane < nr) {
             const size_t l3 = 3 * (size_t)s_rleaf[lane];
-            s_rdel[lane][0] = (float)(a.com[l3] - ox);
-            s_rdel[lane][1] = (float)(a.com[l3 + 1] - oy);
-            s_rdel[lane][2] = (float)(a.com[l3 + 2] - oz);
+            s_rdel[lane][0] = (float)((a.com[l3] - ox) * lam);
+            s_rdel[lane][1] = (float)((a.com[l3 + 1] - oy) * lam);
+            s_rdel[lane][2] = (float)((a.com[l3 + 2] - oz) * lam);
         }
         __syncthreads();
 #pragma unroll
@@ -893,7 +902,7 @@ __global__ __launch_bounds__(64) void k_walk_group32(WalkArgs a) {
         __syncthreads();
         if constexpr (kAdj) {
             adj_self<float>(s_bd, nq, xr, yr, zr, wxr, wyr, wzr, eps2, ax, ay, az);
-            adj_scatter<float>(s_bd, s_bid, nq, s_tg, eps2, lane, a.gbody);
+            adj_scatter<float>(s_bd, s_bid, nq, s_tg, eps2, lane, a.gbody, lam * lam * lam / mu, lam * lam);
         } else {
             eval_list32<false, kGuard, kPot>(s_bd, s_q, nq, xr, yr, zr, eps2, ax, ay, az, phi);
         }
@@ -991,15 +1000,19 @@ __global__ __launch_bounds__(64) void k_walk_group32(WalkArgs a) {
     cnt += (uint32_t)(nn + nq);
     if (valid) {
         const uint32_t o = a.tperm[i];
+        // back to physical units (exact): a * lam^2 / mu, phi * lam / mu, and
+        // for the adjoint's dL/dx of the body's own position lam^3 / mu
         if constexpr (kAdj) {
-            a.gself[3 * (size_t)o] = (double)ax;
-            a.gself[3 * (size_t)o + 1] = (double)ay;
-            a.gself[3 * (size_t)o + 2] = (double)az;
+            const double back = lam * lam * lam / mu;
+            a.gself[3 * (size_t)o] = (double)ax * back;
+            a.gself[3 * (size_t)o + 1] = (double)ay * back;
+            a.gself[3 * (size_t)o + 2] = (double)az * back;
         } else {
-            a.ax[o] = (double)ax;
-            a.ay[o] = (double)ay;
-            a.az[o] = (double)az;
-            if constexpr (kPot) a.pot[o] = (double)phi;
+            const double back = lam * lam / mu;
+            a.ax[o] = (double)ax * back;
+            a.ay[o] = (double)ay * back;
+            a.az[o] = (double)az * back;
+            if constexpr (kPot) a.pot[o] = (double)phi * (lam / mu);
         }
         if (a.counts) a.counts[o] = cnt;
     }
@@ -1105,6 +1118,7 @@ hipError_t launch_walk_to(bhx_ctx* c, hipStream_t s, bool group, int precision, 
     a.nodes = c->tree.walk;
     a.nodes32 = c->tree.walk32;
     a.pay32 = c->tree.pay32;
+    a.scale32 = c->sys_scale;
     a.com = c->tree.com;
     a.tb4 = c->tb4;
     a.tleaf = c->tleaf;
@@ -1158,6 +1172,7 @@ hipError_t launch_walk_potential(bhx_ctx* c, hipStream_t s, double* ox, double* 
     a.nodes = c->tree.walk;
     a.nodes32 = c->tree.walk32;
     a.pay32 = c->tree.pay32;
+    a.scale32 = c->sys_scale;
     a.com = c->tree.com;
     a.tb4 = c->tb4;
     a.tleaf = c->tleaf;
@@ -1203,6 +1218,7 @@ hipError_t launch_walk_adjoint(bhx_ctx* c, const WalkTree& t, int precision, con
     a.nodes = t.walk;
     a.nodes32 = t.walk32;
     a.pay32 = t.pay32;
+    a.scale32 = t.scale32;
     a.com = t.com;
     a.tb4 = t.tb4;
     a.tleaf = t.tleaf;

@@ -333,6 +333,48 @@ def test_walk_unsoftened(bhx, prec, quad, group):
     assert np.median(e5) < (1e-3 if group else 5e-3)
 
 
+@pytest.mark.parametrize("scale_exp", [30, -30])
+@pytest.mark.parametrize("prec", [0, 1])
+def test_walk_scale_equivariance(bhx, prec, scale_exp):
+    """Positions and softening times s = 2^k scale every FP64 operation of bounds,
+    keys and tree exactly, so the tree is the same and accelerations scale by 1/s^2."""
+    n = 4000
+    pos, vel, mass = bhx.plummer(n, seed=31)
+    s = 2.0 ** scale_exp
+    acc = []
+    for f in (1.0, s):
+        with _engine(bhx, n, theta=0.5, eps=0.01 * f, group_walk=1, precision=prec, leaf_capacity=16, mode=1,
+                     quadrupole=1) as e:
+            e.set_bodies(pos * f, vel, mass)
+            c, sz = e.bounds()
+            e.build(c, sz)
+            acc.append(e.walk() * f * f)
+    d = np.abs(acc[1] - acc[0]).max() / np.abs(acc[0]).max()
+    print("scale 2^%d prec %d: max relative difference %.3e" % (scale_exp, prec, d), flush=True)
+    # FP64 scales exactly; the FP32 walk works in per-system power-of-two units,
+    # so it scales exactly too (without them r^-5 and r^-7 leave the FP32 range)
+    assert d == 0.0
+
+
+def test_fp32_walk_in_si_units(bhx):
+    """Solar-system units (lengths ~1e11 m, masses ~1e30 kg, G = 1): the FP32 group
+    walk stays finite and agrees with the FP64 walk on the same tree."""
+    n, L, M = 4000, 1.496e11, 1.989e30
+    pos, vel, mass = bhx.plummer(n, seed=37)
+    res = []
+    for prec in (0, 1):
+        with _engine(bhx, n, theta=0.5, eps=0.01 * L, group_walk=1, precision=prec, leaf_capacity=16, mode=1,
+                     quadrupole=1) as e:
+            e.set_bodies(pos * L, vel, mass * M)
+            c, sz = e.bounds()
+            e.build(c, sz)
+            res.append(e.walk())
+    assert np.all(np.isfinite(res[1]))
+    d = np.linalg.norm(res[1] - res[0], axis=1) / np.linalg.norm(res[0], axis=1)
+    print("SI units: FP32 vs FP64 walk median %.3e max %.3e" % (np.median(d), d.max()), flush=True)
+    assert np.median(d) < 1e-5 and d.max() < 1e-3
+
+
 @pytest.mark.parametrize("prec", [0, 1])
 def test_group_walk_is_at_least_as_accurate(bhx, plummer16k, prec):
     pos, vel, mass = plummer16k
