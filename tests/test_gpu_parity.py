@@ -280,6 +280,47 @@ def test_empty_and_errors(bhx):
             e.build(np.array([np.nan, 0, 0]), 1.0)
 
 
+TINY_CFGS = {
+    "as_run_fp64": dict(theta=0.5, eps=0.01, dt=1e-3),
+    "group_fp32_quad": dict(theta=0.5, eps=0.01, dt=1e-3, leaf_capacity=32, mode=1, precision=1, quadrupole=1,
+                            group_walk=1, compensated=1),
+    "group_fp64": dict(theta=0.5, eps=0.01, dt=1e-3, leaf_capacity=4, mode=1, precision=0, group_walk=1),
+    "body_fp32": dict(theta=0.5, eps=0.01, dt=1e-3, leaf_capacity=4, mode=1, precision=1, group_walk=0),
+}
+
+
+@pytest.mark.parametrize("name", sorted(TINY_CFGS))
+def test_tiny_systems(bhx, name):
+    """One body feels nothing; two bodies follow the softened two-body force (equal and
+    opposite); two coincident bodies add nothing to each other, softened or not."""
+    cfg = TINY_CFGS[name]
+    eps = cfg["eps"]
+    with _engine(bhx, 4, **cfg) as e:
+        e.set_bodies(np.array([[0.3, -0.2, 0.1]]), np.array([[0.1, 0.0, 0.0]]), np.array([1.0]))
+        e.step(3)
+        p, v, m, a = e.get_bodies()
+        assert np.all(a == 0) and np.allclose(p, [[0.3 + 3e-4, -0.2, 0.1]], rtol=0, atol=1e-15)
+        x = np.array([[0.0, 0.0, 0.0], [0.7, -0.4, 0.2]])
+        m2 = np.array([0.75, 0.25])
+        e.set_bodies(x, np.zeros_like(x), m2)
+        c, sz = e.bounds()
+        e.build(c, sz)
+        acc = e.walk()
+        d = x[1] - x[0]
+        f = d / (d @ d + eps * eps) ** 1.5
+        tol = 1e-14 if cfg.get("precision", 0) == 0 else 1e-6
+        assert np.allclose(acc[0], m2[1] * f, rtol=tol, atol=0) and np.allclose(acc[1], -m2[0] * f, rtol=tol, atol=0)
+        e.step(5)
+        assert all(np.all(np.isfinite(q)) for q in e.get_bodies())
+        for soft in (eps, 0.0):
+            e.set_config(eps=soft)
+            e.set_bodies(np.array([[0.1, 0.1, 0.1]] * 2), np.zeros((2, 3)), np.array([0.5, 0.5]))
+            c, sz = e.bounds()
+            e.build(c, sz)
+            acc = e.walk()
+            assert np.all(acc == 0), (soft, acc)
+
+
 def test_diagnostics_momentum(bhx, plummer16k):
     pos, vel, mass = plummer16k
     with _engine(bhx, N1) as e:
