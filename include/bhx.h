@@ -27,7 +27,7 @@ typedef enum bhx_status {
     BHX_E_ARG = 2,         /* an argument is out of range                         */
     BHX_E_DEPTH = 3,       /* cells at the 21-level cap hold more bodies than fit  */
     BHX_E_HASH_FULL = 4,   /* node hash table could not take every node            */
-    BHX_E_NONFINITE = 5,   /* non-finite position or bounds                        */
+    BHX_E_NONFINITE = 5,   /* non-finite bounds, or a step met a non-finite state  */
     BHX_E_HIP = 6,         /* a HIP runtime call failed                            */
     BHX_E_NOMEM = 7,       /* device allocation failed                             */
     BHX_E_STATE = 8,       /* call out of order (e.g. walk before build)           */
@@ -190,7 +190,10 @@ int bhx_walk(bhx_ctx *ctx, double *acc_out, uint64_t *counts);
 int bhx_interaction_counts(bhx_ctx *ctx, uint64_t *counts);
 
 /* One frame in the reference order (ofApp::update then ofApp::draw,
- * ofApp.cpp:169-177,248-282), repeated nsteps times on the resident bodies. */
+ * ofApp.cpp:169-177,248-282), repeated nsteps times on the resident bodies.
+ * Returns BHX_E_NONFINITE when some frame met a non-finite position (at its
+ * keys) or made a non-finite velocity (at its closing kick); the bodies are
+ * left as the frames made them.  (The reference checks neither.) */
 int bhx_step(bhx_ctx *ctx, int nsteps);
 
 /* bhx_step with every frame replayed from a captured HIP graph: no host wait

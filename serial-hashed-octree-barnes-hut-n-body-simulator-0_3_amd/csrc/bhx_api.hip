@@ -260,9 +260,26 @@ int build_on_device(bhx_ctx* c, hipStream_t s) {
     return BHX_OK;
 }
 
+__global__ void k_fill_u32_api(uint32_t* __restrict__ p, size_t n, uint32_t v) {
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) p[i] = v;
+}
+
+// Steps flag a non-finite position (at each frame's keys) or velocity (at
+// each closing kick); the flag is cleared when a stepping call starts and when
+// it has been reported.
+hipError_t clear_nonfinite(bhx_ctx* c, hipStream_t s) {
+    hipLaunchKernelGGL(k_fill_u32_api, dim3(1), dim3(64), 0, s, &c->ctr->nonfinite, (size_t)1, 0u);
+    return hipGetLastError();
+}
+
 int read_counters(bhx_ctx* c, hipStream_t s) {
     HIPCHK(hipMemcpyAsync(c->ctr_host, c->ctr, sizeof(Counters), hipMemcpyDeviceToHost, s));
     HIPCHK(hipStreamSynchronize(s));
+    if (c->ctr_host->nonfinite) {
+        HIPCHK(clear_nonfinite(c, s));
+        HIPCHK(hipStreamSynchronize(s));
+        return fail(c, BHX_E_NONFINITE, "a step produced a non-finite position or velocity");
+    }
     if (c->ctr_host->sort_fail) return fail(c, BHX_E_SORT, "radix sort look-back did not complete");
     if (c->ctr_host->hash_fail) return fail(c, BHX_E_HASH_FULL, "node hash table full");
     return BHX_OK;
@@ -278,10 +295,6 @@ __global__ void k_pick3(const double* __restrict__ x, const double* __restrict__
     }
 }
 
-__global__ void k_fill_u32_api(uint32_t* __restrict__ p, size_t n, uint32_t v) {
-    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) p[i] = v;
-}
-
 // One frame in the reference order with no host wait: the tree is built into
 // the arrays it has (an overflow is flagged on the device, the walk then does
 // nothing), and the side branch computing the next bounds is joined at the
@@ -292,7 +305,7 @@ int enqueue_step_async(bhx_ctx* c, hipStream_t s, bool linear) {
     double* b_next = c->bounds_slot(cur ^ 1);
     const SysView sv = c->sys();
     const double dt = c->cfg.dt;
-    HIPCHK(launch_morton(c->px, c->py, c->pz, c->n, sv.id, b_cur, c->keys, c->vals, s));
+    HIPCHK(launch_morton(c->px, c->py, c->pz, c->n, sv.id, b_cur, c->keys, c->vals, s, &c->ctr->nonfinite));
     HIPCHK(sort_pairs(c, c->keys, c->vals, s));
     HIPCHK(launch_permute_kick_drift(c, c->vals, dt, s));
     if (linear) {
@@ -309,7 +322,7 @@ int enqueue_step_async(bhx_ctx* c, hipStream_t s, bool linear) {
     size_t total = 0;
     HIPCHK(build_tree(c, s, &total, false));
     HIPCHK(launch_walk(c, s));
-    HIPCHK(launch_kick(c, dt, s));
+    HIPCHK(launch_kick(c, dt, s, &c->ctr->nonfinite));
     if (!linear) HIPCHK(hipStreamWaitEvent(s, c->ev_bounds, 0));
     c->bslot = cur ^ 1;
     return BHX_OK;
@@ -1082,6 +1095,7 @@ int bhx_step(bhx_ctx* c, int nsteps) {
     hipStream_t s = c->s_main;
     const double dt = c->cfg.dt;
     const auto t0 = std::chrono::steady_clock::now();
+    HIPCHK(clear_nonfinite(c, s));
     for (int step = 0; step < nsteps && c->n; ++step) {
         const int cur = c->bslot;
         double* b_cur = c->bounds_slot(cur);
@@ -1095,7 +1109,7 @@ int bhx_step(bhx_ctx* c, int nsteps) {
             HIPCHK(hipStreamWaitEvent(s, c->ev_bounds, 0));
         }
         if (prof) HIPCHK(hipEventRecord(c->ev[1], s));
-        HIPCHK(launch_morton(c->px, c->py, c->pz, c->n, sv.id, b_cur, c->keys, c->vals, s));
+        HIPCHK(launch_morton(c->px, c->py, c->pz, c->n, sv.id, b_cur, c->keys, c->vals, s, &c->ctr->nonfinite));
         if (prof) HIPCHK(hipEventRecord(c->ev[2], s));
         HIPCHK(sort_pairs(c, c->keys, c->vals, s));
         if (prof) HIPCHK(hipEventRecord(c->ev[3], s));
@@ -1125,7 +1139,7 @@ int bhx_step(bhx_ctx* c, int nsteps) {
         if (prof) HIPCHK(hipEventRecord(c->ev[6], s));
         HIPCHK(launch_walk(c, s));
         if (prof) HIPCHK(hipEventRecord(c->ev[7], s));
-        HIPCHK(launch_kick(c, dt, s));
+        HIPCHK(launch_kick(c, dt, s, &c->ctr->nonfinite));
         if (prof) HIPCHK(hipEventRecord(c->ev[8], s));
         c->bslot = cur ^ 1;
         c->bounds_valid = true;
@@ -1145,8 +1159,8 @@ int bhx_step(bhx_ctx* c, int nsteps) {
         }
     }
     int rc = read_counters(c, s);
-    if (rc != BHX_OK) return rc;
     HIPCHK(hipStreamSynchronize(c->s_side));
+    if (rc != BHX_OK) return rc;
     // keep the build bounds of the last frame for node downloads
     if (c->tree_built) {
         c->tree_bounds_h.resize(4 * (size_t)c->nsys);
@@ -1170,6 +1184,7 @@ int bhx_step_graph(bhx_ctx* c, int nsteps) {
         HIPCHK(launch_bounds(c, c->bounds_slot(c->bslot), s));
         c->bounds_valid = true;
     }
+    HIPCHK(clear_nonfinite(c, s));
     int rc = take_snapshot(c, c->snaps[0], s);
     if (rc != BHX_OK) return rc;
     bool done = false;

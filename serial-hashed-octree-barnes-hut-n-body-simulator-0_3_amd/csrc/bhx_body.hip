@@ -181,13 +181,16 @@ __global__ __launch_bounds__(kBlock) void k_morton(const double* __restrict__ px
                                                    const uint32_t* __restrict__ sysid,
                                                    const double* __restrict__ bounds4,
                                                    uint64_t* __restrict__ keys,
-                                                   uint32_t* __restrict__ vals) {
+                                                   uint32_t* __restrict__ vals,
+                                                   uint32_t* __restrict__ nonfinite) {
     for (size_t i = blockIdx.x * (size_t)kBlock + threadIdx.x; i < n; i += (size_t)gridDim.x * kBlock) {
         const double size = bounds4[4 * (size_t)(sysid ? sysid[i] : 0u) + 3];
         const double frac = 2097152.0 / (2.0 * size);
-        double xs = px[i] + size;
-        double ys = py[i] + size;
-        double zs = pz[i] + size;
+        const double x = px[i], y = py[i], z = pz[i];
+        if (nonfinite && !(isfinite(x) && isfinite(y) && isfinite(z))) *nonfinite = 1u;
+        double xs = x + size;
+        double ys = y + size;
+        double zs = z + size;
         xs = xs * frac;
         ys = ys * frac;
         zs = zs * frac;
@@ -316,21 +319,23 @@ __global__ __launch_bounds__(kBlock) void k_kick(double* __restrict__ vx, double
                                                  double* __restrict__ vz, const double* __restrict__ ax,
                                                  const double* __restrict__ ay, const double* __restrict__ az,
                                                  double* __restrict__ cvx, double* __restrict__ cvy,
-                                                 double* __restrict__ cvz, size_t n, double dt) {
+                                                 double* __restrict__ cvz, size_t n, double dt,
+                                                 uint32_t* __restrict__ nonfinite) {
     for (size_t i = blockIdx.x * (size_t)kBlock + threadIdx.x; i < n; i += (size_t)gridDim.x * kBlock) {
+        double v0 = vx[i], v1 = vy[i], v2 = vz[i];
         if (!kComp) {
-            vx[i] = vx[i] + (ax[i] * dt) * 0.5;
-            vy[i] = vy[i] + (ay[i] * dt) * 0.5;
-            vz[i] = vz[i] + (az[i] * dt) * 0.5;
+            v0 = v0 + (ax[i] * dt) * 0.5;
+            v1 = v1 + (ay[i] * dt) * 0.5;
+            v2 = v2 + (az[i] * dt) * 0.5;
         } else {
-            double v0 = vx[i], v1 = vy[i], v2 = vz[i];
             double c0 = cvx[i], c1 = cvy[i], c2 = cvz[i];
             kahan_add(v0, c0, (ax[i] * dt) * 0.5);
             kahan_add(v1, c1, (ay[i] * dt) * 0.5);
             kahan_add(v2, c2, (az[i] * dt) * 0.5);
-            vx[i] = v0; vy[i] = v1; vz[i] = v2;
             cvx[i] = c0; cvy[i] = c1; cvz[i] = c2;
         }
+        vx[i] = v0; vy[i] = v1; vz[i] = v2;
+        if (nonfinite && !(isfinite(v0) && isfinite(v1) && isfinite(v2))) *nonfinite = 1u;
     }
 }
 
@@ -366,8 +371,9 @@ hipError_t launch_bounds(bhx_ctx* c, double* bounds4, hipStream_t s, bool exclus
 }
 
 hipError_t launch_morton(const double* px, const double* py, const double* pz, size_t n, const uint32_t* sysid,
-                         const double* b4, uint64_t* keys, uint32_t* vals, hipStream_t s) {
-    hipLaunchKernelGGL(k_morton, dim3(grid_for(n)), dim3(kBlock), 0, s, px, py, pz, n, sysid, b4, keys, vals);
+                         const double* b4, uint64_t* keys, uint32_t* vals, hipStream_t s, uint32_t* nonfinite) {
+    hipLaunchKernelGGL(k_morton, dim3(grid_for(n)), dim3(kBlock), 0, s, px, py, pz, n, sysid, b4, keys, vals,
+                       nonfinite);
     return hipGetLastError();
 }
 
@@ -441,13 +447,13 @@ hipError_t launch_kick_drift(bhx_ctx* c, double dt, hipStream_t s) {
     return launch_permute_kick_drift(c, nullptr, dt, s);
 }
 
-hipError_t launch_kick(bhx_ctx* c, double dt, hipStream_t s) {
+hipError_t launch_kick(bhx_ctx* c, double dt, hipStream_t s, uint32_t* nonfinite) {
     if (c->cfg.compensated)
         hipLaunchKernelGGL(k_kick<true>, dim3(grid_for(c->n)), dim3(kBlock), 0, s, c->vx, c->vy, c->vz, c->ax, c->ay,
-                           c->az, c->cvx, c->cvy, c->cvz, c->n, dt);
+                           c->az, c->cvx, c->cvy, c->cvz, c->n, dt, nonfinite);
     else
         hipLaunchKernelGGL(k_kick<false>, dim3(grid_for(c->n)), dim3(kBlock), 0, s, c->vx, c->vy, c->vz, c->ax, c->ay,
-                           c->az, c->cvx, c->cvy, c->cvz, c->n, dt);
+                           c->az, c->cvx, c->cvy, c->cvz, c->n, dt, nonfinite);
     return hipGetLastError();
 }
 

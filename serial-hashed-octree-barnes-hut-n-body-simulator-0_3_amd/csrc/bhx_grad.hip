@@ -77,6 +77,7 @@ __global__ void k_zero_flags(Counters* __restrict__ ctr) {
     if (threadIdx.x == 0) {
         ctr->max_nodes = 0;
         ctr->overflow_seen = 0;
+        ctr->nonfinite = 0;
     }
 }
 

@@ -88,7 +88,7 @@ struct Counters {          // small device-side status block
     uint32_t unsplit;
     uint32_t hash_fail;
     uint32_t sort_fail;
-    uint32_t nonfinite;
+    uint32_t nonfinite;       // a step met a non-finite position or velocity
     uint32_t node_overflow;   // the last build asked for more nodes than the tree holds
     uint32_t max_nodes;       // largest node count a build asked for (sticky)
     uint32_t overflow_seen;   // some build overflowed (sticky until the host clears it)
@@ -282,8 +282,10 @@ void free_grad(bhx_ctx* c);                           // recorded trajectory (bh
 // nothing else keeps the GPU busy (one system only: the chain's workgroup then
 // takes a whole compute unit)
 hipError_t launch_bounds(bhx_ctx* c, double* bounds4, hipStream_t s, bool exclusive = false);
+// nonfinite (optional): set to 1 when some position is not finite
 hipError_t launch_morton(const double* px, const double* py, const double* pz, size_t n, const uint32_t* sysid,
-                         const double* bounds4, uint64_t* keys, uint32_t* vals, hipStream_t s);
+                         const double* bounds4, uint64_t* keys, uint32_t* vals, hipStream_t s,
+                         uint32_t* nonfinite = nullptr);
 // tree path keys; the root edge is tscale * (bounds size)
 hipError_t launch_treekeys(const double* px, const double* py, const double* pz, size_t n, const uint32_t* sysid,
                            const double* bounds4, double tscale, uint64_t* keys, uint32_t* vals, hipStream_t s);
@@ -300,7 +302,8 @@ hipError_t launch_gather(double* const* src, double* const* dst, int narr, const
                          size_t n, hipStream_t s);
 hipError_t launch_permute_kick_drift(bhx_ctx* c, const uint32_t* perm, double dt, hipStream_t s);
 hipError_t launch_kick_drift(bhx_ctx* c, double dt, hipStream_t s);
-hipError_t launch_kick(bhx_ctx* c, double dt, hipStream_t s);
+// nonfinite (optional): set to 1 when some kicked velocity is not finite
+hipError_t launch_kick(bhx_ctx* c, double dt, hipStream_t s, uint32_t* nonfinite = nullptr);
 hipError_t launch_tree_gather(bhx_ctx* c, hipStream_t s);
 // exchange the body arrays with their gather targets, as a permute/kick/drift does
 void swap_body_buffers(bhx_ctx* c);

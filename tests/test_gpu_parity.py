@@ -280,6 +280,43 @@ def test_empty_and_errors(bhx):
             e.build(np.array([np.nan, 0, 0]), 1.0)
 
 
+@pytest.mark.parametrize("name", ["as_run_fp64", "group_fp32_quad", "adjoint_fp32"])
+def test_nonfinite_state_is_reported(bhx, name):
+    """A NaN velocity or an infinite mass makes the stepping calls (eager, graph,
+    recorded) fail with BHX_E_NONFINITE (every loop in a frame is bounded by
+    integer counts, so the frames still end); once reported the flag is clear,
+    and finite bodies set afterwards step exactly as in a fresh engine."""
+    adjoint = dict(theta=0.5, eps=0.01, dt=1e-3, leaf_capacity=32, mode=1, precision=1, group_walk=1,
+                   permute_acc=1)
+    cfg = adjoint if name == "adjoint_fp32" else TINY_CFGS[name]
+    n = 4096
+    pos, vel, mass = bhx.plummer(n, seed=7)
+    bad_v = vel.copy()
+    bad_v[100, 1] = np.nan
+    bad_m = mass.copy()
+    bad_m[5] = np.inf
+    with _engine(bhx, n, **cfg) as e:
+        for v, m in ((bad_v, mass), (vel, bad_m)):
+            calls = [("step", e.step), ("step_graph", e.step_graph)]
+            if cfg is adjoint:
+                calls.append(("record", lambda k: e.record(k, checkpoint_every=1)))
+            for label, call in calls:
+                print(name, label, flush=True)
+                e.set_bodies(pos, v, m)
+                with pytest.raises(bhx.BhxError) as err:
+                    call(2)
+                assert err.value.code == 5
+        e.set_bodies(pos, vel, mass)
+        e.step(2)
+        e.step_graph(1)
+        p1, v1, _, a1 = e.get_bodies()
+    with _engine(bhx, n, **cfg) as f:
+        f.set_bodies(pos, vel, mass)
+        f.step(3)
+        p2, v2, _, a2 = f.get_bodies()
+    assert np.array_equal(p1, p2) and np.array_equal(v1, v2) and np.array_equal(a1, a2)
+
+
 TINY_CFGS = {
     "as_run_fp64": dict(theta=0.5, eps=0.01, dt=1e-3),
     "group_fp32_quad": dict(theta=0.5, eps=0.01, dt=1e-3, leaf_capacity=32, mode=1, precision=1, quadrupole=1,
