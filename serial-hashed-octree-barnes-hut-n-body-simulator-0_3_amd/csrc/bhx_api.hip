@@ -125,7 +125,8 @@ void free_all(bhx_ctx* c) {
         if (p) { (void)hipFree(p); p = nullptr; }
     void* vp[] = {c->keys, c->keys_alt, c->vals, c->vals_alt, c->mkeys, c->tkeys, c->tperm, c->nn, c->noff,
                   c->nlo, c->scan_tmp, c->sort_hist, c->sort_status, c->sort_tile_ctr, c->wcount, c->ctr, c->tb4, c->tleaf,
-                  c->sys_off, c->sysid, c->sys_root, c->sys_scale, c->gtab, c->skey, c->skey_alt, c->sidx, c->sidx_alt};
+                  c->sys_off, c->sysid, c->sys_root, c->sys_scale, c->gtab, c->skey, c->skey_alt, c->sidx, c->sidx_alt,
+                  c->sum_tile, c->sum_cnt, c->sum_piece};
     for (void* p : vp)
         if (p) (void)hipFree(p);
     if (c->ctr_host) (void)hipHostFree(c->ctr_host);
@@ -313,7 +314,7 @@ int enqueue_step_async(bhx_ctx* c, hipStream_t s, bool linear) {
     } else {
         HIPCHK(hipEventRecord(c->ev_drift, s));
         HIPCHK(hipStreamWaitEvent(c->s_side, c->ev_drift, 0));
-        HIPCHK(launch_bounds(c, b_next, c->s_side, true));
+        HIPCHK(launch_bounds(c, b_next, c->s_side));
         HIPCHK(hipEventRecord(c->ev_bounds, c->s_side));
     }
     HIPCHK(launch_treekeys(c->px, c->py, c->pz, c->n, sv.id, b_cur, c->cfg_tscale(), c->tkeys, c->tperm, s));
@@ -523,6 +524,10 @@ int bhx_create(const bhx_config* cfg, size_t capacity, bhx_ctx** out) {
         chk(dalloc(&c->sort_hist, 8 * 256), "alloc sort");
         chk(dalloc(&c->sort_tile_ctr, 8), "alloc sort");
         chk(dalloc(&c->sort_status, 8 * 256 * (c->sort_tiles + 1)), "alloc sort");
+        c->sum_tiles_cap = sum_tiles_for(n);
+        chk(dalloc(&c->sum_tile, 6 * c->sum_tiles_cap), "alloc bounds");
+        chk(dalloc(&c->sum_cnt, 3 * c->sum_tiles_cap), "alloc bounds");
+        chk(dalloc(&c->sum_piece, 3 * 64 * c->sum_tiles_cap), "alloc bounds");
         chk(dalloc(&c->ctr, 1), "alloc counters");
         chk(hipHostMalloc((void**)&c->ctr_host, sizeof(Counters), hipHostMallocDefault), "alloc pinned");
         if (rc == BHX_OK) chk(alloc_tree(c, 2 * n + 1024), "alloc tree");
@@ -1117,7 +1122,7 @@ int bhx_step(bhx_ctx* c, int nsteps) {
         HIPCHK(hipEventRecord(c->ev_drift, s));
         if (prof) HIPCHK(hipEventRecord(c->ev[4], s));
         HIPCHK(hipStreamWaitEvent(c->s_side, c->ev_drift, 0));
-        HIPCHK(launch_bounds(c, b_next, c->s_side, true));
+        HIPCHK(launch_bounds(c, b_next, c->s_side));
         HIPCHK(hipEventRecord(c->ev_bounds, c->s_side));
         // tree on x_{f+1} with the bounds of x_f
         c->tree_valid = c->tree_built = false;

@@ -1,22 +1,25 @@
 // bhx_body.hip — per-body kernels whose FP64 results must match the reference
 // bit for bit: bounds, Morton keys, tree path keys, gathers and the leapfrog.
 // Compiled with -ffp-contract=off: every operation rounds as written.
+#include <climits>
+
 #include "bhx_internal.hpp"
 
 namespace bhx {
 namespace {
 
 constexpr int kBlock = 256;
-// Chunk sizes of the sequential mean (doubles per axis and buffer; each row
-// is padded by 16 doubles so that the chain's read-ahead never needs a bounds
-// test).  Exclusive: 2 x 3 x (3328 + 16) doubles = 156.75 KiB of LDS, so the
-// workgroup holds its compute unit alone and no other kernel's waves slow the
-// dependent adds; it can only start on a drained compute unit, so it is used
-// where nothing else keeps the GPU full (the frame's own side branch of a
-// single system).  Shared: 2 x 3 x (1024 + 16) doubles = 48.75 KiB, which
-// finds room beside other waves.
-constexpr int kMeanChunkExclusive = 3328;
+// Chunk of the per-system sequential mean (doubles per axis and buffer; each
+// row is padded by 16 doubles so that the chain's read-ahead never needs a
+// bounds test): 2 x 3 x (1024 + 16) doubles = 48.75 KiB of LDS, which finds
+// room beside other waves.
 constexpr int kMeanChunkShared = 1024;
+// The one-system mean in parallel: tiles of 1024 bodies, at most 64 pieces per
+// tile and axis.
+constexpr int kSumTile = 1024;
+constexpr int kSumPieces = 64;
+constexpr int32_t kPieceSingle = INT32_MIN;
+constexpr uint32_t kTileSerial = 0xFFFFFFFFu;
 
 inline unsigned grid_for(size_t n, int per_block = kBlock, unsigned cap = 65535u * 16u) {
     size_t g = (n + per_block - 1) / per_block;
@@ -115,6 +118,353 @@ __global__ __launch_bounds__(kBlock) void k_bounds_mean(const double* __restrict
         out4[lane] = s;
     }
     if (tid == 0) out4[3] = 0.0;
+}
+
+// ---- the one-system mean: the same sequential sum, computed in parallel
+//
+// The sum s_k = fl(s_{k-1} + x_k) (s_0 = 0) is one dependent chain.  Within a
+// binade the rounding does not depend on where s is: let u = 2^(e-52) be the
+// spacing of the doubles in [2^e, 2^(e+1)), s_{k-1} a multiple of u, and
+// r_k = x_k/u rounded to the nearest integer, with x_k/u not halfway between
+// two integers.  Then t = s_{k-1} + u*r_k is within u/2 of the exact sum, and
+// when 2^e + u <= |t| <= 2^(e+1) - u no other double is that close, so
+// s_k = t.  A run of additions whose partial sums stay in one binade is
+// therefore s_end = s_start + u * (r_a + ... + r_b): an integer sum, which
+// tiles compute in parallel.  The chain that remains walks the runs in order
+// and, per run, checks the condition exactly from s_start and the least and
+// greatest prefix of the r's; a run that fails the check (or that holds a
+// halfway case, a non-finite value or a too-large increment) is added one
+// element at a time.  The result equals the sequential sum bit for bit,
+// whatever the estimate below; the estimate only decides where runs break.
+//
+// Estimated partial sums (any order of additions) choose each element's
+// binade; elements whose estimate is near a binade edge stand alone.
+
+__device__ __forceinline__ double readlane_d(double v, uint32_t j) {
+    const unsigned long long b = (unsigned long long)__double_as_longlong(v);
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)b, (int)j);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(b >> 32), (int)j);
+    return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+
+// Exclusive prefix over the 256 threads of a block (all threads call it);
+// `total` gets the block's sum.  s_w: 4 slots of shared memory.
+template <typename T>
+__device__ __forceinline__ T block_excl_scan(T v, T* s_w, T& total) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    T inc = v;
+    for (int o = 1; o < 64; o <<= 1) {
+        const T t = __shfl_up(inc, o, 64);
+        if (lane >= o) inc = inc + t;
+    }
+    if (lane == 63) s_w[wave] = inc;
+    __syncthreads();
+    T off = 0;
+    total = 0;
+    for (int w = 0; w < kBlock / 64; ++w) {
+        const T x = s_w[w];
+        if (w < wave) off = off + x;
+        total = total + x;
+    }
+    __syncthreads();
+    return off + (inc - v);
+}
+
+// Tile sums of the three axes (estimate: any order).
+__global__ __launch_bounds__(kBlock) void k_sum_tiles(const double* __restrict__ px, const double* __restrict__ py,
+                                                      const double* __restrict__ pz, size_t n,
+                                                      double* __restrict__ tsum) {
+    const size_t base = (size_t)blockIdx.x * kSumTile;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    double a0 = 0.0, a1 = 0.0, a2 = 0.0;
+    for (int i = threadIdx.x; i < kSumTile; i += kBlock) {
+        const size_t k = base + i;
+        if (k < n) {
+            a0 = a0 + px[k];
+            a1 = a1 + py[k];
+            a2 = a2 + pz[k];
+        }
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+        a0 = a0 + __shfl_down(a0, o, 64);
+        a1 = a1 + __shfl_down(a1, o, 64);
+        a2 = a2 + __shfl_down(a2, o, 64);
+    }
+    __shared__ double s_w[3][kBlock / 64];
+    if (lane == 0) {
+        s_w[0][wave] = a0;
+        s_w[1][wave] = a1;
+        s_w[2][wave] = a2;
+    }
+    __syncthreads();
+    if (threadIdx.x < 3) {
+        const double* w = s_w[threadIdx.x];
+        tsum[3 * (size_t)blockIdx.x + threadIdx.x] = (w[0] + w[1]) + (w[2] + w[3]);
+    }
+}
+
+// Exclusive prefix of the tile sums, per axis (one block).
+__global__ __launch_bounds__(kBlock) void k_sum_scan(const double* __restrict__ tsum, uint32_t ntiles,
+                                                     double* __restrict__ tpre) {
+    __shared__ double s_w[kBlock / 64];
+    for (int a = 0; a < 3; ++a) {
+        double carry = 0.0;
+        for (uint32_t t0 = 0; t0 < ntiles; t0 += kBlock) {
+            const uint32_t t = t0 + threadIdx.x;
+            const double v = t < ntiles ? tsum[3 * (size_t)t + a] : 0.0;
+            double total;
+            const double ex = block_excl_scan(v, s_w, total);
+            if (t < ntiles) tpre[3 * (size_t)t + a] = carry + ex;
+            carry = carry + total;
+        }
+    }
+}
+
+// Pieces of tile blockIdx.x, axis blockIdx.y: thread j holds elements 4j..4j+3.
+__global__ __launch_bounds__(kBlock) void k_sum_pieces(const double* __restrict__ px, const double* __restrict__ py,
+                                                       const double* __restrict__ pz, size_t n,
+                                                       const double* __restrict__ tpre, uint32_t* __restrict__ cnt,
+                                                       SumPiece* __restrict__ pieces) {
+    const uint32_t t = blockIdx.x, a = blockIdx.y;
+    const double* __restrict__ x = a == 0 ? px : (a == 1 ? py : pz);
+    const size_t base = (size_t)t * kSumTile;
+    const int count = (int)((n - base) < (size_t)kSumTile ? (n - base) : (size_t)kSumTile);
+    const int tid = threadIdx.x, lane = tid & 63;
+    __shared__ int32_t s_code[kSumTile];
+    __shared__ double s_wd[kBlock / 64];
+    __shared__ int s_wi[kBlock / 64];
+    __shared__ int s_pstart[kSumPieces];
+    __shared__ int32_t s_pcode[kSumPieces];
+    __shared__ double s_pbase[kSumPieces], s_plast[kSumPieces], s_pval[kSumPieces];
+    __shared__ long long s_pmin[kSumPieces], s_pmax[kSumPieces];
+
+    double v[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        const int k = 4 * tid + i;
+        v[i] = k < count ? x[base + k] : 0.0;
+    }
+    // estimated partial sums -> binade, increment and standing of each element
+    double loc[4], run = 0.0;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        run = run + v[i];
+        loc[i] = run;
+    }
+    double tot;
+    const double p0 = tpre[3 * (size_t)t + a] + block_excl_scan(run, s_wd, tot);
+    int32_t code[4];
+    double r[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        const double ap = fabs(p0 + loc[i]);
+        bool single = !(ap >= 0x1p-960 && ap < 0x1p1000);
+        int e = 0;
+        double rv = 0.0;
+        if (!single) {
+            int ex;
+            const double fm = frexp(ap, &ex);
+            e = ex - 1;
+            if (fm < 0.5 + 0x1p-25 || fm > 1.0 - 0x1p-24) single = true;
+            const double q = ldexp(v[i], 52 - e);
+            rv = rint(q);
+            if (!(fabs(rv) < 0x1p42) || fabs(q - rv) == 0.5) single = true;
+        }
+        code[i] = single ? kPieceSingle : e;
+        r[i] = single ? 0.0 : rv;
+        const int k = 4 * tid + i;
+        if (k < count) s_code[k] = code[i];
+    }
+    if (tid < kSumPieces) {
+        s_pmin[tid] = LLONG_MAX;
+        s_pmax[tid] = LLONG_MIN;
+    }
+    __syncthreads();
+    // piece boundaries: a change of binade, and around every single element
+    bool st[4], last[4];
+    int nst = 0;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        const int k = 4 * tid + i;
+        st[i] = false;
+        last[i] = false;
+        if (k < count) {
+            const int32_t prev = k > 0 ? s_code[k - 1] : 0;
+            st[i] = k == 0 || code[i] == kPieceSingle || prev == kPieceSingle || code[i] != prev;
+            const int32_t next = k + 1 < count ? s_code[k + 1] : 0;
+            last[i] = k + 1 == count || next == kPieceSingle || code[i] == kPieceSingle || next != code[i];
+        }
+        nst += st[i] ? 1 : 0;
+    }
+    int np;
+    const int pex = block_excl_scan(nst, s_wi, np);
+    if (np > kSumPieces) {   // (np is the same in every thread)
+        if (tid == 0) cnt[3 * (size_t)t + a] = kTileSerial;
+        return;
+    }
+    // exact prefix of the increments over the tile (integers below 2^52)
+    double rl[4], rr = 0.0;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        rr = rr + r[i];
+        rl[i] = rr;
+    }
+    double rtot;
+    const double rex = block_excl_scan(rr, s_wd, rtot);
+    int pid = pex - 1;
+    // least and greatest prefix per run piece: this thread's share first
+    int cur = -1;
+    long long mn = LLONG_MAX, mx = LLONG_MIN;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        const int k = 4 * tid + i;
+        if (k >= count) continue;
+        const double R = rex + rl[i];
+        if (st[i]) {
+            ++pid;
+            s_pstart[pid] = k;
+            s_pcode[pid] = code[i];
+            s_pbase[pid] = R - r[i];
+            s_pval[pid] = v[i];
+        }
+        if (last[i]) s_plast[pid] = R;
+        if (code[i] == kPieceSingle) continue;
+        if (pid != cur) {
+            if (cur >= 0) {
+                atomicMin(&s_pmin[cur], mn);
+                atomicMax(&s_pmax[cur], mx);
+            }
+            cur = pid;
+            mn = LLONG_MAX;
+            mx = LLONG_MIN;
+        }
+        const long long Ri = (long long)R;
+        mn = Ri < mn ? Ri : mn;
+        mx = Ri > mx ? Ri : mx;
+    }
+    // a wave whose run elements all lie in one piece reduces before its one atomic
+    const int c0 = __shfl(cur, 0, 64);
+    const bool same = (cur == c0 || cur < 0) && c0 >= 0;
+    if (__all(same)) {
+        for (int o = 32; o > 0; o >>= 1) {
+            const long long omn = __shfl_xor(mn, o, 64), omx = __shfl_xor(mx, o, 64);
+            mn = omn < mn ? omn : mn;
+            mx = omx > mx ? omx : mx;
+        }
+        if (lane == 0) {
+            atomicMin(&s_pmin[c0], mn);
+            atomicMax(&s_pmax[c0], mx);
+        }
+    } else if (cur >= 0) {
+        atomicMin(&s_pmin[cur], mn);
+        atomicMax(&s_pmax[cur], mx);
+    }
+    __syncthreads();
+    if (tid < np) {
+        SumPiece q;
+        const int start = s_pstart[tid];
+        const int end = tid + 1 < np ? s_pstart[tid + 1] : count;
+        q.e = s_pcode[tid];
+        q.span = (uint32_t)start | ((uint32_t)(end - start) << 16);
+        if (q.e == kPieceSingle) {
+            q.sum = s_pval[tid];
+            q.lo = 0.0;
+            q.hi = 0.0;
+        } else {
+            const double b = s_pbase[tid];
+            q.sum = s_plast[tid] - b;
+            q.lo = (double)s_pmin[tid] - b;
+            q.hi = (double)s_pmax[tid] - b;
+        }
+        pieces[(3 * (size_t)t + a) * kSumPieces + tid] = q;
+    }
+    if (tid == 0) cnt[3 * (size_t)t + a] = (uint32_t)np;
+}
+
+// s + x[0] + x[1] + ... + x[len-1], left to right (s is the same in every lane).
+__device__ double add_one_by_one(const double* __restrict__ x, int len, double s, int lane) {
+    for (int k0 = 0; k0 < len; k0 += 64) {
+        const int m = (len - k0) < 64 ? (len - k0) : 64;
+        const double v = lane < m ? x[k0 + lane] : 0.0;
+        if (m == 64) {
+#pragma unroll
+            for (int j = 0; j < 64; ++j) s = s + readlane_d(v, j);
+        } else {
+            for (int j = 0; j < m; ++j) s = s + readlane_d(v, j);
+        }
+    }
+    return s;
+}
+
+// One piece applied to the running sum (all arguments the same in every lane).
+__device__ __forceinline__ double add_piece(double s, double sum, double lo, double hi, int32_t e, uint32_t span,
+                                            const double* __restrict__ xt, int lane) {
+    if (e == kPieceSingle) return s + sum;
+    const double S = s * ldexp(1.0, 52 - e);   // exact: a power-of-two scaling
+    const double A = S + lo, B = S + hi;
+    const bool ok = S == rint(S) && ((A >= 0x1p52 + 1.0 && B <= 0x1p53 - 1.0) ||
+                                     (B <= -(0x1p52 + 1.0) && A >= -(0x1p53 - 1.0)));
+    if (ok) return (S + sum) * ldexp(1.0, e - 52);
+    return add_one_by_one(xt + (span & 0xFFFFu), (int)(span >> 16), s, lane);
+}
+
+// The chain over the pieces: wave a sums axis a.  Lane i holds the piece
+// count and first piece of tile t0 + i; the next 64 tiles' are loaded while
+// these are applied.
+__global__ __launch_bounds__(192) void k_sum_walk(const double* __restrict__ px, const double* __restrict__ py,
+                                                  const double* __restrict__ pz, size_t n, uint32_t ntiles,
+                                                  const uint32_t* __restrict__ cnt,
+                                                  const SumPiece* __restrict__ pieces, double* __restrict__ bounds4) {
+    const int a = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const double* __restrict__ x = a == 0 ? px : (a == 1 ? py : pz);
+    // the chain issues ahead of other kernels' waves on its SIMD
+    __builtin_amdgcn_s_setprio(3);
+    uint32_t nc = 0;
+    SumPiece nq{};
+    if ((uint32_t)lane < ntiles) {
+        nc = cnt[3 * (size_t)lane + a];
+        nq = pieces[(3 * (size_t)lane + a) * kSumPieces];
+    }
+    double s = 0.0;
+    for (uint32_t t0 = 0; t0 < ntiles; t0 += 64) {
+        const uint32_t cc = nc;
+        const SumPiece cq = nq;
+        const uint32_t tn = t0 + 64 + (uint32_t)lane;
+        if (tn < ntiles) {
+            nc = cnt[3 * (size_t)tn + a];
+            nq = pieces[(3 * (size_t)tn + a) * kSumPieces];
+        }
+        const uint32_t m = (ntiles - t0) < 64u ? (ntiles - t0) : 64u;
+        for (uint32_t i = 0; i < m; ++i) {
+            const size_t base = (size_t)(t0 + i) * kSumTile;
+            const double* __restrict__ xt = x + base;
+            const int count = (int)((n - base) < (size_t)kSumTile ? (n - base) : (size_t)kSumTile);
+            const uint32_t c = (uint32_t)__builtin_amdgcn_readlane((int)cc, (int)i);
+            if (c == kTileSerial) {
+                s = add_one_by_one(xt, count, s, lane);
+            } else if (c == 1) {
+                s = add_piece(s, readlane_d(cq.sum, i), readlane_d(cq.lo, i), readlane_d(cq.hi, i),
+                              __builtin_amdgcn_readlane(cq.e, (int)i),
+                              (uint32_t)__builtin_amdgcn_readlane((int)cq.span, (int)i), xt, lane);
+            } else {
+                SumPiece q{};
+                if ((uint32_t)lane < c) q = pieces[(3 * (size_t)(t0 + i) + a) * kSumPieces + lane];
+                for (uint32_t j = 0; j < c; ++j)
+                    s = add_piece(s, readlane_d(q.sum, j), readlane_d(q.lo, j), readlane_d(q.hi, j),
+                                  __builtin_amdgcn_readlane(q.e, (int)j),
+                                  (uint32_t)__builtin_amdgcn_readlane((int)q.span, (int)j), xt, lane);
+            }
+        }
+    }
+    if (lane == 0) {
+        const double dn = (double)n;
+        if (dn != 0.0) {
+            const double inv = 1.0 / dn;
+            s = s * inv;
+        }
+        bounds4[a] = s;
+        if (a == 0) bounds4[3] = 0.0;
+    }
 }
 
 // OctantBounds size — OctantUtils.cpp:47-58: max over bodies and axes of
@@ -356,15 +706,28 @@ __global__ __launch_bounds__(kBlock) void k_tree_gather(const double* __restrict
 
 }  // namespace
 
-hipError_t launch_bounds(bhx_ctx* c, double* bounds4, hipStream_t s, bool exclusive) {
+size_t sum_tiles_for(size_t n) { return (n + kSumTile - 1) / kSumTile + 1; }
+
+hipError_t launch_bounds(bhx_ctx* c, double* bounds4, hipStream_t s) {
     const SysView sv = c->sys();
     if (sv.nsys == 0 || c->n == 0) return hipSuccess;
-    if (exclusive && sv.nsys == 1)
-        hipLaunchKernelGGL(k_bounds_mean<kMeanChunkExclusive>, dim3(1), dim3(kBlock), 0, s, c->px, c->py, c->pz, sv,
-                           bounds4);
-    else
+    if (sv.nsys == 1) {
+        // one system: the chain over runs of additions (several systems run
+        // their short chains side by side below)
+        const uint32_t nt = (uint32_t)((c->n + kSumTile - 1) / kSumTile);
+        if (nt + 1 > c->sum_tiles_cap) return hipErrorInvalidValue;
+        double* tsum = c->sum_tile;
+        double* tpre = c->sum_tile + 3 * c->sum_tiles_cap;
+        hipLaunchKernelGGL(k_sum_tiles, dim3(nt), dim3(kBlock), 0, s, c->px, c->py, c->pz, c->n, tsum);
+        hipLaunchKernelGGL(k_sum_scan, dim3(1), dim3(kBlock), 0, s, tsum, nt, tpre);
+        hipLaunchKernelGGL(k_sum_pieces, dim3(nt, 3), dim3(kBlock), 0, s, c->px, c->py, c->pz, c->n, tpre,
+                           c->sum_cnt, c->sum_piece);
+        hipLaunchKernelGGL(k_sum_walk, dim3(1), dim3(192), 0, s, c->px, c->py, c->pz, c->n, nt, c->sum_cnt,
+                           c->sum_piece, bounds4);
+    } else {
         hipLaunchKernelGGL(k_bounds_mean<kMeanChunkShared>, dim3(sv.nsys), dim3(kBlock), 0, s, c->px, c->py, c->pz,
                            sv, bounds4);
+    }
     hipLaunchKernelGGL(k_bounds_size, dim3(grid_for(c->max_sys_n, kBlock * 8, 2048), sv.nsys), dim3(kBlock), 0, s,
                        c->px, c->py, c->pz, sv, bounds4);
     return hipGetLastError();

@@ -83,6 +83,16 @@ struct DeviceTree {
     uint32_t* hval = nullptr;
 };
 
+// One piece of a 1024-element tile of the bounds' sequential sum: a run of
+// additions whose partial sums share one binade (added at once, exactly), or
+// one plain addition.
+struct SumPiece {
+    double sum;       // run: sum of the increments in units of the binade's ulp; single: the value
+    double lo, hi;    // run: least and greatest prefix of those increments
+    int32_t e;        // run: binade exponent of its partial sums; INT32_MIN: one plain addition
+    uint32_t span;    // first element within the tile | element count << 16
+};
+
 struct Counters {          // small device-side status block
     uint32_t total_nodes;
     uint32_t unsplit;
@@ -196,6 +206,11 @@ struct bhx_ctx {
     uint32_t* sys_root = nullptr;              // root node of each system in the current tree
     double* sys_scale = nullptr;               // per system: FP32 walk length and mass factors
     uint4* gtab = nullptr;                     // walk groups: {first body, end body, system, 0}
+    // the sequential mean of one system: 1024-body tiles
+    size_t sum_tiles_cap = 0;
+    double* sum_tile = nullptr;                // tile sums (3 per tile), then their exclusive prefix
+    uint32_t* sum_cnt = nullptr;               // pieces per tile and axis
+    bhx::SumPiece* sum_piece = nullptr;        // 64 per tile and axis
     size_t gtab_cap = 0;
     uint32_t ngroups = 0;
     std::vector<double> tree_bounds_h;         // (centre, size) per system the last tree was built with
@@ -277,11 +292,11 @@ void free_grad(bhx_ctx* c);                           // recorded trajectory (bh
 
 // --- launchers implemented in the .hip files (all enqueue on `s`) ----------
 // OctantBounds of the resident bodies of every system into bounds4 (centre,
-// size: 4 doubles per system)
-// exclusive: the next frame's bounds on the frame's own side branch, where
-// nothing else keeps the GPU busy (one system only: the chain's workgroup then
-// takes a whole compute unit)
-hipError_t launch_bounds(bhx_ctx* c, double* bounds4, hipStream_t s, bool exclusive = false);
+// size: 4 doubles per system).  Uses the context's sum_* scratch: one bounds
+// computation at a time per context.
+hipError_t launch_bounds(bhx_ctx* c, double* bounds4, hipStream_t s);
+// tiles of the one-system sequential mean for n bodies
+size_t sum_tiles_for(size_t n);
 // nonfinite (optional): set to 1 when some position is not finite
 hipError_t launch_morton(const double* px, const double* py, const double* pz, size_t n, const uint32_t* sysid,
                          const double* bounds4, uint64_t* keys, uint32_t* vals, hipStream_t s,

@@ -6,6 +6,8 @@ as-run node payload (equal power-of-two masses make every mass sum exact).
 Tolerances (written at each check): accelerations, because the GPU sums the
 same interaction terms in a different order.
 """
+import zlib
+
 import numpy as np
 import pytest
 
@@ -41,6 +43,68 @@ def test_bounds_and_keys_bitexact(bhx, plummer16k):
         assert np.array_equal(c, oc) and s == os_
         keys = e.morton_keys(s)
         assert np.array_equal(keys, orc.morton_keys(pos, s))
+
+
+def _mean_case(bhx, name):
+    """Positions whose left-to-right sums are hard for a parallel evaluation:
+    random order (partial sums near zero: every tile added one by one), large
+    offsets, steady drift across many binades, halfway cases, sums hovering at
+    a binade edge, huge and subnormal magnitudes, non-finite values, ragged
+    sizes, and the Morton-ordered Plummer sphere the step sees."""
+    rng = np.random.default_rng(zlib.crc32(name.encode()))
+    if name.startswith("size"):
+        n = int(name[4:])
+        return bhx.plummer(n, seed=n)[0] if n > 1 else np.array([[0.3, -0.2, 0.1]])
+    n = 100003
+    if name == "random_order":
+        return rng.permutation(bhx.plummer(n, seed=2)[0])
+    if name == "morton_1m":
+        pos = bhx.plummer(1 << 20, seed=1)[0]
+        _, s = orc.bounds(pos)
+        return pos[np.argsort(orc.morton_keys(pos, s), kind="stable")]
+    if name == "offset":
+        return rng.standard_normal((n, 3)) + 1e8
+    if name == "drift":
+        return 1.0 + rng.uniform(-1, 1, (300000, 3))
+    if name == "halves":
+        return rng.integers(-1000, 1001, (n, 3)) * 0.25
+    if name == "hover":
+        x = np.empty((n, 3))
+        for a in range(3):
+            s = 0.0
+            for k in range(n):
+                v = 1024.0 if k == 0 else (1024.0 - s + 1e-9 * rng.uniform(-1, 1) if k % 2 else
+                                           1e-7 * rng.uniform(-1, 1))
+                x[k, a] = v
+                s = s + v
+        return x
+    if name == "huge":
+        return 1e300 * rng.uniform(-1, 1, (n, 3))
+    if name == "subnormal":
+        return 1e-310 * rng.uniform(-1, 1, (n, 3))
+    if name == "up_down":
+        sgn = np.where(np.arange(n) < n // 2, 1.0, -1.0)[:, None]
+        return sgn * (1.0 + 1e-3 * rng.uniform(-1, 1, (n, 3)))
+    if name == "nonfinite":
+        x = rng.standard_normal((n, 3))
+        x[5000, 0], x[70000, 1] = np.nan, np.inf
+        return x
+    raise ValueError(name)
+
+
+@pytest.mark.parametrize("name", ["size1", "size2", "size1023", "size1024", "size1025", "size2049", "random_order",
+                                  "morton_1m", "offset", "drift", "halves", "hover", "huge", "subnormal", "up_down",
+                                  "nonfinite"])
+def test_bounds_sequential_mean_bitexact(bhx, name):
+    """The one-system centre is the left-to-right FP64 sum (OctantUtils.cpp:31-38)
+    bit for bit, however the tiles split it into runs."""
+    pos = _mean_case(bhx, name)
+    n = pos.shape[0]
+    with _engine(bhx, n) as e:
+        e.set_bodies(pos, None, np.ones(n))
+        c, s = e.bounds()
+    oc, os_ = orc.bounds(pos)
+    assert c.tobytes() == oc.tobytes() and np.float64(s).tobytes() == np.float64(os_).tobytes()
 
 
 def test_keys_out_of_domain_x86_semantics(bhx):
