@@ -526,8 +526,8 @@ int bhx_create(const bhx_config* cfg, size_t capacity, bhx_ctx** out) {
         chk(dalloc(&c->sort_status, 8 * 256 * (c->sort_tiles + 1)), "alloc sort");
         c->sum_tiles_cap = sum_tiles_for(n);
         chk(dalloc(&c->sum_tile, 6 * c->sum_tiles_cap), "alloc bounds");
-        chk(dalloc(&c->sum_cnt, 3 * c->sum_tiles_cap), "alloc bounds");
-        chk(dalloc(&c->sum_piece, 3 * 64 * c->sum_tiles_cap), "alloc bounds");
+        chk(dalloc(&c->sum_cnt, 6 * c->sum_tiles_cap + 8), "alloc bounds");
+        chk(dalloc(&c->sum_piece, 6 * 64 * c->sum_tiles_cap), "alloc bounds");
         chk(dalloc(&c->ctr, 1), "alloc counters");
         chk(hipHostMalloc((void**)&c->ctr_host, sizeof(Counters), hipHostMallocDefault), "alloc pinned");
         if (rc == BHX_OK) chk(alloc_tree(c, 2 * n + 1024), "alloc tree");

@@ -18,8 +18,9 @@ constexpr int kMeanChunkShared = 1024;
 // tile and axis.
 constexpr int kSumTile = 1024;
 constexpr int kSumPieces = 64;
-constexpr int32_t kPieceSingle = INT32_MIN;
-constexpr uint32_t kTileSerial = 0xFFFFFFFFu;
+constexpr int32_t kCodeSingle = INT32_MIN;   // an element that stands alone
+constexpr int16_t kPieceSingle = -32768;     // SumPiece::e of one plain addition
+constexpr int16_t kPieceTile = -32767;       // SumPiece::e of a tile added element by element
 
 inline unsigned grid_for(size_t n, int per_block = kBlock, unsigned cap = 65535u * 16u) {
     size_t g = (n + per_block - 1) / per_block;
@@ -270,7 +271,7 @@ __global__ __launch_bounds__(kBlock) void k_sum_pieces(const double* __restrict_
             rv = rint(q);
             if (!(fabs(rv) < 0x1p42) || fabs(q - rv) == 0.5) single = true;
         }
-        code[i] = single ? kPieceSingle : e;
+        code[i] = single ? kCodeSingle : e;
         r[i] = single ? 0.0 : rv;
         const int k = 4 * tid + i;
         if (k < count) s_code[k] = code[i];
@@ -290,16 +291,24 @@ __global__ __launch_bounds__(kBlock) void k_sum_pieces(const double* __restrict_
         last[i] = false;
         if (k < count) {
             const int32_t prev = k > 0 ? s_code[k - 1] : 0;
-            st[i] = k == 0 || code[i] == kPieceSingle || prev == kPieceSingle || code[i] != prev;
+            st[i] = k == 0 || code[i] == kCodeSingle || prev == kCodeSingle || code[i] != prev;
             const int32_t next = k + 1 < count ? s_code[k + 1] : 0;
-            last[i] = k + 1 == count || next == kPieceSingle || code[i] == kPieceSingle || next != code[i];
+            last[i] = k + 1 == count || next == kCodeSingle || code[i] == kCodeSingle || next != code[i];
         }
         nst += st[i] ? 1 : 0;
     }
     int np;
     const int pex = block_excl_scan(nst, s_wi, np);
-    if (np > kSumPieces) {   // (np is the same in every thread)
-        if (tid == 0) cnt[3 * (size_t)t + a] = kTileSerial;
+    if (np > kSumPieces) {   // (np is the same in every thread): one piece, element by element
+        if (tid == 0) {
+            SumPiece q;
+            q.sum = q.lo = q.hi = 0.0;
+            q.e = kPieceTile;
+            q.len = (uint16_t)count;
+            q.start = (uint32_t)base;
+            pieces[(3 * (size_t)t + a) * kSumPieces] = q;
+            cnt[3 * (size_t)t + a] = 1u;
+        }
         return;
     }
     // exact prefix of the increments over the tile (integers below 2^52)
@@ -328,7 +337,7 @@ __global__ __launch_bounds__(kBlock) void k_sum_pieces(const double* __restrict_
             s_pval[pid] = v[i];
         }
         if (last[i]) s_plast[pid] = R;
-        if (code[i] == kPieceSingle) continue;
+        if (code[i] == kCodeSingle) continue;
         if (pid != cur) {
             if (cur >= 0) {
                 atomicMin(&s_pmin[cur], mn);
@@ -364,9 +373,11 @@ __global__ __launch_bounds__(kBlock) void k_sum_pieces(const double* __restrict_
         SumPiece q;
         const int start = s_pstart[tid];
         const int end = tid + 1 < np ? s_pstart[tid + 1] : count;
-        q.e = s_pcode[tid];
-        q.span = (uint32_t)start | ((uint32_t)(end - start) << 16);
-        if (q.e == kPieceSingle) {
+        const int32_t code = s_pcode[tid];
+        q.e = code == kCodeSingle ? kPieceSingle : (int16_t)code;
+        q.len = (uint16_t)(end - start);
+        q.start = (uint32_t)(base + start);
+        if (code == kCodeSingle) {
             q.sum = s_pval[tid];
             q.lo = 0.0;
             q.hi = 0.0;
@@ -381,80 +392,172 @@ __global__ __launch_bounds__(kBlock) void k_sum_pieces(const double* __restrict_
     if (tid == 0) cnt[3 * (size_t)t + a] = (uint32_t)np;
 }
 
-// s + x[0] + x[1] + ... + x[len-1], left to right (s is the same in every lane).
-__device__ double add_one_by_one(const double* __restrict__ x, int len, double s, int lane) {
-    for (int k0 = 0; k0 < len; k0 += 64) {
-        const int m = (len - k0) < 64 ? (len - k0) : 64;
-        const double v = lane < m ? x[k0 + lane] : 0.0;
-        if (m == 64) {
-#pragma unroll
-            for (int j = 0; j < 64; ++j) s = s + readlane_d(v, j);
-        } else {
-            for (int j = 0; j < m; ++j) s = s + readlane_d(v, j);
+// Exclusive prefix of the piece counts per axis (one block); entry ntiles
+// holds the axis's total.
+__global__ __launch_bounds__(kBlock) void k_sum_offsets(const uint32_t* __restrict__ cnt, uint32_t ntiles,
+                                                        uint32_t* __restrict__ off) {
+    __shared__ uint32_t s_w[kBlock / 64];
+    for (int a = 0; a < 3; ++a) {
+        uint32_t carry = 0;
+        for (uint32_t t0 = 0; t0 < ntiles; t0 += kBlock) {
+            const uint32_t t = t0 + threadIdx.x;
+            const uint32_t v = t < ntiles ? cnt[3 * (size_t)t + a] : 0u;
+            uint32_t total;
+            const uint32_t ex = block_excl_scan(v, s_w, total);
+            if (t < ntiles) off[(size_t)a * (ntiles + 1) + t] = carry + ex;
+            carry += total;
         }
+        if (threadIdx.x == 0) off[(size_t)a * (ntiles + 1) + ntiles] = carry;
+    }
+}
+
+// The pieces of every tile, one contiguous stream per axis.
+__global__ __launch_bounds__(64) void k_sum_compact(const uint32_t* __restrict__ cnt, const uint32_t* __restrict__ off,
+                                                    uint32_t ntiles, const SumPiece* __restrict__ pieces,
+                                                    SumPiece* __restrict__ out, size_t axis_stride) {
+    const uint32_t t = blockIdx.x, a = blockIdx.y;
+    const uint32_t c = cnt[3 * (size_t)t + a];
+    if (threadIdx.x < c)
+        out[(size_t)a * axis_stride + off[(size_t)a * (ntiles + 1) + t] + threadIdx.x] =
+            pieces[(3 * (size_t)t + a) * kSumPieces + threadIdx.x];
+}
+
+// s + buf[0] + buf[1] + ... + buf[len-1], left to right, every lane alike.
+// Register double buffering keeps the LDS reads of the next 16 values in
+// flight while the current 16 are added; buf holds 32 doubles of padding for
+// the last read-ahead.
+__device__ __forceinline__ double add_from_lds(const double* buf, int len, double s) {
+    int j = 0;
+    if (len >= 32) {
+        double t0[16], t1[16];
+#pragma unroll
+        for (int u = 0; u < 16; ++u) t0[u] = buf[u];
+        for (; j + 32 <= len; j += 32) {
+#pragma unroll
+            for (int u = 0; u < 16; ++u) t1[u] = buf[j + 16 + u];
+            __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+            for (int u = 0; u < 16; ++u) s = s + t0[u];
+            __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+            for (int u = 0; u < 16; ++u) t0[u] = buf[j + 32 + u];
+            __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+            for (int u = 0; u < 16; ++u) s = s + t1[u];
+            __builtin_amdgcn_sched_barrier(0);
+        }
+    }
+    for (; j < len; ++j) s = s + buf[j];
+    return s;
+}
+
+// Elements x[0 .. len) of one chunk (1 <= len <= 1024), 16 per lane (lane +
+// 64 k); entries past len repeat x[len-1] (never added).  The loads take no
+// branch and nothing reads them before they are written to LDS, so all 16
+// stay in flight while the previous chunk is added.
+__device__ __forceinline__ void load_chunk(const double* __restrict__ x, int len, int lane, double (&v)[16]) {
+#pragma unroll
+    for (int k = 0; k < 16; ++k) {
+        const int i = lane + 64 * k;
+        v[k] = x[i < len ? i : len - 1];
+    }
+}
+
+// One chunk into a 1024-entry LDS buffer.
+__device__ __forceinline__ void chunk_to_lds(const double (&v)[16], int lane, double* buf) {
+#pragma unroll
+    for (int k = 0; k < 16; ++k) buf[lane + 64 * k] = v[k];
+    // the wave's LDS writes complete before any lane reads them back
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+}
+
+// s + x[0] + x[1] + ... + x[len-1], left to right, every lane alike (len >= 1).
+// Chunks of up to 1024 elements pass through two LDS buffers: the next chunk's
+// loads go out before the current chunk's additions and are written to the
+// other buffer after them.
+__device__ __forceinline__ double add_range(const double* __restrict__ x, int len, double s, int lane,
+                                            double* buf0, double* buf1) {
+    int c0 = 0, cl = len < kSumTile ? len : kSumTile;
+    double v[16];
+    load_chunk(x, cl, lane, v);
+    chunk_to_lds(v, lane, buf0);
+    double* cur = buf0;
+    double* nxt = buf1;
+    for (;;) {
+        const int n0 = c0 + cl;
+        const int nl = (len - n0) < kSumTile ? (len - n0) : kSumTile;
+        if (nl > 0) load_chunk(x + n0, nl, lane, v);
+        s = add_from_lds(cur, cl, s);
+        if (nl <= 0) break;
+        chunk_to_lds(v, lane, nxt);
+        double* t = cur;
+        cur = nxt;
+        nxt = t;
+        c0 = n0;
+        cl = nl;
     }
     return s;
 }
 
-// One piece applied to the running sum (all arguments the same in every lane).
-__device__ __forceinline__ double add_piece(double s, double sum, double lo, double hi, int32_t e, uint32_t span,
-                                            const double* __restrict__ xt, int lane) {
-    if (e == kPieceSingle) return s + sum;
-    const double S = s * ldexp(1.0, 52 - e);   // exact: a power-of-two scaling
-    const double A = S + lo, B = S + hi;
-    const bool ok = S == rint(S) && ((A >= 0x1p52 + 1.0 && B <= 0x1p53 - 1.0) ||
-                                     (B <= -(0x1p52 + 1.0) && A >= -(0x1p53 - 1.0)));
-    if (ok) return (S + sum) * ldexp(1.0, e - 52);
-    return add_one_by_one(xt + (span & 0xFFFFu), (int)(span >> 16), s, lane);
-}
-
-// The chain over the pieces: wave a sums axis a.  Lane i holds the piece
-// count and first piece of tile t0 + i; the next 64 tiles' are loaded while
-// these are applied.
-__global__ __launch_bounds__(192) void k_sum_walk(const double* __restrict__ px, const double* __restrict__ py,
+// The chain over the pieces: workgroup a (one wave) sums axis a, so the three
+// chains issue from different compute units.  Lane j holds piece g0 + j
+// of the axis's stream; the next 64 are loaded while these are applied.
+// Consecutive whole-tile pieces are consecutive tiles, added as one range.
+__global__ __launch_bounds__(64) void k_sum_walk(const double* __restrict__ px, const double* __restrict__ py,
                                                   const double* __restrict__ pz, size_t n, uint32_t ntiles,
-                                                  const uint32_t* __restrict__ cnt,
-                                                  const SumPiece* __restrict__ pieces, double* __restrict__ bounds4) {
-    const int a = threadIdx.x >> 6, lane = threadIdx.x & 63;
+                                                  const uint32_t* __restrict__ off,
+                                                  const SumPiece* __restrict__ stream, size_t axis_stride,
+                                                  double* __restrict__ bounds4) {
+    const int a = (int)blockIdx.x;
+    const int lane = threadIdx.x;
     const double* __restrict__ x = a == 0 ? px : (a == 1 ? py : pz);
+    __shared__ double s_buf[2][kSumTile + 32];
     // the chain issues ahead of other kernels' waves on its SIMD
     __builtin_amdgcn_s_setprio(3);
-    uint32_t nc = 0;
-    SumPiece nq{};
-    if ((uint32_t)lane < ntiles) {
-        nc = cnt[3 * (size_t)lane + a];
-        nq = pieces[(3 * (size_t)lane + a) * kSumPieces];
-    }
+    const uint32_t np = (uint32_t)__builtin_amdgcn_readfirstlane((int)off[(size_t)a * (ntiles + 1) + ntiles]);
+    const SumPiece* __restrict__ ps = stream + (size_t)a * axis_stride;
+    SumPiece q{}, nq{};
+    if ((uint32_t)lane < np) q = ps[lane];
+    if (64u + lane < np) nq = ps[64 + lane];
     double s = 0.0;
-    for (uint32_t t0 = 0; t0 < ntiles; t0 += 64) {
-        const uint32_t cc = nc;
-        const SumPiece cq = nq;
-        const uint32_t tn = t0 + 64 + (uint32_t)lane;
-        if (tn < ntiles) {
-            nc = cnt[3 * (size_t)tn + a];
-            nq = pieces[(3 * (size_t)tn + a) * kSumPieces];
-        }
-        const uint32_t m = (ntiles - t0) < 64u ? (ntiles - t0) : 64u;
-        for (uint32_t i = 0; i < m; ++i) {
-            const size_t base = (size_t)(t0 + i) * kSumTile;
-            const double* __restrict__ xt = x + base;
-            const int count = (int)((n - base) < (size_t)kSumTile ? (n - base) : (size_t)kSumTile);
-            const uint32_t c = (uint32_t)__builtin_amdgcn_readlane((int)cc, (int)i);
-            if (c == kTileSerial) {
-                s = add_one_by_one(xt, count, s, lane);
-            } else if (c == 1) {
-                s = add_piece(s, readlane_d(cq.sum, i), readlane_d(cq.lo, i), readlane_d(cq.hi, i),
-                              __builtin_amdgcn_readlane(cq.e, (int)i),
-                              (uint32_t)__builtin_amdgcn_readlane((int)cq.span, (int)i), xt, lane);
-            } else {
-                SumPiece q{};
-                if ((uint32_t)lane < c) q = pieces[(3 * (size_t)(t0 + i) + a) * kSumPieces + lane];
-                for (uint32_t j = 0; j < c; ++j)
-                    s = add_piece(s, readlane_d(q.sum, j), readlane_d(q.lo, j), readlane_d(q.hi, j),
-                                  __builtin_amdgcn_readlane(q.e, (int)j),
-                                  (uint32_t)__builtin_amdgcn_readlane((int)q.span, (int)j), xt, lane);
+    for (uint32_t g0 = 0; g0 < np; g0 += 64) {
+        const uint32_t m = (np - g0) < 64u ? (np - g0) : 64u;
+        const unsigned long long tiles = __ballot((uint32_t)lane < m && q.e == kPieceTile);
+        for (uint32_t j = 0; j < m; ++j) {
+            const uint32_t w = (uint32_t)__builtin_amdgcn_readlane((int)(((uint32_t)(uint16_t)q.e) |
+                                                                       ((uint32_t)q.len << 16)), (int)j);
+            const int e = (int16_t)(w & 0xFFFFu);
+            if (e == kPieceSingle) {
+                s = s + readlane_d(q.sum, j);
+                continue;
             }
+            const uint32_t start = (uint32_t)__builtin_amdgcn_readlane((int)q.start, (int)j);
+            if (e == kPieceTile) {
+                // whole-tile pieces j .. j+k-1 (this block): one contiguous range
+                const unsigned long long rest = ~(tiles >> j);
+                uint32_t k = rest ? (uint32_t)__builtin_ctzll(rest) : 64u - j;
+                k = k < m - j ? k : m - j;
+                const uint32_t jl = j + k - 1;
+                const uint32_t wl = (uint32_t)__builtin_amdgcn_readlane((int)q.len, (int)jl) & 0xFFFFu;
+                const uint32_t sl = (uint32_t)__builtin_amdgcn_readlane((int)q.start, (int)jl);
+                s = add_range(x + start, (int)(sl + wl - start), s, lane, s_buf[0], s_buf[1]);
+                j = jl;
+                continue;
+            }
+            // a run: one exact addition when its partial sums stay in the binade
+            const double sum = readlane_d(q.sum, j), lo = readlane_d(q.lo, j), hi = readlane_d(q.hi, j);
+            const double S = s * ldexp(1.0, 52 - e);   // exact: a power-of-two scaling
+            const double A = S + lo, B = S + hi;
+            const bool ok = S == rint(S) && ((A >= 0x1p52 + 1.0 && B <= 0x1p53 - 1.0) ||
+                                             (B <= -(0x1p52 + 1.0) && A >= -(0x1p53 - 1.0)));
+            if (ok)
+                s = (S + sum) * ldexp(1.0, e - 52);
+            else   // element by element
+                s = add_range(x + start, (int)(w >> 16), s, lane, s_buf[0], s_buf[1]);
         }
+        q = nq;
+        if (g0 + 128u + lane < np) nq = ps[g0 + 128u + lane];
     }
     if (lane == 0) {
         const double dn = (double)n;
@@ -720,10 +823,17 @@ hipError_t launch_bounds(bhx_ctx* c, double* bounds4, hipStream_t s) {
         double* tpre = c->sum_tile + 3 * c->sum_tiles_cap;
         hipLaunchKernelGGL(k_sum_tiles, dim3(nt), dim3(kBlock), 0, s, c->px, c->py, c->pz, c->n, tsum);
         hipLaunchKernelGGL(k_sum_scan, dim3(1), dim3(kBlock), 0, s, tsum, nt, tpre);
-        hipLaunchKernelGGL(k_sum_pieces, dim3(nt, 3), dim3(kBlock), 0, s, c->px, c->py, c->pz, c->n, tpre,
-                           c->sum_cnt, c->sum_piece);
-        hipLaunchKernelGGL(k_sum_walk, dim3(1), dim3(192), 0, s, c->px, c->py, c->pz, c->n, nt, c->sum_cnt,
-                           c->sum_piece, bounds4);
+        uint32_t* cnt = c->sum_cnt;
+        uint32_t* off = c->sum_cnt + 3 * c->sum_tiles_cap;
+        SumPiece* pieces = c->sum_piece;
+        SumPiece* stream = c->sum_piece + 3 * (size_t)kSumPieces * c->sum_tiles_cap;
+        const size_t axis_stride = (size_t)kSumPieces * c->sum_tiles_cap;
+        hipLaunchKernelGGL(k_sum_pieces, dim3(nt, 3), dim3(kBlock), 0, s, c->px, c->py, c->pz, c->n, tpre, cnt,
+                           pieces);
+        hipLaunchKernelGGL(k_sum_offsets, dim3(1), dim3(kBlock), 0, s, cnt, nt, off);
+        hipLaunchKernelGGL(k_sum_compact, dim3(nt, 3), dim3(64), 0, s, cnt, off, nt, pieces, stream, axis_stride);
+        hipLaunchKernelGGL(k_sum_walk, dim3(3), dim3(64), 0, s, c->px, c->py, c->pz, c->n, nt, off, stream,
+                           axis_stride, bounds4);
     } else {
         hipLaunchKernelGGL(k_bounds_mean<kMeanChunkShared>, dim3(sv.nsys), dim3(kBlock), 0, s, c->px, c->py, c->pz,
                            sv, bounds4);

@@ -89,8 +89,10 @@ struct DeviceTree {
 struct SumPiece {
     double sum;       // run: sum of the increments in units of the binade's ulp; single: the value
     double lo, hi;    // run: least and greatest prefix of those increments
-    int32_t e;        // run: binade exponent of its partial sums; INT32_MIN: one plain addition
-    uint32_t span;    // first element within the tile | element count << 16
+    int16_t e;        // run: binade exponent of its partial sums; -32768: one plain addition;
+                      // -32767: a whole tile added element by element
+    uint16_t len;     // elements
+    uint32_t start;   // first element (body index)
 };
 
 struct Counters {          // small device-side status block
@@ -209,8 +211,8 @@ struct bhx_ctx {
     // the sequential mean of one system: 1024-body tiles
     size_t sum_tiles_cap = 0;
     double* sum_tile = nullptr;                // tile sums (3 per tile), then their exclusive prefix
-    uint32_t* sum_cnt = nullptr;               // pieces per tile and axis
-    bhx::SumPiece* sum_piece = nullptr;        // 64 per tile and axis
+    uint32_t* sum_cnt = nullptr;               // pieces per tile and axis, then their exclusive prefix
+    bhx::SumPiece* sum_piece = nullptr;        // 64 per tile and axis, then the same compacted per axis
     size_t gtab_cap = 0;
     uint32_t ngroups = 0;
     std::vector<double> tree_bounds_h;         // (centre, size) per system the last tree was built with
