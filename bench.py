@@ -68,6 +68,38 @@ def walk_kernel_ms(e, reps):
     return float(np.mean(ms)), e.last_timing()
 
 
+# Algorithmic HBM bytes per body of each timed phase of the FP32 step (compensated KDK, accelerations
+# permuted with the bodies, one system); r = tree nodes per body.
+PHASE_BYTES = {
+    "keys": lambda r: 36.0,                 # positions in; key and index out
+    "sort": lambda r: 200.0,                # 8 passes x (key + index in and out) + one key read for the histograms
+    "kick_drift": lambda r: 236.0,          # index, pos, vel, mass, acc, both compensations in; all but acc out
+    "treekeys": lambda r: 304.0,            # tree keys (36) + their sort (200) + tree-order gather (68)
+    "build": lambda r: 40.0 + 208.0 * r,    # keys and tree-order bodies; node records, hash slots, moments
+    "walk": lambda r: 56.0 + 32.0 * r,      # bodies in, accelerations out, each node record once
+    "kick": lambda r: 120.0,                # velocity, its compensation and acceleration in; both out
+}
+BOUNDS_BYTES = 72.0                         # positions read three times (two for the mean, one for the size)
+PEAK_HBM_GBPS = 8000.0
+
+
+def phase_roofline(phases, n, nodes, step_ms):
+    """Per phase: algorithmic bytes, the rate its event time gives and the fraction of peak HBM; and the
+    same for the whole step (bounds included: they run on a side stream inside the step)."""
+    r = nodes / n
+    out = {}
+    for k, f in PHASE_BYTES.items():
+        ms = phases.get(k, 0.0)
+        if ms > 0:
+            gbs = f(r) * n / (ms * 1e-3) / 1e9
+            out[k] = {"bytes_per_body": round(f(r), 1), "ms": round(ms, 4), "GBps": round(gbs, 1),
+                      "frac": round(gbs / PEAK_HBM_GBPS, 4)}
+    per_body = BOUNDS_BYTES + sum(f(r) for f in PHASE_BYTES.values())
+    gbs = per_body * n / (step_ms * 1e-3) / 1e9
+    return out, {"bytes_per_body": round(per_body, 1), "nodes_per_body": round(r, 4), "GBps": round(gbs, 1),
+                 "frac": round(gbs / PEAK_HBM_GBPS, 4)}
+
+
 def force_error(e, bhx, nsub, seed, eps):
     n = e.n
     rng = np.random.default_rng(seed)
@@ -364,13 +396,15 @@ def main():
         ms32 = 1e3 * t32 / args.steps
         log("fp32 %.3f ms/step" % ms32)
         walk_ms, phases = walk_kernel_ms(e, 3)
+        nodes = len(e.nodes())
         # algorithmic work: the reference's own per-body interaction lists on this tree;
         # the extra entries of the conservative group lists count as overhead, not work
         inter_ref = float(e.interaction_counts().astype(np.float64).sum())
         acc, cnt = e.walk(counts=True)
         inter = float(cnt.astype(np.float64).sum())
         err32 = force_error(e, bhx, 4096, 1234, eps)
-        out["fp32"] = dict(ms=ms32, walk_ms=walk_ms, phases=phases, inter=inter, inter_ref=inter_ref, err=err32)
+        out["fp32"] = dict(ms=ms32, walk_ms=walk_ms, phases=phases, inter=inter, inter_ref=inter_ref, err=err32,
+                           nodes=nodes)
         if not args.no_fp64:
             e.set_config(precision=bhx.FP64)
             log("fp64: warmup %d, timing %d steps" % (args.warmup, args.steps))
@@ -408,6 +442,8 @@ def main():
         "force_error_fp32": out["fp32"]["err"],
         "phases_ms_fp32": {k: round(v, 4) for k, v in out["fp32"]["phases"].items()},
     }
+    line["phase_roofline_fp32"], line["step_hbm_fp32"] = phase_roofline(out["fp32"]["phases"], n,
+                                                                        out["fp32"]["nodes"], ms32)
     if "fp64" in out:
         f64 = out["fp64"]
         ach64 = f64["inter_ref"] * FLOPS_PER_INTERACTION / (f64["walk_ms"] * 1e-3) / 1e12
