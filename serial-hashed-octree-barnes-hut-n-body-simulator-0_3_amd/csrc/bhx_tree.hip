@@ -413,6 +413,7 @@ struct Payload {
 };
 
 struct ChildFromGlobal {
+    static constexpr bool kPreload = true;   // far memory: load all children before summing
     const TreeArgs& t;
     __device__ double mass(uint32_t c) const { return t.mass[c]; }
     __device__ double com(uint32_t c, int k) const { return t.com[3 * (size_t)c + k]; }
@@ -421,6 +422,7 @@ struct ChildFromGlobal {
 };
 
 struct ChildFromSlice {
+    static constexpr bool kPreload = false;  // LDS: child by child keeps registers (and occupancy) low
     const double* m;
     const double (*c)[3];
     const uint32_t* f;
@@ -495,29 +497,72 @@ __device__ Payload moments_of(uint32_t p, int intended, const TreeArgs& t, const
             // fetched independently rather than along the pre-order skip chain
             const uint4 ca = *reinterpret_cast<const uint4*>(t.child + 8 * (size_t)p);
             const uint4 cb4 = *reinterpret_cast<const uint4*>(t.child + 8 * (size_t)p + 4);
-            const uint32_t ids[8] = {ca.x, ca.y, ca.z, ca.w, cb4.x, cb4.y, cb4.z, cb4.w};
+            const uint32_t ids0[8] = {ca.x, ca.y, ca.z, ca.w, cb4.x, cb4.y, cb4.z, cb4.w};
+            if constexpr (Child::kPreload) {
+                // every child's record is loaded before any sum uses it (an absent
+                // octant reads the node's own slot, never used), so the loads
+                // overlap instead of one round trip per child
+                uint32_t ids[8];
+                double mc[8], c0[8], c1[8], c2[8];
+                uint32_t fc[8];
 #pragma unroll
-            for (int o = 0; o < 8; ++o) {
-                if (!((cb >> o) & 1u)) continue;
-                const uint32_t c = ids[o];
-                const double mc = ch.mass(c);
-                M += mc;
-                sx += mc * ch.com(c, 0);
-                sy += mc * ch.com(c, 1);
-                sz += mc * ch.com(c, 2);
-                const uint32_t fc = ch.first(c);
-                if (fc < first) first = fc;
-            }
-            if (!intended) {
-                cx = px[first]; cy = py[first]; cz = pz[first];
-            } else {
-                cx = sx / M; cy = sy / M; cz = sz / M;
+                for (int o = 0; o < 8; ++o) {
+                    ids[o] = ((cb >> o) & 1u) ? ids0[o] : p;
+                    mc[o] = ch.mass(ids[o]);
+                    c0[o] = ch.com(ids[o], 0);
+                    c1[o] = ch.com(ids[o], 1);
+                    c2[o] = ch.com(ids[o], 2);
+                    fc[o] = ch.first(ids[o]);
+                }
 #pragma unroll
                 for (int o = 0; o < 8; ++o) {
                     if (!((cb >> o) & 1u)) continue;
-                    const uint32_t c = ids[o];
-                    quad_add(q, ch.mass(c), ch.com(c, 0) - cx, ch.com(c, 1) - cy, ch.com(c, 2) - cz);
-                    for (int k = 0; k < 6; ++k) q[k] += ch.quad(c, k);
+                    M += mc[o];
+                    sx += mc[o] * c0[o];
+                    sy += mc[o] * c1[o];
+                    sz += mc[o] * c2[o];
+                    if (fc[o] < first) first = fc[o];
+                }
+                if (!intended) {
+                    cx = px[first]; cy = py[first]; cz = pz[first];
+                } else {
+                    cx = sx / M; cy = sy / M; cz = sz / M;
+                    double qc[8][6];
+#pragma unroll
+                    for (int o = 0; o < 8; ++o)
+#pragma unroll
+                        for (int k = 0; k < 6; ++k) qc[o][k] = ch.quad(ids[o], k);
+#pragma unroll
+                    for (int o = 0; o < 8; ++o) {
+                        if (!((cb >> o) & 1u)) continue;
+                        quad_add(q, mc[o], c0[o] - cx, c1[o] - cy, c2[o] - cz);
+                        for (int k = 0; k < 6; ++k) q[k] += qc[o][k];
+                    }
+                }
+            } else {
+#pragma unroll
+                for (int o = 0; o < 8; ++o) {
+                    if (!((cb >> o) & 1u)) continue;
+                    const uint32_t c = ids0[o];
+                    const double mcc = ch.mass(c);
+                    M += mcc;
+                    sx += mcc * ch.com(c, 0);
+                    sy += mcc * ch.com(c, 1);
+                    sz += mcc * ch.com(c, 2);
+                    const uint32_t fcc = ch.first(c);
+                    if (fcc < first) first = fcc;
+                }
+                if (!intended) {
+                    cx = px[first]; cy = py[first]; cz = pz[first];
+                } else {
+                    cx = sx / M; cy = sy / M; cz = sz / M;
+#pragma unroll
+                    for (int o = 0; o < 8; ++o) {
+                        if (!((cb >> o) & 1u)) continue;
+                        const uint32_t c = ids0[o];
+                        quad_add(q, ch.mass(c), ch.com(c, 0) - cx, ch.com(c, 1) - cy, ch.com(c, 2) - cz);
+                        for (int k = 0; k < 6; ++k) q[k] += ch.quad(c, k);
+                    }
                 }
             }
         }
@@ -646,7 +691,7 @@ __global__ __launch_bounds__(kBlock) void k_moments_local(const Counters* __rest
     }
 }
 
-__global__ __launch_bounds__(1024) void k_moments_rest(const Counters* __restrict__ ctr, SysView sv, int intended,
+__global__ __launch_bounds__(256) void k_moments_rest(const Counters* __restrict__ ctr, SysView sv, int intended,
                                                        TreeArgs t, const double* __restrict__ bounds4,
                                                        const double* __restrict__ px, const double* __restrict__ py,
                                                        const double* __restrict__ pz, const double* __restrict__ tx,
@@ -759,7 +804,7 @@ hipError_t build_tree(bhx_ctx* c, hipStream_t s, size_t* total_out, bool sync) {
                        c->tm, c->sys_scale);
     hipLaunchKernelGGL(k_moments_local, dim3(slices), dim3(kBlock), 0, s, c->ctr, sv, intended, t, b4, c->px, c->py,
                        c->pz, c->tx, c->ty, c->tz, c->tm, c->tperm, c->tree.by_level, nrest);
-    hipLaunchKernelGGL(k_moments_rest, dim3(1), dim3(1024), 0, s, c->ctr, sv, intended, t, b4, c->px, c->py, c->pz,
+    hipLaunchKernelGGL(k_moments_rest, dim3(1), dim3(256), 0, s, c->ctr, sv, intended, t, b4, c->px, c->py, c->pz,
                        c->tx, c->ty, c->tz, c->tm, c->tperm, c->tree.by_level, nrest, (uint32_t)slices);
     return hipGetLastError();
 }
