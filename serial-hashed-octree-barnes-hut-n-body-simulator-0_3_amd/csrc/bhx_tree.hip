@@ -234,11 +234,11 @@ __device__ __forceinline__ void descend_center(uint64_t key, int L, const double
     }
 }
 
-// Writes every node: key, body range, pre-order skip, parent, level, centre
-// (createChildNode / HOTNode child constructor, HashedOctree.cpp:697-711,
-// HashedNode.cpp:48-61).  Node ids: the nodes starting at body i are
-// off[i] + (root first, then levels lo, lo+1, ...); a subtree ends where the
-// nodes of its end body begin.
+// Writes every node's key, first body, level and centre, the roots whole,
+// and the parent links that need no search (createChildNode / HOTNode child
+// constructor, HashedOctree.cpp:697-711, HashedNode.cpp:48-61).  Node ids:
+// the nodes starting at body i are off[i] + (root first, then levels lo,
+// lo+1, ...); k_emit_links finishes each node's run.
 __global__ __launch_bounds__(kBlock) void k_emit(const uint64_t* __restrict__ keys, uint32_t n, int B, SysView sv,
                                                  const uint32_t* __restrict__ nn, const uint32_t* __restrict__ nlo,
                                                  const uint32_t* __restrict__ off, const double* __restrict__ bounds4,
@@ -274,28 +274,52 @@ __global__ __launch_bounds__(kBlock) void k_emit(const uint64_t* __restrict__ ke
         for (int L = lo; L < lo + (int)cnt; ++L, ++id) {
             if (id >= cap) break;
             const uint64_t k = prefix(keys[i], L);
-            const uint32_t end = run_end(keys, lim, i, L);
-            uint32_t par;
-            if (L > lo) par = id - 1;
-            else if (L == 1) par = off[base];
-            else {
-                const uint32_t ps = run_start(keys, base, i, L - 1);
-                par = off[ps] + (ps == base ? 1u : 0u) + (uint32_t)(L - 1 - (int)nlo[ps]);
-            }
             t.key[id] = k;
             t.start[id] = i;
-            t.count[id] = end - i;
-            t.skip[id] = end < n ? off[end] : total;
-            t.parent[id] = par;
             t.level[id] = (uint32_t)L;
-            atomicOr(&t.child_byte[par], 1u << (unsigned)(k & 7));
-            t.child[8 * (size_t)par + (unsigned)(k & 7)] = id;   // one writer per slot
+            if (L > lo || L == 1) {   // the parent is the level above here, or the root
+                const uint32_t par = L > lo ? id - 1 : off[base];
+                t.parent[id] = par;
+                atomicOr(&t.child_byte[par], 1u << (unsigned)(k & 7));
+                t.child[8 * (size_t)par + (unsigned)(k & 7)] = id;   // one writer per slot
+            }
             double cx, cy, cz, s;
             descend_center(k, L, b4, t.tscale, cx, cy, cz, s);
             t.center[3 * (size_t)id] = cx;
             t.center[3 * (size_t)id + 1] = cy;
             t.center[3 * (size_t)id + 2] = cz;
-            if (L == kMaxLevel && (int)(end - i) > B) atomicAdd(&ctr->unsplit, 1u);
+        }
+    }
+}
+
+// One thread per node below the roots: the level-L run of its first body
+// gives its body count and pre-order skip; the shallowest node a body starts
+// (below level 1) finds its parent in the run of the level above.  Per node
+// rather than per body, so the long searches of the top levels run side by
+// side instead of one after another in the thread of a run's first body.
+__global__ __launch_bounds__(kBlock) void k_emit_links(const uint64_t* __restrict__ keys, uint32_t n, int B,
+                                                       SysView sv, const uint32_t* __restrict__ nlo,
+                                                       const uint32_t* __restrict__ off, uint32_t cap, TreeArgs t,
+                                                       Counters* __restrict__ ctr) {
+    const uint32_t total = ctr->total_nodes;
+    const uint32_t nodes = total < cap ? total : cap;
+    for (size_t id = blockIdx.x * (size_t)kBlock + threadIdx.x; id < nodes; id += (size_t)gridDim.x * kBlock) {
+        const int L = (int)t.level[id];
+        if (L == 0) continue;
+        const uint32_t i = t.start[id];
+        const uint32_t sys = sys_of(sv, i);
+        const uint32_t base = sv.off[sys], lim = sv.off[sys + 1];
+        const uint32_t end = run_end(keys, lim, i, L);
+        t.count[id] = end - i;
+        t.skip[id] = end < n ? off[end] : total;
+        if (L == kMaxLevel && (int)(end - i) > B) atomicAdd(&ctr->unsplit, 1u);
+        if (L == (int)nlo[i] && L > 1) {
+            const uint32_t ps = run_start(keys, base, i, L - 1);
+            const uint32_t par = off[ps] + (ps == base ? 1u : 0u) + (uint32_t)(L - 1 - (int)nlo[ps]);
+            const uint64_t k = t.key[id];
+            t.parent[id] = par;
+            atomicOr(&t.child_byte[par], 1u << (unsigned)(k & 7));
+            t.child[8 * (size_t)par + (unsigned)(k & 7)] = (uint32_t)id;   // one writer per slot
         }
     }
 }
@@ -789,6 +813,8 @@ hipError_t build_tree(bhx_ctx* c, hipStream_t s, size_t* total_out, bool sync) {
     hipLaunchKernelGGL(k_fill_u32, dim3(grid_for(span)), dim3(kBlock), 0, s, c->tree.child_byte, span, 0u);
     hipLaunchKernelGGL(k_emit, dim3(grid_for(n)), dim3(kBlock), 0, s, c->tkeys, n, B, sv, c->nn, c->nlo, c->noff, b4,
                        cap, t, c->sys_root, c->ctr);
+    hipLaunchKernelGGL(k_emit_links, dim3(grid_for(span)), dim3(kBlock), 0, s, c->tkeys, n, B, sv, c->nlo, c->noff,
+                       cap, t, c->ctr);
     hipLaunchKernelGGL(k_fill_u64, dim3(grid_for(c->tree.hcap)), dim3(kBlock), 0, s, c->tree.hkey, c->tree.hcap,
                        0ull);
     hipLaunchKernelGGL(k_hash_insert, dim3(grid_for(span)), dim3(kBlock), 0, s, c->tree.key, c->tree.start, sv,
