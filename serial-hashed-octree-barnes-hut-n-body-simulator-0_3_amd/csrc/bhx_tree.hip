@@ -473,6 +473,10 @@ __device__ Payload moments_of(uint32_t p, int intended, const TreeArgs& t, const
         double q[6] = {0, 0, 0, 0, 0, 0};
         uint32_t first = 0xFFFFFFFFu;
         double cx, cy, cz;
+        // the FP32 copies are in the system's scaled units (exact powers of two)
+        const uint32_t sysn = sys_of(sv, st);
+        const double lam = t.scale32[2 * (size_t)sysn], mu = t.scale32[2 * (size_t)sysn + 1];
+        bool bodies_done = false;   // the leaf's FP32 body copies already written
         if (leaf) {
             // bodies in batches of 8 with every load of a batch in flight before
             // the (sequential, in body order) sums use them
@@ -501,6 +505,7 @@ __device__ Payload moments_of(uint32_t p, int intended, const TreeArgs& t, const
                 cx = sx / M; cy = sy / M; cz = sz / M;
             }
             if (intended && cnt > 1) {
+                // quadrupole and the FP32 body copies from one pass over the bodies
                 for (uint32_t j0 = st; j0 < e; j0 += 8) {
                     double mj[8], xj[8], yj[8], zj[8];
 #pragma unroll
@@ -511,9 +516,14 @@ __device__ Payload moments_of(uint32_t p, int intended, const TreeArgs& t, const
 #pragma unroll
                     for (int k = 0; k < 8; ++k) {
                         if (j0 + k >= e) break;
-                        quad_add(q, mj[k], xj[k] - cx, yj[k] - cy, zj[k] - cz);
+                        const double dx = xj[k] - cx, dy = yj[k] - cy, dz = zj[k] - cz;
+                        quad_add(q, mj[k], dx, dy, dz);
+                        t.tb4[j0 + k] = make_float4((float)(dx * lam), (float)(dy * lam), (float)(dz * lam),
+                                                    (float)(mj[k] * mu));
+                        t.tleaf[j0 + k] = p;
                     }
                 }
+                bodies_done = true;
             }
         } else {
             // children in octant order = pre-order: the sums keep their order; the
@@ -595,9 +605,6 @@ __device__ Payload moments_of(uint32_t p, int intended, const TreeArgs& t, const
         t.com[3 * (size_t)p] = cx;
         t.com[3 * (size_t)p + 1] = cy;
         t.com[3 * (size_t)p + 2] = cz;
-        // the FP32 copies in the system's scaled units (exact powers of two)
-        const uint32_t sysn = sys_of(sv, st);
-        const double lam = t.scale32[2 * (size_t)sysn], mu = t.scale32[2 * (size_t)sysn + 1];
         const double lam2 = lam * lam, qs = mu * lam2;
         for (int k = 0; k < 6; ++k) {
             t.quad[6 * (size_t)p + k] = q[k];
@@ -632,7 +639,7 @@ __device__ Payload moments_of(uint32_t p, int intended, const TreeArgs& t, const
         for (int k = 0; k < 6; ++k) p32.q[k] = cnt > 1 ? (float)(q[k] * qs) : 0.0f;
         p32.pad = 0u;
         t.pay32[p] = p32;
-        if (leaf) {
+        if (leaf && !bodies_done) {
             // compact FP32 copy of the leaf's bodies: offsets from the leaf's
             // centre of mass (which the walk record carries), and the leaf id
             const uint32_t e = st + cnt;
