@@ -428,19 +428,32 @@ __global__ __launch_bounds__(64) void k_sum_compact(const uint32_t* __restrict__
 // the last read-ahead.
 __device__ __forceinline__ double add_from_lds(const double* buf, int len, double s) {
     int j = 0;
+    const double2* b2 = reinterpret_cast<const double2*>(buf);   // 16-byte reads
     if (len >= 32) {
         double t0[16], t1[16];
 #pragma unroll
-        for (int u = 0; u < 16; ++u) t0[u] = buf[u];
+        for (int u = 0; u < 8; ++u) {
+            const double2 w = b2[u];
+            t0[2 * u] = w.x;
+            t0[2 * u + 1] = w.y;
+        }
         for (; j + 32 <= len; j += 32) {
 #pragma unroll
-            for (int u = 0; u < 16; ++u) t1[u] = buf[j + 16 + u];
+            for (int u = 0; u < 8; ++u) {
+                const double2 w = b2[(j + 16) / 2 + u];
+                t1[2 * u] = w.x;
+                t1[2 * u + 1] = w.y;
+            }
             __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
             for (int u = 0; u < 16; ++u) s = s + t0[u];
             __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
-            for (int u = 0; u < 16; ++u) t0[u] = buf[j + 32 + u];
+            for (int u = 0; u < 8; ++u) {
+                const double2 w = b2[(j + 32) / 2 + u];
+                t0[2 * u] = w.x;
+                t0[2 * u + 1] = w.y;
+            }
             __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
             for (int u = 0; u < 16; ++u) s = s + t1[u];
@@ -512,7 +525,7 @@ __global__ __launch_bounds__(64) void k_sum_walk(const double* __restrict__ px, 
     const int a = (int)blockIdx.x;
     const int lane = threadIdx.x;
     const double* __restrict__ x = a == 0 ? px : (a == 1 ? py : pz);
-    __shared__ double s_buf[2][kSumTile + 32];
+    __shared__ __align__(16) double s_buf[2][kSumTile + 32];
     // the chain issues ahead of other kernels' waves on its SIMD
     __builtin_amdgcn_s_setprio(3);
     const uint32_t np = (uint32_t)__builtin_amdgcn_readfirstlane((int)off[(size_t)a * (ntiles + 1) + ntiles]);
