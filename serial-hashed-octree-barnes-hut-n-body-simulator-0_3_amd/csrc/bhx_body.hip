@@ -35,12 +35,15 @@ inline unsigned grid_for(size_t n, int per_block = kBlock, unsigned cap = 65535u
 // multiply by fl(1/N).  The sum is one dependent chain per axis; waves 1..3
 // stream chunks of positions into LDS (double buffered) while lanes 0..2 of
 // wave 0 run the x, y and z chains over the staged chunk.  One workgroup per
-// system; the chains of different systems run side by side.
+// system; the chains of different systems run side by side.  gate (one
+// system): runs only when gate[3] is set (see k_sum_offsets).
 template <int kMeanChunk>
 __global__ __launch_bounds__(kBlock) void k_bounds_mean(const double* __restrict__ px,
                                                         const double* __restrict__ py,
                                                         const double* __restrict__ pz, SysView sv,
-                                                        double* __restrict__ bounds4) {
+                                                        double* __restrict__ bounds4,
+                                                        const uint32_t* __restrict__ gate) {
+    if (gate && gate[3] == 0u) return;
     constexpr int kMeanRow = kMeanChunk + 16;
     __shared__ double buf[2][3][kMeanRow];
     const int tid = threadIdx.x;
@@ -174,7 +177,8 @@ __device__ __forceinline__ T block_excl_scan(T v, T* s_w, T& total) {
 // Tile sums of the three axes (estimate: any order).
 __global__ __launch_bounds__(kBlock) void k_sum_tiles(const double* __restrict__ px, const double* __restrict__ py,
                                                       const double* __restrict__ pz, size_t n,
-                                                      double* __restrict__ tsum) {
+                                                      double* __restrict__ tsum, uint32_t* __restrict__ gate) {
+    if (blockIdx.x == 0 && threadIdx.x < 4) gate[threadIdx.x] = 0u;
     const size_t base = (size_t)blockIdx.x * kSumTile;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     double a0 = 0.0, a1 = 0.0, a2 = 0.0;
@@ -225,7 +229,7 @@ __global__ __launch_bounds__(kBlock) void k_sum_scan(const double* __restrict__ 
 __global__ __launch_bounds__(kBlock) void k_sum_pieces(const double* __restrict__ px, const double* __restrict__ py,
                                                        const double* __restrict__ pz, size_t n,
                                                        const double* __restrict__ tpre, uint32_t* __restrict__ cnt,
-                                                       SumPiece* __restrict__ pieces) {
+                                                       SumPiece* __restrict__ pieces, uint32_t* __restrict__ gate) {
     const uint32_t t = blockIdx.x, a = blockIdx.y;
     const double* __restrict__ x = a == 0 ? px : (a == 1 ? py : pz);
     const size_t base = (size_t)t * kSumTile;
@@ -308,6 +312,7 @@ __global__ __launch_bounds__(kBlock) void k_sum_pieces(const double* __restrict_
             q.start = (uint32_t)base;
             pieces[(3 * (size_t)t + a) * kSumPieces] = q;
             cnt[3 * (size_t)t + a] = 1u;
+            atomicAdd(&gate[a], 1u);
         }
         return;
     }
@@ -393,9 +398,17 @@ __global__ __launch_bounds__(kBlock) void k_sum_pieces(const double* __restrict_
 }
 
 // Exclusive prefix of the piece counts per axis (one block); entry ntiles
-// holds the axis's total.
+// holds the axis's total.  When more than three quarters of some axis's tiles
+// are added element by element (unsorted input: partial sums near zero), the
+// plain chain (k_bounds_mean) is faster than the walk over pieces; gate[3]
+// selects it.  Either gives the same bits.
 __global__ __launch_bounds__(kBlock) void k_sum_offsets(const uint32_t* __restrict__ cnt, uint32_t ntiles,
-                                                        uint32_t* __restrict__ off) {
+                                                        uint32_t* __restrict__ off, uint32_t* __restrict__ gate) {
+    if (threadIdx.x == 0) {
+        uint32_t mx = gate[0] > gate[1] ? gate[0] : gate[1];
+        mx = mx > gate[2] ? mx : gate[2];
+        gate[3] = 4ull * mx > 3ull * ntiles ? 1u : 0u;
+    }
     __shared__ uint32_t s_w[kBlock / 64];
     for (int a = 0; a < 3; ++a) {
         uint32_t carry = 0;
@@ -521,7 +534,8 @@ __global__ __launch_bounds__(64) void k_sum_walk(const double* __restrict__ px, 
                                                   const double* __restrict__ pz, size_t n, uint32_t ntiles,
                                                   const uint32_t* __restrict__ off,
                                                   const SumPiece* __restrict__ stream, size_t axis_stride,
-                                                  double* __restrict__ bounds4) {
+                                                  double* __restrict__ bounds4, const uint32_t* __restrict__ gate) {
+    if (gate[3] != 0u) return;   // the plain chain runs instead
     const int a = (int)blockIdx.x;
     const int lane = threadIdx.x;
     const double* __restrict__ x = a == 0 ? px : (a == 1 ? py : pz);
@@ -834,7 +848,8 @@ hipError_t launch_bounds(bhx_ctx* c, double* bounds4, hipStream_t s) {
         if (nt + 1 > c->sum_tiles_cap) return hipErrorInvalidValue;
         double* tsum = c->sum_tile;
         double* tpre = c->sum_tile + 3 * c->sum_tiles_cap;
-        hipLaunchKernelGGL(k_sum_tiles, dim3(nt), dim3(kBlock), 0, s, c->px, c->py, c->pz, c->n, tsum);
+        uint32_t* gate = c->sum_cnt + 6 * c->sum_tiles_cap;   // 4 words
+        hipLaunchKernelGGL(k_sum_tiles, dim3(nt), dim3(kBlock), 0, s, c->px, c->py, c->pz, c->n, tsum, gate);
         hipLaunchKernelGGL(k_sum_scan, dim3(1), dim3(kBlock), 0, s, tsum, nt, tpre);
         uint32_t* cnt = c->sum_cnt;
         uint32_t* off = c->sum_cnt + 3 * c->sum_tiles_cap;
@@ -842,14 +857,16 @@ hipError_t launch_bounds(bhx_ctx* c, double* bounds4, hipStream_t s) {
         SumPiece* stream = c->sum_piece + 3 * (size_t)kSumPieces * c->sum_tiles_cap;
         const size_t axis_stride = (size_t)kSumPieces * c->sum_tiles_cap;
         hipLaunchKernelGGL(k_sum_pieces, dim3(nt, 3), dim3(kBlock), 0, s, c->px, c->py, c->pz, c->n, tpre, cnt,
-                           pieces);
-        hipLaunchKernelGGL(k_sum_offsets, dim3(1), dim3(kBlock), 0, s, cnt, nt, off);
+                           pieces, gate);
+        hipLaunchKernelGGL(k_sum_offsets, dim3(1), dim3(kBlock), 0, s, cnt, nt, off, gate);
         hipLaunchKernelGGL(k_sum_compact, dim3(nt, 3), dim3(64), 0, s, cnt, off, nt, pieces, stream, axis_stride);
         hipLaunchKernelGGL(k_sum_walk, dim3(3), dim3(64), 0, s, c->px, c->py, c->pz, c->n, nt, off, stream,
-                           axis_stride, bounds4);
+                           axis_stride, bounds4, gate);
+        hipLaunchKernelGGL(k_bounds_mean<kMeanChunkShared>, dim3(1), dim3(kBlock), 0, s, c->px, c->py, c->pz, sv,
+                           bounds4, gate);
     } else {
         hipLaunchKernelGGL(k_bounds_mean<kMeanChunkShared>, dim3(sv.nsys), dim3(kBlock), 0, s, c->px, c->py, c->pz,
-                           sv, bounds4);
+                           sv, bounds4, (const uint32_t*)nullptr);
     }
     hipLaunchKernelGGL(k_bounds_size, dim3(grid_for(c->max_sys_n, kBlock * 8, 2048), sv.nsys), dim3(kBlock), 0, s,
                        c->px, c->py, c->pz, sv, bounds4);
