@@ -288,6 +288,15 @@ int bhx_grad_timing(bhx_ctx *ctx, double *record_ms, double *adjoint_ms);
 int bhx_direct(bhx_ctx *ctx, const int64_t *targets, size_t nt, double eps,
                double *acc_out, double *pot_out);
 
+/* Which evaluation the last one-system OctantBounds centre took (bhx_bounds,
+ * bhx_system_bounds with one system, or a frame's bounds): out[0..2] = tiles
+ * of x, y, z whose partial sums had to be added element by element, out[3] =
+ * 1 when the plain left-to-right chain ran (more than three quarters of some
+ * axis's tiles were such tiles), 0 when the chain over exact runs ran.  Both
+ * give the same bits; this reports the path for tests and profiling.  (No
+ * reference counterpart.) */
+int bhx_bounds_path(bhx_ctx *ctx, uint32_t out[4]);
+
 /* FP64 kinetic energy, linear and angular momentum of the resident bodies. */
 int bhx_diagnostics(bhx_ctx *ctx, double *ekin, double mom[3], double angmom[3]);
 

@@ -27,6 +27,16 @@
 #define _POSIX_C_SOURCE 199309L
 #include "bhx_oracle.h"
 
+/* Planted mistakes, for tests only.  tests/test_oracle.py compiles this file
+ * a second time with one of these defined and checks that the known-answer
+ * tests for the reference's rounding order FAIL on the result, so each KAT
+ * provably distinguishes the reference's form from the plausible alternative:
+ *   ORC_MUTANT_MEAN_DIV   centre = sum / N        (reference: sum * fl(1/N))
+ *   ORC_MUTANT_HALF_DT    kick  += a * (dt * 0.5) (reference: (a * dt) * 0.5)
+ *   ORC_MUTANT_FMA        kick and drift contracted into fused multiply-adds
+ *   ORC_MUTANT_MAC_MUL    s^2 < theta^2 d^2       (reference: s^2 / d^2 < theta^2)
+ * None is defined in the oracle the tests and the bench load. */
+
 #include <math.h>
 #include <stdlib.h>
 #include <string.h>
@@ -49,10 +59,16 @@ void orc_bounds(const double *pos, size_t n, double center[3], double *size)
     }
     double dn = (double)n;
     if (dn != 0.0) {
+#ifdef ORC_MUTANT_MEAN_DIV
+        cx = cx / dn;
+        cy = cy / dn;
+        cz = cz / dn;
+#else
         double inv = 1.0 / dn;
         cx = cx * inv;
         cy = cy * inv;
         cz = cz * inv;
+#endif
     }
     double s = 0.0;
     for (size_t i = 0; i < n; ++i) {
@@ -157,19 +173,42 @@ void orc_gather(double *arr, size_t width, const uint64_t *idx, size_t n)
     free(tmp);
 }
 
+/* The half kick v + (a*dt)*0.5 of Body.hpp:310,322, two roundings then an
+ * addition (no fused multiply-add). */
+static inline double orc_half_kick(double v, double a, double dt)
+{
+#if defined(ORC_MUTANT_HALF_DT)
+    return v + a * (dt * 0.5);
+#elif defined(ORC_MUTANT_FMA)
+    return fma(a, dt * 0.5, v);
+#else
+    return v + (a * dt) * 0.5;
+#endif
+}
+
+/* The drift x + v*dt of Body.hpp:311, rounded product then addition. */
+static inline double orc_drift(double x, double v, double dt)
+{
+#ifdef ORC_MUTANT_FMA
+    return fma(v, dt, x);
+#else
+    return x + v * dt;
+#endif
+}
+
 /* ComputePositionAtHalfTimeStep — Body.hpp:306-313: v += (a*dt)*0.5; x += v*dt. */
 void orc_kick_drift(double dt, double *pos, double *vel, const double *acc, size_t n)
 {
     for (size_t i = 0; i < 3 * n; ++i) {
-        vel[i] = vel[i] + (acc[i] * dt) * 0.5;
-        pos[i] = pos[i] + vel[i] * dt;
+        vel[i] = orc_half_kick(vel[i], acc[i], dt);
+        pos[i] = orc_drift(pos[i], vel[i], dt);
     }
 }
 
 /* ComputeVelocityAndPosition — Body.hpp:317-324: v += (a*dt)*0.5. */
 void orc_kick(double dt, double *vel, const double *acc, size_t n)
 {
-    for (size_t i = 0; i < 3 * n; ++i) vel[i] = vel[i] + (acc[i] * dt) * 0.5;
+    for (size_t i = 0; i < 3 * n; ++i) vel[i] = orc_half_kick(vel[i], acc[i], dt);
 }
 
 /* ------------------------------------------------------------------------- */
@@ -193,6 +232,11 @@ static uint64_t orc_hash64(uint64_t k)
     k ^= k >> 33; k *= 0xc4ceb9fe1a85ec53ull;
     k ^= k >> 33;
     return k;
+}
+
+void orc_tree_path_keys(const double *pos, size_t n, const double c[3], double s, uint64_t *keys)
+{
+    for (size_t i = 0; i < n; ++i) keys[i] = orc_tree_path_key(pos + 3 * i, c, s);
 }
 
 static void orc_map_put(orc_tree *t, uint64_t key, int64_t val);
@@ -581,15 +625,24 @@ void orc_tree_moments_intended(orc_tree *t, const double *pos, const double *mas
 /* ------------------------------------------------------------------------- */
 
 /* LinearHashedOctree::MAC — HashedOctree.cpp:872-885: size^2/d^2 < theta^2,
- * d measured to the node's baryCenter. */
+ * d measured from the point r to the node's baryCenter. */
+int orc_mac_raw(double size, const double com[3], const double r[3], double theta)
+{
+    double dx = com[0] - r[0];
+    double dy = com[1] - r[1];
+    double dz = com[2] - r[2];
+    double d2 = dx * dx + dy * dy + dz * dz;
+    double s2 = size * size;
+#ifdef ORC_MUTANT_MAC_MUL
+    return s2 < (theta * theta) * d2;
+#else
+    return (s2 / d2) < (theta * theta);
+#endif
+}
+
 static inline int orc_mac(const orc_node *nd, const double p[3], double theta)
 {
-    double dx = nd->com[0] - p[0];
-    double dy = nd->com[1] - p[1];
-    double dz = nd->com[2] - p[2];
-    double d2 = dx * dx + dy * dy + dz * dz;
-    double s2 = nd->size * nd->size;
-    return (s2 / d2) < (theta * theta);
+    return orc_mac_raw(nd->size, nd->com, p, theta);
 }
 
 /* One term of ComputeHOTForceInteractionList — HashedOctree.hpp:404-455:

@@ -55,6 +55,7 @@ void     orc_kick(double dt, double *vel, const double *acc, size_t n);
 
 /* ---- octree ---- */
 uint64_t  orc_tree_path_key(const double p[3], const double c[3], double s);
+void      orc_tree_path_keys(const double *pos, size_t n, const double c[3], double s, uint64_t *keys);
 orc_tree *orc_tree_build_insert(const double *pos, const double *mass, size_t n,
                                 const double c[3], double s);
 orc_tree *orc_tree_build_recursive(const double *pos, const double *mass, size_t n,
@@ -67,6 +68,7 @@ int64_t   orc_tree_lookup(const orc_tree *t, uint64_t key);
 void      orc_tree_free(orc_tree *t);
 
 /* ---- walk and forces ---- */
+int  orc_mac_raw(double size, const double com[3], const double r[3], double theta);
 void orc_walk(const orc_tree *t, const double *pos, const double *mass, size_t n,
               double theta, double eps, int use_quad, double *acc, int64_t *counts);
 void orc_walk_subset(const orc_tree *t, const double *pos, const double *mass,

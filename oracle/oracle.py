@@ -60,6 +60,8 @@ def lib():
         L.orc_kick_drift.argtypes = [D, P, P, P, S]
         L.orc_kick.argtypes = [D, P, P, S]
         L.orc_tree_path_key.argtypes = [P, P, D]; L.orc_tree_path_key.restype = U64
+        L.orc_tree_path_keys.argtypes = [P, S, P, D, P]
+        L.orc_mac_raw.argtypes = [D, P, P, D]; L.orc_mac_raw.restype = I
         L.orc_tree_build_insert.argtypes = [P, P, S, P, D]; L.orc_tree_build_insert.restype = P
         L.orc_tree_build_recursive.argtypes = [P, P, S, P, D, I]; L.orc_tree_build_recursive.restype = P
         L.orc_tree_moments_intended.argtypes = [P, P, P]
@@ -125,10 +127,27 @@ def tree_path_keys(pos, c, s):
     pos = _c(pos, np.float64)
     c = _c(c, np.float64)
     out = np.empty(pos.shape[0], np.uint64)
-    L = lib()
-    for i in range(pos.shape[0]):
-        out[i] = L.orc_tree_path_key(_p(pos[i]), _p(c), s)
+    lib().orc_tree_path_keys(_p(pos), pos.shape[0], _p(c), s, _p(out))
     return out
+
+
+def mac(size, com, r, theta):
+    """LinearHashedOctree::MAC (HashedOctree.cpp:872-885) for one node and point."""
+    return bool(lib().orc_mac_raw(size, _p(_c(com, np.float64)), _p(_c(r, np.float64)), theta))
+
+
+def build_mutant(define, out_path):
+    """Compile a copy of the oracle with one planted mistake (ORC_MUTANT_*) and load it:
+    used by the tests to show that a known-answer test tells the two forms apart."""
+    subprocess.run(["make", "-s", "-C", HERE, "mutant", "MUTANT=" + define, "OUT=" + str(out_path)], check=True)
+    L = ctypes.CDLL(str(out_path))
+    P, S, D = ctypes.c_void_p, ctypes.c_size_t, ctypes.c_double
+    L.orc_bounds.argtypes = [P, S, P, P]
+    L.orc_kick_drift.argtypes = [D, P, P, P, S]
+    L.orc_kick.argtypes = [D, P, P, S]
+    L.orc_mac_raw.argtypes = [D, P, P, D]
+    L.orc_mac_raw.restype = ctypes.c_int
+    return L
 
 
 class Tree:

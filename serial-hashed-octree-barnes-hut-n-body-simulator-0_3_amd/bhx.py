@@ -88,6 +88,7 @@ def lib():
             "bhx_vtk_write_bodies": [ctypes.c_char_p, P, P, P, S],
             "bhx_vtk_write_nodes": [ctypes.c_char_p, P, S, I, I],
             "bhx_export_vtk": [P, ctypes.c_char_p, ctypes.c_char_p, I, I],
+            "bhx_bounds_path": [P, P],
         }
         for name, args in sigs.items():
             f = getattr(L, name)
@@ -107,6 +108,15 @@ def _f64(a, shape=None):
     a = np.ascontiguousarray(a, dtype=np.float64)
     if shape is not None:
         a = a.reshape(shape)
+    return a
+
+
+def _f64_exact(a, shape, what):
+    """float64 C-contiguous copy of `a`, which must have exactly `shape` (the C side
+    reads that many values through the pointer)."""
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    if a.shape != tuple(shape):
+        raise ValueError("%s: expected shape %s, got %s" % (what, tuple(shape), a.shape))
     return a
 
 
@@ -180,9 +190,16 @@ def _path(path):
 def write_snapshot(path, pos, vel, mass, acc=None, offsets=None, time=0.0, step=0, config=None, comp=None):
     """Write host arrays as a snapshot file (see bhx_snapshot_write); comp = (6n,)
     or (2, n, 3): position then velocity compensation terms."""
-    pos, vel, mass = _f64(pos), _f64(vel), _f64(mass)
-    acc = None if acc is None else _f64(acc)
+    mass = _f64(mass)
+    n = mass.size
+    if mass.ndim != 1:
+        raise ValueError("mass: expected shape (n,), got %s" % (mass.shape,))
+    pos = _f64_exact(pos, (n, 3), "pos")
+    vel = _f64_exact(vel, (n, 3), "vel")
+    acc = None if acc is None else _f64_exact(acc, (n, 3), "acc")
     comp = None if comp is None else _f64(comp).reshape(-1)
+    if comp is not None and comp.size != 6 * n:
+        raise ValueError("comp: expected 6n = %d values, got %d" % (6 * n, comp.size))
     off = None if offsets is None else np.ascontiguousarray(offsets, dtype=np.uint64)
     info = SnapshotInfo()
     info.n = mass.size
@@ -251,6 +268,9 @@ class Engine:
             raise BhxError(rc, "bhx_create")
         self.h = h
         self.capacity = capacity
+        # bumped by every record() (and set_bodies()): a backward pass checks that the
+        # recording it differentiates is still the one its forward pass made
+        self.record_generation = 0
 
     def close(self):
         if getattr(self, "h", None):
@@ -290,13 +310,17 @@ class Engine:
         return v.value
 
     def set_bodies(self, pos, vel, mass):
-        pos = _f64(pos, (-1, 3))
-        mass = _f64(mass)
-        vel = None if vel is None else _f64(vel, (-1, 3))
-        self._chk(lib().bhx_set_bodies(self.h, _p(pos), _p(vel), _p(mass), pos.shape[0]), "bhx_set_bodies")
+        pos = np.ascontiguousarray(pos, dtype=np.float64)
+        if pos.ndim != 2 or pos.shape[1] != 3:
+            raise ValueError("pos: expected shape (n, 3), got %s" % (pos.shape,))
+        n = pos.shape[0]
+        mass = _f64_exact(mass, (n,), "mass")
+        vel = None if vel is None else _f64_exact(vel, (n, 3), "vel")
+        self._chk(lib().bhx_set_bodies(self.h, _p(pos), _p(vel), _p(mass), n), "bhx_set_bodies")
+        self.record_generation += 1    # the recording (if any) is of other bodies now
 
     def set_accelerations(self, acc):
-        acc = _f64(acc, (-1, 3))
+        acc = _f64_exact(acc, (self.n, 3), "acc")
         self._chk(lib().bhx_set_accelerations(self.h, _p(acc)), "bhx_set_accelerations")
 
     def get_bodies(self):
@@ -437,6 +461,7 @@ class Engine:
     # --- differentiation
     def record(self, nsteps, checkpoint_every=100, init_acc=True):
         """Run and record nsteps frames for adjoint(); see bhx_record."""
+        self.record_generation += 1
         self._chk(lib().bhx_record(self.h, nsteps, checkpoint_every, int(init_acc)), "bhx_record")
 
     def recorded_bodies(self):
@@ -516,6 +541,13 @@ class Engine:
         self._chk(lib().bhx_energy(self.h, ctypes.byref(k), ctypes.byref(w), ctypes.byref(v), _p(phi)),
                   "bhx_energy")
         return (k.value, w.value, v.value, phi) if potentials else (k.value, w.value, v.value)
+
+    def bounds_path(self):
+        """(element-by-element tiles of x, y, z; 1 if the plain chain ran) of the last
+        one-system bounds."""
+        out = np.zeros(4, np.uint32)
+        self._chk(lib().bhx_bounds_path(self.h, _p(out)), "bhx_bounds_path")
+        return [int(v) for v in out]
 
     def diagnostics(self):
         ek = np.zeros(1)

@@ -7,7 +7,9 @@ walk's decisions frozen at the recorded frames.  PyTorch only carries the
 tensors; every frame and every adjoint runs in libbhx.
 
 An engine holds one recording at a time: call backward before the next
-simulate on the same engine.
+simulate on the same engine.  A backward pass whose recording has since been
+replaced (another simulate, record or set_bodies on that engine) raises
+instead of returning the gradients of the other trajectory.
 """
 import os
 import sys
@@ -29,11 +31,16 @@ class _Trajectory(torch.autograd.Function):
         engine.record(nsteps, checkpoint_every, init_acc)
         P, V, _, _ = engine.recorded_bodies()
         ctx.engine = engine
+        ctx.generation = engine.record_generation
         ctx.devices = (x0.device, v0.device, m.device)
         return torch.from_numpy(P).to(dev), torch.from_numpy(V).to(dev)
 
     @staticmethod
     def backward(ctx, gx, gv):
+        if ctx.engine.record_generation != ctx.generation:
+            raise RuntimeError("the engine's recording was replaced after this simulate(): "
+                               "call backward before simulating again on the same engine")
+
         def host(g):
             return None if g is None else g.detach().to("cpu", torch.float64).contiguous().numpy()
         gx0, gv0, _, gm = ctx.engine.adjoint(host(gx), host(gv))

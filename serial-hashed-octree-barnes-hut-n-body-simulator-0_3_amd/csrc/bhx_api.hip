@@ -770,6 +770,17 @@ int bhx_bounds(bhx_ctx* c, double center[3], double* size) {
     return BHX_OK;
 }
 
+int bhx_bounds_path(bhx_ctx* c, uint32_t out[4]) {
+    if (!c || !out) return c ? fail(c, BHX_E_NULL, "out is a null pointer") : BHX_E_NULL;
+    HIPCHK(hipSetDevice(c->device));
+    HIPCHK(hipStreamSynchronize(c->s_side));
+    HIPCHK(hipStreamSynchronize(c->s_main));
+    // the gate words follow the per-tile piece counts and their prefixes
+    const uint32_t* gate = c->sum_cnt + 6 * c->sum_tiles_cap;
+    HIPCHK(hipMemcpy(out, gate, 4 * sizeof(uint32_t), hipMemcpyDeviceToHost));
+    return BHX_OK;
+}
+
 int bhx_morton_keys(bhx_ctx* c, double size, uint64_t* keys_out) {
     if (!c) return BHX_E_NULL;
     HIPCHK(hipSetDevice(c->device));

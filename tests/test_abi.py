@@ -65,3 +65,25 @@ def test_torch_wrapper_imports():
     from tests.helpers import load_bhx_torch
     bt = load_bhx_torch()
     assert callable(bt.simulate) and callable(bt.make_engine)
+
+
+def test_binding_rejects_mismatched_shapes(tmp_path):
+    """The binding checks array shapes before the C side reads n (or 3n) values
+    through each pointer: no GPU call is made for a rejected call."""
+    bhx = load_bhx()
+    e = bhx.Engine.__new__(bhx.Engine)     # no context: the checks come first
+    e.h = None
+    e.record_generation = 0
+    pos = np.zeros((4, 3))
+    with pytest.raises(ValueError):
+        e.set_bodies(pos, None, np.ones(3))
+    with pytest.raises(ValueError):
+        e.set_bodies(pos, np.zeros((3, 3)), np.ones(4))
+    with pytest.raises(ValueError):
+        e.set_bodies(np.zeros(12), None, np.ones(4))
+    with pytest.raises(ValueError):
+        bhx.write_snapshot(str(tmp_path / "x.snap"), pos, np.zeros((3, 3)), np.ones(4))
+    with pytest.raises(ValueError):
+        bhx.write_snapshot(str(tmp_path / "x.snap"), pos, np.zeros((4, 3)), np.ones(4), acc=np.zeros((5, 3)))
+    bhx.write_snapshot(str(tmp_path / "ok.snap"), pos, np.zeros((4, 3)), np.ones(4))
+    assert bhx.read_snapshot(str(tmp_path / "ok.snap"))["mass"].size == 4

@@ -76,3 +76,55 @@ def test_ensemble_mixed_sizes(bhx):
     for k, (p, _, _) in enumerate(parts):
         oc, os_ = orc.bounds(p)
         assert np.array_equal(C[k], oc) and S[k] == os_, k
+
+
+def _monotone_then_hover(n, seed):
+    """About 80 % positive values (partial sums grow steadily: runs of whole binades)
+    followed by about 20 % values that keep every partial sum within a few thousand
+    ulps of the binade edge 2^17 (each such element needs its own addition, so those
+    tiles are added element by element), for each axis."""
+    rng = np.random.default_rng(seed)
+    m = (4 * n) // 5
+    x = np.empty((n, 3))
+    for a in range(3):
+        x[:m, a] = rng.uniform(0.5, 1.0, m)
+        s = 0.0
+        for k in range(m):
+            s = s + x[k, a]
+        edge = 131072.0
+        for k in range(m, n):
+            if (k - m) % 2 == 0:
+                v = edge - s + 1e-7 * rng.uniform(-1, 1)
+            else:
+                v = 1e-5 * rng.uniform(-1, 1)
+            x[k, a] = v
+            s = s + v
+    return x
+
+
+def test_chain_over_runs_with_element_by_element_stretch(bhx):
+    """The chain over exact runs (not the plain chain) handles a stretch of about 20
+    consecutive tiles added element by element, bit for bit: the walker's merged
+    ranges over several tiles and its multi-chunk staging through LDS.  The path
+    taken is read back from the device."""
+    n = 100003
+    pos = _monotone_then_hover(n, 11)
+    print("bounds monotone+hover n=%d" % n, flush=True)
+    with bhx.Engine(n) as e:
+        e.set_bodies(pos, np.zeros_like(pos), np.full(n, 1.0 / n))
+        c, s = e.bounds()
+        path = e.bounds_path()
+        # a zero-mean sphere in generator order: the partial sums wander near zero,
+        # (almost) every tile is added element by element and the plain chain runs
+        rp = bhx.plummer(n, seed=2)[0]
+        e.set_bodies(rp, np.zeros_like(rp), np.full(n, 1.0 / n))
+        c2, s2 = e.bounds()
+        path2 = e.bounds_path()
+    tiles = (n + 1023) // 1024
+    print("path %s (of %d tiles); shuffled %s" % (path, tiles, path2), flush=True)
+    oc, os_ = orc.bounds(pos)
+    assert c.tobytes() == oc.tobytes() and s == os_
+    assert path[3] == 0 and all(10 <= path[a] <= tiles // 2 for a in range(3))
+    oc2, os2 = orc.bounds(rp)
+    assert c2.tobytes() == oc2.tobytes() and s2 == os2
+    assert path2[3] == 1 and min(path2[:3]) > (3 * tiles) // 4

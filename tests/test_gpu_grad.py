@@ -196,3 +196,24 @@ def test_torch_autograd_through_engine(bhx):
         xT1, vT1 = bt.simulate(x1, v0.detach(), m.detach(), T, e, checkpoint_every=3)
         loss1 = ((xT1 - tgt) ** 2).sum(dim=1).mean() + 0.5 * (vT1 ** 2).sum()
         assert float(loss1.detach()) < float(loss.detach())
+
+
+def test_torch_backward_after_recording_replaced_raises(bhx):
+    """A second simulate on the same engine replaces the recording; the first
+    graph's backward must raise rather than return the other trajectory's
+    gradients."""
+    import torch
+    from tests.helpers import load_bhx_torch
+    bt = load_bhx_torch()
+    n, T = 256, 3
+    pos, vel, mass = _bodies(bhx, n, 44)
+    x0 = torch.tensor(pos, requires_grad=True)
+    v0 = torch.tensor(vel, requires_grad=True)
+    m = torch.tensor(mass, requires_grad=True)
+    with bt.make_engine(n, eps=0.05, dt=2e-3, leaf_capacity=8, precision=0) as e:
+        xa, _ = bt.simulate(x0, v0, m, T, e, checkpoint_every=3)
+        xb, _ = bt.simulate(x0 * 1.001, v0, m, T, e, checkpoint_every=3)
+        with pytest.raises(RuntimeError):
+            xa.sum().backward()
+        xb.sum().backward()                   # the current recording still works
+        assert torch.isfinite(x0.grad).all()

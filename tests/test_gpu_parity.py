@@ -13,6 +13,7 @@ import pytest
 
 from oracle import oracle as orc
 from tests.helpers import load_bhx
+from tests.kats import DRIFT_KAT, KICK_KAT, MEAN_KAT_C, MEAN_KAT_POS, TINY
 
 pytestmark = pytest.mark.gpu
 
@@ -331,6 +332,28 @@ def test_kick_drift_kick_bitexact(bhx):
     p1, v1 = orc.kick_drift(1e-3, pos, vel, acc)
     v2 = orc.kick(1e-3, v1, acc)
     assert np.array_equal(gp, p1) and np.array_equal(gv, v2)
+
+
+def test_rounding_order_kats(bhx):
+    """The hand-derived known answers that fix the reference's rounding order
+    (centre = sum * fl(1/N), kick (a dt) 0.5, drift x + fl(v dt), no fused
+    multiply-adds, subnormal products kept), on the engine."""
+    with _engine(bhx, 8) as e:
+        e.set_bodies(MEAN_KAT_POS, None, np.ones(3))
+        c, s = e.bounds()
+        assert list(c) == MEAN_KAT_C and s == 4.0 - MEAN_KAT_C[2]
+        e.set_bodies(np.zeros((1, 3)), KICK_KAT["v"], np.ones(1))
+        e.set_accelerations(KICK_KAT["a"])
+        e.kick(1e-3)
+        assert e.get_bodies()[1][0, 0] == KICK_KAT["want"]
+        e.set_bodies(DRIFT_KAT["x"], DRIFT_KAT["v"], np.ones(1))
+        e.kick_drift(1e-3)
+        assert e.get_bodies()[0][0, 0] == DRIFT_KAT["want"]
+        e.set_bodies(np.zeros((1, 3)), None, np.ones(1))
+        e.set_accelerations([[TINY, 0.0, 0.0]])
+        e.kick_drift(2.75)
+        p, v, _, _ = e.get_bodies()
+        assert v[0, 0] == 2 * TINY and p[0, 0] == 6 * TINY
 
 
 def test_empty_and_errors(bhx):

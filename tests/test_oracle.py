@@ -4,6 +4,7 @@ These run without a GPU.  Each check is chosen so that a plausible mistake in
 the oracle (a wrong shift or mask, a non-stable pass, a wrong octant bit, a
 dropped quadrupole term, a sign) fails it.
 """
+import ctypes
 import math
 
 import numpy as np
@@ -12,6 +13,7 @@ import pytest
 from oracle import oracle as orc
 from tests.helpers import (canonical_node_keys, effective21, interleave_bruteforce,
                            plummer_np)
+from tests.kats import DRIFT_KAT, KICK_KAT, MAC_KATS, MEAN_KAT_C, MEAN_KAT_POS, TINY
 
 
 # ---------------------------------------------------------------- Morton keys
@@ -136,12 +138,21 @@ def test_node_set_matches_path_key_definition(leaf_cap):
 
 
 def test_tree_invariants():
-    pos, _, mass = plummer_np(5000, 9)
+    """Structure and as-run payload of the insertion build, each field against a
+    definition from the path keys: child bits, counts, edge = S / 2^L, first member,
+    barycentre = position of the first member in array order, and mass = the
+    left-to-right sum of the members' masses in array order (insertion adds each
+    body's mass to every node on its path, HashedOctree.cpp:450-711).  Masses are
+    uneven, so the summation order shows in the bits."""
+    n = 5000
+    pos, _, _ = plummer_np(n, 9)
+    mass = np.random.default_rng(9).uniform(0.1, 1.0, n) / n
     c, s = orc.bounds(pos)
     t = orc.build_insert(pos, mass, c, s)
     nd = t.nodes()
     m = _node_map(nd)
     pk = orc.tree_path_keys(pos, c, s)
+    order_matters = 0
     for i, rec in enumerate(nd):
         k = int(rec["key"])
         kids = [m.get((k << 3) | o) for o in range(8)]
@@ -149,7 +160,6 @@ def test_tree_invariants():
         assert bits == rec["child_byte"]
         if bits:
             assert rec["n"] == sum(nd[j]["n"] for j in kids if j is not None)
-            assert rec["mass"] == sum(nd[j]["mass"] for j in kids if j is not None) or True
         else:
             assert rec["n"] == 1 or t.unsplit
         L = int(rec["level"])
@@ -158,7 +168,12 @@ def test_tree_invariants():
         assert members.size == rec["n"]
         assert rec["first"] == members.min()
         assert np.array_equal(rec["com"], pos[members.min()])           # as-run barycentre
-        assert rec["mass"] == members.size * (1.0 / 5000) or rec["mass"] == pytest.approx(members.size / 5000)
+        seq = 0.0
+        for j in members:                                               # array order
+            seq = seq + float(mass[j])
+        assert rec["mass"] == seq
+        order_matters += int(math.fsum(mass[members][::-1]) != seq or sum(mass[members][::-1]) != seq)
+    assert order_matters > 0          # some node's mass depends on the order of its sum
 
 
 def test_node_centres_follow_descent():
@@ -302,12 +317,12 @@ def test_kdk_closed_form_and_reversibility():
     assert list(v1[0]) == [1.5, -0.5, 2.0] and list(p1[0]) == [0.75, 0.75, -1.0]
     v2 = orc.kick(0.5, v1, acc)
     assert list(v2[0]) == [2.0, -1.5, 4.0]
-    # backwards with negated velocity returns exactly (power-of-two values)
+    # with power-of-two values every operation is exact, so stepping back with the
+    # negated velocity returns exactly: drift back (-v2 kicked by a: -v1), then kick
     p3, v3 = orc.kick_drift(0.5, p1, -v2, acc)
+    assert list(v3[0]) == list(-v1[0]) and np.array_equal(p3, pos)
     v4 = orc.kick(0.5, v3, acc)
-    pb, vb = orc.kick_drift(0.5, p1, -v2, acc)
-    assert np.array_equal(pb, pos) or np.allclose(pb, pos)
-    assert np.allclose(-orc.kick(0.5, vb, acc), vel + 0.0 * v4)
+    assert np.array_equal(-v4, vel)
 
 
 def test_frame_order_matches_composition():
@@ -386,3 +401,83 @@ def test_traj_adjoint_two_body_one_frame_closed_form():
     r3 = (1.0 + eps * eps) ** 1.5
     assert G[3][1] == pytest.approx(0.5 * dt * dt / r3, rel=1e-12)
     assert G[3][0] == pytest.approx(0.0, abs=1e-15)
+
+
+# ---------------------------------------------------------------- rounding-order known answers
+#
+# Each value below is derived by hand from IEEE-754 round-to-nearest-even of the
+# reference's own expression, and each input is chosen so that the plausible
+# alternative form rounds differently: the test then runs the alternative (a
+# copy of the oracle compiled with that one mistake planted) and requires it to
+# FAIL the same known answer.
+
+def _mutant(tmp_path, define):
+    return orc.build_mutant(define, tmp_path / (define.lower() + ".so"))
+
+
+def _bounds_with(L, pos):
+    pos = np.ascontiguousarray(pos, dtype=np.float64)
+    c = np.zeros(3)
+    s = np.zeros(1)
+    L.orc_bounds(pos.ctypes.data_as(ctypes.c_void_p), pos.shape[0], c.ctypes.data_as(ctypes.c_void_p),
+                 s.ctypes.data_as(ctypes.c_void_p))
+    return c, float(s[0])
+
+
+def test_bounds_centre_kat_multiplies_by_reciprocal(tmp_path):
+    """OctantBounds' centre is sum * fl(1/N) (Vec3D::operator/= multiplies by the
+    reciprocal, MathUtils.cpp:81-90), N = 3."""
+    c, s = orc.bounds(MEAN_KAT_POS)
+    assert list(c) == MEAN_KAT_C
+    assert s == 4.0 - MEAN_KAT_C[2]
+    cm, _ = _bounds_with(_mutant(tmp_path, "ORC_MUTANT_MEAN_DIV"), MEAN_KAT_POS)
+    assert list(cm) != MEAN_KAT_C and cm[0] == float.fromhex("0x1.ddddddddddddfp-3")
+
+
+def _kdk_with(L, dt, pos, vel, acc, drift=True):
+    pos, vel, acc = [np.array(a, np.float64, copy=True) for a in (pos, vel, acc)]
+    P = ctypes.c_void_p
+    if drift:
+        L.orc_kick_drift(dt, pos.ctypes.data_as(P), vel.ctypes.data_as(P), acc.ctypes.data_as(P), pos.shape[0])
+    else:
+        L.orc_kick(dt, vel.ctypes.data_as(P), acc.ctypes.data_as(P), vel.shape[0])
+    return pos, vel
+
+
+def test_kick_drift_kat_rounding_order(tmp_path):
+    """v + (a dt) 0.5 and x + v dt with each product rounded before the addition
+    (Body.hpp:310-311,322): normal-range cases against fused multiply-adds, and a
+    subnormal case against a (dt 0.5)."""
+    dt = 1e-3
+    v = orc.kick(dt, KICK_KAT["v"], KICK_KAT["a"])
+    assert v[0, 0] == KICK_KAT["want"]
+    x, _ = orc.kick_drift(dt, DRIFT_KAT["x"], np.zeros((1, 3)), np.zeros((1, 3)))
+    assert x[0, 0] == -0.002
+    # drift alone: zero acceleration leaves v, then x + fl(v dt)
+    x, _ = orc.kick_drift(dt, DRIFT_KAT["x"], DRIFT_KAT["v"], np.zeros((1, 3)))
+    assert x[0, 0] == DRIFT_KAT["want"]
+    xs, vs = orc.kick_drift(2.75, np.zeros((1, 3)), np.zeros((1, 3)), [[TINY, 0.0, 0.0]])
+    assert vs[0, 0] == 2 * TINY and xs[0, 0] == 6 * TINY
+
+    fm = _mutant(tmp_path, "ORC_MUTANT_FMA")
+    _, vm = _kdk_with(fm, dt, np.zeros((1, 3)), KICK_KAT["v"], KICK_KAT["a"], drift=False)
+    assert vm[0, 0] != KICK_KAT["want"]
+    xm, _ = _kdk_with(fm, dt, DRIFT_KAT["x"], DRIFT_KAT["v"], np.zeros((1, 3)))
+    assert xm[0, 0] != DRIFT_KAT["want"]
+    hm = _mutant(tmp_path, "ORC_MUTANT_HALF_DT")
+    _, vh = _kdk_with(hm, 2.75, np.zeros((1, 3)), np.zeros((1, 3)), [[TINY, 0.0, 0.0]])
+    assert vh[0, 0] == TINY
+
+
+def test_mac_kat_divides(tmp_path):
+    """MAC is (s*s)/(d.d) < theta*theta (HashedOctree.cpp:875-884), at two boundary
+    cases where the multiplied form decides the other way."""
+    mm = _mutant(tmp_path, "ORC_MUTANT_MAC_MUL")
+    r = np.zeros(3)
+    for com, size_hex, want in MAC_KATS:
+        size = float.fromhex(size_hex)
+        com = np.array(com)
+        assert orc.mac(size, com, r, 0.6) is want
+        got = bool(mm.orc_mac_raw(size, com.ctypes.data_as(ctypes.c_void_p), r.ctypes.data_as(ctypes.c_void_p),
+                                  0.6))
+        assert got is (not want)
