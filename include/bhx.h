@@ -146,6 +146,20 @@ int bhx_sort_keys(bhx_ctx *ctx, uint64_t *keys, uint64_t *idx, size_t n);
  * bhx_morton_keys call.  perm_out (may be NULL) receives new[i] = old[perm[i]]. */
 int bhx_sort_bodies(bhx_ctx *ctx, uint64_t *perm_out);
 
+/* bhx_sort_keys on DEVICE arrays with no allocation.  No counterpart in the
+ * reference: for callers that cannot allocate device memory (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc).  keys
+ * and idx are device pointers to n values on `device` (a HIP device ordinal),
+ * sorted in place on `stream` (a hipStream_t; NULL = the null stream) into the
+ * same stable ascending order as bhx_sort_keys, idx carried along.  All
+ * working memory is a static device array of about 26 MiB per device, held
+ * until the process ends; it fixes the maximum n at BHX_NOALLOC_MAX_KEYS
+ * (BHX_E_CAPACITY beyond).  Calls on one device share that array: they are
+ * serialised and each returns when its sort is done (the stream is
+ * synchronised).  BHX_E_SORT: the look-back did not complete. */
+#define BHX_NOALLOC_MAX_KEYS ((size_t)1 << 20)
+int bhx_sort_keys_noalloc(int device, void *stream, uint64_t *keys, uint64_t *idx, size_t n);
+
 /* ReorderBodiesByIndexes(Body*, size_t*, n) — Body.hpp:257-273 with a caller
  * permutation. */
 int bhx_reorder_bodies(bhx_ctx *ctx, const uint64_t *perm);

@@ -89,6 +89,7 @@ def lib():
             "bhx_vtk_write_nodes": [ctypes.c_char_p, P, S, I, I],
             "bhx_export_vtk": [P, ctypes.c_char_p, ctypes.c_char_p, I, I],
             "bhx_bounds_path": [P, P],
+            "bhx_sort_keys_noalloc": [I, P, P, P, S],
         }
         for name, args in sigs.items():
             f = getattr(L, name)

@@ -27,6 +27,7 @@ UNITS = {
     "bhx_walk.hip": [],
     "bhx_api.hip": [],
     "bhx_grad.hip": [],
+    "bhx_noalloc.hip": [],
 }
 HOST_UNITS = {"bhx_ic.cpp": ["-O2", "-fPIC", "-std=c++17", "-ffp-contract=off", "-pthread"],
               "bhx_io.cpp": ["-O2", "-fPIC", "-std=c++17", "-Wall"]}
