@@ -203,6 +203,71 @@ int bhx_walk(bhx_ctx *ctx, double *acc_out, uint64_t *counts);
  * selects.  The resident accelerations are left unchanged. */
 int bhx_interaction_counts(bhx_ctx *ctx, uint64_t *counts);
 
+/* ---- the reference's per-body tree API -------------------------------------
+ * ComputeHOTOctreeForce (HashedOctree.hpp:262-272) is bhx_walk, batched.  Code
+ * that calls the per-body pair itself, the MAC, or edits the node table uses
+ * these, on the node table of the last build (one system).  Node indexes are
+ * the pre-order indexes of bhx_get_nodes; a record whose key reads 0 there was
+ * erased.  Nodes put or erased since the build leave bhx_walk refusing
+ * (BHX_E_STATE) until the next build; these calls see the edits. */
+
+/* TraverseHOTInteractionList(tree, bodyPosition, theta, walkList, interactList)
+ * → long (HashedOctree.hpp:336-385) with LinearHashedOctree::MAC
+ * (HashedOctree.cpp:872-885), for nq points (xyz interleaved): the explicit-
+ * stack depth-first search from the root in the reference's order (children
+ * scanned in octant order through the hash table, accepted children
+ * appended, opened ones pushed, the last pushed popped first; a popped leaf
+ * appended unless its barycentre equals the point).  Point q's list is
+ * entries[offsets[q] .. offsets[q+1]) (offsets: nq + 1 values, offsets[0] =
+ * 0; offsets[q+1] - offsets[q] is the reference's return value).  An entry
+ * >= 0 is a node index; an entry < 0 is -(1 + i) for body i (array order) of
+ * a leaf that holds several bodies (leaf capacity > 1, or a cell at the
+ * 21-level cap), each such body except one at the point itself, in tree
+ * order.  entries NULL: only offsets and *total; else cap must be >= total
+ * (BHX_E_CAPACITY).  The walkList / interactList scratch of the reference is
+ * device memory here. */
+int bhx_interaction_lists(bhx_ctx *ctx, const double *points, size_t nq, double theta, uint64_t *offsets,
+                          int64_t *entries, size_t cap, size_t *total);
+
+/* ComputeHOTForceInteractionList(bodyPosition, bodyMass, acceleration,
+ * interactList, listLength) — HashedOctree.hpp:393-459: acc (3 per point, in
+ * and out) += the softened terms of each point's entries (as returned by
+ * bhx_interaction_lists) in list order, with the reference's FP64 operation
+ * sequence and the configured eps (the reference's SOFTENING); the
+ * quadrupole correction for nodes of more than one body when the
+ * configuration enables it.  Offsets need not start at 0 (nq + 1 values). */
+int bhx_force_lists(bhx_ctx *ctx, const double *points, size_t nq, const uint64_t *offsets,
+                    const int64_t *entries, double *acc);
+
+/* LinearHashedOctree::MAC(node, r, theta) — HashedOctree.cpp:872-885:
+ * accept[k] = (size^2 / |baryCenter - r|^2 < theta^2) for node node_ids[k] and
+ * point k (size: the cell edge of the node's level). */
+int bhx_mac(bhx_ctx *ctx, const int64_t *node_ids, const double *points, size_t n, double theta,
+            uint8_t *accept);
+
+/* LinearHashedOctree::insertHOTNode(node) — HashedOctree.cpp:152-205: the
+ * record (key, center, com, mass, quad, n, first, level, child_byte,
+ * body_start; level must match the key) replaces the node of that key or is
+ * appended as a new node and entered into the hash table; *id_out receives its
+ * index.  A node's size follows from its level (the reference's recurrence);
+ * the record's size field is not stored. */
+int bhx_node_put(bhx_ctx *ctx, const bhx_node *node, int64_t *id_out);
+
+/* LinearHashedOctree::deleteNode(key) — HashedOctree.cpp:647-659: the node
+ * leaves the hash table (lookups miss it; parents keep their child bits, and
+ * traversals skip the missing child).  *erased = 1 if it was there. */
+int bhx_node_erase(bhx_ctx *ctx, uint64_t key, int *erased);
+
+/* pruneEmptyNodes(lookUpNode(key)) — HashedOctree.cpp:610-639, and
+ * PruneEmptyNodesFromTree (HashedOctree.hpp:248-252) with key = ROOT_KEY (1):
+ * every node at or below `key` with N = 0 and no children is erased. */
+int bhx_prune_empty(bhx_ctx *ctx, uint64_t key, size_t *removed);
+
+/* decodeMortonKey(key, size) — MortonKeys.hpp:204-228 for n keys: the three
+ * 21-bit coordinates scaled back as c / 2^21 * 2 * size - size (xyz
+ * interleaved). */
+int bhx_decode_morton_keys(bhx_ctx *ctx, const uint64_t *keys, size_t n, double size, double *pos);
+
 /* One frame in the reference order (ofApp::update then ofApp::draw,
  * ofApp.cpp:169-177,248-282), repeated nsteps times on the resident bodies.
  * Returns BHX_E_NONFINITE when some frame met a non-finite position (at its

@@ -128,6 +128,19 @@ uint64_t orc_morton_key(const double p[3], double size)
     return (1ull << 63) | (X << 2) | (Y << 1) | Z;
 }
 
+/* decodeMortonKey — MortonKeys.hpp:204-228: de-interleave the three 21-bit
+ * coordinates, then c / 2^21 * 2 * size - size, left to right. */
+void orc_decode_morton_key(uint64_t key, double size, double out[3])
+{
+    uint64_t c[3] = {0, 0, 0};
+    for (int b = 0; b < 21; ++b) {
+        c[0] |= ((key >> (3 * b + 2)) & 1ull) << b;
+        c[1] |= ((key >> (3 * b + 1)) & 1ull) << b;
+        c[2] |= ((key >> (3 * b)) & 1ull) << b;
+    }
+    for (int a = 0; a < 3; ++a) out[a] = (double)c[a] / 2097152.0 * 2.0 * size - size;
+}
+
 /* ofApp::update() key loop — ofApp.cpp:172-175. */
 void orc_morton_keys(const double *pos, size_t n, double size, uint64_t *keys)
 {
@@ -713,6 +726,24 @@ static int64_t orc_list(const orc_tree *t, const double *pos, const double p[3],
             }
         }
     }
+    return nl;
+}
+
+/* One point's list in the reference's order: node index and kind (0: node,
+ * 1: member body of a multi-body leaf, index = body). */
+int64_t orc_tree_list(const orc_tree *t, const double *pos, const double p[3], double theta,
+                      int64_t *ids, int32_t *is_body)
+{
+    size_t cap = t->count + t->nmembers + 8;
+    int64_t *stack = (int64_t *)malloc(cap * sizeof(int64_t));
+    orc_entry *list = (orc_entry *)malloc(cap * sizeof(orc_entry));
+    int64_t nl = orc_list(t, pos, p, theta, stack, list);
+    for (int64_t e = 0; e < nl; ++e) {
+        if (ids) ids[e] = list[e].id;
+        if (is_body) is_body[e] = list[e].body;
+    }
+    free(stack);
+    free(list);
     return nl;
 }
 

@@ -48,6 +48,7 @@ uint64_t orc_x86_cvt_u64(double u);
 uint64_t orc_spread21(uint64_t t);
 uint64_t orc_morton_key(const double p[3], double size);
 void     orc_morton_keys(const double *pos, size_t n, double size, uint64_t *keys);
+void     orc_decode_morton_key(uint64_t key, double size, double out[3]);
 void     orc_radix_sort(uint64_t *keys, uint64_t *idx, size_t n);
 void     orc_gather(double *arr, size_t width, const uint64_t *idx, size_t n);
 void     orc_kick_drift(double dt, double *pos, double *vel, const double *acc, size_t n);
@@ -69,6 +70,8 @@ void      orc_tree_free(orc_tree *t);
 
 /* ---- walk and forces ---- */
 int  orc_mac_raw(double size, const double com[3], const double r[3], double theta);
+int64_t orc_tree_list(const orc_tree *t, const double *pos, const double p[3], double theta,
+                      int64_t *ids, int32_t *is_body);
 void orc_walk(const orc_tree *t, const double *pos, const double *mass, size_t n,
               double theta, double eps, int use_quad, double *acc, int64_t *counts);
 void orc_walk_subset(const orc_tree *t, const double *pos, const double *mass,

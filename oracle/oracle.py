@@ -62,6 +62,8 @@ def lib():
         L.orc_tree_path_key.argtypes = [P, P, D]; L.orc_tree_path_key.restype = U64
         L.orc_tree_path_keys.argtypes = [P, S, P, D, P]
         L.orc_mac_raw.argtypes = [D, P, P, D]; L.orc_mac_raw.restype = I
+        L.orc_decode_morton_key.argtypes = [U64, D, P]
+        L.orc_tree_list.argtypes = [P, P, P, D, P, P]; L.orc_tree_list.restype = ctypes.c_int64
         L.orc_tree_build_insert.argtypes = [P, P, S, P, D]; L.orc_tree_build_insert.restype = P
         L.orc_tree_build_recursive.argtypes = [P, P, S, P, D, I]; L.orc_tree_build_recursive.restype = P
         L.orc_tree_moments_intended.argtypes = [P, P, P]
@@ -131,6 +133,13 @@ def tree_path_keys(pos, c, s):
     return out
 
 
+def decode_morton_key(key, size):
+    """decodeMortonKey (MortonKeys.hpp:204-228)."""
+    out = np.zeros(3)
+    lib().orc_decode_morton_key(int(key) & 0xFFFFFFFFFFFFFFFF, size, _p(out))
+    return out
+
+
 def mac(size, com, r, theta):
     """LinearHashedOctree::MAC (HashedOctree.cpp:872-885) for one node and point."""
     return bool(lib().orc_mac_raw(size, _p(_c(com, np.float64)), _p(_c(r, np.float64)), theta))
@@ -180,6 +189,16 @@ class Tree:
 
     def moments_intended(self):
         lib().orc_tree_moments_intended(self.h, _p(self._pos), _p(self._mass))
+
+    def interaction_list(self, point, theta):
+        """TraverseHOTInteractionList for one point, in the reference's order:
+        (ids, is_body): a node index, or a body index where is_body is 1."""
+        p = _c(point, np.float64)
+        cap = len(self) + self._pos.shape[0] + 8
+        ids = np.zeros(cap, np.int64)
+        kind = np.zeros(cap, np.int32)
+        nl = lib().orc_tree_list(self.h, _p(self._pos), _p(p), theta, _p(ids), _p(kind))
+        return ids[:nl], kind[:nl]
 
     def walk(self, theta, eps, quad=False, targets=None):
         n = self._pos.shape[0]

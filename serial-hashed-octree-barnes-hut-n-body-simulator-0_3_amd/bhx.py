@@ -90,6 +90,9 @@ def lib():
             "bhx_export_vtk": [P, ctypes.c_char_p, ctypes.c_char_p, I, I],
             "bhx_bounds_path": [P, P],
             "bhx_sort_keys_noalloc": [I, P, P, P, S],
+            "bhx_interaction_lists": [P, P, S, D, P, P, S, P], "bhx_force_lists": [P, P, S, P, P, P],
+            "bhx_mac": [P, P, P, S, D, P], "bhx_node_put": [P, P, P], "bhx_node_erase": [P, ctypes.c_uint64, P],
+            "bhx_prune_empty": [P, ctypes.c_uint64, P], "bhx_decode_morton_keys": [P, P, S, D, P],
         }
         for name, args in sigs.items():
             f = getattr(L, name)
@@ -542,6 +545,66 @@ class Engine:
         self._chk(lib().bhx_energy(self.h, ctypes.byref(k), ctypes.byref(w), ctypes.byref(v), _p(phi)),
                   "bhx_energy")
         return (k.value, w.value, v.value, phi) if potentials else (k.value, w.value, v.value)
+
+    # --- the reference's per-body tree API (one system, last build)
+    def interaction_lists(self, points, theta):
+        """TraverseHOTInteractionList for each point, in the reference's order:
+        (offsets (nq+1,), entries) -- entry >= 0 a node index, -(1+i) body i."""
+        pts = np.ascontiguousarray(points, dtype=np.float64).reshape(-1, 3)
+        nq = pts.shape[0]
+        off = np.zeros(nq + 1, np.uint64)
+        tot = ctypes.c_size_t()
+        self._chk(lib().bhx_interaction_lists(self.h, _p(pts), nq, theta, _p(off), None, 0, ctypes.byref(tot)),
+                  "bhx_interaction_lists")
+        ent = np.zeros(max(1, tot.value), np.int64)
+        self._chk(lib().bhx_interaction_lists(self.h, _p(pts), nq, theta, _p(off), _p(ent), ent.size,
+                                              ctypes.byref(tot)), "bhx_interaction_lists")
+        return off, ent[:tot.value]
+
+    def force_lists(self, points, offsets, entries, acc=None):
+        """ComputeHOTForceInteractionList for each point: acc (default 0) plus its list's terms."""
+        pts = np.ascontiguousarray(points, dtype=np.float64).reshape(-1, 3)
+        nq = pts.shape[0]
+        off = np.ascontiguousarray(offsets, dtype=np.uint64)
+        if off.shape != (nq + 1,):
+            raise ValueError("offsets: expected %d values" % (nq + 1))
+        ent = np.ascontiguousarray(entries, dtype=np.int64)
+        if ent.size < int(off[-1]):
+            raise ValueError("entries: fewer values than offsets[-1]")
+        a = np.zeros((nq, 3)) if acc is None else _f64_exact(acc, (nq, 3), "acc").copy()
+        self._chk(lib().bhx_force_lists(self.h, _p(pts), nq, _p(off), _p(ent), _p(a)), "bhx_force_lists")
+        return a
+
+    def mac(self, node_ids, points, theta):
+        ids = np.ascontiguousarray(node_ids, dtype=np.int64)
+        pts = _f64_exact(points, (ids.size, 3), "points")
+        out = np.zeros(ids.size, np.uint8)
+        self._chk(lib().bhx_mac(self.h, _p(ids), _p(pts), ids.size, theta, _p(out)), "bhx_mac")
+        return out.astype(bool)
+
+    def node_put(self, record):
+        """insertHOTNode: record is a NODE_DTYPE scalar/array of one; returns its node index."""
+        rec = np.ascontiguousarray(np.asarray(record, dtype=NODE_DTYPE).reshape(1))
+        nid = ctypes.c_int64()
+        self._chk(lib().bhx_node_put(self.h, _p(rec), ctypes.byref(nid)), "bhx_node_put")
+        return nid.value
+
+    def node_erase(self, key):
+        done = ctypes.c_int()
+        self._chk(lib().bhx_node_erase(self.h, int(key), ctypes.byref(done)), "bhx_node_erase")
+        return bool(done.value)
+
+    def prune_empty(self, key=1):
+        n = ctypes.c_size_t()
+        self._chk(lib().bhx_prune_empty(self.h, int(key), ctypes.byref(n)), "bhx_prune_empty")
+        return n.value
+
+    def decode_morton_keys(self, keys, size):
+        keys = np.ascontiguousarray(keys, dtype=np.uint64)
+        out = np.zeros((keys.size, 3))
+        self._chk(lib().bhx_decode_morton_keys(self.h, _p(keys), keys.size, size, _p(out)),
+                  "bhx_decode_morton_keys")
+        return out
 
     def bounds_path(self):
         """(element-by-element tiles of x, y, z; 1 if the plain chain ran) of the last

@@ -28,6 +28,7 @@ UNITS = {
     "bhx_api.hip": [],
     "bhx_grad.hip": [],
     "bhx_noalloc.hip": [],
+    "bhx_dropin.hip": ["-ffp-contract=off"],
 }
 HOST_UNITS = {"bhx_ic.cpp": ["-O2", "-fPIC", "-std=c++17", "-ffp-contract=off", "-pthread"],
               "bhx_io.cpp": ["-O2", "-fPIC", "-std=c++17", "-Wall"]}

@@ -257,6 +257,7 @@ int build_on_device(bhx_ctx* c, hipStream_t s) {
     if (total > c->tree.cap) return fail(c, BHX_E_NOMEM, "node capacity");
     c->n_nodes = total;
     c->tree_valid = c->tree_built = true;
+    c->tree_edited = false;
     c->tree_n = c->n;
     return BHX_OK;
 }
@@ -456,6 +457,40 @@ int step_graph_for(bhx_ctx* c, hipGraphExec_t* out, bool linear) {
     c->graph_valid = true;
     *out = c->graphs[2 * pair].exec;
     return BHX_OK;
+}
+
+// Temporary device buffers of one API call, freed when the call returns.
+struct DevTmp {
+    std::vector<void*> ptrs;
+    ~DevTmp() {
+        for (void* p : ptrs) (void)hipFree(p);
+    }
+    template <typename T>
+    hipError_t get(T** out, size_t count) {
+        *out = nullptr;
+        void* p = nullptr;
+        const hipError_t e = hipMalloc(&p, (count ? count : 1) * sizeof(T));
+        if (e == hipSuccess) {
+            ptrs.push_back(p);
+            *out = static_cast<T*>(p);
+        }
+        return e;
+    }
+};
+
+// The per-body tree API needs the node table of a one-system build.
+int need_tree_table(bhx_ctx* c) {
+    if (!c->tree_built) return fail(c, BHX_E_STATE, "no tree built");
+    if (c->nsys > 1) return fail(c, BHX_E_STATE, "several systems: the per-body tree API serves one system");
+    return BHX_OK;
+}
+
+// Level of a node key (sentinel bit at 3 * level), or -1 when it is not a key.
+int key_level(uint64_t k) {
+    if (k == 0) return -1;
+    const int top = 63 - __builtin_clzll(k);
+    if (top % 3 != 0 || top / 3 > kMaxLevel) return -1;
+    return top / 3;
 }
 
 }  // namespace bhx
@@ -1053,6 +1088,8 @@ int bhx_unsplit_leaves(bhx_ctx* c, size_t* count) {
 
 int bhx_walk(bhx_ctx* c, double* acc_out, uint64_t* counts) {
     if (!c) return BHX_E_NULL;
+    if (c->tree_edited) return fail(c, BHX_E_STATE, "nodes were put or erased since the build: rebuild, or use "
+                                                    "bhx_interaction_lists and bhx_force_lists");
     if (!c->tree_valid) return fail(c, BHX_E_STATE, "walk before build");
     HIPCHK(hipSetDevice(c->device));
     hipStream_t s = c->s_main;
@@ -1414,6 +1451,235 @@ int bhx_diagnostics(bhx_ctx* c, double* ekin, double mom[3], double angmom[3]) {
     if (ekin) *ekin = h[0];
     if (mom) { mom[0] = h[1]; mom[1] = h[2]; mom[2] = h[3]; }
     if (angmom) { angmom[0] = h[4]; angmom[1] = h[5]; angmom[2] = h[6]; }
+    return BHX_OK;
+}
+
+
+// ---- the reference's per-body tree API -----------------------------------
+
+int bhx_interaction_lists(bhx_ctx* c, const double* points, size_t nq, double theta, uint64_t* offsets,
+                          int64_t* entries, size_t cap, size_t* total) {
+    if (!c || !offsets || (nq && !points)) return c ? fail(c, BHX_E_NULL, "null argument") : BHX_E_NULL;
+    int rc = need_tree_table(c);
+    if (rc != BHX_OK) return rc;
+    if (!(theta >= 0.0)) return fail(c, BHX_E_ARG, "theta must be >= 0");
+    offsets[0] = 0;
+    if (total) *total = 0;
+    if (nq == 0 || c->n_nodes == 0) {
+        for (size_t q = 0; q <= nq; ++q) offsets[q] = 0;
+        return BHX_OK;
+    }
+    HIPCHK(hipSetDevice(c->device));
+    hipStream_t s = c->s_main;
+    HIPCHK(hipStreamSynchronize(c->s_side));
+    DevTmp tmp;
+    double* dp = nullptr;
+    uint64_t* dcnt = nullptr;
+    uint32_t* derr = nullptr;
+    HIPCHK(tmp.get(&dp, 3 * nq));
+    HIPCHK(tmp.get(&dcnt, nq + 1));
+    HIPCHK(tmp.get(&derr, 1));
+    HIPCHK(hipMemsetAsync(derr, 0, sizeof(uint32_t), s));
+    HIPCHK(hipMemcpyAsync(dp, points, 3 * nq * sizeof(double), hipMemcpyHostToDevice, s));
+    HIPCHK(launch_ref_lists(c, dp, nq, theta, nullptr, dcnt, nullptr, derr, s));
+    std::vector<uint64_t> cnt(nq);
+    uint32_t herr = 0;
+    HIPCHK(hipMemcpyAsync(cnt.data(), dcnt, nq * sizeof(uint64_t), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipMemcpyAsync(&herr, derr, sizeof herr, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    if (herr) return fail(c, BHX_E_DEPTH, "traversal stack bound exceeded");
+    for (size_t q = 0; q < nq; ++q) offsets[q + 1] = offsets[q] + cnt[q];
+    const size_t tot = offsets[nq];
+    if (total) *total = tot;
+    if (!entries || tot == 0) return BHX_OK;
+    if (cap < tot) return fail(c, BHX_E_CAPACITY, "entries hold fewer values than the lists");
+    uint64_t* doff = nullptr;
+    int64_t* dent = nullptr;
+    HIPCHK(tmp.get(&doff, nq + 1));
+    HIPCHK(tmp.get(&dent, tot));
+    HIPCHK(hipMemcpyAsync(doff, offsets, (nq + 1) * sizeof(uint64_t), hipMemcpyHostToDevice, s));
+    HIPCHK(launch_ref_lists(c, dp, nq, theta, doff, dcnt, dent, derr, s));
+    HIPCHK(hipMemcpyAsync(entries, dent, tot * sizeof(int64_t), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    return BHX_OK;
+}
+
+int bhx_force_lists(bhx_ctx* c, const double* points, size_t nq, const uint64_t* offsets, const int64_t* entries,
+                    double* acc) {
+    if (!c || (nq && (!points || !offsets || !acc))) return c ? fail(c, BHX_E_NULL, "null argument") : BHX_E_NULL;
+    int rc = need_tree_table(c);
+    if (rc != BHX_OK) return rc;
+    if (nq == 0) return BHX_OK;
+    for (size_t q = 0; q < nq; ++q)
+        if (offsets[q + 1] < offsets[q]) return fail(c, BHX_E_ARG, "offsets must not decrease");
+    const size_t tot = offsets[nq] - offsets[0];
+    if (tot && !entries) return fail(c, BHX_E_NULL, "entries is a null pointer");
+    HIPCHK(hipSetDevice(c->device));
+    hipStream_t s = c->s_main;
+    HIPCHK(hipStreamSynchronize(c->s_side));
+    DevTmp tmp;
+    double *dp = nullptr, *dacc = nullptr;
+    uint64_t* doff = nullptr;
+    int64_t* dent = nullptr;
+    uint32_t* derr = nullptr;
+    HIPCHK(tmp.get(&dp, 3 * nq));
+    HIPCHK(tmp.get(&dacc, 3 * nq));
+    HIPCHK(tmp.get(&doff, nq + 1));
+    HIPCHK(tmp.get(&dent, tot));
+    HIPCHK(tmp.get(&derr, 1));
+    std::vector<uint64_t> off(offsets, offsets + nq + 1);
+    for (auto& o : off) o -= offsets[0];
+    HIPCHK(hipMemsetAsync(derr, 0, sizeof(uint32_t), s));
+    HIPCHK(hipMemcpyAsync(dp, points, 3 * nq * sizeof(double), hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemcpyAsync(dacc, acc, 3 * nq * sizeof(double), hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemcpyAsync(doff, off.data(), (nq + 1) * sizeof(uint64_t), hipMemcpyHostToDevice, s));
+    if (tot) HIPCHK(hipMemcpyAsync(dent, entries + offsets[0], tot * sizeof(int64_t), hipMemcpyHostToDevice, s));
+    HIPCHK(launch_force_lists(c, dp, nq, doff, dent, dacc, derr, s));
+    uint32_t herr = 0;
+    HIPCHK(hipMemcpyAsync(&herr, derr, sizeof herr, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    if (herr) return fail(c, BHX_E_ARG, "a list entry names no node or body of the tree");
+    HIPCHK(hipMemcpyAsync(acc, dacc, 3 * nq * sizeof(double), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    return BHX_OK;
+}
+
+int bhx_mac(bhx_ctx* c, const int64_t* node_ids, const double* points, size_t n, double theta, uint8_t* accept) {
+    if (!c || (n && (!node_ids || !points || !accept))) return c ? fail(c, BHX_E_NULL, "null argument") : BHX_E_NULL;
+    int rc = need_tree_table(c);
+    if (rc != BHX_OK) return rc;
+    if (n == 0) return BHX_OK;
+    HIPCHK(hipSetDevice(c->device));
+    hipStream_t s = c->s_main;
+    DevTmp tmp;
+    int64_t* did = nullptr;
+    double* dp = nullptr;
+    uint8_t* dout = nullptr;
+    uint32_t* derr = nullptr;
+    HIPCHK(tmp.get(&did, n));
+    HIPCHK(tmp.get(&dp, 3 * n));
+    HIPCHK(tmp.get(&dout, n));
+    HIPCHK(tmp.get(&derr, 1));
+    HIPCHK(hipMemsetAsync(derr, 0, sizeof(uint32_t), s));
+    HIPCHK(hipMemcpyAsync(did, node_ids, n * sizeof(int64_t), hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemcpyAsync(dp, points, 3 * n * sizeof(double), hipMemcpyHostToDevice, s));
+    HIPCHK(launch_mac(c, did, dp, n, theta, dout, derr, s));
+    uint32_t herr = 0;
+    HIPCHK(hipMemcpyAsync(accept, dout, n, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipMemcpyAsync(&herr, derr, sizeof herr, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    if (herr) return fail(c, BHX_E_ARG, "a node index names no node of the tree");
+    return BHX_OK;
+}
+
+int bhx_node_put(bhx_ctx* c, const bhx_node* rec, int64_t* id_out) {
+    if (!c || !rec) return c ? fail(c, BHX_E_NULL, "node is a null pointer") : BHX_E_NULL;
+    int rc = need_tree_table(c);
+    if (rc != BHX_OK) return rc;
+    const int lvl = key_level(rec->key);
+    if (lvl < 0 || lvl != rec->level) return fail(c, BHX_E_ARG, "key and level do not match");
+    if (rec->n < 0 || rec->n > (int64_t)UINT32_MAX || rec->child_byte > 0xFFu)
+        return fail(c, BHX_E_ARG, "count or child bits out of range");
+    HIPCHK(hipSetDevice(c->device));
+    hipStream_t s = c->s_main;
+    HIPCHK(hipStreamSynchronize(c->s_side));
+    DevTmp tmp;
+    uint64_t* dk = nullptr;
+    int64_t* di = nullptr;
+    uint32_t* derr = nullptr;
+    HIPCHK(tmp.get(&dk, 1));
+    HIPCHK(tmp.get(&di, 1));
+    HIPCHK(tmp.get(&derr, 1));
+    HIPCHK(hipMemsetAsync(derr, 0, sizeof(uint32_t), s));
+    HIPCHK(hipMemcpyAsync(dk, &rec->key, sizeof(uint64_t), hipMemcpyHostToDevice, s));
+    int64_t id = -1;
+    if (c->n_nodes) {
+        HIPCHK(launch_lookup(c, 0, dk, 1, di, s));
+        HIPCHK(hipMemcpyAsync(&id, di, sizeof id, hipMemcpyDeviceToHost, s));
+        HIPCHK(hipStreamSynchronize(s));
+    }
+    const bool insert = id < 0;
+    if (insert) {
+        if (c->n_nodes >= c->tree.cap) return fail(c, BHX_E_CAPACITY, "the node table is full");
+        id = (int64_t)c->n_nodes;
+    }
+    HIPCHK(launch_node_put(c, *rec, (uint32_t)id, insert, derr, s));
+    uint32_t herr = 0;
+    HIPCHK(hipMemcpyAsync(&herr, derr, sizeof herr, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    if (herr) return fail(c, BHX_E_HASH_FULL, "node hash table full");
+    if (insert) ++c->n_nodes;
+    c->tree_valid = false;
+    c->tree_edited = true;
+    if (id_out) *id_out = id;
+    return BHX_OK;
+}
+
+int bhx_node_erase(bhx_ctx* c, uint64_t key, int* erased) {
+    if (!c) return BHX_E_NULL;
+    int rc = need_tree_table(c);
+    if (rc != BHX_OK) return rc;
+    if (erased) *erased = 0;
+    if (c->n_nodes == 0 || key_level(key) < 0) return BHX_OK;
+    HIPCHK(hipSetDevice(c->device));
+    hipStream_t s = c->s_main;
+    HIPCHK(hipStreamSynchronize(c->s_side));
+    DevTmp tmp;
+    uint32_t* ddone = nullptr;
+    HIPCHK(tmp.get(&ddone, 1));
+    HIPCHK(launch_node_erase(c, key, ddone, s));
+    uint32_t done = 0;
+    HIPCHK(hipMemcpyAsync(&done, ddone, sizeof done, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    if (done) {
+        c->tree_valid = false;
+        c->tree_edited = true;
+    }
+    if (erased) *erased = done ? 1 : 0;
+    return BHX_OK;
+}
+
+int bhx_prune_empty(bhx_ctx* c, uint64_t key, size_t* removed) {
+    if (!c) return BHX_E_NULL;
+    int rc = need_tree_table(c);
+    if (rc != BHX_OK) return rc;
+    if (removed) *removed = 0;
+    const int lvl = key_level(key);
+    if (lvl < 0) return fail(c, BHX_E_ARG, "not a node key");
+    if (c->n_nodes == 0) return BHX_OK;
+    HIPCHK(hipSetDevice(c->device));
+    hipStream_t s = c->s_main;
+    HIPCHK(hipStreamSynchronize(c->s_side));
+    DevTmp tmp;
+    uint32_t* dn = nullptr;
+    HIPCHK(tmp.get(&dn, 1));
+    HIPCHK(hipMemsetAsync(dn, 0, sizeof(uint32_t), s));
+    HIPCHK(launch_prune(c, key, (uint32_t)lvl, dn, s));
+    uint32_t n = 0;
+    HIPCHK(hipMemcpyAsync(&n, dn, sizeof n, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    if (n) {
+        c->tree_valid = false;
+        c->tree_edited = true;
+    }
+    if (removed) *removed = n;
+    return BHX_OK;
+}
+
+int bhx_decode_morton_keys(bhx_ctx* c, const uint64_t* keys, size_t n, double size, double* pos) {
+    if (!c || (n && (!keys || !pos))) return c ? fail(c, BHX_E_NULL, "null argument") : BHX_E_NULL;
+    if (n == 0) return BHX_OK;
+    HIPCHK(hipSetDevice(c->device));
+    hipStream_t s = c->s_main;
+    DevTmp tmp;
+    uint64_t* dk = nullptr;
+    double* dp = nullptr;
+    HIPCHK(tmp.get(&dk, n));
+    HIPCHK(tmp.get(&dp, 3 * n));
+    HIPCHK(hipMemcpyAsync(dk, keys, n * sizeof(uint64_t), hipMemcpyHostToDevice, s));
+    HIPCHK(launch_decode(dk, n, size, dp, s));
+    HIPCHK(hipMemcpyAsync(pos, dp, 3 * n * sizeof(double), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
     return BHX_OK;
 }
 

@@ -119,6 +119,25 @@ struct SysView {
 
 __device__ __forceinline__ uint32_t sys_of(const SysView& v, uint32_t i) { return v.id ? v.id[i] : 0u; }
 
+// Node hash table: open addressing with linear probing, an empty slot holds
+// key 0 (every valid key carries the root sentinel bit).  A deleted node
+// keeps its key in its slot (so probe sequences stay intact) with this value.
+constexpr uint32_t kHashErased = 0xFFFFFFFFu;
+
+__device__ __forceinline__ uint64_t mix64(uint64_t k) {
+    k ^= k >> 33;
+    k *= 0xff51afd7ed558ccdull;
+    k ^= k >> 33;
+    k *= 0xc4ceb9fe1a85ec53ull;
+    k ^= k >> 33;
+    return k;
+}
+
+// Home slot of (system, key); system 0 hashes the key alone.
+__device__ __forceinline__ uint64_t hash_home(uint64_t k, uint32_t sys) {
+    return mix64(k ^ ((uint64_t)sys * 0x9e3779b97f4a7c15ull));
+}
+
 // The FP32 walks work in scaled units: a system's lengths times lam (the power
 // of two nearest below the inverse root edge) and masses times mu (the same
 // for its first body's mass).  Power-of-two scalings are exact, so wherever
@@ -197,6 +216,7 @@ struct bhx_ctx {
     size_t n_nodes = 0;
     bool tree_valid = false;   // tree built on the current bodies (walk allowed)
     bool tree_built = false;   // node table of the last build still intact (downloads, lookups)
+    bool tree_edited = false;  // nodes put or erased since the build (the walks' records are stale)
     size_t tree_n = 0;         // bodies the last build covered
     // independent systems
     uint32_t nsys = 1;
@@ -358,6 +378,19 @@ hipError_t launch_walk_potential(bhx_ctx* c, hipStream_t s, double* ox, double* 
 hipError_t launch_walk_adjoint(bhx_ctx* c, const WalkTree& t, int precision, const double* wx, const double* wy,
                                const double* wz, double* gself, double* gnode, double* gbody, hipStream_t s);
 hipError_t launch_diag(bhx_ctx* c, double* out7, hipStream_t s);
+
+// the reference's per-body tree API (bhx_dropin.hip); one system
+hipError_t launch_ref_lists(bhx_ctx* c, const double* pts, size_t nq, double theta, const uint64_t* offsets,
+                            uint64_t* counts, int64_t* entries, uint32_t* err, hipStream_t s);
+hipError_t launch_force_lists(bhx_ctx* c, const double* pts, size_t nq, const uint64_t* offsets,
+                              const int64_t* entries, double* acc, uint32_t* err, hipStream_t s);
+hipError_t launch_mac(bhx_ctx* c, const int64_t* ids, const double* pts, size_t n, double theta, uint8_t* out,
+                      uint32_t* err, hipStream_t s);
+hipError_t launch_node_put(bhx_ctx* c, const bhx_node& rec, uint32_t id, bool insert, uint32_t* err,
+                           hipStream_t s);
+hipError_t launch_node_erase(bhx_ctx* c, uint64_t key, uint32_t* done, hipStream_t s);
+hipError_t launch_prune(bhx_ctx* c, uint64_t root_key, uint32_t root_level, uint32_t* removed, hipStream_t s);
+hipError_t launch_decode(const uint64_t* keys, size_t n, double size, double* out, hipStream_t s);
 
 hipError_t scan_exclusive_u32(const uint32_t* in, uint32_t* out, size_t n, uint32_t* tmp,
                               uint32_t* total_dev, hipStream_t s);

@@ -345,20 +345,6 @@ __global__ void k_fill_u64(uint64_t* __restrict__ p, size_t n, uint64_t v) {
 
 // ---------------------------------------------------------------- hash table
 
-__device__ __forceinline__ uint64_t mix64(uint64_t k) {
-    k ^= k >> 33;
-    k *= 0xff51afd7ed558ccdull;
-    k ^= k >> 33;
-    k *= 0xc4ceb9fe1a85ec53ull;
-    k ^= k >> 33;
-    return k;
-}
-
-// Home slot of (system, key); system 0 hashes the key alone.
-__device__ __forceinline__ uint64_t hash_home(uint64_t k, uint32_t sys) {
-    return mix64(k ^ ((uint64_t)sys * 0x9e3779b97f4a7c15ull));
-}
-
 // insertHOTNode — HashedOctree.cpp:152-205: open addressing with linear
 // probing in HBM; an empty slot holds key 0 (no valid key is 0: every key
 // carries the root sentinel bit).  Every node is unique, so an insert takes
@@ -399,7 +385,8 @@ __global__ __launch_bounds__(kBlock) void k_hash_lookup(const uint64_t* __restri
             for (uint64_t probe = 0; probe <= mask; ++probe) {
                 const uint64_t s = hkey[h];
                 if (s == 0) break;
-                if (s == k && sys_of(sv, start[hval[h]]) == sys) { r = (int64_t)hval[h]; break; }
+                const uint32_t v = hval[h];
+                if (s == k && v != kHashErased && sys_of(sv, start[v]) == sys) { r = (int64_t)v; break; }
                 h = (h + 1) & mask;
             }
         }

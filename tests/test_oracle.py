@@ -481,3 +481,84 @@ def test_mac_kat_divides(tmp_path):
         got = bool(mm.orc_mac_raw(size, com.ctypes.data_as(ctypes.c_void_p), r.ctypes.data_as(ctypes.c_void_p),
                                   0.6))
         assert got is (not want)
+
+
+# ---------------------------------------------------------------- decodeMortonKey, list order
+
+def test_decode_morton_key_inverts_the_encoding():
+    """decodeMortonKey (MortonKeys.hpp:204-228): the survey's encode KATs decode back
+    to their points, and at a power-of-two size (every operation exact) any key
+    decodes to its three 21-bit grid coordinates c as c * 2^-20 * S - S."""
+    assert list(orc.decode_morton_key(0xF000000000000000, 1.0)) == [0.0, 0.0, 0.0]
+    assert list(orc.decode_morton_key(0x8000000000000000, 1.0)) == [-1.0, -1.0, -1.0]
+    assert list(orc.decode_morton_key(0xDC00000000000000, 1.0)) == [0.5, -0.5, 0.0]
+    rng = np.random.default_rng(3)
+    for _ in range(200):
+        x, y, z = (int(v) for v in rng.integers(0, 1 << 21, 3))
+        key = interleave_bruteforce(x, y, z)
+        S = 2.0 ** int(rng.integers(-5, 6))
+        got = orc.decode_morton_key(key, S)
+        assert list(got) == [c * 2.0 ** -20 * S - S for c in (x, y, z)]
+        # and encoding the decoded grid point gives the key back
+        assert orc.morton_key(got, S) == key
+
+
+def _reference_order_list(nodes, by_key, members, pos, p, theta):
+    """The interaction list of point p written recursively: a cell emits its
+    accepted children in octant order, then searches its opened children from
+    octant 7 down to 0 (the explicit stack of HashedOctree.hpp:351-380 pops the
+    last pushed child first); a leaf emits itself (or its member bodies)
+    unless at p.  Items: ('n', key) or ('b', body)."""
+    out = []
+    theta2 = theta * theta
+
+    def mac(rec):
+        d = rec["com"] - p
+        d2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2]
+        with np.errstate(divide="ignore"):      # d = 0 (the point itself): s^2/0 = inf, opened
+            return (rec["size"] * rec["size"]) / d2 < theta2
+
+    def visit(rec):
+        if rec["child_byte"]:
+            opened = []
+            for o in range(8):
+                if rec["child_byte"] >> o & 1:
+                    ch = nodes[by_key[(int(rec["key"]) << 3) | o]]
+                    if mac(ch):
+                        out.append(("n", int(ch["key"])))
+                    else:
+                        opened.append(ch)
+            for ch in reversed(opened):
+                visit(ch)
+        elif rec["member_count"] <= 1:
+            if np.any(rec["com"] != p):
+                out.append(("n", int(rec["key"])))
+        else:
+            for i in members[int(rec["key"])]:
+                if np.any(pos[i] != p):
+                    out.append(("b", int(i)))
+
+    visit(nodes[by_key[1]])
+    return out
+
+
+@pytest.mark.parametrize("cap", [1, 4])
+def test_interaction_list_order_matches_recursive_definition(cap):
+    pos, _, mass = plummer_np(600, 12)
+    c, s = orc.bounds(pos)
+    t = orc.build_insert(pos, mass, c, s) if cap == 1 else orc.build_recursive(pos, mass, c, s, cap)
+    nd = t.nodes()
+    by_key = {int(k): i for i, k in enumerate(nd["key"])}
+    pk = orc.tree_path_keys(pos, c, s)
+    members = {}
+    for rec in nd[nd["child_byte"] == 0]:
+        L = int(rec["level"])
+        members[int(rec["key"])] = np.nonzero((pk >> np.uint64(63 - 3 * L)) == rec["key"])[0]
+    _, counts = t.walk(0.5, 0.01)
+    for q in list(range(0, 600, 37)) + [-1]:
+        p = pos[q] if q >= 0 else np.array([0.3, -0.1, 2.0])
+        ids, kind = t.interaction_list(p, 0.5)
+        got = [("b", int(i)) if k else ("n", int(nd["key"][i])) for i, k in zip(ids, kind)]
+        assert got == _reference_order_list(nd, by_key, members, pos, p, 0.5)
+        if q >= 0:
+            assert len(got) == counts[q]
