@@ -71,6 +71,10 @@ typedef struct bhx_config {
                               recurrence from a root edge of twice the size, so
                               every cell contains its bodies and the opening test
                               uses the true edge.  Morton keys are unchanged.       */
+    int32_t group_size;    /* FP32 group walk: bodies that share one interaction list,
+                              64, 32 or 16 (one 64-body wave walks 1, 2 or 4 lists);
+                              smaller groups evaluate fewer pairs but keep more
+                              lists (bhx_autotune measures which is faster).  0: 64 */
 } bhx_config;
 
 typedef struct bhx_ctx bhx_ctx;
@@ -452,7 +456,7 @@ int bhx_ic_galaxy_pair(size_t n, uint64_t seed, const bhx_galaxy_params *p, int 
  * uint64 step, uint32 flags (bit 0: accelerations stored, bit 1:
  * compensation terms stored), uint32 0, then the configuration (theta, eps,
  * dt as doubles; leaf_capacity, mode, precision, quadrupole, permute_acc,
- * compensated, device, group_walk, geometry as int32), zero padding — followed by
+ * compensated, device, group_walk, geometry, group_size as int32), zero padding — followed by
  * pos[3n], vel[3n], mass[n] (doubles, xyz interleaved), acc[3n] if stored,
  * the position then velocity compensation terms [6n] if stored, and
  * offsets[nsys + 1] (uint64). */

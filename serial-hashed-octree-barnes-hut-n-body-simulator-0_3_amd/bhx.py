@@ -40,7 +40,8 @@ class Config(ctypes.Structure):
     _fields_ = [("theta", ctypes.c_double), ("eps", ctypes.c_double), ("dt", ctypes.c_double),
                 ("leaf_capacity", ctypes.c_int32), ("mode", ctypes.c_int32), ("precision", ctypes.c_int32),
                 ("quadrupole", ctypes.c_int32), ("permute_acc", ctypes.c_int32), ("compensated", ctypes.c_int32),
-                ("device", ctypes.c_int32), ("group_walk", ctypes.c_int32), ("geometry", ctypes.c_int32)]
+                ("device", ctypes.c_int32), ("group_walk", ctypes.c_int32), ("geometry", ctypes.c_int32),
+                ("group_size", ctypes.c_int32)]
 
 
 class Timing(ctypes.Structure):

@@ -124,7 +124,7 @@ void free_all(bhx_ctx* c) {
     for (double*& p : c->alt)
         if (p) { (void)hipFree(p); p = nullptr; }
     void* vp[] = {c->keys, c->keys_alt, c->vals, c->vals_alt, c->mkeys, c->tkeys, c->tperm, c->nn, c->noff,
-                  c->nlo, c->scan_tmp, c->sort_hist, c->sort_status, c->sort_tile_ctr, c->wcount, c->ctr, c->tb4, c->tleaf,
+                  c->nlo, c->scan_tmp, c->sort_hist, c->sort_status, c->sort_tile_ctr, c->wcount, c->wq, c->ctr, c->tb4, c->tleaf,
                   c->sys_off, c->sysid, c->sys_root, c->sys_scale, c->gtab, c->skey, c->skey_alt, c->sidx, c->sidx_alt,
                   c->sum_tile, c->sum_cnt, c->sum_piece};
     for (void* p : vp)
@@ -145,8 +145,13 @@ int check_cfg(const bhx_config* cfg) {
     if (cfg->mode != BHX_MODE_AS_RUN && cfg->mode != BHX_MODE_INTENDED) return BHX_E_ARG;
     if (cfg->precision != BHX_FP64 && cfg->precision != BHX_FP32) return BHX_E_ARG;
     if (cfg->geometry != 0 && cfg->geometry != 1) return BHX_E_ARG;
+    if (cfg->group_size != 0 && cfg->group_size != 16 && cfg->group_size != 32 && cfg->group_size != 64)
+        return BHX_E_ARG;
     return BHX_OK;
 }
+
+// Lists per 64-body wave of the FP32 group walk for a configured group size.
+int walk_lists_for(const bhx_config* cfg) { return cfg->group_size == 16 ? 4 : (cfg->group_size == 32 ? 2 : 1); }
 
 // Per-system buffers for `nsys` systems (bounds slots, offsets, roots).
 hipError_t ensure_sys_cap(bhx_ctx* c, size_t nsys) {
@@ -523,6 +528,7 @@ int bhx_create(const bhx_config* cfg, size_t capacity, bhx_ctx** out) {
     c->cfg = *cfg;
     c->device = cfg->device;
     c->cap = capacity;
+    c->walk_sub = walk_lists_for(cfg);
     int rc = BHX_OK;
     auto chk = [&](hipError_t e, const char* what) {
         if (e != hipSuccess && rc == BHX_OK) rc = fail(c, e == hipErrorOutOfMemory ? BHX_E_NOMEM : BHX_E_HIP, what, e);
@@ -555,6 +561,7 @@ int bhx_create(const bhx_config* cfg, size_t capacity, bhx_ctx** out) {
         chk(dalloc(&c->nlo, n + 1), "alloc build");
         chk(dalloc(&c->scan_tmp, n / 2048 + 16), "alloc scan");
         chk(dalloc(&c->wcount, n), "alloc counts");
+        chk(dalloc(&c->wq, 64), "alloc work counters");
         c->sort_tiles = sort_tiles_for(n);
         chk(dalloc(&c->sort_hist, 8 * 256), "alloc sort");
         chk(dalloc(&c->sort_tile_ctr, 8), "alloc sort");
@@ -596,6 +603,7 @@ int bhx_set_config(bhx_ctx* c, const bhx_config* cfg) {
     const bool tree_shape = cfg->leaf_capacity != c->cfg.leaf_capacity || cfg->mode != c->cfg.mode ||
                             cfg->geometry != c->cfg.geometry;
     c->cfg = *cfg;
+    c->walk_sub = walk_lists_for(cfg);
     if (tree_shape) c->tree_valid = false;
     drop_graphs(c);   // captured launches hold the old parameters
     return BHX_OK;

@@ -45,7 +45,7 @@ struct alignas(32) WalkVisit32 {
     uint32_t meta;          // level (bits 0-4) | leaf (bit 5) | min(count, 2^26-1) << 6
     uint32_t skip;
     uint32_t start;         // first body in tree order
-    uint32_t pad;
+    uint32_t child_byte;    // bit o: a child in octant o (its id in the octant table)
     float size2;            // cell edge squared
 };
 static_assert(sizeof(WalkVisit32) == 32, "FP32 visit record is 32 bytes");
@@ -201,6 +201,8 @@ struct bhx_ctx {
     size_t sort_tiles = 0;
     // walk scratch
     uint32_t* wcount = nullptr;                // interaction counts per body (array order)
+    uint32_t* wq = nullptr;                    // persistent walks: 8 per-XCD group counters
+    int walk_sub = 1;                          // FP32 group walk: lists per 64-body wave (1, 2 or 4)
     // bounds: two slots of (cx, cy, cz, size)
     // (nsys x 4 doubles per slot; slots 0 and 1 alternate between frames, slot 2 is scratch)
     double* bounds = nullptr;

@@ -101,10 +101,10 @@ void encode_header(unsigned char* h, const bhx_snapshot_info& in, uint64_t nsys,
     put_f64(h + 56, in.cfg.theta);
     put_f64(h + 64, in.cfg.eps);
     put_f64(h + 72, in.cfg.dt);
-    const int32_t ints[9] = {in.cfg.leaf_capacity, in.cfg.mode,        in.cfg.precision, in.cfg.quadrupole,
-                             in.cfg.permute_acc,   in.cfg.compensated, in.cfg.device,    in.cfg.group_walk,
-                             in.cfg.geometry};
-    for (int k = 0; k < 9; ++k) put_u32(h + 80 + 4 * k, (uint32_t)ints[k]);
+    const int32_t ints[10] = {in.cfg.leaf_capacity, in.cfg.mode,        in.cfg.precision, in.cfg.quadrupole,
+                              in.cfg.permute_acc,   in.cfg.compensated, in.cfg.device,    in.cfg.group_walk,
+                              in.cfg.geometry,      in.cfg.group_size};
+    for (int k = 0; k < 10; ++k) put_u32(h + 80 + 4 * k, (uint32_t)ints[k]);
 }
 
 int decode_header(const unsigned char* h, long file_bytes, bhx_snapshot_info* out) {
@@ -123,10 +123,10 @@ int decode_header(const unsigned char* h, long file_bytes, bhx_snapshot_info* ou
     in.cfg.theta = get_f64(h + 56);
     in.cfg.eps = get_f64(h + 64);
     in.cfg.dt = get_f64(h + 72);
-    int32_t* ints[9] = {&in.cfg.leaf_capacity, &in.cfg.mode,        &in.cfg.precision, &in.cfg.quadrupole,
-                        &in.cfg.permute_acc,   &in.cfg.compensated, &in.cfg.device,    &in.cfg.group_walk,
-                        &in.cfg.geometry};
-    for (int k = 0; k < 9; ++k) *ints[k] = (int32_t)get_u32(h + 80 + 4 * k);
+    int32_t* ints[10] = {&in.cfg.leaf_capacity, &in.cfg.mode,        &in.cfg.precision, &in.cfg.quadrupole,
+                         &in.cfg.permute_acc,   &in.cfg.compensated, &in.cfg.device,    &in.cfg.group_walk,
+                         &in.cfg.geometry,      &in.cfg.group_size};
+    for (int k = 0; k < 10; ++k) *ints[k] = (int32_t)get_u32(h + 80 + 4 * k);   // older files: group_size 0
     // sizes must match the file exactly (guards against truncation and against
     // counts that would overflow the arithmetic below)
     if (in.n > ((uint64_t)1 << 40) || in.nsys == 0 || in.nsys > ((uint64_t)1 << 32)) return BHX_E_IO;

@@ -618,7 +618,7 @@ __device__ Payload moments_of(uint32_t p, int intended, const TreeArgs& t, const
         v32.meta = (uint32_t)L | (leaf ? 32u : 0u) | ((cnt < (1u << 26) ? cnt : (1u << 26) - 1) << 6);
         v32.skip = t.skip[p];
         v32.start = st;
-        v32.pad = 0u;
+        v32.child_byte = cb;
         v32.size2 = (float)(s * s * lam2);
         t.walk32[p] = v32;
         WalkPay32 p32;

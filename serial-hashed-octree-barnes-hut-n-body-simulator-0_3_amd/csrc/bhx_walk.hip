@@ -79,6 +79,10 @@ struct WalkArgs {
     double* gself;
     double* gnode;
     double* gbody;
+    // persistent walks: per-XCD counters of the groups taken (8, zero at launch)
+    uint32_t* wq;
+    const uint32_t* child;   // octant table: 8 child ids per node
+    const uint32_t* child_byte;
 };
 
 // One term of ComputeHOTForceInteractionList (HashedOctree.hpp:411-454) in
@@ -708,6 +712,9 @@ __device__ __forceinline__ Node32 load_node32(ConstU64 raw, uint32_t p) {
 template <bool kQuad, bool kGuard, bool kPot = false>
 __device__ __forceinline__ void eval_list32(const float (*nd)[4], const float (*q)[6], int nn, float xr, float yr,
                                             float zr, float eps2, float& ax, float& ay, float& az, float& phi) {
+    // two entries per pass: their LDS reads share one address register
+    // (constant offsets) instead of two register moves per entry
+#pragma unroll 2
     for (int k = 0; k < nn; ++k) {
         const float dx = nd[k][0] - xr, dy = nd[k][1] - yr, dz = nd[k][2] - zr;
         float inv, D2 = 0.f;
@@ -1018,6 +1025,677 @@ __global__ __launch_bounds__(64) void k_walk_group32(WalkArgs a) {
     }
 }
 
+// ---------------------------------------------------------------- lane-parallel FP32 walk
+//
+// The shared-list rule of k_walk_group32 (a cell enters the group's list only
+// when every body of the group accepts it, and is opened as soon as one body
+// opens it), with the traversal spread over the lanes instead of one cell per
+// visit.  Each step takes up to 64 cells from a stack in LDS, one per lane;
+// lane j tests its cell against every body of the group at once through the
+// least |d|^2 over the bodies (fl(theta^2 x) is monotone in x, so every body
+// accepts exactly when edge^2 < theta^2 min|d|^2); accepted cells join the
+// cell list, opened leaves the body list, and the children of opened cells
+// (from the octant table, one 32-byte load per opened cell) go on the stack.
+// A group's ~2,800 dependent visits become ~50 independent steps.  Should
+// the stack not hold the children of every opened cell, the cells left over
+// are walked one cell at a time, as k_walk_group32 does (no cell is lost).
+//
+// Persistent: the grid is what the chip holds at once.  XCD x (workgroups
+// x, x + 8, ... are dispatched to it) owns the contiguous tree-order slice
+// of groups [x * per, (x + 1) * per); its waves take groups from that slice
+// through an atomic counter, then from the other XCDs' slices, and leave
+// when every slice is taken.
+
+constexpr int kLpStack = 1024;     // LDS stack of cells to test
+constexpr int kLpCells = 128;      // accepted cell ids waiting for evaluation (64 at a time)
+constexpr int kLpBodies = 96;
+
+// The next group for this wave (the same value in every lane), or `groups`.
+__device__ __forceinline__ uint32_t lp_next_group(uint32_t* wq, uint32_t groups, uint32_t lane) {
+    uint32_t g = groups;
+    if (lane == 0) {
+        const uint32_t per = (groups + 7u) / 8u;
+        const uint32_t xcd = blockIdx.x & 7u;
+        for (uint32_t k = 0; k < 8; ++k) {
+            const uint32_t x = (xcd + k) & 7u;
+            const uint32_t lo = x * per;
+            const uint32_t len = lo < groups ? (groups - lo < per ? groups - lo : per) : 0u;
+            if (len == 0u) continue;
+            if (__hip_atomic_load(wq + x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= len) continue;
+            const uint32_t t = atomicAdd(wq + x, 1u);
+            if (t < len) {
+                g = lo + t;
+                break;
+            }
+        }
+    }
+    return __shfl(g, 0, 64);
+}
+
+__device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v, uint32_t lane) {
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        const uint32_t t = __shfl_up(v, off, 64);
+        if ((int)lane >= off) v += t;
+    }
+    return v;
+}
+
+// Evaluation of the lane-parallel walk's lists.  Sub-group q's cells are the
+// K = 64 / S entries nd[q K .. q K + K); lane l (sub-group l / K) walks its
+// own K entries.  Bodies are one list for the whole wave; a body entry counts
+// for the sub-groups in its mask only.
+template <int S, bool kQuad, bool kGuard, bool kPot>
+__device__ __forceinline__ void lp_eval_cells(const float (*nd)[4], const float (*q)[6], uint32_t lane, float xr,
+                                              float yr, float zr, float eps2, float& ax, float& ay, float& az,
+                                              float& phi) {
+    constexpr int K = 64 / S;
+    const int off = S == 1 ? 0 : (int)(lane / K) * K;
+    int n = K;
+    asm volatile("" : "+s"(n));   // a loop of K entries, not K unrolled copies (registers)
+    eval_list32<kQuad, kGuard, kPot>(nd + off, q + off, n, xr, yr, zr, eps2, ax, ay, az, phi);
+}
+
+template <int S, bool kGuard, bool kPot>
+__device__ __forceinline__ void lp_eval_bodies(const float (*bd)[4], const uint8_t* bm, int nb, uint32_t lane,
+                                               float xr, float yr, float zr, float eps2, float& ax, float& ay,
+                                               float& az, float& phi) {
+    if constexpr (S == 1) {
+        eval_list32<false, kGuard, kPot>(bd, nullptr, nb, xr, yr, zr, eps2, ax, ay, az, phi);
+    } else {
+        const uint32_t bit = 1u << (lane / (64 / S));
+        for (int k = 0; k < nb; ++k) {
+            const float m = (bm[k] & bit) ? bd[k][3] : 0.f;
+            const float dx = bd[k][0] - xr, dy = bd[k][1] - yr, dz = bd[k][2] - zr;
+            float inv;
+            if (kGuard) {
+                const float d2 = dx * dx + dy * dy + dz * dz;
+                inv = d2 > 0.f ? __builtin_amdgcn_rsqf(d2 + eps2) : 0.f;
+            } else {
+                inv = __builtin_amdgcn_rsqf(fmaf(dz, dz, fmaf(dy, dy, fmaf(dx, dx, eps2))));
+            }
+            const float c = m * (inv * inv * inv);
+            ax = fmaf(c, dx, ax);
+            ay = fmaf(c, dy, ay);
+            az = fmaf(c, dz, az);
+            if (kPot) phi -= (dx * dx + dy * dy + dz * dz) > 0.f ? m * inv : 0.f;
+        }
+    }
+}
+
+template <bool kQuad, bool kGuard, bool kPot, int S>
+__global__ __launch_bounds__(64) void k_walk_lp32(WalkArgs a) {
+    static_assert(S == 1 || S == 2 || S == 4, "1, 2 or 4 sub-groups per wave");
+    constexpr int K = 64 / S;                  // bodies per sub-group, cells per sub-group and flush
+    constexpr uint32_t kAll = (1u << S) - 1u;
+    __shared__ float s_nd[kCells32][4];
+    __shared__ float s_q[kQuad ? kCells32 : 1][6];
+    __shared__ float s_bd[kLpBodies][4];
+    __shared__ uint8_t s_bm[S > 1 ? kLpBodies : 1];
+    __shared__ uint32_t s_rstart[kRanges32], s_roff[kRanges32], s_rleaf[kRanges32];
+    __shared__ uint8_t s_rmask[kRanges32];
+    __shared__ float s_rdel[kRanges32][3];
+    __shared__ float4 s_tgt[64];
+    __shared__ uint32_t s_cid[S][kLpCells];
+    __shared__ uint32_t s_stack[kLpStack];
+    __shared__ uint8_t s_smask[S > 1 ? kLpStack : 1];
+    __shared__ uint8_t s_om[S > 1 ? 32 : 1];   // one-cell-at-a-time walk: open mask per level
+
+    if (*a.abort_flag) return;
+    const uint32_t lane = threadIdx.x;
+    const uint32_t sg = lane / K;              // this lane's sub-group
+    ConstU64 raw = (ConstU64)a.nodes32;
+    const uint4* __restrict__ rec4 = reinterpret_cast<const uint4*>(a.nodes32);
+    ConstD tx = (ConstD)a.tx;
+    ConstD ty = (ConstD)a.ty;
+    ConstD tz = (ConstD)a.tz;
+    const float th2 = a.theta2f;
+
+    for (;;) {
+        const uint32_t grp = lp_next_group(a.wq, a.groups, lane);
+        if (grp >= a.groups) break;
+        const uint4 gr = a.gtab[grp];
+        const uint32_t g0 = gr.x;
+        const uint32_t i = g0 + lane;
+        const bool valid = i < gr.y;
+        const uint64_t vmask = __builtin_amdgcn_ballot_w64(valid);
+        uint32_t live_sg = 0;                  // sub-groups with at least one body
+#pragma unroll
+        for (int q = 0; q < S; ++q)
+            if ((vmask >> (q * K)) & (K == 64 ? ~0ull : ((1ull << K) - 1ull))) live_sg |= 1u << q;
+        const uint32_t iv = valid ? i : g0;
+        const uint32_t rid = a.sys_root[gr.z];
+        double ox = tx[g0], oy = ty[g0], oz = tz[g0];
+        asm volatile("" : "+v"(ox), "+v"(oy), "+v"(oz));
+        const uint32_t own = a.tleaf[iv];
+        const float4 ob = a.tb4[iv];
+        const double lam = a.scale32[2 * (size_t)gr.z], mu = a.scale32[2 * (size_t)gr.z + 1];
+        const float xr = (float)((a.com[3 * (size_t)own] - ox) * lam) + ob.x;
+        const float yr = (float)((a.com[3 * (size_t)own + 1] - oy) * lam) + ob.y;
+        const float zr = (float)((a.com[3 * (size_t)own + 2] - oz) * lam) + ob.z;
+        const float eps2 = (float)(a.eps2 * lam * lam);
+        const double* b4s = a.b4 + 4 * (size_t)gr.z;
+        const float xs = (float)((tx[iv] - b4s[0]) * lam), ys = (float)((ty[iv] - b4s[1]) * lam),
+                    zs = (float)((tz[iv] - b4s[2]) * lam);
+        float ax = 0.f, ay = 0.f, az = 0.f, phi = 0.f;
+        uint32_t cnt = 0;
+        int nc[S];                             // cells waiting per sub-group
+#pragma unroll
+        for (int q = 0; q < S; ++q) nc[q] = 0;
+        int nr = 0, nq = 0;                    // leaf ranges, bodies in them
+
+        __syncthreads();                       // the previous group is done with the lists
+        s_tgt[lane] = valid ? make_float4(xs, ys, zs, 0.f) : make_float4(INFINITY, INFINITY, INFINITY, 0.f);
+
+        // one chunk of every sub-group's cell list: up to K cells each
+        auto flush_cells = [&]() {
+            __syncthreads();
+            const uint32_t k = lane % K;
+            int t = 0;
+#pragma unroll
+            for (int q = 0; q < S; ++q)
+                if ((int)sg == q) t = nc[q] < K ? nc[q] : K;
+            const bool has = (int)k < t;
+            const uint32_t id = has ? s_cid[sg][k] : 0u;
+            {
+                // lane l gathers entry l: sub-group l / K, position l % K
+                const WalkPay32& c = a.pay32[id];
+                s_nd[lane][0] = (float)((a.com[3 * (size_t)id] - ox) * lam);
+                s_nd[lane][1] = (float)((a.com[3 * (size_t)id + 1] - oy) * lam);
+                s_nd[lane][2] = (float)((a.com[3 * (size_t)id + 2] - oz) * lam);
+                s_nd[lane][3] = has ? c.mass : 0.f;      // padding: a massless cell at a finite place
+                if (kQuad) {
+#pragma unroll
+                    for (int u = 0; u < 6; ++u) s_q[lane][u] = has ? c.q[u] : 0.f;
+                }
+            }
+            // the rest of each list moves to its front
+            uint32_t rest[kLpCells / 64];
+#pragma unroll
+            for (int h = 0; h < kLpCells / 64; ++h) rest[h] = 0u;
+            __syncthreads();
+            lp_eval_cells<S, kQuad, kGuard, kPot>(s_nd, s_q, lane, xr, yr, zr, eps2, ax, ay, az, phi);
+            cnt += (uint32_t)t;
+#pragma unroll
+            for (int q = 0; q < S; ++q) {
+                const int tq = nc[q] < K ? nc[q] : K;
+#pragma unroll
+                for (int h = 0; h < kLpCells / 64; ++h) {
+                    const int e = (int)lane + 64 * h;
+                    rest[h] = e + tq < nc[q] ? s_cid[q][e + tq] : 0u;
+                }
+                __syncthreads();
+#pragma unroll
+                for (int h = 0; h < kLpCells / 64; ++h) {
+                    const int e = (int)lane + 64 * h;
+                    if (e + tq < nc[q]) s_cid[q][e] = rest[h];
+                }
+                nc[q] -= tq;
+            }
+            __syncthreads();
+        };
+        auto flush_bodies = [&]() {
+            __syncthreads();
+            if ((int)lane < nr) {
+                const size_t l3 = 3 * (size_t)s_rleaf[lane];
+                s_rdel[lane][0] = (float)((a.com[l3] - ox) * lam);
+                s_rdel[lane][1] = (float)((a.com[l3 + 1] - oy) * lam);
+                s_rdel[lane][2] = (float)((a.com[l3 + 2] - oz) * lam);
+            }
+            __syncthreads();
+#pragma unroll
+            for (int h = 0; h < (kLpBodies + 63) / 64; ++h) {
+                const int e = (int)lane + 64 * h;
+                if (e < nq) {
+                    int lo = 0, hi = nr - 1;
+                    while (lo < hi) {
+                        const int mid = (lo + hi + 1) >> 1;
+                        if ((int)s_roff[mid] <= e) lo = mid; else hi = mid - 1;
+                    }
+                    const uint32_t j = s_rstart[lo] + (uint32_t)(e - (int)s_roff[lo]);
+                    const float4 b = a.tb4[j];
+                    s_bd[e][0] = s_rdel[lo][0] + b.x;
+                    s_bd[e][1] = s_rdel[lo][1] + b.y;
+                    s_bd[e][2] = s_rdel[lo][2] + b.z;
+                    s_bd[e][3] = b.w;
+                    if constexpr (S > 1) s_bm[e] = s_rmask[lo];
+                }
+            }
+            __syncthreads();
+            lp_eval_bodies<S, kGuard, kPot>(s_bd, s_bm, nq, lane, xr, yr, zr, eps2, ax, ay, az, phi);
+            if constexpr (S == 1) {
+                cnt += (uint32_t)nq;
+            } else {
+                for (int e = 0; e < nq; ++e) cnt += (s_bm[e] >> sg) & 1u;
+            }
+            nr = 0;
+            nq = 0;
+        };
+        // the bodies of leaf `leafid` (count bodies from `st`) join the body list
+        // for the sub-groups in `mask`
+        auto add_leaf = [&](uint32_t leafid, uint32_t st, uint32_t count, uint32_t mask) {
+            for (uint32_t k = 0; k < count; k += 64) {
+                const int m = (int)((count - k) < 64u ? (count - k) : 64u);
+                if (nq + m > kLpBodies || nr == kRanges32) flush_bodies();
+                if (lane == 0) {
+                    s_rstart[nr] = st + k;
+                    s_roff[nr] = (uint32_t)nq;
+                    s_rleaf[nr] = leafid;
+                    if constexpr (S > 1) s_rmask[nr] = (uint8_t)mask;
+                }
+                ++nr;
+                nq += m;
+            }
+        };
+        // flush while every list has a full chunk, or some list could not take
+        // another step's 64 cells
+        auto maybe_flush = [&]() {
+            for (;;) {
+                bool all_full = true, any_high = false;
+#pragma unroll
+                for (int q = 0; q < S; ++q) {
+                    if (((live_sg >> q) & 1u) && nc[q] < K) all_full = false;
+                    if (nc[q] > kLpCells - 64) any_high = true;
+                }
+                if (!(all_full || any_high)) break;
+                flush_cells();
+            }
+        };
+        // the subtree below an opened cell p0 (open for the sub-groups in m0),
+        // one cell at a time (the stack is full)
+        auto walk_subtree = [&](uint32_t p0, uint32_t end, uint32_t m0) {
+            const uint32_t L0 = raw[4 * (size_t)p0 + 1] >> 32 & 31u;   // meta of p0: its level
+            if constexpr (S > 1) {
+                if (lane == 0) s_om[L0] = (uint8_t)m0;
+                __syncthreads();
+            }
+            uint32_t q = p0 + 1;
+            while (q < end) {
+                q = __builtin_amdgcn_readfirstlane(q);
+                const Node32 nd = load_node32(raw, q);
+                const uint32_t nskip = nd.skip > q ? nd.skip : q + 1;
+                const uint32_t L = nd.meta & 31u;
+                const uint32_t A = S > 1 ? (uint32_t)s_om[L - 1] : 1u;   // open mask of its parent
+                const float dx = nd.cx - xs, dy = nd.cy - ys, dz = nd.cz - zs;
+                const float d2 = dx * dx + dy * dy + dz * dz;
+                const uint64_t ob = __builtin_amdgcn_ballot_w64(!(nd.size2 < th2 * d2)) & vmask;
+                uint32_t O = 0;                // sub-groups that open it
+#pragma unroll
+                for (int s2 = 0; s2 < S; ++s2)
+                    if ((ob >> (s2 * K)) & (K == 64 ? ~0ull : ((1ull << K) - 1ull))) O |= 1u << s2;
+                O &= A;
+                const uint32_t acc = A & ~O & live_sg;
+                if (acc) {
+#pragma unroll
+                    for (int s2 = 0; s2 < S; ++s2)
+                        if ((acc >> s2) & 1u) {
+                            if (lane == 0) s_cid[s2][nc[s2]] = q;
+                            ++nc[s2];
+                        }
+                    maybe_flush();
+                }
+                if (O == 0u) {
+                    q = nskip;
+                } else if (nd.meta & 32u) {
+                    add_leaf(q, nd.start, nd.meta >> 6, O);
+                    q = nskip;
+                } else {
+                    if constexpr (S > 1) {
+                        __syncthreads();
+                        if (lane == 0) s_om[L] = (uint8_t)O;
+                        __syncthreads();
+                    }
+                    q = q + 1;
+                }
+            }
+        };
+
+        int top = 0;
+        {
+            const Node32 root = load_node32(raw, rid);
+            if (root.meta & 32u) {
+                add_leaf(rid, root.start, root.meta >> 6, live_sg);   // a one-leaf tree: all its bodies
+            } else {
+                const uint32_t cb = __builtin_amdgcn_readfirstlane(rec4[2 * (size_t)rid + 1].z);
+                if (lane < 8u && ((cb >> lane) & 1u)) {
+                    const uint32_t slot = __popc(cb & ((1u << lane) - 1u));
+                    s_stack[slot] = a.child[8 * (size_t)rid + lane];
+                    if constexpr (S > 1) s_smask[slot] = (uint8_t)live_sg;
+                }
+                top = __popc(cb);
+            }
+        }
+        __syncthreads();
+        while (top > 0) {
+            const int k = top < 64 ? top : 64;
+            const int base = top - k;
+            const bool live = (int)lane < k;
+            const uint32_t p = live ? s_stack[base + lane] : rid;
+            const uint32_t A = live ? (S > 1 ? (uint32_t)s_smask[base + lane] : 1u) : 0u;
+            top = base;
+            const uint4 r0 = rec4[2 * (size_t)p], r1 = rec4[2 * (size_t)p + 1];
+            const float cx = __uint_as_float(r0.x), cy = __uint_as_float(r0.y), cz = __uint_as_float(r0.z);
+            const uint32_t meta = r0.w, skip = r1.x, start = r1.y, cb = r1.z;
+            const float size2 = __uint_as_float(r1.w);
+            // least |d|^2 over each sub-group's bodies (absent bodies sit at infinity)
+            float md2[S];
+#pragma unroll
+            for (int q = 0; q < S; ++q) {
+                float m = INFINITY;
+#pragma unroll 8
+                for (int b = 0; b < K; ++b) {
+                    const float4 t = s_tgt[q * K + b];
+                    const float dx = cx - t.x, dy = cy - t.y, dz = cz - t.z;
+                    m = fminf(m, dx * dx + dy * dy + dz * dz);
+                }
+                md2[q] = m;
+            }
+            uint32_t P = 0;                    // sub-groups where every body accepts the cell
+#pragma unroll
+            for (int q = 0; q < S; ++q)
+                if (size2 < th2 * md2[q]) P |= 1u << q;
+            const uint32_t acc = A & P;
+            const uint32_t O = A & ~P;          // sub-groups that open it
+            const bool leaf = (meta & 32u) != 0u;
+            const bool open_in = O != 0u && !leaf;
+            // children of the opened cells onto the stack, as far as it holds them
+            const uint32_t nch = open_in ? (uint32_t)__popc(cb) : 0u;
+            const uint32_t incl = wave_incl_scan(nch, lane);
+            const uint32_t room = (uint32_t)(kLpStack - top);
+            const uint64_t m_fit = __builtin_amdgcn_ballot_w64(incl <= room);   // a prefix of the lanes
+            const int nfit = __popcll(m_fit);
+            const bool push = open_in && ((m_fit >> lane) & 1ull);
+            const uint64_t m_over = __builtin_amdgcn_ballot_w64(open_in && !push);
+            const uint64_t m_leaf = __builtin_amdgcn_ballot_w64(O != 0u && leaf);
+            const uint32_t pushed = nfit > 0 ? __shfl(incl, nfit - 1, 64) : 0u;
+            __syncthreads();                   // every lane has read its cell before the stack is written
+            if (push) {
+                const uint4 ca = *reinterpret_cast<const uint4*>(a.child + 8 * (size_t)p);
+                const uint4 cbv = *reinterpret_cast<const uint4*>(a.child + 8 * (size_t)p + 4);
+                const uint32_t ids[8] = {ca.x, ca.y, ca.z, ca.w, cbv.x, cbv.y, cbv.z, cbv.w};
+                uint32_t w = (uint32_t)top + incl - nch;
+#pragma unroll
+                for (int o = 0; o < 8; ++o)
+                    if ((cb >> o) & 1u) {
+                        if constexpr (S > 1) s_smask[w] = (uint8_t)O;
+                        s_stack[w++] = ids[o];
+                    }
+            }
+            top += (int)pushed;
+            // accepted cells join their sub-groups' lists
+#pragma unroll
+            for (int q = 0; q < S; ++q) {
+                const bool aq = (acc >> q) & 1u;
+                const uint64_t mq = __builtin_amdgcn_ballot_w64(aq);
+                if (aq)
+                    s_cid[q][nc[q] + (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(mq >> 32),
+                                                                   __builtin_amdgcn_mbcnt_lo((uint32_t)mq, 0u))] = p;
+                nc[q] += __popcll(mq);
+            }
+            __syncthreads();
+            maybe_flush();
+            // opened leaves: their bodies, for the sub-groups that opened them
+            for (uint64_t m = m_leaf; m; m &= m - 1) {
+                const int l = __builtin_ctzll(m);
+                const uint32_t lid = __shfl(p, l, 64), lst = __shfl(start, l, 64), lme = __shfl(meta, l, 64);
+                add_leaf(lid, lst, lme >> 6, __shfl(O, l, 64));
+            }
+            // opened cells whose children found no room: one cell at a time
+            for (uint64_t m = m_over; m; m &= m - 1) {
+                const int l = __builtin_ctzll(m);
+                const uint32_t p0 = __shfl(p, l, 64), end = __shfl(skip, l, 64);
+                walk_subtree(p0, end > p0 ? end : p0 + 1, __shfl(O, l, 64));
+            }
+            __syncthreads();
+        }
+        for (;;) {
+            bool any = false;
+#pragma unroll
+            for (int q = 0; q < S; ++q) any = any || nc[q] > 0;
+            if (!any) break;
+            flush_cells();
+        }
+        if (nq > 0) flush_bodies();
+        if (valid) {
+            const uint32_t o = a.tperm[i];
+            const double back = lam * lam / mu;
+            a.ax[o] = (double)ax * back;
+            a.ay[o] = (double)ay * back;
+            a.az[o] = (double)az * back;
+            if constexpr (kPot) a.pot[o] = (double)phi * (lam / mu);
+            if (a.counts) a.counts[o] = cnt;
+        }
+    }
+}
+
+// ---------------------------------------------------------------- lane-parallel FP64 walk
+//
+// k_walk_lp32's traversal and persistent XCD-sliced queue over the FP64 walk
+// records: the reference's opening test size^2 / d^2 < theta^2 in FP64 from
+// absolute coordinates (fl(s^2 / x) is non-increasing in x, so every body
+// accepts exactly when the test passes at the least |d|^2), and the
+// reference's term, 1/sqrt(D2) and 1/D2 each correctly rounded.  One list
+// per 64-body wave.
+constexpr int kLp64Cells = 64;
+constexpr int kLp64Bodies = 64;
+
+template <bool kQuad, bool kGuard, bool kPot = false>
+__global__ __launch_bounds__(64) void k_walk_lp64(WalkArgs a) {
+    __shared__ double s_nd[kLp64Cells][4];
+    __shared__ double s_q[kQuad ? kLp64Cells : 1][6];
+    __shared__ double s_bd[kLp64Bodies][4];
+    __shared__ uint32_t s_rstart[kRanges32], s_roff[kRanges32];
+    __shared__ double4 s_tgt[64];
+    __shared__ uint32_t s_cid[kLpCells];
+    __shared__ uint32_t s_stack[kLpStack];
+
+    if (*a.abort_flag) return;
+    const uint32_t lane = threadIdx.x;
+    ConstU64 raw = (ConstU64)a.nodes;
+    const uint4* __restrict__ rec4 = reinterpret_cast<const uint4*>(a.nodes);
+    ConstF qf = (ConstF)a.quadf;
+    (void)qf;
+    const double th2 = a.theta2;
+    const double eps2 = a.eps2;
+
+    for (;;) {
+        const uint32_t grp = lp_next_group(a.wq, a.groups, lane);
+        if (grp >= a.groups) break;
+        const uint4 gr = a.gtab[grp];
+        const uint32_t g0 = gr.x;
+        const uint32_t i = g0 + lane;
+        const bool valid = i < gr.y;
+        const uint64_t vmask = __builtin_amdgcn_ballot_w64(valid);
+        const uint32_t iv = valid ? i : g0;
+        const uint32_t rid = a.sys_root[gr.z];
+        const double x = a.tx[iv], y = a.ty[iv], z = a.tz[iv];
+        double ax = 0, ay = 0, az = 0, phi = 0;
+        uint32_t cnt = 0;
+        int nc = 0, nr = 0, nq = 0;
+
+        __syncthreads();
+        s_tgt[lane] = valid ? make_double4(x, y, z, 0.0) : make_double4(INFINITY, INFINITY, INFINITY, 0.0);
+
+        auto flush_cells = [&](int nn) {
+            __syncthreads();
+            if ((int)lane < nn) {
+                const uint32_t id = s_cid[lane];
+                const WalkNode w = a.nodes[id];
+                s_nd[lane][0] = w.cx;
+                s_nd[lane][1] = w.cy;
+                s_nd[lane][2] = w.cz;
+                s_nd[lane][3] = w.mass;
+                if (kQuad) {
+                    const bool many = w.count > 1;
+#pragma unroll
+                    for (int u = 0; u < 6; ++u) s_q[lane][u] = many ? a.quad[6 * (size_t)id + u] : 0.0;
+                }
+            }
+            __syncthreads();
+            eval_nodes<double, kQuad, kGuard, kPot>(s_nd, s_q, nn, x, y, z, eps2, ax, ay, az, phi);
+            cnt += (uint32_t)nn;
+        };
+        auto flush_64 = [&]() {
+            flush_cells(64);
+            const uint32_t rest = (int)lane + 64 < nc ? s_cid[64 + lane] : 0u;
+            __syncthreads();
+            if ((int)lane + 64 < nc) s_cid[lane] = rest;
+            nc -= 64;
+            __syncthreads();
+        };
+        auto flush_bodies = [&]() {
+            __syncthreads();
+            const int e = (int)lane;
+            if (e < nq) {
+                int lo = 0, hi = nr - 1;
+                while (lo < hi) {
+                    const int mid = (lo + hi + 1) >> 1;
+                    if ((int)s_roff[mid] <= e) lo = mid; else hi = mid - 1;
+                }
+                const uint32_t j = s_rstart[lo] + (uint32_t)(e - (int)s_roff[lo]);
+                s_bd[e][0] = a.tx[j];
+                s_bd[e][1] = a.ty[j];
+                s_bd[e][2] = a.tz[j];
+                s_bd[e][3] = a.tm[j];
+            }
+            __syncthreads();
+            eval_bodies<double, kGuard, kPot>(s_bd, nq, x, y, z, eps2, ax, ay, az, phi);
+            cnt += (uint32_t)nq;
+            nr = 0;
+            nq = 0;
+        };
+        auto add_leaf = [&](uint32_t st, uint32_t count) {
+            for (uint32_t k = 0; k < count; k += 64) {
+                const int m = (int)((count - k) < 64u ? (count - k) : 64u);
+                if (nq + m > kLp64Bodies || nr == kRanges32) flush_bodies();
+                if (lane == 0) {
+                    s_rstart[nr] = st + k;
+                    s_roff[nr] = (uint32_t)nq;
+                }
+                ++nr;
+                nq += m;
+            }
+        };
+        auto add_cell = [&](uint32_t id) {
+            if (lane == 0) s_cid[nc] = id;
+            ++nc;
+            if (nc == 64) flush_64();
+        };
+        auto walk_subtree = [&](uint32_t p0, uint32_t end) {
+            uint32_t q = p0 + 1;
+            while (q < end) {
+                q = __builtin_amdgcn_readfirstlane(q);
+                const WalkNode nd = load_node(raw, q);
+                const uint32_t nskip = nd.skip > q ? nd.skip : q + 1;
+                const double dx = nd.cx - x, dy = nd.cy - y, dz = nd.cz - z;
+                const double d2 = dist2(dx, dy, dz);
+                const bool opened = (__builtin_amdgcn_ballot_w64(!mac_t(nd.size2, d2, th2, 0.f, 0.f)) & vmask) != 0ull;
+                if (!opened) {
+                    add_cell(q);
+                    q = nskip;
+                } else if (nd.level_leaf & 256u) {
+                    add_leaf(nd.start, nd.count);
+                    q = nskip;
+                } else {
+                    q = q + 1;
+                }
+            }
+        };
+
+        int top = 0;
+        {
+            const WalkNode root = load_node(raw, rid);
+            if (root.level_leaf & 256u) {
+                add_leaf(root.start, root.count);
+            } else {
+                const uint32_t cb = __builtin_amdgcn_readfirstlane(a.child_byte[rid]);
+                if (lane < 8u && ((cb >> lane) & 1u))
+                    s_stack[__popc(cb & ((1u << lane) - 1u))] = a.child[8 * (size_t)rid + lane];
+                top = __popc(cb);
+            }
+        }
+        __syncthreads();
+        while (top > 0) {
+            const int k = top < 64 ? top : 64;
+            const int base = top - k;
+            const bool live = (int)lane < k;
+            const uint32_t p = live ? s_stack[base + lane] : rid;
+            top = base;
+            const uint4 r0 = rec4[4 * (size_t)p], r1 = rec4[4 * (size_t)p + 1], r2 = rec4[4 * (size_t)p + 2];
+            const double cx = __hiloint2double((int)r0.y, (int)r0.x), cy = __hiloint2double((int)r0.w, (int)r0.z);
+            const double cz = __hiloint2double((int)r1.y, (int)r1.x);
+            const double size2 = __hiloint2double((int)r2.y, (int)r2.x);
+            const uint32_t skip = r2.z, start = r2.w;
+            const uint4 r3 = rec4[4 * (size_t)p + 3];
+            const uint32_t count = r3.x, ll = r3.y;
+            const uint32_t cb = live ? a.child_byte[p] : 0u;
+            double md2 = INFINITY;
+#pragma unroll 8
+            for (int b = 0; b < 64; ++b) {
+                const double4 t = s_tgt[b];
+                const double dx = cx - t.x, dy = cy - t.y, dz = cz - t.z;
+                md2 = fmin(md2, dist2(dx, dy, dz));
+            }
+            const bool pass = live && mac_t(size2, md2, th2, 0.f, 0.f);
+            const bool leaf = (ll & 256u) != 0u;
+            const bool open_in = live && !pass && !leaf;
+            const uint64_t m_acc = __builtin_amdgcn_ballot_w64(pass);
+            const uint64_t m_leaf = __builtin_amdgcn_ballot_w64(live && !pass && leaf);
+            const uint32_t nch = open_in ? (uint32_t)__popc(cb) : 0u;
+            const uint32_t incl = wave_incl_scan(nch, lane);
+            const uint32_t room = (uint32_t)(kLpStack - top);
+            const uint64_t m_fit = __builtin_amdgcn_ballot_w64(incl <= room);
+            const int nfit = __popcll(m_fit);
+            const bool push = open_in && ((m_fit >> lane) & 1ull);
+            const uint64_t m_over = __builtin_amdgcn_ballot_w64(open_in && !push);
+            const uint32_t pushed = nfit > 0 ? __shfl(incl, nfit - 1, 64) : 0u;
+            __syncthreads();
+            if (push) {
+                const uint4 ca = *reinterpret_cast<const uint4*>(a.child + 8 * (size_t)p);
+                const uint4 cbv = *reinterpret_cast<const uint4*>(a.child + 8 * (size_t)p + 4);
+                const uint32_t ids[8] = {ca.x, ca.y, ca.z, ca.w, cbv.x, cbv.y, cbv.z, cbv.w};
+                uint32_t w = (uint32_t)top + incl - nch;
+#pragma unroll
+                for (int o = 0; o < 8; ++o)
+                    if ((cb >> o) & 1u) s_stack[w++] = ids[o];
+            }
+            top += (int)pushed;
+            if (pass) s_cid[nc + (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(m_acc >> 32),
+                                                              __builtin_amdgcn_mbcnt_lo((uint32_t)m_acc, 0u))] = p;
+            nc += __popcll(m_acc);
+            __syncthreads();
+            if (nc >= 64) flush_64();
+            for (uint64_t m = m_leaf; m; m &= m - 1) {
+                const int l = __builtin_ctzll(m);
+                add_leaf(__shfl(start, l, 64), __shfl(count, l, 64));
+            }
+            for (uint64_t m = m_over; m; m &= m - 1) {
+                const int l = __builtin_ctzll(m);
+                const uint32_t p0 = __shfl(p, l, 64), end = __shfl(skip, l, 64);
+                walk_subtree(p0, end > p0 ? end : p0 + 1);
+            }
+            __syncthreads();
+        }
+        if (nc > 0) {
+            flush_cells(nc);
+            nc = 0;
+        }
+        if (nq > 0) flush_bodies();
+        if (valid) {
+            const uint32_t o = a.tperm[i];
+            a.ax[o] = ax;
+            a.ay[o] = ay;
+            a.az[o] = az;
+            if constexpr (kPot) a.pot[o] = phi;
+            if (a.counts) a.counts[o] = cnt;
+        }
+    }
+}
+
+__global__ void k_fill_wq(uint32_t* __restrict__ wq) {
+    if (threadIdx.x < 8) wq[threadIdx.x] = 0u;
+}
+
 // FP64 direct sum over all bodies (array order) for a list of targets; the
 // same softened monopole term, j != i by index.  pot = -sum m_j / sqrt(r^2+eps^2).
 __global__ __launch_bounds__(kBlock) void k_direct(const double* __restrict__ px, const double* __restrict__ py,
@@ -1105,6 +1783,37 @@ __global__ void k_diag_final(const double* __restrict__ part, int nparts, double
     }
 }
 
+// Workgroups of a persistent walk: as many one-wave workgroups as the chip
+// holds at once (a multiple of 8, one share per XCD), at most one per group.
+template <class K>
+unsigned persistent_grid(K kernel, unsigned groups) {
+    static int per_cu = 0, ncu = 0;
+    if (per_cu == 0) {
+        int dev = 0, pc = 0, nc = 0;
+        if (hipGetDevice(&dev) != hipSuccess ||
+            hipOccupancyMaxActiveBlocksPerMultiprocessor(&pc, kernel, 64, 0) != hipSuccess ||
+            hipDeviceGetAttribute(&nc, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+            pc = nc = 0;
+        ncu = nc > 0 ? nc : 256;
+        per_cu = pc > 0 ? pc : 1;
+    }
+    unsigned g = (unsigned)(per_cu * ncu);
+    g = (g + 7u) & ~7u;
+    const unsigned need = 8u * ((groups + 7u) / 8u);
+    return g < need ? g : need;
+}
+
+#define BHX_LP32S(Q, G, P, S)                                                                                  \
+    hipLaunchKernelGGL((k_walk_lp32<Q, G, P, S>), dim3(persistent_grid(k_walk_lp32<Q, G, P, S>, groups)), dim3(64), \
+                       0, s, a)
+#define BHX_LP32(Q, G, P)                                                 \
+    do {                                                                   \
+        hipLaunchKernelGGL(k_fill_wq, dim3(1), dim3(64), 0, s, a.wq);      \
+        if (sub == 4) BHX_LP32S(Q, G, P, 4);                               \
+        else if (sub == 2) BHX_LP32S(Q, G, P, 2);                          \
+        else BHX_LP32S(Q, G, P, 1);                                        \
+    } while (0)
+
 }  // namespace
 
 hipError_t launch_walk(bhx_ctx* c, hipStream_t s) {
@@ -1140,17 +1849,24 @@ hipError_t launch_walk_to(bhx_ctx* c, hipStream_t s, bool group, int precision, 
     a.tscale = c->tree_tscale;
     a.ax = ox; a.ay = oy; a.az = oz;
     a.counts = counts;
+    a.wq = c->wq;
+    a.child = c->tree.child;
+    a.child_byte = c->tree.child_byte;
+    const int sub = c->walk_sub;
     const unsigned groups = c->ngroups;
     const unsigned grid = (groups + kWaves - 1) / kWaves;
     const bool quad = c->cfg.quadrupole != 0;
     if (group) {
         const bool guard = !(a.eps2 > 0.0);
-#define BHX_GROUP(T, Q, G) hipLaunchKernelGGL((k_walk_group<T, Q, G>), dim3(groups), dim3(64), 0, s, a)
+#define BHX_GROUP(T, Q, G)                                                                                   \
+    do {                                                                                                      \
+        hipLaunchKernelGGL(k_fill_wq, dim3(1), dim3(64), 0, s, a.wq);                                         \
+        hipLaunchKernelGGL((k_walk_lp64<Q, G>), dim3(persistent_grid(k_walk_lp64<Q, G>, groups)), dim3(64), 0, s, \
+                           a);                                                                                \
+    } while (0)
         if (precision == BHX_FP32) {
-#define BHX_GROUP32(Q, G) hipLaunchKernelGGL((k_walk_group32<Q, G>), dim3(8 * ((groups + 7) / 8)), dim3(64), 0, s, a)
-            if (quad) { if (guard) BHX_GROUP32(true, true); else BHX_GROUP32(true, false); }
-            else { if (guard) BHX_GROUP32(false, true); else BHX_GROUP32(false, false); }
-#undef BHX_GROUP32
+            if (quad) { if (guard) BHX_LP32(true, true, false); else BHX_LP32(true, false, false); }
+            else { if (guard) BHX_LP32(false, true, false); else BHX_LP32(false, false, false); }
         } else {
             if (quad) { if (guard) BHX_GROUP(double, true, true); else BHX_GROUP(double, true, false); }
             else { if (guard) BHX_GROUP(double, false, true); else BHX_GROUP(double, false, false); }
@@ -1194,16 +1910,22 @@ hipError_t launch_walk_potential(bhx_ctx* c, hipStream_t s, double* ox, double* 
     a.tscale = c->tree_tscale;
     a.ax = ox; a.ay = oy; a.az = oz;
     a.pot = pot;
+    a.wq = c->wq;
+    a.child = c->tree.child;
+    a.child_byte = c->tree.child_byte;
+    const int sub = c->walk_sub;
     const unsigned groups = c->ngroups;
     const bool quad = c->cfg.quadrupole != 0, guard = !(a.eps2 > 0.0);
     if (c->cfg.precision == BHX_FP32) {
-#define BHX_POT32(Q, G) \
-    hipLaunchKernelGGL((k_walk_group32<Q, G, false, true>), dim3(8 * ((groups + 7) / 8)), dim3(64), 0, s, a)
-        if (quad) { if (guard) BHX_POT32(true, true); else BHX_POT32(true, false); }
-        else { if (guard) BHX_POT32(false, true); else BHX_POT32(false, false); }
-#undef BHX_POT32
+        if (quad) { if (guard) BHX_LP32(true, true, true); else BHX_LP32(true, false, true); }
+        else { if (guard) BHX_LP32(false, true, true); else BHX_LP32(false, false, true); }
     } else {
-#define BHX_POT64(Q, G) hipLaunchKernelGGL((k_walk_group<double, Q, G, false, true>), dim3(groups), dim3(64), 0, s, a)
+#define BHX_POT64(Q, G)                                                                                        \
+    do {                                                                                                        \
+        hipLaunchKernelGGL(k_fill_wq, dim3(1), dim3(64), 0, s, a.wq);                                           \
+        hipLaunchKernelGGL((k_walk_lp64<Q, G, true>), dim3(persistent_grid(k_walk_lp64<Q, G, true>, groups)),   \
+                           dim3(64), 0, s, a);                                                                  \
+    } while (0)
         if (quad) { if (guard) BHX_POT64(true, true); else BHX_POT64(true, false); }
         else { if (guard) BHX_POT64(false, true); else BHX_POT64(false, false); }
 #undef BHX_POT64

@@ -80,6 +80,9 @@ def test_plummer_1m_fp32_50_steps(bhx):
         err = np.linalg.norm(acc32[tg] - dacc, axis=1) / np.linalg.norm(dacc, axis=1)
         print("force error median %.3e p99 %.3e" % (np.median(err), np.quantile(err, 0.99)), flush=True)
         assert np.median(err) < 1e-3 and np.quantile(err, 0.99) < 5e-3
+        # precision alone: the FP32 walk with the FP64 walk's 64-body lists
+        e.set_config(group_size=64)
+        acc32 = e.walk()
         e.set_config(precision=0)
         acc64 = e.walk()
         d = np.linalg.norm(acc32 - acc64, axis=1) / np.linalg.norm(acc64, axis=1)
