@@ -168,8 +168,9 @@ def galaxy_bench(bhx, n=4194304, steps=20, warmup=3, seed=2026):
 
 def ensemble_bench(bhx, nsys, steps, warmup, seed=1):
     """Batched ensemble (all systems in one context, one graph launch per frame) against the
-    same systems stepped one after another (one system at a time in one context, eager frames).
-    Both start from the same state and run the same frames; final states must agree bitwise."""
+    same systems stepped one after another in one context: with graph frames (like for like:
+    the same frames, one system at a time) and with eager frames.  All start from the same
+    state and run the same frames; final states must agree bitwise."""
     t0 = time.perf_counter()
     pos, vel, mass, off = bhx.ensemble(nsys, seed=seed)
     n = pos.shape[0]
@@ -182,12 +183,13 @@ def ensemble_bench(bhx, nsys, steps, warmup, seed=1):
     with bhx.Engine(n, **cfg) as e:
         e.set_bodies(pos, vel, mass)
         e.set_systems(off)
-        chosen, table = e.autotune(caps=(16, 32, 64), steps=2, max_err=1e-3)
-        res["autotune"] = {"chosen_leaf_capacity": chosen,
-                           "candidates": [{"leaf_capacity": c, "ms_per_step": ms, "median_err": er}
-                                          for c, ms, er in table]}
+        chosen, gchosen, table = e.autotune(caps=(16, 32, 64), group_sizes=(64, 32), steps=2, max_err=1e-3)
+        res["autotune"] = {"chosen_leaf_capacity": chosen, "chosen_group_size": gchosen,
+                           "candidates": [{"leaf_capacity": c, "group_size": g, "ms_per_step": ms, "median_err": er}
+                                          for c, g, ms, er in table]}
         cfg["leaf_capacity"] = chosen
-        log("ensemble autotune: leaf capacity %d %s" % (chosen, table))
+        cfg["group_size"] = gchosen
+        log("ensemble autotune: leaf capacity %d, group size %d %s" % (chosen, gchosen, table))
         e.step_graph(warmup)
         e.sync()
         t0 = time.perf_counter()
@@ -196,25 +198,36 @@ def ensemble_bench(bhx, nsys, steps, warmup, seed=1):
         tb = time.perf_counter() - t0
         P, V, M, A = e.get_bodies()
     log("ensemble batched: %.2f ms/step" % (1e3 * tb / steps))
-    tl = 0.0
-    same = True
+    looped = {}
     with bhx.Engine(int(sizes.max()), **cfg) as s:
-        for k in range(nsys):
-            a, b = int(off[k]), int(off[k + 1])
-            s.set_bodies(pos[a:b], vel[a:b], mass[a:b])
-            s.step(warmup)
-            s.sync()
-            t0 = time.perf_counter()
-            s.step(steps)
-            s.sync()
-            tl += time.perf_counter() - t0
-            p1, v1, m1, a1 = s.get_bodies()
-            same = same and np.array_equal(P[a:b], p1) and np.array_equal(V[a:b], v1) and np.array_equal(A[a:b], a1)
-    log("ensemble looped: %.2f ms/step, bitwise equal: %s" % (1e3 * tl / steps, same))
-    res.update({"batched_ms_per_step": 1e3 * tb / steps, "looped_ms_per_step": 1e3 * tl / steps,
-                "batched_body_updates_per_s": n * steps / tb, "looped_body_updates_per_s": n * steps / tl,
-                "batched_over_looped": tl / tb, "bitwise_equal_to_looped": bool(same),
-                "looped": "each system alone in one context, bhx_step per system (eager frames)",
+        for mode in ("graph", "eager"):
+            tl = 0.0
+            same = True
+            for k in range(nsys):
+                a, b = int(off[k]), int(off[k + 1])
+                s.set_bodies(pos[a:b], vel[a:b], mass[a:b])
+                run = s.step_graph if mode == "graph" else s.step
+                run(warmup)                  # graph frames: captured here, replayed below
+                s.sync()
+                t0 = time.perf_counter()
+                run(steps)
+                s.sync()
+                tl += time.perf_counter() - t0
+                p1, v1, m1, a1 = s.get_bodies()
+                same = same and np.array_equal(P[a:b], p1) and np.array_equal(V[a:b], v1) and \
+                    np.array_equal(A[a:b], a1)
+            looped[mode] = (tl, same)
+            log("ensemble looped (%s frames): %.2f ms/step, bitwise equal: %s" % (mode, 1e3 * tl / steps, same))
+    tg, same_g = looped["graph"]
+    te, same_e = looped["eager"]
+    res.update({"batched_ms_per_step": 1e3 * tb / steps, "looped_ms_per_step": 1e3 * tg / steps,
+                "looped_eager_ms_per_step": 1e3 * te / steps,
+                "batched_body_updates_per_s": n * steps / tb, "looped_body_updates_per_s": n * steps / tg,
+                "batched_over_looped": tg / tb, "batched_over_looped_eager": te / tb,
+                "bitwise_equal_to_looped": bool(same_g and same_e),
+                "looped": "each system alone in one context, bhx_step_graph per system (graphs captured in the "
+                          "warm-up frames; one host wait per system)",
+                "looped_eager": "each system alone in one context, bhx_step per system (eager frames)",
                 "batched": "all systems in one context (bhx_set_systems), bhx_step_graph"})
     return res
 
@@ -263,50 +276,85 @@ def adjoint_bench(bhx, n=65536, steps=1000, every=100, seed=5):
     return res
 
 
-def cpu_baseline(pos, vel, mass, theta, eps, dt, cap, nsub, seed):
-    """Serial oracle (oracle/liboracle.so, one core) on a bounded sample of the workload:
-    every O(N log N) phase over all N bodies, the walk for `nsub` seeded bodies,
-    extrapolated to N walk evaluations."""
+def _oracle_cflags():
+    """The oracle's compile flags (oracle/Makefile)."""
+    try:
+        for line in open(os.path.join(ROOT, "oracle", "Makefile")):
+            if line.startswith("CFLAGS"):
+                return line.split("=", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(pos, vel, mass, theta, eps, dt, cap):
+    """Serial oracle (oracle/liboracle.so) for one full step of the workload on ONE core
+    of this host (the process is pinned to a single CPU for the duration): bounds, keys,
+    sort, kick/drift, bucket-32 build with intended moments, the quadrupole walk for every
+    body (no sample, no extrapolation) and the closing kick."""
     from oracle import oracle as orc
     n = pos.shape[0]
-    t = {}
-    t0 = time.perf_counter()
-    c, s = orc.bounds(pos)
-    t["bounds"] = time.perf_counter() - t0
-    t0 = time.perf_counter()
-    keys = orc.morton_keys(pos, s)
-    t["keys"] = time.perf_counter() - t0
-    t0 = time.perf_counter()
-    _, idx = orc.radix_sort(keys)
-    idx = idx.astype(np.int64)
-    p, v, m = pos[idx], vel[idx], mass[idx]
-    t["sort"] = time.perf_counter() - t0
-    t0 = time.perf_counter()
-    p, v = orc.kick_drift(dt, p, v, np.zeros_like(p))
-    t["kick_drift"] = time.perf_counter() - t0
-    t0 = time.perf_counter()
-    tree = orc.build_recursive(p, m, c, s, cap)
-    tree.moments_intended()
-    t["build"] = time.perf_counter() - t0
-    rng = np.random.default_rng(seed)
-    tg = np.sort(rng.choice(n, size=min(nsub, n), replace=False))
-    t0 = time.perf_counter()
-    acc, cnt = tree.walk(theta, eps, quad=True, targets=tg)
-    t_walk_sub = time.perf_counter() - t0
-    t["walk"] = t_walk_sub * n / tg.size
-    t0 = time.perf_counter()
-    orc.kick(dt, v, np.zeros_like(v))
-    t["kick"] = time.perf_counter() - t0
+    allowed = sorted(os.sched_getaffinity(0))
+    core = allowed[0]
+    os.sched_setaffinity(0, {core})
+    try:
+        t = {}
+        t0 = time.perf_counter()
+        c, s = orc.bounds(pos)
+        t["bounds"] = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        keys = orc.morton_keys(pos, s)
+        t["keys"] = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        _, idx = orc.radix_sort(keys)
+        idx = idx.astype(np.int64)
+        p, v, m = pos[idx], vel[idx], mass[idx]
+        t["sort"] = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        p, v = orc.kick_drift(dt, p, v, np.zeros_like(p))
+        t["kick_drift"] = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        tree = orc.build_recursive(p, m, c, s, cap)
+        tree.moments_intended()
+        t["build"] = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        acc, cnt = tree.walk(theta, eps, quad=True)
+        t["walk"] = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        orc.kick(dt, v, acc)
+        t["kick"] = time.perf_counter() - t0
+    finally:
+        os.sched_setaffinity(0, set(allowed))
     step = sum(t.values())
     return {
         "value": n / step, "unit": "body-updates/s", "cores": 1, "kind": "port",
-        "sample": ("serial oracle, FP64: bounds, keys, sort, kick/drift, bucket-32 build + moments and kick "
-                   "over all %d bodies; walk for %d seeded bodies (%.1f s) extrapolated x%.0f" %
-                   (n, tg.size, t_walk_sub, n / tg.size)),
+        "sample": "one full step of all %d bodies (no sampling): serial oracle, FP64, the walk with the "
+                  "reference's per-body lists and quadrupoles" % n,
+        "step_s": round(step, 3),
         "phase_s": {k: round(v_, 4) for k, v_ in t.items()},
-        "cpu": _cpu_model(),
+        "cpu": _cpu_model(), "pinned_core": core, "nproc": os.cpu_count(), "affinity_cpus": len(allowed),
+        "compiler_flags": "gcc " + _oracle_cflags(),
         "mean_interactions": float(np.mean(cnt)),
     }
+
+
+def c2_error_series(bhx, pos, vel, mass, cfg, steps=50, nsub=4096, seed=4096):
+    """Force error of the FP32 walk after each of `steps` frames of the precision workload,
+    against the FP64 direct sum on a seeded sample of `nsub` array positions."""
+    n = pos.shape[0]
+    tg = np.sort(np.random.default_rng(seed).choice(n, size=nsub, replace=False))
+    series = []
+    with bhx.Engine(n, **cfg) as e:
+        e.set_bodies(pos, vel, mass)
+        for _ in range(steps):
+            e.step_graph(1)
+            ref, _ = e.direct(tg, cfg["eps"])
+            acc = e.get_bodies()[3][tg]
+            err = np.linalg.norm(acc - ref, axis=1) / np.linalg.norm(ref, axis=1)
+            series.append([float(np.median(err)), float(np.percentile(err, 90)), float(np.percentile(err, 99))])
+    a = np.array(series)
+    return {"steps": steps, "subsample": nsub, "columns": ["median", "p90", "p99"], "per_step": series,
+            "max_median": float(a[:, 0].max()), "max_p99": float(a[:, 2].max())}
 
 
 def copy_calibration(mib):
@@ -364,7 +412,7 @@ def main():
     ap.add_argument("--n", type=int, default=1 << 20)
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-walk-sample", type=int, default=1 << 19)
+    ap.add_argument("--no-error-series", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-ensemble", action="store_true")
     ap.add_argument("--ensemble-systems", type=int, default=512)
@@ -436,7 +484,7 @@ def main():
                    "n": n, "theta": theta, "eps": eps, "leaf_capacity": cap, "dt": dt},
         "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / PEAK_FP32_TFLOPS, "traffic": read_traffic(n),
-                     "kernel": "k_walk_group32<true, false>", "kernel_ms": walk_ms,
+                     "kernel": "k_walk_lp32<true, false, false, 1>", "kernel_ms": walk_ms,
                      "work": "%d reference-list interactions x %d flops (group lists evaluate %d)" % (
                          out["fp32"]["inter_ref"], FLOPS_PER_INTERACTION, out["fp32"]["inter"])},
         "force_error_fp32": out["fp32"]["err"],
@@ -452,6 +500,9 @@ def main():
                              "achieved_tflops": ach64, "frac_fp64_peak": ach64 / PEAK_FP64_TFLOPS,
                              "phases_ms": {k: round(v, 4) for k, v in f64["phases"].items()}}
         line["fp32_over_fp64_walk"] = f64["walk_ms"] / walk_ms
+    if not args.no_error_series:
+        log("force-error series over 50 steps")
+        line["force_error_series_fp32"] = c2_error_series(bhx, pos, vel, mass, cfg)
     if not args.no_ensemble:
         line["ensemble"] = ensemble_bench(bhx, args.ensemble_systems, 10, 2)
     if not args.no_galaxy:
@@ -463,8 +514,8 @@ def main():
         log("energy drift (N=16384, 100 steps)")
         line["energy_drift"] = energy_drift(bhx)
     if not args.no_cpu_baseline:
-        log("cpu baseline (serial oracle, bounded sample)")
-        line["cpu_baseline"] = cpu_baseline(pos, vel, mass, theta, eps, dt, cap, args.cpu_walk_sample, 99)
+        log("cpu baseline (serial oracle, one full step, one core)")
+        line["cpu_baseline"] = cpu_baseline(pos, vel, mass, theta, eps, dt, cap)
     print(json.dumps(line), flush=True)
 
 

@@ -319,20 +319,24 @@ int bhx_lookup_in(bhx_ctx *ctx, size_t system, const uint64_t *keys, size_t n, i
 /* One candidate of bhx_autotune. */
 typedef struct bhx_tune_entry {
     int32_t leaf_capacity;
-    int32_t pad;
+    int32_t group_size;     /* bodies per interaction list of the FP32 group walk */
     double ms_per_step;     /* wall time per bhx_step_graph frame */
     double median_err;      /* median |a - a*| / |a*| on the sample, a* = FP64 direct sum */
 } bhx_tune_entry;
 
-/* Picks the leaf capacity (bodies per leaf bucket, which sets how much of
- * each walk group's list is direct body terms): from the current state,
- * for each of the ncaps candidates runs `steps` graph frames, times them and
- * measures the median force error of the last frame on `nsample` seeded
- * bodies; keeps the fastest candidate whose median error is <= max_err (the
- * configured capacity if none is).  The bodies are left in their starting
- * state; out (may be NULL) receives one entry per candidate. */
-int bhx_autotune(bhx_ctx *ctx, const int32_t *caps, size_t ncaps, int steps, double max_err, size_t nsample,
-                 uint64_t seed, bhx_tune_entry *out, int32_t *chosen);
+/* Picks the walk tiling: the leaf capacity (bodies per leaf bucket, which
+ * sets how much of each list is direct body terms) and the group size
+ * (bodies sharing one interaction list: 64, 32 or 16, see bhx_config).  From
+ * the current state, for each of the ncaps x ngroups candidates runs `steps`
+ * graph frames, times them and measures the median force error of the last
+ * frame on `nsample` seeded bodies; keeps the fastest candidate whose median
+ * error is <= max_err (the configured pair if none is).  group_sizes NULL
+ * (or ngroups 0): the configured group size only.  The bodies are left in
+ * their starting state; out (may be NULL) receives one entry per candidate
+ * (capacity-major); chosen_group may be NULL. */
+int bhx_autotune(bhx_ctx *ctx, const int32_t *caps, size_t ncaps, const int32_t *group_sizes, size_t ngroups,
+                 int steps, double max_err, size_t nsample, uint64_t seed, bhx_tune_entry *out, int32_t *chosen,
+                 int32_t *chosen_group);
 
 /* ---- differentiation (no reference counterpart) -------------------------- */
 

@@ -27,7 +27,8 @@ NODE_DTYPE = np.dtype([
     ("skip", "<i8"), ("body_start", "<i8"),
 ])
 assert NODE_DTYPE.itemsize == 160
-TUNE_DTYPE = np.dtype([("leaf_capacity", "<i4"), ("pad", "<i4"), ("ms_per_step", "<f8"), ("median_err", "<f8")])
+TUNE_DTYPE = np.dtype([("leaf_capacity", "<i4"), ("group_size", "<i4"), ("ms_per_step", "<f8"),
+                       ("median_err", "<f8")])
 
 
 class BhxError(RuntimeError):
@@ -76,7 +77,7 @@ def lib():
             "bhx_ic_ensemble": [S, ctypes.c_uint64, P, P, P, I, P, P, P],
             "bhx_set_systems": [P, P, S], "bhx_num_systems": [P, P], "bhx_system_bounds": [P, P, P],
             "bhx_build_systems": [P, P, P, P], "bhx_step_graph": [P, I],
-            "bhx_autotune": [P, P, S, I, D, S, ctypes.c_uint64, P, P],
+            "bhx_autotune": [P, P, S, P, S, I, D, S, ctypes.c_uint64, P, P, P],
             "bhx_record": [P, I, I, I], "bhx_recorded_bodies": [P, P, P, P, P],
             "bhx_adjoint": [P, P, P, P, P, P, P, P], "bhx_grad_timing": [P, P, P], "bhx_system_roots": [P, P], "bhx_lookup_in": [P, S, P, S, P],
             "bhx_get_systems": [P, P], "bhx_energy": [P, P, P, P, P],
@@ -452,16 +453,27 @@ class Engine:
     def step_graph(self, nsteps=1):
         self._chk(lib().bhx_step_graph(self.h, nsteps), "bhx_step_graph")
 
-    def autotune(self, caps=(8, 16, 24, 32, 48, 64), steps=3, max_err=1e-3, nsample=4096, seed=7):
-        """Leaf capacity with the fastest graph frames at median force error <= max_err (set on
-        the context; bodies left as they were).  Returns (chosen, [(cap, ms_per_step, median_err)])."""
+    def autotune(self, caps=(8, 16, 24, 32, 48, 64), steps=3, max_err=1e-3, nsample=4096, seed=7, group_sizes=None):
+        """Walk tiling with the fastest graph frames at median force error <= max_err: leaf
+        capacity and (with group_sizes) bodies per interaction list, set on the context
+        (bodies left as they were).  Returns (chosen_capacity, [(cap, ms_per_step,
+        median_err)]); with group_sizes, (chosen_capacity, chosen_group_size,
+        [(cap, group_size, ms_per_step, median_err)])."""
         caps = np.ascontiguousarray(caps, dtype=np.int32)
-        out = np.zeros(caps.size, TUNE_DTYPE)
-        ch = ctypes.c_int32()
-        self._chk(lib().bhx_autotune(self.h, _p(caps), caps.size, steps, max_err, nsample, seed, _p(out),
-                                     ctypes.byref(ch)), "bhx_autotune")
+        gs = None if group_sizes is None else np.ascontiguousarray(group_sizes, dtype=np.int32)
+        ng = 1 if gs is None else gs.size
+        out = np.zeros(caps.size * ng, TUNE_DTYPE)
+        ch, chg = ctypes.c_int32(), ctypes.c_int32()
+        self._chk(lib().bhx_autotune(self.h, _p(caps), caps.size, _p(gs), 0 if gs is None else gs.size, steps,
+                                     max_err, nsample, seed, _p(out), ctypes.byref(ch), ctypes.byref(chg)),
+                  "bhx_autotune")
         self.cfg.leaf_capacity = ch.value
-        return ch.value, [(int(r["leaf_capacity"]), float(r["ms_per_step"]), float(r["median_err"])) for r in out]
+        self.cfg.group_size = chg.value
+        if gs is None:
+            return ch.value, [(int(r["leaf_capacity"]), float(r["ms_per_step"]), float(r["median_err"]))
+                              for r in out]
+        return ch.value, chg.value, [(int(r["leaf_capacity"]), int(r["group_size"]), float(r["ms_per_step"]),
+                                      float(r["median_err"])) for r in out]
 
     # --- differentiation
     def record(self, nsteps, checkpoint_every=100, init_acc=True):

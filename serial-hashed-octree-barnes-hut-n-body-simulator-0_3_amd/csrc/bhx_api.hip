@@ -1284,12 +1284,21 @@ int bhx_step_graph(bhx_ctx* c, int nsteps) {
     return BHX_OK;
 }
 
-int bhx_autotune(bhx_ctx* c, const int32_t* caps, size_t ncaps, int steps, double max_err, size_t nsample,
-                 uint64_t seed, bhx_tune_entry* out, int32_t* chosen) {
+int bhx_autotune(bhx_ctx* c, const int32_t* caps, size_t ncaps, const int32_t* group_sizes, size_t ngroups,
+                 int steps, double max_err, size_t nsample, uint64_t seed, bhx_tune_entry* out, int32_t* chosen,
+                 int32_t* chosen_group) {
     if (!c || !caps) return c ? fail(c, BHX_E_NULL, "caps is a null pointer") : BHX_E_NULL;
     if (ncaps == 0 || steps < 1 || nsample == 0) return fail(c, BHX_E_ARG, "empty tuning set");
     for (size_t k = 0; k < ncaps; ++k)
         if (caps[k] < 1 || caps[k] > kMaxLeafCap) return fail(c, BHX_E_ARG, "leaf capacity out of range");
+    const int32_t own_group = c->cfg.group_size;
+    if (!group_sizes || ngroups == 0) {
+        group_sizes = &own_group;
+        ngroups = 1;
+    }
+    for (size_t g = 0; g < ngroups; ++g)
+        if (group_sizes[g] != 0 && group_sizes[g] != 16 && group_sizes[g] != 32 && group_sizes[g] != 64)
+            return fail(c, BHX_E_ARG, "group size must be 16, 32 or 64");
     if (c->n == 0) return fail(c, BHX_E_STATE, "no bodies");
     HIPCHK(hipSetDevice(c->device));
     hipStream_t s = c->s_main;
@@ -1324,16 +1333,19 @@ int bhx_autotune(bhx_ctx* c, const int32_t* caps, size_t ncaps, int steps, doubl
     HIPCHK(dalloc(&dwalk, 3 * ns));
     HIPCHK(hipMemcpyAsync(dt, tg.data(), ns * sizeof(int64_t), hipMemcpyHostToDevice, s));
     const bhx_config base = c->cfg;
-    int32_t best = base.leaf_capacity;
+    int32_t best = base.leaf_capacity, best_group = base.group_size;
     double best_ms = 0.0;
     bool found = false;
     std::vector<double> ref(3 * ns), walk(3 * ns), err(ns);
-    for (size_t k = 0; k < ncaps && rc == BHX_OK; ++k) {
+    for (size_t kk = 0; kk < ncaps * ngroups && rc == BHX_OK; ++kk) {
+        const size_t k = kk / ngroups, g = kk % ngroups;
         bhx_config cfg = base;
         cfg.leaf_capacity = caps[k];
+        cfg.group_size = group_sizes[g];
         rc = restore_snapshot(c, c->snaps[1], s);
         if (rc != BHX_OK) break;
         c->cfg = cfg;
+        c->walk_sub = walk_lists_for(&cfg);
         drop_graphs(c);
         rc = bhx_step_graph(c, 1);                     // capture + first frame
         if (rc != BHX_OK) break;
@@ -1362,29 +1374,33 @@ int bhx_autotune(bhx_ctx* c, const int32_t* caps, size_t ncaps, int steps, doubl
         std::nth_element(err.begin(), err.begin() + ns / 2, err.end());
         const double med = err[ns / 2];
         if (out) {
-            out[k].leaf_capacity = caps[k];
-            out[k].pad = 0;
-            out[k].ms_per_step = ms;
-            out[k].median_err = med;
+            out[kk].leaf_capacity = caps[k];
+            out[kk].group_size = group_sizes[g];
+            out[kk].ms_per_step = ms;
+            out[kk].median_err = med;
         }
         if (med <= max_err && (!found || ms < best_ms)) {
             found = true;
             best = caps[k];
+            best_group = group_sizes[g];
             best_ms = ms;
         }
     }
     (void)hipFree(dt);
     (void)hipFree(dref);
     (void)hipFree(dwalk);
-    // back to the starting state, with the chosen capacity
+    // back to the starting state, with the chosen capacity and group size
     c->cfg = base;
     c->cfg.leaf_capacity = best;
+    c->cfg.group_size = best_group;
+    c->walk_sub = walk_lists_for(&c->cfg);
     drop_graphs(c);
     const int rc2 = restore_snapshot(c, c->snaps[1], s);
     HIPCHK(hipStreamSynchronize(s));
     if (rc != BHX_OK) return rc;
     if (rc2 != BHX_OK) return rc2;
     if (chosen) *chosen = best;
+    if (chosen_group) *chosen_group = best_group;
     return BHX_OK;
 }
 

@@ -193,6 +193,12 @@ def test_autotune_leaves_state_and_sets_capacity(bhx):
         # nothing is allowed: the configured capacity stays
         c2, _ = e.autotune(caps=(8, 32), steps=1, max_err=0.0, nsample=256)
         assert c2 == chosen
+        # the group size too: every (capacity, group size) pair is measured
+        c3, g3, table = e.autotune(caps=(chosen,), group_sizes=(64, 32, 16), steps=1, max_err=1e-2, nsample=256)
+        assert c3 == chosen and g3 in (64, 32, 16) and [t[1] for t in table] == [64, 32, 16]
+        assert all(t[2] > 0 and 0 < t[3] < 1e-2 for t in table)
+        e.set_config(group_size=0)
+        assert all(np.array_equal(x, y) for x, y in zip(before, e.get_bodies())), "state restored"
         e.step(2)
         P = e.get_bodies()
     with bhx.Engine(pos.shape[0], **dict(cfg, leaf_capacity=chosen)) as r:
