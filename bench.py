@@ -232,19 +232,21 @@ def ensemble_bench(bhx, nsys, steps, warmup, seed=1):
     return res
 
 
-def adjoint_bench(bhx, n=65536, steps=1000, every=100, seed=5):
+def adjoint_bench(bhx, n=65536, steps=1000, every=100, seed=5, quadrupole=0):
     """Forward + reverse pass over a recorded trajectory against the forward alone.
     Loss: mean |x_T - x_T*|^2, the target trajectory from initial conditions perturbed by
-    1e-3 (positions, velocities) and 1% (masses).  FP64 state, FP32 walk, monopole."""
+    1e-3 (positions, velocities) and 1% (masses).  FP64 state, FP32 walk, monopole or
+    quadrupole cells."""
     pos, vel, mass = bhx.plummer(n, seed=seed)
     rng = np.random.default_rng(seed + 1)
     pos_t = pos + 1e-3 * rng.normal(size=pos.shape)
     vel_t = vel + 1e-3 * rng.normal(size=vel.shape)
     mass_t = mass * (1.0 + 0.01 * rng.normal(size=mass.shape))
     cfg = dict(theta=0.5, eps=0.01, dt=1e-3, leaf_capacity=32, mode=bhx.INTENDED, precision=bhx.FP32,
-               quadrupole=0, compensated=1, group_walk=1)
+               quadrupole=quadrupole, compensated=1, group_walk=1)
     res = {"n": n, "steps": steps, "checkpoint_every": every, "theta": 0.5, "eps": 0.01, "dt": 1e-3,
-           "walk": "FP32 group walk, monopole", "loss": "mean |x_T - x_T*|^2"}
+           "walk": "FP32 group walk, " + ("quadrupole" if quadrupole else "monopole"),
+           "loss": "mean |x_T - x_T*|^2"}
     with bhx.Engine(n, **cfg) as e:
         e.set_bodies(pos_t, vel_t, mass_t)
         e.record(steps, every)
@@ -267,7 +269,8 @@ def adjoint_bench(bhx, n=65536, steps=1000, every=100, seed=5):
         t0 = time.perf_counter()
         gx0, gv0, _, gm = e.adjoint(gx)
         t_adj = time.perf_counter() - t0
-    log("adjoint: forward %.3f s, recorded forward %.3f s, reverse %.3f s" % (t_fwd, t_rec, t_adj))
+    log("adjoint (quadrupole=%d): forward %.3f s, recorded forward %.3f s, reverse %.3f s"
+        % (quadrupole, t_fwd, t_rec, t_adj))
     res.update({"forward_s": t_fwd, "recorded_forward_s": t_rec, "adjoint_s": t_adj, "loss_value": loss,
                 "grad_norm_x0": float(np.linalg.norm(gx0)), "grad_norm_m": float(np.linalg.norm(gm)),
                 "fwd_adjoint_over_fwd": (t_rec + t_adj) / t_fwd,
@@ -510,6 +513,7 @@ def main():
     if not args.no_adjoint:
         line["adjoint"] = adjoint_bench(bhx)
         line["fwd_adjoint_over_fwd"] = line["adjoint"]["fwd_adjoint_over_fwd"]
+        line["adjoint_quadrupole"] = adjoint_bench(bhx, quadrupole=1)
     if not args.no_drift:
         log("energy drift (N=16384, 100 steps)")
         line["energy_drift"] = energy_drift(bhx)

@@ -345,12 +345,24 @@ int bhx_autotune(bhx_ctx *ctx, const int32_t *caps, size_t ncaps, const int32_t 
  * checkpoint_every frames.  init_acc != 0 first sets the accelerations to the
  * walk of the starting positions (a_0 = A(x_0)); otherwise the resident
  * accelerations are the frames' a_0.  Needs group_walk = 1, the intended
- * payload, quadrupole = 0 and permute_acc = 1. */
+ * payload, permute_acc = 1 and one list per 64-body group (group_size 0 or
+ * 64); monopole or quadrupole cells, FP64 or FP32 walk.  Changing the
+ * configuration drops the recording. */
 int bhx_record(bhx_ctx *ctx, int nsteps, int checkpoint_every, int init_acc);
 
 /* State after the recorded frames, in the body order at the start of the
  * recording (any pointer may be NULL). */
 int bhx_recorded_bodies(bhx_ctx *ctx, double *pos, double *vel, double *mass, double *acc);
+
+/* Decision digest of the last recording (FP64 walk only): a wrapping sum of
+ * 64-bit hashes over every recorded tree (each body's starting identity,
+ * tree slot, leaf node and leaf key) and every group's interaction list
+ * (each accepted cell and opened leaf with its group), each mixed with its
+ * frame index, the a_0 walk included.  Two recordings whose digests are
+ * equal walked the same trees and lists in every frame (up to a 64-bit hash
+ * collision): the same frozen decisions, so finite differences between them
+ * see the derivative bhx_adjoint computes. */
+int bhx_record_digest(bhx_ctx *ctx, uint64_t *digest);
 
 /* Reverse pass of the recorded frames: from dL/dx_T, dL/dv_T, dL/da_T (each
  * 3 per body in the starting order, NULL = 0) to dL/dx_0, dL/dv_0, dL/da_0

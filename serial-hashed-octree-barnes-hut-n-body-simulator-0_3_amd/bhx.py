@@ -78,7 +78,7 @@ def lib():
             "bhx_set_systems": [P, P, S], "bhx_num_systems": [P, P], "bhx_system_bounds": [P, P, P],
             "bhx_build_systems": [P, P, P, P], "bhx_step_graph": [P, I],
             "bhx_autotune": [P, P, S, P, S, I, D, S, ctypes.c_uint64, P, P, P],
-            "bhx_record": [P, I, I, I], "bhx_recorded_bodies": [P, P, P, P, P],
+            "bhx_record": [P, I, I, I], "bhx_recorded_bodies": [P, P, P, P, P], "bhx_record_digest": [P, P],
             "bhx_adjoint": [P, P, P, P, P, P, P, P], "bhx_grad_timing": [P, P, P], "bhx_system_roots": [P, P], "bhx_lookup_in": [P, S, P, S, P],
             "bhx_get_systems": [P, P], "bhx_energy": [P, P, P, P, P],
             "bhx_galaxy_defaults": [P], "bhx_ic_galaxy_pair": [S, ctypes.c_uint64, P, I, P, P, P],
@@ -487,6 +487,12 @@ class Engine:
         pos, vel, acc, mass = np.empty((n, 3)), np.empty((n, 3)), np.empty((n, 3)), np.empty(n)
         self._chk(lib().bhx_recorded_bodies(self.h, _p(pos), _p(vel), _p(mass), _p(acc)), "bhx_recorded_bodies")
         return pos, vel, mass, acc
+
+    def record_digest(self):
+        """Decision digest of the last recording (FP64 walk): equal digests, equal trees and lists."""
+        d = ctypes.c_uint64()
+        self._chk(lib().bhx_record_digest(self.h, ctypes.byref(d)), "bhx_record_digest")
+        return d.value
 
     def adjoint(self, gx=None, gv=None, ga=None):
         """dL/d(x0, v0, a0, m) from dL/d(x_T, v_T, a_T), all in the starting body order."""

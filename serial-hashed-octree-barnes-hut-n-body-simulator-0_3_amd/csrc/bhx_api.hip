@@ -124,7 +124,7 @@ void free_all(bhx_ctx* c) {
     for (double*& p : c->alt)
         if (p) { (void)hipFree(p); p = nullptr; }
     void* vp[] = {c->keys, c->keys_alt, c->vals, c->vals_alt, c->mkeys, c->tkeys, c->tperm, c->nn, c->noff,
-                  c->nlo, c->scan_tmp, c->sort_hist, c->sort_status, c->sort_tile_ctr, c->wcount, c->wq, c->ctr, c->tb4, c->tleaf,
+                  c->nlo, c->scan_tmp, c->sort_hist, c->sort_status, c->sort_tile_ctr, c->wcount, c->wq, c->digest, c->ctr, c->tb4, c->tleaf,
                   c->sys_off, c->sysid, c->sys_root, c->sys_scale, c->gtab, c->skey, c->skey_alt, c->sidx, c->sidx_alt,
                   c->sum_tile, c->sum_cnt, c->sum_piece};
     for (void* p : vp)
@@ -562,6 +562,7 @@ int bhx_create(const bhx_config* cfg, size_t capacity, bhx_ctx** out) {
         chk(dalloc(&c->scan_tmp, n / 2048 + 16), "alloc scan");
         chk(dalloc(&c->wcount, n), "alloc counts");
         chk(dalloc(&c->wq, 64), "alloc work counters");
+        chk(dalloc(&c->digest, 4), "alloc digest");
         c->sort_tiles = sort_tiles_for(n);
         chk(dalloc(&c->sort_hist, 8 * 256), "alloc sort");
         chk(dalloc(&c->sort_tile_ctr, 8), "alloc sort");
@@ -575,6 +576,7 @@ int bhx_create(const bhx_config* cfg, size_t capacity, bhx_ctx** out) {
         if (rc == BHX_OK) chk(alloc_tree(c, 2 * n + 1024), "alloc tree");
         if (rc == BHX_OK) rc = set_layout(c, std::vector<uint32_t>{0u, 0u});
         if (rc == BHX_OK) chk(hipMemsetAsync(c->ctr, 0, sizeof(Counters), c->s_main), "memset");
+        if (rc == BHX_OK) chk(hipMemsetAsync(c->digest, 0, 4 * sizeof(unsigned long long), c->s_main), "memset");
         if (rc == BHX_OK) chk(hipStreamSynchronize(c->s_main), "sync");
     }
     if (rc != BHX_OK) {
@@ -602,6 +604,14 @@ int bhx_set_config(bhx_ctx* c, const bhx_config* cfg) {
     if (cfg->device != c->device) return fail(c, BHX_E_ARG, "device cannot change");
     const bool tree_shape = cfg->leaf_capacity != c->cfg.leaf_capacity || cfg->mode != c->cfg.mode ||
                             cfg->geometry != c->cfg.geometry;
+    const bhx_config& o = c->cfg;
+    const bool same = cfg->theta == o.theta && cfg->eps == o.eps && cfg->dt == o.dt &&
+                      cfg->leaf_capacity == o.leaf_capacity && cfg->mode == o.mode && cfg->precision == o.precision &&
+                      cfg->quadrupole == o.quadrupole && cfg->permute_acc == o.permute_acc &&
+                      cfg->compensated == o.compensated && cfg->group_walk == o.group_walk &&
+                      cfg->geometry == o.geometry && cfg->group_size == o.group_size;
+    // a recording's backward pass re-runs its frames: they must be the recorded ones
+    if (!same) free_grad(c);
     c->cfg = *cfg;
     c->walk_sub = walk_lists_for(cfg);
     if (tree_shape) c->tree_valid = false;

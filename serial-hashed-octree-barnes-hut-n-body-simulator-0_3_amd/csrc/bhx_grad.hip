@@ -21,6 +21,12 @@
 // term's adjoint (dL/dX_c, dL/dM_c) is pushed down to every body of the cell:
 //     dL/dx_k += (m_k / M_c) dL/dX_c
 //     dL/dm_k += dL/dM_c + dL/dX_c . (x_k - X_c) / M_c.
+// With quadrupoles, a cell of more than one body also carries G = dL/dQ_c
+// (symmetric), and Q_c = sum m_k (3 s_k s_k^T - |s_k|^2 I), s_k = x_k - X_c,
+// gives
+//     dL/dx_k += m_k (6 G s_k - 2 tr(G) s_k)
+//     dL/dm_k += 3 s_k . G s_k - tr(G) |s_k|^2
+// (the path through X_c vanishes: sum m_k s_k = 0).
 // Compensated summation in the leapfrog is treated as exact arithmetic.
 #include <algorithm>
 #include <chrono>
@@ -107,34 +113,58 @@ __global__ __launch_bounds__(kBlock) void k_adj_weights(const double* __restrict
     }
 }
 
-// Cell adjoints pushed down to the bodies (tree order j), written in array order.
+// Cell adjoints pushed down to the bodies (tree order j), written in array
+// order.  kQuad: gnode holds 10 per node and the cells' body counts come
+// from the walk records (walk or walk32, whichever the slot holds); a cell of
+// one body has no quadrupole.
+template <bool kQuad>
 __global__ __launch_bounds__(kBlock) void k_adj_push(const uint32_t* __restrict__ tleaf, const uint32_t* __restrict__ parent,
                                                      const double* __restrict__ mass, const double* __restrict__ com,
                                                      const double* __restrict__ gnode, const double* __restrict__ gbody,
                                                      const double* __restrict__ tx, const double* __restrict__ ty,
                                                      const double* __restrict__ tz, const double* __restrict__ tm,
                                                      const uint32_t* __restrict__ tperm, size_t n,
+                                                     const WalkNode* __restrict__ walk,
+                                                     const WalkVisit32* __restrict__ walk32,
                                                      double* __restrict__ push) {
+    constexpr int gs = kQuad ? 10 : 4;
     for (size_t j = blockIdx.x * (size_t)kBlock + threadIdx.x; j < n; j += (size_t)gridDim.x * kBlock) {
         double ux = 0.0, uy = 0.0, uz = 0.0, sm = 0.0;
+        double qx = 0.0, qy = 0.0, qz = 0.0;
+        const double x = tx[j], y = ty[j], z = tz[j];
         for (uint32_t c = tleaf[j]; c != kNone; c = parent[c]) {
             const double M = mass[c];
             if (!(M > 0.0)) continue;
-            const double* g = gnode + 4 * (size_t)c;
+            const double* g = gnode + gs * (size_t)c;
             const double* X = com + 3 * (size_t)c;
             const double iM = 1.0 / M;
             ux += g[0] * iM;
             uy += g[1] * iM;
             uz += g[2] * iM;
             sm += g[3] - (g[0] * X[0] + g[1] * X[1] + g[2] * X[2]) * iM;
+            if constexpr (kQuad) {
+                const uint32_t cnt = walk ? walk[c].count : (walk32[c].meta >> 6);
+                if (cnt > 1u) {
+                    const double* G = g + 4;
+                    const double sx = x - X[0], sy = y - X[1], sz = z - X[2];
+                    const double gx = G[0] * sx + G[1] * sy + G[2] * sz;
+                    const double gy = G[1] * sx + G[3] * sy + G[4] * sz;
+                    const double gz = G[2] * sx + G[4] * sy + G[5] * sz;
+                    const double tr = G[0] + G[3] + G[5];
+                    qx += 6.0 * gx - 2.0 * tr * sx;
+                    qy += 6.0 * gy - 2.0 * tr * sy;
+                    qz += 6.0 * gz - 2.0 * tr * sz;
+                    sm += 3.0 * (sx * gx + sy * gy + sz * gz) - tr * (sx * sx + sy * sy + sz * sz);
+                }
+            }
         }
         const double* gb = gbody + 4 * j;
         const double m = tm[j];
         const size_t o = tperm[j];
-        push[4 * o] = m * ux + gb[0];
-        push[4 * o + 1] = m * uy + gb[1];
-        push[4 * o + 2] = m * uz + gb[2];
-        push[4 * o + 3] = sm + tx[j] * ux + ty[j] * uy + tz[j] * uz + gb[3];
+        push[4 * o] = m * (ux + qx) + gb[0];
+        push[4 * o + 1] = m * (uy + qy) + gb[1];
+        push[4 * o + 2] = m * (uz + qz) + gb[2];
+        push[4 * o + 3] = sm + x * ux + y * uy + z * uz + gb[3];
     }
 }
 
@@ -167,6 +197,44 @@ __global__ __launch_bounds__(kBlock) void k_adj_init(double* __restrict__ X, dou
     }
 }
 
+// Decision digest, tree half: every body's starting identity, tree slot, leaf
+// id and leaf key hashed with the frame index, summed (wrapping) into
+// digest[0]; then the frame index the next walk mixes in.  ident maps array
+// slots to starting bodies (null: the array order is the starting order).
+// The leaf keys fix the node set (every node is a prefix of a leaf key) and
+// the tree slots the groups, so equal sums mean equal trees and groups.  A
+// build that ran out of nodes left no tree (the recording repeats with more
+// room): nothing is read then, and ids out of range are never followed.
+constexpr uint64_t kDigestTree = 0xa4093822299f31d0ull, kDigestFrame = 0x9e3779b97f4a7c15ull;
+__global__ __launch_bounds__(kBlock) void k_tree_digest(const uint32_t* __restrict__ tperm,
+                                                        const uint32_t* __restrict__ tleaf,
+                                                        const uint64_t* __restrict__ key,
+                                                        const uint32_t* __restrict__ ident, size_t n, uint64_t frame,
+                                                        const Counters* __restrict__ ctr, size_t cap,
+                                                        unsigned long long* __restrict__ digest) {
+    if (ctr->node_overflow) return;
+    uint64_t h = 0;
+    for (size_t j = blockIdx.x * (size_t)kBlock + threadIdx.x; j < n; j += (size_t)gridDim.x * kBlock) {
+        const uint32_t leaf = tleaf[j];
+        const uint32_t o = tperm[j];
+        const uint64_t body = ident ? (o < n ? ident[o] : 0xFFFFFFFFull) : o;
+        const uint64_t lk = leaf < cap ? key[leaf] : 0ull;
+        h += mix64(mix64(lk ^ kDigestTree) + ((body << 32) | (uint64_t)j) + (uint64_t)leaf * 0x632be59bd9b4e019ull +
+                   frame * kDigestFrame);
+    }
+    for (int off = 32; off > 0; off >>= 1) h += (uint64_t)__shfl_xor((unsigned long long)h, off, 64);
+    if ((threadIdx.x & 63u) == 0) atomicAdd(digest, (unsigned long long)h);
+    if (blockIdx.x == 0 && threadIdx.x == 0) digest[2] = frame + 1;
+}
+
+__global__ void k_digest_set(unsigned long long* __restrict__ digest, unsigned long long on, unsigned long long frame) {
+    if (threadIdx.x == 0) {
+        digest[0] = 0ull;
+        digest[1] = on;
+        digest[2] = frame;
+    }
+}
+
 }  // namespace
 
 // What the adjoint walk of one frame reads.
@@ -185,10 +253,11 @@ struct GradSlot {
     uint32_t* sys_root = nullptr;
     double* b4 = nullptr;
     double* scale32 = nullptr;
+    double* quad = nullptr;                    // FP64 walk with quadrupoles: 6 per node
 };
 
 struct GradRecord {
-    int nsteps = 0, every = 0, init_acc = 0, precision = 0;
+    int nsteps = 0, every = 0, init_acc = 0, precision = 0, quad = 0;
     double dt = 0.0;
     size_t n = 0, nsys = 0, state_len = 0;
     size_t node_cap = 0;                       // node records per store slot
@@ -232,13 +301,16 @@ namespace {
 
 size_t align_up(size_t v) { return (v + 255) & ~(size_t)255; }
 
-size_t slot_bytes(size_t n, size_t nodes, size_t nsys) {
+// FP64 quadrupoles ride along only for the FP64 walk (the FP32 payload holds its own).
+bool slot_quad(int precision, int quad) { return quad && precision != BHX_FP32; }
+
+size_t slot_bytes(size_t n, size_t nodes, size_t nsys, bool quad) {
     return align_up(4 * n) + 4 * align_up(8 * n) + align_up(4 * n) + align_up(16 * n) + align_up(4 * n) +
            align_up(64 * nodes) + align_up(4 * nodes) + align_up(24 * nodes) + align_up(8 * nodes) +
-           align_up(4 * nsys) + align_up(32 * nsys) + align_up(16 * nsys);
+           align_up(4 * nsys) + align_up(32 * nsys) + align_up(16 * nsys) + (quad ? align_up(48 * nodes) : 0);
 }
 
-GradSlot carve_slot(char*& p, size_t n, size_t nodes, size_t nsys, int precision) {
+GradSlot carve_slot(char*& p, size_t n, size_t nodes, size_t nsys, int precision, bool quad) {
     GradSlot s;
     auto take = [&](size_t bytes) {
         char* r = p;
@@ -266,6 +338,7 @@ GradSlot carve_slot(char*& p, size_t n, size_t nodes, size_t nsys, int precision
     s.sys_root = (uint32_t*)take(4 * nsys);
     s.b4 = (double*)take(32 * nsys);
     s.scale32 = (double*)take(16 * nsys);
+    if (quad) s.quad = (double*)take(48 * nodes);
     return s;
 }
 
@@ -338,6 +411,7 @@ int store_frame(bhx_ctx* c, const GradSlot& d, size_t nodes, int precision, bool
     add_job(J, c->sys_root, d.sys_root, 4 * c->nsys);
     add_job(J, c->bounds_slot(c->tree_slot), d.b4, 32 * c->nsys);
     add_job(J, c->sys_scale, d.scale32, 16 * c->nsys);
+    if (d.quad) add_job(J, c->tree.quad, d.quad, 48 * nodes);
     HIPCHK(run_jobs(J, 16 * (n > nodes ? n : nodes), s));
     return BHX_OK;
 }
@@ -355,6 +429,7 @@ WalkTree tree_of(const GradSlot& d) {
     t.tperm = d.tperm;
     t.sys_root = d.sys_root;
     t.b4 = d.b4;
+    t.quad = d.quad;
     return t;
 }
 
@@ -385,7 +460,9 @@ Work carve_work(double* w, size_t n, size_t nodes) {
     return k;
 }
 
-size_t work_len(size_t n, size_t nodes) { return 2 * 10 * n + 3 * n + 3 * n + 4 * n + 4 * n + 4 * nodes; }
+size_t work_len(size_t n, size_t nodes, bool quad) {
+    return 2 * 10 * n + 3 * n + 3 * n + 4 * n + 4 * n + (quad ? 10 : 4) * nodes;
+}
 
 // One adjoint walk on stored tree d with weights (B + U hdt); own-position
 // terms into k.gself, cell and body terms pushed down into k.push.
@@ -394,11 +471,16 @@ int adjoint_walk(bhx_ctx* c, GradRecord* g, const Work& k, const GradSlot& d, co
     const size_t n = c->n;
     hipLaunchKernelGGL(k_adj_weights, dim3(grid_of(n)), dim3(kBlock), 0, s, B, U, n, hdt, k.wx, k.wy, k.wz);
     hipLaunchKernelGGL(k_fill_d, dim3(grid_of(4 * n)), dim3(kBlock), 0, s, k.gbody, 4 * n, 0.0);
-    hipLaunchKernelGGL(k_fill_d, dim3(grid_of(4 * g->work_nodes)), dim3(kBlock), 0, s, k.gnode, 4 * g->work_nodes, 0.0);
+    const size_t gs = g->quad ? 10 : 4;
+    hipLaunchKernelGGL(k_fill_d, dim3(grid_of(gs * g->work_nodes)), dim3(kBlock), 0, s, k.gnode, gs * g->work_nodes, 0.0);
     HIPCHK(hipGetLastError());
     HIPCHK(launch_walk_adjoint(c, tree_of(d), g->precision, k.wx, k.wy, k.wz, k.gself, k.gnode, k.gbody, s));
-    hipLaunchKernelGGL(k_adj_push, dim3(grid_of(n)), dim3(kBlock), 0, s, d.tleaf, d.parent, d.mass, d.com, k.gnode,
-                       k.gbody, d.tx, d.ty, d.tz, d.tm, d.tperm, n, k.push);
+    if (g->quad)
+        hipLaunchKernelGGL(k_adj_push<true>, dim3(grid_of(n)), dim3(kBlock), 0, s, d.tleaf, d.parent, d.mass, d.com,
+                           k.gnode, k.gbody, d.tx, d.ty, d.tz, d.tm, d.tperm, n, d.walk, d.walk32, k.push);
+    else
+        hipLaunchKernelGGL(k_adj_push<false>, dim3(grid_of(n)), dim3(kBlock), 0, s, d.tleaf, d.parent, d.mass, d.com,
+                           k.gnode, k.gbody, d.tx, d.ty, d.tz, d.tm, d.tperm, n, d.walk, d.walk32, k.push);
     HIPCHK(hipGetLastError());
     return BHX_OK;
 }
@@ -411,8 +493,10 @@ int bhx_record(bhx_ctx* c, int nsteps, int checkpoint_every, int init_acc) {
     if (!c) return BHX_E_NULL;
     if (nsteps < 1 || checkpoint_every < 1) return fail(c, BHX_E_ARG, "step count and checkpoint interval must be >= 1");
     const bhx_config& cf = c->cfg;
-    if (!cf.group_walk || cf.mode != BHX_MODE_INTENDED || cf.quadrupole || !cf.permute_acc)
-        return fail(c, BHX_E_STATE, "differentiation needs group_walk=1, the intended payload, quadrupole=0, permute_acc=1");
+    if (!cf.group_walk || cf.mode != BHX_MODE_INTENDED || !cf.permute_acc)
+        return fail(c, BHX_E_STATE, "differentiation needs group_walk=1, the intended payload, permute_acc=1");
+    if (c->walk_sub != 1)
+        return fail(c, BHX_E_STATE, "differentiation needs one interaction list per 64-body group (group_size 64)");
     if (c->n == 0) return fail(c, BHX_E_STATE, "no bodies");
     HIPCHK(hipSetDevice(c->device));
     hipStream_t s = c->s_main;
@@ -425,6 +509,7 @@ int bhx_record(bhx_ctx* c, int nsteps, int checkpoint_every, int init_acc) {
     g->every = checkpoint_every;
     g->init_acc = init_acc ? 1 : 0;
     g->precision = cf.precision;
+    g->quad = cf.quadrupole ? 1 : 0;
     g->dt = cf.dt;
     g->n = c->n;
     g->nsys = c->nsys;
@@ -432,10 +517,15 @@ int bhx_record(bhx_ctx* c, int nsteps, int checkpoint_every, int init_acc) {
     const size_t n = c->n;
     const int nck = (nsteps + checkpoint_every - 1) / checkpoint_every;
     int rc = BHX_OK;
+    // the decision digest is collected by the FP64 group walk
+    const bool digest = cf.precision != BHX_FP32;
+    c->digest_valid = false;
     auto bail = [&](int code) {
+        hipLaunchKernelGGL(k_digest_set, dim3(1), dim3(64), 0, s, c->digest, 0ull, 0ull);
         free_grad(c);
         return code;
     };
+    uint64_t init_digest = 0;
     g->ckpt.assign(nck, nullptr);
     for (double*& p : g->ckpt)
         if (dalloc(&p, g->state_len) != hipSuccess) return bail(fail(c, BHX_E_NOMEM, "checkpoint allocation"));
@@ -448,14 +538,22 @@ int bhx_record(bhx_ctx* c, int nsteps, int checkpoint_every, int init_acc) {
         c->bounds_valid = true;
         rc = build_on_device(c, s);
         if (rc != BHX_OK) return bail(rc);
+        if (digest) hipLaunchKernelGGL(k_digest_set, dim3(1), dim3(64), 0, s, c->digest, 1ull, 0ull);
         HIPCHK(launch_walk(c, s));
+        if (digest) {
+            hipLaunchKernelGGL(k_tree_digest, dim3(grid_of(n)), dim3(kBlock), 0, s, c->tperm, c->tleaf, c->tree.key,
+                               nullptr, n, 0ull, c->ctr, c->tree.cap, c->digest);
+            HIPCHK(hipGetLastError());
+            HIPCHK(hipMemcpyAsync(&init_digest, c->digest, sizeof init_digest, hipMemcpyDeviceToHost, s));
+            HIPCHK(hipStreamSynchronize(s));
+        }
         const size_t nodes = c->n_nodes;
         void* st = nullptr;
-        if (hipMalloc(&st, slot_bytes(n, nodes, c->nsys)) != hipSuccess)
+        if (hipMalloc(&st, slot_bytes(n, nodes, c->nsys, slot_quad(g->precision, g->quad))) != hipSuccess)
             return bail(fail(c, BHX_E_NOMEM, "store allocation"));
         g->init_store = st;
         char* p = (char*)st;
-        g->init_slot = carve_slot(p, n, nodes, c->nsys, g->precision);
+        g->init_slot = carve_slot(p, n, nodes, c->nsys, g->precision, slot_quad(g->precision, g->quad));
         rc = store_frame(c, g->init_slot, nodes, g->precision, false, s);
         if (rc != BHX_OK) return bail(rc);
         g->work_nodes = nodes;
@@ -472,6 +570,7 @@ int bhx_record(bhx_ctx* c, int nsteps, int checkpoint_every, int init_acc) {
         }
         hipLaunchKernelGGL(k_iota32, dim3(grid_of(n)), dim3(kBlock), 0, s, g->ident, n);
         hipLaunchKernelGGL(k_zero_flags, dim3(1), dim3(64), 0, s, c->ctr);   // count nodes from here
+        if (digest) hipLaunchKernelGGL(k_digest_set, dim3(1), dim3(64), 0, s, c->digest, 1ull, 1ull);
         for (int f = 0; f < nsteps; ++f) {
             if (f % checkpoint_every == 0) {
                 rc = save_state(c, g->ckpt[f / checkpoint_every], s);
@@ -481,6 +580,9 @@ int bhx_record(bhx_ctx* c, int nsteps, int checkpoint_every, int init_acc) {
             if (rc != BHX_OK) return bail(rc);
             hipLaunchKernelGGL(k_compose, dim3(grid_of(n)), dim3(kBlock), 0, s, g->ident, c->vals, n, g->ident_tmp);
             std::swap(g->ident, g->ident_tmp);
+            if (digest)
+                hipLaunchKernelGGL(k_tree_digest, dim3(grid_of(n)), dim3(kBlock), 0, s, c->tperm, c->tleaf,
+                                   c->tree.key, g->ident, n, (uint64_t)f + 1, c->ctr, c->tree.cap, c->digest);
         }
         rc = read_counters(c, s);
         if (rc != BHX_OK) return bail(rc);
@@ -496,7 +598,15 @@ int bhx_record(bhx_ctx* c, int nsteps, int checkpoint_every, int init_acc) {
     if (g->node_cap > g->work_nodes) g->work_nodes = g->node_cap;
     rc = save_state(c, g->final_state, s);
     if (rc != BHX_OK) return bail(rc);
+    uint64_t frames_digest = 0;
+    if (digest) {
+        HIPCHK(hipMemcpyAsync(&frames_digest, c->digest, sizeof frames_digest, hipMemcpyDeviceToHost, s));
+        hipLaunchKernelGGL(k_digest_set, dim3(1), dim3(64), 0, s, c->digest, 0ull, 0ull);   // stop collecting
+        HIPCHK(hipGetLastError());
+    }
     HIPCHK(hipStreamSynchronize(s));
+    c->digest_host = init_digest + frames_digest;
+    c->digest_valid = digest;
     c->n_nodes = c->ctr_host->total_nodes;
     c->tree_valid = c->tree_built = true;
     c->tree_n = c->n;
@@ -529,6 +639,14 @@ int bhx_recorded_bodies(bhx_ctx* c, double* pos, double* vel, double* mass, doub
     return BHX_OK;
 }
 
+int bhx_record_digest(bhx_ctx* c, uint64_t* out) {
+    if (!c || !out) return c ? fail(c, BHX_E_NULL, "digest output is a null pointer") : BHX_E_NULL;
+    if (!c->grad) return fail(c, BHX_E_STATE, "no recorded trajectory");
+    if (!c->digest_valid) return fail(c, BHX_E_STATE, "the decision digest is collected by the FP64 walk only");
+    *out = c->digest_host;
+    return BHX_OK;
+}
+
 int bhx_adjoint(bhx_ctx* c, const double* gx, const double* gv, const double* ga, double* gx0, double* gv0,
                 double* gm, double* ga0) {
     if (!c) return BHX_E_NULL;
@@ -542,13 +660,13 @@ int bhx_adjoint(bhx_ctx* c, const double* gx, const double* gv, const double* ga
     const size_t n = c->n;
     const int every = g->every;
     if (!g->seg_store) {
-        const size_t one = slot_bytes(n, g->node_cap, c->nsys);
+        const size_t one = slot_bytes(n, g->node_cap, c->nsys, slot_quad(g->precision, g->quad));
         void* st = nullptr;
         if (hipMalloc(&st, 2 * one * (size_t)every) != hipSuccess) return fail(c, BHX_E_NOMEM, "store allocation");
         g->seg_store = st;
         char* p = (char*)st;
         g->slots.resize(2 * (size_t)every);
-        for (auto& sl : g->slots) sl = carve_slot(p, n, g->node_cap, c->nsys, g->precision);
+        for (auto& sl : g->slots) sl = carve_slot(p, n, g->node_cap, c->nsys, g->precision, slot_quad(g->precision, g->quad));
     }
     if (!g->events) {
         g->events = true;
@@ -559,7 +677,7 @@ int bhx_adjoint(bhx_ctx* c, const double* gx, const double* gv, const double* ga
         HIPCHK(hipEventCreateWithFlags(&g->ev_ready, hipEventDisableTiming));
     }
     if (!g->work) {
-        if (dalloc(&g->work, work_len(n, g->work_nodes)) != hipSuccess) return fail(c, BHX_E_NOMEM, "adjoint allocation");
+        if (dalloc(&g->work, work_len(n, g->work_nodes, g->quad != 0)) != hipSuccess) return fail(c, BHX_E_NOMEM, "adjoint allocation");
     }
     hipStream_t sa = c->s_adj;
     Work k = carve_work(g->work, n, g->work_nodes);

@@ -202,6 +202,12 @@ struct bhx_ctx {
     // walk scratch
     uint32_t* wcount = nullptr;                // interaction counts per body (array order)
     uint32_t* wq = nullptr;                    // persistent walks: 8 per-XCD group counters
+    // decision digest of the FP64 walk (recordings): [0] sum of hashed list
+    // entries and tree memberships, [1] non-zero while a recording collects
+    // it, [2] the frame index mixed into the hashes
+    unsigned long long* digest = nullptr;
+    uint64_t digest_host = 0;                  // the last recording's digest
+    bool digest_valid = false;
     int walk_sub = 1;                          // FP32 group walk: lists per 64-body wave (1, 2 or 4)
     // bounds: two slots of (cx, cy, cz, size)
     // (nsys x 4 doubles per slot; slots 0 and 1 alternate between frames, slot 2 is scratch)
@@ -369,11 +375,13 @@ struct WalkTree {
     const uint32_t* tperm;
     const uint32_t* sys_root;
     const double* b4;
+    const double* quad;     // FP64 quadrupoles, 6 per node (FP64 walk with quadrupoles)
 };
 // Adjoint of the group walk on tree t with frozen lists: per body weights w
 // (array order) -> dL/dx of the own position (gself, 3 per body, array
-// order) and per-entry (dL/dX, dL/dM) accumulated into gnode (4 per node)
-// and gbody (4 per body, tree order), which must start at zero.
+// order) and per-entry (dL/dX, dL/dM) accumulated into gnode (4 per node;
+// 10 with quadrupoles: dL/dQ xx xy xz yy yz zz after them) and gbody (4 per
+// body, tree order), which must start at zero.
 // group walk of the current tree that also writes each body's potential
 // (array order); accelerations go to ox/oy/oz
 hipError_t launch_walk_potential(bhx_ctx* c, hipStream_t s, double* ox, double* oy, double* oz, double* pot);

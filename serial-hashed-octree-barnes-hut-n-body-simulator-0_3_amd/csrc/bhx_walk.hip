@@ -83,6 +83,8 @@ struct WalkArgs {
     uint32_t* wq;
     const uint32_t* child;   // octant table: 8 child ids per node
     const uint32_t* child_byte;
+    // decision digest (FP64 group walk): [0] sum, [1] collect when non-zero, [2] frame index
+    unsigned long long* digest;
 };
 
 // One term of ComputeHOTForceInteractionList (HashedOctree.hpp:411-454) in
@@ -444,9 +446,37 @@ __device__ __forceinline__ void eval_bodies(const T (*bd)[4], int nb, T xr, T yr
 // lane = target over the entries, the entry side lane = entry over the 64
 // targets of the group, so neither needs a cross-lane reduction; entry sums
 // go to memory with FP64 atomics.
+//
+// Quadrupole cells add f = 2.5 (d.Qd) r^-7 d - r^-5 Qd (the forward term).
+// With q = Qd, s = d.q, u = w.d, p = Qw:
+//   d(w.f)/dd = r^-5 [5 u r^-2 q + 2.5 s r^-2 w - p + (5 (w.q) r^-2 - 17.5 s u r^-4) d]
+// (target side -=, entry side +=), and w.f is linear in Q:
+//   d(w.f)/dQ = 2.5 u r^-7 d d^T - r^-5 (w d^T + d w^T) / 2
+// (symmetric; stored xx xy xz yy yz zz, off-diagonals once), summed per entry
+// into its node's slots 4..9 and carried to the bodies by the cell's
+// moment definition Q = sum m_k (3 s_k s_k^T - |s_k|^2 I), s_k = x_k - X.
 template <typename T>
-__device__ __forceinline__ void adj_self(const T (*ent)[4], int ne, T xr, T yr, T zr, T wx, T wy, T wz, T eps2,
-                                         T& gx, T& gy, T& gz) {
+__device__ __forceinline__ void quad_adj_d(T dx, T dy, T dz, T wx, T wy, T wz, const T* Q, T inv2, T inv5, T& gx,
+                                           T& gy, T& gz) {
+    const T qx = Q[0] * dx + Q[1] * dy + Q[2] * dz;
+    const T qy = Q[1] * dx + Q[3] * dy + Q[4] * dz;
+    const T qz = Q[2] * dx + Q[4] * dy + Q[5] * dz;
+    const T px = Q[0] * wx + Q[1] * wy + Q[2] * wz;
+    const T py = Q[1] * wx + Q[3] * wy + Q[4] * wz;
+    const T pz = Q[2] * wx + Q[4] * wy + Q[5] * wz;
+    const T s = dx * qx + dy * qy + dz * qz;
+    const T u = dx * wx + dy * wy + dz * wz;
+    const T wq = wx * qx + wy * qy + wz * qz;
+    const T cq = (T)5 * u * inv2, cw = (T)2.5 * s * inv2;
+    const T cd = ((T)5 * wq - (T)17.5 * s * u * inv2) * inv2;
+    gx += inv5 * (cq * qx + cw * wx - px + cd * dx);
+    gy += inv5 * (cq * qy + cw * wy - py + cd * dy);
+    gz += inv5 * (cq * qz + cw * wz - pz + cd * dz);
+}
+
+template <typename T, bool kQuad = false>
+__device__ __forceinline__ void adj_self(const T (*ent)[4], const T (*q)[6], int ne, T xr, T yr, T zr, T wx, T wy,
+                                         T wz, T eps2, T& gx, T& gy, T& gz) {
     for (int k = 0; k < ne; ++k) {
         const T dx = ent[k][0] - xr, dy = ent[k][1] - yr, dz = ent[k][2] - zr;
         const T d2 = dist2(dx, dy, dz);
@@ -460,15 +490,33 @@ __device__ __forceinline__ void adj_self(const T (*ent)[4], int ne, T xr, T yr, 
         gx -= c3 * wx - c5 * dx;
         gy -= c3 * wy - c5 * dy;
         gz -= c3 * wz - c5 * dz;
+        if constexpr (kQuad) {
+            T hx = 0, hy = 0, hz = 0;
+            quad_adj_d<T>(dx, dy, dz, wx, wy, wz, q[k], inv2, inv3 * inv2, hx, hy, hz);
+            gx -= hx;
+            gy -= hy;
+            gz -= hz;
+        }
     }
 }
 
-template <typename T>
-__device__ __forceinline__ void adj_scatter(const T (*ent)[4], const uint32_t* ids, int ne, const T (*tg)[6], T eps2,
-                                            uint32_t lane, double* __restrict__ g4, double sx = 1.0, double sm = 1.0) {
+// Entry side.  g4 holds `gs` doubles per entry (4: dL/dX, dL/dM; 10: and
+// dL/dQ); sx, sm, sq take the sums back to physical units.
+template <typename T, bool kQuad = false>
+__device__ __forceinline__ void adj_scatter(const T (*ent)[4], const T (*q)[6], const uint32_t* ids, int ne,
+                                            const T (*tg)[6], T eps2, uint32_t lane, double* __restrict__ g4,
+                                            double sx = 1.0, double sm = 1.0, double sq = 1.0) {
+    constexpr int gs = kQuad ? 10 : 4;
     for (int e = (int)lane; e < ne; e += 64) {
         const T X = ent[e][0], Y = ent[e][1], Z = ent[e][2];
         T gx = 0, gy = 0, gz = 0, h = 0;
+        T hx = 0, hy = 0, hz = 0;                     // quadrupole part of dL/dX
+        T G[6] = {0, 0, 0, 0, 0, 0};                   // dL/dQ
+        T Q[6] = {0, 0, 0, 0, 0, 0};
+        if constexpr (kQuad) {
+#pragma unroll
+            for (int k = 0; k < 6; ++k) Q[k] = q[e][k];
+        }
         for (int t = 0; t < 64; ++t) {
             const T dx = X - tg[t][0], dy = Y - tg[t][1], dz = Z - tg[t][2];
             const T wx = tg[t][3], wy = tg[t][4], wz = tg[t][5];
@@ -483,13 +531,28 @@ __device__ __forceinline__ void adj_scatter(const T (*ent)[4], const uint32_t* i
             gy += inv3 * wy - c5 * dy;
             gz += inv3 * wz - c5 * dz;
             h += inv3 * dw;
+            if constexpr (kQuad) {
+                const T inv5 = inv3 * inv2;
+                quad_adj_d<T>(dx, dy, dz, wx, wy, wz, Q, inv2, inv5, hx, hy, hz);
+                const T c = (T)2.5 * dw * inv5 * inv2, hh = (T)0.5 * inv5;
+                G[0] += c * dx * dx - inv5 * wx * dx;
+                G[1] += c * dx * dy - hh * (wx * dy + wy * dx);
+                G[2] += c * dx * dz - hh * (wx * dz + wz * dx);
+                G[3] += c * dy * dy - inv5 * wy * dy;
+                G[4] += c * dy * dz - hh * (wy * dz + wz * dy);
+                G[5] += c * dz * dz - inv5 * wz * dz;
+            }
         }
         const T M = ent[e][3];
-        double* g = g4 + 4 * (size_t)ids[e];
-        unsafeAtomicAdd(g, (double)(M * gx) * sx);
-        unsafeAtomicAdd(g + 1, (double)(M * gy) * sx);
-        unsafeAtomicAdd(g + 2, (double)(M * gz) * sx);
+        double* g = g4 + gs * (size_t)ids[e];
+        unsafeAtomicAdd(g, (double)(M * gx + hx) * sx);
+        unsafeAtomicAdd(g + 1, (double)(M * gy + hy) * sx);
+        unsafeAtomicAdd(g + 2, (double)(M * gz + hz) * sx);
         unsafeAtomicAdd(g + 3, (double)h * sm);
+        if constexpr (kQuad) {
+#pragma unroll
+            for (int k = 0; k < 6; ++k) unsafeAtomicAdd(g + 4 + k, (double)G[k] * sq);
+        }
     }
 }
 
@@ -523,7 +586,6 @@ __device__ __forceinline__ void gather_bodies(T (*bd)[4], const uint32_t* rstart
 
 template <typename T, bool kQuad, bool kGuard, bool kAdj = false, bool kPot = false>
 __global__ __launch_bounds__(64) void k_walk_group(WalkArgs a) {
-    static_assert(!(kAdj && kQuad), "the adjoint walk is monopole only");
     static_assert(!(kAdj && kPot), "the adjoint walk has no potential");
     __shared__ T s_nd[kNodeCap][4];
     __shared__ T s_q[kQuad ? kNodeCap : 1][6];
@@ -583,8 +645,8 @@ __global__ __launch_bounds__(64) void k_walk_group(WalkArgs a) {
             if (nn == kNodeCap) {
                 __syncthreads();
                 if constexpr (kAdj) {
-                    adj_self<T>(s_nd, nn, xr, yr, zr, wxr, wyr, wzr, eps2, ax, ay, az);
-                    adj_scatter<T>(s_nd, s_nid, nn, s_tg, eps2, lane, a.gnode);
+                    adj_self<T, kQuad>(s_nd, s_q, nn, xr, yr, zr, wxr, wyr, wzr, eps2, ax, ay, az);
+                    adj_scatter<T, kQuad>(s_nd, s_q, s_nid, nn, s_tg, eps2, lane, a.gnode);
                 } else {
                     eval_nodes<T, kQuad, kGuard, kPot>(s_nd, s_q, nn, xr, yr, zr, eps2, ax, ay, az, phi);
                 }
@@ -614,8 +676,8 @@ __global__ __launch_bounds__(64) void k_walk_group(WalkArgs a) {
                     gather_bodies<T>(s_bd, s_rstart, s_roff, nr, nq, lane, a, ox, oy, oz, kAdj ? s_bid : nullptr);
                     __syncthreads();
                     if constexpr (kAdj) {
-                        adj_self<T>(s_bd, nq, xr, yr, zr, wxr, wyr, wzr, eps2, ax, ay, az);
-                        adj_scatter<T>(s_bd, s_bid, nq, s_tg, eps2, lane, a.gbody);
+                        adj_self<T>(s_bd, s_q, nq, xr, yr, zr, wxr, wyr, wzr, eps2, ax, ay, az);
+                        adj_scatter<T>(s_bd, s_q, s_bid, nq, s_tg, eps2, lane, a.gbody);
                     } else {
                         eval_bodies<T, kGuard, kPot>(s_bd, nq, xr, yr, zr, eps2, ax, ay, az, phi);
                     }
@@ -638,16 +700,16 @@ __global__ __launch_bounds__(64) void k_walk_group(WalkArgs a) {
     }
     __syncthreads();
     if constexpr (kAdj) {
-        adj_self<T>(s_nd, nn, xr, yr, zr, wxr, wyr, wzr, eps2, ax, ay, az);
-        adj_scatter<T>(s_nd, s_nid, nn, s_tg, eps2, lane, a.gnode);
+        adj_self<T, kQuad>(s_nd, s_q, nn, xr, yr, zr, wxr, wyr, wzr, eps2, ax, ay, az);
+        adj_scatter<T, kQuad>(s_nd, s_q, s_nid, nn, s_tg, eps2, lane, a.gnode);
     } else {
         eval_nodes<T, kQuad, kGuard, kPot>(s_nd, s_q, nn, xr, yr, zr, eps2, ax, ay, az, phi);
     }
     gather_bodies<T>(s_bd, s_rstart, s_roff, nr, nq, lane, a, ox, oy, oz, kAdj ? s_bid : nullptr);
     __syncthreads();
     if constexpr (kAdj) {
-        adj_self<T>(s_bd, nq, xr, yr, zr, wxr, wyr, wzr, eps2, ax, ay, az);
-        adj_scatter<T>(s_bd, s_bid, nq, s_tg, eps2, lane, a.gbody);
+        adj_self<T>(s_bd, s_q, nq, xr, yr, zr, wxr, wyr, wzr, eps2, ax, ay, az);
+        adj_scatter<T>(s_bd, s_q, s_bid, nq, s_tg, eps2, lane, a.gbody);
     } else {
         eval_bodies<T, kGuard, kPot>(s_bd, nq, xr, yr, zr, eps2, ax, ay, az, phi);
     }
@@ -801,7 +863,6 @@ __device__ __forceinline__ void gather_cells32(float (*nd)[4], float (*q)[6], ui
 
 template <bool kQuad, bool kGuard, bool kAdj = false, bool kPot = false>
 __global__ __launch_bounds__(64) void k_walk_group32(WalkArgs a) {
-    static_assert(!(kAdj && kQuad), "the adjoint walk is monopole only");
     static_assert(!(kAdj && kPot), "the adjoint walk has no potential");
     constexpr int kBodies32 = bodies32<kAdj>();
     __shared__ float s_nd[kCells32][4];
@@ -871,8 +932,9 @@ __global__ __launch_bounds__(64) void k_walk_group32(WalkArgs a) {
         gather_cells32<kQuad>(s_nd, s_q, cell_ids, nn, lane, a.pay32, a.com, ox, oy, oz, lam);
         __syncthreads();
         if constexpr (kAdj) {
-            adj_self<float>(s_nd, nn, xr, yr, zr, wxr, wyr, wzr, eps2, ax, ay, az);
-            adj_scatter<float>(s_nd, s_nid, nn, s_tg, eps2, lane, a.gnode, lam * lam * lam / mu, lam * lam);
+            adj_self<float, kQuad>(s_nd, s_q, nn, xr, yr, zr, wxr, wyr, wzr, eps2, ax, ay, az);
+            adj_scatter<float, kQuad>(s_nd, s_q, s_nid, nn, s_tg, eps2, lane, a.gnode, lam * lam * lam / mu, lam * lam,
+                                      (lam * lam) * (lam * lam));
         } else {
             eval_list32<kQuad, kGuard, kPot>(s_nd, s_q, nn, xr, yr, zr, eps2, ax, ay, az, phi);
         }
@@ -908,8 +970,8 @@ __global__ __launch_bounds__(64) void k_walk_group32(WalkArgs a) {
         }
         __syncthreads();
         if constexpr (kAdj) {
-            adj_self<float>(s_bd, nq, xr, yr, zr, wxr, wyr, wzr, eps2, ax, ay, az);
-            adj_scatter<float>(s_bd, s_bid, nq, s_tg, eps2, lane, a.gbody, lam * lam * lam / mu, lam * lam);
+            adj_self<float>(s_bd, s_q, nq, xr, yr, zr, wxr, wyr, wzr, eps2, ax, ay, az);
+            adj_scatter<float>(s_bd, s_q, s_bid, nq, s_tg, eps2, lane, a.gbody, lam * lam * lam / mu, lam * lam);
         } else {
             eval_list32<false, kGuard, kPot>(s_bd, s_q, nq, xr, yr, zr, eps2, ax, ay, az, phi);
         }
@@ -1478,6 +1540,8 @@ __global__ __launch_bounds__(64) void k_walk_lp32(WalkArgs a) {
 // per 64-body wave.
 constexpr int kLp64Cells = 64;
 constexpr int kLp64Bodies = 64;
+constexpr uint64_t kDigestCell = 0x243f6a8885a308d3ull, kDigestLeaf = 0x13198a2e03707344ull;
+constexpr uint64_t kDigestFrame = 0x9e3779b97f4a7c15ull;
 
 template <bool kQuad, bool kGuard, bool kPot = false>
 __global__ __launch_bounds__(64) void k_walk_lp64(WalkArgs a) {
@@ -1497,6 +1561,10 @@ __global__ __launch_bounds__(64) void k_walk_lp64(WalkArgs a) {
     (void)qf;
     const double th2 = a.theta2;
     const double eps2 = a.eps2;
+    // decision digest: every list entry hashed with its group and the frame
+    const bool dig = a.digest && a.digest[1] != 0ull;
+    const uint64_t salt = dig ? (uint64_t)a.digest[2] * kDigestFrame : 0ull;
+    uint64_t hsum = 0;
 
     for (;;) {
         const uint32_t grp = lp_next_group(a.wq, a.groups, lane);
@@ -1520,6 +1588,7 @@ __global__ __launch_bounds__(64) void k_walk_lp64(WalkArgs a) {
             __syncthreads();
             if ((int)lane < nn) {
                 const uint32_t id = s_cid[lane];
+                if (dig) hsum += mix64((((uint64_t)grp << 32) | id) ^ kDigestCell ^ salt);
                 const WalkNode w = a.nodes[id];
                 s_nd[lane][0] = w.cx;
                 s_nd[lane][1] = w.cy;
@@ -1565,6 +1634,7 @@ __global__ __launch_bounds__(64) void k_walk_lp64(WalkArgs a) {
             nq = 0;
         };
         auto add_leaf = [&](uint32_t st, uint32_t count) {
+            if (dig && lane == 0) hsum += mix64((((uint64_t)grp << 32) | st) ^ kDigestLeaf ^ salt);
             for (uint32_t k = 0; k < count; k += 64) {
                 const int m = (int)((count - k) < 64u ? (count - k) : 64u);
                 if (nq + m > kLp64Bodies || nr == kRanges32) flush_bodies();
@@ -1689,6 +1759,11 @@ __global__ __launch_bounds__(64) void k_walk_lp64(WalkArgs a) {
             if constexpr (kPot) a.pot[o] = phi;
             if (a.counts) a.counts[o] = cnt;
         }
+    }
+    if (dig) {
+        // wrapping sums commute: the digest does not depend on which wave walked which group
+        for (int off = 32; off > 0; off >>= 1) hsum += (uint64_t)__shfl_xor((unsigned long long)hsum, off, 64);
+        if (lane == 0) atomicAdd(a.digest, (unsigned long long)hsum);
     }
 }
 
@@ -1852,6 +1927,7 @@ hipError_t launch_walk_to(bhx_ctx* c, hipStream_t s, bool group, int precision, 
     a.wq = c->wq;
     a.child = c->tree.child;
     a.child_byte = c->tree.child_byte;
+    a.digest = c->digest;
     const int sub = c->walk_sub;
     const unsigned groups = c->ngroups;
     const unsigned grid = (groups + kWaves - 1) / kWaves;
@@ -1961,15 +2037,20 @@ hipError_t launch_walk_adjoint(bhx_ctx* c, const WalkTree& t, int precision, con
     a.gself = gself;
     a.gnode = gnode;
     a.gbody = gbody;
+    a.quad = t.quad;
     const unsigned groups = c->ngroups;
     const bool guard = !(a.eps2 > 0.0);
-    if (precision == BHX_FP32) {
-        if (guard) hipLaunchKernelGGL((k_walk_group32<false, true, true>), dim3(8 * ((groups + 7) / 8)), dim3(64), 0, s, a);
-        else hipLaunchKernelGGL((k_walk_group32<false, false, true>), dim3(8 * ((groups + 7) / 8)), dim3(64), 0, s, a);
-    } else {
-        if (guard) hipLaunchKernelGGL((k_walk_group<double, false, true, true>), dim3(groups), dim3(64), 0, s, a);
-        else hipLaunchKernelGGL((k_walk_group<double, false, false, true>), dim3(groups), dim3(64), 0, s, a);
-    }
+    const bool quad = c->cfg.quadrupole != 0;
+#define BHX_ADJ(Q, G)                                                                                             \
+    do {                                                                                                          \
+        if (precision == BHX_FP32)                                                                                \
+            hipLaunchKernelGGL((k_walk_group32<Q, G, true>), dim3(8 * ((groups + 7) / 8)), dim3(64), 0, s, a);    \
+        else                                                                                                      \
+            hipLaunchKernelGGL((k_walk_group<double, Q, G, true>), dim3(groups), dim3(64), 0, s, a);              \
+    } while (0)
+    if (quad) { if (guard) BHX_ADJ(true, true); else BHX_ADJ(true, false); }
+    else { if (guard) BHX_ADJ(false, true); else BHX_ADJ(false, false); }
+#undef BHX_ADJ
     return hipGetLastError();
 }
 

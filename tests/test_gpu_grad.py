@@ -31,11 +31,6 @@ def _weights(n, seed):
     return rng.normal(size=(n, 3)), rng.normal(size=(n, 3))
 
 
-def _loss(e, gx, gv):
-    P, V, M, A = e.recorded_bodies()
-    return float((gx * P).sum() + (gv * V).sum())
-
-
 @pytest.mark.parametrize("prec", [0, 1])
 def test_adjoint_theta0_matches_oracle_direct_sum(bhx, prec):
     n, T = 300, 12
@@ -62,32 +57,111 @@ def test_adjoint_theta0_matches_oracle_direct_sum(bhx, prec):
     assert np.all(G[2] == 0.0)
 
 
-def test_adjoint_matches_frozen_finite_differences(bhx):
-    """theta = 0.5 with cells: central differences of the engine's own frames.  A step
-    of 1e-6 moves no opening decision here (checked: the same trajectory structure),
-    so the differences see the frozen-decision derivative."""
+def _perturbed(pos, vel, mass, arg, i, a, dh):
+    p2, v2, m2 = pos.copy(), vel.copy(), mass.copy()
+    if arg == 0:
+        p2[i, a] += dh
+    elif arg == 1:
+        v2[i, a] += dh
+    else:
+        m2[i] += dh
+    return p2, v2, m2
+
+
+def _frozen_fd(e, pos, vel, mass, T, picks, loss, digest0, h=1e-6):
+    """Central differences of loss(recorded bodies) for each pick (arg 0: x, 1: v, 2: m).
+    Every perturbed recording must report the base recording's decision digest: the
+    same trees and interaction lists in every frame, so the difference quotient is
+    one of the frozen-decision function the adjoint differentiates."""
+    out = []
+    for (arg, i, a) in picks:
+        vals = []
+        for sgn in (1.0, -1.0):
+            e.set_bodies(*_perturbed(pos, vel, mass, arg, i, a, sgn * h))
+            e.record(T, checkpoint_every=T, init_acc=True)
+            assert e.record_digest() == digest0, ("a decision moved", arg, i, a, sgn)
+            vals.append(loss(e.recorded_bodies()))
+        out.append((vals[0] - vals[1]) / (2 * h))
+    return out
+
+
+def _grad_of(G, arg, i, a):
+    return G[arg][i, a] if arg < 2 else G[3][i]
+
+
+@pytest.mark.parametrize("quad", [0, 1])
+def test_adjoint_matches_frozen_finite_differences(bhx, quad):
+    """theta = 0.5 with cells, monopole and quadrupole: central differences of the
+    engine's own frames on positions, velocities and masses, every perturbed run
+    proven on the same decisions by its digest."""
     n, T, h = 400, 10, 1e-6
     pos, vel, mass = _bodies(bhx, n, 9)
     gx, gv = _weights(n, 10)
-    with bhx.Engine(n, **DIFF) as e:
+    print("frozen FD n=%d T=%d quadrupole=%d" % (n, T, quad), flush=True)
+    with bhx.Engine(n, **dict(DIFF, quadrupole=quad)) as e:
         e.set_bodies(pos, vel, mass)
         e.record(T, checkpoint_every=4, init_acc=True)
+        d0 = e.record_digest()
         G = e.adjoint(gx, gv)
         rng = np.random.default_rng(11)
-        picks = [(int(i), int(a)) for i, a in zip(rng.choice(n, 8, replace=False), rng.integers(0, 3, 8))]
-        for (i, a) in picks:
-            for arg in (0, 1):
-                vals = []
-                for sgn in (1.0, -1.0):
-                    p2, v2 = pos.copy(), vel.copy()
-                    (p2 if arg == 0 else v2)[i, a] += sgn * h
-                    e.set_bodies(p2, v2, mass)
-                    e.record(T, checkpoint_every=100, init_acc=True)
-                    vals.append(_loss(e, gx, gv))
-                fd = (vals[0] - vals[1]) / (2 * h)
-                got = G[arg][i, a]
-                # central differences: O(h^2) truncation plus ~1e-16 |L| / h roundoff
-                assert abs(fd - got) <= 2e-6 * max(1.0, abs(fd)), (arg, i, a, fd, got)
+        picks = [(arg, int(i), int(a)) for arg in (0, 1, 2)
+                 for i, a in zip(rng.choice(n, 8, replace=False), rng.integers(0, 3, 8))]
+        fds = _frozen_fd(e, pos, vel, mass, T, picks, lambda r: _loss_of(r, gx, gv), d0, h)
+    for (arg, i, a), fd in zip(picks, fds):
+        got = _grad_of(G, arg, i, a)
+        # central differences: O(h^2) truncation plus ~1e-16 |L| / h roundoff
+        assert abs(fd - got) <= 2e-6 * max(1.0, abs(fd)), (arg, i, a, fd, got)
+
+
+def _loss_of(rec, gx, gv):
+    P, V, M, A = rec
+    return float((gx * P).sum() + (gv * V).sum())
+
+
+def test_record_digest_tracks_decisions(bhx):
+    """Equal starting states give equal digests; a body moved across the cluster
+    changes the trees and lists and so the digest; the FP32 walk collects none."""
+    n, T = 500, 5
+    pos, vel, mass = _bodies(bhx, n, 40)
+    with bhx.Engine(n, **DIFF) as e:
+        e.set_bodies(pos, vel, mass)
+        e.record(T, checkpoint_every=2, init_acc=True)
+        d0 = e.record_digest()
+        e.set_bodies(pos, vel, mass)
+        e.record(T, checkpoint_every=5, init_acc=True)
+        assert e.record_digest() == d0
+        p2 = pos.copy()
+        p2[7] = -p2[7] + 0.3
+        e.set_bodies(p2, vel, mass)
+        e.record(T, checkpoint_every=5, init_acc=True)
+        assert e.record_digest() != d0
+        e.set_config(precision=1)
+        with pytest.raises(bhx.BhxError):
+            e.record_digest()                        # the recording was dropped with the config
+        e.set_bodies(pos, vel, mass)
+        e.record(T, checkpoint_every=5, init_acc=True)
+        with pytest.raises(bhx.BhxError):
+            e.record_digest()                        # FP64 walk only
+
+
+def test_adjoint_quadrupole_fp32_walk_matches_fp64_walk(bhx):
+    """The FP32 walk's quadrupole adjoint (scaled units) against the FP64 walk's on
+    the same starting state: the same function up to FP32 rounding and the rare list
+    entry whose opening test rounds the other way."""
+    n, T = 3000, 6
+    pos, vel, mass = _bodies(bhx, n, 41)
+    gx, gv = _weights(n, 42)
+    res = []
+    for prec in (0, 1):
+        with bhx.Engine(n, **dict(DIFF, quadrupole=1, precision=prec, eps=0.01, leaf_capacity=16)) as e:
+            e.set_bodies(pos, vel, mass)
+            e.record(T, checkpoint_every=3, init_acc=True)
+            res.append(e.adjoint(gx, gv))
+    for k, name in ((0, "x0"), (1, "v0"), (3, "m")):
+        err = np.abs(res[1][k] - res[0][k]).max() / np.abs(res[0][k]).max()
+        # FP32 terms: ~1e-6 relative each; a flipped list entry moves a few terms by the
+        # monopole-quadrupole truncation (~1e-3 of a cell term)
+        assert err < 2e-3, (name, err)
 
 
 def test_adjoint_independent_of_checkpoint_interval(bhx):
@@ -156,12 +230,16 @@ def test_differentiation_argument_errors(bhx):
         e.set_bodies(pos, vel, mass)
         with pytest.raises(bhx.BhxError):
             e.adjoint(np.zeros((100, 3)))           # nothing recorded
-        e.set_config(quadrupole=1)
+        e.set_config(group_size=32)
         with pytest.raises(bhx.BhxError):
-            e.record(3)                              # monopole walk only
-        e.set_config(quadrupole=0)
+            e.record(3)                              # one list per 64-body group
+        e.set_config(group_size=64)
         with pytest.raises(bhx.BhxError):
             e.record(0)
+        e.record(3)
+        e.set_config(theta=0.4)
+        with pytest.raises(bhx.BhxError):
+            e.adjoint(np.zeros((100, 3)))           # a new configuration drops the recording
 
 
 def test_torch_autograd_through_engine(bhx):
