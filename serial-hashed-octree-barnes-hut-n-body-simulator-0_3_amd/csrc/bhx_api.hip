@@ -425,6 +425,7 @@ int graph_frame(bhx_ctx* c, hipStream_t s, bool linear) {
     const int rc = step_graph_for(c, &g, linear);
     if (rc != BHX_OK) return rc;
     HIPCHK(hipGraphLaunch(g, s));
+    c->hash_valid = false;   // the frame built a new tree
     swap_body_buffers(c);
     c->tree_slot = c->bslot;
     c->bslot ^= 1;

@@ -354,6 +354,7 @@ unsigned blocks_for(size_t n, unsigned per) {
 hipError_t launch_ref_lists(bhx_ctx* c, const double* pts, size_t nq, double theta, const uint64_t* offsets,
                             uint64_t* counts, int64_t* entries, uint32_t* err, hipStream_t s) {
     if (nq == 0) return hipSuccess;
+    if (const hipError_t he = ensure_hash(c, s); he != hipSuccess) return he;
     const TreeView v = view_of(c);
     if (entries)
         hipLaunchKernelGGL(k_ref_lists<true>, dim3(blocks_for(nq, kLB)), dim3(kLB), 0, s, v, pts, nq, theta, offsets,
@@ -367,6 +368,7 @@ hipError_t launch_ref_lists(bhx_ctx* c, const double* pts, size_t nq, double the
 hipError_t launch_force_lists(bhx_ctx* c, const double* pts, size_t nq, const uint64_t* offsets,
                               const int64_t* entries, double* acc, uint32_t* err, hipStream_t s) {
     if (nq == 0) return hipSuccess;
+    if (const hipError_t he = ensure_hash(c, s); he != hipSuccess) return he;
     hipLaunchKernelGGL(k_force_lists, dim3(blocks_for(nq, kLB)), dim3(kLB), 0, s, view_of(c), pts, nq, offsets,
                        entries, c->cfg.eps * c->cfg.eps, c->cfg.quadrupole, acc, err);
     return hipGetLastError();
@@ -375,12 +377,14 @@ hipError_t launch_force_lists(bhx_ctx* c, const double* pts, size_t nq, const ui
 hipError_t launch_mac(bhx_ctx* c, const int64_t* ids, const double* pts, size_t n, double theta, uint8_t* out,
                       uint32_t* err, hipStream_t s) {
     if (n == 0) return hipSuccess;
+    if (const hipError_t he = ensure_hash(c, s); he != hipSuccess) return he;
     hipLaunchKernelGGL(k_mac, dim3(blocks_for(n, kLB)), dim3(kLB), 0, s, view_of(c), ids, pts, n, theta, out, err);
     return hipGetLastError();
 }
 
 hipError_t launch_node_put(bhx_ctx* c, const bhx_node& rec, uint32_t id, bool insert, uint32_t* err,
                            hipStream_t s) {
+    if (const hipError_t he = ensure_hash(c, s); he != hipSuccess) return he;
     DeviceTree& t = c->tree;
     NodeRec r{};
     r.key = rec.key;
@@ -402,12 +406,14 @@ hipError_t launch_node_put(bhx_ctx* c, const bhx_node& rec, uint32_t id, bool in
 }
 
 hipError_t launch_node_erase(bhx_ctx* c, uint64_t key, uint32_t* done, hipStream_t s) {
+    if (const hipError_t he = ensure_hash(c, s); he != hipSuccess) return he;
     DeviceTree& t = c->tree;
     hipLaunchKernelGGL(k_node_erase, dim3(1), dim3(64), 0, s, key, t.key, t.hkey, t.hval, (uint64_t)(t.hcap - 1), done);
     return hipGetLastError();
 }
 
 hipError_t launch_prune(bhx_ctx* c, uint64_t root_key, uint32_t root_level, uint32_t* removed, hipStream_t s) {
+    if (const hipError_t he = ensure_hash(c, s); he != hipSuccess) return he;
     DeviceTree& t = c->tree;
     if (c->n_nodes == 0) return hipSuccess;
     unsigned g = blocks_for(c->n_nodes, 256);

@@ -225,6 +225,7 @@ struct bhx_ctx {
     bool tree_valid = false;   // tree built on the current bodies (walk allowed)
     bool tree_built = false;   // node table of the last build still intact (downloads, lookups)
     bool tree_edited = false;  // nodes put or erased since the build (the walks' records are stale)
+    bool hash_valid = false;   // the node hash table holds the current tree (built on first use)
     size_t tree_n = 0;         // bodies the last build covered
     // independent systems
     uint32_t nsys = 1;
@@ -355,6 +356,9 @@ void swap_body_buffers(bhx_ctx* c);
 
 hipError_t build_tree(bhx_ctx* c, hipStream_t s, size_t* total_out, bool sync);
 hipError_t launch_lookup(bhx_ctx* c, uint32_t sys, const uint64_t* keys, size_t n, int64_t* ids, hipStream_t s);
+// The node hash table of the current tree: built on the first lookup or edit
+// after a build (the frames themselves never read it).
+hipError_t ensure_hash(bhx_ctx* c, hipStream_t s);
 
 hipError_t launch_walk(bhx_ctx* c, hipStream_t s);
 hipError_t launch_walk_to(bhx_ctx* c, hipStream_t s, bool group, int precision, double* ox, double* oy, double* oz,

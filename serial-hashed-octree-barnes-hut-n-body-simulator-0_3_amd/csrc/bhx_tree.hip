@@ -657,9 +657,11 @@ __device__ Payload moments_of(uint32_t p, int intended, const TreeArgs& t, const
 
 // Moments in two launches.  Phase 1: workgroup b owns the pre-order slice
 // of nodes [256 b, 256 b + 256); a node whose whole subtree [p, skip) lies in
-// the slice is combined there, deepest level first, with a workgroup barrier
-// between levels (children and parent share the compute unit, so no
-// device-wide ordering is needed).  Nodes whose subtree leaves the slice
+// the slice is combined there: every leaf at once first (leaves read only
+// their bodies, so all their loads overlap instead of one level's leaves
+// waiting for the level below), then the inner nodes deepest level first,
+// with a workgroup barrier between levels (children and parent share the
+// compute unit, so no device-wide ordering is needed).  Nodes whose subtree leaves the slice
 // are ancestors of the first node after the slice, so a slice has at most
 // one per level: they are listed by level (rest[L * slices + i], count
 // nrest[L]) for phase 2, where one workgroup combines them level by level.
@@ -685,26 +687,29 @@ __global__ __launch_bounds__(kBlock) void k_moments_local(const Counters* __rest
     if (threadIdx.x == 0) s_levels = 0;
     __syncthreads();
     uint32_t L = 0;
-    bool local = false;
+    bool local = false, leaf = false;
     if (valid) {
         L = t.level[p];
-        local = t.skip[p] <= b1;
-        if (local) atomicOr(&s_levels, 1u << L);
-        else rest[L * (size_t)gridDim.x + atomicAdd(&nrest[L], 1u)] = p;
+        local = t.skip[p] <= b1;   // always for a leaf: its subtree is itself
+        leaf = t.child_byte[p] == 0u;
+        if (local && !leaf) atomicOr(&s_levels, 1u << L);
+        else if (!local) rest[L * (size_t)gridDim.x + atomicAdd(&nrest[L], 1u)] = p;
     }
+    const ChildFromSlice slice{s_m, s_c, s_f, s_q, b0};
+    auto combine = [&]() {
+        const Payload r = moments_of(p, intended, t, bounds4, sv, px, py, pz, tx, ty, tz, tm, tperm, slice);
+        const uint32_t i = threadIdx.x;
+        s_m[i] = r.M;
+        s_c[i][0] = r.cx; s_c[i][1] = r.cy; s_c[i][2] = r.cz;
+        for (int k = 0; k < 6; ++k) s_q[i][k] = r.q[k];
+        s_f[i] = r.first;
+    };
+    if (leaf) combine();
     __syncthreads();
     const uint32_t levels = s_levels;
-    const ChildFromSlice slice{s_m, s_c, s_f, s_q, b0};
     for (int l = kMaxLevel; l >= 0; --l) {
         if (!((levels >> l) & 1u)) continue;   // block-uniform
-        if (local && (int)L == l) {
-            const Payload r = moments_of(p, intended, t, bounds4, sv, px, py, pz, tx, ty, tz, tm, tperm, slice);
-            const uint32_t i = threadIdx.x;
-            s_m[i] = r.M;
-            s_c[i][0] = r.cx; s_c[i][1] = r.cy; s_c[i][2] = r.cz;
-            for (int k = 0; k < 6; ++k) s_q[i][k] = r.q[k];
-            s_f[i] = r.first;
-        }
+        if (local && !leaf && (int)L == l) combine();
         __syncthreads();
     }
 }
@@ -809,10 +814,7 @@ hipError_t build_tree(bhx_ctx* c, hipStream_t s, size_t* total_out, bool sync) {
                        cap, t, c->sys_root, c->ctr);
     hipLaunchKernelGGL(k_emit_links, dim3(grid_for(span)), dim3(kBlock), 0, s, c->tkeys, n, B, sv, c->nlo, c->noff,
                        cap, t, c->ctr);
-    hipLaunchKernelGGL(k_fill_u64, dim3(grid_for(c->tree.hcap)), dim3(kBlock), 0, s, c->tree.hkey, c->tree.hcap,
-                       0ull);
-    hipLaunchKernelGGL(k_hash_insert, dim3(grid_for(span)), dim3(kBlock), 0, s, c->tree.key, c->tree.start, sv,
-                       c->tree.hkey, c->tree.hval, (uint64_t)(c->tree.hcap - 1), c->ctr);
+    c->hash_valid = false;
     const int intended = c->cfg.mode == BHX_MODE_INTENDED;
     // per-level lists of slice-straddling nodes: (kMaxLevel + 1) x slices
     // entries, within the node capacity (slices = capacity / 256, rounded up)
@@ -829,7 +831,21 @@ hipError_t build_tree(bhx_ctx* c, hipStream_t s, size_t* total_out, bool sync) {
     return hipGetLastError();
 }
 
+hipError_t ensure_hash(bhx_ctx* c, hipStream_t s) {
+    if (c->hash_valid) return hipSuccess;
+    const size_t span = c->tree.cap;   // the insert stops at the node count the build left on the device
+    hipLaunchKernelGGL(k_fill_u64, dim3(grid_for(c->tree.hcap)), dim3(kBlock), 0, s, c->tree.hkey, c->tree.hcap,
+                       0ull);
+    hipLaunchKernelGGL(k_hash_insert, dim3(grid_for(span)), dim3(kBlock), 0, s, c->tree.key, c->tree.start, c->sys(),
+                       c->tree.hkey, c->tree.hval, (uint64_t)(c->tree.hcap - 1), c->ctr);
+    const hipError_t e = hipGetLastError();
+    if (e == hipSuccess) c->hash_valid = true;
+    return e;
+}
+
 hipError_t launch_lookup(bhx_ctx* c, uint32_t sys, const uint64_t* keys, size_t n, int64_t* ids, hipStream_t s) {
+    const hipError_t e = ensure_hash(c, s);
+    if (e != hipSuccess) return e;
     const DeviceTree& t = c->tree;
     hipLaunchKernelGGL(k_hash_lookup, dim3(grid_for(n)), dim3(kBlock), 0, s, t.hkey, t.hval, t.start, c->sys(), sys,
                        (uint64_t)(t.hcap - 1), keys, n, ids);

@@ -337,8 +337,9 @@ __global__ __launch_bounds__(kBlock) void k_walk32(WalkArgs a) {
 // Node records arrive as one 64-byte scalar burst and the bodies of queued
 // leaves in one batch of loads per flush.  FP32 works in offsets from the
 // group's first body (only the offsets are rounded); FP64 keeps absolute
-// coordinates, the reference's unfused opening test and its term
-// 1/sqrt(D2) * 1/D2.
+// coordinates and the reference's unfused opening test, and forms the term's
+// 1/sqrt(D2) and 1/D2 from a Newton-refined reciprocal square root (the
+// per-body walk k_walk64 keeps the reference's rounded divisions).
 constexpr int kNodeCap = 128;
 constexpr int kBodyCap = 128;
 constexpr int kRangeCap = 64;
@@ -347,16 +348,26 @@ template <typename T>
 __device__ __forceinline__ T rsqrt_t(T r2);
 template <>
 __device__ __forceinline__ float rsqrt_t<float>(float r2) { return __builtin_amdgcn_rsqf(r2); }
+// FP64: the hardware estimate refined by two Newton steps y += y (1 - x y^2) / 2
+// (each squares the relative error; the result is within ~1 ulp of 1/sqrt(x)).
+// Scaling x by 4^k scales every step by 2^-k exactly.  x = 0 gives a NaN or
+// infinity, which the zero-separation guard discards.
 template <>
-__device__ __forceinline__ double rsqrt_t<double>(double r2) { return 1.0 / sqrt(r2); }
+__device__ __forceinline__ double rsqrt_t<double>(double r2) {
+    double y = __builtin_amdgcn_rsq(r2);
+    double e = fma(-r2 * y, y, 1.0);
+    y = fma(0.5 * y, e, y);
+    e = fma(-r2 * y, y, 1.0);
+    return fma(0.5 * y, e, y);
+}
 
-// 1/r2 from 1/sqrt(r2): squared in FP32, the reference's separate division in FP64
+// 1/r2 from 1/sqrt(r2): squared (within ~2 ulp)
 template <typename T>
 __device__ __forceinline__ T inv_sq_t(T inv, T r2);
 template <>
 __device__ __forceinline__ float inv_sq_t<float>(float inv, float) { return inv * inv; }
 template <>
-__device__ __forceinline__ double inv_sq_t<double>(double, double r2) { return 1.0 / r2; }
+__device__ __forceinline__ double inv_sq_t<double>(double inv, double) { return inv * inv; }
 
 // The reference opening test size^2/d^2 < theta^2: exact FP64 form, and the
 // FP32 form size^2 < theta^2 d^2 (equal away from rounding at the boundary).
@@ -1536,7 +1547,7 @@ __global__ __launch_bounds__(64) void k_walk_lp32(WalkArgs a) {
 // records: the reference's opening test size^2 / d^2 < theta^2 in FP64 from
 // absolute coordinates (fl(s^2 / x) is non-increasing in x, so every body
 // accepts exactly when the test passes at the least |d|^2), and the
-// reference's term, 1/sqrt(D2) and 1/D2 each correctly rounded.  One list
+// reference's term with 1/sqrt(D2) and 1/D2 within ~1 and ~2 ulp.  One list
 // per 64-body wave.
 constexpr int kLp64Cells = 64;
 constexpr int kLp64Bodies = 64;
