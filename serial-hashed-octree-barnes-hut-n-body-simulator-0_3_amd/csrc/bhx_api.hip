@@ -126,6 +126,8 @@ void free_all(bhx_ctx* c) {
     void* vp[] = {c->keys, c->keys_alt, c->vals, c->vals_alt, c->mkeys, c->tkeys, c->tperm, c->nn, c->noff,
                   c->nlo, c->scan_tmp, c->sort_hist, c->sort_status, c->sort_tile_ctr, c->wcount, c->wq, c->digest, c->ctr, c->tb4, c->tleaf,
                   c->sys_off, c->sysid, c->sys_root, c->sys_scale, c->gtab, c->skey, c->skey_alt, c->sidx, c->sidx_alt,
+                  c->tperm_old, c->inv, c->sort2.keys_alt, c->sort2.vals_alt, c->sort2.hist, c->sort2.tile_ctr,
+                  c->sort2.status, c->sort2.skey, c->sort2.skey_alt, c->sort2.sidx, c->sort2.sidx_alt,
                   c->sum_tile, c->sum_cnt, c->sum_piece};
     for (void* p : vp)
         if (p) (void)hipFree(p);
@@ -134,6 +136,9 @@ void free_all(bhx_ctx* c) {
         if (e) (void)hipEventDestroy(e);
     if (c->ev_drift) (void)hipEventDestroy(c->ev_drift);
     if (c->ev_bounds) (void)hipEventDestroy(c->ev_bounds);
+    if (c->ev_fork) (void)hipEventDestroy(c->ev_fork);
+    if (c->ev_tree) (void)hipEventDestroy(c->ev_tree);
+    if (c->s_tree) (void)hipStreamDestroy(c->s_tree);
     if (c->s_main) (void)hipStreamDestroy(c->s_main);
     if (c->s_side) (void)hipStreamDestroy(c->s_side);
     if (c->s_adj) (void)hipStreamDestroy(c->s_adj);
@@ -216,6 +221,10 @@ int set_layout(bhx_ctx* c, const std::vector<uint32_t>& off) {
             HIPCHK(dalloc(&c->skey_alt, c->cap));
             HIPCHK(dalloc(&c->sidx, c->cap));
             HIPCHK(dalloc(&c->sidx_alt, c->cap));
+            HIPCHK(dalloc(&c->sort2.skey, c->cap));
+            HIPCHK(dalloc(&c->sort2.skey_alt, c->cap));
+            HIPCHK(dalloc(&c->sort2.sidx, c->cap));
+            HIPCHK(dalloc(&c->sort2.sidx_alt, c->cap));
         }
         std::vector<uint32_t> id(off[nsys]);
         for (uint32_t k = 0; k < nsys; ++k)
@@ -302,6 +311,32 @@ __global__ void k_pick3(const double* __restrict__ x, const double* __restrict__
     }
 }
 
+// The tree branch of a frame on its own stream: path keys of the drifted
+// positions (in the order before the body sort) and their sort.
+hipError_t fork_tree_branch(bhx_ctx* c, const double* b_cur, double dt, hipStream_t s) {
+    hipError_t e = hipEventRecord(c->ev_fork, s);
+    if (e == hipSuccess) e = hipStreamWaitEvent(c->s_tree, c->ev_fork, 0);
+    if (e == hipSuccess) e = launch_drift_treekeys(c, dt, b_cur, c->s_tree);
+    if (e == hipSuccess) e = sort_pairs(c, c->tkeys, c->tperm_old, c->s_tree, &c->sort2);
+    if (e == hipSuccess) e = hipEventRecord(c->ev_tree, c->s_tree);
+    return e;
+}
+
+// The frame's tree order after its kick/drift: joined from the tree branch,
+// or path keys of the new positions sorted in line.
+hipError_t tree_order(bhx_ctx* c, const double* b_cur, bool branch, hipStream_t s) {
+    hipError_t e;
+    if (branch) {
+        e = hipStreamWaitEvent(s, c->ev_tree, 0);
+        if (e == hipSuccess) e = launch_tree_order(c, s);
+        return e;
+    }
+    e = launch_treekeys(c->px, c->py, c->pz, c->n, c->sys().id, b_cur, c->cfg_tscale(), c->tkeys, c->tperm, s);
+    if (e == hipSuccess) e = sort_pairs(c, c->tkeys, c->tperm, s);
+    if (e == hipSuccess) e = launch_tree_gather(c, s);
+    return e;
+}
+
 // One frame in the reference order with no host wait: the tree is built into
 // the arrays it has (an overflow is flagged on the device, the walk then does
 // nothing), and the side branch computing the next bounds is joined at the
@@ -312,9 +347,13 @@ int enqueue_step_async(bhx_ctx* c, hipStream_t s, bool linear) {
     double* b_next = c->bounds_slot(cur ^ 1);
     const SysView sv = c->sys();
     const double dt = c->cfg.dt;
+    // accelerations that move with their bodies make the drift independent of
+    // the body sort: the tree branch then runs beside it
+    const bool branch = !linear && c->cfg.permute_acc;
+    if (branch) HIPCHK(fork_tree_branch(c, b_cur, dt, s));
     HIPCHK(launch_morton(c->px, c->py, c->pz, c->n, sv.id, b_cur, c->keys, c->vals, s, &c->ctr->nonfinite));
     HIPCHK(sort_pairs(c, c->keys, c->vals, s));
-    HIPCHK(launch_permute_kick_drift(c, c->vals, dt, s));
+    HIPCHK(launch_permute_kick_drift(c, c->vals, dt, s, branch ? c->inv : nullptr));
     if (linear) {
         HIPCHK(launch_bounds(c, b_next, s));
     } else {
@@ -323,9 +362,7 @@ int enqueue_step_async(bhx_ctx* c, hipStream_t s, bool linear) {
         HIPCHK(launch_bounds(c, b_next, c->s_side));
         HIPCHK(hipEventRecord(c->ev_bounds, c->s_side));
     }
-    HIPCHK(launch_treekeys(c->px, c->py, c->pz, c->n, sv.id, b_cur, c->cfg_tscale(), c->tkeys, c->tperm, s));
-    HIPCHK(sort_pairs(c, c->tkeys, c->tperm, s));
-    HIPCHK(launch_tree_gather(c, s));
+    HIPCHK(tree_order(c, b_cur, branch, s));
     size_t total = 0;
     HIPCHK(build_tree(c, s, &total, false));
     HIPCHK(launch_walk(c, s));
@@ -539,8 +576,11 @@ int bhx_create(const bhx_config* cfg, size_t capacity, bhx_ctx** out) {
         chk(hipStreamCreateWithFlags(&c->s_main, hipStreamNonBlocking), "stream");
         chk(hipStreamCreateWithFlags(&c->s_side, hipStreamNonBlocking), "stream");
         chk(hipStreamCreateWithFlags(&c->s_adj, hipStreamNonBlocking), "stream");
+        chk(hipStreamCreateWithFlags(&c->s_tree, hipStreamNonBlocking), "stream");
         chk(hipEventCreateWithFlags(&c->ev_drift, hipEventDisableTiming), "event");
         chk(hipEventCreateWithFlags(&c->ev_bounds, hipEventDisableTiming), "event");
+        chk(hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming), "event");
+        chk(hipEventCreateWithFlags(&c->ev_tree, hipEventDisableTiming), "event");
         for (hipEvent_t& e : c->ev) chk(hipEventCreate(&e), "event");
         const size_t n = capacity;
         double** dp[] = {&c->px, &c->py, &c->pz, &c->vx, &c->vy, &c->vz, &c->m, &c->ax, &c->ay, &c->az,
@@ -568,6 +608,13 @@ int bhx_create(const bhx_config* cfg, size_t capacity, bhx_ctx** out) {
         chk(dalloc(&c->sort_hist, 8 * 256), "alloc sort");
         chk(dalloc(&c->sort_tile_ctr, 8), "alloc sort");
         chk(dalloc(&c->sort_status, 8 * 256 * (c->sort_tiles + 1)), "alloc sort");
+        chk(dalloc(&c->tperm_old, n), "alloc perm");
+        chk(dalloc(&c->inv, n), "alloc perm");
+        chk(dalloc(&c->sort2.keys_alt, n), "alloc keys");
+        chk(dalloc(&c->sort2.vals_alt, n), "alloc vals");
+        chk(dalloc(&c->sort2.hist, 8 * 256), "alloc sort");
+        chk(dalloc(&c->sort2.tile_ctr, 8), "alloc sort");
+        chk(dalloc(&c->sort2.status, 8 * 256 * (c->sort_tiles + 1)), "alloc sort");
         c->sum_tiles_cap = sum_tiles_for(n);
         chk(dalloc(&c->sum_tile, 6 * c->sum_tiles_cap), "alloc bounds");
         chk(dalloc(&c->sum_cnt, 6 * c->sum_tiles_cap + 8), "alloc bounds");
@@ -1181,21 +1228,22 @@ int bhx_step(bhx_ctx* c, int nsteps) {
             HIPCHK(hipStreamWaitEvent(s, c->ev_bounds, 0));
         }
         if (prof) HIPCHK(hipEventRecord(c->ev[1], s));
+        // the tree on x_{f+1} with the bounds of x_f: beside the body sort when
+        // the accelerations move with their bodies
+        const bool branch = c->cfg.permute_acc != 0;
+        if (branch) HIPCHK(fork_tree_branch(c, b_cur, dt, s));
         HIPCHK(launch_morton(c->px, c->py, c->pz, c->n, sv.id, b_cur, c->keys, c->vals, s, &c->ctr->nonfinite));
         if (prof) HIPCHK(hipEventRecord(c->ev[2], s));
         HIPCHK(sort_pairs(c, c->keys, c->vals, s));
         if (prof) HIPCHK(hipEventRecord(c->ev[3], s));
-        HIPCHK(launch_permute_kick_drift(c, c->vals, dt, s));
+        HIPCHK(launch_permute_kick_drift(c, c->vals, dt, s, branch ? c->inv : nullptr));
         HIPCHK(hipEventRecord(c->ev_drift, s));
         if (prof) HIPCHK(hipEventRecord(c->ev[4], s));
         HIPCHK(hipStreamWaitEvent(c->s_side, c->ev_drift, 0));
         HIPCHK(launch_bounds(c, b_next, c->s_side));
         HIPCHK(hipEventRecord(c->ev_bounds, c->s_side));
-        // tree on x_{f+1} with the bounds of x_f
         c->tree_valid = c->tree_built = false;
-        HIPCHK(launch_treekeys(c->px, c->py, c->pz, c->n, sv.id, b_cur, c->cfg_tscale(), c->tkeys, c->tperm, s));
-        HIPCHK(sort_pairs(c, c->tkeys, c->tperm, s));
-        HIPCHK(launch_tree_gather(c, s));
+        HIPCHK(tree_order(c, b_cur, branch, s));
         if (prof) HIPCHK(hipEventRecord(c->ev[5], s));
         size_t total = 0;
         for (int attempt = 0; attempt < 2; ++attempt) {

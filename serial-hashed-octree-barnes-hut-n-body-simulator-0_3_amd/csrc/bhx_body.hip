@@ -682,6 +682,24 @@ __global__ __launch_bounds__(kBlock) void k_morton(const double* __restrict__ px
     }
 }
 
+// Root edge: the bounds size (reference geometry) or twice it (enclosing
+// cube); the factor is a power of two, so both recurrences are exact.
+__device__ __forceinline__ uint64_t path_key(double x, double y, double z, const double* __restrict__ b4,
+                                             double tscale) {
+    double cx = b4[0], cy = b4[1], cz = b4[2], s = b4[3] * tscale;
+    uint64_t k = 1;
+#pragma unroll
+    for (int L = 1; L <= kMaxLevel; ++L) {
+        const unsigned ox = x > cx, oy = y > cy, oz = z > cz;
+        k = (k << 3) | (ox << 2) | (oy << 1) | oz;
+        s = s * 0.5;
+        cx = cx + ((ox ? 1.0 : -1.0) * 0.5) * s;
+        cy = cy + ((oy ? 1.0 : -1.0) * 0.5) * s;
+        cz = cz + ((oz ? 1.0 : -1.0) * 0.5) * s;
+    }
+    return k;
+}
+
 // The descent a body makes through insertBody (HashedOctree.cpp:474-505):
 // octant by strict > against the cell centre (DetermineOctant,
 // OctantUtils.hpp:64-70), child key parent<<3|o (GetChildKey :97-101), child
@@ -695,23 +713,7 @@ __global__ __launch_bounds__(kBlock) void k_treekeys(const double* __restrict__ 
                                                      uint64_t* __restrict__ keys,
                                                      uint32_t* __restrict__ vals) {
     for (size_t i = blockIdx.x * (size_t)kBlock + threadIdx.x; i < n; i += (size_t)gridDim.x * kBlock) {
-        const double* b4 = bounds4 + 4 * (size_t)(sysid ? sysid[i] : 0u);
-        // root edge: the bounds size (reference geometry) or twice it (enclosing
-        // cube); the factor is a power of two, so both recurrences are exact
-        const double c0x = b4[0], c0y = b4[1], c0z = b4[2], s0 = b4[3] * tscale;
-        const double x = px[i], y = py[i], z = pz[i];
-        double cx = c0x, cy = c0y, cz = c0z, s = s0;
-        uint64_t k = 1;
-#pragma unroll
-        for (int L = 1; L <= kMaxLevel; ++L) {
-            const unsigned ox = x > cx, oy = y > cy, oz = z > cz;
-            k = (k << 3) | (ox << 2) | (oy << 1) | oz;
-            s = s * 0.5;
-            cx = cx + ((ox ? 1.0 : -1.0) * 0.5) * s;
-            cy = cy + ((oy ? 1.0 : -1.0) * 0.5) * s;
-            cz = cz + ((oz ? 1.0 : -1.0) * 0.5) * s;
-        }
-        keys[i] = k;
+        keys[i] = path_key(px[i], py[i], pz[i], bounds4 + 4 * (size_t)(sysid ? sysid[i] : 0u), tscale);
         vals[i] = (uint32_t)i;
     }
 }
@@ -758,38 +760,115 @@ __device__ __forceinline__ void kahan_add(double& s, double& c, double inc) {
     s = t;
 }
 
+// One body's kick then drift from its state at index j (acceleration at ja);
+// the frame's x_{f+1} and v_{f+1/2}.
+struct Drifted {
+    double x, y, z, vx, vy, vz, cpx, cpy, cpz, cvx, cvy, cvz;
+};
+template <bool kComp>
+__device__ __forceinline__ Drifted kick_drift_body(const BodyArrays& b, size_t j, size_t ja, double dt) {
+    Drifted d;
+    d.x = b.px[j]; d.y = b.py[j]; d.z = b.pz[j];
+    d.vx = b.vx[j]; d.vy = b.vy[j]; d.vz = b.vz[j];
+    const double ax = b.ax[ja], ay = b.ay[ja], az = b.az[ja];
+    if (!kComp) {
+        kick_drift_one(d.x, d.vx, ax, dt);
+        kick_drift_one(d.y, d.vy, ay, dt);
+        kick_drift_one(d.z, d.vz, az, dt);
+        d.cpx = d.cpy = d.cpz = d.cvx = d.cvy = d.cvz = 0.0;
+    } else {
+        d.cvx = b.cvx[j]; d.cvy = b.cvy[j]; d.cvz = b.cvz[j];
+        d.cpx = b.cpx[j]; d.cpy = b.cpy[j]; d.cpz = b.cpz[j];
+        kahan_add(d.vx, d.cvx, (ax * dt) * 0.5);
+        kahan_add(d.vy, d.cvy, (ay * dt) * 0.5);
+        kahan_add(d.vz, d.cvz, (az * dt) * 0.5);
+        kahan_add(d.x, d.cpx, d.vx * dt);
+        kahan_add(d.y, d.cpy, d.vy * dt);
+        kahan_add(d.z, d.cpz, d.vz * dt);
+    }
+    return d;
+}
+
 // Fused RadixSortBodies gather (Body.hpp:296) + kick/drift (Body.hpp:306-313)
 // for the reference frame order: bodies are reordered by `perm` and moved in
 // one pass.  With permute_acc = 0 the acceleration slot i is not reordered
-// (the as-run behaviour, ofApp.cpp:176,248).
+// (the as-run behaviour, ofApp.cpp:176,248).  `inv` (may be null) receives
+// the inverse permutation: inv[perm[i]] = i.
 template <bool kComp>
 __global__ __launch_bounds__(kBlock) void k_permute_kick_drift(BodyArrays b, const uint32_t* __restrict__ perm,
-                                                               size_t n, double dt, int permute_acc) {
+                                                               size_t n, double dt, int permute_acc,
+                                                               uint32_t* __restrict__ inv) {
     for (size_t i = blockIdx.x * (size_t)kBlock + threadIdx.x; i < n; i += (size_t)gridDim.x * kBlock) {
         const size_t j = perm ? perm[i] : i;
         const size_t ja = permute_acc ? j : i;
-        double x = b.px[j], y = b.py[j], z = b.pz[j];
-        double vx = b.vx[j], vy = b.vy[j], vz = b.vz[j];
-        const double ax = b.ax[ja], ay = b.ay[ja], az = b.az[ja];
-        if (!kComp) {
-            kick_drift_one(x, vx, ax, dt);
-            kick_drift_one(y, vy, ay, dt);
-            kick_drift_one(z, vz, az, dt);
-        } else {
-            double cvx = b.cvx[j], cvy = b.cvy[j], cvz = b.cvz[j];
-            double cpx = b.cpx[j], cpy = b.cpy[j], cpz = b.cpz[j];
-            kahan_add(vx, cvx, (ax * dt) * 0.5);
-            kahan_add(vy, cvy, (ay * dt) * 0.5);
-            kahan_add(vz, cvz, (az * dt) * 0.5);
-            kahan_add(x, cpx, vx * dt);
-            kahan_add(y, cpy, vy * dt);
-            kahan_add(z, cpz, vz * dt);
-            b.ocvx[i] = cvx; b.ocvy[i] = cvy; b.ocvz[i] = cvz;
-            b.ocpx[i] = cpx; b.ocpy[i] = cpy; b.ocpz[i] = cpz;
+        const Drifted d = kick_drift_body<kComp>(b, j, ja, dt);
+        if (kComp) {
+            b.ocvx[i] = d.cvx; b.ocvy[i] = d.cvy; b.ocvz[i] = d.cvz;
+            b.ocpx[i] = d.cpx; b.ocpy[i] = d.cpy; b.ocpz[i] = d.cpz;
         }
-        b.opx[i] = x; b.opy[i] = y; b.opz[i] = z;
-        b.ovx[i] = vx; b.ovy[i] = vy; b.ovz[i] = vz;
+        b.opx[i] = d.x; b.opy[i] = d.y; b.opz[i] = d.z;
+        b.ovx[i] = d.vx; b.ovy[i] = d.vy; b.ovz[i] = d.vz;
         b.om[i] = b.m[j];
+        if (inv) inv[j] = (uint32_t)i;
+    }
+}
+
+// The tree branch of a frame whose accelerations move with their bodies: the
+// path key of every body's drifted position, in the order before the body
+// sort (kick_drift_body gives bitwise the positions k_permute_kick_drift
+// writes), with the identity payload.
+template <bool kComp>
+__global__ __launch_bounds__(kBlock) void k_drift_treekeys(BodyArrays b, size_t n, double dt,
+                                                           const uint32_t* __restrict__ sysid,
+                                                           const double* __restrict__ bounds4, double tscale,
+                                                           uint64_t* __restrict__ keys, uint32_t* __restrict__ vals) {
+    for (size_t j = blockIdx.x * (size_t)kBlock + threadIdx.x; j < n; j += (size_t)gridDim.x * kBlock) {
+        const Drifted d = kick_drift_body<kComp>(b, j, j, dt);
+        keys[j] = path_key(d.x, d.y, d.z, bounds4 + 4 * (size_t)(sysid ? sysid[j] : 0u), tscale);
+        vals[j] = (uint32_t)j;
+    }
+}
+
+// Tree order in the new array order from the tree branch's sort: slot t holds
+// inv[told[t]].  The branch sorted stably from the order before the body
+// sort, so bodies of equal path key sit in that order; the frame's tree order
+// has them in the new array order, so each run of equal keys (rare: one
+// finest cell) is put in ascending new index by the run's first slot.  Then
+// the tree-order copies of positions and masses.
+__global__ __launch_bounds__(kBlock) void k_tree_order(const uint64_t* __restrict__ tkeys,
+                                                       const uint32_t* __restrict__ told,
+                                                       const uint32_t* __restrict__ inv, size_t n,
+                                                       const double* __restrict__ px, const double* __restrict__ py,
+                                                       const double* __restrict__ pz, const double* __restrict__ m,
+                                                       uint32_t* __restrict__ tperm, double* __restrict__ tx,
+                                                       double* __restrict__ ty, double* __restrict__ tz,
+                                                       double* __restrict__ tm) {
+    for (size_t t = blockIdx.x * (size_t)kBlock + threadIdx.x; t < n; t += (size_t)gridDim.x * kBlock) {
+        const uint64_t k = tkeys[t];
+        if (t > 0 && tkeys[t - 1] == k) continue;   // inside a run: its first slot places it
+        size_t e = t + 1;
+        while (e < n && tkeys[e] == k) ++e;
+        if (e > t + 1) {
+            // insertion sort of the run by new index (in place in tperm)
+            for (size_t u = t; u < e; ++u) {
+                const uint32_t v = inv[told[u]];
+                size_t w = u;
+                while (w > t && tperm[w - 1] > v) {
+                    tperm[w] = tperm[w - 1];
+                    --w;
+                }
+                tperm[w] = v;
+            }
+        } else {
+            tperm[t] = inv[told[t]];
+        }
+        for (size_t u = t; u < e; ++u) {
+            const uint32_t j = tperm[u];
+            tx[u] = px[j];
+            ty[u] = py[j];
+            tz[u] = pz[j];
+            tm[u] = m[j];
+        }
     }
 }
 
@@ -930,15 +1009,15 @@ static void swap_alt(bhx_ctx* c, bool comp) {
     }
 }
 
-hipError_t launch_permute_kick_drift(bhx_ctx* c, const uint32_t* perm, double dt, hipStream_t s) {
+hipError_t launch_permute_kick_drift(bhx_ctx* c, const uint32_t* perm, double dt, hipStream_t s, uint32_t* inv) {
     BodyArrays b = body_arrays(c);
     const bool comp = c->cfg.compensated != 0;
     if (comp)
         hipLaunchKernelGGL(k_permute_kick_drift<true>, dim3(grid_for(c->n)), dim3(kBlock), 0, s, b, perm, c->n, dt,
-                           c->cfg.permute_acc);
+                           c->cfg.permute_acc, inv);
     else
         hipLaunchKernelGGL(k_permute_kick_drift<false>, dim3(grid_for(c->n)), dim3(kBlock), 0, s, b, perm, c->n, dt,
-                           c->cfg.permute_acc);
+                           c->cfg.permute_acc, inv);
     hipError_t e = hipGetLastError();
     if (e == hipSuccess) swap_alt(c, comp);
     return e;
@@ -957,6 +1036,24 @@ hipError_t launch_kick(bhx_ctx* c, double dt, hipStream_t s, uint32_t* nonfinite
     else
         hipLaunchKernelGGL(k_kick<false>, dim3(grid_for(c->n)), dim3(kBlock), 0, s, c->vx, c->vy, c->vz, c->ax, c->ay,
                            c->az, c->cvx, c->cvy, c->cvz, c->n, dt, nonfinite);
+    return hipGetLastError();
+}
+
+hipError_t launch_drift_treekeys(bhx_ctx* c, double dt, const double* b4, hipStream_t s) {
+    const BodyArrays b = body_arrays(c);
+    const uint32_t* sysid = c->sys().id;
+    if (c->cfg.compensated)
+        hipLaunchKernelGGL(k_drift_treekeys<true>, dim3(grid_for(c->n)), dim3(kBlock), 0, s, b, c->n, dt, sysid, b4,
+                           c->cfg_tscale(), c->tkeys, c->tperm_old);
+    else
+        hipLaunchKernelGGL(k_drift_treekeys<false>, dim3(grid_for(c->n)), dim3(kBlock), 0, s, b, c->n, dt, sysid, b4,
+                           c->cfg_tscale(), c->tkeys, c->tperm_old);
+    return hipGetLastError();
+}
+
+hipError_t launch_tree_order(bhx_ctx* c, hipStream_t s) {
+    hipLaunchKernelGGL(k_tree_order, dim3(grid_for(c->n)), dim3(kBlock), 0, s, c->tkeys, c->tperm_old, c->inv, c->n,
+                       c->px, c->py, c->pz, c->m, c->tperm, c->tx, c->ty, c->tz, c->tm);
     return hipGetLastError();
 }
 

@@ -154,6 +154,16 @@ __device__ __forceinline__ double pow2_inverse_near(double v) {
 
 }  // namespace bhx
 
+namespace bhx {
+// Working memory of one radix sort (sort_pairs); the segmented second stage
+// (several systems) uses skey..sidx_alt.
+struct SortScratch {
+    uint64_t *keys_alt = nullptr, *skey = nullptr, *skey_alt = nullptr;
+    uint32_t *vals_alt = nullptr, *hist = nullptr, *status = nullptr, *tile_ctr = nullptr;
+    uint32_t *sidx = nullptr, *sidx_alt = nullptr;
+};
+}  // namespace bhx
+
 struct bhx_ctx {
     bhx_config cfg;
     int device = 0;
@@ -164,8 +174,13 @@ struct bhx_ctx {
     // pool: a stream created later can share the main stream's queue, and its
     // kernels then serialise with the main stream's
     hipStream_t s_adj = nullptr;
+    // the tree branch of a frame (path keys of the drifted positions and
+    // their sort) beside the body sort; created after the other three
+    hipStream_t s_tree = nullptr;
     hipEvent_t ev_drift = nullptr;
     hipEvent_t ev_bounds = nullptr;
+    hipEvent_t ev_fork = nullptr;
+    hipEvent_t ev_tree = nullptr;
     hipEvent_t ev[12] = {};
     bool profiling = false;
     bhx_timing timing{};
@@ -189,6 +204,8 @@ struct bhx_ctx {
     double *tx = nullptr, *ty = nullptr, *tz = nullptr, *tm = nullptr;
     uint64_t* tkeys = nullptr;                 // sorted tree path keys
     uint32_t* tperm = nullptr;                 // tree order -> array index
+    uint32_t* tperm_old = nullptr;             // tree order -> array index before the frame's body sort
+    uint32_t* inv = nullptr;                   // array index before the body sort -> after
     float4* tb4 = nullptr;                     // tree order: offset from the leaf's centre of mass, mass
     uint32_t* tleaf = nullptr;                 // tree order: node id of the body's leaf
     // build scratch
@@ -273,6 +290,8 @@ struct bhx_ctx {
     // segmented sort scratch (several systems only)
     uint64_t *skey = nullptr, *skey_alt = nullptr;
     uint32_t *sidx = nullptr, *sidx_alt = nullptr;
+    // the tree branch's own sort scratch (it runs beside the body sort)
+    bhx::SortScratch sort2;
 
     bhx::SysView sys() const { return bhx::SysView{sys_off, nsys > 1 ? sysid : nullptr, nsys}; }
     // root edge / bounds size of the configured cell geometry (exact: 1 or 2)
@@ -342,11 +361,20 @@ hipError_t radix_sort_pairs(uint64_t* keys, uint32_t* vals, uint64_t* keys_alt, 
                             size_t n, uint32_t* hist, uint32_t* status, uint32_t* tile_ctr,
                             Counters* ctr, hipStream_t s, int passes = 8);
 // stable sort of (key, value) pairs within each system's body range
-hipError_t sort_pairs(bhx_ctx* c, uint64_t* keys, uint32_t* vals, hipStream_t s);
+// Stable sort of (key, value) pairs, system by system; `sc` null: the
+// context's main sort scratch.
+hipError_t sort_pairs(bhx_ctx* c, uint64_t* keys, uint32_t* vals, hipStream_t s, const SortScratch* sc = nullptr);
 
 hipError_t launch_gather(double* const* src, double* const* dst, int narr, const uint32_t* perm,
                          size_t n, hipStream_t s);
-hipError_t launch_permute_kick_drift(bhx_ctx* c, const uint32_t* perm, double dt, hipStream_t s);
+hipError_t launch_permute_kick_drift(bhx_ctx* c, const uint32_t* perm, double dt, hipStream_t s,
+                                     uint32_t* inv = nullptr);
+// The tree branch of a frame (accelerations moving with their bodies): path
+// keys of the drifted positions into tkeys, identity into tperm_old, in the
+// order before the body sort; then, after the body sort and kick/drift,
+// launch_tree_order forms tperm in the new order and the tree-order copies.
+hipError_t launch_drift_treekeys(bhx_ctx* c, double dt, const double* b4, hipStream_t s);
+hipError_t launch_tree_order(bhx_ctx* c, hipStream_t s);
 hipError_t launch_kick_drift(bhx_ctx* c, double dt, hipStream_t s);
 // nonfinite (optional): set to 1 when some kicked velocity is not finite
 hipError_t launch_kick(bhx_ctx* c, double dt, hipStream_t s, uint32_t* nonfinite = nullptr);

@@ -26,6 +26,7 @@ constexpr uint32_t kFlagA = 1u << 30;         // tile aggregate available
 constexpr uint32_t kFlagP = 2u << 30;         // inclusive prefix available
 constexpr uint32_t kCountMask = (1u << 30) - 1;
 constexpr uint32_t kSpinLimit = 1u << 22;
+constexpr int kLookWindow = 4;               // look-back statuses read at once
 
 __device__ __forceinline__ void st_status(uint32_t* p, uint32_t v) {
     __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -161,12 +162,29 @@ __global__ __launch_bounds__(kBlock) void k_sort_pass(const uint64_t* __restrict
 
     uint32_t excl = 0;
     if (tile > 0) {
+        // kLookWindow predecessors' statuses in flight at once, read nearest first
         int64_t t = (int64_t)tile - 1;
         uint32_t spins = 0;
         for (;;) {
-            const uint32_t v = ld_status(status + (size_t)t * kBins + tid);
-            const uint32_t flag = v & ~kCountMask;
-            if (flag == 0) {
+            uint32_t v[kLookWindow];
+#pragma unroll
+            for (int k = 0; k < kLookWindow; ++k)
+                v[k] = t - k >= 0 ? ld_status(status + (size_t)(t - k) * kBins + tid) : kFlagP;
+            bool done = false, stall = false;
+#pragma unroll
+            for (int k = 0; k < kLookWindow; ++k) {
+                if (done || stall) continue;
+                const uint32_t flag = v[k] & ~kCountMask;
+                if (flag == 0) {          // not published yet: wait on this one
+                    stall = true;
+                    t -= k;
+                    continue;
+                }
+                excl += v[k] & kCountMask;
+                if (flag == kFlagP || t - k == 0) done = true;
+            }
+            if (done) break;
+            if (stall) {
                 if (++spins > kSpinLimit) {
                     ctr->sort_fail = 1;
                     break;
@@ -174,9 +192,7 @@ __global__ __launch_bounds__(kBlock) void k_sort_pass(const uint64_t* __restrict
                 __builtin_amdgcn_s_sleep(1);
                 continue;
             }
-            excl += v & kCountMask;
-            if (flag == kFlagP || t == 0) break;
-            --t;
+            t -= kLookWindow;
         }
         st_status(st + tid, kFlagP | (excl + tot));
     }
@@ -283,18 +299,25 @@ unsigned grid_n(size_t n) {
 // Several systems: a stable sort by key, then a stable sort by system (LSD
 // order), gives every system's range sorted by key with ties in their prior
 // order — the single-system order, system by system.
-hipError_t sort_pairs(bhx_ctx* c, uint64_t* keys, uint32_t* vals, hipStream_t s) {
+hipError_t sort_pairs(bhx_ctx* c, uint64_t* keys, uint32_t* vals, hipStream_t s, const SortScratch* sc) {
+    SortScratch m;
+    if (!sc) {
+        m.keys_alt = c->keys_alt; m.vals_alt = c->vals_alt;
+        m.hist = c->sort_hist; m.status = c->sort_status; m.tile_ctr = c->sort_tile_ctr;
+        m.skey = c->skey; m.skey_alt = c->skey_alt; m.sidx = c->sidx; m.sidx_alt = c->sidx_alt;
+        sc = &m;
+    }
     const size_t n = c->n;
-    hipError_t e = radix_sort_pairs(keys, vals, c->keys_alt, c->vals_alt, n, c->sort_hist, c->sort_status,
-                                    c->sort_tile_ctr, c->ctr, s);
+    hipError_t e = radix_sort_pairs(keys, vals, sc->keys_alt, sc->vals_alt, n, sc->hist, sc->status, sc->tile_ctr,
+                                    c->ctr, s);
     if (e != hipSuccess || c->nsys <= 1 || n <= 1) return e;
-    hipLaunchKernelGGL(k_sys_keys, dim3(grid_n(n)), dim3(kBlock), 0, s, vals, c->sysid, n, c->skey, c->sidx);
-    e = radix_sort_pairs(c->skey, c->sidx, c->skey_alt, c->sidx_alt, n, c->sort_hist, c->sort_status,
-                         c->sort_tile_ctr, c->ctr, s, 2);
+    hipLaunchKernelGGL(k_sys_keys, dim3(grid_n(n)), dim3(kBlock), 0, s, vals, c->sysid, n, sc->skey, sc->sidx);
+    e = radix_sort_pairs(sc->skey, sc->sidx, sc->skey_alt, sc->sidx_alt, n, sc->hist, sc->status, sc->tile_ctr, c->ctr,
+                         s, 2);
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(k_sys_gather, dim3(grid_n(n)), dim3(kBlock), 0, s, keys, vals, c->sidx, n, c->keys_alt,
-                       c->vals_alt);
-    hipLaunchKernelGGL(k_copy_pairs, dim3(grid_n(n)), dim3(kBlock), 0, s, c->keys_alt, c->vals_alt, n, keys, vals);
+    hipLaunchKernelGGL(k_sys_gather, dim3(grid_n(n)), dim3(kBlock), 0, s, keys, vals, sc->sidx, n, sc->keys_alt,
+                       sc->vals_alt);
+    hipLaunchKernelGGL(k_copy_pairs, dim3(grid_n(n)), dim3(kBlock), 0, s, sc->keys_alt, sc->vals_alt, n, keys, vals);
     return hipGetLastError();
 }
 
