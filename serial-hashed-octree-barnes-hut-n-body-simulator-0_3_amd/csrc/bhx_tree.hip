@@ -126,8 +126,13 @@ __global__ __launch_bounds__(kBlock) void k_scan_down(const uint32_t* __restrict
 // counting half of the insertion loop, HashedOctree.cpp:474-505) plus the
 // root when the body opens its system.  Windows and shared prefixes never
 // reach across a system boundary.
+// Also clears the octant masks of the first `nzero` node slots for the
+// build's atomic ORs (the grid covers them: about twice the bodies).
 __global__ __launch_bounds__(kBlock) void k_count(const uint64_t* __restrict__ keys, uint32_t n, int B, SysView sv,
-                                                  uint32_t* __restrict__ nn, uint32_t* __restrict__ nlo) {
+                                                  uint32_t* __restrict__ nn, uint32_t* __restrict__ nlo,
+                                                  uint32_t* __restrict__ child_byte, size_t nzero) {
+    for (size_t z = blockIdx.x * (size_t)kBlock + threadIdx.x; z < nzero; z += (size_t)gridDim.x * kBlock)
+        child_byte[z] = 0u;
     __shared__ uint64_t s_key[kBlock + 2 * kMaxLeafCap];
     __shared__ uint32_t s_sys[kBlock + 2 * kMaxLeafCap];
     __shared__ int8_t s_w[kBlock + kMaxLeafCap];
@@ -326,14 +331,35 @@ __global__ __launch_bounds__(kBlock) void k_emit_links(const uint64_t* __restric
 
 // Total node count against the capacity: the clamped count bounds every
 // later kernel of the build, the overflow flags tell the host to grow.
-__global__ void k_check_cap(uint32_t cap, Counters* __restrict__ ctr) {
-    if (threadIdx.x != 0) return;
-    const uint32_t total = ctr->total_nodes;
-    const bool over = total > cap;
-    ctr->node_overflow = over ? 1u : 0u;
-    if (over) ctr->overflow_seen = 1u;
-    if (total > ctr->max_nodes) ctr->max_nodes = total;
-    ctr->nodes_kept = over ? cap : total;
+// FP32 walk factors of system k: lengths from the tree's root edge, masses
+// from the system's first body (tree order), both powers of two.
+__device__ __forceinline__ void scale32_of(uint32_t k, const double* __restrict__ bounds4, double tscale,
+                                           const SysView& sv, const double* __restrict__ tm,
+                                           double* __restrict__ scale) {
+    const uint32_t first = sv.off[k];
+    scale[2 * (size_t)k] = pow2_inverse_near(bounds4[4 * (size_t)k + 3] * tscale);
+    scale[2 * (size_t)k + 1] = sv.off[k + 1] > first ? pow2_inverse_near(fabs(tm[first])) : 1.0;
+}
+
+// The node count against the capacity (the clamped count bounds every later
+// kernel of the build, the overflow flags tell the host to grow), the
+// build's counters and per-level straddler counts cleared, and the FP32 walk
+// factors of every system: the build's small jobs in one launch.
+__global__ void k_build_setup(uint32_t cap, Counters* __restrict__ ctr, uint32_t* __restrict__ nrest, int nlevels,
+                              const double* __restrict__ bounds4, double tscale, SysView sv,
+                              const double* __restrict__ tm, double* __restrict__ scale) {
+    if (threadIdx.x == 0) {
+        const uint32_t total = ctr->total_nodes;
+        const bool over = total > cap;
+        ctr->node_overflow = over ? 1u : 0u;
+        if (over) ctr->overflow_seen = 1u;
+        if (total > ctr->max_nodes) ctr->max_nodes = total;
+        ctr->nodes_kept = over ? cap : total;
+        ctr->unsplit = 0u;
+        ctr->hash_fail = 0u;
+    }
+    for (int l = threadIdx.x; l < nlevels; l += blockDim.x) nrest[l] = 0u;
+    for (uint32_t k = threadIdx.x; k < sv.nsys; k += blockDim.x) scale32_of(k, bounds4, tscale, sv, tm, scale);
 }
 
 __global__ void k_fill_u32(uint32_t* __restrict__ p, size_t n, uint32_t v) {
@@ -445,39 +471,72 @@ struct ChildFromSlice {
     __device__ double quad(uint32_t i, int k) const { return q[i - b0][k]; }
 };
 
+constexpr int kLeafBatch = 8;    // leaf bodies loaded at once by the moments
+
+// What combining node p reads of its own records, loaded in one batch
+// (ahead of the level loop in the slice kernel: none of it depends on the
+// children).
+struct NodeInfo {
+    uint32_t L, st, cnt, cb, skip;
+    uint32_t ch[8];          // child id by octant (where cb has the bit)
+    double lam, mu;          // the system's FP32 factors
+    double b4[4];            // the system's root bounds
+};
+
+__device__ __forceinline__ NodeInfo node_info(uint32_t p, const TreeArgs& t, const SysView& sv,
+                                              const double* __restrict__ bounds4) {
+    NodeInfo I;
+    I.L = t.level[p];
+    I.st = t.start[p];
+    I.cnt = t.count[p];
+    I.cb = t.child_byte[p];
+    I.skip = t.skip[p];
+    const uint4 ca = *reinterpret_cast<const uint4*>(t.child + 8 * (size_t)p);
+    const uint4 cb4 = *reinterpret_cast<const uint4*>(t.child + 8 * (size_t)p + 4);
+    I.ch[0] = ca.x; I.ch[1] = ca.y; I.ch[2] = ca.z; I.ch[3] = ca.w;
+    I.ch[4] = cb4.x; I.ch[5] = cb4.y; I.ch[6] = cb4.z; I.ch[7] = cb4.w;
+    const uint32_t sysn = sys_of(sv, I.st);
+    I.lam = t.scale32[2 * (size_t)sysn];
+    I.mu = t.scale32[2 * (size_t)sysn + 1];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) I.b4[k] = bounds4[4 * (size_t)sysn + k];
+    return I;
+}
+
 template <class Child>
-__device__ Payload moments_of(uint32_t p, int intended, const TreeArgs& t, const double* __restrict__ bounds4,
-                              const SysView& sv, const double* __restrict__ px, const double* __restrict__ py,
+__device__ Payload moments_of(uint32_t p, const NodeInfo& I, int intended, const TreeArgs& t,
+                              const double* __restrict__ px, const double* __restrict__ py,
                               const double* __restrict__ pz, const double* __restrict__ tx,
                               const double* __restrict__ ty, const double* __restrict__ tz,
                               const double* __restrict__ tm, const uint32_t* __restrict__ tperm,
                               const Child& ch) {
     {
-        const int L = (int)t.level[p];
-        const uint32_t st = t.start[p], cnt = t.count[p], cb = t.child_byte[p];
+        const int L = (int)I.L;
+        const uint32_t st = I.st, cnt = I.cnt, cb = I.cb;
         const bool leaf = cb == 0;
         double M = 0.0, sx = 0.0, sy = 0.0, sz = 0.0;
         double q[6] = {0, 0, 0, 0, 0, 0};
         uint32_t first = 0xFFFFFFFFu;
         double cx, cy, cz;
         // the FP32 copies are in the system's scaled units (exact powers of two)
-        const uint32_t sysn = sys_of(sv, st);
-        const double lam = t.scale32[2 * (size_t)sysn], mu = t.scale32[2 * (size_t)sysn + 1];
+        const double lam = I.lam, mu = I.mu;
         bool bodies_done = false;   // the leaf's FP32 body copies already written
         if (leaf) {
-            // bodies in batches of 8 with every load of a batch in flight before
-            // the (sequential, in body order) sums use them
+            // bodies in batches of kLeafBatch with every load of a batch in flight
+            // before the (sequential, in body order) sums use them; a leaf of at
+            // most one batch keeps its bodies in registers for the second pass
             const uint32_t e = st + cnt;
-            for (uint32_t j0 = st; j0 < e; j0 += 8) {
-                double mj[8], xj[8], yj[8], zj[8];
-                uint32_t aj[8];
+            const bool one = cnt <= (uint32_t)kLeafBatch;
+            double mj[kLeafBatch], xj[kLeafBatch], yj[kLeafBatch], zj[kLeafBatch];
+            for (uint32_t j0 = st; j0 < e; j0 += kLeafBatch) {
+                uint32_t aj[kLeafBatch];
 #pragma unroll
-                for (int k = 0; k < 8; ++k) {
+                for (int k = 0; k < kLeafBatch; ++k) {
                     const uint32_t j = j0 + k < e ? j0 + k : j0;
                     mj[k] = tm[j]; xj[k] = tx[j]; yj[k] = ty[j]; zj[k] = tz[j]; aj[k] = tperm[j];
                 }
 #pragma unroll
-                for (int k = 0; k < 8; ++k) {
+                for (int k = 0; k < kLeafBatch; ++k) {
                     if (j0 + k >= e) break;
                     M += mj[k];
                     sx += mj[k] * xj[k];
@@ -493,15 +552,16 @@ __device__ Payload moments_of(uint32_t p, int intended, const TreeArgs& t, const
             }
             if (intended && cnt > 1) {
                 // quadrupole and the FP32 body copies from one pass over the bodies
-                for (uint32_t j0 = st; j0 < e; j0 += 8) {
-                    double mj[8], xj[8], yj[8], zj[8];
+                for (uint32_t j0 = st; j0 < e; j0 += kLeafBatch) {
+                    if (!one) {
 #pragma unroll
-                    for (int k = 0; k < 8; ++k) {
-                        const uint32_t j = j0 + k < e ? j0 + k : j0;
-                        mj[k] = tm[j]; xj[k] = tx[j]; yj[k] = ty[j]; zj[k] = tz[j];
+                        for (int k = 0; k < kLeafBatch; ++k) {
+                            const uint32_t j = j0 + k < e ? j0 + k : j0;
+                            mj[k] = tm[j]; xj[k] = tx[j]; yj[k] = ty[j]; zj[k] = tz[j];
+                        }
                     }
 #pragma unroll
-                    for (int k = 0; k < 8; ++k) {
+                    for (int k = 0; k < kLeafBatch; ++k) {
                         if (j0 + k >= e) break;
                         const double dx = xj[k] - cx, dy = yj[k] - cy, dz = zj[k] - cz;
                         quad_add(q, mj[k], dx, dy, dz);
@@ -516,9 +576,7 @@ __device__ Payload moments_of(uint32_t p, int intended, const TreeArgs& t, const
             // children in octant order = pre-order: the sums keep their order; the
             // child ids come from the octant table, so the child records are
             // fetched independently rather than along the pre-order skip chain
-            const uint4 ca = *reinterpret_cast<const uint4*>(t.child + 8 * (size_t)p);
-            const uint4 cb4 = *reinterpret_cast<const uint4*>(t.child + 8 * (size_t)p + 4);
-            const uint32_t ids0[8] = {ca.x, ca.y, ca.z, ca.w, cb4.x, cb4.y, cb4.z, cb4.w};
+            const uint32_t* ids0 = I.ch;
             if constexpr (Child::kPreload) {
                 // every child's record is loaded before any sum uses it (an absent
                 // octant reads the node's own slot, never used), so the loads
@@ -597,14 +655,14 @@ __device__ Payload moments_of(uint32_t p, int intended, const TreeArgs& t, const
             t.quad[6 * (size_t)p + k] = q[k];
             t.quadf[6 * (size_t)p + k] = (float)(q[k] * qs);
         }
-        const double* b4n = bounds4 + 4 * (size_t)sysn;
+        const double* b4n = I.b4;
         double s = b4n[3] * t.tscale;
         for (int l = 0; l < L; ++l) s = s * 0.5;
         WalkNode w;
         w.cx = cx; w.cy = cy; w.cz = cz;
         w.mass = M;
         w.size2 = s * s;
-        w.skip = t.skip[p];
+        w.skip = I.skip;
         w.start = st;
         w.count = cnt;
         w.level_leaf = (uint32_t)L | (leaf ? 256u : 0u);
@@ -616,7 +674,7 @@ __device__ Payload moments_of(uint32_t p, int intended, const TreeArgs& t, const
         v32.cy = (float)((cy - b4n[1]) * lam);
         v32.cz = (float)((cz - b4n[2]) * lam);
         v32.meta = (uint32_t)L | (leaf ? 32u : 0u) | ((cnt < (1u << 26) ? cnt : (1u << 26) - 1) << 6);
-        v32.skip = t.skip[p];
+        v32.skip = I.skip;
         v32.start = st;
         v32.child_byte = cb;
         v32.size2 = (float)(s * s * lam2);
@@ -688,16 +746,18 @@ __global__ __launch_bounds__(kBlock) void k_moments_local(const Counters* __rest
     __syncthreads();
     uint32_t L = 0;
     bool local = false, leaf = false;
+    NodeInfo info{};
     if (valid) {
-        L = t.level[p];
-        local = t.skip[p] <= b1;   // always for a leaf: its subtree is itself
-        leaf = t.child_byte[p] == 0u;
+        info = node_info(p, t, sv, bounds4);
+        L = info.L;
+        local = info.skip <= b1;   // always for a leaf: its subtree is itself
+        leaf = info.cb == 0u;
         if (local && !leaf) atomicOr(&s_levels, 1u << L);
         else if (!local) rest[L * (size_t)gridDim.x + atomicAdd(&nrest[L], 1u)] = p;
     }
     const ChildFromSlice slice{s_m, s_c, s_f, s_q, b0};
     auto combine = [&]() {
-        const Payload r = moments_of(p, intended, t, bounds4, sv, px, py, pz, tx, ty, tz, tm, tperm, slice);
+        const Payload r = moments_of(p, info, intended, t, px, py, pz, tx, ty, tz, tm, tperm, slice);
         const uint32_t i = threadIdx.x;
         s_m[i] = r.M;
         s_c[i][0] = r.cx; s_c[i][1] = r.cy; s_c[i][2] = r.cz;
@@ -723,25 +783,38 @@ __global__ __launch_bounds__(256) void k_moments_rest(const Counters* __restrict
                                                        const uint32_t* __restrict__ rest,
                                                        const uint32_t* __restrict__ nrest, uint32_t slices) {
     if (ctr->node_overflow) return;
-    for (int l = kMaxLevel; l >= 0; --l) {
-        const uint32_t m = nrest[l];
-        if (m == 0) continue;   // block-uniform
-        for (uint32_t i = threadIdx.x; i < m; i += blockDim.x)
-            (void)moments_of(rest[l * (size_t)slices + i], intended, t, bounds4, sv, px, py, pz, tx, ty, tz, tm, tperm,
-                             ChildFromGlobal{t});
-        __syncthreads();
+    // a level's node records do not depend on the level below: each thread's
+    // first node of the next non-empty level is fetched while this one combines
+    int l = kMaxLevel;
+    while (l >= 0 && nrest[l] == 0) --l;
+    uint32_t pn = 0;
+    NodeInfo next{};
+    if (l >= 0 && threadIdx.x < nrest[l]) {
+        pn = rest[l * (size_t)slices + threadIdx.x];
+        next = node_info(pn, t, sv, bounds4);
     }
-}
-
-// FP32 walk factors of each system: lengths from the tree's root edge, masses
-// from the system's first body (tree order), both powers of two.
-__global__ void k_scale32(const double* __restrict__ bounds4, double tscale, SysView sv,
-                          const double* __restrict__ tm, double* __restrict__ scale) {
-    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= sv.nsys) return;
-    const uint32_t first = sv.off[k];
-    scale[2 * (size_t)k] = pow2_inverse_near(bounds4[4 * (size_t)k + 3] * tscale);
-    scale[2 * (size_t)k + 1] = sv.off[k + 1] > first ? pow2_inverse_near(fabs(tm[first])) : 1.0;
+    while (l >= 0) {
+        const uint32_t m = nrest[l];
+        const uint32_t p0 = pn;
+        const NodeInfo info0 = next;
+        int ln = l - 1;
+        while (ln >= 0 && nrest[ln] == 0) --ln;
+        if (ln >= 0 && threadIdx.x < nrest[ln]) {
+            pn = rest[ln * (size_t)slices + threadIdx.x];
+            next = node_info(pn, t, sv, bounds4);
+        }
+        for (uint32_t i = threadIdx.x; i < m; i += blockDim.x) {
+            if (i == threadIdx.x) {
+                (void)moments_of(p0, info0, intended, t, px, py, pz, tx, ty, tz, tm, tperm, ChildFromGlobal{t});
+            } else {
+                const uint32_t p = rest[l * (size_t)slices + i];
+                (void)moments_of(p, node_info(p, t, sv, bounds4), intended, t, px, py, pz, tx, ty, tz, tm, tperm,
+                                 ChildFromGlobal{t});
+            }
+        }
+        __syncthreads();
+        l = ln;
+    }
 }
 
 TreeArgs tree_args(bhx_ctx* c) {
@@ -790,11 +863,13 @@ hipError_t build_tree(bhx_ctx* c, hipStream_t s, size_t* total_out, bool sync) {
     c->tree_slot = c->bslot;
     c->tree_tscale = c->cfg_tscale();
     uint32_t* total_dev = &c->ctr->total_nodes;
-    hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, s, &c->ctr->unsplit, (size_t)2, 0u);  // unsplit, hash_fail
-    hipLaunchKernelGGL(k_count, dim3(grid_for(n)), dim3(kBlock), 0, s, c->tkeys, n, B, sv, c->nn, c->nlo);
+    uint32_t* nrest = c->tree.lvl;   // per-level counts of slice-straddling nodes
+    hipLaunchKernelGGL(k_count, dim3(grid_for(n)), dim3(kBlock), 0, s, c->tkeys, n, B, sv, c->nn, c->nlo,
+                       c->tree.child_byte, (size_t)cap);
     hipError_t e = scan_exclusive_u32(c->nn, c->noff, n, c->scan_tmp, total_dev, s);
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(k_check_cap, dim3(1), dim3(64), 0, s, cap, c->ctr);
+    hipLaunchKernelGGL(k_build_setup, dim3(1), dim3(kBlock), 0, s, cap, c->ctr, nrest, kMaxLevel + 1, b4,
+                       c->tree_tscale, sv, c->tm, c->sys_scale);
     size_t span = cap;
     if (sync) {
         e = hipMemcpyAsync(c->ctr_host, c->ctr, sizeof(Counters), hipMemcpyDeviceToHost, s);
@@ -809,7 +884,6 @@ hipError_t build_tree(bhx_ctx* c, hipStream_t s, size_t* total_out, bool sync) {
         *total_out = 0;
     }
     TreeArgs t = tree_args(c);
-    hipLaunchKernelGGL(k_fill_u32, dim3(grid_for(span)), dim3(kBlock), 0, s, c->tree.child_byte, span, 0u);
     hipLaunchKernelGGL(k_emit, dim3(grid_for(n)), dim3(kBlock), 0, s, c->tkeys, n, B, sv, c->nn, c->nlo, c->noff, b4,
                        cap, t, c->sys_root, c->ctr);
     hipLaunchKernelGGL(k_emit_links, dim3(grid_for(span)), dim3(kBlock), 0, s, c->tkeys, n, B, sv, c->nlo, c->noff,
@@ -818,12 +892,8 @@ hipError_t build_tree(bhx_ctx* c, hipStream_t s, size_t* total_out, bool sync) {
     const int intended = c->cfg.mode == BHX_MODE_INTENDED;
     // per-level lists of slice-straddling nodes: (kMaxLevel + 1) x slices
     // entries, within the node capacity (slices = capacity / 256, rounded up)
-    uint32_t* nrest = c->tree.lvl;
     const unsigned slices = grid_for(span, kBlock, 0xFFFFFFFFu);
     if ((size_t)(kMaxLevel + 1) * slices > c->tree.cap) return hipErrorInvalidValue;
-    hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, s, nrest, (size_t)(kMaxLevel + 1), 0u);
-    hipLaunchKernelGGL(k_scale32, dim3((sv.nsys + kBlock - 1) / kBlock), dim3(kBlock), 0, s, b4, c->tree_tscale, sv,
-                       c->tm, c->sys_scale);
     hipLaunchKernelGGL(k_moments_local, dim3(slices), dim3(kBlock), 0, s, c->ctr, sv, intended, t, b4, c->px, c->py,
                        c->pz, c->tx, c->ty, c->tz, c->tm, c->tperm, c->tree.by_level, nrest);
     hipLaunchKernelGGL(k_moments_rest, dim3(1), dim3(256), 0, s, c->ctr, sv, intended, t, b4, c->px, c->py, c->pz,

@@ -389,13 +389,27 @@ __device__ __forceinline__ float dist2(float dx, float dy, float dz) { return dx
 // kPot: also the softened potential phi -= M/r + (d.Q.d) / (2 r^5) (a = -grad
 // phi for the force above); an entry at zero separation (the body itself)
 // adds no potential.
+// r^2 = |d|^2 + eps^2 of a list term: three fused multiply-adds, or, where the
+// zero-separation test needs |d|^2 itself (no softening, or the potential),
+// |d|^2 as the opening test forms it plus eps^2.
+template <typename T, bool kNeedD2>
+__device__ __forceinline__ void softened_r2(T dx, T dy, T dz, T eps2, T& d2, T& r2) {
+    if constexpr (kNeedD2) {
+        d2 = dist2(dx, dy, dz);
+        r2 = d2 + eps2;
+    } else {
+        r2 = fma(dz, dz, fma(dy, dy, fma(dx, dx, eps2)));
+        d2 = r2;   // not read
+    }
+}
+
 template <typename T, bool kQuad, bool kGuard, bool kPot = false>
 __device__ __forceinline__ void eval_nodes(const T (*nd)[4], const T (*q)[6], int nn, T xr, T yr, T zr, T eps2,
                                            T& ax, T& ay, T& az, T& phi) {
     for (int k = 0; k < nn; ++k) {
         const T dx = nd[k][0] - xr, dy = nd[k][1] - yr, dz = nd[k][2] - zr;
-        const T d2 = dist2(dx, dy, dz);
-        const T r2 = d2 + eps2;
+        T d2, r2;
+        softened_r2<T, kGuard || kPot>(dx, dy, dz, eps2, d2, r2);
         T inv = rsqrt_t<T>(r2);
         T inv2 = inv_sq_t<T>(inv, r2);
         if (kGuard) {   // both factors: in FP64 1/r2 comes from r2, which is 0 here
@@ -430,8 +444,8 @@ __device__ __forceinline__ void eval_bodies(const T (*bd)[4], int nb, T xr, T yr
                                             T& az, T& phi) {
     for (int k = 0; k < nb; ++k) {
         const T dx = bd[k][0] - xr, dy = bd[k][1] - yr, dz = bd[k][2] - zr;
-        const T d2 = dist2(dx, dy, dz);
-        const T r2 = d2 + eps2;
+        T d2, r2;
+        softened_r2<T, kGuard || kPot>(dx, dy, dz, eps2, d2, r2);
         T inv = rsqrt_t<T>(r2);
         T inv2 = inv_sq_t<T>(inv, r2);
         if (kGuard) {   // both factors: in FP64 1/r2 comes from r2, which is 0 here
