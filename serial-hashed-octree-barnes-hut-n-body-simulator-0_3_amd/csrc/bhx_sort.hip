@@ -5,7 +5,9 @@
 // offsets, which amounts to a stable full sort by the 64-bit key; the same
 // order comes out of eight stable passes over disjoint bytes.
 //
-// Each pass is a single "onesweep" kernel (Adinets & Merrill 2022): a
+// A pass whose digit is the same for every key (its histogram has one
+// occupied bin) is a plain copy.  Each other pass is a single "onesweep"
+// kernel (Adinets & Merrill 2022): a
 // workgroup ranks a tile of 4096 keys in LDS, publishes its per-digit counts,
 // and finds its global offsets by decoupled look-back (Merrill & Garland
 // 2016) over the tiles before it.  Tile indices come from an atomic counter
@@ -111,6 +113,20 @@ __global__ __launch_bounds__(kBlock) void k_sort_pass(const uint64_t* __restrict
     __shared__ uint32_t s_vals[kTile];
 
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    // a digit every key shares (one occupied bin: its count is n) leaves the
+    // order as it is; the pass is then a copy (keys in a small range, or
+    // systems whose high key bytes agree)
+    {
+        const uint32_t g = goff[pass * kBins + tid];
+        const uint32_t gn = tid + 1 < kBins ? goff[pass * kBins + tid + 1] : (uint32_t)n;
+        if (__syncthreads_count(gn - g == (uint32_t)n) != 0) {
+            for (size_t i = blockIdx.x * (size_t)kBlock + tid; i < n; i += (size_t)gridDim.x * kBlock) {
+                kout[i] = kin[i];
+                vout[i] = vin[i];
+            }
+            return;
+        }
+    }
     if (tid == 0) s_tile = atomicAdd(tile_ctr + pass, 1u);
     for (int j = tid; j < (kBlock / 64) * kBins; j += kBlock) (&s_wcnt[0][0])[j] = 0;
     __syncthreads();

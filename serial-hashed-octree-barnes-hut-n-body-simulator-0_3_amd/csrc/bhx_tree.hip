@@ -584,6 +584,7 @@ __device__ Payload moments_of(uint32_t p, const NodeInfo& I, int intended, const
                 uint32_t ids[8];
                 double mc[8], c0[8], c1[8], c2[8];
                 uint32_t fc[8];
+                double qc[8][6];   // with the other loads: one round trip, not two
 #pragma unroll
                 for (int o = 0; o < 8; ++o) {
                     ids[o] = ((cb >> o) & 1u) ? ids0[o] : p;
@@ -592,6 +593,10 @@ __device__ Payload moments_of(uint32_t p, const NodeInfo& I, int intended, const
                     c1[o] = ch.com(ids[o], 1);
                     c2[o] = ch.com(ids[o], 2);
                     fc[o] = ch.first(ids[o]);
+                    if (intended) {
+#pragma unroll
+                        for (int k = 0; k < 6; ++k) qc[o][k] = ch.quad(ids[o], k);
+                    }
                 }
 #pragma unroll
                 for (int o = 0; o < 8; ++o) {
@@ -606,11 +611,6 @@ __device__ Payload moments_of(uint32_t p, const NodeInfo& I, int intended, const
                     cx = px[first]; cy = py[first]; cz = pz[first];
                 } else {
                     cx = sx / M; cy = sy / M; cz = sz / M;
-                    double qc[8][6];
-#pragma unroll
-                    for (int o = 0; o < 8; ++o)
-#pragma unroll
-                        for (int k = 0; k < 6; ++k) qc[o][k] = ch.quad(ids[o], k);
 #pragma unroll
                     for (int o = 0; o < 8; ++o) {
                         if (!((cb >> o) & 1u)) continue;

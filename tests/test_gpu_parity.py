@@ -130,6 +130,26 @@ def test_sort_keys_stable(bhx, n):
     assert np.array_equal(sk, keys[ref])
 
 
+@pytest.mark.parametrize("case", ["narrow", "equal", "top_only"])
+def test_sort_keys_constant_digits(bhx, case):
+    """Digits that every key shares make their passes plain copies: the result is still the
+    stable sort (numpy) — keys within a narrow range (the high bytes agree), all keys equal
+    (every pass a copy), and keys that differ only in the top byte."""
+    n = 70001
+    rng = np.random.default_rng(7)
+    if case == "narrow":
+        keys = (np.uint64(1 << 63) | np.uint64(0x123456) << np.uint64(24)) + rng.integers(0, 1 << 20, n).astype(np.uint64)
+    elif case == "equal":
+        keys = np.full(n, np.uint64(0x8000000000000abc))
+    else:
+        keys = (rng.integers(0, 256, n).astype(np.uint64) << np.uint64(56)) | np.uint64(0x00ffffffffffff00)
+    with _engine(bhx, n) as e:
+        sk, idx = e.sort_keys(keys)
+    ref = np.argsort(keys, kind="stable")
+    assert np.array_equal(idx.astype(np.int64), ref)
+    assert np.array_equal(sk, keys[ref])
+
+
 def test_sort_bodies_matches_reference_radix(bhx, plummer16k):
     pos, vel, mass = plummer16k
     with _engine(bhx, N1) as e:
