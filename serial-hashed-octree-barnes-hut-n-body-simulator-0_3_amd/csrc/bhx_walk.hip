@@ -1136,6 +1136,7 @@ __global__ __launch_bounds__(64) void k_walk_group32(WalkArgs a) {
 constexpr int kLpStack = 1024;     // LDS stack of cells to test
 constexpr int kLpCells = 128;      // accepted cell ids waiting for evaluation (64 at a time)
 constexpr int kLpBodies = 96;
+constexpr int kLpUnd = 128;        // undecided cells (at most 63 + 64 wait at once)
 
 // The next group for this wave (the same value in every lane), or `groups`.
 __device__ __forceinline__ uint32_t lp_next_group(uint32_t* wq, uint32_t groups, uint32_t lane) {
@@ -1165,6 +1166,26 @@ __device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v, uint32_t lane) {
         const uint32_t t = __shfl_up(v, off, 64);
         if ((int)lane >= off) v += t;
     }
+    return v;
+}
+
+// |d|^2 of the lane-parallel walk's opening test, in one fixed rounding order:
+// fl(fl(fl(dy dy) + dx dx) + dz dz) with fused multiply-adds.  Every place that
+// decides a cell (the exact least |d|^2, the box bound, the sample bodies) uses
+// this one expression, so their values compare exactly.
+__device__ __forceinline__ float lp_d2(float dx, float dy, float dz) {
+    return fmaf(dz, dz, fmaf(dx, dx, dy * dy));
+}
+
+__device__ __forceinline__ float wave_minf(float v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v = fminf(v, __shfl_xor(v, off, 64));
+    return v;
+}
+
+__device__ __forceinline__ float wave_maxf(float v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v = fmaxf(v, __shfl_xor(v, off, 64));
     return v;
 }
 
@@ -1227,6 +1248,7 @@ __global__ __launch_bounds__(64) void k_walk_lp32(WalkArgs a) {
     __shared__ uint32_t s_stack[kLpStack];
     __shared__ uint8_t s_smask[S > 1 ? kLpStack : 1];
     __shared__ uint8_t s_om[S > 1 ? 32 : 1];   // one-cell-at-a-time walk: open mask per level
+    __shared__ uint32_t s_und[S == 1 ? kLpUnd : 1];   // cells the bounds leave undecided
 
     if (*a.abort_flag) return;
     const uint32_t lane = threadIdx.x;
@@ -1453,34 +1475,82 @@ __global__ __launch_bounds__(64) void k_walk_lp32(WalkArgs a) {
             }
         }
         __syncthreads();
-        while (top > 0) {
-            const int k = top < 64 ? top : 64;
-            const int base = top - k;
+        // One wave, one list (S == 1): most cells are decided without the least
+        // |d|^2 over all 64 bodies.  With lo/hi the group's bounding box, the
+        // per-axis distance fl(c - clamp(c, lo, hi)) is at most |fl(c - x_b)| for
+        // every body b (rounding is monotone), so lp_d2 of it is at most every
+        // body's lp_d2: edge^2 < theta^2 d2_box means every body accepts.  A body's
+        // own lp_d2 is at least the least one: edge^2 >= theta^2 d2_b for one of
+        // four sample bodies means some body opens.  Cells neither bound decides
+        // wait in s_und and get the exact least |d|^2, 64 at a time.  The
+        // decisions are the exact rule's; only the order cells join the lists
+        // differs.
+        float blo_x = 0.f, blo_y = 0.f, blo_z = 0.f, bhi_x = 0.f, bhi_y = 0.f, bhi_z = 0.f;
+        float4 smp[4];
+        if constexpr (S == 1) {
+            blo_x = wave_minf(valid ? xs : INFINITY);
+            blo_y = wave_minf(valid ? ys : INFINITY);
+            blo_z = wave_minf(valid ? zs : INFINITY);
+            bhi_x = wave_maxf(valid ? xs : -INFINITY);
+            bhi_y = wave_maxf(valid ? ys : -INFINITY);
+            bhi_z = wave_maxf(valid ? zs : -INFINITY);
+            const int nv = __popcll(vmask);    // the valid bodies are lanes 0 .. nv - 1
+#pragma unroll
+            for (int h = 0; h < 4; ++h) smp[h] = s_tgt[(nv * h) / 4 + nv / 8];
+        }
+        int nund = 0;                          // cells in s_und
+        while (top > 0 || nund > 0) {
+            // the exact test once 64 cells wait for it, or when nothing else is left
+            const bool exact = S == 1 && (nund >= 64 || top == 0);
+            const int have = exact ? nund : top;
+            const int k = have < 64 ? have : 64;
+            const int base = have - k;
             const bool live = (int)lane < k;
-            const uint32_t p = live ? s_stack[base + lane] : rid;
-            const uint32_t A = live ? (S > 1 ? (uint32_t)s_smask[base + lane] : 1u) : 0u;
-            top = base;
+            uint32_t p, A;
+            if (exact) {
+                p = live ? s_und[base + lane] : rid;
+                A = live ? 1u : 0u;
+                nund = base;
+            } else {
+                p = live ? s_stack[base + lane] : rid;
+                A = live ? (S > 1 ? (uint32_t)s_smask[base + lane] : 1u) : 0u;
+                top = base;
+            }
             const uint4 r0 = rec4[2 * (size_t)p], r1 = rec4[2 * (size_t)p + 1];
             const float cx = __uint_as_float(r0.x), cy = __uint_as_float(r0.y), cz = __uint_as_float(r0.z);
             const uint32_t meta = r0.w, skip = r1.x, start = r1.y, cb = r1.z;
             const float size2 = __uint_as_float(r1.w);
-            // least |d|^2 over each sub-group's bodies (absent bodies sit at infinity)
-            float md2[S];
-#pragma unroll
-            for (int q = 0; q < S; ++q) {
-                float m = INFINITY;
-#pragma unroll 8
-                for (int b = 0; b < K; ++b) {
-                    const float4 t = s_tgt[q * K + b];
-                    const float dx = cx - t.x, dy = cy - t.y, dz = cz - t.z;
-                    m = fminf(m, dx * dx + dy * dy + dz * dz);
-                }
-                md2[q] = m;
-            }
             uint32_t P = 0;                    // sub-groups where every body accepts the cell
+            if (S > 1 || exact) {
+                // least |d|^2 over each sub-group's bodies (absent bodies sit at infinity)
 #pragma unroll
-            for (int q = 0; q < S; ++q)
-                if (size2 < th2 * md2[q]) P |= 1u << q;
+                for (int q = 0; q < S; ++q) {
+                    float m = INFINITY;
+#pragma unroll 8
+                    for (int b = 0; b < K; ++b) {
+                        const float4 t = s_tgt[q * K + b];
+                        m = fminf(m, lp_d2(cx - t.x, cy - t.y, cz - t.z));
+                    }
+                    if (size2 < th2 * m) P |= 1u << q;
+                }
+            } else {
+                const float d2b = lp_d2(cx - __builtin_amdgcn_fmed3f(blo_x, cx, bhi_x),
+                                        cy - __builtin_amdgcn_fmed3f(blo_y, cy, bhi_y),
+                                        cz - __builtin_amdgcn_fmed3f(blo_z, cz, bhi_z));
+                float ds = INFINITY;
+#pragma unroll
+                for (int h = 0; h < 4; ++h) ds = fminf(ds, lp_d2(cx - smp[h].x, cy - smp[h].y, cz - smp[h].z));
+                const bool acc_sure = size2 < th2 * d2b;
+                const bool open_sure = !(size2 < th2 * ds);
+                const bool und = live && !acc_sure && !open_sure;
+                const uint64_t mu = __builtin_amdgcn_ballot_w64(und);
+                if (und)
+                    s_und[nund + (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(mu >> 32),
+                                                                __builtin_amdgcn_mbcnt_lo((uint32_t)mu, 0u))] = p;
+                nund += __popcll(mu);
+                if (und) A = 0u;
+                if (acc_sure) P = 1u;
+            }
             const uint32_t acc = A & P;
             const uint32_t O = A & ~P;          // sub-groups that open it
             const bool leaf = (meta & 32u) != 0u;
