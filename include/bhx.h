@@ -342,13 +342,25 @@ int bhx_autotune(bhx_ctx *ctx, const int32_t *caps, size_t ncaps, const int32_t 
 
 /* Runs nsteps frames from the resident state (as bhx_step_graph) and records
  * them for bhx_adjoint: a device checkpoint of the state every
- * checkpoint_every frames.  init_acc != 0 first sets the accelerations to the
+ * checkpoint_every frames, and, when it fits the memory set by
+ * bhx_set_record_memory, every frame's walk data (tree, body orders,
+ * payloads: about 60 bytes per body and 130 per node), so that bhx_adjoint
+ * re-runs no frames.  init_acc != 0 first sets the accelerations to the
  * walk of the starting positions (a_0 = A(x_0)); otherwise the resident
  * accelerations are the frames' a_0.  Needs group_walk = 1, the intended
  * payload, permute_acc = 1 and one list per 64-body group (group_size 0 or
  * 64); monopole or quadrupole cells, FP64 or FP32 walk.  Changing the
  * configuration drops the recording. */
 int bhx_record(bhx_ctx *ctx, int nsteps, int checkpoint_every, int init_acc);
+
+/* Device memory (bytes) bhx_record may use to keep every frame's walk data;
+ * 0: keep none (the reverse pass re-runs each checkpoint segment);
+ * UINT64_MAX (the default): half of the device memory free at recording. */
+int bhx_set_record_memory(bhx_ctx *ctx, uint64_t bytes);
+
+/* Whether the last recording kept every frame's walk data (*kept = 1) or
+ * its reverse pass re-runs the checkpoint segments (*kept = 0). */
+int bhx_record_keeps_frames(bhx_ctx *ctx, int *kept);
 
 /* State after the recorded frames, in the body order at the start of the
  * recording (any pointer may be NULL). */
@@ -369,11 +381,13 @@ int bhx_record_digest(bhx_ctx *ctx, uint64_t *digest);
  * (3 per body) and dL/dm (1 per body), all in the starting order; NULL
  * outputs are skipped.  Every walk is differentiated with the tree, the
  * interaction lists and the body orders of its recorded frame (frozen
- * decisions); the frames of each checkpoint segment are re-run, bitwise as
- * recorded.  dL/da_0 is 0 when bhx_record computed a_0 (its adjoint flows
- * into dL/dx_0 and dL/dm).  The bodies are left at the end of the recording.
- * Entry sums use FP64 atomics, so results may differ in the last bits
- * between runs. */
+ * decisions), read from the recording's stored frames or, when the recording
+ * did not keep them, from the frames of each checkpoint segment re-run
+ * bitwise as recorded.  dL/da_0 is 0 when bhx_record computed a_0 (its
+ * adjoint flows into dL/dx_0 and dL/dm).  The bodies are left at the end of
+ * the recording.  With the FP32 walk every sum has a fixed order, so results
+ * repeat bit for bit; the FP64 walk's entry sums use FP64 atomics, so its
+ * results may differ in the last bits between runs. */
 int bhx_adjoint(bhx_ctx *ctx, const double *gx, const double *gv, const double *ga,
                 double *gx0, double *gv0, double *gm, double *ga0);
 

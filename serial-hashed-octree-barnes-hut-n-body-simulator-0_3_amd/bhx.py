@@ -79,7 +79,8 @@ def lib():
             "bhx_build_systems": [P, P, P, P], "bhx_step_graph": [P, I],
             "bhx_autotune": [P, P, S, P, S, I, D, S, ctypes.c_uint64, P, P, P],
             "bhx_record": [P, I, I, I], "bhx_recorded_bodies": [P, P, P, P, P], "bhx_record_digest": [P, P],
-            "bhx_adjoint": [P, P, P, P, P, P, P, P], "bhx_grad_timing": [P, P, P], "bhx_system_roots": [P, P], "bhx_lookup_in": [P, S, P, S, P],
+            "bhx_adjoint": [P, P, P, P, P, P, P, P], "bhx_grad_timing": [P, P, P],
+            "bhx_set_record_memory": [P, ctypes.c_uint64], "bhx_record_keeps_frames": [P, P], "bhx_system_roots": [P, P], "bhx_lookup_in": [P, S, P, S, P],
             "bhx_get_systems": [P, P], "bhx_energy": [P, P, P, P, P],
             "bhx_galaxy_defaults": [P], "bhx_ic_galaxy_pair": [S, ctypes.c_uint64, P, I, P, P, P],
             "bhx_snapshot_write": [ctypes.c_char_p, P, P, P, P, P, P, P],
@@ -480,6 +481,18 @@ class Engine:
         """Run and record nsteps frames for adjoint(); see bhx_record."""
         self.record_generation += 1
         self._chk(lib().bhx_record(self.h, nsteps, checkpoint_every, int(init_acc)), "bhx_record")
+
+    def set_record_memory(self, nbytes):
+        """Device memory record() may use to keep every frame (0: none, re-run the checkpoint segments;
+        None: half of the free device memory)."""
+        v = (1 << 64) - 1 if nbytes is None else int(nbytes)
+        self._chk(lib().bhx_set_record_memory(self.h, ctypes.c_uint64(v)), "bhx_set_record_memory")
+
+    def record_keeps_frames(self):
+        """True when the last recording kept every frame (its reverse pass re-runs nothing)."""
+        k = ctypes.c_int()
+        self._chk(lib().bhx_record_keeps_frames(self.h, ctypes.byref(k)), "bhx_record_keeps_frames")
+        return bool(k.value)
 
     def recorded_bodies(self):
         """(pos, vel, mass, acc) after the recorded frames, in the order the recording started with."""

@@ -254,6 +254,7 @@ struct GradSlot {
     double* b4 = nullptr;
     double* scale32 = nullptr;
     double* quad = nullptr;                    // FP64 walk with quadrupoles: 6 per node
+    uint32_t* child = nullptr;                 // octant table, 8 per node
 };
 
 struct GradRecord {
@@ -269,12 +270,17 @@ struct GradRecord {
     GradSlot init_slot;
     void* seg_store = nullptr;                 // two banks of `every` frame slots
     std::vector<GradSlot> slots;
+    void* all_store = nullptr;                 // every recorded frame's slot, when it fits the budget
+    std::vector<GradSlot> all_slots;
+    size_t all_nodes = 0;                      // node records per slot of all_store
     // the reverse pass re-runs segment k-1 on the main stream while the
     // adjoint of segment k runs on the context's adjoint stream
     bool events = false;
     hipEvent_t ev_stored[2] = {}, ev_consumed[2] = {}, ev_ready = nullptr;
     double* work = nullptr;                    // adjoint vectors
     size_t work_nodes = 0;
+    bhx::AdjLists lists;                       // FP32 adjoint: stored group lists and their evaluation
+    void* lists_mem = nullptr;
     double record_ms = 0.0, adjoint_ms = 0.0;
 };
 
@@ -286,7 +292,8 @@ void free_grad(bhx_ctx* c) {
     for (double* p : g->ckpt)
         if (p) (void)hipFree(p);
     if (c->s_adj) (void)hipStreamSynchronize(c->s_adj);
-    void* ptrs[] = {g->final_state, g->ident, g->ident_tmp, g->init_store, g->seg_store, g->work};
+    void* ptrs[] = {g->final_state, g->ident, g->ident_tmp, g->init_store, g->seg_store, g->work, g->lists_mem,
+                    g->all_store};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     for (hipEvent_t e : {g->ev_stored[0], g->ev_stored[1], g->ev_consumed[0], g->ev_consumed[1], g->ev_ready})
@@ -307,7 +314,8 @@ bool slot_quad(int precision, int quad) { return quad && precision != BHX_FP32; 
 size_t slot_bytes(size_t n, size_t nodes, size_t nsys, bool quad) {
     return align_up(4 * n) + 4 * align_up(8 * n) + align_up(4 * n) + align_up(16 * n) + align_up(4 * n) +
            align_up(64 * nodes) + align_up(4 * nodes) + align_up(24 * nodes) + align_up(8 * nodes) +
-           align_up(4 * nsys) + align_up(32 * nsys) + align_up(16 * nsys) + (quad ? align_up(48 * nodes) : 0);
+           align_up(4 * nsys) + align_up(32 * nsys) + align_up(16 * nsys) + (quad ? align_up(48 * nodes) : 0) +
+           align_up(32 * nodes);
 }
 
 GradSlot carve_slot(char*& p, size_t n, size_t nodes, size_t nsys, int precision, bool quad) {
@@ -339,6 +347,7 @@ GradSlot carve_slot(char*& p, size_t n, size_t nodes, size_t nsys, int precision
     s.b4 = (double*)take(32 * nsys);
     s.scale32 = (double*)take(16 * nsys);
     if (quad) s.quad = (double*)take(48 * nodes);
+    s.child = (uint32_t*)take(32 * nodes);
     return s;
 }
 
@@ -412,6 +421,7 @@ int store_frame(bhx_ctx* c, const GradSlot& d, size_t nodes, int precision, bool
     add_job(J, c->bounds_slot(c->tree_slot), d.b4, 32 * c->nsys);
     add_job(J, c->sys_scale, d.scale32, 16 * c->nsys);
     if (d.quad) add_job(J, c->tree.quad, d.quad, 48 * nodes);
+    add_job(J, c->tree.child, d.child, 32 * nodes);
     HIPCHK(run_jobs(J, 16 * (n > nodes ? n : nodes), s));
     return BHX_OK;
 }
@@ -430,6 +440,7 @@ WalkTree tree_of(const GradSlot& d) {
     t.sys_root = d.sys_root;
     t.b4 = d.b4;
     t.quad = d.quad;
+    t.child = d.child;
     return t;
 }
 
@@ -474,7 +485,8 @@ int adjoint_walk(bhx_ctx* c, GradRecord* g, const Work& k, const GradSlot& d, co
     const size_t gs = g->quad ? 10 : 4;
     hipLaunchKernelGGL(k_fill_d, dim3(grid_of(gs * g->work_nodes)), dim3(kBlock), 0, s, k.gnode, gs * g->work_nodes, 0.0);
     HIPCHK(hipGetLastError());
-    HIPCHK(launch_walk_adjoint(c, tree_of(d), g->precision, k.wx, k.wy, k.wz, k.gself, k.gnode, k.gbody, s));
+    HIPCHK(launch_walk_adjoint(c, tree_of(d), g->precision, k.wx, k.wy, k.wz, k.gself, k.gnode, k.gbody, &g->lists,
+                               s));
     if (g->quad)
         hipLaunchKernelGGL(k_adj_push<true>, dim3(grid_of(n)), dim3(kBlock), 0, s, d.tleaf, d.parent, d.mass, d.com,
                            k.gnode, k.gbody, d.tx, d.ty, d.tz, d.tm, d.tperm, n, d.walk, d.walk32, k.push);
@@ -482,6 +494,50 @@ int adjoint_walk(bhx_ctx* c, GradRecord* g, const Work& k, const GradSlot& d, co
         hipLaunchKernelGGL(k_adj_push<false>, dim3(grid_of(n)), dim3(kBlock), 0, s, d.tleaf, d.parent, d.mass, d.com,
                            k.gnode, k.gbody, d.tx, d.ty, d.tz, d.tm, d.tperm, n, d.walk, d.walk32, k.push);
     HIPCHK(hipGetLastError());
+    return BHX_OK;
+}
+
+// The FP32 adjoint's list regions and entry records, in one allocation.  A
+// region per walk group of 8,192 entries (about twice a typical 64-body group
+// list), fewer when there are many groups (at most 2^26 entries in all); what
+// does not fit is evaluated by the traversal itself.
+int alloc_adj_lists(bhx_ctx* c, GradRecord* g) {
+    AdjLists& L = g->lists;
+    const size_t groups = c->ngroups > 0 ? c->ngroups : 1;
+    size_t cap = ((size_t)1 << 26) / groups;
+    cap = cap > 8192 ? 8192 : cap;
+    cap = cap < 1024 ? 1024 : cap & ~(size_t)63;
+    const size_t pairs = groups * cap;
+    const size_t ps = g->quad ? 10 : 4;
+    // sort keys: cell ids below node_cap (every stored tree's node ids), bodies
+    // node_cap + tree-order index
+    const size_t kmax = g->work_nodes + c->n;
+    int bits = 1;
+    while (bits < 64 && (((size_t)1 << bits) <= kmax)) ++bits;
+    const int passes = (bits + 7) / 8;
+    const size_t tiles = sort_tiles_for(pairs);
+    const size_t bytes[] = {groups * cap * 4, groups * 8, (groups + 1) * 4, pairs * ps * 8, pairs * 8, pairs * 8,
+                            pairs * 4, pairs * 4, 8 * 256 * 4, (size_t)passes * tiles * 256 * 4, 8 * 4, kmax * 8};
+    size_t total = 0;
+    for (size_t b : bytes) total += align_up(b);
+    if (hipMalloc(&g->lists_mem, total) != hipSuccess) {
+        g->lists_mem = nullptr;
+        return fail(c, BHX_E_NOMEM, "adjoint list allocation");
+    }
+    char* p = (char*)g->lists_mem;
+    void** dst[] = {(void**)&L.lists, (void**)&L.list_n, (void**)&L.off, (void**)&L.part, (void**)&L.key,
+                    (void**)&L.key_alt, (void**)&L.val, (void**)&L.val_alt, (void**)&L.hist, (void**)&L.status,
+                    (void**)&L.tile_ctr, (void**)&L.run};
+    for (size_t k = 0; k < sizeof bytes / sizeof bytes[0]; ++k) {
+        *dst[k] = p;
+        p += align_up(bytes[k]);
+    }
+    L.cap = (uint32_t)cap;
+    L.node_cap = (uint32_t)g->work_nodes;
+    L.max_pairs = pairs;
+    L.groups = c->ngroups;
+    L.passes = passes;
+    L.nkeys = kmax;
     return BHX_OK;
 }
 
@@ -568,6 +624,34 @@ int bhx_record(bhx_ctx* c, int nsteps, int checkpoint_every, int init_acc) {
             rc = load_state(c, g->ckpt[0], s);
             if (rc != BHX_OK) return bail(rc);
         }
+        // every frame's walk data, when it fits the memory budget: the reverse
+        // pass then reads it instead of re-running the frames from checkpoints
+        if (g->all_store) {
+            HIPCHK(hipStreamSynchronize(s));
+            (void)hipFree(g->all_store);
+            g->all_store = nullptr;
+            g->all_slots.clear();
+        }
+        {
+            const size_t one = slot_bytes(n, c->tree.cap, c->nsys, slot_quad(g->precision, g->quad));
+            uint64_t budget = c->record_memory;
+            if (budget == UINT64_MAX) {
+                size_t fr = 0, tot = 0;
+                budget = hipMemGetInfo(&fr, &tot) == hipSuccess ? fr / 2 : 0;
+            }
+            if (one <= budget / (uint64_t)nsteps) {
+                if (hipMalloc(&g->all_store, one * (size_t)nsteps) == hipSuccess) {
+                    char* p = (char*)g->all_store;
+                    g->all_slots.resize((size_t)nsteps);
+                    for (auto& sl : g->all_slots)
+                        sl = carve_slot(p, n, c->tree.cap, c->nsys, g->precision, slot_quad(g->precision, g->quad));
+                    g->all_nodes = c->tree.cap;
+                } else {
+                    (void)hipGetLastError();
+                    g->all_store = nullptr;
+                }
+            }
+        }
         hipLaunchKernelGGL(k_iota32, dim3(grid_of(n)), dim3(kBlock), 0, s, g->ident, n);
         hipLaunchKernelGGL(k_zero_flags, dim3(1), dim3(64), 0, s, c->ctr);   // count nodes from here
         if (digest) hipLaunchKernelGGL(k_digest_set, dim3(1), dim3(64), 0, s, c->digest, 1ull, 1ull);
@@ -578,6 +662,10 @@ int bhx_record(bhx_ctx* c, int nsteps, int checkpoint_every, int init_acc) {
             }
             rc = graph_frame(c, s);
             if (rc != BHX_OK) return bail(rc);
+            if (g->all_store) {
+                rc = store_frame(c, g->all_slots[(size_t)f], g->all_nodes, g->precision, true, s);
+                if (rc != BHX_OK) return bail(rc);
+            }
             hipLaunchKernelGGL(k_compose, dim3(grid_of(n)), dim3(kBlock), 0, s, g->ident, c->vals, n, g->ident_tmp);
             std::swap(g->ident, g->ident_tmp);
             if (digest)
@@ -659,7 +747,8 @@ int bhx_adjoint(bhx_ctx* c, const double* gx, const double* gv, const double* ga
     HIPCHK(hipStreamSynchronize(c->s_side));
     const size_t n = c->n;
     const int every = g->every;
-    if (!g->seg_store) {
+    const bool kept = g->all_store != nullptr;   // every frame stored while recording: nothing to re-run
+    if (!kept && !g->seg_store) {
         const size_t one = slot_bytes(n, g->node_cap, c->nsys, slot_quad(g->precision, g->quad));
         void* st = nullptr;
         if (hipMalloc(&st, 2 * one * (size_t)every) != hipSuccess) return fail(c, BHX_E_NOMEM, "store allocation");
@@ -678,6 +767,15 @@ int bhx_adjoint(bhx_ctx* c, const double* gx, const double* gv, const double* ga
     }
     if (!g->work) {
         if (dalloc(&g->work, work_len(n, g->work_nodes, g->quad != 0)) != hipSuccess) return fail(c, BHX_E_NOMEM, "adjoint allocation");
+    }
+    if (g->lists_mem && g->lists.groups != c->ngroups) {
+        (void)hipStreamSynchronize(c->s_adj);
+        (void)hipFree(g->lists_mem);
+        g->lists_mem = nullptr;
+    }
+    if (g->precision == BHX_FP32 && !g->lists_mem) {
+        int rc = alloc_adj_lists(c, g);
+        if (rc != BHX_OK) return rc;
     }
     hipStream_t sa = c->s_adj;
     Work k = carve_work(g->work, n, g->work_nodes);
@@ -722,18 +820,22 @@ int bhx_adjoint(bhx_ctx* c, const double* gx, const double* gv, const double* ga
         HIPCHK(hipEventRecord(g->ev_stored[bank], s));
         return BHX_OK;
     };
-    rc = rerun(nseg - 1);
-    if (rc != BHX_OK) return rc;
+    if (!kept) {
+        rc = rerun(nseg - 1);
+        if (rc != BHX_OK) return rc;
+    }
     for (int seg = nseg - 1; seg >= 0; --seg) {
         const int bank = seg & 1;
         const int f0 = seg * every, f1 = std::min(g->nsteps, f0 + every);
-        if (seg > 0) {   // queued ahead: runs while this segment's adjoint walks do
-            rc = rerun(seg - 1);
-            if (rc != BHX_OK) return rc;
+        if (!kept) {
+            if (seg > 0) {   // queued ahead: runs while this segment's adjoint walks do
+                rc = rerun(seg - 1);
+                if (rc != BHX_OK) return rc;
+            }
+            HIPCHK(hipStreamWaitEvent(sa, g->ev_stored[bank], 0));
         }
-        HIPCHK(hipStreamWaitEvent(sa, g->ev_stored[bank], 0));
         for (int f = f1 - 1; f >= f0; --f) {
-            const GradSlot& d = g->slots[(size_t)bank * every + (f - f0)];
+            const GradSlot& d = kept ? g->all_slots[(size_t)f] : g->slots[(size_t)bank * every + (f - f0)];
             rc = adjoint_walk(c, g, k, d, k.A[cur], k.V[cur], 0.5 * g->dt, sa);
             if (rc != BHX_OK) return rc;
             hipLaunchKernelGGL(k_adj_kdk, dim3(grid_of(n)), dim3(kBlock), 0, sa, k.X[cur], k.V[cur], k.M[cur],
@@ -742,8 +844,10 @@ int bhx_adjoint(bhx_ctx* c, const double* gx, const double* gv, const double* ga
             HIPCHK(hipGetLastError());
             cur ^= 1;
         }
-        HIPCHK(hipEventRecord(g->ev_consumed[bank], sa));
-        bank_read[bank] = true;
+        if (!kept) {
+            HIPCHK(hipEventRecord(g->ev_consumed[bank], sa));
+            bank_read[bank] = true;
+        }
     }
     if (g->init_acc) {
         // a_0 = A(x_0, m): its adjoint weights are dL/da_0
@@ -768,6 +872,19 @@ int bhx_adjoint(bhx_ctx* c, const double* gx, const double* gv, const double* ga
     if (rc != BHX_OK) return rc;
     if (c->ctr_host->overflow_seen) return fail(c, BHX_E_STATE, "a re-run frame differs from the recording");
     g->adjoint_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    return BHX_OK;
+}
+
+int bhx_set_record_memory(bhx_ctx* c, uint64_t bytes) {
+    if (!c) return BHX_E_NULL;
+    c->record_memory = bytes;
+    return BHX_OK;
+}
+
+int bhx_record_keeps_frames(bhx_ctx* c, int* kept) {
+    if (!c || !kept) return c ? fail(c, BHX_E_NULL, "output is a null pointer") : BHX_E_NULL;
+    if (!c->grad) return fail(c, BHX_E_STATE, "no recorded trajectory");
+    *kept = c->grad->all_store != nullptr;
     return BHX_OK;
 }
 

@@ -174,6 +174,10 @@ struct bhx_ctx {
     // pool: a stream created later can share the main stream's queue, and its
     // kernels then serialise with the main stream's
     hipStream_t s_adj = nullptr;
+    // device memory a recording may use to keep every frame's walk data, so
+    // that the reverse pass re-runs no frames (bhx_set_record_memory);
+    // UINT64_MAX: half of the free device memory when recording
+    uint64_t record_memory = UINT64_MAX;
     // the tree branch of a frame (path keys of the drifted positions and
     // their sort) beside the body sort; created after the other three
     hipStream_t s_tree = nullptr;
@@ -357,9 +361,12 @@ hipError_t launch_treekeys(const double* px, const double* py, const double* pz,
 hipError_t launch_iota(uint32_t* vals, size_t n, hipStream_t s);
 
 size_t sort_tiles_for(size_t n);
+// `ndev` non-null: the pair count is read on the device from *ndev (at most
+// n; the scratch and the grid are sized for n).  An even number of passes
+// leaves the sorted pairs in keys/vals, an odd number in keys_alt/vals_alt.
 hipError_t radix_sort_pairs(uint64_t* keys, uint32_t* vals, uint64_t* keys_alt, uint32_t* vals_alt,
                             size_t n, uint32_t* hist, uint32_t* status, uint32_t* tile_ctr,
-                            Counters* ctr, hipStream_t s, int passes = 8);
+                            Counters* ctr, hipStream_t s, int passes = 8, const uint32_t* ndev = nullptr);
 // stable sort of (key, value) pairs within each system's body range
 // Stable sort of (key, value) pairs, system by system; `sc` null: the
 // context's main sort scratch.
@@ -408,6 +415,7 @@ struct WalkTree {
     const uint32_t* sys_root;
     const double* b4;
     const double* quad;     // FP64 quadrupoles, 6 per node (FP64 walk with quadrupoles)
+    const uint32_t* child;  // octant table: 8 child ids per node (the FP32 adjoint's traversal)
 };
 // Adjoint of the group walk on tree t with frozen lists: per body weights w
 // (array order) -> dL/dx of the own position (gself, 3 per body, array
@@ -417,8 +425,31 @@ struct WalkTree {
 // group walk of the current tree that also writes each body's potential
 // (array order); accelerations go to ox/oy/oz
 hipError_t launch_walk_potential(bhx_ctx* c, hipStream_t s, double* ox, double* oy, double* oz, double* pot);
+// Working memory of the FP32 adjoint walk: each walk group's list region
+// (`cap` entries: cell ids from the front, tree-order body indices from the
+// back) and stored counts; the places of the lists in their concatenation
+// (groups + 1, the last is the total); per stored entry a record of 4 (10
+// with quadrupoles) doubles and a (key, place) pair, sorted by key with
+// radix-sort scratch for `max_pairs` = groups x cap pairs in `passes` passes,
+// and each key's run of records (nkeys = node_cap + bodies).
+struct AdjLists {
+    uint32_t* lists = nullptr;
+    uint2* list_n = nullptr;
+    uint32_t* off = nullptr;
+    double* part = nullptr;
+    uint64_t *key = nullptr, *key_alt = nullptr;
+    uint32_t *val = nullptr, *val_alt = nullptr;
+    uint32_t *hist = nullptr, *status = nullptr, *tile_ctr = nullptr;
+    uint2* run = nullptr;                      // per key: its records' range in the sorted order
+    uint32_t cap = 0, node_cap = 0;
+    size_t max_pairs = 0, groups = 0, nkeys = 0;
+    int passes = 2;
+};
+// The FP32 walk needs `L` (its stored lists and their evaluation); the FP64
+// walk sums the entries with atomics and ignores it.
 hipError_t launch_walk_adjoint(bhx_ctx* c, const WalkTree& t, int precision, const double* wx, const double* wy,
-                               const double* wz, double* gself, double* gnode, double* gbody, hipStream_t s);
+                               const double* wz, double* gself, double* gnode, double* gbody, const AdjLists* L,
+                               hipStream_t s);
 hipError_t launch_diag(bhx_ctx* c, double* out7, hipStream_t s);
 
 // the reference's per-body tree API (bhx_dropin.hip); one system

@@ -70,9 +70,11 @@ __global__ void k_sort_init(uint32_t* __restrict__ hist, uint32_t* __restrict__ 
     }
 }
 
-// Digit histograms of all eight passes in one read of the keys.
+// Digit histograms of all eight passes in one read of the keys.  With
+// `ndev` the key count is read from device memory (at most `n`).
 __global__ __launch_bounds__(kBlock) void k_sort_hist(const uint64_t* __restrict__ keys, size_t n,
-                                                      uint32_t* __restrict__ hist) {
+                                                      uint32_t* __restrict__ hist, const uint32_t* __restrict__ ndev) {
+    if (ndev) n = *ndev;
     __shared__ uint32_t h[kPasses * kBins];
     for (int j = threadIdx.x; j < kPasses * kBins; j += kBlock) h[j] = 0;
     __syncthreads();
@@ -103,7 +105,9 @@ __global__ __launch_bounds__(kBlock) void k_sort_pass(const uint64_t* __restrict
                                                       const uint32_t* __restrict__ goff,
                                                       uint32_t* __restrict__ status,
                                                       uint32_t* __restrict__ tile_ctr,
-                                                      Counters* __restrict__ ctr) {
+                                                      Counters* __restrict__ ctr,
+                                                      const uint32_t* __restrict__ ndev) {
+    if (ndev) n = *ndev;                       // the grid covers the largest count
     __shared__ uint32_t s_tile;
     __shared__ uint32_t s_wcnt[kBlock / 64][kBins];
     __shared__ uint32_t s_tstart[kBins];
@@ -132,6 +136,9 @@ __global__ __launch_bounds__(kBlock) void k_sort_pass(const uint64_t* __restrict
     __syncthreads();
     const uint32_t tile = s_tile;
     const size_t base = (size_t)tile * kTile;
+    // tiles past the device-side count hold no keys: every tile below them
+    // took its index earlier and looks back only at lower tiles
+    if (base >= n) return;
     const int shift = 8 * pass;
 
     // Warp-striped load: item j of a lane sits at base + wave*1024 + j*64 + lane,
@@ -248,9 +255,10 @@ size_t sort_tiles_for(size_t n) { return (n + kTile - 1) / kTile; }
 
 hipError_t radix_sort_pairs(uint64_t* keys, uint32_t* vals, uint64_t* keys_alt, uint32_t* vals_alt, size_t n,
                             uint32_t* hist, uint32_t* status, uint32_t* tile_ctr, Counters* ctr, hipStream_t s,
-                            int passes) {
-    if (n <= 1) return hipSuccess;
-    if (passes < 2 || passes > kPasses || (passes & 1)) return hipErrorInvalidValue;   // result lands in `keys`
+                            int passes, const uint32_t* ndev) {
+    if (passes < 1 || passes > kPasses) return hipErrorInvalidValue;
+    // an even pass count leaves the result in keys/vals, an odd one in the alt buffers
+    if (n == 0 || (n == 1 && !ndev && !(passes & 1))) return hipSuccess;
     const size_t tiles = sort_tiles_for(n);
     const size_t nstatus = (size_t)passes * tiles * kBins;
     unsigned ig = (unsigned)((nstatus + kBlock - 1) / kBlock);
@@ -259,13 +267,13 @@ hipError_t radix_sort_pairs(uint64_t* keys, uint32_t* vals, uint64_t* keys_alt, 
     unsigned hg = (unsigned)((n + kBlock * 16 - 1) / (kBlock * 16));
     if (hg > 1024) hg = 1024;
     if (hg == 0) hg = 1;
-    hipLaunchKernelGGL(k_sort_hist, dim3(hg), dim3(kBlock), 0, s, keys, n, hist);
+    hipLaunchKernelGGL(k_sort_hist, dim3(hg), dim3(kBlock), 0, s, keys, n, hist, ndev);
     hipLaunchKernelGGL(k_sort_scan, dim3(1), dim3(kBlock), 0, s, hist);
     for (int p = 0; p < passes; ++p) {
         const bool even = (p & 1) == 0;
         hipLaunchKernelGGL(k_sort_pass, dim3((unsigned)tiles), dim3(kBlock), 0, s, even ? keys : keys_alt,
                            even ? vals : vals_alt, even ? keys_alt : keys, even ? vals_alt : vals, n, p, hist,
-                           status + (size_t)p * tiles * kBins, tile_ctr, ctr);
+                           status + (size_t)p * tiles * kBins, tile_ctr, ctr, ndev);
     }
     return hipGetLastError();
 }

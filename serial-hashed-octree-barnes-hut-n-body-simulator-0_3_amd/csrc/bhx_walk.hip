@@ -13,6 +13,9 @@
 // records which lanes opened the current ancestor at that level, so each
 // lane interacts with exactly the nodes of its own reference interaction
 // list (the same set, not the same order).
+#include <algorithm>
+#include <mutex>
+
 #include "bhx_internal.hpp"
 
 namespace bhx {
@@ -81,6 +84,21 @@ struct WalkArgs {
     double* gbody;
     // persistent walks: per-XCD counters of the groups taken (8, zero at launch)
     uint32_t* wq;
+    // FP32 adjoint: per group a region of list_cap entries (cell ids from the
+    // front, tree-order body indices from the back) and the two counts; lists
+    // that do not fit are evaluated by the traversal itself
+    uint32_t* lists;
+    uint2* list_n;
+    uint32_t list_cap;
+    // and the evaluation kernel's entry sums: one record of `part_stride`
+    // doubles per stored entry at its place in the groups' concatenated lists
+    // (list_off), with a sort key (cell id, or node_cap + tree-order body
+    // index) and the place itself
+    const uint32_t* list_off;
+    double* part;
+    uint64_t* pkey;
+    uint32_t* pval;
+    uint32_t node_cap;
     const uint32_t* child;   // octant table: 8 child ids per node
     const uint32_t* child_byte;
     // decision digest (FP64 group walk): [0] sum, [1] collect when non-zero, [2] frame index
@@ -525,13 +543,12 @@ __device__ __forceinline__ void adj_self(const T (*ent)[4], const T (*q)[6], int
     }
 }
 
-// Entry side.  g4 holds `gs` doubles per entry (4: dL/dX, dL/dM; 10: and
-// dL/dQ); sx, sm, sq take the sums back to physical units.
-template <typename T, bool kQuad = false>
-__device__ __forceinline__ void adj_scatter(const T (*ent)[4], const T (*q)[6], const uint32_t* ids, int ne,
-                                            const T (*tg)[6], T eps2, uint32_t lane, double* __restrict__ g4,
-                                            double sx = 1.0, double sm = 1.0, double sq = 1.0) {
-    constexpr int gs = kQuad ? 10 : 4;
+// Entry side: lane = entry e over the 64 targets; `sink(e, X, Y, Z, h, G)`
+// receives the entry's sums in scaled units: dL/dX (X, Y, Z), dL/dM (h) and,
+// for quadrupole cells, dL/dQ (G, xx xy xz yy yz zz).
+template <typename T, bool kQuad, class Sink>
+__device__ __forceinline__ void adj_entry_sums(const T (*ent)[4], const T (*q)[6], int ne, const T (*tg)[6], T eps2,
+                                               uint32_t lane, Sink&& sink) {
     for (int e = (int)lane; e < ne; e += 64) {
         const T X = ent[e][0], Y = ent[e][1], Z = ent[e][2];
         T gx = 0, gy = 0, gz = 0, h = 0;
@@ -569,16 +586,28 @@ __device__ __forceinline__ void adj_scatter(const T (*ent)[4], const T (*q)[6], 
             }
         }
         const T M = ent[e][3];
+        sink(e, M * gx + hx, M * gy + hy, M * gz + hz, h, G);
+    }
+}
+
+// Entry side summed into memory: g4 holds `gs` doubles per entry (4: dL/dX,
+// dL/dM; 10: and dL/dQ); sx, sm, sq take the sums back to physical units.
+template <typename T, bool kQuad = false>
+__device__ __forceinline__ void adj_scatter(const T (*ent)[4], const T (*q)[6], const uint32_t* ids, int ne,
+                                            const T (*tg)[6], T eps2, uint32_t lane, double* __restrict__ g4,
+                                            double sx = 1.0, double sm = 1.0, double sq = 1.0) {
+    constexpr int gs = kQuad ? 10 : 4;
+    adj_entry_sums<T, kQuad>(ent, q, ne, tg, eps2, lane, [&](int e, T X, T Y, T Z, T h, const T* G) {
         double* g = g4 + gs * (size_t)ids[e];
-        unsafeAtomicAdd(g, (double)(M * gx + hx) * sx);
-        unsafeAtomicAdd(g + 1, (double)(M * gy + hy) * sx);
-        unsafeAtomicAdd(g + 2, (double)(M * gz + hz) * sx);
+        unsafeAtomicAdd(g, (double)X * sx);
+        unsafeAtomicAdd(g + 1, (double)Y * sx);
+        unsafeAtomicAdd(g + 2, (double)Z * sx);
         unsafeAtomicAdd(g + 3, (double)h * sm);
         if constexpr (kQuad) {
 #pragma unroll
             for (int k = 0; k < 6; ++k) unsafeAtomicAdd(g + 4 + k, (double)G[k] * sq);
         }
-    }
+    });
 }
 
 // Bodies of the queued leaf ranges into LDS with every load in flight at
@@ -1231,15 +1260,19 @@ __device__ __forceinline__ void lp_eval_bodies(const float (*bd)[4], const uint8
     }
 }
 
-template <bool kQuad, bool kGuard, bool kPot, int S>
+template <bool kQuad, bool kGuard, bool kPot, int S, bool kAdj = false>
 __global__ __launch_bounds__(64) void k_walk_lp32(WalkArgs a) {
     static_assert(S == 1 || S == 2 || S == 4, "1, 2 or 4 sub-groups per wave");
+    static_assert(!kAdj || (S == 1 && !kPot), "the adjoint walk keeps one list per wave and has no potential");
     constexpr int K = 64 / S;                  // bodies per sub-group, cells per sub-group and flush
     constexpr uint32_t kAll = (1u << S) - 1u;
+    constexpr int kBodiesLp = kAdj ? 128 : kLpBodies;   // the adjoint's body flushes fill both scatter halves
     __shared__ float s_nd[kCells32][4];
     __shared__ float s_q[kQuad ? kCells32 : 1][6];
-    __shared__ float s_bd[kLpBodies][4];
-    __shared__ uint8_t s_bm[S > 1 ? kLpBodies : 1];
+    __shared__ float s_bd[kBodiesLp][4];
+    __shared__ uint8_t s_bm[S > 1 ? kBodiesLp : 1];
+    __shared__ uint32_t s_bid[kAdj ? kBodiesLp : 1];   // adjoint: tree-order index of each body entry
+    __shared__ float s_tg[kAdj ? 64 : 1][6];            // adjoint: targets' positions and weights
     __shared__ uint32_t s_rstart[kRanges32], s_roff[kRanges32], s_rleaf[kRanges32];
     __shared__ uint8_t s_rmask[kRanges32];
     __shared__ float s_rdel[kRanges32][3];
@@ -1287,7 +1320,20 @@ __global__ __launch_bounds__(64) void k_walk_lp32(WalkArgs a) {
         const float xs = (float)((tx[iv] - b4s[0]) * lam), ys = (float)((ty[iv] - b4s[1]) * lam),
                     zs = (float)((tz[iv] - b4s[2]) * lam);
         float ax = 0.f, ay = 0.f, az = 0.f, phi = 0.f;
+        // adjoint: the body's weight w = dL/da (scaled units cancel: the walk
+        // output is linear in w) and, in ax/ay/az, dL/dx of its own position
+        float wxr = 0.f, wyr = 0.f, wzr = 0.f;
+        if constexpr (kAdj) {
+            if (valid) {
+                const uint32_t o = a.tperm[i];
+                wxr = (float)a.wx[o];
+                wyr = (float)a.wy[o];
+                wzr = (float)a.wz[o];
+            }
+        }
         uint32_t cnt = 0;
+        uint32_t lc = 0, lb = 0;               // adjoint: cell and body entries stored
+        const size_t lbase = kAdj ? (size_t)grp * a.list_cap : 0;
         int nc[S];                             // cells waiting per sub-group
 #pragma unroll
         for (int q = 0; q < S; ++q) nc[q] = 0;
@@ -1295,6 +1341,10 @@ __global__ __launch_bounds__(64) void k_walk_lp32(WalkArgs a) {
 
         __syncthreads();                       // the previous group is done with the lists
         s_tgt[lane] = valid ? make_float4(xs, ys, zs, 0.f) : make_float4(INFINITY, INFINITY, INFINITY, 0.f);
+        if constexpr (kAdj) {
+            s_tg[lane][0] = xr; s_tg[lane][1] = yr; s_tg[lane][2] = zr;
+            s_tg[lane][3] = wxr; s_tg[lane][4] = wyr; s_tg[lane][5] = wzr;
+        }
 
         // one chunk of every sub-group's cell list: up to K cells each
         auto flush_cells = [&]() {
@@ -1306,24 +1356,42 @@ __global__ __launch_bounds__(64) void k_walk_lp32(WalkArgs a) {
                 if ((int)sg == q) t = nc[q] < K ? nc[q] : K;
             const bool has = (int)k < t;
             const uint32_t id = has ? s_cid[sg][k] : 0u;
-            {
-                // lane l gathers entry l: sub-group l / K, position l % K
-                const WalkPay32& c = a.pay32[id];
-                s_nd[lane][0] = (float)((a.com[3 * (size_t)id] - ox) * lam);
-                s_nd[lane][1] = (float)((a.com[3 * (size_t)id + 1] - oy) * lam);
-                s_nd[lane][2] = (float)((a.com[3 * (size_t)id + 2] - oz) * lam);
-                s_nd[lane][3] = has ? c.mass : 0.f;      // padding: a massless cell at a finite place
-                if (kQuad) {
+            bool stored = false;
+            if constexpr (kAdj) {
+                // the adjoint stores the list for the evaluation kernel while it fits
+                if (lc + lb + (uint32_t)t <= a.list_cap) {
+                    if (has) a.lists[lbase + lc + lane] = id;
+                    lc += (uint32_t)t;
+                    stored = true;
+                }
+            }
+            if (!stored) {
+                {
+                    // lane l gathers entry l: sub-group l / K, position l % K
+                    const WalkPay32& c = a.pay32[id];
+                    s_nd[lane][0] = (float)((a.com[3 * (size_t)id] - ox) * lam);
+                    s_nd[lane][1] = (float)((a.com[3 * (size_t)id + 1] - oy) * lam);
+                    s_nd[lane][2] = (float)((a.com[3 * (size_t)id + 2] - oz) * lam);
+                    s_nd[lane][3] = has ? c.mass : 0.f;      // padding: a massless cell at a finite place
+                    if (kQuad) {
 #pragma unroll
-                    for (int u = 0; u < 6; ++u) s_q[lane][u] = has ? c.q[u] : 0.f;
+                        for (int u = 0; u < 6; ++u) s_q[lane][u] = has ? c.q[u] : 0.f;
+                    }
+                }
+                __syncthreads();
+                if constexpr (kAdj) {
+                    // the list's first t entries: their ids are s_cid[0][0 .. t)
+                    adj_self<float, kQuad>(s_nd, s_q, t, xr, yr, zr, wxr, wyr, wzr, eps2, ax, ay, az);
+                    adj_scatter<float, kQuad>(s_nd, s_q, s_cid[0], t, s_tg, eps2, lane, a.gnode,
+                                              lam * lam * lam / mu, lam * lam, (lam * lam) * (lam * lam));
+                } else {
+                    lp_eval_cells<S, kQuad, kGuard, kPot>(s_nd, s_q, lane, xr, yr, zr, eps2, ax, ay, az, phi);
                 }
             }
             // the rest of each list moves to its front
             uint32_t rest[kLpCells / 64];
 #pragma unroll
             for (int h = 0; h < kLpCells / 64; ++h) rest[h] = 0u;
-            __syncthreads();
-            lp_eval_cells<S, kQuad, kGuard, kPot>(s_nd, s_q, lane, xr, yr, zr, eps2, ax, ay, az, phi);
             cnt += (uint32_t)t;
 #pragma unroll
             for (int q = 0; q < S; ++q) {
@@ -1345,6 +1413,28 @@ __global__ __launch_bounds__(64) void k_walk_lp32(WalkArgs a) {
         };
         auto flush_bodies = [&]() {
             __syncthreads();
+            if constexpr (kAdj) {
+                if (lc + lb + (uint32_t)nq <= a.list_cap) {
+#pragma unroll
+                    for (int h = 0; h < (kBodiesLp + 63) / 64; ++h) {
+                        const int e = (int)lane + 64 * h;
+                        if (e < nq) {
+                            int lo = 0, hi = nr - 1;
+                            while (lo < hi) {
+                                const int mid = (lo + hi + 1) >> 1;
+                                if ((int)s_roff[mid] <= e) lo = mid; else hi = mid - 1;
+                            }
+                            a.lists[lbase + a.list_cap - 1u - (lb + (uint32_t)e)] =
+                                s_rstart[lo] + (uint32_t)(e - (int)s_roff[lo]);
+                        }
+                    }
+                    lb += (uint32_t)nq;
+                    cnt += (uint32_t)nq;
+                    nr = 0;
+                    nq = 0;
+                    return;
+                }
+            }
             if ((int)lane < nr) {
                 const size_t l3 = 3 * (size_t)s_rleaf[lane];
                 s_rdel[lane][0] = (float)((a.com[l3] - ox) * lam);
@@ -1353,7 +1443,7 @@ __global__ __launch_bounds__(64) void k_walk_lp32(WalkArgs a) {
             }
             __syncthreads();
 #pragma unroll
-            for (int h = 0; h < (kLpBodies + 63) / 64; ++h) {
+            for (int h = 0; h < (kBodiesLp + 63) / 64; ++h) {
                 const int e = (int)lane + 64 * h;
                 if (e < nq) {
                     int lo = 0, hi = nr - 1;
@@ -1368,10 +1458,16 @@ __global__ __launch_bounds__(64) void k_walk_lp32(WalkArgs a) {
                     s_bd[e][2] = s_rdel[lo][2] + b.z;
                     s_bd[e][3] = b.w;
                     if constexpr (S > 1) s_bm[e] = s_rmask[lo];
+                    if constexpr (kAdj) s_bid[e] = j;
                 }
             }
             __syncthreads();
-            lp_eval_bodies<S, kGuard, kPot>(s_bd, s_bm, nq, lane, xr, yr, zr, eps2, ax, ay, az, phi);
+            if constexpr (kAdj) {
+                adj_self<float>(s_bd, s_q, nq, xr, yr, zr, wxr, wyr, wzr, eps2, ax, ay, az);
+                adj_scatter<float>(s_bd, s_q, s_bid, nq, s_tg, eps2, lane, a.gbody, lam * lam * lam / mu, lam * lam);
+            } else {
+                lp_eval_bodies<S, kGuard, kPot>(s_bd, s_bm, nq, lane, xr, yr, zr, eps2, ax, ay, az, phi);
+            }
             if constexpr (S == 1) {
                 cnt += (uint32_t)nq;
             } else {
@@ -1385,7 +1481,7 @@ __global__ __launch_bounds__(64) void k_walk_lp32(WalkArgs a) {
         auto add_leaf = [&](uint32_t leafid, uint32_t st, uint32_t count, uint32_t mask) {
             for (uint32_t k = 0; k < count; k += 64) {
                 const int m = (int)((count - k) < 64u ? (count - k) : 64u);
-                if (nq + m > kLpBodies || nr == kRanges32) flush_bodies();
+                if (nq + m > kBodiesLp || nr == kRanges32) flush_bodies();
                 if (lane == 0) {
                     s_rstart[nr] = st + k;
                     s_roff[nr] = (uint32_t)nq;
@@ -1613,14 +1709,259 @@ __global__ __launch_bounds__(64) void k_walk_lp32(WalkArgs a) {
             flush_cells();
         }
         if (nq > 0) flush_bodies();
+        if constexpr (kAdj) {
+            if (lane == 0) a.list_n[grp] = make_uint2(lc, lb);
+        }
         if (valid) {
             const uint32_t o = a.tperm[i];
-            const double back = lam * lam / mu;
-            a.ax[o] = (double)ax * back;
-            a.ay[o] = (double)ay * back;
-            a.az[o] = (double)az * back;
-            if constexpr (kPot) a.pot[o] = (double)phi * (lam / mu);
+            if constexpr (kAdj) {
+                // dL/dx of the own position (the lists evaluated here; the
+                // evaluation kernel adds the stored ones) back to physical
+                // units (exact): lam^3 / mu
+                const double back = lam * lam * lam / mu;
+                a.gself[3 * (size_t)o] = (double)ax * back;
+                a.gself[3 * (size_t)o + 1] = (double)ay * back;
+                a.gself[3 * (size_t)o + 2] = (double)az * back;
+            } else {
+                const double back = lam * lam / mu;
+                a.ax[o] = (double)ax * back;
+                a.ay[o] = (double)ay * back;
+                a.az[o] = (double)az * back;
+                if constexpr (kPot) a.pot[o] = (double)phi * (lam / mu);
+            }
             if (a.counts) a.counts[o] = cnt;
+        }
+    }
+}
+
+// The stored adjoint lists of k_walk_lp32<..., kAdj> evaluated with W waves
+// per group: wave w takes every W-th chunk of 64 cells, then of 64 bodies,
+// gathers it into its own LDS slots, and runs both adjoint passes on it (lane
+// = target over the chunk; lane = entry over the 64 targets, whose sums are
+// written as the entry's record at its place in the concatenated lists, to be
+// added up per entry in a fixed order after a sort by entry).  The waves only
+// meet at the end, where their own-position sums are added in wave order and
+// added to what the traversal evaluated.  Entry positions are formed exactly
+// as the traversal forms them.
+__device__ __forceinline__ void wave_lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+}
+
+template <bool kQuad, int W>
+__global__ __launch_bounds__(64 * W) void k_adj_lists32(WalkArgs a) {
+    __shared__ float s_tg[64][6];
+    __shared__ float s_nd[W][64][4];
+    __shared__ float s_q[W][kQuad ? 64 : 1][6];
+    __shared__ uint32_t s_id[W][64];
+    __shared__ float s_red[W][64][3];
+    const uint32_t grp = xcd_group(blockIdx.x, a.groups);
+    if (grp >= a.groups) return;               // the whole workgroup
+    const uint32_t w = threadIdx.x / 64u, lane = threadIdx.x % 64u;
+    const uint4 gr = a.gtab[grp];
+    const uint32_t g0 = gr.x;
+    const uint32_t i = g0 + lane;
+    const bool valid = i < gr.y;
+    const uint32_t iv = valid ? i : g0;
+    ConstD tx = (ConstD)a.tx;
+    ConstD ty = (ConstD)a.ty;
+    ConstD tz = (ConstD)a.tz;
+    double ox = tx[g0], oy = ty[g0], oz = tz[g0];
+    asm volatile("" : "+v"(ox), "+v"(oy), "+v"(oz));
+    const uint32_t own = a.tleaf[iv];
+    const float4 ob = a.tb4[iv];
+    const double lam = a.scale32[2 * (size_t)gr.z], mu = a.scale32[2 * (size_t)gr.z + 1];
+    const float xr = (float)((a.com[3 * (size_t)own] - ox) * lam) + ob.x;
+    const float yr = (float)((a.com[3 * (size_t)own + 1] - oy) * lam) + ob.y;
+    const float zr = (float)((a.com[3 * (size_t)own + 2] - oz) * lam) + ob.z;
+    const float eps2 = (float)(a.eps2 * lam * lam);
+    float wxr = 0.f, wyr = 0.f, wzr = 0.f;
+    uint32_t o = 0;
+    if (valid) {
+        o = a.tperm[i];
+        wxr = (float)a.wx[o];
+        wyr = (float)a.wy[o];
+        wzr = (float)a.wz[o];
+    }
+    if (w == 0) {
+        s_tg[lane][0] = xr; s_tg[lane][1] = yr; s_tg[lane][2] = zr;
+        s_tg[lane][3] = wxr; s_tg[lane][4] = wyr; s_tg[lane][5] = wzr;
+    }
+    __syncthreads();
+    const uint2 cnt = a.list_n[grp];
+    const size_t base = (size_t)grp * a.list_cap;
+    const size_t off = a.list_off[grp];
+    constexpr size_t kPs = kQuad ? 10 : 4;
+    const double sx = lam * lam * lam / mu, sm = lam * lam, sq = (lam * lam) * (lam * lam);
+    float ax = 0.f, ay = 0.f, az = 0.f;
+    for (uint32_t c0 = 64u * w; c0 < cnt.x; c0 += 64u * W) {
+        const int ne = (int)(cnt.x - c0 < 64u ? cnt.x - c0 : 64u);
+        if ((int)lane < ne) {
+            const uint32_t id = a.lists[base + c0 + lane];
+            const WalkPay32& c = a.pay32[id];
+            s_nd[w][lane][0] = (float)((a.com[3 * (size_t)id] - ox) * lam);
+            s_nd[w][lane][1] = (float)((a.com[3 * (size_t)id + 1] - oy) * lam);
+            s_nd[w][lane][2] = (float)((a.com[3 * (size_t)id + 2] - oz) * lam);
+            s_nd[w][lane][3] = c.mass;
+            if constexpr (kQuad) {
+#pragma unroll
+                for (int u = 0; u < 6; ++u) s_q[w][lane][u] = c.q[u];
+            }
+            s_id[w][lane] = id;
+        }
+        wave_lds_sync();
+        adj_self<float, kQuad>(s_nd[w], s_q[w], ne, xr, yr, zr, wxr, wyr, wzr, eps2, ax, ay, az);
+        adj_entry_sums<float, kQuad>(s_nd[w], s_q[w], ne, s_tg, eps2, lane,
+                                     [&](int e, float X, float Y, float Z, float h, const float* G) {
+            const size_t pos = off + c0 + (uint32_t)e;
+            double* p = a.part + kPs * pos;
+            p[0] = (double)X * sx;
+            p[1] = (double)Y * sx;
+            p[2] = (double)Z * sx;
+            p[3] = (double)h * sm;
+            if constexpr (kQuad) {
+#pragma unroll
+                for (int k = 0; k < 6; ++k) p[4 + k] = (double)G[k] * sq;
+            }
+            a.pkey[pos] = s_id[w][e];
+            a.pval[pos] = (uint32_t)pos;
+        });
+        wave_lds_sync();
+    }
+    for (uint32_t b0 = 64u * w; b0 < cnt.y; b0 += 64u * W) {
+        const int ne = (int)(cnt.y - b0 < 64u ? cnt.y - b0 : 64u);
+        if ((int)lane < ne) {
+            const uint32_t j = a.lists[base + a.list_cap - 1u - (b0 + lane)];
+            const size_t l3 = 3 * (size_t)a.tleaf[j];
+            const float4 b = a.tb4[j];
+            s_nd[w][lane][0] = (float)((a.com[l3] - ox) * lam) + b.x;
+            s_nd[w][lane][1] = (float)((a.com[l3 + 1] - oy) * lam) + b.y;
+            s_nd[w][lane][2] = (float)((a.com[l3 + 2] - oz) * lam) + b.z;
+            s_nd[w][lane][3] = b.w;
+            s_id[w][lane] = j;
+        }
+        wave_lds_sync();
+        adj_self<float>(s_nd[w], s_q[w], ne, xr, yr, zr, wxr, wyr, wzr, eps2, ax, ay, az);
+        adj_entry_sums<float, false>(s_nd[w], s_q[w], ne, s_tg, eps2, lane,
+                                     [&](int e, float X, float Y, float Z, float h, const float*) {
+            const size_t pos = off + cnt.x + b0 + (uint32_t)e;
+            double* p = a.part + kPs * pos;
+            p[0] = (double)X * sx;
+            p[1] = (double)Y * sx;
+            p[2] = (double)Z * sx;
+            p[3] = (double)h * sm;
+            a.pkey[pos] = (uint64_t)a.node_cap + s_id[w][e];
+            a.pval[pos] = (uint32_t)pos;
+        });
+        wave_lds_sync();
+    }
+    s_red[w][lane][0] = ax;
+    s_red[w][lane][1] = ay;
+    s_red[w][lane][2] = az;
+    __syncthreads();
+    if (w == 0 && valid) {
+        float gx = s_red[0][lane][0], gy = s_red[0][lane][1], gz = s_red[0][lane][2];
+#pragma unroll
+        for (int k = 1; k < W; ++k) {
+            gx += s_red[k][lane][0];
+            gy += s_red[k][lane][1];
+            gz += s_red[k][lane][2];
+        }
+        a.gself[3 * (size_t)o] += (double)gx * sx;
+        a.gself[3 * (size_t)o + 1] += (double)gy * sx;
+        a.gself[3 * (size_t)o + 2] += (double)gz * sx;
+    }
+}
+
+// Places of the groups' lists in their concatenation: exclusive scan of the
+// stored entry counts (one workgroup); the total goes to off[groups].
+__global__ __launch_bounds__(1024) void k_list_scan(const uint2* __restrict__ list_n, uint32_t groups,
+                                                    uint32_t* __restrict__ off) {
+    __shared__ uint32_t s_w[16];
+    __shared__ uint32_t s_carry;
+    const uint32_t t = threadIdx.x, lane = t % 64u, wv = t / 64u;
+    if (t == 0) s_carry = 0;
+    __syncthreads();
+    for (uint32_t b = 0; b < groups; b += 1024u) {
+        const uint32_t g = b + t;
+        const uint2 c = g < groups ? list_n[g] : make_uint2(0u, 0u);
+        const uint32_t v = c.x + c.y;
+        const uint32_t incl = wave_incl_scan(v, lane);
+        if (lane == 63u) s_w[wv] = incl;
+        __syncthreads();
+        uint32_t wb = 0, tot = 0;
+        for (uint32_t k = 0; k < 16u; ++k) {
+            if (k < wv) wb += s_w[k];
+            tot += s_w[k];
+        }
+        const uint32_t carry = s_carry;
+        if (g < groups) off[g] = carry + wb + incl - v;
+        __syncthreads();
+        if (t == 0) s_carry = carry + tot;
+        __syncthreads();
+    }
+    if (t == 0) off[groups] = s_carry;
+}
+
+// Entry sums after the sort by entry.  k_run_bounds marks where each key's
+// run of records starts and ends in the sorted order (keys without records
+// keep the empty range the fill left); k_run_sum gives each key one wave,
+// whose lanes add every 64th record of the run in sorted order (the order of
+// the places, the sort being stable) and then combine the 64 lane sums in a
+// fixed butterfly order, and adds the result to the entry's accumulator:
+// gnode (`gs` per cell: dL/dX, dL/dM, dL/dQ) or gbody (4 per body).  The
+// sums do not depend on timing: the gradients repeat bit for bit.
+__global__ __launch_bounds__(256) void k_run_bounds(const uint64_t* __restrict__ key, const uint32_t* __restrict__ ntot,
+                                                    uint2* __restrict__ run) {
+    const uint32_t n = *ntot;
+    for (uint32_t i = blockIdx.x * 256u + threadIdx.x; i < n; i += gridDim.x * 256u) {
+        const uint64_t k = key[i];
+        if (i == 0 || key[i - 1] != k) run[k].x = i;
+        if (i + 1 == n || key[i + 1] != k) run[k].y = i + 1;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_fill_runs(uint2* __restrict__ run, size_t nkeys) {
+    for (size_t i = blockIdx.x * 256u + threadIdx.x; i < nkeys; i += (size_t)gridDim.x * 256u) run[i] = make_uint2(0u, 0u);
+}
+
+template <int GS>
+__global__ __launch_bounds__(256) void k_run_sum(const uint2* __restrict__ run, const uint32_t* __restrict__ val,
+                                                 const double* __restrict__ part, uint32_t node_cap, size_t nkeys,
+                                                 double* __restrict__ gnode, double* __restrict__ gbody) {
+    const uint32_t lane = threadIdx.x % 64u;
+    for (size_t k = (blockIdx.x * 256u + threadIdx.x) / 64u; k < nkeys; k += (size_t)gridDim.x * 4u) {
+        const uint2 r = run[k];
+        if (r.y <= r.x) continue;              // the whole wave: no records for this key
+        const bool cell = k < node_cap;
+        double acc[GS];
+#pragma unroll
+        for (int u = 0; u < GS; ++u) acc[u] = 0.0;
+        for (uint32_t j = r.x + lane; j < r.y; j += 64u) {
+            const double* p = part + (size_t)GS * val[j];
+            if (cell) {
+#pragma unroll
+                for (int u = 0; u < GS; ++u) acc[u] += p[u];
+            } else {
+#pragma unroll
+                for (int u = 0; u < 4; ++u) acc[u] += p[u];
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < GS; ++u) {
+#pragma unroll
+            for (int off = 32; off > 0; off >>= 1) acc[u] += __shfl_xor(acc[u], off, 64);
+        }
+        if (lane == 0) {
+            if (cell) {
+                double* g = gnode + (size_t)GS * k;
+#pragma unroll
+                for (int u = 0; u < GS; ++u) g[u] += acc[u];
+            } else {
+                double* g = gbody + 4 * (k - node_cap);
+#pragma unroll
+                for (int u = 0; u < 4; ++u) g[u] += acc[u];
+            }
         }
     }
 }
@@ -1955,17 +2296,29 @@ __global__ void k_diag_final(const double* __restrict__ part, int nparts, double
 
 // Workgroups of a persistent walk: as many one-wave workgroups as the chip
 // holds at once (a multiple of 8, one share per XCD), at most one per group.
+// The occupancy query is made once per kernel (the kernels share one
+// signature, so the cache is keyed by the kernel's address).
 template <class K>
 unsigned persistent_grid(K kernel, unsigned groups) {
-    static int per_cu = 0, ncu = 0;
-    if (per_cu == 0) {
-        int dev = 0, pc = 0, nc = 0;
-        if (hipGetDevice(&dev) != hipSuccess ||
-            hipOccupancyMaxActiveBlocksPerMultiprocessor(&pc, kernel, 64, 0) != hipSuccess ||
-            hipDeviceGetAttribute(&nc, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
-            pc = nc = 0;
-        ncu = nc > 0 ? nc : 256;
-        per_cu = pc > 0 ? pc : 1;
+    struct Entry { const void* k; int per_cu, ncu; };
+    static Entry cache[32];
+    static int used = 0;
+    static std::mutex mu;
+    int per_cu = 0, ncu = 0;
+    {
+        std::lock_guard<std::mutex> lock(mu);
+        for (int e = 0; e < used; ++e)
+            if (cache[e].k == (const void*)kernel) { per_cu = cache[e].per_cu; ncu = cache[e].ncu; }
+        if (per_cu == 0) {
+            int dev = 0, pc = 0, nc = 0;
+            if (hipGetDevice(&dev) != hipSuccess ||
+                hipOccupancyMaxActiveBlocksPerMultiprocessor(&pc, kernel, 64, 0) != hipSuccess ||
+                hipDeviceGetAttribute(&nc, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+                pc = nc = 0;
+            ncu = nc > 0 ? nc : 256;
+            per_cu = pc > 0 ? pc : 1;
+            if (used < 32) cache[used++] = Entry{(const void*)kernel, per_cu, ncu};
+        }
     }
     unsigned g = (unsigned)(per_cu * ncu);
     g = (g + 7u) & ~7u;
@@ -2105,7 +2458,8 @@ hipError_t launch_walk_potential(bhx_ctx* c, hipStream_t s, double* ox, double* 
 }
 
 hipError_t launch_walk_adjoint(bhx_ctx* c, const WalkTree& t, int precision, const double* wx, const double* wy,
-                               const double* wz, double* gself, double* gnode, double* gbody, hipStream_t s) {
+                               const double* wz, double* gself, double* gnode, double* gbody, const AdjLists* L,
+                               hipStream_t s) {
     if (c->n == 0 || c->ngroups == 0) return hipSuccess;
     WalkArgs a{};
     a.nodes = t.walk;
@@ -2133,15 +2487,51 @@ hipError_t launch_walk_adjoint(bhx_ctx* c, const WalkTree& t, int precision, con
     a.gnode = gnode;
     a.gbody = gbody;
     a.quad = t.quad;
+    a.child = t.child;
+    // the adjoint stream runs beside the main stream's walks: counters of its own
+    a.wq = c->wq + 8;
+    if (precision == BHX_FP32 && !L) return hipErrorInvalidValue;
+    if (L) {
+        a.lists = L->lists;
+        a.list_n = L->list_n;
+        a.list_cap = L->cap;
+        a.list_off = L->off;
+        a.part = L->part;
+        a.pkey = L->key;
+        a.pval = L->val;
+        a.node_cap = L->node_cap;
+    }
     const unsigned groups = c->ngroups;
+    const unsigned bgrid = 8u * ((groups + 7u) / 8u);
+    const unsigned rgrid = L ? (unsigned)std::min<size_t>((L->max_pairs + 255) / 256, 8192) : 1u;
+    const unsigned kgrid = L ? (unsigned)std::min<size_t>((L->nkeys + 255) / 256, 4096) : 1u;
+    const unsigned wgrid = L ? (unsigned)std::min<size_t>((L->nkeys + 3) / 4, 65536) : 1u;
+    hipError_t e = hipSuccess;
     const bool guard = !(a.eps2 > 0.0);
     const bool quad = c->cfg.quadrupole != 0;
 #define BHX_ADJ(Q, G)                                                                                             \
     do {                                                                                                          \
-        if (precision == BHX_FP32)                                                                                \
-            hipLaunchKernelGGL((k_walk_group32<Q, G, true>), dim3(8 * ((groups + 7) / 8)), dim3(64), 0, s, a);    \
-        else                                                                                                      \
+        if (precision == BHX_FP32) {                                                                              \
+            hipLaunchKernelGGL(k_fill_wq, dim3(1), dim3(64), 0, s, a.wq);                                         \
+            hipLaunchKernelGGL((k_walk_lp32<Q, G, false, 1, true>),                                               \
+                               dim3(persistent_grid(k_walk_lp32<Q, G, false, 1, true>, groups)), dim3(64), 0, s, a); \
+            hipLaunchKernelGGL(k_list_scan, dim3(1), dim3(1024), 0, s, a.list_n, groups, L->off);                 \
+            if (groups < 8192u)                                                                                   \
+                hipLaunchKernelGGL((k_adj_lists32<Q, 8>), dim3(bgrid), dim3(512), 0, s, a);                       \
+            else                                                                                                  \
+                hipLaunchKernelGGL((k_adj_lists32<Q, 4>), dim3(bgrid), dim3(256), 0, s, a);                       \
+            e = radix_sort_pairs(L->key, L->val, L->key_alt, L->val_alt, L->max_pairs, L->hist, L->status,       \
+                                 L->tile_ctr, c->ctr, s, L->passes, L->off + groups);                             \
+            if (e != hipSuccess) return e;                                                                        \
+            hipLaunchKernelGGL(k_fill_runs, dim3(kgrid), dim3(256), 0, s, L->run, L->nkeys);                      \
+            hipLaunchKernelGGL(k_run_bounds, dim3(rgrid), dim3(256), 0, s, (L->passes & 1) ? L->key_alt : L->key,  \
+                               L->off + groups, L->run);                                                          \
+            hipLaunchKernelGGL((k_run_sum<Q ? 10 : 4>), dim3(wgrid), dim3(256), 0, s, L->run,                     \
+                               (L->passes & 1) ? L->val_alt : L->val, L->part, L->node_cap, L->nkeys, gnode,       \
+                               gbody);                                                                            \
+        } else {                                                                                                  \
             hipLaunchKernelGGL((k_walk_group<double, Q, G, true>), dim3(groups), dim3(64), 0, s, a);              \
+        }                                                                                                         \
     } while (0)
     if (quad) { if (guard) BHX_ADJ(true, true); else BHX_ADJ(true, false); }
     else { if (guard) BHX_ADJ(false, true); else BHX_ADJ(false, false); }

@@ -169,15 +169,44 @@ def test_adjoint_independent_of_checkpoint_interval(bhx):
     pos, vel, mass = _bodies(bhx, n, 12)
     gx, gv = _weights(n, 13)
     res = []
-    for every in (1, 4, 100):
+    # every frame kept by the recording (default memory), or the segments re-run (none kept)
+    for every, memory in ((1, None), (4, None), (100, None), (4, 0), (100, 0)):
         with bhx.Engine(n, **DIFF) as e:
+            e.set_record_memory(memory)
             e.set_bodies(pos, vel, mass)
             e.record(T, checkpoint_every=every, init_acc=True)
+            assert e.record_keeps_frames() == (memory is None)
             res.append(e.adjoint(gx, gv))
     for r in res[1:]:
         for k in (0, 1, 3):
             # FP64 atomics sum entry terms in any order: last-bit differences only
             assert np.allclose(r[k], res[0][k], rtol=1e-12, atol=1e-12 * np.abs(res[0][k]).max())
+
+
+@pytest.mark.parametrize("quad", [0, 1])
+def test_fp32_adjoint_repeats_bitwise(bhx, quad):
+    """FP32 walk: the entry sums are added in a fixed order (sorted by entry, no
+    atomics), so the gradients repeat bit for bit: twice on one recording, with
+    the frames kept or re-run, and for any checkpoint interval."""
+    n, T = 3000, 8
+    pos, vel, mass = _bodies(bhx, n, 40 + quad)
+    gx, gv = _weights(n, 41)
+    cfg = dict(DIFF, precision=1, quadrupole=quad, leaf_capacity=16)
+    res = []
+    for every, memory in ((3, None), (3, 0), (8, 0), (1, None)):
+        with bhx.Engine(n, **cfg) as e:
+            e.set_record_memory(memory)
+            e.set_bodies(pos, vel, mass)
+            e.record(T, checkpoint_every=every, init_acc=True)
+            assert e.record_keeps_frames() == (memory is None)
+            g = e.adjoint(gx, gv)
+            again = e.adjoint(gx, gv)
+            for k in (0, 1, 3):
+                assert np.array_equal(g[k], again[k]), (every, memory, k)
+            res.append(g)
+    for r in res[1:]:
+        for k in (0, 1, 3):
+            assert np.array_equal(r[k], res[0][k]), k
 
 
 def test_record_matches_plain_frames_and_leaves_end_state(bhx):
