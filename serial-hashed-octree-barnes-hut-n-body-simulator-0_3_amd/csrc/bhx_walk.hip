@@ -880,273 +880,18 @@ __device__ __forceinline__ void eval_list32(const float (*nd)[4], const float (*
     }
 }
 
-// List capacities of the FP32 walk, sized so that one workgroup (one wave)
-// uses under 5 KiB of LDS and eight waves fit on every SIMD.  The adjoint
-// walk scatters with one lane per list entry and keeps 128 bodies, so that
-// its body flushes fill both halves of the scatter.
+// List capacities of the FP32 walk: 64 cells per flush (one per lane), 32
+// leaf ranges.
 constexpr int kCells32 = 64;
 constexpr int kRanges32 = 32;
-template <bool kAdj>
-constexpr int bodies32() { return kAdj ? 128 : 96; }
-
-// Accepted cells of the FP32 walk are queued as node ids only, one per lane
-// of a vector register (lane j holds entry j: kCells32 is the wave width); a
-// flush gathers them lane-parallel (lane j fetches entry j's record) into the
-// group-relative FP32 entries the evaluation reads.
 static_assert(kCells32 == 64, "the FP32 cell list is one id per lane");
-template <bool kQuad>
-__device__ __forceinline__ void gather_cells32(float (*nd)[4], float (*q)[6], uint32_t id, int nn,
-                                               uint32_t lane, const WalkPay32* __restrict__ pay,
-                                               const double* __restrict__ com, double ox, double oy, double oz,
-                                               double lam) {
-    {
-        const int e = (int)lane;
-        if (e < nn) {
-            const WalkPay32& c = pay[id];
-            nd[e][0] = (float)((com[3 * (size_t)id] - ox) * lam);
-            nd[e][1] = (float)((com[3 * (size_t)id + 1] - oy) * lam);
-            nd[e][2] = (float)((com[3 * (size_t)id + 2] - oz) * lam);
-            nd[e][3] = c.mass;
-            if (kQuad) {
-#pragma unroll
-                for (int u = 0; u < 6; ++u) q[e][u] = c.q[u];
-            }
-        }
-    }
-}
-
-template <bool kQuad, bool kGuard, bool kAdj = false, bool kPot = false>
-__global__ __launch_bounds__(64) void k_walk_group32(WalkArgs a) {
-    static_assert(!(kAdj && kPot), "the adjoint walk has no potential");
-    constexpr int kBodies32 = bodies32<kAdj>();
-    __shared__ float s_nd[kCells32][4];
-    __shared__ float s_q[kQuad ? kCells32 : 1][6];
-    __shared__ uint32_t s_nid[kAdj ? kCells32 : 1];   // the adjoint's scatter reads the ids from LDS
-    __shared__ float s_bd[kBodies32][4];
-    __shared__ uint32_t s_rstart[kRanges32], s_roff[kRanges32];
-    __shared__ float s_rdel[kRanges32][3];    // leaf's centre minus the group origin
-    __shared__ uint32_t s_rleaf[kRanges32];   // leaf node of each range
-    __shared__ uint32_t s_bid[kAdj ? kBodies32 : 1];
-    __shared__ float s_tg[kAdj ? 64 : 1][6];
-
-    const uint32_t grp = xcd_group(blockIdx.x, a.groups);
-    if (grp >= a.groups || *a.abort_flag) return;
-    const uint4 gr = a.gtab[grp];
-    const uint32_t g0 = gr.x;
-    const uint32_t lane = threadIdx.x;
-    const uint32_t i = g0 + lane;
-    const bool valid = i < gr.y;
-    const uint64_t vmask = __builtin_amdgcn_ballot_w64(valid);
-    const uint32_t iv = valid ? i : g0;
-    const uint32_t rid = a.sys_root[gr.z];
-    ConstU64 raw = (ConstU64)a.nodes32;
-    ConstD tx = (ConstD)a.tx;
-    ConstD ty = (ConstD)a.ty;
-    ConstD tz = (ConstD)a.tz;
-    double ox = tx[g0], oy = ty[g0], oz = tz[g0];
-    // keep the origin in vector registers: the per-cell v_add_f64 then reads
-    // the cell centre as its one scalar operand instead of copying the origin
-    // out of scalar registers on every visit
-    asm volatile("" : "+v"(ox), "+v"(oy), "+v"(oz));
-    // own position = (own leaf's centre - origin) + own offset: the same two
-    // roundings the body gets as a list entry
-    const uint32_t own = a.tleaf[iv];
-    const float4 ob = a.tb4[iv];
-    // FP32 quantities are in the system's scaled units (exact powers of two)
-    const double lam = a.scale32[2 * (size_t)gr.z], mu = a.scale32[2 * (size_t)gr.z + 1];
-    const float xr = (float)((a.com[3 * (size_t)own] - ox) * lam) + ob.x;
-    const float yr = (float)((a.com[3 * (size_t)own + 1] - oy) * lam) + ob.y;
-    const float zr = (float)((a.com[3 * (size_t)own + 2] - oz) * lam) + ob.z;
-    const float eps2 = (float)(a.eps2 * lam * lam);
-    // the opening test edge^2 < theta^2 d^2 in FP32 coordinates relative to the
-    // system's root centre, the frame of the records' centres (the decision
-    // only; theta = 0 opens every cell)
-    const double* b4s = a.b4 + 4 * (size_t)gr.z;
-    const float xs = (float)((tx[iv] - b4s[0]) * lam), ys = (float)((ty[iv] - b4s[1]) * lam),
-                zs = (float)((tz[iv] - b4s[2]) * lam);
-    const float th2 = a.theta2f;
-    float ax = 0.f, ay = 0.f, az = 0.f, phi = 0.f;
-    float wxr = 0.f, wyr = 0.f, wzr = 0.f;
-    if constexpr (kAdj) {
-        if (valid) {
-            const uint32_t o = a.tperm[i];
-            wxr = (float)a.wx[o];
-            wyr = (float)a.wy[o];
-            wzr = (float)a.wz[o];
-        }
-        s_tg[lane][0] = xr; s_tg[lane][1] = yr; s_tg[lane][2] = zr;
-        s_tg[lane][3] = wxr; s_tg[lane][4] = wyr; s_tg[lane][5] = wzr;
-    }
-    uint32_t cell_ids = 0;   // lane j: node id of queued cell j
-    auto flush_cells = [&](int nn) {
-        if constexpr (kAdj) {
-            if ((int)lane < nn) s_nid[lane] = cell_ids;
-        }
-        __syncthreads();
-        gather_cells32<kQuad>(s_nd, s_q, cell_ids, nn, lane, a.pay32, a.com, ox, oy, oz, lam);
-        __syncthreads();
-        if constexpr (kAdj) {
-            adj_self<float, kQuad>(s_nd, s_q, nn, xr, yr, zr, wxr, wyr, wzr, eps2, ax, ay, az);
-            adj_scatter<float, kQuad>(s_nd, s_q, s_nid, nn, s_tg, eps2, lane, a.gnode, lam * lam * lam / mu, lam * lam,
-                                      (lam * lam) * (lam * lam));
-        } else {
-            eval_list32<kQuad, kGuard, kPot>(s_nd, s_q, nn, xr, yr, zr, eps2, ax, ay, az, phi);
-        }
-        __syncthreads();
-    };
-    auto flush_bodies = [&](int nr, int nq) {
-        __syncthreads();
-        // each range's leaf centre relative to the group origin, from its FP64 centre
-        if ((int)lane < nr) {
-            const size_t l3 = 3 * (size_t)s_rleaf[lane];
-            s_rdel[lane][0] = (float)((a.com[l3] - ox) * lam);
-            s_rdel[lane][1] = (float)((a.com[l3 + 1] - oy) * lam);
-            s_rdel[lane][2] = (float)((a.com[l3 + 2] - oz) * lam);
-        }
-        __syncthreads();
-#pragma unroll
-        for (int h = 0; h < (kBodies32 + 63) / 64; ++h) {
-            const int e = (int)lane + 64 * h;
-            if (e < nq) {
-                int lo = 0, hi = nr - 1;
-                while (lo < hi) {
-                    const int mid = (lo + hi + 1) >> 1;
-                    if ((int)s_roff[mid] <= e) lo = mid; else hi = mid - 1;
-                }
-                const uint32_t j = s_rstart[lo] + (uint32_t)(e - (int)s_roff[lo]);
-                const float4 b = a.tb4[j];
-                s_bd[e][0] = s_rdel[lo][0] + b.x;
-                s_bd[e][1] = s_rdel[lo][1] + b.y;
-                s_bd[e][2] = s_rdel[lo][2] + b.z;
-                s_bd[e][3] = b.w;
-                if constexpr (kAdj) s_bid[e] = j;
-            }
-        }
-        __syncthreads();
-        if constexpr (kAdj) {
-            adj_self<float>(s_bd, s_q, nq, xr, yr, zr, wxr, wyr, wzr, eps2, ax, ay, az);
-            adj_scatter<float>(s_bd, s_q, s_bid, nq, s_tg, eps2, lane, a.gbody, lam * lam * lam / mu, lam * lam);
-        } else {
-            eval_list32<false, kGuard, kPot>(s_bd, s_q, nq, xr, yr, zr, eps2, ax, ay, az, phi);
-        }
-        __syncthreads();
-    };
-    // The traversal fills the lists without touching the accumulators; it
-    // stops when a list needs evaluating — the cell list is full, or the next
-    // leaf chunk (up to 64 bodies) does not fit the body or range list; that
-    // leaf is visited again after the flush — and the outer loop evaluates
-    // it.  A leaf of more than 64 bodies is queued one chunk per visit (kres =
-    // the next chunk).  Only the root has level 0, and the root is never
-    // accepted: a leaf root queues its bodies before the loop.
-    uint32_t cnt = 0;
-    int nn = 0, nr = 0, nq = 0;
-    const Node32 root = load_node32(raw, rid);
-    const uint32_t end = root.skip;
-    uint32_t p = rid + 1;
-    uint32_t kres = 0;
-    if (root.meta & 32u) {
-        const uint32_t count = root.meta >> 6;
-        for (uint32_t k = 0; k < count; k += 64) {
-            const int m = (int)((count - k) < 64u ? (count - k) : 64u);
-            if (nq + m > kBodies32 || nr == kRanges32) {
-                flush_bodies(nr, nq);
-                cnt += (uint32_t)nq;
-                nr = 0;
-                nq = 0;
-            }
-            if (lane == 0) {
-                s_rstart[nr] = root.start + k;
-                s_roff[nr] = (uint32_t)nq;
-                s_rleaf[nr] = rid;
-            }
-            ++nr;
-            nq += m;
-        }
-        p = end;
-    }
-    while (p < end) {
-        bool bodies_full = false;
-        for (;;) {
-            p = __builtin_amdgcn_readfirstlane(p);
-            const Node32 nd = load_node32(raw, p);
-            const uint32_t nskip = nd.skip > p ? nd.skip : p + 1;   // always forward
-            const float dx = nd.cx - xs, dy = nd.cy - ys, dz = nd.cz - zs;
-            const float d2 = dx * dx + dy * dy + dz * dz;
-            const bool opened = (__builtin_amdgcn_ballot_w64(!(nd.size2 < th2 * d2)) & vmask) != 0ull;
-            if (!opened) {
-                cell_ids = lane == (uint32_t)nn ? p : cell_ids;
-                ++nn;
-                p = nskip;
-                if (nn == kCells32 || p >= end) break;
-                continue;
-            }
-            if (!(nd.meta & 32u)) {
-                p = p + 1;   // first child: inside the subtree, before end
-                continue;
-            }
-            {
-                const uint32_t count = nd.meta >> 6;
-                const int m = (int)((count - kres) < 64u ? (count - kres) : 64u);
-                if (nq + m > kBodies32 || nr == kRanges32) {
-                    bodies_full = true;
-                    break;
-                }
-                if (lane == 0) {
-                    s_rstart[nr] = nd.start + kres;
-                    s_roff[nr] = (uint32_t)nq;
-                    s_rleaf[nr] = p;
-                }
-                ++nr;
-                nq += m;
-                kres += 64u;
-                if (kres >= count) {
-                    kres = 0;
-                    p = nskip;
-                }
-            }
-            if (p >= end) break;
-        }
-        if (nn == kCells32) {
-            flush_cells(nn);
-            cnt += (uint32_t)nn;
-            nn = 0;
-        }
-        if (bodies_full) {
-            flush_bodies(nr, nq);
-            cnt += (uint32_t)nq;
-            nr = 0;
-            nq = 0;
-        }
-    }
-    flush_cells(nn);
-    flush_bodies(nr, nq);
-    cnt += (uint32_t)(nn + nq);
-    if (valid) {
-        const uint32_t o = a.tperm[i];
-        // back to physical units (exact): a * lam^2 / mu, phi * lam / mu, and
-        // for the adjoint's dL/dx of the body's own position lam^3 / mu
-        if constexpr (kAdj) {
-            const double back = lam * lam * lam / mu;
-            a.gself[3 * (size_t)o] = (double)ax * back;
-            a.gself[3 * (size_t)o + 1] = (double)ay * back;
-            a.gself[3 * (size_t)o + 2] = (double)az * back;
-        } else {
-            const double back = lam * lam / mu;
-            a.ax[o] = (double)ax * back;
-            a.ay[o] = (double)ay * back;
-            a.az[o] = (double)az * back;
-            if constexpr (kPot) a.pot[o] = (double)phi * (lam / mu);
-        }
-        if (a.counts) a.counts[o] = cnt;
-    }
-}
 
 // ---------------------------------------------------------------- lane-parallel FP32 walk
 //
-// The shared-list rule of k_walk_group32 (a cell enters the group's list only
-// when every body of the group accepts it, and is opened as soon as one body
-// opens it), with the traversal spread over the lanes instead of one cell per
-// visit.  Each step takes up to 64 cells from a stack in LDS, one per lane;
+// The shared-list rule of the group walks (a cell enters the group's list
+// only when every body of the group accepts it, and is opened as soon as one
+// body opens it), with the traversal spread over the lanes instead of one
+// cell per visit.  Each step takes up to 64 cells from a stack in LDS, one per lane;
 // lane j tests its cell against every body of the group at once through the
 // least |d|^2 over the bodies (fl(theta^2 x) is monotone in x, so every body
 // accepts exactly when edge^2 < theta^2 min|d|^2); accepted cells join the
@@ -1154,7 +899,7 @@ __global__ __launch_bounds__(64) void k_walk_group32(WalkArgs a) {
 // (from the octant table, one 32-byte load per opened cell) go on the stack.
 // A group's ~2,800 dependent visits become ~50 independent steps.  Should
 // the stack not hold the children of every opened cell, the cells left over
-// are walked one cell at a time, as k_walk_group32 does (no cell is lost).
+// are walked one cell at a time along the pre-order (no cell is lost).
 //
 // Persistent: the grid is what the chip holds at once.  XCD x (workgroups
 // x, x + 8, ... are dispatched to it) owns the contiguous tree-order slice
