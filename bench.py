@@ -487,7 +487,7 @@ def main():
                    "n": n, "theta": theta, "eps": eps, "leaf_capacity": cap, "dt": dt},
         "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / PEAK_FP32_TFLOPS, "traffic": read_traffic(n),
-                     "kernel": "k_walk_lp32<true, false, false, 1>", "kernel_ms": walk_ms,
+                     "kernel": "k_walk_lp32<true, false, false, 1, false>", "kernel_ms": walk_ms,
                      "work": "%d reference-list interactions x %d flops (group lists evaluate %d)" % (
                          out["fp32"]["inter_ref"], FLOPS_PER_INTERACTION, out["fp32"]["inter"])},
         "force_error_fp32": out["fp32"]["err"],
