@@ -963,6 +963,18 @@ __device__ __forceinline__ float wave_maxf(float v) {
     return v;
 }
 
+__device__ __forceinline__ double wave_mind(double v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v = fmin(v, __shfl_xor(v, off, 64));
+    return v;
+}
+
+__device__ __forceinline__ double wave_maxd(double v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off, 64));
+    return v;
+}
+
 // Evaluation of the lane-parallel walk's lists.  Sub-group q's cells are the
 // K = 64 / S entries nd[q K .. q K + K); lane l (sub-group l / K) walks its
 // own K entries.  Bodies are one list for the whole wave; a body entry counts
@@ -1733,6 +1745,7 @@ __global__ __launch_bounds__(64) void k_walk_lp64(WalkArgs a) {
     __shared__ double4 s_tgt[64];
     __shared__ uint32_t s_cid[kLpCells];
     __shared__ uint32_t s_stack[kLpStack];
+    __shared__ uint32_t s_und[kLpUnd];        // cells the bounds leave undecided
 
     if (*a.abort_flag) return;
     const uint32_t lane = threadIdx.x;
@@ -1866,12 +1879,32 @@ __global__ __launch_bounds__(64) void k_walk_lp64(WalkArgs a) {
             }
         }
         __syncthreads();
-        while (top > 0) {
-            const int k = top < 64 ? top : 64;
-            const int base = top - k;
+        // Most cells need no least |d|^2 over the 64 bodies (the FP32 walk's
+        // rule): with lo/hi the group's bounding box, fl(c - clamp(c, lo, hi))
+        // per axis is at most |fl(c - x_b)| for every body (rounding is
+        // monotone), so dist2 of it is at most every body's, and fl(s^2 / x) is
+        // non-increasing in x: the test passing at the box means every body
+        // accepts.  Failing at one of four sample bodies means some body opens.
+        // The other cells wait in s_und for the exact test, 64 at a time.
+        const double blo_x = wave_mind(valid ? x : INFINITY), bhi_x = wave_maxd(valid ? x : -INFINITY);
+        const double blo_y = wave_mind(valid ? y : INFINITY), bhi_y = wave_maxd(valid ? y : -INFINITY);
+        const double blo_z = wave_mind(valid ? z : INFINITY), bhi_z = wave_maxd(valid ? z : -INFINITY);
+        const int nv = __popcll(vmask);        // the valid bodies are lanes 0 .. nv - 1
+        int nund = 0;
+        while (top > 0 || nund > 0) {
+            const bool exact = nund >= 64 || top == 0;
+            const int have = exact ? nund : top;
+            const int k = have < 64 ? have : 64;
+            const int base = have - k;
             const bool live = (int)lane < k;
-            const uint32_t p = live ? s_stack[base + lane] : rid;
-            top = base;
+            uint32_t p;
+            if (exact) {
+                p = live ? s_und[base + lane] : rid;
+                nund = base;
+            } else {
+                p = live ? s_stack[base + lane] : rid;
+                top = base;
+            }
             const uint4 r0 = rec4[4 * (size_t)p], r1 = rec4[4 * (size_t)p + 1], r2 = rec4[4 * (size_t)p + 2];
             const double cx = __hiloint2double((int)r0.y, (int)r0.x), cy = __hiloint2double((int)r0.w, (int)r0.z);
             const double cz = __hiloint2double((int)r1.y, (int)r1.x);
@@ -1880,18 +1913,39 @@ __global__ __launch_bounds__(64) void k_walk_lp64(WalkArgs a) {
             const uint4 r3 = rec4[4 * (size_t)p + 3];
             const uint32_t count = r3.x, ll = r3.y;
             const uint32_t cb = live ? a.child_byte[p] : 0u;
-            double md2 = INFINITY;
+            bool pass = false, decided = live;
+            if (exact) {
+                double md2 = INFINITY;
 #pragma unroll 8
-            for (int b = 0; b < 64; ++b) {
-                const double4 t = s_tgt[b];
-                const double dx = cx - t.x, dy = cy - t.y, dz = cz - t.z;
-                md2 = fmin(md2, dist2(dx, dy, dz));
+                for (int b = 0; b < 64; ++b) {
+                    const double4 t = s_tgt[b];
+                    const double dx = cx - t.x, dy = cy - t.y, dz = cz - t.z;
+                    md2 = fmin(md2, dist2(dx, dy, dz));
+                }
+                pass = live && mac_t(size2, md2, th2, 0.f, 0.f);
+            } else {
+                const double d2b = dist2(cx - fmin(fmax(cx, blo_x), bhi_x), cy - fmin(fmax(cy, blo_y), bhi_y),
+                                         cz - fmin(fmax(cz, blo_z), bhi_z));
+                bool open_sure = false;
+#pragma unroll
+                for (int h = 0; h < 4; ++h) {
+                    const double4 t = s_tgt[(nv * h) / 4 + nv / 8];
+                    open_sure = open_sure || !mac_t(size2, dist2(cx - t.x, cy - t.y, cz - t.z), th2, 0.f, 0.f);
+                }
+                const bool acc_sure = mac_t(size2, d2b, th2, 0.f, 0.f);
+                const bool und = live && !acc_sure && !open_sure;
+                const uint64_t mu = __builtin_amdgcn_ballot_w64(und);
+                if (und)
+                    s_und[nund + (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(mu >> 32),
+                                                                __builtin_amdgcn_mbcnt_lo((uint32_t)mu, 0u))] = p;
+                nund += __popcll(mu);
+                pass = live && acc_sure;
+                decided = live && !und;
             }
-            const bool pass = live && mac_t(size2, md2, th2, 0.f, 0.f);
             const bool leaf = (ll & 256u) != 0u;
-            const bool open_in = live && !pass && !leaf;
+            const bool open_in = decided && !pass && !leaf;
             const uint64_t m_acc = __builtin_amdgcn_ballot_w64(pass);
-            const uint64_t m_leaf = __builtin_amdgcn_ballot_w64(live && !pass && leaf);
+            const uint64_t m_leaf = __builtin_amdgcn_ballot_w64(decided && !pass && leaf);
             const uint32_t nch = open_in ? (uint32_t)__popc(cb) : 0u;
             const uint32_t incl = wave_incl_scan(nch, lane);
             const uint32_t room = (uint32_t)(kLpStack - top);
