@@ -385,9 +385,8 @@ int bhx_record_digest(bhx_ctx *ctx, uint64_t *digest);
  * did not keep them, from the frames of each checkpoint segment re-run
  * bitwise as recorded.  dL/da_0 is 0 when bhx_record computed a_0 (its
  * adjoint flows into dL/dx_0 and dL/dm).  The bodies are left at the end of
- * the recording.  With the FP32 walk every sum has a fixed order, so results
- * repeat bit for bit; the FP64 walk's entry sums use FP64 atomics, so its
- * results may differ in the last bits between runs. */
+ * the recording.  Every sum has a fixed order (no atomics), so results repeat
+ * bit for bit. */
 int bhx_adjoint(bhx_ctx *ctx, const double *gx, const double *gv, const double *ga,
                 double *gx0, double *gv0, double *gm, double *ga0);
 

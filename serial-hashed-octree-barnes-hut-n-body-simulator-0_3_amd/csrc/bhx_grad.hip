@@ -255,6 +255,7 @@ struct GradSlot {
     double* scale32 = nullptr;
     double* quad = nullptr;                    // FP64 walk with quadrupoles: 6 per node
     uint32_t* child = nullptr;                 // octant table, 8 per node
+    uint32_t* child_byte = nullptr;            // octant mask, 1 per node
 };
 
 struct GradRecord {
@@ -315,7 +316,7 @@ size_t slot_bytes(size_t n, size_t nodes, size_t nsys, bool quad) {
     return align_up(4 * n) + 4 * align_up(8 * n) + align_up(4 * n) + align_up(16 * n) + align_up(4 * n) +
            align_up(64 * nodes) + align_up(4 * nodes) + align_up(24 * nodes) + align_up(8 * nodes) +
            align_up(4 * nsys) + align_up(32 * nsys) + align_up(16 * nsys) + (quad ? align_up(48 * nodes) : 0) +
-           align_up(32 * nodes);
+           align_up(32 * nodes) + align_up(4 * nodes);
 }
 
 GradSlot carve_slot(char*& p, size_t n, size_t nodes, size_t nsys, int precision, bool quad) {
@@ -348,6 +349,7 @@ GradSlot carve_slot(char*& p, size_t n, size_t nodes, size_t nsys, int precision
     s.scale32 = (double*)take(16 * nsys);
     if (quad) s.quad = (double*)take(48 * nodes);
     s.child = (uint32_t*)take(32 * nodes);
+    s.child_byte = (uint32_t*)take(4 * nodes);
     return s;
 }
 
@@ -422,6 +424,7 @@ int store_frame(bhx_ctx* c, const GradSlot& d, size_t nodes, int precision, bool
     add_job(J, c->sys_scale, d.scale32, 16 * c->nsys);
     if (d.quad) add_job(J, c->tree.quad, d.quad, 48 * nodes);
     add_job(J, c->tree.child, d.child, 32 * nodes);
+    add_job(J, c->tree.child_byte, d.child_byte, 4 * nodes);
     HIPCHK(run_jobs(J, 16 * (n > nodes ? n : nodes), s));
     return BHX_OK;
 }
@@ -441,6 +444,7 @@ WalkTree tree_of(const GradSlot& d) {
     t.b4 = d.b4;
     t.quad = d.quad;
     t.child = d.child;
+    t.child_byte = d.child_byte;
     return t;
 }
 
@@ -497,7 +501,7 @@ int adjoint_walk(bhx_ctx* c, GradRecord* g, const Work& k, const GradSlot& d, co
     return BHX_OK;
 }
 
-// The FP32 adjoint's list regions and entry records, in one allocation.  A
+// The adjoint's list regions and entry records, in one allocation.  A
 // region per walk group of 8,192 entries (about twice a typical 64-body group
 // list), fewer when there are many groups (at most 2^26 entries in all); what
 // does not fit is evaluated by the traversal itself.
@@ -773,7 +777,7 @@ int bhx_adjoint(bhx_ctx* c, const double* gx, const double* gv, const double* ga
         (void)hipFree(g->lists_mem);
         g->lists_mem = nullptr;
     }
-    if (g->precision == BHX_FP32 && !g->lists_mem) {
+    if (!g->lists_mem) {
         int rc = alloc_adj_lists(c, g);
         if (rc != BHX_OK) return rc;
     }

@@ -415,7 +415,8 @@ struct WalkTree {
     const uint32_t* sys_root;
     const double* b4;
     const double* quad;     // FP64 quadrupoles, 6 per node (FP64 walk with quadrupoles)
-    const uint32_t* child;  // octant table: 8 child ids per node (the FP32 adjoint's traversal)
+    const uint32_t* child;  // octant table: 8 child ids per node (the adjoint's traversal)
+    const uint32_t* child_byte;  // octant mask per node (the FP64 adjoint's traversal)
 };
 // Adjoint of the group walk on tree t with frozen lists: per body weights w
 // (array order) -> dL/dx of the own position (gself, 3 per body, array
@@ -445,8 +446,7 @@ struct AdjLists {
     size_t max_pairs = 0, groups = 0, nkeys = 0;
     int passes = 2;
 };
-// The FP32 walk needs `L` (its stored lists and their evaluation); the FP64
-// walk sums the entries with atomics and ignores it.
+// `L`: the walk's stored lists and their evaluation (both precisions).
 hipError_t launch_walk_adjoint(bhx_ctx* c, const WalkTree& t, int precision, const double* wx, const double* wy,
                                const double* wz, double* gself, double* gnode, double* gbody, const AdjLists* L,
                                hipStream_t s);

@@ -340,27 +340,22 @@ __global__ __launch_bounds__(kBlock) void k_walk32(WalkArgs a) {
     }
 }
 
-// ---------------------------------------------------------------- group walk
+// ---------------------------------------------------------------- group walks: the shared terms
 //
-// One wave walks for a group of 64 consecutive tree-order bodies and builds
-// ONE interaction list that every lane evaluates.  Each lane applies the
+// The group walks (k_walk_lp32, k_walk_lp64) build ONE interaction list for
+// a group of 64 consecutive tree-order bodies that every lane evaluates: a
+// cell enters the shared list only when every body accepts it under the
 // reference opening test (size^2/d^2 < theta^2, d to the cell's centre of
-// mass) to every cell the wave visits; a cell enters the shared list only
-// when every lane accepts it, and is opened as soon as one lane opens it.
-// The shared list therefore refines each body's own reference list (a cell
-// is replaced by its children wherever any neighbour needs them), so no body
-// is approximated more coarsely than the reference would, and all 64 lanes
-// do useful work on every entry.  Accepted cells go to an LDS multipole list,
-// opened leaves to a queue of body ranges; full lists are evaluated densely.
-// Node records arrive as one 64-byte scalar burst and the bodies of queued
-// leaves in one batch of loads per flush.  FP32 works in offsets from the
-// group's first body (only the offsets are rounded); FP64 keeps absolute
-// coordinates and the reference's unfused opening test, and forms the term's
-// 1/sqrt(D2) and 1/D2 from a Newton-refined reciprocal square root (the
-// per-body walk k_walk64 keeps the reference's rounded divisions).
-constexpr int kNodeCap = 128;
-constexpr int kBodyCap = 128;
-constexpr int kRangeCap = 64;
+// mass), and is opened as soon as one body opens it.  The shared list
+// therefore refines each body's own reference list (a cell is replaced by
+// its children wherever any neighbour needs them), so no body is
+// approximated more coarsely than the reference would, and all 64 lanes do
+// useful work on every entry.  Below: the list terms and their adjoints.
+// FP32 works in offsets from the group's first body (only the offsets are
+// rounded); FP64 keeps absolute coordinates and the reference's unfused
+// opening test, and forms the term's 1/sqrt(D2) and 1/D2 from a
+// Newton-refined reciprocal square root (the per-body walk k_walk64 keeps
+// the reference's rounded divisions).
 
 template <typename T>
 __device__ __forceinline__ T rsqrt_t(T r2);
@@ -610,179 +605,6 @@ __device__ __forceinline__ void adj_scatter(const T (*ent)[4], const T (*q)[6], 
     });
 }
 
-// Bodies of the queued leaf ranges into LDS with every load in flight at
-// once: lane l fills slots l and l+64, finding each slot's range by bisection
-// over the ranges' first slots.  `bid` (adjoint walk) receives each slot's
-// tree-order body index.
-template <typename T>
-__device__ __forceinline__ void gather_bodies(T (*bd)[4], const uint32_t* rstart, const uint32_t* roff, int nr, int nq,
-                                              uint32_t lane, const WalkArgs& a, double ox, double oy, double oz,
-                                              uint32_t* bid = nullptr) {
-#pragma unroll
-    for (int h = 0; h < kBodyCap / 64; ++h) {
-        const int e = (int)lane + 64 * h;
-        if (e < nq) {
-            int lo = 0, hi = nr - 1;
-            while (lo < hi) {
-                const int mid = (lo + hi + 1) >> 1;
-                if ((int)roff[mid] <= e) lo = mid; else hi = mid - 1;
-            }
-            const uint32_t j = rstart[lo] + (uint32_t)(e - (int)roff[lo]);
-            const double bx = a.tx[j], by = a.ty[j], bz = a.tz[j], bm = a.tm[j];
-            bd[e][0] = (T)(bx - ox);
-            bd[e][1] = (T)(by - oy);
-            bd[e][2] = (T)(bz - oz);
-            bd[e][3] = (T)bm;
-            if (bid) bid[e] = j;
-        }
-    }
-}
-
-template <typename T, bool kQuad, bool kGuard, bool kAdj = false, bool kPot = false>
-__global__ __launch_bounds__(64) void k_walk_group(WalkArgs a) {
-    static_assert(!(kAdj && kPot), "the adjoint walk has no potential");
-    __shared__ T s_nd[kNodeCap][4];
-    __shared__ T s_q[kQuad ? kNodeCap : 1][6];
-    __shared__ T s_bd[kBodyCap][4];
-    __shared__ uint32_t s_rstart[kRangeCap], s_roff[kRangeCap];
-    __shared__ uint32_t s_nid[kAdj ? kNodeCap : 1], s_bid[kAdj ? kBodyCap : 1];
-    __shared__ T s_tg[kAdj ? 64 : 1][6];
-
-    if (blockIdx.x >= a.groups || *a.abort_flag) return;
-    const uint4 gr = a.gtab[blockIdx.x];
-    const uint32_t g0 = gr.x;
-    const uint32_t lane = threadIdx.x;
-    const uint32_t i = g0 + lane;
-    const bool valid = i < gr.y;
-    const uint32_t iv = valid ? i : g0;
-    const uint32_t rid = a.sys_root[gr.z];
-    ConstU64 raw = (ConstU64)a.nodes;
-    ConstF qf = (ConstF)a.quadf;
-    ConstD qd = (ConstD)a.quad;
-    const bool rel = sizeof(T) == 4;
-    const double ox = rel ? a.tx[g0] : 0.0, oy = rel ? a.ty[g0] : 0.0, oz = rel ? a.tz[g0] : 0.0;
-    const T xr = (T)(a.tx[iv] - ox), yr = (T)(a.ty[iv] - oy), zr = (T)(a.tz[iv] - oz);
-    const T eps2 = (T)a.eps2;
-    const double th2d = a.theta2;
-    const float th2f = a.theta2f;
-    T ax = 0, ay = 0, az = 0, phi = 0;
-    T wxr = 0, wyr = 0, wzr = 0;
-    if constexpr (kAdj) {
-        if (valid) {
-            const uint32_t o = a.tperm[i];
-            wxr = (T)a.wx[o];
-            wyr = (T)a.wy[o];
-            wzr = (T)a.wz[o];
-        }
-        s_tg[lane][0] = xr; s_tg[lane][1] = yr; s_tg[lane][2] = zr;
-        s_tg[lane][3] = wxr; s_tg[lane][4] = wyr; s_tg[lane][5] = wzr;
-    }
-    uint32_t cnt = 0;
-    int nn = 0, nr = 0, nq = 0;            // queued cells, leaf ranges, bodies in those ranges
-    const WalkNode root = load_node(raw, rid);
-    const uint32_t end = root.skip;
-    uint32_t p = (root.level_leaf & 256u) ? rid : rid + 1;
-    while (p < end) {
-        p = __builtin_amdgcn_readfirstlane(p);
-        const WalkNode nd = load_node(raw, p);
-        // a subtree always ends after its root; the clamp keeps the walk moving
-        // forward (and finite) whatever the record holds
-        const uint32_t nskip = nd.skip > p ? nd.skip : p + 1;
-        const uint32_t L = nd.level_leaf & 255u;
-        const bool leaf = (nd.level_leaf & 256u) != 0;
-        const T cx = (T)(nd.cx - ox), cy = (T)(nd.cy - oy), cz = (T)(nd.cz - oz);
-        const T dx = cx - xr, dy = cy - yr, dz = cz - zr;
-        const T d2 = dist2(dx, dy, dz);
-        const bool pass = (L != 0) & mac_t((T)nd.size2, d2, (T)th2d, nd.size2f, th2f);
-        const bool opened = __ballot(valid & !pass) != 0ull;
-        if (!opened) {
-            if (nn == kNodeCap) {
-                __syncthreads();
-                if constexpr (kAdj) {
-                    adj_self<T, kQuad>(s_nd, s_q, nn, xr, yr, zr, wxr, wyr, wzr, eps2, ax, ay, az);
-                    adj_scatter<T, kQuad>(s_nd, s_q, s_nid, nn, s_tg, eps2, lane, a.gnode);
-                } else {
-                    eval_nodes<T, kQuad, kGuard, kPot>(s_nd, s_q, nn, xr, yr, zr, eps2, ax, ay, az, phi);
-                }
-                cnt += (uint32_t)nn;
-                nn = 0;
-                __syncthreads();
-            }
-            if (lane == 0) {
-                s_nd[nn][0] = cx;
-                s_nd[nn][1] = cy;
-                s_nd[nn][2] = cz;
-                s_nd[nn][3] = sizeof(T) == 4 ? (T)nd.massf : (T)nd.mass;
-                if constexpr (kAdj) s_nid[nn] = p;
-            }
-            if (kQuad && lane < 6) {
-                const bool many = nd.count > 1;
-                s_q[nn][lane] = many ? (sizeof(T) == 4 ? (T)qf[6 * (size_t)p + lane] : (T)qd[6 * (size_t)p + lane])
-                                     : (T)0;
-            }
-            ++nn;
-            p = nskip;
-        } else if (leaf) {
-            for (uint32_t k = 0; k < nd.count; k += 64) {
-                const uint32_t m = (nd.count - k) < 64u ? (nd.count - k) : 64u;
-                if (nq + (int)m > kBodyCap || nr == kRangeCap) {
-                    __syncthreads();
-                    gather_bodies<T>(s_bd, s_rstart, s_roff, nr, nq, lane, a, ox, oy, oz, kAdj ? s_bid : nullptr);
-                    __syncthreads();
-                    if constexpr (kAdj) {
-                        adj_self<T>(s_bd, s_q, nq, xr, yr, zr, wxr, wyr, wzr, eps2, ax, ay, az);
-                        adj_scatter<T>(s_bd, s_q, s_bid, nq, s_tg, eps2, lane, a.gbody);
-                    } else {
-                        eval_bodies<T, kGuard, kPot>(s_bd, nq, xr, yr, zr, eps2, ax, ay, az, phi);
-                    }
-                    cnt += (uint32_t)nq;
-                    nr = 0;
-                    nq = 0;
-                    __syncthreads();
-                }
-                if (lane == 0) {
-                    s_rstart[nr] = nd.start + k;
-                    s_roff[nr] = (uint32_t)nq;
-                }
-                ++nr;
-                nq += (int)m;
-            }
-            p = nskip;
-        } else {
-            p = p + 1;
-        }
-    }
-    __syncthreads();
-    if constexpr (kAdj) {
-        adj_self<T, kQuad>(s_nd, s_q, nn, xr, yr, zr, wxr, wyr, wzr, eps2, ax, ay, az);
-        adj_scatter<T, kQuad>(s_nd, s_q, s_nid, nn, s_tg, eps2, lane, a.gnode);
-    } else {
-        eval_nodes<T, kQuad, kGuard, kPot>(s_nd, s_q, nn, xr, yr, zr, eps2, ax, ay, az, phi);
-    }
-    gather_bodies<T>(s_bd, s_rstart, s_roff, nr, nq, lane, a, ox, oy, oz, kAdj ? s_bid : nullptr);
-    __syncthreads();
-    if constexpr (kAdj) {
-        adj_self<T>(s_bd, s_q, nq, xr, yr, zr, wxr, wyr, wzr, eps2, ax, ay, az);
-        adj_scatter<T>(s_bd, s_q, s_bid, nq, s_tg, eps2, lane, a.gbody);
-    } else {
-        eval_bodies<T, kGuard, kPot>(s_bd, nq, xr, yr, zr, eps2, ax, ay, az, phi);
-    }
-    cnt += (uint32_t)(nn + nq);
-    if (valid) {
-        const uint32_t o = a.tperm[i];
-        if constexpr (kAdj) {
-            a.gself[3 * (size_t)o] = (double)ax;
-            a.gself[3 * (size_t)o + 1] = (double)ay;
-            a.gself[3 * (size_t)o + 2] = (double)az;
-        } else {
-            a.ax[o] = (double)ax;
-            a.ay[o] = (double)ay;
-            a.az[o] = (double)az;
-            if constexpr (kPot) a.pot[o] = (double)phi;
-        }
-        if (a.counts) a.counts[o] = cnt;
-    }
-}
 
 // FP32 group walk over the compact layout: 32-byte visit records (one scalar
 // burst per visited cell) hold what the opening test needs — the centre
@@ -792,8 +614,7 @@ __global__ __launch_bounds__(64) void k_walk_group(WalkArgs a) {
 // list is evaluated; opened leaves are gathered as 16-byte offsets from the
 // leaf's centre of mass plus that centre's offset from the group origin.
 // Every lane builds its own position the same way (its leaf's centre offset +
-// its own offset), so a body meets itself at exactly d = 0.  Same
-// shared-list rule as k_walk_group.
+// its own offset), so a body meets itself at exactly d = 0.
 //
 // Work distribution: consecutive workgroups land on the 8 XCDs in turn, each
 // with its own L2.  Group g is given to block (g mod per) * 8 + g / per, so
@@ -1491,7 +1312,7 @@ __global__ __launch_bounds__(64) void k_walk_lp32(WalkArgs a) {
     }
 }
 
-// The stored adjoint lists of k_walk_lp32<..., kAdj> evaluated with W waves
+// The stored adjoint lists of k_walk_lp32 / k_walk_lp64<..., kAdj> evaluated with W waves
 // per group: wave w takes every W-th chunk of 64 cells, then of 64 bodies,
 // gathers it into its own LDS slots, and runs both adjoint passes on it (lane
 // = target over the chunk; lane = entry over the 64 targets, whose sums are
@@ -1505,13 +1326,14 @@ __device__ __forceinline__ void wave_lds_sync() {
     __builtin_amdgcn_wave_barrier();
 }
 
-template <bool kQuad, int W>
-__global__ __launch_bounds__(64 * W) void k_adj_lists32(WalkArgs a) {
-    __shared__ float s_tg[64][6];
-    __shared__ float s_nd[W][64][4];
-    __shared__ float s_q[W][kQuad ? 64 : 1][6];
+template <typename T, bool kQuad, int W>
+__global__ __launch_bounds__(64 * W) void k_adj_lists(WalkArgs a) {
+    constexpr bool k32 = sizeof(T) == 4;       // FP32: scaled group-relative units; FP64: absolute
+    __shared__ T s_tg[64][6];
+    __shared__ T s_nd[W][64][4];
+    __shared__ T s_q[W][kQuad ? 64 : 1][6];
     __shared__ uint32_t s_id[W][64];
-    __shared__ float s_red[W][64][3];
+    __shared__ T s_red[W][64][3];
     const uint32_t grp = xcd_group(blockIdx.x, a.groups);
     if (grp >= a.groups) return;               // the whole workgroup
     const uint32_t w = threadIdx.x / 64u, lane = threadIdx.x % 64u;
@@ -1523,22 +1345,32 @@ __global__ __launch_bounds__(64 * W) void k_adj_lists32(WalkArgs a) {
     ConstD tx = (ConstD)a.tx;
     ConstD ty = (ConstD)a.ty;
     ConstD tz = (ConstD)a.tz;
-    double ox = tx[g0], oy = ty[g0], oz = tz[g0];
+    double ox = k32 ? tx[g0] : 0.0, oy = k32 ? ty[g0] : 0.0, oz = k32 ? tz[g0] : 0.0;
     asm volatile("" : "+v"(ox), "+v"(oy), "+v"(oz));
-    const uint32_t own = a.tleaf[iv];
-    const float4 ob = a.tb4[iv];
-    const double lam = a.scale32[2 * (size_t)gr.z], mu = a.scale32[2 * (size_t)gr.z + 1];
-    const float xr = (float)((a.com[3 * (size_t)own] - ox) * lam) + ob.x;
-    const float yr = (float)((a.com[3 * (size_t)own + 1] - oy) * lam) + ob.y;
-    const float zr = (float)((a.com[3 * (size_t)own + 2] - oz) * lam) + ob.z;
-    const float eps2 = (float)(a.eps2 * lam * lam);
-    float wxr = 0.f, wyr = 0.f, wzr = 0.f;
+    double lam = 1.0, mu = 1.0;
+    T xr, yr, zr, eps2;
+    if constexpr (k32) {
+        const uint32_t own = a.tleaf[iv];
+        const float4 ob = a.tb4[iv];
+        lam = a.scale32[2 * (size_t)gr.z];
+        mu = a.scale32[2 * (size_t)gr.z + 1];
+        xr = (float)((a.com[3 * (size_t)own] - ox) * lam) + ob.x;
+        yr = (float)((a.com[3 * (size_t)own + 1] - oy) * lam) + ob.y;
+        zr = (float)((a.com[3 * (size_t)own + 2] - oz) * lam) + ob.z;
+        eps2 = (float)(a.eps2 * lam * lam);
+    } else {
+        xr = tx[iv];
+        yr = ty[iv];
+        zr = tz[iv];
+        eps2 = a.eps2;
+    }
+    T wxr = 0, wyr = 0, wzr = 0;
     uint32_t o = 0;
     if (valid) {
         o = a.tperm[i];
-        wxr = (float)a.wx[o];
-        wyr = (float)a.wy[o];
-        wzr = (float)a.wz[o];
+        wxr = (T)a.wx[o];
+        wyr = (T)a.wy[o];
+        wzr = (T)a.wz[o];
     }
     if (w == 0) {
         s_tg[lane][0] = xr; s_tg[lane][1] = yr; s_tg[lane][2] = zr;
@@ -1549,27 +1381,41 @@ __global__ __launch_bounds__(64 * W) void k_adj_lists32(WalkArgs a) {
     const size_t base = (size_t)grp * a.list_cap;
     const size_t off = a.list_off[grp];
     constexpr size_t kPs = kQuad ? 10 : 4;
+    // back to physical units (exact powers of two; 1 for FP64)
     const double sx = lam * lam * lam / mu, sm = lam * lam, sq = (lam * lam) * (lam * lam);
-    float ax = 0.f, ay = 0.f, az = 0.f;
+    T ax = 0, ay = 0, az = 0;
     for (uint32_t c0 = 64u * w; c0 < cnt.x; c0 += 64u * W) {
         const int ne = (int)(cnt.x - c0 < 64u ? cnt.x - c0 : 64u);
         if ((int)lane < ne) {
             const uint32_t id = a.lists[base + c0 + lane];
-            const WalkPay32& c = a.pay32[id];
-            s_nd[w][lane][0] = (float)((a.com[3 * (size_t)id] - ox) * lam);
-            s_nd[w][lane][1] = (float)((a.com[3 * (size_t)id + 1] - oy) * lam);
-            s_nd[w][lane][2] = (float)((a.com[3 * (size_t)id + 2] - oz) * lam);
-            s_nd[w][lane][3] = c.mass;
-            if constexpr (kQuad) {
+            if constexpr (k32) {
+                const WalkPay32& c = a.pay32[id];
+                s_nd[w][lane][0] = (float)((a.com[3 * (size_t)id] - ox) * lam);
+                s_nd[w][lane][1] = (float)((a.com[3 * (size_t)id + 1] - oy) * lam);
+                s_nd[w][lane][2] = (float)((a.com[3 * (size_t)id + 2] - oz) * lam);
+                s_nd[w][lane][3] = c.mass;
+                if constexpr (kQuad) {
 #pragma unroll
-                for (int u = 0; u < 6; ++u) s_q[w][lane][u] = c.q[u];
+                    for (int u = 0; u < 6; ++u) s_q[w][lane][u] = c.q[u];
+                }
+            } else {
+                const WalkNode nd = a.nodes[id];
+                s_nd[w][lane][0] = nd.cx;
+                s_nd[w][lane][1] = nd.cy;
+                s_nd[w][lane][2] = nd.cz;
+                s_nd[w][lane][3] = nd.mass;
+                if constexpr (kQuad) {
+                    const bool many = nd.count > 1;
+#pragma unroll
+                    for (int u = 0; u < 6; ++u) s_q[w][lane][u] = many ? a.quad[6 * (size_t)id + u] : 0.0;
+                }
             }
             s_id[w][lane] = id;
         }
         wave_lds_sync();
-        adj_self<float, kQuad>(s_nd[w], s_q[w], ne, xr, yr, zr, wxr, wyr, wzr, eps2, ax, ay, az);
-        adj_entry_sums<float, kQuad>(s_nd[w], s_q[w], ne, s_tg, eps2, lane,
-                                     [&](int e, float X, float Y, float Z, float h, const float* G) {
+        adj_self<T, kQuad>(s_nd[w], s_q[w], ne, xr, yr, zr, wxr, wyr, wzr, eps2, ax, ay, az);
+        adj_entry_sums<T, kQuad>(s_nd[w], s_q[w], ne, s_tg, eps2, lane,
+                                 [&](int e, T X, T Y, T Z, T h, const T* G) {
             const size_t pos = off + c0 + (uint32_t)e;
             double* p = a.part + kPs * pos;
             p[0] = (double)X * sx;
@@ -1589,18 +1435,25 @@ __global__ __launch_bounds__(64 * W) void k_adj_lists32(WalkArgs a) {
         const int ne = (int)(cnt.y - b0 < 64u ? cnt.y - b0 : 64u);
         if ((int)lane < ne) {
             const uint32_t j = a.lists[base + a.list_cap - 1u - (b0 + lane)];
-            const size_t l3 = 3 * (size_t)a.tleaf[j];
-            const float4 b = a.tb4[j];
-            s_nd[w][lane][0] = (float)((a.com[l3] - ox) * lam) + b.x;
-            s_nd[w][lane][1] = (float)((a.com[l3 + 1] - oy) * lam) + b.y;
-            s_nd[w][lane][2] = (float)((a.com[l3 + 2] - oz) * lam) + b.z;
-            s_nd[w][lane][3] = b.w;
+            if constexpr (k32) {
+                const size_t l3 = 3 * (size_t)a.tleaf[j];
+                const float4 b = a.tb4[j];
+                s_nd[w][lane][0] = (float)((a.com[l3] - ox) * lam) + b.x;
+                s_nd[w][lane][1] = (float)((a.com[l3 + 1] - oy) * lam) + b.y;
+                s_nd[w][lane][2] = (float)((a.com[l3 + 2] - oz) * lam) + b.z;
+                s_nd[w][lane][3] = b.w;
+            } else {
+                s_nd[w][lane][0] = tx[j];
+                s_nd[w][lane][1] = ty[j];
+                s_nd[w][lane][2] = tz[j];
+                s_nd[w][lane][3] = a.tm[j];
+            }
             s_id[w][lane] = j;
         }
         wave_lds_sync();
-        adj_self<float>(s_nd[w], s_q[w], ne, xr, yr, zr, wxr, wyr, wzr, eps2, ax, ay, az);
-        adj_entry_sums<float, false>(s_nd[w], s_q[w], ne, s_tg, eps2, lane,
-                                     [&](int e, float X, float Y, float Z, float h, const float*) {
+        adj_self<T>(s_nd[w], s_q[w], ne, xr, yr, zr, wxr, wyr, wzr, eps2, ax, ay, az);
+        adj_entry_sums<T, false>(s_nd[w], s_q[w], ne, s_tg, eps2, lane,
+                                 [&](int e, T X, T Y, T Z, T h, const T*) {
             const size_t pos = off + cnt.x + b0 + (uint32_t)e;
             double* p = a.part + kPs * pos;
             p[0] = (double)X * sx;
@@ -1617,7 +1470,7 @@ __global__ __launch_bounds__(64 * W) void k_adj_lists32(WalkArgs a) {
     s_red[w][lane][2] = az;
     __syncthreads();
     if (w == 0 && valid) {
-        float gx = s_red[0][lane][0], gy = s_red[0][lane][1], gz = s_red[0][lane][2];
+        T gx = s_red[0][lane][0], gy = s_red[0][lane][1], gz = s_red[0][lane][2];
 #pragma unroll
         for (int k = 1; k < W; ++k) {
             gx += s_red[k][lane][0];
@@ -1736,11 +1589,14 @@ constexpr int kLp64Bodies = 64;
 constexpr uint64_t kDigestCell = 0x243f6a8885a308d3ull, kDigestLeaf = 0x13198a2e03707344ull;
 constexpr uint64_t kDigestFrame = 0x9e3779b97f4a7c15ull;
 
-template <bool kQuad, bool kGuard, bool kPot = false>
+template <bool kQuad, bool kGuard, bool kPot = false, bool kAdj = false>
 __global__ __launch_bounds__(64) void k_walk_lp64(WalkArgs a) {
+    static_assert(!(kAdj && kPot), "the adjoint walk has no potential");
     __shared__ double s_nd[kLp64Cells][4];
     __shared__ double s_q[kQuad ? kLp64Cells : 1][6];
     __shared__ double s_bd[kLp64Bodies][4];
+    __shared__ uint32_t s_bid[kAdj ? kLp64Bodies : 1];   // adjoint: tree-order index of each body entry
+    __shared__ double s_tg[kAdj ? 64 : 1][6];            // adjoint: targets' positions and weights
     __shared__ uint32_t s_rstart[kRanges32], s_roff[kRanges32];
     __shared__ double4 s_tgt[64];
     __shared__ uint32_t s_cid[kLpCells];
@@ -1774,12 +1630,38 @@ __global__ __launch_bounds__(64) void k_walk_lp64(WalkArgs a) {
         double ax = 0, ay = 0, az = 0, phi = 0;
         uint32_t cnt = 0;
         int nc = 0, nr = 0, nq = 0;
+        // adjoint: the body's weight w = dL/da; ax/ay/az collect dL/dx of its own
+        // position; the lists are stored (cell ids from the front of the group's
+        // region, tree-order body indices from the back) while they fit
+        double wxr = 0, wyr = 0, wzr = 0;
+        uint32_t lc = 0, lb = 0;
+        const size_t lbase = kAdj ? (size_t)grp * a.list_cap : 0;
+        if constexpr (kAdj) {
+            if (valid) {
+                const uint32_t o = a.tperm[i];
+                wxr = a.wx[o];
+                wyr = a.wy[o];
+                wzr = a.wz[o];
+            }
+        }
 
         __syncthreads();
         s_tgt[lane] = valid ? make_double4(x, y, z, 0.0) : make_double4(INFINITY, INFINITY, INFINITY, 0.0);
+        if constexpr (kAdj) {
+            s_tg[lane][0] = x; s_tg[lane][1] = y; s_tg[lane][2] = z;
+            s_tg[lane][3] = wxr; s_tg[lane][4] = wyr; s_tg[lane][5] = wzr;
+        }
 
         auto flush_cells = [&](int nn) {
             __syncthreads();
+            if constexpr (kAdj) {
+                if (lc + lb + (uint32_t)nn <= a.list_cap) {
+                    if ((int)lane < nn) a.lists[lbase + lc + lane] = s_cid[lane];
+                    lc += (uint32_t)nn;
+                    cnt += (uint32_t)nn;
+                    return;
+                }
+            }
             if ((int)lane < nn) {
                 const uint32_t id = s_cid[lane];
                 if (dig) hsum += mix64((((uint64_t)grp << 32) | id) ^ kDigestCell ^ salt);
@@ -1795,7 +1677,12 @@ __global__ __launch_bounds__(64) void k_walk_lp64(WalkArgs a) {
                 }
             }
             __syncthreads();
-            eval_nodes<double, kQuad, kGuard, kPot>(s_nd, s_q, nn, x, y, z, eps2, ax, ay, az, phi);
+            if constexpr (kAdj) {
+                adj_self<double, kQuad>(s_nd, s_q, nn, x, y, z, wxr, wyr, wzr, eps2, ax, ay, az);
+                adj_scatter<double, kQuad>(s_nd, s_q, s_cid, nn, s_tg, eps2, lane, a.gnode);
+            } else {
+                eval_nodes<double, kQuad, kGuard, kPot>(s_nd, s_q, nn, x, y, z, eps2, ax, ay, az, phi);
+            }
             cnt += (uint32_t)nn;
         };
         auto flush_64 = [&]() {
@@ -1809,20 +1696,39 @@ __global__ __launch_bounds__(64) void k_walk_lp64(WalkArgs a) {
         auto flush_bodies = [&]() {
             __syncthreads();
             const int e = (int)lane;
+            uint32_t j = 0;
             if (e < nq) {
                 int lo = 0, hi = nr - 1;
                 while (lo < hi) {
                     const int mid = (lo + hi + 1) >> 1;
                     if ((int)s_roff[mid] <= e) lo = mid; else hi = mid - 1;
                 }
-                const uint32_t j = s_rstart[lo] + (uint32_t)(e - (int)s_roff[lo]);
+                j = s_rstart[lo] + (uint32_t)(e - (int)s_roff[lo]);
+            }
+            if constexpr (kAdj) {
+                if (lc + lb + (uint32_t)nq <= a.list_cap) {
+                    if (e < nq) a.lists[lbase + a.list_cap - 1u - (lb + (uint32_t)e)] = j;
+                    lb += (uint32_t)nq;
+                    cnt += (uint32_t)nq;
+                    nr = 0;
+                    nq = 0;
+                    return;
+                }
+            }
+            if (e < nq) {
                 s_bd[e][0] = a.tx[j];
                 s_bd[e][1] = a.ty[j];
                 s_bd[e][2] = a.tz[j];
                 s_bd[e][3] = a.tm[j];
+                if constexpr (kAdj) s_bid[e] = j;
             }
             __syncthreads();
-            eval_bodies<double, kGuard, kPot>(s_bd, nq, x, y, z, eps2, ax, ay, az, phi);
+            if constexpr (kAdj) {
+                adj_self<double>(s_bd, s_q, nq, x, y, z, wxr, wyr, wzr, eps2, ax, ay, az);
+                adj_scatter<double>(s_bd, s_q, s_bid, nq, s_tg, eps2, lane, a.gbody);
+            } else {
+                eval_bodies<double, kGuard, kPot>(s_bd, nq, x, y, z, eps2, ax, ay, az, phi);
+            }
             cnt += (uint32_t)nq;
             nr = 0;
             nq = 0;
@@ -1986,12 +1892,21 @@ __global__ __launch_bounds__(64) void k_walk_lp64(WalkArgs a) {
             nc = 0;
         }
         if (nq > 0) flush_bodies();
+        if constexpr (kAdj) {
+            if (lane == 0) a.list_n[grp] = make_uint2(lc, lb);
+        }
         if (valid) {
             const uint32_t o = a.tperm[i];
-            a.ax[o] = ax;
-            a.ay[o] = ay;
-            a.az[o] = az;
-            if constexpr (kPot) a.pot[o] = phi;
+            if constexpr (kAdj) {
+                a.gself[3 * (size_t)o] = ax;
+                a.gself[3 * (size_t)o + 1] = ay;
+                a.gself[3 * (size_t)o + 2] = az;
+            } else {
+                a.ax[o] = ax;
+                a.ay[o] = ay;
+                a.az[o] = az;
+                if constexpr (kPot) a.pot[o] = phi;
+            }
             if (a.counts) a.counts[o] = cnt;
         }
     }
@@ -2287,10 +2202,11 @@ hipError_t launch_walk_adjoint(bhx_ctx* c, const WalkTree& t, int precision, con
     a.gbody = gbody;
     a.quad = t.quad;
     a.child = t.child;
+    a.child_byte = t.child_byte;
     // the adjoint stream runs beside the main stream's walks: counters of its own
     a.wq = c->wq + 8;
-    if (precision == BHX_FP32 && !L) return hipErrorInvalidValue;
-    if (L) {
+    if (!L) return hipErrorInvalidValue;
+    {
         a.lists = L->lists;
         a.list_n = L->list_n;
         a.list_cap = L->cap;
@@ -2310,15 +2226,23 @@ hipError_t launch_walk_adjoint(bhx_ctx* c, const WalkTree& t, int precision, con
     const bool quad = c->cfg.quadrupole != 0;
 #define BHX_ADJ(Q, G)                                                                                             \
     do {                                                                                                          \
-        if (precision == BHX_FP32) {                                                                              \
-            hipLaunchKernelGGL(k_fill_wq, dim3(1), dim3(64), 0, s, a.wq);                                         \
+        hipLaunchKernelGGL(k_fill_wq, dim3(1), dim3(64), 0, s, a.wq);                                             \
+        if (precision == BHX_FP32)                                                                                \
             hipLaunchKernelGGL((k_walk_lp32<Q, G, false, 1, true>),                                               \
                                dim3(persistent_grid(k_walk_lp32<Q, G, false, 1, true>, groups)), dim3(64), 0, s, a); \
-            hipLaunchKernelGGL(k_list_scan, dim3(1), dim3(1024), 0, s, a.list_n, groups, L->off);                 \
+        else                                                                                                      \
+            hipLaunchKernelGGL((k_walk_lp64<Q, G, false, true>),                                                  \
+                               dim3(persistent_grid(k_walk_lp64<Q, G, false, true>, groups)), dim3(64), 0, s, a); \
+        hipLaunchKernelGGL(k_list_scan, dim3(1), dim3(1024), 0, s, a.list_n, groups, L->off);                     \
+        if (precision == BHX_FP32) {                                                                              \
             if (groups < 8192u)                                                                                   \
-                hipLaunchKernelGGL((k_adj_lists32<Q, 8>), dim3(bgrid), dim3(512), 0, s, a);                       \
+                hipLaunchKernelGGL((k_adj_lists<float, Q, 8>), dim3(bgrid), dim3(512), 0, s, a);                  \
             else                                                                                                  \
-                hipLaunchKernelGGL((k_adj_lists32<Q, 4>), dim3(bgrid), dim3(256), 0, s, a);                       \
+                hipLaunchKernelGGL((k_adj_lists<float, Q, 4>), dim3(bgrid), dim3(256), 0, s, a);                  \
+        } else {                                                                                                  \
+            hipLaunchKernelGGL((k_adj_lists<double, Q, 4>), dim3(bgrid), dim3(256), 0, s, a);                     \
+        }                                                                                                         \
+        {                                                                                                         \
             e = radix_sort_pairs(L->key, L->val, L->key_alt, L->val_alt, L->max_pairs, L->hist, L->status,       \
                                  L->tile_ctr, c->ctr, s, L->passes, L->off + groups);                             \
             if (e != hipSuccess) return e;                                                                        \
@@ -2328,8 +2252,6 @@ hipError_t launch_walk_adjoint(bhx_ctx* c, const WalkTree& t, int precision, con
             hipLaunchKernelGGL((k_run_sum<Q ? 10 : 4>), dim3(wgrid), dim3(256), 0, s, L->run,                     \
                                (L->passes & 1) ? L->val_alt : L->val, L->part, L->node_cap, L->nkeys, gnode,       \
                                gbody);                                                                            \
-        } else {                                                                                                  \
-            hipLaunchKernelGGL((k_walk_group<double, Q, G, true>), dim3(groups), dim3(64), 0, s, a);              \
         }                                                                                                         \
     } while (0)
     if (quad) { if (guard) BHX_ADJ(true, true); else BHX_ADJ(true, false); }
