@@ -728,7 +728,10 @@ static_assert(kCells32 == 64, "the FP32 cell list is one id per lane");
 // through an atomic counter, then from the other XCDs' slices, and leave
 // when every slice is taken.
 
-constexpr int kLpStack = 1024;     // LDS stack of cells to test
+// LDS stack of cells to test: 768 entries keep the FP32 walk's LDS at 10 KB
+// (16 one-wave workgroups per compute unit, as many as its registers allow;
+// 1,024 entries: 14); overflowing cells are walked one at a time
+constexpr int kLpStack = 768;
 constexpr int kLpCells = 128;      // accepted cell ids waiting for evaluation (64 at a time)
 constexpr int kLpBodies = 96;
 constexpr int kLpUnd = 128;        // undecided cells (at most 63 + 64 wait at once)
