@@ -358,6 +358,12 @@ int bhx_record(bhx_ctx *ctx, int nsteps, int checkpoint_every, int init_acc);
  * UINT64_MAX (the default): half of the device memory free at recording. */
 int bhx_set_record_memory(bhx_ctx *ctx, uint64_t bytes);
 
+/* Entries of each walk group's interaction list the reverse pass stores for
+ * its evaluation kernel (0: the default, 8,192, fewer when there are many
+ * groups; else a multiple of 64, at least 64).  A longer list is evaluated
+ * by the traversal itself; the gradients agree to rounding. */
+int bhx_set_adjoint_list_capacity(bhx_ctx *ctx, uint32_t entries);
+
 /* Whether the last recording kept every frame's walk data (*kept = 1) or
  * its reverse pass re-runs the checkpoint segments (*kept = 0). */
 int bhx_record_keeps_frames(bhx_ctx *ctx, int *kept);

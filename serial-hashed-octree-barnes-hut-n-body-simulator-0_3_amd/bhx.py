@@ -80,7 +80,8 @@ def lib():
             "bhx_autotune": [P, P, S, P, S, I, D, S, ctypes.c_uint64, P, P, P],
             "bhx_record": [P, I, I, I], "bhx_recorded_bodies": [P, P, P, P, P], "bhx_record_digest": [P, P],
             "bhx_adjoint": [P, P, P, P, P, P, P, P], "bhx_grad_timing": [P, P, P],
-            "bhx_set_record_memory": [P, ctypes.c_uint64], "bhx_record_keeps_frames": [P, P], "bhx_system_roots": [P, P], "bhx_lookup_in": [P, S, P, S, P],
+            "bhx_set_record_memory": [P, ctypes.c_uint64], "bhx_record_keeps_frames": [P, P],
+            "bhx_set_adjoint_list_capacity": [P, ctypes.c_uint32], "bhx_system_roots": [P, P], "bhx_lookup_in": [P, S, P, S, P],
             "bhx_get_systems": [P, P], "bhx_energy": [P, P, P, P, P],
             "bhx_galaxy_defaults": [P], "bhx_ic_galaxy_pair": [S, ctypes.c_uint64, P, I, P, P, P],
             "bhx_snapshot_write": [ctypes.c_char_p, P, P, P, P, P, P, P],
@@ -487,6 +488,11 @@ class Engine:
         None: half of the free device memory)."""
         v = (1 << 64) - 1 if nbytes is None else int(nbytes)
         self._chk(lib().bhx_set_record_memory(self.h, ctypes.c_uint64(v)), "bhx_set_record_memory")
+
+    def set_adjoint_list_capacity(self, entries):
+        """List entries per walk group the reverse pass stores (0: the default)."""
+        self._chk(lib().bhx_set_adjoint_list_capacity(self.h, ctypes.c_uint32(int(entries))),
+                  "bhx_set_adjoint_list_capacity")
 
     def record_keeps_frames(self):
         """True when the last recording kept every frame (its reverse pass re-runs nothing)."""

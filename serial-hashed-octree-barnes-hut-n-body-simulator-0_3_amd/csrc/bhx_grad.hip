@@ -511,6 +511,7 @@ int alloc_adj_lists(bhx_ctx* c, GradRecord* g) {
     size_t cap = ((size_t)1 << 26) / groups;
     cap = cap > 8192 ? 8192 : cap;
     cap = cap < 1024 ? 1024 : cap & ~(size_t)63;
+    if (c->adj_list_cap) cap = c->adj_list_cap;
     const size_t pairs = groups * cap;
     const size_t ps = g->quad ? 10 : 4;
     // sort keys: cell ids below node_cap (every stored tree's node ids), bodies
@@ -772,7 +773,8 @@ int bhx_adjoint(bhx_ctx* c, const double* gx, const double* gv, const double* ga
     if (!g->work) {
         if (dalloc(&g->work, work_len(n, g->work_nodes, g->quad != 0)) != hipSuccess) return fail(c, BHX_E_NOMEM, "adjoint allocation");
     }
-    if (g->lists_mem && g->lists.groups != c->ngroups) {
+    if (g->lists_mem && (g->lists.groups != c->ngroups ||
+                         (c->adj_list_cap != 0 && g->lists.cap != c->adj_list_cap))) {
         (void)hipStreamSynchronize(c->s_adj);
         (void)hipFree(g->lists_mem);
         g->lists_mem = nullptr;
@@ -882,6 +884,14 @@ int bhx_adjoint(bhx_ctx* c, const double* gx, const double* gv, const double* ga
 int bhx_set_record_memory(bhx_ctx* c, uint64_t bytes) {
     if (!c) return BHX_E_NULL;
     c->record_memory = bytes;
+    return BHX_OK;
+}
+
+int bhx_set_adjoint_list_capacity(bhx_ctx* c, uint32_t entries) {
+    if (!c) return BHX_E_NULL;
+    if (entries != 0 && (entries < 64 || entries % 64 != 0))
+        return fail(c, BHX_E_ARG, "adjoint list capacity must be 0 or a multiple of 64, at least 64");
+    c->adj_list_cap = entries;
     return BHX_OK;
 }
 

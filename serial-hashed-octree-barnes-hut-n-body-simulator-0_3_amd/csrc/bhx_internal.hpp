@@ -178,6 +178,9 @@ struct bhx_ctx {
     // that the reverse pass re-runs no frames (bhx_set_record_memory);
     // UINT64_MAX: half of the free device memory when recording
     uint64_t record_memory = UINT64_MAX;
+    // entries per walk group the adjoint stores for its evaluation kernel
+    // (bhx_set_adjoint_list_capacity); 0: the default
+    uint32_t adj_list_cap = 0;
     // the tree branch of a frame (path keys of the drifted positions and
     // their sort) beside the body sort; created after the other three
     hipStream_t s_tree = nullptr;

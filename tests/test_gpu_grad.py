@@ -209,6 +209,35 @@ def test_adjoint_repeats_bitwise(bhx, prec, quad):
             assert np.array_equal(r[k], res[0][k]), k
 
 
+@pytest.mark.parametrize("prec", [1, 0])
+def test_adjoint_overflowing_lists(bhx, prec):
+    """Lists longer than their stored region are evaluated by the traversal
+    itself (entry sums with atomics): a 64- and a 640-entry region (every
+    group overflows, after one or after several flushes) give the stored
+    lists' gradients to rounding; bad capacities are refused."""
+    n, T = 2500, 5
+    pos, vel, mass = _bodies(bhx, n, 50 + prec)
+    gx, gv = _weights(n, 51)
+    cfg = dict(DIFF, precision=prec, quadrupole=1, leaf_capacity=16)
+    res = []
+    for cap in (0, 64, 640):
+        with bhx.Engine(n, **cfg) as e:
+            e.set_adjoint_list_capacity(cap)
+            e.set_bodies(pos, vel, mass)
+            e.record(T, checkpoint_every=T, init_acc=True)
+            res.append(e.adjoint(gx, gv))
+    for r in res[1:]:
+        for k in (0, 1, 3):
+            scale = np.abs(res[0][k]).max()
+            err = np.abs(r[k] - res[0][k]).max() / scale
+            # the same terms summed in another order (FP32 walk: FP32 group sums)
+            assert err < (1e-5 if prec == 1 else 1e-12), (k, err)
+    with bhx.Engine(64, **cfg) as e:
+        for bad in (1, 63, 100):
+            with pytest.raises(bhx.BhxError):
+                e.set_adjoint_list_capacity(bad)
+
+
 def test_record_matches_plain_frames_and_leaves_end_state(bhx):
     n, T = 700, 7
     pos, vel, mass = _bodies(bhx, n, 14)
