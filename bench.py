@@ -69,32 +69,38 @@ def walk_kernel_ms(e, reps):
 
 
 # Algorithmic HBM bytes per body of each timed phase of the FP32 step (compensated KDK, accelerations
-# permuted with the bodies, one system); r = tree nodes per body.
+# permuted with the bodies, one system); r = tree nodes per body.  Each entry names the event windows
+# it covers: the tree branch (tree keys, their sort) is forked onto a stream of its own before the
+# Morton keys and runs beside them, the body sort and the gather + kick + drift, so those four windows
+# are one phase — apart, the tree-order window would be charged the branch's bytes for only the time
+# it waits on the branch.
 PHASE_BYTES = {
-    "keys": lambda r: 36.0,                 # positions in; key and index out
-    "sort": lambda r: 200.0,                # 8 passes x (key + index in and out) + one key read for the histograms
-    "kick_drift": lambda r: 236.0,          # index, pos, vel, mass, acc, both compensations in; all but acc out
-    "treekeys": lambda r: 304.0,            # tree keys (36) + their sort (200) + tree-order gather (68)
-    "build": lambda r: 40.0 + 208.0 * r,    # keys and tree-order bodies; node records, hash slots, moments
-    "walk": lambda r: 56.0 + 32.0 * r,      # bodies in, accelerations out, each node record once
-    "kick": lambda r: 120.0,                # velocity, its compensation and acceleration in; both out
+    # Morton keys: positions in, key and index out (36); body sort: 8 passes x (key + index in and
+    # out) + one key read for the histograms (200); gather + kick + drift: index, pos, vel, mass, acc,
+    # both compensations in, all but acc out (236); tree keys (36) + their sort (200) + tree-order
+    # gather (68)
+    "keys_sorts_kick_drift_tree": (("keys", "sort", "kick_drift", "treekeys"),
+                                   lambda r: 36.0 + 200.0 + 236.0 + 304.0),
+    "build": (("build",), lambda r: 40.0 + 208.0 * r),      # keys and tree-order bodies; nodes, hash, moments
+    "walk": (("walk",), lambda r: 56.0 + 32.0 * r),         # bodies in, accelerations out, each node once
+    "kick": (("kick",), lambda r: 120.0),                   # velocity, its compensation and acceleration
 }
 BOUNDS_BYTES = 72.0                         # positions read three times (two for the mean, one for the size)
 PEAK_HBM_GBPS = 8000.0
 
 
 def phase_roofline(phases, n, nodes, step_ms):
-    """Per phase: algorithmic bytes, the rate its event time gives and the fraction of peak HBM; and the
-    same for the whole step (bounds included: they run on a side stream inside the step)."""
+    """Per phase: algorithmic bytes, the rate its event windows give and the fraction of peak HBM; and
+    the same for the whole step (bounds included: they run on a side stream inside the step)."""
     r = nodes / n
     out = {}
-    for k, f in PHASE_BYTES.items():
-        ms = phases.get(k, 0.0)
+    for k, (windows, f) in PHASE_BYTES.items():
+        ms = sum(phases.get(w, 0.0) for w in windows)
         if ms > 0:
             gbs = f(r) * n / (ms * 1e-3) / 1e9
             out[k] = {"bytes_per_body": round(f(r), 1), "ms": round(ms, 4), "GBps": round(gbs, 1),
                       "frac": round(gbs / PEAK_HBM_GBPS, 4)}
-    per_body = BOUNDS_BYTES + sum(f(r) for f in PHASE_BYTES.values())
+    per_body = BOUNDS_BYTES + sum(f(r) for _, f in PHASE_BYTES.values())
     gbs = per_body * n / (step_ms * 1e-3) / 1e9
     return out, {"bytes_per_body": round(per_body, 1), "nodes_per_body": round(r, 4), "GBps": round(gbs, 1),
                  "frac": round(gbs / PEAK_HBM_GBPS, 4)}
