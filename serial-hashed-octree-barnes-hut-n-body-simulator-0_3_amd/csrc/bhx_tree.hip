@@ -471,7 +471,7 @@ struct ChildFromSlice {
     __device__ double quad(uint32_t i, int k) const { return q[i - b0][k]; }
 };
 
-constexpr int kLeafBatch = 8;    // leaf bodies loaded at once by the moments
+constexpr int kStage = 1024;     // leaf bodies staged in LDS at once by the slice kernel
 
 // What combining node p reads of its own records, loaded in one batch
 // (ahead of the level loop in the slice kernel: none of it depends on the
@@ -503,214 +503,143 @@ __device__ __forceinline__ NodeInfo node_info(uint32_t p, const TreeArgs& t, con
     return I;
 }
 
+// The node's records: FP64 payload, the FP64 and FP32 walk records and the
+// FP32 payload (the FP32 copies in the system's scaled units, exact powers of
+// two).
+__device__ __forceinline__ void write_node(uint32_t p, const NodeInfo& I, const TreeArgs& t, bool leaf, double M,
+                                           double cx, double cy, double cz, const double* q, uint32_t first) {
+    const int L = (int)I.L;
+    const uint32_t st = I.st, cnt = I.cnt, cb = I.cb;
+    const double lam = I.lam, mu = I.mu;
+    t.mass[p] = M;
+    t.first[p] = first;
+    t.com[3 * (size_t)p] = cx;
+    t.com[3 * (size_t)p + 1] = cy;
+    t.com[3 * (size_t)p + 2] = cz;
+    const double lam2 = lam * lam, qs = mu * lam2;
+    for (int k = 0; k < 6; ++k) {
+        t.quad[6 * (size_t)p + k] = q[k];
+        t.quadf[6 * (size_t)p + k] = (float)(q[k] * qs);
+    }
+    const double* b4n = I.b4;
+    double s = b4n[3] * t.tscale;
+    for (int l = 0; l < L; ++l) s = s * 0.5;
+    WalkNode w;
+    w.cx = cx; w.cy = cy; w.cz = cz;
+    w.mass = M;
+    w.size2 = s * s;
+    w.skip = I.skip;
+    w.start = st;
+    w.count = cnt;
+    w.level_leaf = (uint32_t)L | (leaf ? 256u : 0u);
+    w.size2f = (float)(s * s * lam2);
+    w.massf = (float)(M * mu);
+    t.walk[p] = w;
+    WalkVisit32 v32;
+    v32.cx = (float)((cx - b4n[0]) * lam);
+    v32.cy = (float)((cy - b4n[1]) * lam);
+    v32.cz = (float)((cz - b4n[2]) * lam);
+    v32.meta = (uint32_t)L | (leaf ? 32u : 0u) | ((cnt < (1u << 26) ? cnt : (1u << 26) - 1) << 6);
+    v32.skip = I.skip;
+    v32.start = st;
+    v32.child_byte = cb;
+    v32.size2 = (float)(s * s * lam2);
+    t.walk32[p] = v32;
+    WalkPay32 p32;
+    p32.mass = (float)(M * mu);
+    for (int k = 0; k < 6; ++k) p32.q[k] = cnt > 1 ? (float)(q[k] * qs) : 0.0f;
+    p32.pad = 0u;
+    t.pay32[p] = p32;
+}
+
+// Inner nodes only: the leaves are combined by the slice kernel.
 template <class Child>
 __device__ Payload moments_of(uint32_t p, const NodeInfo& I, int intended, const TreeArgs& t,
                               const double* __restrict__ px, const double* __restrict__ py,
-                              const double* __restrict__ pz, const double* __restrict__ tx,
-                              const double* __restrict__ ty, const double* __restrict__ tz,
-                              const double* __restrict__ tm, const uint32_t* __restrict__ tperm,
-                              const Child& ch) {
-    {
-        const int L = (int)I.L;
-        const uint32_t st = I.st, cnt = I.cnt, cb = I.cb;
-        const bool leaf = cb == 0;
-        double M = 0.0, sx = 0.0, sy = 0.0, sz = 0.0;
-        double q[6] = {0, 0, 0, 0, 0, 0};
-        uint32_t first = 0xFFFFFFFFu;
-        double cx, cy, cz;
-        // the FP32 copies are in the system's scaled units (exact powers of two)
-        const double lam = I.lam, mu = I.mu;
-        bool bodies_done = false;   // the leaf's FP32 body copies already written
-        if (leaf) {
-            // bodies in batches of kLeafBatch with every load of a batch in flight
-            // before the (sequential, in body order) sums use them; a leaf of at
-            // most one batch keeps its bodies in registers for the second pass
-            const uint32_t e = st + cnt;
-            const bool one = cnt <= (uint32_t)kLeafBatch;
-            double mj[kLeafBatch], xj[kLeafBatch], yj[kLeafBatch], zj[kLeafBatch];
-            for (uint32_t j0 = st; j0 < e; j0 += kLeafBatch) {
-                uint32_t aj[kLeafBatch];
+                              const double* __restrict__ pz, const Child& ch) {
+    const uint32_t cb = I.cb;
+    double M = 0.0, sx = 0.0, sy = 0.0, sz = 0.0;
+    double q[6] = {0, 0, 0, 0, 0, 0};
+    uint32_t first = 0xFFFFFFFFu;
+    double cx, cy, cz;
+    // children in octant order = pre-order: the sums keep their order; the
+    // child ids come from the octant table, so the child records are
+    // fetched independently rather than along the pre-order skip chain
+    const uint32_t* ids0 = I.ch;
+    if constexpr (Child::kPreload) {
+        // every child's record is loaded before any sum uses it (an absent
+        // octant reads the node's own slot, never used), so the loads
+        // overlap instead of one round trip per child
+        uint32_t ids[8];
+        double mc[8], c0[8], c1[8], c2[8];
+        uint32_t fc[8];
+        double qc[8][6];   // with the other loads: one round trip, not two
 #pragma unroll
-                for (int k = 0; k < kLeafBatch; ++k) {
-                    const uint32_t j = j0 + k < e ? j0 + k : j0;
-                    mj[k] = tm[j]; xj[k] = tx[j]; yj[k] = ty[j]; zj[k] = tz[j]; aj[k] = tperm[j];
-                }
+        for (int o = 0; o < 8; ++o) {
+            ids[o] = ((cb >> o) & 1u) ? ids0[o] : p;
+            mc[o] = ch.mass(ids[o]);
+            c0[o] = ch.com(ids[o], 0);
+            c1[o] = ch.com(ids[o], 1);
+            c2[o] = ch.com(ids[o], 2);
+            fc[o] = ch.first(ids[o]);
+            if (intended) {
 #pragma unroll
-                for (int k = 0; k < kLeafBatch; ++k) {
-                    if (j0 + k >= e) break;
-                    M += mj[k];
-                    sx += mj[k] * xj[k];
-                    sy += mj[k] * yj[k];
-                    sz += mj[k] * zj[k];
-                    if (aj[k] < first) first = aj[k];
-                }
+                for (int k = 0; k < 6; ++k) qc[o][k] = ch.quad(ids[o], k);
             }
-            if (!intended || cnt == 1) {
-                cx = px[first]; cy = py[first]; cz = pz[first];
-            } else {
-                cx = sx / M; cy = sy / M; cz = sz / M;
-            }
-            if (intended && cnt > 1) {
-                // quadrupole and the FP32 body copies from one pass over the bodies
-                for (uint32_t j0 = st; j0 < e; j0 += kLeafBatch) {
-                    if (!one) {
+        }
 #pragma unroll
-                        for (int k = 0; k < kLeafBatch; ++k) {
-                            const uint32_t j = j0 + k < e ? j0 + k : j0;
-                            mj[k] = tm[j]; xj[k] = tx[j]; yj[k] = ty[j]; zj[k] = tz[j];
-                        }
-                    }
-#pragma unroll
-                    for (int k = 0; k < kLeafBatch; ++k) {
-                        if (j0 + k >= e) break;
-                        const double dx = xj[k] - cx, dy = yj[k] - cy, dz = zj[k] - cz;
-                        quad_add(q, mj[k], dx, dy, dz);
-                        t.tb4[j0 + k] = make_float4((float)(dx * lam), (float)(dy * lam), (float)(dz * lam),
-                                                    (float)(mj[k] * mu));
-                        t.tleaf[j0 + k] = p;
-                    }
-                }
-                bodies_done = true;
-            }
+        for (int o = 0; o < 8; ++o) {
+            if (!((cb >> o) & 1u)) continue;
+            M += mc[o];
+            sx += mc[o] * c0[o];
+            sy += mc[o] * c1[o];
+            sz += mc[o] * c2[o];
+            if (fc[o] < first) first = fc[o];
+        }
+        if (!intended) {
+            cx = px[first]; cy = py[first]; cz = pz[first];
         } else {
-            // children in octant order = pre-order: the sums keep their order; the
-            // child ids come from the octant table, so the child records are
-            // fetched independently rather than along the pre-order skip chain
-            const uint32_t* ids0 = I.ch;
-            if constexpr (Child::kPreload) {
-                // every child's record is loaded before any sum uses it (an absent
-                // octant reads the node's own slot, never used), so the loads
-                // overlap instead of one round trip per child
-                uint32_t ids[8];
-                double mc[8], c0[8], c1[8], c2[8];
-                uint32_t fc[8];
-                double qc[8][6];   // with the other loads: one round trip, not two
+            cx = sx / M; cy = sy / M; cz = sz / M;
 #pragma unroll
-                for (int o = 0; o < 8; ++o) {
-                    ids[o] = ((cb >> o) & 1u) ? ids0[o] : p;
-                    mc[o] = ch.mass(ids[o]);
-                    c0[o] = ch.com(ids[o], 0);
-                    c1[o] = ch.com(ids[o], 1);
-                    c2[o] = ch.com(ids[o], 2);
-                    fc[o] = ch.first(ids[o]);
-                    if (intended) {
-#pragma unroll
-                        for (int k = 0; k < 6; ++k) qc[o][k] = ch.quad(ids[o], k);
-                    }
-                }
-#pragma unroll
-                for (int o = 0; o < 8; ++o) {
-                    if (!((cb >> o) & 1u)) continue;
-                    M += mc[o];
-                    sx += mc[o] * c0[o];
-                    sy += mc[o] * c1[o];
-                    sz += mc[o] * c2[o];
-                    if (fc[o] < first) first = fc[o];
-                }
-                if (!intended) {
-                    cx = px[first]; cy = py[first]; cz = pz[first];
-                } else {
-                    cx = sx / M; cy = sy / M; cz = sz / M;
-#pragma unroll
-                    for (int o = 0; o < 8; ++o) {
-                        if (!((cb >> o) & 1u)) continue;
-                        quad_add(q, mc[o], c0[o] - cx, c1[o] - cy, c2[o] - cz);
-                        for (int k = 0; k < 6; ++k) q[k] += qc[o][k];
-                    }
-                }
-            } else {
-#pragma unroll
-                for (int o = 0; o < 8; ++o) {
-                    if (!((cb >> o) & 1u)) continue;
-                    const uint32_t c = ids0[o];
-                    const double mcc = ch.mass(c);
-                    M += mcc;
-                    sx += mcc * ch.com(c, 0);
-                    sy += mcc * ch.com(c, 1);
-                    sz += mcc * ch.com(c, 2);
-                    const uint32_t fcc = ch.first(c);
-                    if (fcc < first) first = fcc;
-                }
-                if (!intended) {
-                    cx = px[first]; cy = py[first]; cz = pz[first];
-                } else {
-                    cx = sx / M; cy = sy / M; cz = sz / M;
-#pragma unroll
-                    for (int o = 0; o < 8; ++o) {
-                        if (!((cb >> o) & 1u)) continue;
-                        const uint32_t c = ids0[o];
-                        quad_add(q, ch.mass(c), ch.com(c, 0) - cx, ch.com(c, 1) - cy, ch.com(c, 2) - cz);
-                        for (int k = 0; k < 6; ++k) q[k] += ch.quad(c, k);
-                    }
-                }
+            for (int o = 0; o < 8; ++o) {
+                if (!((cb >> o) & 1u)) continue;
+                quad_add(q, mc[o], c0[o] - cx, c1[o] - cy, c2[o] - cz);
+                for (int k = 0; k < 6; ++k) q[k] += qc[o][k];
             }
         }
-        t.mass[p] = M;
-        t.first[p] = first;
-        t.com[3 * (size_t)p] = cx;
-        t.com[3 * (size_t)p + 1] = cy;
-        t.com[3 * (size_t)p + 2] = cz;
-        const double lam2 = lam * lam, qs = mu * lam2;
-        for (int k = 0; k < 6; ++k) {
-            t.quad[6 * (size_t)p + k] = q[k];
-            t.quadf[6 * (size_t)p + k] = (float)(q[k] * qs);
+    } else {
+#pragma unroll
+        for (int o = 0; o < 8; ++o) {
+            if (!((cb >> o) & 1u)) continue;
+            const uint32_t c = ids0[o];
+            const double mcc = ch.mass(c);
+            M += mcc;
+            sx += mcc * ch.com(c, 0);
+            sy += mcc * ch.com(c, 1);
+            sz += mcc * ch.com(c, 2);
+            const uint32_t fcc = ch.first(c);
+            if (fcc < first) first = fcc;
         }
-        const double* b4n = I.b4;
-        double s = b4n[3] * t.tscale;
-        for (int l = 0; l < L; ++l) s = s * 0.5;
-        WalkNode w;
-        w.cx = cx; w.cy = cy; w.cz = cz;
-        w.mass = M;
-        w.size2 = s * s;
-        w.skip = I.skip;
-        w.start = st;
-        w.count = cnt;
-        w.level_leaf = (uint32_t)L | (leaf ? 256u : 0u);
-        w.size2f = (float)(s * s * lam2);
-        w.massf = (float)(M * mu);
-        t.walk[p] = w;
-        WalkVisit32 v32;
-        v32.cx = (float)((cx - b4n[0]) * lam);
-        v32.cy = (float)((cy - b4n[1]) * lam);
-        v32.cz = (float)((cz - b4n[2]) * lam);
-        v32.meta = (uint32_t)L | (leaf ? 32u : 0u) | ((cnt < (1u << 26) ? cnt : (1u << 26) - 1) << 6);
-        v32.skip = I.skip;
-        v32.start = st;
-        v32.child_byte = cb;
-        v32.size2 = (float)(s * s * lam2);
-        t.walk32[p] = v32;
-        WalkPay32 p32;
-        p32.mass = (float)(M * mu);
-        for (int k = 0; k < 6; ++k) p32.q[k] = cnt > 1 ? (float)(q[k] * qs) : 0.0f;
-        p32.pad = 0u;
-        t.pay32[p] = p32;
-        if (leaf && !bodies_done) {
-            // compact FP32 copy of the leaf's bodies: offsets from the leaf's
-            // centre of mass (which the walk record carries), and the leaf id
-            const uint32_t e = st + cnt;
-            for (uint32_t j0 = st; j0 < e; j0 += 8) {
-                double mj[8], xj[8], yj[8], zj[8];
+        if (!intended) {
+            cx = px[first]; cy = py[first]; cz = pz[first];
+        } else {
+            cx = sx / M; cy = sy / M; cz = sz / M;
 #pragma unroll
-                for (int k = 0; k < 8; ++k) {
-                    const uint32_t j = j0 + k < e ? j0 + k : j0;
-                    mj[k] = tm[j]; xj[k] = tx[j]; yj[k] = ty[j]; zj[k] = tz[j];
-                }
-#pragma unroll
-                for (int k = 0; k < 8; ++k) {
-                    if (j0 + k >= e) break;
-                    t.tb4[j0 + k] = make_float4((float)((xj[k] - cx) * lam), (float)((yj[k] - cy) * lam),
-                                                (float)((zj[k] - cz) * lam), (float)(mj[k] * mu));
-                    t.tleaf[j0 + k] = p;
-                }
+            for (int o = 0; o < 8; ++o) {
+                if (!((cb >> o) & 1u)) continue;
+                const uint32_t c = ids0[o];
+                quad_add(q, ch.mass(c), ch.com(c, 0) - cx, ch.com(c, 1) - cy, ch.com(c, 2) - cz);
+                for (int k = 0; k < 6; ++k) q[k] += ch.quad(c, k);
             }
         }
-        Payload out;
-        out.M = M;
-        out.cx = cx; out.cy = cy; out.cz = cz;
-        for (int k = 0; k < 6; ++k) out.q[k] = q[k];
-        out.first = first;
-        return out;
     }
+    write_node(p, I, t, false, M, cx, cy, cz, q, first);
+    Payload out;
+    out.M = M;
+    out.cx = cx; out.cy = cy; out.cz = cz;
+    for (int k = 0; k < 6; ++k) out.q[k] = q[k];
+    out.first = first;
+    return out;
 }
 
 // Moments in two launches.  Phase 1: workgroup b owns the pre-order slice
@@ -732,9 +661,11 @@ __global__ __launch_bounds__(kBlock) void k_moments_local(const Counters* __rest
                                                           const double* __restrict__ tm,
                                                           const uint32_t* __restrict__ tperm,
                                                           uint32_t* __restrict__ rest, uint32_t* __restrict__ nrest) {
-    __shared__ uint32_t s_levels;
+    __shared__ uint32_t s_levels, s_lo, s_hi;
     __shared__ double s_m[kBlock], s_c[kBlock][3], s_q[kBlock][6];
     __shared__ uint32_t s_f[kBlock];
+    __shared__ double s_bm[kStage], s_bx[kStage], s_by[kStage], s_bz[kStage];
+    __shared__ uint32_t s_bp[kStage];
     if (ctr->node_overflow) return;   // the node table is incomplete: nothing to combine
     const uint32_t total = ctr->total_nodes;
     const uint32_t b0 = blockIdx.x * kBlock;
@@ -742,7 +673,11 @@ __global__ __launch_bounds__(kBlock) void k_moments_local(const Counters* __rest
     const uint32_t b1 = b0 + kBlock < total ? b0 + kBlock : total;
     const uint32_t p = b0 + threadIdx.x;
     const bool valid = p < b1;
-    if (threadIdx.x == 0) s_levels = 0;
+    if (threadIdx.x == 0) {
+        s_levels = 0;
+        s_lo = 0xFFFFFFFFu;
+        s_hi = 0;
+    }
     __syncthreads();
     uint32_t L = 0;
     bool local = false, leaf = false;
@@ -754,17 +689,83 @@ __global__ __launch_bounds__(kBlock) void k_moments_local(const Counters* __rest
         leaf = info.cb == 0u;
         if (local && !leaf) atomicOr(&s_levels, 1u << L);
         else if (!local) rest[L * (size_t)gridDim.x + atomicAdd(&nrest[L], 1u)] = p;
+        if (leaf) {
+            atomicMin(&s_lo, info.st);
+            atomicMax(&s_hi, info.st + info.cnt);
+        }
     }
     const ChildFromSlice slice{s_m, s_c, s_f, s_q, b0};
     auto combine = [&]() {
-        const Payload r = moments_of(p, info, intended, t, px, py, pz, tx, ty, tz, tm, tperm, slice);
+        const Payload r = moments_of(p, info, intended, t, px, py, pz, slice);
         const uint32_t i = threadIdx.x;
         s_m[i] = r.M;
         s_c[i][0] = r.cx; s_c[i][1] = r.cy; s_c[i][2] = r.cz;
         for (int k = 0; k < 6; ++k) s_q[i][k] = r.q[k];
         s_f[i] = r.first;
     };
-    if (leaf) combine();
+    __syncthreads();
+    // The leaves: their bodies (one tree-order range for the slice) pass
+    // through LDS kStage at a time in coalesced loads, and each leaf's thread
+    // adds up its own bodies from there in body order: masses, first moments
+    // and the least array index, then (the centre of mass known) the
+    // quadrupole and the FP32 body copies.
+    {
+        const uint32_t lo = s_lo, hi = s_hi;
+        const uint32_t st = info.st, e = info.st + info.cnt;
+        const double lam = info.lam, mu = info.mu;
+        double M = 0.0, sx = 0.0, sy = 0.0, sz = 0.0, cx = 0.0, cy = 0.0, cz = 0.0;
+        double q[6] = {0, 0, 0, 0, 0, 0};
+        uint32_t first = 0xFFFFFFFFu;
+        for (int pass = 0; pass < 2; ++pass) {
+            if (pass == 1 && leaf) {
+                if (!intended || info.cnt == 1) {
+                    cx = px[first]; cy = py[first]; cz = pz[first];
+                } else {
+                    cx = sx / M; cy = sy / M; cz = sz / M;
+                }
+            }
+            for (uint32_t c0 = lo; c0 < hi; c0 += kStage) {
+                const uint32_t c1 = hi - c0 < (uint32_t)kStage ? hi : c0 + kStage;
+                for (uint32_t j = c0 + threadIdx.x; j < c1; j += kBlock) {
+                    s_bm[j - c0] = tm[j];
+                    s_bx[j - c0] = tx[j];
+                    s_by[j - c0] = ty[j];
+                    s_bz[j - c0] = tz[j];
+                    if (pass == 0) s_bp[j - c0] = tperm[j];
+                }
+                __syncthreads();
+                if (leaf) {
+                    const uint32_t a0 = st > c0 ? st : c0, a1 = e < c1 ? e : c1;
+                    for (uint32_t j = a0; j < a1; ++j) {
+                        const double mj = s_bm[j - c0], xj = s_bx[j - c0], yj = s_by[j - c0], zj = s_bz[j - c0];
+                        if (pass == 0) {
+                            M += mj;
+                            sx += mj * xj;
+                            sy += mj * yj;
+                            sz += mj * zj;
+                            const uint32_t aj = s_bp[j - c0];
+                            if (aj < first) first = aj;
+                        } else {
+                            const double dx = xj - cx, dy = yj - cy, dz = zj - cz;
+                            if (intended && info.cnt > 1) quad_add(q, mj, dx, dy, dz);
+                            t.tb4[j] = make_float4((float)(dx * lam), (float)(dy * lam), (float)(dz * lam),
+                                                   (float)(mj * mu));
+                            t.tleaf[j] = p;
+                        }
+                    }
+                }
+                __syncthreads();
+            }
+        }
+        if (leaf) {
+            write_node(p, info, t, true, M, cx, cy, cz, q, first);
+            const uint32_t i = threadIdx.x;
+            s_m[i] = M;
+            s_c[i][0] = cx; s_c[i][1] = cy; s_c[i][2] = cz;
+            for (int k = 0; k < 6; ++k) s_q[i][k] = q[k];
+            s_f[i] = first;
+        }
+    }
     __syncthreads();
     const uint32_t levels = s_levels;
     for (int l = kMaxLevel; l >= 0; --l) {
@@ -805,10 +806,10 @@ __global__ __launch_bounds__(256) void k_moments_rest(const Counters* __restrict
         }
         for (uint32_t i = threadIdx.x; i < m; i += blockDim.x) {
             if (i == threadIdx.x) {
-                (void)moments_of(p0, info0, intended, t, px, py, pz, tx, ty, tz, tm, tperm, ChildFromGlobal{t});
+                (void)moments_of(p0, info0, intended, t, px, py, pz, ChildFromGlobal{t});
             } else {
                 const uint32_t p = rest[l * (size_t)slices + i];
-                (void)moments_of(p, node_info(p, t, sv, bounds4), intended, t, px, py, pz, tx, ty, tz, tm, tperm,
+                (void)moments_of(p, node_info(p, t, sv, bounds4), intended, t, px, py, pz,
                                  ChildFromGlobal{t});
             }
         }
