@@ -365,8 +365,7 @@ int enqueue_step_async(bhx_ctx* c, hipStream_t s, bool linear) {
     HIPCHK(tree_order(c, b_cur, branch, s));
     size_t total = 0;
     HIPCHK(build_tree(c, s, &total, false));
-    HIPCHK(launch_walk(c, s));
-    HIPCHK(launch_kick(c, dt, s, &c->ctr->nonfinite));
+    HIPCHK(launch_walk_kick(c, dt, s, &c->ctr->nonfinite));
     if (!linear) HIPCHK(hipStreamWaitEvent(s, c->ev_bounds, 0));
     c->bslot = cur ^ 1;
     return BHX_OK;
@@ -1257,10 +1256,15 @@ int bhx_step(bhx_ctx* c, int nsteps) {
         c->tree_valid = c->tree_built = true;
         c->tree_n = c->n;
         if (prof) HIPCHK(hipEventRecord(c->ev[6], s));
-        HIPCHK(launch_walk(c, s));
-        if (prof) HIPCHK(hipEventRecord(c->ev[7], s));
-        HIPCHK(launch_kick(c, dt, s, &c->ctr->nonfinite));
-        if (prof) HIPCHK(hipEventRecord(c->ev[8], s));
+        if (prof) {
+            // timed apart: the walk alone, then the kick
+            HIPCHK(launch_walk(c, s));
+            HIPCHK(hipEventRecord(c->ev[7], s));
+            HIPCHK(launch_kick(c, dt, s, &c->ctr->nonfinite));
+            HIPCHK(hipEventRecord(c->ev[8], s));
+        } else {
+            HIPCHK(launch_walk_kick(c, dt, s, &c->ctr->nonfinite));
+        }
         c->bslot = cur ^ 1;
         c->bounds_valid = true;
         if (prof) {

@@ -753,12 +753,6 @@ __device__ __forceinline__ void kick_drift_one(double& x, double& v, double a, d
     v = v + (a * dt) * 0.5;
     x = x + v * dt;
 }
-__device__ __forceinline__ void kahan_add(double& s, double& c, double inc) {
-    const double y = inc - c;
-    const double t = s + y;
-    c = (t - s) - y;
-    s = t;
-}
 
 // One body's kick then drift from its state at index j (acceleration at ja);
 // the frame's x_{f+1} and v_{f+1/2}.
@@ -883,15 +877,11 @@ __global__ __launch_bounds__(kBlock) void k_kick(double* __restrict__ vx, double
     for (size_t i = blockIdx.x * (size_t)kBlock + threadIdx.x; i < n; i += (size_t)gridDim.x * kBlock) {
         double v0 = vx[i], v1 = vy[i], v2 = vz[i];
         if (!kComp) {
-            v0 = v0 + (ax[i] * dt) * 0.5;
-            v1 = v1 + (ay[i] * dt) * 0.5;
-            v2 = v2 + (az[i] * dt) * 0.5;
+            half_kick(v0, v1, v2, nullptr, ax[i], ay[i], az[i], dt);
         } else {
-            double c0 = cvx[i], c1 = cvy[i], c2 = cvz[i];
-            kahan_add(v0, c0, (ax[i] * dt) * 0.5);
-            kahan_add(v1, c1, (ay[i] * dt) * 0.5);
-            kahan_add(v2, c2, (az[i] * dt) * 0.5);
-            cvx[i] = c0; cvy[i] = c1; cvz[i] = c2;
+            double c[3] = {cvx[i], cvy[i], cvz[i]};
+            half_kick(v0, v1, v2, c, ax[i], ay[i], az[i], dt);
+            cvx[i] = c[0]; cvy[i] = c[1]; cvz[i] = c[2];
         }
         vx[i] = v0; vy[i] = v1; vz[i] = v2;
         if (nonfinite && !(isfinite(v0) && isfinite(v1) && isfinite(v2))) *nonfinite = 1u;

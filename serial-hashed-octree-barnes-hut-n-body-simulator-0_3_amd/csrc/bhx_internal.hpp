@@ -119,6 +119,31 @@ struct SysView {
 
 __device__ __forceinline__ uint32_t sys_of(const SysView& v, uint32_t i) { return v.id ? v.id[i] : 0u; }
 
+// s += inc keeping the rounding error in c (Kahan), fed into the next addition.
+__device__ __forceinline__ void kahan_add(double& s, double& c, double inc) {
+    const double y = inc - c;
+    const double t = s + y;
+    c = (t - s) - y;
+    s = t;
+}
+
+// The closing half kick of a frame for one body — ComputeVelocityAndPosition,
+// Body.hpp:317-324: v = v + (a*dt)*0.5, or compensated (c: the velocity's
+// running rounding error).  One definition for every kernel that applies it,
+// so the bits do not depend on which one did.
+__device__ __forceinline__ void half_kick(double& v0, double& v1, double& v2, double* c, double a0, double a1,
+                                          double a2, double dt) {
+    if (!c) {
+        v0 = v0 + (a0 * dt) * 0.5;
+        v1 = v1 + (a1 * dt) * 0.5;
+        v2 = v2 + (a2 * dt) * 0.5;
+    } else {
+        kahan_add(v0, c[0], (a0 * dt) * 0.5);
+        kahan_add(v1, c[1], (a1 * dt) * 0.5);
+        kahan_add(v2, c[2], (a2 * dt) * 0.5);
+    }
+}
+
 // Node hash table: open addressing with linear probing, an empty slot holds
 // key 0 (every valid key carries the root sentinel bit).  A deleted node
 // keeps its key in its slot (so probe sequences stay intact) with this value.
@@ -399,8 +424,14 @@ hipError_t launch_lookup(bhx_ctx* c, uint32_t sys, const uint64_t* keys, size_t 
 hipError_t ensure_hash(bhx_ctx* c, hipStream_t s);
 
 hipError_t launch_walk(bhx_ctx* c, hipStream_t s);
+// the walk with the frame's closing half kick (v += (a dt)/2) fused where it can be
+hipError_t launch_walk_kick(bhx_ctx* c, double dt, hipStream_t s, uint32_t* nonfinite);
+struct WalkKick {
+    double dt;
+    uint32_t* nonfinite;
+};
 hipError_t launch_walk_to(bhx_ctx* c, hipStream_t s, bool group, int precision, double* ox, double* oy, double* oz,
-                          uint32_t* counts);
+                          uint32_t* counts, const WalkKick* kick = nullptr);
 hipError_t launch_direct(bhx_ctx* c, const int64_t* targets, size_t nt, double eps, double* acc,
                          double* pot, hipStream_t s);
 

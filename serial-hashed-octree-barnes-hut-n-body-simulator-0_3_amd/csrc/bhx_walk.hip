@@ -103,6 +103,14 @@ struct WalkArgs {
     const uint32_t* child_byte;
     // decision digest (FP64 group walk): [0] sum, [1] collect when non-zero, [2] frame index
     unsigned long long* digest;
+    // the frame's closing half kick, applied by the group walks as each body's
+    // acceleration is written (kick: 0 none, 1 plain, 2 compensated)
+    uint32_t kick;
+    double kdt;
+    double *kvx, *kvy, *kvz;      // velocities (array order)
+    double *kcx, *kcy, *kcz;      // their compensation terms
+    uint32_t* knonfinite;         // set when a kicked velocity is not finite
+
 };
 
 // One term of ComputeHOTForceInteractionList (HashedOctree.hpp:411-454) in
@@ -699,6 +707,21 @@ __device__ __forceinline__ void eval_list32(const float (*nd)[4], const float (*
         }
         if (kPot) phi -= (dx * dx + dy * dy + dz * dz) > 0.f ? pt : 0.f;
     }
+}
+
+// The closing half kick of body o with the acceleration just written for it
+// (the same arithmetic as the kick kernel).
+__device__ __forceinline__ void walk_kick(const WalkArgs& a, uint32_t o, double ax, double ay, double az) {
+    double v0 = a.kvx[o], v1 = a.kvy[o], v2 = a.kvz[o];
+    if (a.kick == 2u) {
+        double c[3] = {a.kcx[o], a.kcy[o], a.kcz[o]};
+        half_kick(v0, v1, v2, c, ax, ay, az, a.kdt);
+        a.kcx[o] = c[0]; a.kcy[o] = c[1]; a.kcz[o] = c[2];
+    } else {
+        half_kick(v0, v1, v2, nullptr, ax, ay, az, a.kdt);
+    }
+    a.kvx[o] = v0; a.kvy[o] = v1; a.kvz[o] = v2;
+    if (a.knonfinite && !(isfinite(v0) && isfinite(v1) && isfinite(v2))) *a.knonfinite = 1u;
 }
 
 // List capacities of the FP32 walk: 64 cells per flush (one per lane), 32
@@ -1305,10 +1328,12 @@ __global__ __launch_bounds__(64) void k_walk_lp32(WalkArgs a) {
                 a.gself[3 * (size_t)o + 2] = (double)az * back;
             } else {
                 const double back = lam * lam / mu;
-                a.ax[o] = (double)ax * back;
-                a.ay[o] = (double)ay * back;
-                a.az[o] = (double)az * back;
+                const double gx = (double)ax * back, gy = (double)ay * back, gz = (double)az * back;
+                a.ax[o] = gx;
+                a.ay[o] = gy;
+                a.az[o] = gz;
                 if constexpr (kPot) a.pot[o] = (double)phi * (lam / mu);
+                if (a.kick) walk_kick(a, o, gx, gy, gz);
             }
             if (a.counts) a.counts[o] = cnt;
         }
@@ -1909,6 +1934,7 @@ __global__ __launch_bounds__(64) void k_walk_lp64(WalkArgs a) {
                 a.ay[o] = ay;
                 a.az[o] = az;
                 if constexpr (kPot) a.pot[o] = phi;
+                if (a.kick) walk_kick(a, o, ax, ay, az);
             }
             if (a.counts) a.counts[o] = cnt;
         }
@@ -2060,10 +2086,29 @@ hipError_t launch_walk(bhx_ctx* c, hipStream_t s) {
     return launch_walk_to(c, s, c->cfg.group_walk != 0, c->cfg.precision, c->ax, c->ay, c->az, c->wcount);
 }
 
+// The walk and the frame's closing half kick: the group walks kick each body
+// as they write its acceleration; after a per-body walk the kick kernel runs.
+hipError_t launch_walk_kick(bhx_ctx* c, double dt, hipStream_t s, uint32_t* nonfinite) {
+    if (c->cfg.group_walk == 0) {
+        hipError_t e = launch_walk(c, s);
+        return e != hipSuccess ? e : launch_kick(c, dt, s, nonfinite);
+    }
+    WalkKick k{dt, nonfinite};
+    return launch_walk_to(c, s, true, c->cfg.precision, c->ax, c->ay, c->az, c->wcount, &k);
+}
+
 hipError_t launch_walk_to(bhx_ctx* c, hipStream_t s, bool group, int precision, double* ox, double* oy, double* oz,
-                          uint32_t* counts) {
+                          uint32_t* counts, const WalkKick* kick) {
     if (c->n == 0) return hipSuccess;
+    if (kick && !group) return hipErrorInvalidValue;   // only the group walks kick
     WalkArgs a{};
+    if (kick) {
+        a.kick = c->cfg.compensated ? 2u : 1u;
+        a.kdt = kick->dt;
+        a.kvx = c->vx; a.kvy = c->vy; a.kvz = c->vz;
+        a.kcx = c->cvx; a.kcy = c->cvy; a.kcz = c->cvz;
+        a.knonfinite = kick->nonfinite;
+    }
     a.nodes = c->tree.walk;
     a.nodes32 = c->tree.walk32;
     a.pay32 = c->tree.pay32;
