@@ -207,6 +207,12 @@ int bhx_walk(bhx_ctx *ctx, double *acc_out, uint64_t *counts);
  * selects.  The resident accelerations are left unchanged. */
 int bhx_interaction_counts(bhx_ctx *ctx, uint64_t *counts);
 
+/* The same per-body reference lists, split for ComputeHOTForceInteractionList
+ * (HashedOctree.hpp:393-459): counts as above, and quad_counts[i] = how many of
+ * body i's entries take the quadrupole branch (an accepted node of more than
+ * one body, quadrupoles on; 0 with them off).  Accelerations unchanged. */
+int bhx_interaction_counts_quad(bhx_ctx *ctx, uint64_t *counts, uint64_t *quad_counts);
+
 /* ---- the reference's per-body tree API -------------------------------------
  * ComputeHOTOctreeForce (HashedOctree.hpp:262-272) is bhx_walk, batched.  Code
  * that calls the per-body pair itself, the MAC, or edits the node table uses

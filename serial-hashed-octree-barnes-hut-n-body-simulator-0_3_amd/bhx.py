@@ -69,7 +69,7 @@ def lib():
             "bhx_sort_bodies": [P, P], "bhx_reorder_bodies": [P, P], "bhx_kick_drift": [P, D], "bhx_kick": [P, D],
             "bhx_build": [P, P, D, P], "bhx_get_nodes": [P, P, S, P], "bhx_lookup": [P, P, S, P],
             "bhx_get_tree_order": [P, P, P], "bhx_unsplit_leaves": [P, P], "bhx_walk": [P, P, P],
-            "bhx_interaction_counts": [P, P],
+            "bhx_interaction_counts": [P, P], "bhx_interaction_counts_quad": [P, P, P],
             "bhx_step": [P, I], "bhx_last_timing": [P, P], "bhx_set_profiling": [P, I],
             "bhx_direct": [P, P, S, D, P, P], "bhx_diagnostics": [P, P, P, P],
             "bhx_ic_plummer": [S, ctypes.c_uint64, D, D, P, P, P],
@@ -443,11 +443,16 @@ class Engine:
         self._chk(lib().bhx_walk(self.h, _p(acc), _p(cnt)), "bhx_walk")
         return (acc, cnt) if counts else acc
 
-    def interaction_counts(self):
-        """Reference per-body interaction-list lengths on the last tree (accelerations untouched)."""
+    def interaction_counts(self, quad=False):
+        """Reference per-body interaction-list lengths on the last tree (accelerations untouched);
+        with quad, also how many entries of each list take the quadrupole branch."""
         cnt = np.empty(self.n, np.uint64)
-        self._chk(lib().bhx_interaction_counts(self.h, _p(cnt)), "bhx_interaction_counts")
-        return cnt
+        if not quad:
+            self._chk(lib().bhx_interaction_counts(self.h, _p(cnt)), "bhx_interaction_counts")
+            return cnt
+        qc = np.empty(self.n, np.uint64)
+        self._chk(lib().bhx_interaction_counts_quad(self.h, _p(cnt), _p(qc)), "bhx_interaction_counts_quad")
+        return cnt, qc
 
     def step(self, nsteps=1):
         self._chk(lib().bhx_step(self.h, nsteps), "bhx_step")

@@ -1189,6 +1189,29 @@ int bhx_interaction_counts(bhx_ctx* c, uint64_t* counts) {
     return BHX_OK;
 }
 
+int bhx_interaction_counts_quad(bhx_ctx* c, uint64_t* counts, uint64_t* quad_counts) {
+    if (!c || !counts || !quad_counts)
+        return c ? fail(c, BHX_E_NULL, "counts is a null pointer") : BHX_E_NULL;
+    if (!c->tree_valid) return fail(c, BHX_E_STATE, "no tree built");
+    if (c->n == 0) return BHX_OK;
+    HIPCHK(hipSetDevice(c->device));
+    hipStream_t s = c->s_main;
+    // the same per-body walk as bhx_interaction_counts; the quadrupole-term
+    // counts go to a stale gather target (n doubles hold n u32), their u64
+    // copy to another
+    uint32_t* qc = reinterpret_cast<uint32_t*>(c->alt[12]);
+    HIPCHK(launch_walk_to(c, s, false, BHX_FP64, c->alt[13], c->alt[14], c->alt[15], c->wcount, nullptr, qc));
+    uint64_t* tmp = reinterpret_cast<uint64_t*>(c->stage);
+    uint64_t* qtmp = reinterpret_cast<uint64_t*>(c->alt[11]);
+    hipLaunchKernelGGL(k_u32_to_u64, dim3(grid_for(c->n)), dim3(kBlock), 0, s, c->wcount, c->n, tmp);
+    hipLaunchKernelGGL(k_u32_to_u64, dim3(grid_for(c->n)), dim3(kBlock), 0, s, (const uint32_t*)qc, c->n, qtmp);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(counts, tmp, c->n * 8, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipMemcpyAsync(quad_counts, qtmp, c->n * 8, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    return BHX_OK;
+}
+
 int bhx_set_profiling(bhx_ctx* c, int on) {
     if (!c) return BHX_E_NULL;
     c->profiling = on != 0;

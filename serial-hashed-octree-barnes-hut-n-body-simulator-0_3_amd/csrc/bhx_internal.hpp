@@ -431,7 +431,7 @@ struct WalkKick {
     uint32_t* nonfinite;
 };
 hipError_t launch_walk_to(bhx_ctx* c, hipStream_t s, bool group, int precision, double* ox, double* oy, double* oz,
-                          uint32_t* counts, const WalkKick* kick = nullptr);
+                          uint32_t* counts, const WalkKick* kick = nullptr, uint32_t* qcounts = nullptr);
 hipError_t launch_direct(bhx_ctx* c, const int64_t* targets, size_t nt, double eps, double* acc,
                          double* pot, hipStream_t s);
 

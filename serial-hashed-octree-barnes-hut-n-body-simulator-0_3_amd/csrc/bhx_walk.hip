@@ -75,6 +75,7 @@ struct WalkArgs {
     double *ax, *ay, *az;
     double* pot;             // potential per body (array order), potential walks only
     uint32_t* counts;
+    uint32_t* qcounts;       // per-body reference walk: the interactions that carry the quadrupole term
     // adjoint walk: weights w = dL/da per body (array order) in, dL/dx of each
     // body's own position out (array order, 3 per body), and per list entry
     // (dL/dX, dL/dM) summed over the targets: 4 per node, 4 per body (tree order)
@@ -163,7 +164,7 @@ __global__ __launch_bounds__(kBlock) void k_walk64(WalkArgs a) {
     ConstD tz = (ConstD)a.tz;
     ConstD tm = (ConstD)a.tm;
     double ax = 0.0, ay = 0.0, az = 0.0;
-    uint32_t cnt = 0;
+    uint32_t cnt = 0, qcnt = 0;
     uint32_t openbits = 1;                 // bit L: this lane opened the current level-L ancestor
     const bool root_leaf = (nodes[root].level_leaf & 256u) != 0;
     uint32_t p = root_leaf ? root : root + 1;
@@ -198,6 +199,7 @@ __global__ __launch_bounds__(kBlock) void k_walk64(WalkArgs a) {
                 if (kQuad && ncount > 1) q = a.quad + 6 * (size_t)p;
                 term64(ax, ay, az, dx, dy, dz, d2, m, a.eps2, q);
                 ++cnt;
+                if (q) ++qcnt;
             }
         }
         const bool open = active && !pass;
@@ -228,6 +230,7 @@ __global__ __launch_bounds__(kBlock) void k_walk64(WalkArgs a) {
         a.ay[o] = ay;
         a.az[o] = az;
         if (a.counts) a.counts[o] = cnt;
+        if (a.qcounts) a.qcounts[o] = qcnt;
     }
 }
 
@@ -2098,10 +2101,11 @@ hipError_t launch_walk_kick(bhx_ctx* c, double dt, hipStream_t s, uint32_t* nonf
 }
 
 hipError_t launch_walk_to(bhx_ctx* c, hipStream_t s, bool group, int precision, double* ox, double* oy, double* oz,
-                          uint32_t* counts, const WalkKick* kick) {
+                          uint32_t* counts, const WalkKick* kick, uint32_t* qcounts) {
     if (c->n == 0) return hipSuccess;
     if (kick && !group) return hipErrorInvalidValue;   // only the group walks kick
     WalkArgs a{};
+    a.qcounts = qcounts;
     if (kick) {
         a.kick = c->cfg.compensated ? 2u : 1u;
         a.kdt = kick->dt;

@@ -629,6 +629,18 @@ def test_group_walk_is_at_least_as_accurate(bhx, plummer16k, prec):
     _, ocnt = t.walk(0.5, 0.01, quad=True)
     assert np.array_equal(cnt_ref.astype(np.int64), ocnt)     # reference list lengths
     assert np.all(cnt_g.astype(np.int64) >= ocnt)              # shared lists refine every body's list
+    # the entries that take the quadrupole branch: nodes of more than one body in
+    # the reference's own list of each sampled body
+    with _engine(bhx, N1, group_walk=1, **kw) as e:
+        e.set_bodies(pos, vel, mass)
+        e.build(c, s)
+        cnt_q, qc = e.interaction_counts(quad=True)
+    assert np.array_equal(cnt_q, cnt_ref)
+    nn = t.nodes()["n"]
+    for i in range(0, N1, 512):
+        ids, kind = t.interaction_list(pos[i], 0.5)
+        assert int(qc[i]) == int(np.count_nonzero((kind == 0) & (nn[ids * (kind == 0)] > 1))), i
+    assert 0 < qc.sum() < cnt_ref.sum()
 
 
 def test_group_walk_step_energy(bhx):
