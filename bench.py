@@ -83,8 +83,9 @@ PHASE_BYTES = {
     "keys_sorts_kick_drift_tree": (("keys", "sort", "kick_drift", "treekeys"),
                                    lambda r: 36.0 + 200.0 + 236.0 + 304.0),
     "build": (("build",), lambda r: 40.0 + 208.0 * r),      # keys and tree-order bodies; nodes, hash, moments
-    "walk": (("walk",), lambda r: 56.0 + 32.0 * r),         # bodies in, accelerations out, each node once
-    "kick": (("kick",), lambda r: 120.0),                   # velocity, its compensation and acceleration
+    # bodies in, accelerations out, each node once; the closing kick's velocity and compensation in and
+    # out (the group walk applies it as it writes each acceleration)
+    "walk_kick": (("walk", "kick"), lambda r: 56.0 + 32.0 * r + 96.0),
 }
 BOUNDS_BYTES = 72.0                         # positions read three times (two for the mean, one for the size)
 PEAK_HBM_GBPS = 8000.0

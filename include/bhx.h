@@ -95,7 +95,9 @@ typedef struct bhx_node {
     int64_t  body_start;   /* first body of the node in tree order                  */
 } bhx_node;
 
-typedef struct bhx_timing {   /* milliseconds of the last bhx_step, per phase */
+typedef struct bhx_timing {   /* milliseconds of the last bhx_step, per phase; with the group walks
+                                 the closing kick is applied by the walk kernel (walk includes it,
+                                 kick reads 0) */
     double bounds, keys, sort, kick_drift, treekeys, build, moments, walk, kick, total;
 } bhx_timing;
 

@@ -1279,15 +1279,9 @@ int bhx_step(bhx_ctx* c, int nsteps) {
         c->tree_valid = c->tree_built = true;
         c->tree_n = c->n;
         if (prof) HIPCHK(hipEventRecord(c->ev[6], s));
-        if (prof) {
-            // timed apart: the walk alone, then the kick
-            HIPCHK(launch_walk(c, s));
-            HIPCHK(hipEventRecord(c->ev[7], s));
-            HIPCHK(launch_kick(c, dt, s, &c->ctr->nonfinite));
-            HIPCHK(hipEventRecord(c->ev[8], s));
-        } else {
-            HIPCHK(launch_walk_kick(c, dt, s, &c->ctr->nonfinite));
-        }
+        HIPCHK(launch_walk_kick(c, dt, s, &c->ctr->nonfinite));
+        if (prof) HIPCHK(hipEventRecord(c->ev[7], s));
+        if (prof) HIPCHK(hipEventRecord(c->ev[8], s));   // the kick is the walk's (group walks)
         c->bslot = cur ^ 1;
         c->bounds_valid = true;
         if (prof) {
