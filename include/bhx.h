@@ -158,7 +158,7 @@ int bhx_sort_bodies(bhx_ctx *ctx, uint64_t *perm_out);
  * and idx are device pointers to n values on `device` (a HIP device ordinal),
  * sorted in place on `stream` (a hipStream_t; NULL = the null stream) into the
  * same stable ascending order as bhx_sort_keys, idx carried along.  All
- * working memory is a static device array of about 26 MiB per device, held
+ * working memory is a static device array of about 25 MiB per device, held
  * until the process ends; it fixes the maximum n at BHX_NOALLOC_MAX_KEYS
  * (BHX_E_CAPACITY beyond).  Calls on one device share that array: they are
  * serialised and each returns when its sort is done (the stream is

@@ -388,6 +388,7 @@ hipError_t launch_treekeys(const double* px, const double* py, const double* pz,
                            const double* bounds4, double tscale, uint64_t* keys, uint32_t* vals, hipStream_t s);
 hipError_t launch_iota(uint32_t* vals, size_t n, hipStream_t s);
 
+constexpr size_t kSortTileKeys = 8192;   // keys one radix-sort workgroup ranks per pass
 size_t sort_tiles_for(size_t n);
 // `ndev` non-null: the pair count is read on the device from *ndev (at most
 // n; the scratch and the grid are sized for n).  An even number of passes

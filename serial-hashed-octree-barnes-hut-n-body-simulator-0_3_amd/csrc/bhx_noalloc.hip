@@ -19,7 +19,7 @@ namespace bhx {
 namespace {
 
 constexpr size_t kMaxKeys = BHX_NOALLOC_MAX_KEYS;
-constexpr size_t kPasses = 8, kBins = 256, kTileKeys = 4096;
+constexpr size_t kPasses = 8, kBins = 256, kTileKeys = kSortTileKeys;
 constexpr size_t kMaxTiles = (kMaxKeys + kTileKeys - 1) / kTileKeys;
 
 // byte offsets of the regions (each a multiple of 256)

@@ -8,7 +8,7 @@
 // A pass whose digit is the same for every key (its histogram has one
 // occupied bin) is a plain copy.  Each other pass is a single "onesweep"
 // kernel (Adinets & Merrill 2022): a
-// workgroup ranks a tile of 4096 keys in LDS, publishes its per-digit counts,
+// workgroup ranks a tile of 8192 keys in LDS, publishes its per-digit counts,
 // and finds its global offsets by decoupled look-back (Merrill & Garland
 // 2016) over the tiles before it.  Tile indices come from an atomic counter
 // in launch order, so every tile a workgroup waits on is already running or
@@ -20,8 +20,10 @@ namespace bhx {
 namespace {
 
 constexpr int kBlock = 256;
+constexpr int kPassBlock = 512;              // threads of a pass workgroup (8 waves)
 constexpr int kItems = 16;
-constexpr int kTile = kBlock * kItems;        // 4096 keys per tile
+constexpr int kTile = kPassBlock * kItems;    // 8192 keys per tile
+static_assert(kTile == (int)kSortTileKeys, "the tile the scratch is sized for");
 constexpr int kPasses = 8;
 constexpr int kBins = 256;
 constexpr uint32_t kFlagA = 1u << 30;         // tile aggregate available
@@ -37,7 +39,8 @@ __device__ __forceinline__ uint32_t ld_status(uint32_t* p) {
     return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-// Exclusive scan of one value per thread over a 256-thread block.
+// Exclusive scan of one value per thread over a B-thread block.
+template <int B = kBlock>
 __device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t v, uint32_t* s_wsum, uint32_t* total) {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     uint32_t x = v;
@@ -50,7 +53,7 @@ __device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t v, uint32_t* s
     __syncthreads();
     uint32_t wbase = 0, tot = 0;
 #pragma unroll
-    for (int w = 0; w < kBlock / 64; ++w) {
+    for (int w = 0; w < B / 64; ++w) {
         const uint32_t ws = s_wsum[w];
         if (w < wave) wbase += ws;
         tot += ws;
@@ -98,7 +101,7 @@ __global__ __launch_bounds__(kBlock) void k_sort_scan(uint32_t* __restrict__ his
     }
 }
 
-__global__ __launch_bounds__(kBlock) void k_sort_pass(const uint64_t* __restrict__ kin,
+__global__ __launch_bounds__(kPassBlock) void k_sort_pass(const uint64_t* __restrict__ kin,
                                                       const uint32_t* __restrict__ vin,
                                                       uint64_t* __restrict__ kout,
                                                       uint32_t* __restrict__ vout, size_t n, int pass,
@@ -108,23 +111,29 @@ __global__ __launch_bounds__(kBlock) void k_sort_pass(const uint64_t* __restrict
                                                       Counters* __restrict__ ctr,
                                                       const uint32_t* __restrict__ ndev) {
     if (ndev) n = *ndev;                       // the grid covers the largest count
+    constexpr int kWaves = kPassBlock / 64;
     __shared__ uint32_t s_tile;
-    __shared__ uint32_t s_wcnt[kBlock / 64][kBins];
+    __shared__ uint32_t s_wcnt[kWaves][kBins];
     __shared__ uint32_t s_tstart[kBins];
     __shared__ uint32_t s_gbase[kBins];
-    __shared__ uint32_t s_wsum[kBlock / 64];
+    __shared__ uint32_t s_wsum[kWaves];
     __shared__ uint64_t s_keys[kTile];
     __shared__ uint32_t s_vals[kTile];
 
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    const bool dig = tid < kBins;              // threads 0 .. 255 also stand for one digit each
     // a digit every key shares (one occupied bin: its count is n) leaves the
     // order as it is; the pass is then a copy (keys in a small range, or
     // systems whose high key bytes agree)
     {
-        const uint32_t g = goff[pass * kBins + tid];
-        const uint32_t gn = tid + 1 < kBins ? goff[pass * kBins + tid + 1] : (uint32_t)n;
-        if (__syncthreads_count(gn - g == (uint32_t)n) != 0) {
-            for (size_t i = blockIdx.x * (size_t)kBlock + tid; i < n; i += (size_t)gridDim.x * kBlock) {
+        bool one = false;
+        if (dig) {
+            const uint32_t g = goff[pass * kBins + tid];
+            const uint32_t gn = tid + 1 < kBins ? goff[pass * kBins + tid + 1] : (uint32_t)n;
+            one = gn - g == (uint32_t)n;
+        }
+        if (__syncthreads_count(one) != 0) {
+            for (size_t i = blockIdx.x * (size_t)kPassBlock + tid; i < n; i += (size_t)gridDim.x * kPassBlock) {
                 kout[i] = kin[i];
                 vout[i] = vin[i];
             }
@@ -132,7 +141,7 @@ __global__ __launch_bounds__(kBlock) void k_sort_pass(const uint64_t* __restrict
         }
     }
     if (tid == 0) s_tile = atomicAdd(tile_ctr + pass, 1u);
-    for (int j = tid; j < (kBlock / 64) * kBins; j += kBlock) (&s_wcnt[0][0])[j] = 0;
+    for (int j = tid; j < kWaves * kBins; j += kPassBlock) (&s_wcnt[0][0])[j] = 0;
     __syncthreads();
     const uint32_t tile = s_tile;
     const size_t base = (size_t)tile * kTile;
@@ -176,15 +185,20 @@ __global__ __launch_bounds__(kBlock) void k_sort_pass(const uint64_t* __restrict
     }
     __syncthreads();
 
-    // Per digit (thread = digit): counts of the four waves, tile total.
-    const uint32_t c0 = s_wcnt[0][tid], c1 = s_wcnt[1][tid], c2 = s_wcnt[2][tid], c3 = s_wcnt[3][tid];
-    const uint32_t tot = c0 + c1 + c2 + c3;
+    // Per digit (thread = digit): counts of the waves, tile total.
+    uint32_t cw[kWaves];
+    uint32_t tot = 0;
+#pragma unroll
+    for (int w = 0; w < kWaves; ++w) {
+        cw[w] = dig ? s_wcnt[w][tid] : 0u;
+        tot += cw[w];
+    }
     uint32_t* st = status + (size_t)tile * kBins;
-    st_status(st + tid, (tile == 0 ? kFlagP : kFlagA) | tot);
-    const uint32_t tstart = block_exclusive_scan(tot, s_wsum, nullptr);
+    if (dig) st_status(st + tid, (tile == 0 ? kFlagP : kFlagA) | tot);
+    const uint32_t tstart = block_exclusive_scan<kPassBlock>(tot, s_wsum, nullptr);
 
     uint32_t excl = 0;
-    if (tile > 0) {
+    if (dig && tile > 0) {
         // kLookWindow predecessors' statuses in flight at once, read nearest first
         int64_t t = (int64_t)tile - 1;
         uint32_t spins = 0;
@@ -219,12 +233,16 @@ __global__ __launch_bounds__(kBlock) void k_sort_pass(const uint64_t* __restrict
         }
         st_status(st + tid, kFlagP | (excl + tot));
     }
-    s_tstart[tid] = tstart;
-    s_gbase[tid] = goff[pass * kBins + tid] + excl;
-    s_wcnt[0][tid] = 0;
-    s_wcnt[1][tid] = c0;
-    s_wcnt[2][tid] = c0 + c1;
-    s_wcnt[3][tid] = c0 + c1 + c2;
+    if (dig) {
+        s_tstart[tid] = tstart;
+        s_gbase[tid] = goff[pass * kBins + tid] + excl;
+        uint32_t run = 0;
+#pragma unroll
+        for (int w = 0; w < kWaves; ++w) {
+            s_wcnt[w][tid] = run;
+            run += cw[w];
+        }
+    }
     __syncthreads();
 
     // Tile-local sorted order in LDS, then coalesced writes per digit run.
@@ -240,7 +258,7 @@ __global__ __launch_bounds__(kBlock) void k_sort_pass(const uint64_t* __restrict
     }
     __syncthreads();
     const uint32_t nvalid = (uint32_t)((n - base) < (size_t)kTile ? (n - base) : (size_t)kTile);
-    for (uint32_t q = tid; q < nvalid; q += kBlock) {
+    for (uint32_t q = tid; q < nvalid; q += kPassBlock) {
         const uint64_t k = s_keys[q];
         const uint32_t d = (uint32_t)(k >> shift) & 0xFF;
         const uint32_t out = s_gbase[d] + (q - s_tstart[d]);
@@ -271,7 +289,7 @@ hipError_t radix_sort_pairs(uint64_t* keys, uint32_t* vals, uint64_t* keys_alt, 
     hipLaunchKernelGGL(k_sort_scan, dim3(1), dim3(kBlock), 0, s, hist);
     for (int p = 0; p < passes; ++p) {
         const bool even = (p & 1) == 0;
-        hipLaunchKernelGGL(k_sort_pass, dim3((unsigned)tiles), dim3(kBlock), 0, s, even ? keys : keys_alt,
+        hipLaunchKernelGGL(k_sort_pass, dim3((unsigned)tiles), dim3(kPassBlock), 0, s, even ? keys : keys_alt,
                            even ? vals : vals_alt, even ? keys_alt : keys, even ? vals_alt : vals, n, p, hist,
                            status + (size_t)p * tiles * kBins, tile_ctr, ctr, ndev);
     }
