@@ -105,7 +105,8 @@ struct Counters {          // small device-side status block
     uint32_t max_nodes;       // largest node count a build asked for (sticky)
     uint32_t overflow_seen;   // some build overflowed (sticky until the host clears it)
     uint32_t nodes_kept;      // min(total_nodes, capacity): the nodes the last build wrote
-    uint32_t pad[7];
+    uint32_t moments_done;    // moments: workgroups of the split phase finished (0 between builds)
+    uint32_t pad[6];
 };
 
 // Independent systems: contiguous body ranges that are bounded, sorted, built

@@ -775,46 +775,70 @@ __global__ __launch_bounds__(kBlock) void k_moments_local(const Counters* __rest
     }
 }
 
-__global__ __launch_bounds__(256) void k_moments_rest(const Counters* __restrict__ ctr, SysView sv, int intended,
+// Phase 2.  One system: workgroup g (of 8) combines the straddling nodes
+// below the root's octant g (a node's octant at level 1 is its key's digit
+// there), level by level, and the last workgroup to finish combines the root
+// (one agent-scope fence per workgroup hands its nodes over).  Several
+// systems: one workgroup combines every straddling node, level by level.
+__global__ __launch_bounds__(256) void k_moments_rest(Counters* __restrict__ ctr, SysView sv, int intended,
                                                        TreeArgs t, const double* __restrict__ bounds4,
                                                        const double* __restrict__ px, const double* __restrict__ py,
-                                                       const double* __restrict__ pz, const double* __restrict__ tx,
-                                                       const double* __restrict__ ty, const double* __restrict__ tz,
-                                                       const double* __restrict__ tm, const uint32_t* __restrict__ tperm,
+                                                       const double* __restrict__ pz,
                                                        const uint32_t* __restrict__ rest,
                                                        const uint32_t* __restrict__ nrest, uint32_t slices) {
     if (ctr->node_overflow) return;
+    const bool split = gridDim.x > 1;
+    const uint32_t g = blockIdx.x;
+    const int lmin = split ? 1 : 0;
     // a level's node records do not depend on the level below: each thread's
     // first node of the next non-empty level is fetched while this one combines
     int l = kMaxLevel;
-    while (l >= 0 && nrest[l] == 0) --l;
+    while (l >= lmin && nrest[l] == 0) --l;
     uint32_t pn = 0;
+    uint64_t kn = 0;
     NodeInfo next{};
-    if (l >= 0 && threadIdx.x < nrest[l]) {
+    if (l >= lmin && threadIdx.x < nrest[l]) {
         pn = rest[l * (size_t)slices + threadIdx.x];
         next = node_info(pn, t, sv, bounds4);
+        kn = t.key[pn];
     }
-    while (l >= 0) {
+    while (l >= lmin) {
         const uint32_t m = nrest[l];
         const uint32_t p0 = pn;
         const NodeInfo info0 = next;
+        const uint64_t k0 = kn;
         int ln = l - 1;
-        while (ln >= 0 && nrest[ln] == 0) --ln;
-        if (ln >= 0 && threadIdx.x < nrest[ln]) {
+        while (ln >= lmin && nrest[ln] == 0) --ln;
+        if (ln >= lmin && threadIdx.x < nrest[ln]) {
             pn = rest[ln * (size_t)slices + threadIdx.x];
             next = node_info(pn, t, sv, bounds4);
+            kn = t.key[pn];
         }
         for (uint32_t i = threadIdx.x; i < m; i += blockDim.x) {
-            if (i == threadIdx.x) {
-                (void)moments_of(p0, info0, intended, t, px, py, pz, ChildFromGlobal{t});
-            } else {
-                const uint32_t p = rest[l * (size_t)slices + i];
-                (void)moments_of(p, node_info(p, t, sv, bounds4), intended, t, px, py, pz,
-                                 ChildFromGlobal{t});
-            }
+            const bool first = i == threadIdx.x;
+            const uint32_t p = first ? p0 : rest[l * (size_t)slices + i];
+            const uint64_t k = first ? k0 : t.key[p];
+            if (split && (uint32_t)((k >> (3 * (l - 1))) & 7u) != g) continue;   // another octant's
+            (void)moments_of(p, first ? info0 : node_info(p, t, sv, bounds4), intended, t, px, py, pz,
+                             ChildFromGlobal{t});
         }
         __syncthreads();
         l = ln;
+    }
+    if (split) {
+        __builtin_amdgcn_s_waitcnt(0);         // this workgroup's records are written ...
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            __threadfence();                   // ... and visible to every compute unit
+            if (atomicAdd(&ctr->moments_done, 1u) + 1u == gridDim.x) {
+                __threadfence();               // the other workgroups' records
+                ctr->moments_done = 0u;
+                if (nrest[0] > 0) {
+                    const uint32_t r = rest[0];
+                    (void)moments_of(r, node_info(r, t, sv, bounds4), intended, t, px, py, pz, ChildFromGlobal{t});
+                }
+            }
+        }
     }
 }
 
@@ -897,8 +921,8 @@ hipError_t build_tree(bhx_ctx* c, hipStream_t s, size_t* total_out, bool sync) {
     if ((size_t)(kMaxLevel + 1) * slices > c->tree.cap) return hipErrorInvalidValue;
     hipLaunchKernelGGL(k_moments_local, dim3(slices), dim3(kBlock), 0, s, c->ctr, sv, intended, t, b4, c->px, c->py,
                        c->pz, c->tx, c->ty, c->tz, c->tm, c->tperm, c->tree.by_level, nrest);
-    hipLaunchKernelGGL(k_moments_rest, dim3(1), dim3(256), 0, s, c->ctr, sv, intended, t, b4, c->px, c->py, c->pz,
-                       c->tx, c->ty, c->tz, c->tm, c->tperm, c->tree.by_level, nrest, (uint32_t)slices);
+    hipLaunchKernelGGL(k_moments_rest, dim3(sv.nsys == 1 ? 8 : 1), dim3(256), 0, s, c->ctr, sv, intended, t, b4, c->px,
+                       c->py, c->pz, c->tree.by_level, nrest, (uint32_t)slices);
     return hipGetLastError();
 }
 
