@@ -101,6 +101,9 @@ hipError_t alloc_tree(bhx_ctx* c, size_t cap) {
     if ((e = dalloc(&t.child, 8 * cap)) != hipSuccess) return e;
     if ((e = dalloc(&t.by_level, cap)) != hipSuccess) return e;
     if ((e = dalloc(&t.lvl, 3 * (kMaxLevel + 1) + 1)) != hipSuccess) return e;
+    // the moments' per-level counts (cleared by every build) and, after them, the
+    // hand-over counts of the split phase (zero between builds)
+    if ((e = hipMemset(t.lvl, 0, (3 * (kMaxLevel + 1) + 1) * sizeof(uint32_t))) != hipSuccess) return e;
     size_t h = 1024;
     while (h < 2 * cap) h <<= 1;
     if ((e = dalloc(&t.hkey, h)) != hipSuccess) return e;

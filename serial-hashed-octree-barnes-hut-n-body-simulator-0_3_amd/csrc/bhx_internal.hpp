@@ -76,7 +76,7 @@ struct DeviceTree {
     WalkPay32* pay32 = nullptr;
     uint32_t* child = nullptr;       // 8 per node: child id by octant (valid where child_byte has the bit)
     uint32_t* by_level = nullptr;    // moments: nodes left for the second phase
-    uint32_t* lvl = nullptr;         // moments: their count
+    uint32_t* lvl = nullptr;         // moments: their count per level, then the split phase's hand-over counts
     // hash table
     size_t hcap = 0;
     uint64_t* hkey = nullptr;
@@ -105,8 +105,7 @@ struct Counters {          // small device-side status block
     uint32_t max_nodes;       // largest node count a build asked for (sticky)
     uint32_t overflow_seen;   // some build overflowed (sticky until the host clears it)
     uint32_t nodes_kept;      // min(total_nodes, capacity): the nodes the last build wrote
-    uint32_t moments_done;    // moments: workgroups of the split phase finished (0 between builds)
-    uint32_t pad[6];
+    uint32_t pad[7];
 };
 
 // Independent systems: contiguous body ranges that are bounded, sorted, built

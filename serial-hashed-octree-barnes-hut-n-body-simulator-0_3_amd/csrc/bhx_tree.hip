@@ -775,21 +775,34 @@ __global__ __launch_bounds__(kBlock) void k_moments_local(const Counters* __rest
     }
 }
 
-// Phase 2.  One system: workgroup g (of 8) combines the straddling nodes
-// below the root's octant g (a node's octant at level 1 is its key's digit
-// there), level by level, and the last workgroup to finish combines the root
-// (one agent-scope fence per workgroup hands its nodes over).  Several
-// systems: one workgroup combines every straddling node, level by level.
+// Phase 2.  One system: workgroup g (of 64) combines the straddling nodes
+// whose octants at levels 1 and 2 (their key's first two digits) are g / 8
+// and g % 8, level by level; the last of an octant's eight workgroups to
+// finish combines that octant's level-1 node, and the last of those the root
+// (each hand-over is one agent-scope fence and one count; the counts, in
+// `meet`, return to zero for the next build).  Several systems: one
+// workgroup combines every straddling node, level by level.
+constexpr int kRestSplit = 64;
+
+__device__ __forceinline__ bool meet_last(uint32_t* cnt, uint32_t of) {
+    __threadfence();                           // this side's records, visible to every compute unit
+    if (atomicAdd(cnt, 1u) + 1u != of) return false;
+    __threadfence();                           // the other sides' records
+    *cnt = 0u;
+    return true;
+}
+
 __global__ __launch_bounds__(256) void k_moments_rest(Counters* __restrict__ ctr, SysView sv, int intended,
                                                        TreeArgs t, const double* __restrict__ bounds4,
                                                        const double* __restrict__ px, const double* __restrict__ py,
                                                        const double* __restrict__ pz,
                                                        const uint32_t* __restrict__ rest,
-                                                       const uint32_t* __restrict__ nrest, uint32_t slices) {
+                                                       const uint32_t* __restrict__ nrest, uint32_t slices,
+                                                       uint32_t* __restrict__ meet) {
     if (ctr->node_overflow) return;
     const bool split = gridDim.x > 1;
     const uint32_t g = blockIdx.x;
-    const int lmin = split ? 1 : 0;
+    const int lmin = split ? 2 : 0;
     // a level's node records do not depend on the level below: each thread's
     // first node of the next non-empty level is fetched while this one combines
     int l = kMaxLevel;
@@ -818,7 +831,7 @@ __global__ __launch_bounds__(256) void k_moments_rest(Counters* __restrict__ ctr
             const bool first = i == threadIdx.x;
             const uint32_t p = first ? p0 : rest[l * (size_t)slices + i];
             const uint64_t k = first ? k0 : t.key[p];
-            if (split && (uint32_t)((k >> (3 * (l - 1))) & 7u) != g) continue;   // another octant's
+            if (split && (uint32_t)((k >> (3 * (l - 2))) & 63u) != g) continue;   // another workgroup's
             (void)moments_of(p, first ? info0 : node_info(p, t, sv, bounds4), intended, t, px, py, pz,
                              ChildFromGlobal{t});
         }
@@ -826,17 +839,18 @@ __global__ __launch_bounds__(256) void k_moments_rest(Counters* __restrict__ ctr
         l = ln;
     }
     if (split) {
-        __builtin_amdgcn_s_waitcnt(0);         // this workgroup's records are written ...
+        __builtin_amdgcn_s_waitcnt(0);         // this workgroup's records are written
         __syncthreads();
-        if (threadIdx.x == 0) {
-            __threadfence();                   // ... and visible to every compute unit
-            if (atomicAdd(&ctr->moments_done, 1u) + 1u == gridDim.x) {
-                __threadfence();               // the other workgroups' records
-                ctr->moments_done = 0u;
-                if (nrest[0] > 0) {
-                    const uint32_t r = rest[0];
-                    (void)moments_of(r, node_info(r, t, sv, bounds4), intended, t, px, py, pz, ChildFromGlobal{t});
-                }
+        if (threadIdx.x == 0 && meet_last(&meet[g / 8], 8u)) {
+            // the octant's level-1 node, when it straddles
+            for (uint32_t i = 0; i < nrest[1]; ++i) {
+                const uint32_t p = rest[1 * (size_t)slices + i];
+                if ((uint32_t)(t.key[p] & 7u) == g / 8)
+                    (void)moments_of(p, node_info(p, t, sv, bounds4), intended, t, px, py, pz, ChildFromGlobal{t});
+            }
+            if (meet_last(&meet[8], 8u) && nrest[0] > 0) {
+                const uint32_t r = rest[0];
+                (void)moments_of(r, node_info(r, t, sv, bounds4), intended, t, px, py, pz, ChildFromGlobal{t});
             }
         }
     }
@@ -921,8 +935,8 @@ hipError_t build_tree(bhx_ctx* c, hipStream_t s, size_t* total_out, bool sync) {
     if ((size_t)(kMaxLevel + 1) * slices > c->tree.cap) return hipErrorInvalidValue;
     hipLaunchKernelGGL(k_moments_local, dim3(slices), dim3(kBlock), 0, s, c->ctr, sv, intended, t, b4, c->px, c->py,
                        c->pz, c->tx, c->ty, c->tz, c->tm, c->tperm, c->tree.by_level, nrest);
-    hipLaunchKernelGGL(k_moments_rest, dim3(sv.nsys == 1 ? 8 : 1), dim3(256), 0, s, c->ctr, sv, intended, t, b4, c->px,
-                       c->py, c->pz, c->tree.by_level, nrest, (uint32_t)slices);
+    hipLaunchKernelGGL(k_moments_rest, dim3(sv.nsys == 1 ? kRestSplit : 1), dim3(256), 0, s, c->ctr, sv, intended, t,
+                       b4, c->px, c->py, c->pz, c->tree.by_level, nrest, (uint32_t)slices, nrest + (kMaxLevel + 1));
     return hipGetLastError();
 }
 
