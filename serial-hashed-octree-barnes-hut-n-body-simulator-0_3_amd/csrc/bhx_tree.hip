@@ -780,8 +780,9 @@ __global__ __launch_bounds__(kBlock) void k_moments_local(const Counters* __rest
 // and g % 8, level by level; the last of an octant's eight workgroups to
 // finish combines that octant's level-1 node, and the last of those the root
 // (each hand-over is one agent-scope fence and one count; the counts, in
-// `meet`, return to zero for the next build).  Several systems: one
-// workgroup combines every straddling node, level by level.
+// `meet`, return to zero for the next build).  Several systems: workgroup g
+// of G combines, level by level, every straddling node of the systems s with
+// s mod G = g (systems share no node, so nothing is handed over).
 constexpr int kRestSplit = 64;
 
 __device__ __forceinline__ bool meet_last(uint32_t* cnt, uint32_t of) {
@@ -800,7 +801,8 @@ __global__ __launch_bounds__(256) void k_moments_rest(Counters* __restrict__ ctr
                                                        const uint32_t* __restrict__ nrest, uint32_t slices,
                                                        uint32_t* __restrict__ meet) {
     if (ctr->node_overflow) return;
-    const bool split = gridDim.x > 1;
+    const bool split = gridDim.x > 1 && sv.nsys == 1;     // by octants below the root
+    const bool bysys = gridDim.x > 1 && sv.nsys > 1;      // by systems
     const uint32_t g = blockIdx.x;
     const int lmin = split ? 2 : 0;
     // a level's node records do not depend on the level below: each thread's
@@ -832,8 +834,9 @@ __global__ __launch_bounds__(256) void k_moments_rest(Counters* __restrict__ ctr
             const uint32_t p = first ? p0 : rest[l * (size_t)slices + i];
             const uint64_t k = first ? k0 : t.key[p];
             if (split && (uint32_t)((k >> (3 * (l - 2))) & 63u) != g) continue;   // another workgroup's
-            (void)moments_of(p, first ? info0 : node_info(p, t, sv, bounds4), intended, t, px, py, pz,
-                             ChildFromGlobal{t});
+            const NodeInfo I = first ? info0 : node_info(p, t, sv, bounds4);
+            if (bysys && sys_of(sv, I.st) % gridDim.x != g) continue;
+            (void)moments_of(p, I, intended, t, px, py, pz, ChildFromGlobal{t});
         }
         __syncthreads();
         l = ln;
@@ -935,7 +938,8 @@ hipError_t build_tree(bhx_ctx* c, hipStream_t s, size_t* total_out, bool sync) {
     if ((size_t)(kMaxLevel + 1) * slices > c->tree.cap) return hipErrorInvalidValue;
     hipLaunchKernelGGL(k_moments_local, dim3(slices), dim3(kBlock), 0, s, c->ctr, sv, intended, t, b4, c->px, c->py,
                        c->pz, c->tx, c->ty, c->tz, c->tm, c->tperm, c->tree.by_level, nrest);
-    hipLaunchKernelGGL(k_moments_rest, dim3(sv.nsys == 1 ? kRestSplit : 1), dim3(256), 0, s, c->ctr, sv, intended, t,
+    const unsigned rest_groups = sv.nsys == 1 ? (unsigned)kRestSplit : (sv.nsys < 256u ? sv.nsys : 256u);
+    hipLaunchKernelGGL(k_moments_rest, dim3(rest_groups), dim3(256), 0, s, c->ctr, sv, intended, t,
                        b4, c->px, c->py, c->pz, c->tree.by_level, nrest, (uint32_t)slices, nrest + (kMaxLevel + 1));
     return hipGetLastError();
 }
