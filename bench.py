@@ -438,6 +438,9 @@ def main():
     ap.add_argument("--no-galaxy", action="store_true")
     ap.add_argument("--calibrate-copy-mib", type=int, default=0,
                     help="first copy this many MiB device-to-device (counter calibration runs)")
+    ap.add_argument("--adjoint-only", type=int, default=0, metavar="FRAMES",
+                    help="only the differentiation leg, monopole then quadrupole cells, over FRAMES frames "
+                         "(kernel profiles of the reverse pass); prints one JSON line of its figures")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build runs the whole Barnes-Hut step on a single MI355X "
@@ -447,6 +450,13 @@ def main():
     bhx = load_bhx()
     if args.calibrate_copy_mib:
         copy_calibration(args.calibrate_copy_mib)
+    if args.adjoint_only:
+        res = {"adjoint_frames": args.adjoint_only}
+        for q, key in ((0, "adjoint"), (1, "adjoint_quadrupole")):
+            log("adjoint: %d frames, quadrupole %d" % (args.adjoint_only, q))
+            res[key] = adjoint_bench(bhx, steps=args.adjoint_only, quadrupole=q)
+        print(json.dumps(res), flush=True)
+        return
     theta, eps, dt, cap = 0.5, 0.01, 5e-4, 32
     n = args.n
     log("generating Plummer N=%d seed=%d" % (n, args.seed))
