@@ -690,3 +690,37 @@ def test_fp32_walk_group_sizes(bhx, group_size):
     assert e0.max() < 1e-4                                   # FP32 rounding over 40,000 terms
     assert np.median(e5) < 1e-3
     assert np.all(cnt5 >= cref)
+
+
+@pytest.mark.parametrize("prec,comp", [(0, 0), (0, 1), (1, 1)])
+def test_group_walk_frame_closing_kick_matches_oracle(bhx, prec, comp):
+    """One frame from zero acceleration with a group walk, which applies the closing half
+    kick itself as it writes each acceleration: positions are x0 + v0 dt exactly (the first
+    half kick adds nothing), and every velocity moved by (a dt) / 2 with a of this frame's walk —
+    checked against the oracle's frame (the reference's per-body walk, whose accelerations
+    the shared lists approximate to ~1e-4), so a kick missing, doubled or applied with the
+    old acceleration would be off by half of a dt."""
+    n, dt = 4096, 1e-3
+    pos, vel, mass = bhx.plummer(n, seed=31)
+    with _engine(bhx, n, theta=0.5, eps=0.01, dt=dt, mode=1, quadrupole=1, leaf_capacity=32, precision=prec,
+                 group_walk=1, compensated=comp, permute_acc=1) as e:
+        e.set_bodies(pos, vel, mass)
+        e.step(1)
+        gp, gv, _, ga = e.get_bodies()
+    p, v, m, a, _, _ = orc.frame(pos, vel, mass, np.zeros_like(pos), dt, 0.5, 0.01, leaf_cap=32, mode=1,
+                                 quad=True, permute_acc=True)
+    gi, oi = np.argsort(gp[:, 0], kind="stable"), np.argsort(p[:, 0], kind="stable")
+    assert np.array_equal(gp[gi], p[oi])                                 # x0 + v0 dt, bit for bit
+    amag = np.linalg.norm(a[oi], axis=1)
+    dv = np.linalg.norm(gv[gi] - v[oi], axis=1) / (amag * dt)             # in units of |a| dt
+    da = np.linalg.norm(ga[gi] - a[oi], axis=1) / amag
+    print("prec %d comp %d: max |dv| / (|a| dt) %.2e, median rel |da| %.2e" % (prec, comp, dv.max(), np.median(da)),
+          flush=True)
+    assert np.median(da) < 1e-3
+    assert dv.max() < 0.05
+    # and the kick used this frame's own accelerations: v = v0 + (a dt) / 2 per body (bodies
+    # matched by their drifted positions, x0 + v0 dt, which the engine reproduces bit for bit)
+    hi = np.argsort((pos + vel * dt)[:, 0], kind="stable")
+    assert np.array_equal((pos + vel * dt)[hi], gp[gi])
+    kick = np.linalg.norm(gv[gi] - vel[hi] - 0.5 * dt * ga[gi], axis=1)
+    assert kick.max() <= 1e-12 * (np.linalg.norm(gv, axis=1).max() + 1.0)
